@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Benchmark of the structural hot path (RDF + S(Q)) on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload lattice_262k]
+
+A step is one frame through the whole path: ingest (wrap, species sort), the
+cell-list pair histogram g_ab(r) and the S(Q) contraction with its shell sums.
+Frames are generated from a seed and copied to HBM before the timed region;
+the timed region runs K frames back to back on the context stream and ends
+with a device synchronisation.  One JSON line is printed.
+
+Fields besides the driver's contract:
+  pairs_per_s         in-range pairs binned per second (RDF)
+  q_atom_terms_per_s  (q-vector, atom) phase terms per second (S(Q))
+  latency_ms_p50/p99  per-frame latency, frames submitted one at a time
+  kernel_ms           mean per-frame time of each stage (HIP events on the
+                      context stream)
+  roofline            the dominant kernel (the S(Q) contraction) against the
+                      FP64 peak: algorithmic flops 8*N*Nq per launch / mean
+                      launch time
+  cpu_baseline        the CPU oracle on the host cores, on a bounded sample of
+                      the same workload, extrapolated to frames/s
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "frames/s + pairs/s (RDF), q·atom terms/s (S(Q)); p50/p99 frame latency"
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "fp16x3": 2.5e15 / 3, "bf16x3": 2.5e15 / 3}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_frames(workload: str, count: int):
+    from trj_analysis_amd import workloads as W
+    if workload == "lattice_262k":
+        wl = W.LATTICE_262K
+        frames = list(W.lattice_262k(nframes=count))
+    elif workload == "lj_binary_4k":
+        wl = W.LJ_BINARY_4K
+        frames = list(W.lj_binary_4k(nframes=count))
+    elif workload == "uniform_1m":
+        wl = W.UNIFORM_1M
+        frames = [W.uniform_1m(seed=2 + k) for k in range(count)]
+    else:
+        raise SystemExit(f"unknown workload {workload}")
+    return wl, frames
+
+
+def cpu_baseline(wl, frame, n_in_range_pairs: float, nq_terms: int, budget_s: float):
+    """Oracle (the CPU port of the reference algorithm) on the host cores."""
+    from oracle import oracle as O
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    box, per = frame.box, (1, 1, 1)
+    # RDF: one full frame through the OpenMP linked-cell oracle.
+    t0 = time.perf_counter()
+    counts = O.rdf_counts(frame.pos, frame.itype, 3, wl.nsp, box, per, wl.rcrdf, wl.deltar)
+    t_rdf = time.perf_counter() - t0
+    # S(Q): long-double direct sums over a sample of the q-set.
+    pos = O.wrap(frame.pos, box, per)
+    rng = np.random.default_rng(0)
+    kv = rng.integers(0, 8, size=(16, 3)).astype(np.int32)
+    t0 = time.perf_counter()
+    O.sq_rho(pos, frame.itype, wl.nsp, box, kv)
+    t16 = time.perf_counter() - t0
+    m = int(max(16, min(20000, 16 * max(0.0, budget_s - t_rdf) / max(t16, 1e-6))))
+    kv = rng.integers(0, 32, size=(m, 3)).astype(np.int32)
+    t0 = time.perf_counter()
+    O.sq_rho(pos, frame.itype, wl.nsp, box, kv)
+    t_m = time.perf_counter() - t0
+    n = len(frame.itype)
+    terms_rate = m * n / t_m
+    t_sq = nq_terms / terms_rate
+    pairs_rate = float(counts.sum()) / t_rdf
+    fps = 1.0 / (t_rdf + t_sq)
+    return {
+        "value": fps, "unit": "frames/s", "cores": cores, "kind": "port",
+        "sample": (f"one {n}-atom frame: RDF in full (linked cells, OpenMP, {t_rdf:.2f} s); S(Q) "
+                   f"long-double direct sums over {m} q-vectors ({t_m:.2f} s), extrapolated to the "
+                   f"{nq_terms // n} vectors of a frame"),
+        "pairs_per_s": pairs_rate, "q_atom_terms_per_s": terms_rate,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="lattice_262k",
+                    choices=["lattice_262k", "lj_binary_4k", "uniform_1m"])
+    ap.add_argument("--precision", default="fp64", choices=list(PEAK))
+    ap.add_argument("--latency-frames", type=int, default=10)
+    ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "; run it with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from trj_analysis_amd import trjhip as T
+
+    pool = min(args.steps + args.warmup, 16)
+    log(f"generating {pool} frames of {args.workload}")
+    wl, frames = make_frames(args.workload, pool)
+    n = len(frames[0].itype)
+    dev = torch.device("cuda:0")
+    dpos = [torch.from_numpy(np.ascontiguousarray(f.pos, dtype=np.float64)).to(dev) for f in frames]
+    dtyp = torch.from_numpy(frames[0].itype.astype(np.int32)).to(dev)
+    torch.cuda.synchronize()
+
+    ctx = T.Context(0)
+    box = frames[0].box
+    st = ctx.rdf_init(T.rdf_params(3, wl.nsp, wl.deltar, wl.rcrdf), box)
+    qset = T.TRJ_QSET_HALFSPACE if wl.qset == "halfspace" else T.TRJ_QSET_REF
+    info = ctx.sq_init(T.sq_params(3, wl.nsp, wl.qmax, wl.qmin, wl.bsc, qset, wl.grid, args.precision),
+                       box)
+    nvec = info.nvec_grid + info.nvec_axis
+
+    def submit(k):
+        ctx.frame_device(dpos[k % pool].data_ptr(), T.TRJ_F64, T.TRJ_AOS, dtyp.data_ptr(), n, box)
+
+    log(f"warmup {args.warmup} frames (N={n}, lsmax={st.lsmax}, q-vectors={nvec})")
+    for k in range(args.warmup):
+        submit(k)
+    ctx.sync()
+    pairs0 = int(ctx.rdf_counts().sum())
+    ctx.timing_reset()
+    torch.cuda.synchronize()
+    log(f"timed {args.steps} frames")
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        submit(args.warmup + k)
+    ctx.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    pairs = int(ctx.rdf_counts().sum()) - pairs0
+    tot, nfr = ctx.timing_totals()
+    kernel_ms = {"ingest": tot[3] / nfr, "rdf": tot[0] / nfr, "sq_contract": tot[1] / nfr,
+                 "sq_shells_axis": tot[2] / nfr}
+    log(f"elapsed {elapsed:.3f} s; kernel ms {kernel_ms}")
+
+    lat = []
+    for k in range(args.latency_frames):
+        s0 = time.perf_counter()
+        submit(k)
+        ctx.sync()
+        lat.append((time.perf_counter() - s0) * 1e3)
+
+    fps = args.steps / elapsed
+    flops = 8.0 * n * info.nvec_grid
+    achieved = flops / (kernel_ms["sq_contract"] * 1e-3)
+    peak = PEAK[args.precision]
+    out = {
+        "metric": METRIC,
+        "value": fps,
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else args.precision,
+        "data": "synthetic",
+        "config": {"workload": args.workload, "n_atoms": n, "nsp": wl.nsp, "rcrdf": wl.rcrdf,
+                   "deltar": wl.deltar, "qmax": wl.qmax, "qset": wl.qset, "q_vectors": nvec,
+                   "lsmax": st.lsmax, "precision": args.precision, "inputs": "HBM-resident float64"},
+        "pairs_per_s": pairs / elapsed,
+        "q_atom_terms_per_s": nvec * n * args.steps / elapsed,
+        "latency_ms_p50": float(np.percentile(lat, 50)),
+        "latency_ms_p99": float(np.percentile(lat, 99)),
+        "kernel_ms": kernel_ms,
+        "roofline": {"bound": "mfma", "kernel": "k_sq_contract_f64", "achieved": achieved / 1e12,
+                     "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "algorithmic": f"8*N*Nq = {flops:.4e} flop per launch (N={n}, Nq={info.nvec_grid})"},
+    }
+    ctx.close()
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(wl, frames[0], pairs / max(args.steps, 1),
+                                           nvec * n, args.cpu_baseline_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

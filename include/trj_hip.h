@@ -1,0 +1,209 @@
+/*
+ * trj_hip.h -- C ABI of the MI355X structural hot path of trj_analysis:
+ * the species-resolved pair histogram g_ab(r) and the static structure
+ * factor S(Q), as hand-written HIP kernels for gfx950.
+ *
+ * The reference has no plugin registry: its "operator API" is the set of
+ * Fortran module procedures of mod_rdf / mod_sq, called by the driver
+ * (src/main/trj_analysis.f90:193-217) and by src/core/util.f90:188-198,
+ * 257-289, with all data passed through module globals (src/core/common.cuf).
+ * Each entry point below names the reference procedure it replaces.
+ *
+ * Conventions
+ *  - Every function returns TRJ_OK (0) or a negative TRJ_E* code; the library
+ *    never exits.  trj_last_error() returns the message of the last failure
+ *    on a context (the reference prints and stops, rdf.cuf:241-242).
+ *  - Indices follow the reference: species types are 1..nsp; histograms are
+ *    laid out as histomix_d(lsmax, nit) (bin fastest, pair index ij slowest);
+ *    S(Q) shells are 0..nqmax.
+ *  - One context per host thread; contexts are independent.  All device work
+ *    of a context runs on its own HIP stream; calls on one context are
+ *    asynchronous until a call that returns results (those synchronise).
+ *  - Plain pointers and sizes only.  Host pointers are borrowed for the
+ *    duration of the call.
+ */
+#ifndef TRJ_HIP_H
+#define TRJ_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TRJ_ABI_VERSION 1
+
+/* Return codes. */
+#define TRJ_OK 0
+#define TRJ_EINVAL (-1)   /* invalid argument or parameter */
+#define TRJ_EDEVICE (-2)  /* HIP runtime error */
+#define TRJ_ENOMEM (-3)   /* device or pinned allocation failed */
+#define TRJ_ESTATE (-4)   /* call out of order (e.g. frame before init) */
+#define TRJ_ERANGE (-5)   /* coordinates outside the range the kernels accept */
+#define TRJ_ELIMIT (-6)   /* size above a documented limit */
+
+/* Position dtype / layout of trj_frame inputs. */
+#define TRJ_F32 0
+#define TRJ_F64 1
+#define TRJ_AOS 0 /* x0 y0 (z0) x1 y1 (z1) ...: the reference's r(ndim,N), stride ndim
+                      (common.cuf:52, netcdf.f90:651) */
+#define TRJ_SOA 1 /* x[0..N) y[0..N) (z[0..N)) */
+
+/* What a frame computes (RDFcomp and/or SQcalc, trj_analysis.f90:211,217). */
+#define TRJ_WANT_RDF 1u
+#define TRJ_WANT_SQ 2u
+
+/* q-vector sets. */
+#define TRJ_QSET_REF 0      /* the reference set: first-octant lattice vectors with
+                               |q| <= qmin (sqfnD, sq.cuf:523-591) plus the three axis
+                               directions for shells nqmin+1..nqmax (sqfact,
+                               sq.cuf:372-454) */
+#define TRJ_QSET_HALFSPACE 1 /* dense grid nx,ny in [-gx/2, gx/2), nz in [0, gz/2),
+                                origin excluded; shells nint(|q|/dq) */
+
+/* S(Q) contraction precision (phases always reduced in FP64). */
+#define TRJ_PREC_FP64 0   /* FP64 MFMA, FP64 accumulation */
+#define TRJ_PREC_FP32 1   /* FP32 MFMA operands and accumulation within a K block */
+#define TRJ_PREC_FP16X3 2 /* FP16 hi+lo split, 3 products, FP32 accumulation */
+#define TRJ_PREC_BF16X3 3 /* BF16 hi+lo split, 3 products, FP32 accumulation */
+
+/* S(Q) compatibility flags. */
+#define TRJ_SQ_NQ_CONSISTENT 1u /* normalise each shell by the number of vectors
+                                   actually summed, instead of the reference's host
+                                   count (sq.cuf:328-342), which misses the on-sphere
+                                   vectors of shell nqmin */
+
+typedef struct trj_ctx trj_ctx;
+
+/* Library and context lifetime.  trj_create selects the HIP device
+ * (util.f90:54-80 gpu_and_header / cudaSetDevice) and creates the context's
+ * stream.  trj_destroy frees everything (clean_memory, util.f90:200-237). */
+int trj_abi_version(void);
+int trj_create(trj_ctx** out, int device);
+void trj_destroy(trj_ctx* ctx);
+const char* trj_last_error(const trj_ctx* ctx);
+
+/* /INPUT_RDF/ and the /INPUT/ fields the path uses (input.f90:64-69). */
+typedef struct {
+  int32_t ndim;        /* 2 or 3 */
+  int32_t nsp;         /* 1..10 (nspmax, common.cuf:41) */
+  double deltar;       /* bin width, > 0 */
+  double rcrdf;        /* cutoff; <= 0 means half the shortest side (rdf.cuf:55-58) */
+  int32_t periodic[3]; /* /INPUT/ periodic: min image on 2.5*L for false axes
+                          (netcdf.f90:603-609) */
+} trj_rdf_params;
+
+/* RDF_init (rdf.cuf:51-122).  box: first frame's cell lengths; the cutoff is
+ * first capped to half the shortest side (netcdf.f90:610-612).  Returns lsmax
+ * and nit = nsp(nsp+1)/2.  Zeroes the accumulated counts. */
+int trj_rdf_init(trj_ctx* ctx, const trj_rdf_params* p, const double box[3],
+                 int32_t* lsmax_out, int32_t* nit_out);
+
+/* /INPUT_SQ/ (input.f90:70) plus the build's options. */
+typedef struct {
+  int32_t ndim;
+  int32_t nsp;
+  double qmax;          /* > 0 */
+  double qmin;          /* the reference has no default; <= 0 means qmax */
+  double bsc[10];       /* scattering lengths per species (default 1.0, input.f90:106-107) */
+  int32_t qset;         /* TRJ_QSET_* */
+  int32_t grid[3];      /* TRJ_QSET_HALFSPACE: full grid extents (even) */
+  int32_t precision;    /* TRJ_PREC_* */
+  uint32_t flags;       /* TRJ_SQ_* */
+} trj_sq_params;
+
+/* Geometry of the q-set, as sq_init computes it (sq.cuf:156-200). */
+typedef struct {
+  double dq;          /* FP32 value of min 2*pi/L (sq.cuf:159) */
+  double fk[3];       /* FP32 values of 2*pi/L per axis */
+  double qmin_eff;    /* qmin after the lattice adjust (sq.cuf:193) */
+  int32_t nqmax;      /* nint(qmax/dq) */
+  int32_t nqmin;      /* min lmax after the mod-4/mod-16 adjust */
+  int32_t lmax[3];
+  int64_t nvec_grid;  /* vectors in the dense (octant or half-space) part */
+  int64_t nvec_axis;  /* axis-direction vectors */
+  int64_t box_ext[3]; /* extent of the dense box computed per axis */
+} trj_sq_info;
+
+/* sq_init (sq.cuf:150-369).  box: first frame's cell lengths; the q-grid is
+ * fixed from here on (fk is set once, sq.cuf:189).  Zeroes the accumulators. */
+int trj_sq_init(trj_ctx* ctx, const trj_sq_params* p, const double box[3], trj_sq_info* info);
+
+/* Host-only planning (no context, no GPU): the q-set geometry and the shell
+ * counts nq(0:nqmax) the given parameters produce (sq.cuf:156-200, 303-343).
+ * nq may be NULL; otherwise it has info->nqmax+1 entries. */
+int trj_sq_plan(const trj_sq_params* p, const double box[3], trj_sq_info* info, int64_t* nq);
+
+/* Host-only: lsmax for the given parameters (rdf.cuf:55-58 after the cap). */
+int trj_rdf_plan(const trj_rdf_params* p, const double box[3], int32_t* lsmax, double* rc_eff);
+
+/* One frame: transfer_cpu_gpu (common.cuf:368-378) + RDFcomp (rdf.cuf:184-353)
+ * + SQcalc (sq.cuf:58-126).  pos: host positions (dtype TRJ_F32/F64, layout
+ * TRJ_AOS/SOA), itype: host species 1..nsp, box: this frame's cell lengths.
+ * Periodic axes are wrapped into [0,L) (floor) on the device.  Asynchronous:
+ * host buffers are staged before return. */
+int trj_frame(trj_ctx* ctx, const void* pos, int32_t dtype, int32_t layout,
+              const int32_t* itype, int64_t n, const double box[3], uint32_t what);
+
+/* Same as trj_frame for inputs already in device memory of this context's
+ * device (HBM-resident frames). */
+int trj_frame_device(trj_ctx* ctx, const void* pos_dev, int32_t dtype, int32_t layout,
+                     const int32_t* itype_dev, int64_t n, const double box[3], uint32_t what);
+
+/* Wait for the context's queued work; reports deferred device errors
+ * (e.g. TRJ_ERANGE from the coordinate check). */
+int trj_sync(trj_ctx* ctx);
+
+/* Accumulated pair counts (the reference's histomix after RDFcomp's host
+ * reduction, rdf.cuf:301-310, kept as exact int64): counts[ij-1][k-1],
+ * nit x lsmax.  Synchronises. */
+int trj_rdf_counts(trj_ctx* ctx, int64_t* counts);
+
+/* printrdf normalisation (log.f90:484-518): rows i=1..lsmax-2, each
+ * r, g_11, g_12, ..., g_1n, g_22, ... (1+nit doubles).  nconf <= 0 uses the
+ * number of RDF frames processed.  Volume and species counts are the last
+ * frame's.  Synchronises. */
+int trj_rdf_gmix(trj_ctx* ctx, int64_t nconf, double* out);
+
+/* rdf.cuf:313-344: first minimum of the running total g(r) with parabolic
+ * interpolation (find_first_minimum_interp, rdf.cuf:755-822). */
+int trj_rdf_first_minimum(trj_ctx* ctx, double* rmin);
+
+/* sq_transfer_gpu_cpu (sq.cuf:128-148): raw shell sums sqf[0..nqmax],
+ * sqfp[s][0..nqmax] (FP64) and the shell counts nq[0..nqmax] used for
+ * normalisation.  Any pointer may be NULL.  Synchronises. */
+int trj_sq_shells(trj_ctx* ctx, double* sqf, double* sqfp, int64_t* nq);
+
+/* printSQ normalisation (log.f90:317-401).  cols: nqmax rows of ncols doubles
+ * (Q, S_NN, n) or, for two species with all bsc == 1, (Q, S_NN, S_cc, S_11,
+ * S_22, S_12, n); mix: nqmax rows of (Q, S_11, ..., S_nn).  nconf <= 0 uses
+ * the number of S(Q) frames processed.  Synchronises. */
+int trj_sq_columns(trj_ctx* ctx, int64_t nconf, double* cols, int32_t* ncols, double* mix);
+
+/* rho_s(q) of the last frame for nvec integer vectors kv[3*v..] that belong
+ * to the dense q-set: rho[(v*nsp + s)*2 + {0,1}].  Debug/parity access to the
+ * contraction output.  Synchronises. */
+int trj_sq_rho(trj_ctx* ctx, int64_t nvec, const int32_t* kv, double* rho);
+
+/* rdf_clear / sq_clear (rdf.cuf:124, sq.cuf:774): zero the accumulators and
+ * frame counters, keep the plans. */
+int trj_reset(trj_ctx* ctx);
+
+/* Timing of the last frame's kernels on the context stream (hipEvents),
+ * milliseconds: [0] RDF (cell build + pairs + merge), [1] S(Q) contraction,
+ * [2] S(Q) shells + axis vectors, [3] ingest (wrap, sort).  (gptime,
+ * common.cuf:328-336, without the host sync per module.) */
+int trj_last_timings(trj_ctx* ctx, double* ms4);
+
+/* Kernel time summed over all frames harvested since the last reset, same
+ * four columns as trj_last_timings; nframes = frames harvested.  Synchronises. */
+int trj_timing_totals(trj_ctx* ctx, double* ms4, int64_t* nframes);
+int trj_timing_reset(trj_ctx* ctx);
+
+/* Device memory the context holds, in bytes. */
+int64_t trj_device_bytes(const trj_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,256 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this
+module, and only as the checker.  The functions restate the reference's
+algorithm; citations (reference file:line) are in trj_oracle.c.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libtrj_oracle.so")
+_lib = None
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc); returns the library path."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+class SqGrid(C.Structure):
+    _fields_ = [
+        ("fk", C.c_float * 3),
+        ("dq", C.c_float),
+        ("nqmax", C.c_int),
+        ("l", C.c_int * 3),
+        ("nqmin", C.c_int),
+        ("qminp", C.c_float),
+        ("qmin2", C.c_float),
+        ("kext", C.c_int * 3),
+    ]
+
+
+_D = C.POINTER(C.c_double)
+_I64 = C.POINTER(C.c_int64)
+_I32 = C.POINTER(C.c_int32)
+
+
+def _declare(L):
+    L.or_nint_d.argtypes = [C.c_double]
+    L.or_nint_d.restype = C.c_long
+    L.or_nint_f.argtypes = [C.c_float]
+    L.or_nint_f.restype = C.c_int
+    L.or_ind_ij.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.or_ind_ij.restype = C.c_int
+    L.or_geometry.argtypes = [C.c_int, _D, _I32, _D, _D, _D, _D]
+    L.or_geometry.restype = None
+    L.or_wrap1.argtypes = [C.c_double, C.c_double]
+    L.or_wrap1.restype = C.c_double
+    L.or_rdf_lsmax.argtypes = [C.c_int, _D, C.c_double, C.c_double]
+    L.or_rdf_lsmax.restype = C.c_int
+    L.or_rdf_brute.argtypes = [C.c_int64, _D, _D, _D, _I32, C.c_int, C.c_int, _D,
+                               C.c_double, C.c_double, C.c_int, _I64]
+    L.or_rdf_brute.restype = None
+    L.or_rdf_cells.argtypes = [C.c_int64, _D, _D, _D, _I32, C.c_int, C.c_int, _D, _I32,
+                               _D, C.c_double, C.c_double, C.c_int, _I64]
+    L.or_rdf_cells.restype = None
+    L.or_rdf_gmix.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _I64,
+                              C.c_int64, _I64, _D]
+    L.or_rdf_gmix.restype = None
+    L.or_first_minimum.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                   C.c_int64, _I64]
+    L.or_first_minimum.restype = C.c_double
+    L.or_sq_setup.argtypes = [C.c_int, _D, C.c_double, C.c_double, C.POINTER(SqGrid)]
+    L.or_sq_setup.restype = None
+    L.or_sq_nq.argtypes = [C.c_int, C.POINTER(SqGrid), _I64]
+    L.or_sq_nq.restype = None
+    L.or_sq_member.argtypes = [C.c_int, C.POINTER(SqGrid), C.c_int, C.c_int, C.c_int]
+    L.or_sq_member.restype = C.c_int
+    L.or_sq_rho.argtypes = [C.c_int64, _D, _D, _D, _I32, C.c_int, _D, C.c_int64, _I32, _D]
+    L.or_sq_rho.restype = None
+    L.or_sq_frame.argtypes = [C.c_int64, _D, _D, _D, _I32, C.c_int, C.c_int, _D,
+                              C.POINTER(SqGrid), _D, _D, _D]
+    L.or_sq_frame.restype = None
+    L.or_sq_columns.argtypes = [C.c_int, C.c_int, C.c_float, C.c_int64, _I64, _D, C.c_int64,
+                                _D, _D, _I64, _D, _D]
+    L.or_sq_columns.restype = C.c_int
+
+
+def _p(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def nint(x: float) -> int:
+    return lib().or_nint_d(float(x))
+
+
+def ind_ij(i: int, j: int, nsp: int) -> int:
+    return lib().or_ind_ij(i, j, nsp)
+
+
+@dataclass
+class Geometry:
+    lmi: np.ndarray
+    side2: float
+    volume: float
+    rcrdf: float
+
+
+def geometry(ndim, box, periodic, rcrdf) -> Geometry:
+    box = _f64(box)
+    per = _i32(periodic)
+    rc = C.c_double(rcrdf)
+    lmi = np.zeros(3)
+    s2 = C.c_double()
+    vol = C.c_double()
+    lib().or_geometry(ndim, _p(box, C.c_double), _p(per, C.c_int32), C.byref(rc),
+                      _p(lmi, C.c_double), C.byref(s2), C.byref(vol))
+    return Geometry(lmi, s2.value, vol.value, rc.value)
+
+
+def wrap(pos: np.ndarray, box, periodic, ndim=3) -> np.ndarray:
+    """pos (N,3) -> FP64 copy with periodic axes wrapped into [0,L)."""
+    out = np.array(pos, dtype=np.float64, copy=True)
+    w = np.vectorize(lib().or_wrap1, otypes=[np.float64])
+    for k in range(ndim):
+        if periodic[k]:
+            out[:, k] = w(out[:, k], float(box[k]))
+    return out
+
+
+def rdf_lsmax(ndim, box, rcrdf, deltar) -> int:
+    box = _f64(box)
+    return lib().or_rdf_lsmax(ndim, _p(box, C.c_double), rcrdf, deltar)
+
+
+def rdf_counts(pos, itype, ndim, nsp, box, periodic, rcrdf, deltar, lsmax=None,
+               method="cells") -> np.ndarray:
+    """Counts (nit, lsmax) int64 for one frame of raw positions (N,3)."""
+    g = geometry(ndim, box, periodic, rcrdf)
+    if lsmax is None:
+        lsmax = rdf_lsmax(ndim, box, rcrdf, deltar)
+    w = wrap(pos, box, periodic, ndim)
+    x, y, z = (np.ascontiguousarray(w[:, k]) for k in range(3))
+    it = _i32(itype)
+    nit = nsp * (nsp + 1) // 2
+    counts = np.zeros((nit, lsmax), dtype=np.int64)
+    n = len(it)
+    if method == "brute":
+        lib().or_rdf_brute(n, _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double),
+                           _p(it, C.c_int32), ndim, nsp, _p(g.lmi, C.c_double), g.side2,
+                           deltar, lsmax, _p(counts, C.c_int64))
+    else:
+        b = _f64(box)
+        per = _i32(periodic)
+        lib().or_rdf_cells(n, _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double),
+                           _p(it, C.c_int32), ndim, nsp, _p(b, C.c_double), _p(per, C.c_int32),
+                           _p(g.lmi, C.c_double), g.side2, deltar, lsmax,
+                           _p(counts, C.c_int64))
+    return counts
+
+
+def rdf_gmix(ndim, nsp, lsmax, deltar, volume, ntype, nconf, counts) -> np.ndarray:
+    nit = nsp * (nsp + 1) // 2
+    out = np.zeros((max(lsmax - 2, 0), 1 + nit))
+    nt = _i64(ntype)
+    c = _i64(counts)
+    lib().or_rdf_gmix(ndim, nsp, lsmax, deltar, volume, _p(nt, C.c_int64), nconf,
+                      _p(c, C.c_int64), _p(out, C.c_double))
+    return out
+
+
+def first_minimum(ndim, nsp, lsmax, deltar, volume, natoms, counts) -> float:
+    c = _i64(counts)
+    return lib().or_first_minimum(ndim, nsp, lsmax, deltar, volume, natoms, _p(c, C.c_int64))
+
+
+def sq_setup(ndim, box, qmax, qmin) -> SqGrid:
+    g = SqGrid()
+    b = _f64(box)
+    lib().or_sq_setup(ndim, _p(b, C.c_double), qmax, qmin, C.byref(g))
+    return g
+
+
+def sq_nq(ndim, g: SqGrid) -> np.ndarray:
+    nq = np.zeros(g.nqmax + 1, dtype=np.int64)
+    lib().or_sq_nq(ndim, C.byref(g), _p(nq, C.c_int64))
+    return nq
+
+
+def sq_member(ndim, g: SqGrid, kx, ky, kz) -> int:
+    return lib().or_sq_member(ndim, C.byref(g), kx, ky, kz)
+
+
+def sq_rho(pos, itype, nsp, box, kvecs) -> np.ndarray:
+    """rho (nvec, nsp) complex128 from long-double direct sums; pos already wrapped."""
+    pos = np.asarray(pos, dtype=np.float64)
+    x, y, z = (np.ascontiguousarray(pos[:, k]) for k in range(3))
+    it = _i32(itype)
+    kv = _i32(kvecs).reshape(-1, 3)
+    b = _f64(box)
+    out = np.zeros((len(kv), nsp, 2))
+    lib().or_sq_rho(len(it), _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double),
+                    _p(it, C.c_int32), nsp, _p(b, C.c_double), len(kv), _p(kv, C.c_int32),
+                    _p(out, C.c_double))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def sq_frame(pos, itype, ndim, nsp, box, g: SqGrid, bsc, sqf=None, sqfp=None):
+    """Accumulate one frame of the reference sum set; pos already wrapped."""
+    pos = np.asarray(pos, dtype=np.float64)
+    x, y, z = (np.ascontiguousarray(pos[:, k]) for k in range(3))
+    it = _i32(itype)
+    b = _f64(box)
+    bs = _f64(bsc)
+    if sqf is None:
+        sqf = np.zeros(g.nqmax + 1)
+    if sqfp is None:
+        sqfp = np.zeros((nsp, g.nqmax + 1))
+    lib().or_sq_frame(len(it), _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double),
+                      _p(it, C.c_int32), ndim, nsp, _p(b, C.c_double), C.byref(g),
+                      _p(bs, C.c_double), _p(sqf, C.c_double), _p(sqfp, C.c_double))
+    return sqf, sqfp
+
+
+def sq_columns(nsp, g: SqGrid, natoms, ntype, bsc, nconf, sqf, sqfp, nq):
+    nt = _i64(ntype)
+    bs = _f64(bsc)
+    sqf = _f64(sqf)
+    sqfp = _f64(sqfp)
+    nq = _i64(nq)
+    cols = np.zeros((g.nqmax, 7))
+    mix = np.zeros((g.nqmax, 1 + nsp))
+    nc = lib().or_sq_columns(nsp, g.nqmax, g.dq, natoms, _p(nt, C.c_int64), _p(bs, C.c_double),
+                             nconf, _p(sqf, C.c_double), _p(sqfp, C.c_double),
+                             _p(nq, C.c_int64), _p(cols, C.c_double), _p(mix, C.c_double))
+    cols = cols.reshape(-1)[: g.nqmax * nc].reshape(g.nqmax, nc)
+    return cols, mix

@@ -1,0 +1,137 @@
+/*
+ * trj_oracle.h -- CPU restatement of the trj_analysis structural hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links or calls
+ * this code; only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg use it, and only as the checker.
+ *
+ * Each function cites the reference file:line it restates (paths relative to
+ * the reference repository root).  Arithmetic precision is stated per function:
+ *   - the RDF pair rule is restated in FP64 with no FMA contraction;
+ *   - the q-grid / shell bookkeeping is restated in FP32, because the reference
+ *     does it in FP32 (myprec = sp, src/core/precision.f90:22) and the shipped
+ *     n(Q) columns depend on FP32 rounding on the qmin sphere;
+ *   - rho(q) is restated with long-double phases and sums (the FP64 oracle).
+ *
+ * Pinning status (what checks each function independently of its own formula):
+ *   or_ind_ij            closed form for nsp<=10, exhaustive (tests/test_oracle_rdf.py)
+ *   or_geometry          closed-form cases (cap, 2.5L non-periodic, volume)
+ *   or_wrap              invariants: result in [0,L), x - result is a multiple of L
+ *   or_rdf_brute         two-atom closed form; simple-cubic lattice shell counts
+ *                        (sums of three squares); pair-count totals; permutation
+ *                        and translation invariance
+ *   or_rdf_cells         equals or_rdf_brute on random frames (several boxes/cutoffs)
+ *   or_rdf_gmix          closed form of log.f90:489-518 on hand-made histograms;
+ *                        ideal-gas g -> 1 statistically
+ *   or_first_minimum     hand-made curves with a known parabola vertex
+ *   or_sq_setup/or_sq_nq golden n(Q) vectors in the reference's shipped outputs
+ *                        (examples/3D/LJmix, 3D/salr, 3D/meam-dyn, 2D/salr)
+ *   or_sq_member         brute force on tiny grids; shells partition the set
+ *   or_sq_rho            single atom |rho|^2 = 1; two atoms 2+2cos(q.dr);
+ *                        perfect lattice gives N^2 on reciprocal-lattice q and
+ *                        ~0 elsewhere; translation invariance
+ *   or_sq_columns        closed form of log.f90:327-399 on hand-made sums;
+ *                        binary-mixture identity S_NN = s11+s22+2 s12
+ * None is "parity unpinned".
+ */
+#ifndef TRJ_ORACLE_H
+#define TRJ_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Fortran NINT (round half away from zero) on doubles and floats. */
+long or_nint_d(double x);
+int or_nint_f(float x);
+
+/* rdf.cuf:741-753: 1-based upper-triangle index of species pair (i,j). */
+int or_ind_ij(int i, int j, int nsp);
+
+/* netcdf.f90:602-622 per-frame geometry.
+ * box: cell lengths; periodic: flags; rcrdf_inout: the sticky cutoff, capped to
+ * half the shortest side (netcdf.f90:610-612).  Outputs: lmi = minimum-image
+ * lengths (sidelv: 2.5*L on non-periodic axes, :603-609), side2 (:613-617),
+ * volume (:618-622). */
+void or_geometry(int ndim, const double box[3], const int32_t periodic[3],
+                 double* rcrdf_inout, double lmi[3], double* side2, double* volume);
+
+/* Coordinate wrap into [0,L) on periodic axes.  The reference truncates toward
+ * zero (netcdf.f90:681); this build floors, then folds the two rounding edge
+ * cases back into [0,L).  Non-periodic axes are copied unchanged. */
+double or_wrap1(double x, double L);
+
+/* rdf.cuf:55-58: lsmax = nint(rcrdf/deltar)+1, or from half the shortest side
+ * when rcrdf <= 0. */
+int or_rdf_lsmax(int ndim, const double box[3], double rcrdf, double deltar);
+
+/* rdf.cuf:440-475 (3D) and rdf.cuf:631-659 (2D): all unordered pairs i<j,
+ * minimum image d - Lmi*nint(d/Lmi), r2 < side2, k = nint(sqrt(r2)/deltar),
+ * kept if 1 <= k <= lsmax.  FP64, no FMA.  Positions must already be wrapped.
+ * counts[(ij-1)*lsmax + (k-1)] += 1 (layout of histomix_d(lsmax,nit)). */
+void or_rdf_brute(int64_t n, const double* x, const double* y, const double* z,
+                  const int32_t* itype, int ndim, int nsp, const double lmi[3],
+                  double side2, double deltar, int lsmax, int64_t* counts);
+
+/* Same result as or_rdf_brute, through linked cells (OpenMP over atoms). */
+void or_rdf_cells(int64_t n, const double* x, const double* y, const double* z,
+                  const int32_t* itype, int ndim, int nsp, const double box[3],
+                  const int32_t periodic[3], const double lmi[3], double side2,
+                  double deltar, int lsmax, int64_t* counts);
+
+/* log.f90:484-518: g_jl(r_i) for i=1..lsmax-2.  out has (lsmax-2) rows of
+ * (1 + nit) doubles: r, then g_11, g_12, ..., g_1n, g_22, ... */
+void or_rdf_gmix(int ndim, int nsp, int lsmax, double deltar, double volume,
+                 const int64_t* ntype, int64_t nconf, const int64_t* counts,
+                 double* out);
+
+/* rdf.cuf:313-342 running total g (3D/2D bulk) and rdf.cuf:755-822 first
+ * minimum with parabolic interpolation.  Returns rmin. */
+double or_first_minimum(int ndim, int nsp, int lsmax, double deltar, double volume,
+                        int64_t natoms, const int64_t* counts);
+
+/* sq.cuf:156-200: q-grid setup in FP32. */
+typedef struct {
+  float fk[3];     /* 2*pi/L per axis (sq.cuf:189) */
+  float dq;        /* sq.cuf:156-160 */
+  int nqmax;       /* sq.cuf:161 */
+  int l[3];        /* lmax per axis after the mod-4 (3D) / mod-16 (2D) adjust */
+  int nqmin;       /* sq.cuf:180/187, reset to 1 when 0 (sq.cuf:195-200) */
+  float qminp;     /* qmin after sq.cuf:191/193 */
+  float qmin2;     /* qmin**2 (sq.cuf:344) */
+  int kext[3];     /* thread-grid extent of sqfnD per axis (sq.cuf:175-179, 183-186) */
+} or_sqgrid;
+void or_sq_setup(int ndim, const double box[3], double qmax, double qmin, or_sqgrid* g);
+
+/* sq.cuf:303-343: host shell counts nq(0:nqmax). */
+void or_sq_nq(int ndim, const or_sqgrid* g, int64_t* nq);
+
+/* sq.cuf:533-555,578: shell of octant vector (kx,ky,kz) in the sqfnD sum set,
+ * or -1 when it is not summed. */
+int or_sq_member(int ndim, const or_sqgrid* g, int kx, int ky, int kz);
+
+/* Long-double rho_s(n) = sum_{j in species s} exp(i 2 pi (n.(r_j/L))), for nvec
+ * integer vectors kv[3*v..] (x,y,z).  itype 1..nsp.  rho out: [v][s][re,im]. */
+void or_sq_rho(int64_t n, const double* x, const double* y, const double* z,
+               const int32_t* itype, int nsp, const double box[3], int64_t nvec,
+               const int32_t* kv, double* rho);
+
+/* Reference sum set (octant below qmin + axis shells above, sq.cuf:58-126,
+ * 372-454, 523-591) for one frame, added into sqf[0..nqmax] and
+ * sqfp[s*(nqmax+1) + shell].  bsc: scattering lengths per species. */
+void or_sq_frame(int64_t n, const double* x, const double* y, const double* z,
+                 const int32_t* itype, int ndim, int nsp, const double box[3],
+                 const or_sqgrid* g, const double* bsc, double* sqf, double* sqfp);
+
+/* log.f90:317-401: printed columns.  Rows i=1..nqmax.  ncols_out = 3 (Q, S_NN,
+ * n) or 7 (Q, S_NN, S_cc, S_11, S_22, S_12, n) for a binary mixture with all
+ * bsc == 1 (log.f90:331,348,380-389); mix has nsp columns S_jj. */
+int or_sq_columns(int nsp, int nqmax, float dq, int64_t natoms, const int64_t* ntype,
+                  const double* bsc, int64_t nconf, const double* sqf, const double* sqfp,
+                  const int64_t* nq, double* cols, double* mix);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,89 @@
+// kernels.h -- launchers of the trj_hip device kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace trj {
+
+// Error bits raised by device kernels, read back at synchronisation points.
+constexpr uint32_t kFlagRange = 1u;  // non-periodic coordinate out of range / non-finite
+constexpr uint32_t kFlagType = 2u;   // species type outside 1..nsp
+
+struct IngestArgs {
+  const void* raw;       // positions as given
+  int dtype;             // TRJ_F32 / TRJ_F64
+  int layout;            // TRJ_AOS (stride ndim) / TRJ_SOA
+  const int32_t* itype;  // 1..nsp
+  int64_t n;
+  int ndim, nsp;
+  int periodic[3];
+  double box[3];
+  double* pos;           // out: wrapped FP64 SoA, 3*n
+  uint32_t* flags;
+};
+void launch_ingest(const IngestArgs& a, hipStream_t s);
+
+// Stable partition of atoms by species: per-block counts (species-major), a
+// scan, then a scatter of the S(Q) turns u = x / Lq into species order.
+void launch_species_count(const int32_t* itype, int64_t n, int nsp, int32_t* blk, int nblk,
+                          hipStream_t s);
+void launch_exclusive_scan(const int32_t* in, int32_t* out, int64_t count, hipStream_t s);
+void launch_species_offsets(const int32_t* blkoff, const int32_t* blk, int nblk, int nsp,
+                            int64_t n, int32_t* sp_off, hipStream_t s);
+void launch_species_scatter(const int32_t* itype, int64_t n, int nsp, const int32_t* blkoff,
+                            int nblk, const double* pos, const double lq[3], int ndim,
+                            double* u, hipStream_t s);
+
+// Cell list.
+struct CellGrid {
+  int nc[3];
+  double inv_h[3];
+  int periodic[3];
+};
+void launch_cell_count(const double* pos, int64_t n, int ndim, const CellGrid& g,
+                       int32_t* cellid, int32_t* count, hipStream_t s);
+void launch_cell_scatter(const double* pos, const int32_t* itype, const int32_t* cellid,
+                         int64_t n, const int32_t* start, int32_t* cursor, double* spos,
+                         int32_t* stype, hipStream_t s);
+
+struct PairArgs {
+  int ndim, nsp, nit, lsmax;
+  double lmi[3], half[3];
+  double side2;
+  float inv_dr;
+  const double* thr;  // lsmax+3 edges
+};
+size_t pair_lds_bytes(int nit, int lsmax);
+// Cell-list pairs (half stencil); every periodic axis needs nc >= 3.
+void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start,
+                      const double* spos, const int32_t* stype, int64_t n, int waves_per_cell,
+                      int nblocks, uint32_t* partial, hipStream_t s);
+// All pairs i<j (small boxes).
+void launch_rdf_allpairs(const PairArgs& a, const double* pos, const int32_t* itype, int64_t n,
+                         int nblocks, uint32_t* partial, hipStream_t s);
+// counts[b] += sum over slices of partial[slice][b].
+void launch_rdf_merge(const uint32_t* partial, int nslices, int nbins, int64_t* counts,
+                      hipStream_t s);
+
+// S(Q) contraction: rho_s(q) over the active tiles of the dense q-box.
+struct ContractArgs {
+  const double* u;        // species-sorted turns, 3*n (SoA)
+  const int32_t* sp_off;  // nsp+1
+  const int32_t* tiles;   // 4 ints per tile: kx0, ky0, kz0, pad
+  int ntiles, ksplit, nsp, ndim;
+  int64_t n;
+  double* rho;            // [split][s][tile][2048][2]
+};
+void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s);
+// Sum the splits, square, and bin into shells (FP64 atomics).
+void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, int ksplit,
+                      int nsp, const double* bsc, int nqmax, double* sqf, double* sqfp,
+                      hipStream_t s);
+// Explicit q-vectors (axis directions): direct FP64 phase sums.
+void launch_sq_list(const double* u, const int32_t* sp_off, int64_t n, int nsp, int ndim,
+                    const int32_t* kv, const int32_t* shell, int nvec, const double* bsc,
+                    int nqmax, double* sqf, double* sqfp, hipStream_t s);
+
+}  // namespace trj

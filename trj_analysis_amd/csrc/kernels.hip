@@ -1,0 +1,692 @@
+// kernels.hip -- gfx950 kernels of the trj_hip library.
+//
+// RDF: positions are wrapped in FP64 on the device, binned into a cell list
+// (counting sort), and each wavefront takes 64 atoms of one home cell and
+// walks the half stencil with wave-uniform j loads.  The pair rule of
+// rdf.cuf:452-463 is evaluated in FP64 without division or sqrt: the minimum
+// image switch point and the bin edges in r^2 are located exactly on the host
+// (plan.cpp), so every decision equals the FP64 oracle's.  Counts go to an
+// LDS histogram per workgroup (uint32), are written once per workgroup, and
+// merged into int64 totals.
+//
+// S(Q): the dense q-box is tiled 8 (kx) x 8 (ky) x 32 (kz); per tile
+// rho_s(q) = sum_j ex_j(kx) ey_j(ky) ez_j(kz) is a complex GEMM with
+// M = 64 (kx,ky) pairs, N = 32 kz values and K = atoms of species s, run on
+// v_mfma_f64_16x16x4_f64 (4 real products per complex product).  Phases come
+// from an exact FP64 reduction of n*u (u = x/L in turns, with the FMA error
+// term) and sincospi, extended by an 8-step angle-addition recurrence.
+// MFMA accumulators are folded into FP64 sums every 1024 atoms (blocked
+// summation keeps the rounding error of 1e5-atom sums near 1e-13).
+//
+// Compiled with -ffp-contract=off: every FP64 operation of the RDF pair rule
+// rounds once, in the order written, like the oracle.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/trj_hip.h"
+#include "kernels.h"
+#include "trj_internal.h"
+
+namespace trj {
+
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ inline double wrap_periodic(double x, double L) {
+  double f = floor(x / L);
+  double t = f * L;
+  double w = x - t;
+  if (w < 0.0) w = w + L;
+  if (w >= L) w = w - L;
+  return w;
+}
+
+__global__ __launch_bounds__(256) void k_ingest(IngestArgs a) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double c[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < a.ndim; ++k) {
+    const int64_t idx = (a.layout == TRJ_SOA) ? k * a.n + i : i * a.ndim + k;
+    c[k] = (a.dtype == TRJ_F32) ? static_cast<double>(static_cast<const float*>(a.raw)[idx])
+                                : static_cast<const double*>(a.raw)[idx];
+  }
+  uint32_t bad = 0;
+  for (int k = 0; k < a.ndim; ++k) {
+    if (!isfinite(c[k])) {
+      bad |= kFlagRange;
+    } else if (a.periodic[k]) {
+      c[k] = wrap_periodic(c[k], a.box[k]);
+    } else if (c[k] < -0.25 * a.box[k] || c[k] >= 1.25 * a.box[k]) {
+      bad |= kFlagRange;
+    }
+  }
+  const int t = a.itype[i];
+  if (t < 1 || t > a.nsp) bad |= kFlagType;
+  if (bad) atomicOr(a.flags, bad);
+  a.pos[i] = c[0];
+  a.pos[a.n + i] = c[1];
+  a.pos[2 * a.n + i] = c[2];
+}
+
+__global__ __launch_bounds__(256) void k_species_count(const int32_t* __restrict__ itype, int64_t n,
+                                                       int nsp, int32_t* __restrict__ blk, int nblk) {
+  __shared__ int cnt[4][kNspMax];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  int s = (i < n) ? itype[i] - 1 : -1;
+  if (s >= nsp) s = -1;
+  for (int sp = 0; sp < nsp; ++sp) {
+    const unsigned long long m = __ballot(s == sp);
+    if (lane == 0) cnt[w][sp] = __popcll(m);
+  }
+  __syncthreads();
+  if (static_cast<int>(threadIdx.x) < nsp) {
+    const int sp = threadIdx.x;
+    blk[static_cast<int64_t>(sp) * nblk + blockIdx.x] = cnt[0][sp] + cnt[1][sp] + cnt[2][sp] + cnt[3][sp];
+  }
+}
+
+// Single-workgroup exclusive scan (count up to a few 1e5 entries).
+__global__ __launch_bounds__(1024) void k_exclusive_scan(const int32_t* __restrict__ in,
+                                                         int32_t* __restrict__ out, int64_t count) {
+  __shared__ int32_t tmp[1024];
+  __shared__ int32_t carry;
+  const int t = threadIdx.x;
+  if (t == 0) carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < count; base += 1024) {
+    const int64_t i = base + t;
+    const int32_t v = (i < count) ? in[i] : 0;
+    tmp[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int32_t x = (t >= off) ? tmp[t - off] : 0;
+      __syncthreads();
+      tmp[t] += x;
+      __syncthreads();
+    }
+    if (i < count) out[i] = carry + tmp[t] - v;
+    __syncthreads();
+    if (t == 1023) carry += tmp[1023];
+    __syncthreads();
+  }
+}
+
+__global__ void k_species_offsets(const int32_t* __restrict__ blkoff, const int32_t* __restrict__ blk,
+                                  int nblk, int nsp, int32_t* __restrict__ sp_off) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int s = 0; s < nsp; ++s) sp_off[s] = blkoff[static_cast<int64_t>(s) * nblk];
+  const int64_t last = static_cast<int64_t>(nsp) * nblk - 1;
+  sp_off[nsp] = blkoff[last] + blk[last];
+}
+
+__global__ __launch_bounds__(256) void k_species_scatter(const int32_t* __restrict__ itype, int64_t n,
+                                                         int nsp, const int32_t* __restrict__ blkoff,
+                                                         int nblk, const double* __restrict__ pos,
+                                                         double lq0, double lq1, double lq2, int ndim,
+                                                         double* __restrict__ u) {
+  __shared__ int cnt[4][kNspMax];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  int s = (i < n) ? itype[i] - 1 : -1;
+  if (s >= nsp) s = -1;
+  unsigned long long mine = 0;
+  for (int sp = 0; sp < nsp; ++sp) {
+    const unsigned long long m = __ballot(s == sp);
+    if (lane == 0) cnt[w][sp] = __popcll(m);
+    if (s == sp) mine = m;
+  }
+  __syncthreads();
+  if (s < 0) return;
+  int off = blkoff[static_cast<int64_t>(s) * nblk + blockIdx.x];
+  for (int ww = 0; ww < w; ++ww) off += cnt[ww][s];
+  off += __popcll(mine & ((1ull << lane) - 1ull));
+  u[off] = pos[i] / lq0;
+  u[n + off] = pos[n + i] / lq1;
+  u[2 * n + off] = (ndim == 3) ? pos[2 * n + i] / lq2 : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_cell_count(const double* __restrict__ pos, int64_t n, int ndim,
+                                                    CellGrid g, int32_t* __restrict__ cellid,
+                                                    int32_t* __restrict__ count) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  int c[3] = {0, 0, 0};
+  for (int k = 0; k < ndim; ++k) {
+    const double x = pos[k * n + i];
+    int ck = static_cast<int>(floor(x * g.inv_h[k]));
+    ck = ck < 0 ? 0 : (ck >= g.nc[k] ? g.nc[k] - 1 : ck);
+    c[k] = ck;
+  }
+  const int id = (c[2] * g.nc[1] + c[1]) * g.nc[0] + c[0];
+  cellid[i] = id;
+  atomicAdd(&count[id], 1);
+}
+
+__global__ __launch_bounds__(256) void k_cell_scatter(const double* __restrict__ pos,
+                                                      const int32_t* __restrict__ itype,
+                                                      const int32_t* __restrict__ cellid, int64_t n,
+                                                      const int32_t* __restrict__ start,
+                                                      int32_t* __restrict__ cursor,
+                                                      double* __restrict__ spos,
+                                                      int32_t* __restrict__ stype) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int c = cellid[i];
+  const int slot = start[c] + atomicAdd(&cursor[c], 1);
+  spos[slot] = pos[i];
+  spos[n + slot] = pos[n + i];
+  spos[2 * n + slot] = pos[2 * n + i];
+  stype[slot] = itype[i];
+}
+
+// rdf.cuf:748-752 for i <= j (callers order the pair).
+__device__ inline int pair_index(int a, int b, int nsp) {
+  return (a - 1) * (nsp - 1) - (a - 1) * (a - 2) / 2 + b;
+}
+
+// nint(sqrt(r2)/deltar) through the exact edges thr[0..lsmax+2]
+// (thr[0] = 0, thr[lsmax+2] = +inf).
+__device__ inline int bin_of(double r2, const double* thr, float inv_dr, int lsmax) {
+  int k = static_cast<int>(sqrtf(static_cast<float>(r2)) * inv_dr + 0.5f);
+  k = k > lsmax + 1 ? lsmax + 1 : k;
+  while (k <= lsmax && r2 >= thr[k + 1]) ++k;
+  while (k > 0 && r2 < thr[k]) --k;
+  return k;
+}
+
+__device__ inline double min_image(double d, double L, double h) {
+  return (d >= h) ? d - L : ((d <= -h) ? d + L : d);
+}
+
+struct Stencil {
+  int n;
+  int off[14][3];
+};
+
+__device__ inline void pair_accumulate(const PairArgs& a, const double* thr, uint32_t* hist,
+                                       bool ok, double xi, double yi, double zi, int ti, double xj,
+                                       double yj, double zj, int tj) {
+  const double dx = min_image(xj - xi, a.lmi[0], a.half[0]);
+  const double dy = min_image(yj - yi, a.lmi[1], a.half[1]);
+  double r2;
+  if (a.ndim == 3) {
+    const double dz = min_image(zj - zi, a.lmi[2], a.half[2]);
+    r2 = dx * dx + dy * dy + dz * dz;
+  } else {
+    r2 = dx * dx + dy * dy;
+  }
+  if (ok && r2 < a.side2) {
+    const int k = bin_of(r2, thr, a.inv_dr, a.lsmax);
+    if (k >= 1 && k <= a.lsmax) {
+      const int lo = ti < tj ? ti : tj, hi = ti < tj ? tj : ti;
+      const int ij = pair_index(lo, hi, a.nsp);
+      atomicAdd(&hist[(ij - 1) * a.lsmax + (k - 1)], 1u);
+    }
+  }
+}
+
+__device__ inline void hist_setup(const PairArgs& a, uint32_t* hist, double* thr) {
+  const int nbins = a.nit * a.lsmax;
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) hist[b] = 0u;
+  for (int b = threadIdx.x; b < a.lsmax + 3; b += blockDim.x) thr[b] = a.thr[b];
+  __syncthreads();
+}
+
+__device__ inline void hist_flush(const PairArgs& a, const uint32_t* hist, uint32_t* partial) {
+  __syncthreads();
+  const int nbins = a.nit * a.lsmax;
+  uint32_t* out = partial + static_cast<int64_t>(blockIdx.x) * nbins;
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) out[b] = hist[b];
+}
+
+__global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stencil st,
+                                                   const int32_t* __restrict__ start,
+                                                   const double* __restrict__ spos,
+                                                   const int32_t* __restrict__ stype, int64_t n,
+                                                   int W, int64_t ntasks,
+                                                   uint32_t* __restrict__ partial) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nbins = a.nit * a.lsmax;
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
+  double* thr = reinterpret_cast<double*>(smem + ((static_cast<size_t>(nbins) * 4 + 15) & ~size_t(15)));
+  hist_setup(a, hist, thr);
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const double* __restrict__ sx = spos;
+  const double* __restrict__ sy = spos + n;
+  const double* __restrict__ sz = spos + 2 * n;
+  const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
+  for (int64_t task = static_cast<int64_t>(blockIdx.x) * 4 + wave; task < ntasks;
+       task += static_cast<int64_t>(gridDim.x) * 4) {
+    const int cell = static_cast<int>(task / W);
+    const int ch0 = static_cast<int>(task % W);
+    const int cs = start[cell], ce = start[cell + 1];
+    const int cx = cell % ncx, cy = (cell / ncx) % ncy, cz = cell / (ncx * ncy);
+    for (int ch = ch0; cs + ch * kWave < ce; ch += W) {
+      const int gi = cs + ch * kWave + lane;
+      const bool vi = gi < ce;
+      const double xi = vi ? sx[gi] : 0.0;
+      const double yi = vi ? sy[gi] : 0.0;
+      const double zi = vi ? sz[gi] : 0.0;
+      const int ti = vi ? stype[gi] : 0;
+      const bool oki = vi && static_cast<unsigned>(ti - 1) < static_cast<unsigned>(a.nsp);
+      for (int o = 0; o < st.n; ++o) {
+        int nx = cx + st.off[o][0], ny = cy + st.off[o][1], nz = cz + st.off[o][2];
+        if (g.periodic[0]) nx = (nx + ncx) % ncx; else if (nx < 0 || nx >= ncx) continue;
+        if (g.periodic[1]) ny = (ny + ncy) % ncy; else if (ny < 0 || ny >= ncy) continue;
+        if (g.periodic[2]) nz = (nz + ncz) % ncz; else if (nz < 0 || nz >= ncz) continue;
+        const int nb = (nz * ncy + ny) * ncx + nx;
+        int js = start[nb];
+        const int je = start[nb + 1];
+        const bool self = (o == 0);
+        if (self) js = cs + ch * kWave + 1;
+        for (int j = js; j < je; ++j) {
+          const int tj = stype[j];
+          const bool ok = oki && (!self || j > gi) &&
+                          static_cast<unsigned>(tj - 1) < static_cast<unsigned>(a.nsp);
+          pair_accumulate(a, thr, hist, ok, xi, yi, zi, ti, sx[j], sy[j], sz[j], tj);
+        }
+      }
+    }
+  }
+  hist_flush(a, hist, partial);
+}
+
+__global__ __launch_bounds__(256) void k_rdf_allpairs(PairArgs a, const double* __restrict__ pos,
+                                                      const int32_t* __restrict__ itype, int64_t n,
+                                                      int nb, uint32_t* __restrict__ partial) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nbins = a.nit * a.lsmax;
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
+  double* thr = reinterpret_cast<double*>(smem + ((static_cast<size_t>(nbins) * 4 + 15) & ~size_t(15)));
+  hist_setup(a, hist, thr);
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const double* __restrict__ px = pos;
+  const double* __restrict__ py = pos + n;
+  const double* __restrict__ pz = pos + 2 * n;
+  const int64_t ntasks = static_cast<int64_t>(nb) * nb;
+  for (int64_t task = static_cast<int64_t>(blockIdx.x) * 4 + wave; task < ntasks;
+       task += static_cast<int64_t>(gridDim.x) * 4) {
+    const int ib = static_cast<int>(task / nb), jb = static_cast<int>(task % nb);
+    if (jb < ib) continue;
+    const int64_t gi = static_cast<int64_t>(ib) * kWave + lane;
+    const bool vi = gi < n;
+    const double xi = vi ? px[gi] : 0.0;
+    const double yi = vi ? py[gi] : 0.0;
+    const double zi = vi ? pz[gi] : 0.0;
+    const int ti = vi ? itype[gi] : 0;
+    const bool oki = vi && static_cast<unsigned>(ti - 1) < static_cast<unsigned>(a.nsp);
+    const int64_t js = static_cast<int64_t>(jb) * kWave;
+    const int64_t je = js + kWave < n ? js + kWave : n;
+    for (int64_t j = js; j < je; ++j) {
+      const int tj = itype[j];
+      const bool ok = oki && (jb > ib || j > gi) &&
+                      static_cast<unsigned>(tj - 1) < static_cast<unsigned>(a.nsp);
+      pair_accumulate(a, thr, hist, ok, xi, yi, zi, ti, px[j], py[j], pz[j], tj);
+    }
+  }
+  hist_flush(a, hist, partial);
+}
+
+__global__ __launch_bounds__(256) void k_rdf_merge(const uint32_t* __restrict__ partial, int nslices,
+                                                   int nbins, int64_t* __restrict__ counts) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= nbins) return;
+  uint64_t s = 0;
+  for (int sl = 0; sl < nslices; ++sl) s += partial[static_cast<int64_t>(sl) * nbins + b];
+  counts[b] += static_cast<int64_t>(s);
+}
+
+// ------------------------------------------------------------------ S(Q)
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int KC = 32;            // atoms staged per chunk
+constexpr int kFoldChunks = 32;   // fold MFMA accumulators every KC*32 = 1024 atoms
+
+// Exact-reduction phase in turns: frac(n*u) with the product's rounding error.
+__device__ inline double frac_turns(int nk, double u) {
+  const double dn = static_cast<double>(nk);
+  const double hi = dn * u;
+  const double lo = fma(dn, u, -hi);
+  return (hi - rint(hi)) + lo;
+}
+
+// e^{2 pi i n u} for n = n0 .. n0+7 into dst (anchor + angle-addition steps).
+__device__ inline void phase8(double u, int n0, double2* dst) {
+  double s, c, ss, cs;
+  sincospi(2.0 * frac_turns(n0, u), &s, &c);
+  sincospi(2.0 * (u - rint(u)), &ss, &cs);
+  double re = c, im = s;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    dst[k] = make_double2(re, im);
+    const double nr = fma(re, cs, -(im * ss));
+    const double ni = fma(re, ss, im * cs);
+    re = nr;
+    im = ni;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sq_contract_f64(ContractArgs a) {
+  __shared__ double2 phx[KC][TKX];
+  __shared__ double2 phy[KC][TKY];
+  __shared__ double2 phz[KC][TKZ];
+  const int tile = blockIdx.x % a.ntiles;
+  const int split = blockIdx.x / a.ntiles;
+  const int kx0 = a.tiles[4 * tile], ky0 = a.tiles[4 * tile + 1], kz0 = a.tiles[4 * tile + 2];
+  const int t = threadIdx.x, w = t / kWave, lane = t & (kWave - 1);
+  const int i16 = lane & 15, kq = lane >> 4;
+  const int m = w * 16 + i16, mx = m / TKY, my = m % TKY;
+  const double* __restrict__ ux = a.u;
+  const double* __restrict__ uy = a.u + a.n;
+  const double* __restrict__ uz = a.u + 2 * a.n;
+  // Phase-staging role of this thread: atom at, segment seg (0: x, 1: y, 2-5: z).
+  const int st_at = t / 6, st_seg = t % 6;
+  for (int s = 0; s < a.nsp; ++s) {
+    const int a0 = a.sp_off[s], a1 = a.sp_off[s + 1];
+    const int64_t len = a1 - a0;
+    const int b0 = a0 + static_cast<int>(len * split / a.ksplit);
+    const int b1 = a0 + static_cast<int>(len * (split + 1) / a.ksplit);
+    d4 accR[2], accI[2], sumR[2], sumI[2];
+    for (int nb = 0; nb < 2; ++nb) {
+      accR[nb] = d4{0.0, 0.0, 0.0, 0.0};
+      accI[nb] = accR[nb];
+      sumR[nb] = accR[nb];
+      sumI[nb] = accR[nb];
+    }
+    int chunks = 0;
+    for (int c0 = b0; c0 < b1; c0 += KC) {
+      __syncthreads();
+      if (t < KC * 6) {
+        const int ai = c0 + st_at;
+        double2* dst;
+        int n0;
+        const double* src;
+        if (st_seg == 0) {
+          dst = phx[st_at];
+          n0 = kx0;
+          src = ux;
+        } else if (st_seg == 1) {
+          dst = phy[st_at];
+          n0 = ky0;
+          src = uy;
+        } else {
+          dst = phz[st_at] + 8 * (st_seg - 2);
+          n0 = kz0 + 8 * (st_seg - 2);
+          src = uz;
+        }
+        if (ai < b1) {
+          phase8(src[ai], n0, dst);
+        } else {
+#pragma unroll
+          for (int k = 0; k < 8; ++k) dst[k] = make_double2(0.0, 0.0);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kk = 0; kk < KC / 4; ++kk) {
+        const int at = kk * 4 + kq;
+        const double2 ex = phx[at][mx];
+        const double2 ey = phy[at][my];
+        const double ar = fma(ex.x, ey.x, -(ex.y * ey.y));
+        const double ai = fma(ex.x, ey.y, ex.y * ey.x);
+        const double2 bz0 = phz[at][i16];
+        const double2 bz1 = phz[at][16 + i16];
+        accR[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bz0.x, accR[0], 0, 0, 0);
+        accR[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bz0.y, accR[0], 0, 0, 0);
+        accI[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bz0.y, accI[0], 0, 0, 0);
+        accI[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bz0.x, accI[0], 0, 0, 0);
+        accR[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bz1.x, accR[1], 0, 0, 0);
+        accR[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bz1.y, accR[1], 0, 0, 0);
+        accI[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bz1.y, accI[1], 0, 0, 0);
+        accI[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bz1.x, accI[1], 0, 0, 0);
+      }
+      if (++chunks == kFoldChunks) {
+        chunks = 0;
+        for (int nb = 0; nb < 2; ++nb) {
+          sumR[nb] += accR[nb];
+          sumI[nb] += accI[nb];
+          accR[nb] = d4{0.0, 0.0, 0.0, 0.0};
+          accI[nb] = accR[nb];
+        }
+      }
+    }
+    for (int nb = 0; nb < 2; ++nb) {
+      sumR[nb] += accR[nb];
+      sumI[nb] += accI[nb];
+    }
+    // D layout of v_mfma_f64_16x16x4_f64: lane holds rows 4*(lane/16)+r, column lane%16.
+    const size_t base = ((static_cast<size_t>(split) * a.nsp + s) * a.ntiles + tile) * TILE_VEC;
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int mrow = w * 16 + 4 * kq + r;
+        const int v = ((mrow / TKY) * TKY + (mrow % TKY)) * TKZ + nb * 16 + i16;
+        double2 val = make_double2(sumR[nb][r], sumI[nb][r]);
+        reinterpret_cast<double2*>(a.rho)[base + v] = val;
+      }
+    }
+  }
+}
+
+struct Bsc {
+  double b[kNspMax];
+};
+
+__global__ __launch_bounds__(256) void k_sq_shells(const double* __restrict__ rho,
+                                                   const int16_t* __restrict__ tile_shell, int ntiles,
+                                                   int ksplit, int nsp, Bsc bsc, int nqmax,
+                                                   double* __restrict__ sqf, double* __restrict__ sqfp,
+                                                   int use_lds) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* acc = reinterpret_cast<double*>(smem);  // [(1+nsp)][nqmax+1]
+  const int ns = nqmax + 1;
+  if (use_lds) {
+    for (int b = threadIdx.x; b < (1 + nsp) * ns; b += 256) acc[b] = 0.0;
+    __syncthreads();
+  }
+  const int64_t total = static_cast<int64_t>(ntiles) * TILE_VEC;
+  const double2* r2 = reinterpret_cast<const double2*>(rho);
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; e < total;
+       e += static_cast<int64_t>(gridDim.x) * 256) {
+    const int sh = tile_shell[e];
+    if (sh < 0) continue;
+    double tr = 0.0, ti = 0.0;
+    for (int s = 0; s < nsp; ++s) {
+      double rr = 0.0, ri = 0.0;
+      for (int sp = 0; sp < ksplit; ++sp) {
+        const double2 v = r2[(static_cast<int64_t>(sp) * nsp + s) * total + e];
+        rr += v.x;
+        ri += v.y;
+      }
+      const double p = rr * rr + ri * ri;
+      if (use_lds)
+        atomicAdd(&acc[(1 + s) * ns + sh], p);
+      else
+        atomicAdd(&sqfp[static_cast<int64_t>(s) * ns + sh], p);
+      tr += bsc.b[s] * rr;
+      ti += bsc.b[s] * ri;
+    }
+    const double pt = tr * tr + ti * ti;
+    if (use_lds)
+      atomicAdd(&acc[sh], pt);
+    else
+      atomicAdd(&sqf[sh], pt);
+  }
+  if (use_lds) {
+    __syncthreads();
+    for (int b = threadIdx.x; b < (1 + nsp) * ns; b += 256) {
+      const double v = acc[b];
+      if (v != 0.0) {
+        if (b < ns)
+          atomicAdd(&sqf[b], v);
+        else
+          atomicAdd(&sqfp[b - ns], v);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sq_list(const double* __restrict__ u,
+                                                 const int32_t* __restrict__ sp_off, int64_t n, int nsp,
+                                                 int ndim, const int32_t* __restrict__ kv,
+                                                 const int32_t* __restrict__ shell, Bsc bsc, int nqmax,
+                                                 double* __restrict__ sqf, double* __restrict__ sqfp) {
+  __shared__ double red[2][256];
+  const int v = blockIdx.x;
+  const int kx = kv[3 * v], ky = kv[3 * v + 1], kz = kv[3 * v + 2];
+  const int sh = shell[v];
+  const int ns = nqmax + 1;
+  double tr = 0.0, ti = 0.0;
+  for (int s = 0; s < nsp; ++s) {
+    double pr = 0.0, pi = 0.0;
+    for (int j = sp_off[s] + threadIdx.x; j < sp_off[s + 1]; j += 256) {
+      double f = frac_turns(kx, u[j]) + frac_turns(ky, u[n + j]);
+      if (ndim == 3) f += frac_turns(kz, u[2 * n + j]);
+      f -= rint(f);
+      double sn, cn;
+      sincospi(2.0 * f, &sn, &cn);
+      pr += cn;
+      pi += sn;
+    }
+    red[0][threadIdx.x] = pr;
+    red[1][threadIdx.x] = pi;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if (static_cast<int>(threadIdx.x) < off) {
+        red[0][threadIdx.x] += red[0][threadIdx.x + off];
+        red[1][threadIdx.x] += red[1][threadIdx.x + off];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const double rr = red[0][0], ri = red[1][0];
+      atomicAdd(&sqfp[static_cast<int64_t>(s) * ns + sh], rr * rr + ri * ri);
+      tr += bsc.b[s] * rr;
+      ti += bsc.b[s] * ri;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(&sqf[sh], tr * tr + ti * ti);
+}
+
+inline unsigned grid1d(int64_t n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
+
+}  // namespace
+
+void launch_ingest(const IngestArgs& a, hipStream_t s) {
+  if (a.n <= 0) return;
+  hipLaunchKernelGGL(k_ingest, dim3(grid1d(a.n, 256)), dim3(256), 0, s, a);
+}
+
+void launch_species_count(const int32_t* itype, int64_t n, int nsp, int32_t* blk, int nblk,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_species_count, dim3(nblk), dim3(256), 0, s, itype, n, nsp, blk, nblk);
+}
+
+void launch_exclusive_scan(const int32_t* in, int32_t* out, int64_t count, hipStream_t s) {
+  hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, s, in, out, count);
+}
+
+void launch_species_offsets(const int32_t* blkoff, const int32_t* blk, int nblk, int nsp, int64_t n,
+                            int32_t* sp_off, hipStream_t s) {
+  (void)n;
+  hipLaunchKernelGGL(k_species_offsets, dim3(1), dim3(64), 0, s, blkoff, blk, nblk, nsp, sp_off);
+}
+
+void launch_species_scatter(const int32_t* itype, int64_t n, int nsp, const int32_t* blkoff, int nblk,
+                            const double* pos, const double lq[3], int ndim, double* u, hipStream_t s) {
+  hipLaunchKernelGGL(k_species_scatter, dim3(nblk), dim3(256), 0, s, itype, n, nsp, blkoff, nblk, pos,
+                     lq[0], lq[1], lq[2], ndim, u);
+}
+
+void launch_cell_count(const double* pos, int64_t n, int ndim, const CellGrid& g, int32_t* cellid,
+                       int32_t* count, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_cell_count, dim3(grid1d(n, 256)), dim3(256), 0, s, pos, n, ndim, g, cellid, count);
+}
+
+void launch_cell_scatter(const double* pos, const int32_t* itype, const int32_t* cellid, int64_t n,
+                         const int32_t* start, int32_t* cursor, double* spos, int32_t* stype,
+                         hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_cell_scatter, dim3(grid1d(n, 256)), dim3(256), 0, s, pos, itype, cellid, n, start,
+                     cursor, spos, stype);
+}
+
+size_t pair_lds_bytes(int nit, int lsmax) {
+  return ((static_cast<size_t>(nit) * lsmax * 4 + 15) & ~size_t(15)) + static_cast<size_t>(lsmax + 3) * 8;
+}
+
+void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start, const double* spos,
+                      const int32_t* stype, int64_t n, int waves_per_cell, int nblocks,
+                      uint32_t* partial, hipStream_t s) {
+  Stencil st{};
+  int k = 0;
+  st.off[k][0] = st.off[k][1] = st.off[k][2] = 0;
+  ++k;
+  const int zr = (a.ndim == 3) ? 1 : 0;
+  for (int dz = -zr; dz <= zr; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const bool fwd = (dz > 0) || (dz == 0 && dy > 0) || (dz == 0 && dy == 0 && dx > 0);
+        if (!fwd) continue;
+        st.off[k][0] = dx;
+        st.off[k][1] = dy;
+        st.off[k][2] = dz;
+        ++k;
+      }
+  st.n = k;
+  const int64_t ncell = static_cast<int64_t>(g.nc[0]) * g.nc[1] * g.nc[2];
+  const int64_t ntasks = ncell * waves_per_cell;
+  hipLaunchKernelGGL(k_rdf_cells, dim3(nblocks), dim3(256), pair_lds_bytes(a.nit, a.lsmax), s, a, g, st,
+                     start, spos, stype, n, waves_per_cell, ntasks, partial);
+}
+
+void launch_rdf_allpairs(const PairArgs& a, const double* pos, const int32_t* itype, int64_t n,
+                         int nblocks, uint32_t* partial, hipStream_t s) {
+  const int nb = static_cast<int>((n + kWave - 1) / kWave);
+  hipLaunchKernelGGL(k_rdf_allpairs, dim3(nblocks), dim3(256), pair_lds_bytes(a.nit, a.lsmax), s, a, pos,
+                     itype, n, nb, partial);
+}
+
+void launch_rdf_merge(const uint32_t* partial, int nslices, int nbins, int64_t* counts, hipStream_t s) {
+  hipLaunchKernelGGL(k_rdf_merge, dim3(grid1d(nbins, 256)), dim3(256), 0, s, partial, nslices, nbins,
+                     counts);
+}
+
+void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s) {
+  if (a.ntiles <= 0) return;
+  hipLaunchKernelGGL(k_sq_contract_f64, dim3(a.ntiles * a.ksplit), dim3(256), 0, s, a);
+}
+
+void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, int ksplit, int nsp,
+                      const double* bsc, int nqmax, double* sqf, double* sqfp, hipStream_t s) {
+  if (ntiles <= 0) return;
+  Bsc b{};
+  for (int i = 0; i < nsp; ++i) b.b[i] = bsc[i];
+  const size_t lds = static_cast<size_t>(1 + nsp) * (nqmax + 1) * sizeof(double);
+  const int use_lds = lds <= 48 * 1024;
+  const int64_t total = static_cast<int64_t>(ntiles) * TILE_VEC;
+  int64_t g = (total + 255) / 256;
+  if (g > 1024) g = 1024;
+  hipLaunchKernelGGL(k_sq_shells, dim3(static_cast<unsigned>(g)), dim3(256), use_lds ? lds : 0, s, rho,
+                     tile_shell, ntiles, ksplit, nsp, b, nqmax, sqf, sqfp, use_lds);
+}
+
+void launch_sq_list(const double* u, const int32_t* sp_off, int64_t n, int nsp, int ndim, const int32_t* kv,
+                    const int32_t* shell, int nvec, const double* bsc, int nqmax, double* sqf, double* sqfp,
+                    hipStream_t s) {
+  if (nvec <= 0) return;
+  Bsc b{};
+  for (int i = 0; i < nsp; ++i) b.b[i] = bsc[i];
+  hipLaunchKernelGGL(k_sq_list, dim3(nvec), dim3(256), 0, s, u, sp_off, n, nsp, ndim, kv, shell, b, nqmax,
+                     sqf, sqfp);
+}
+
+}  // namespace trj

@@ -1,0 +1,813 @@
+// trj_ctx.hip -- context, frame pipeline and C ABI of the trj_hip library
+// (declared in include/trj_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/trj_hip.h"
+#include "kernels.h"
+#include "trj_internal.h"
+
+using namespace trj;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+struct EventSet {
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  bool pending = false;
+  bool has_rdf = false, has_sq = false;
+};
+
+constexpr int kEventRing = 8;
+constexpr size_t kPairLdsLimit = 64 * 1024;
+
+}  // namespace
+
+struct trj_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // device buffers
+  DevBuf raw, itype, pos, spos, stype, cellid, cellcount, cellstart, cursor, partial, counts, thr, blk,
+      blkoff, sp_off, u, rho, tiles, tile_shell, sqf, sqfp, axis_kv, axis_shell, flags;
+  // pinned staging for host inputs (double-buffered)
+  void* pinned[2] = {nullptr, nullptr};
+  size_t pinned_cap[2] = {0, 0};
+  hipEvent_t pinned_done[2] = {nullptr, nullptr};
+  int pinned_next = 0;
+  // RDF
+  bool rdf_ready = false;
+  RdfPlan rdf;
+  int64_t rdf_frames = 0;
+  // S(Q)
+  bool sq_ready = false;
+  SqPlan sq;
+  int64_t sq_frames = 0;
+  int ksplit = 1;
+  std::unordered_map<int64_t, int> tile_index;  // packed (kx0,ky0,kz0) -> active tile
+  // last frame
+  int64_t last_n = 0;
+  double last_volume = 0.0;
+  int last_nsp = 0;
+  // timing
+  EventSet evs[kEventRing];
+  int ev_next = 0;
+  double last_ms[4] = {0, 0, 0, 0};
+  double tot_ms[4] = {0, 0, 0, 0};
+  int64_t tot_frames = 0;
+};
+
+namespace {
+
+int fail(trj_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPC(ctx, call)                                                                   \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail((ctx), TRJ_EDEVICE, std::string(#call " failed: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+int ensure(trj_ctx* c, DevBuf& b, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (b.cap >= bytes) return TRJ_OK;
+  if (b.p) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, TRJ_EDEVICE, hipGetErrorString(e));
+    hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  size_t want = bytes + bytes / 8;
+  hipError_t e = hipMalloc(&b.p, want);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return fail(c, TRJ_ENOMEM, "device allocation of " + std::to_string(want) + " bytes failed");
+  }
+  b.cap = want;
+  return TRJ_OK;
+}
+
+template <class T>
+T* P(DevBuf& b) {
+  return static_cast<T*>(b.p);
+}
+
+void harvest(trj_ctx* c, EventSet& s) {
+  if (!s.pending) return;
+  hipEventSynchronize(s.ev[4]);
+  float ms[4] = {0, 0, 0, 0};
+  hipEventElapsedTime(&ms[3], s.ev[0], s.ev[1]);  // ingest
+  hipEventElapsedTime(&ms[0], s.ev[1], s.ev[2]);  // RDF
+  hipEventElapsedTime(&ms[1], s.ev[2], s.ev[3]);  // S(Q) contraction
+  hipEventElapsedTime(&ms[2], s.ev[3], s.ev[4]);  // S(Q) shells + axis vectors
+  for (int k = 0; k < 4; ++k) {
+    c->last_ms[k] = ms[k];
+    c->tot_ms[k] += ms[k];
+  }
+  c->tot_frames += 1;
+  s.pending = false;
+}
+
+void harvest_all(trj_ctx* c) {
+  // oldest first so that last_ms ends with the newest frame
+  for (int k = 0; k < kEventRing; ++k) harvest(c, c->evs[(c->ev_next + k) % kEventRing]);
+}
+
+int check_flags(trj_ctx* c) {
+  uint32_t f = 0;
+  if (!c->flags.p) return TRJ_OK;
+  HIPC(c, hipMemcpyAsync(&f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  if (f) {
+    HIPC(c, hipMemsetAsync(c->flags.p, 0, sizeof(uint32_t), c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (f & kFlagType) return fail(c, TRJ_EINVAL, "a frame had species types outside 1..nsp");
+    return fail(c, TRJ_ERANGE,
+                "a frame had non-finite coordinates, or coordinates of a non-periodic axis "
+                "outside [-0.25 L, 1.25 L)");
+  }
+  return TRJ_OK;
+}
+
+int sync_all(trj_ctx* c) {
+  HIPC(c, hipStreamSynchronize(c->stream));
+  harvest_all(c);
+  return check_flags(c);
+}
+
+int64_t pack_tile(int x, int y, int z) {
+  return ((static_cast<int64_t>(x) + 4096) << 40) | ((static_cast<int64_t>(y) + 4096) << 20) |
+         (static_cast<int64_t>(z) + 4096);
+}
+
+int validate_box(trj_ctx* c, int ndim, const double box[3]) {
+  if (!box) return fail(c, TRJ_EINVAL, "box is NULL");
+  for (int k = 0; k < ndim; ++k)
+    if (!(box[k] > 0.0) || !std::isfinite(box[k])) return fail(c, TRJ_EINVAL, "box lengths must be positive");
+  return TRJ_OK;
+}
+
+// The frame pipeline, device inputs.
+int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int32_t* itype_dev, int64_t n,
+              const double box_in[3], uint32_t what) {
+  const bool do_rdf = (what & TRJ_WANT_RDF) != 0;
+  const bool do_sq = (what & TRJ_WANT_SQ) != 0;
+  if (do_rdf && !c->rdf_ready) return fail(c, TRJ_ESTATE, "frame asks for RDF before trj_rdf_init");
+  if (do_sq && !c->sq_ready) return fail(c, TRJ_ESTATE, "frame asks for S(Q) before trj_sq_init");
+  if (!do_rdf && !do_sq) return fail(c, TRJ_EINVAL, "frame asks for nothing");
+  const int ndim = do_rdf ? c->rdf.ndim : c->sq.p.ndim;
+  const int nsp = do_rdf ? c->rdf.nsp : c->sq.p.nsp;
+  if (do_rdf && do_sq && (c->rdf.ndim != c->sq.p.ndim || c->rdf.nsp != c->sq.p.nsp))
+    return fail(c, TRJ_EINVAL, "RDF and S(Q) were initialised with different ndim/nsp");
+  if (n < 1) return fail(c, TRJ_EINVAL, "n must be positive");
+  if (n > (int64_t(1) << 31) - 1024) return fail(c, TRJ_ELIMIT, "n above 2^31 is not supported");
+  if (dtype != TRJ_F32 && dtype != TRJ_F64) return fail(c, TRJ_EINVAL, "bad dtype");
+  if (layout != TRJ_AOS && layout != TRJ_SOA) return fail(c, TRJ_EINVAL, "bad layout");
+  if (int r = validate_box(c, ndim, box_in)) return r;
+  double box[3] = {box_in[0], box_in[1], ndim == 3 ? box_in[2] : 1.0};
+  int periodic[3] = {1, 1, 1};
+  if (do_rdf)
+    for (int k = 0; k < 3; ++k) periodic[k] = c->rdf.periodic[k];
+  else
+    for (int k = 0; k < 3; ++k) periodic[k] = 1;
+
+  EventSet& es = c->evs[c->ev_next];
+  harvest(c, es);
+  c->ev_next = (c->ev_next + 1) % kEventRing;
+  HIPC(c, hipEventRecord(es.ev[0], c->stream));
+
+  // Ingest: wrapped FP64 SoA, species offsets.
+  int r;
+  if ((r = ensure(c, c->pos, sizeof(double) * 3 * n))) return r;
+  if ((r = ensure(c, c->flags, sizeof(uint32_t)))) return r;
+  IngestArgs ia{};
+  ia.raw = pos_dev;
+  ia.dtype = dtype;
+  ia.layout = layout;
+  ia.itype = itype_dev;
+  ia.n = n;
+  ia.ndim = ndim;
+  ia.nsp = nsp;
+  for (int k = 0; k < 3; ++k) {
+    ia.periodic[k] = periodic[k];
+    ia.box[k] = box[k];
+  }
+  ia.pos = P<double>(c->pos);
+  ia.flags = P<uint32_t>(c->flags);
+  launch_ingest(ia, c->stream);
+  const int nblk = static_cast<int>((n + 255) / 256);
+  if ((r = ensure(c, c->blk, sizeof(int32_t) * nsp * nblk))) return r;
+  if ((r = ensure(c, c->blkoff, sizeof(int32_t) * nsp * nblk))) return r;
+  if ((r = ensure(c, c->sp_off, sizeof(int32_t) * (nsp + 1)))) return r;
+  launch_species_count(itype_dev, n, nsp, P<int32_t>(c->blk), nblk, c->stream);
+  launch_exclusive_scan(P<int32_t>(c->blk), P<int32_t>(c->blkoff), static_cast<int64_t>(nsp) * nblk, c->stream);
+  launch_species_offsets(P<int32_t>(c->blkoff), P<int32_t>(c->blk), nblk, nsp, n, P<int32_t>(c->sp_off),
+                         c->stream);
+  if (do_sq) {
+    // Species-contiguous turns u = x / Lq for the contraction (Lq: the box the
+    // q-grid was built on, sq.cuf:189).
+    if ((r = ensure(c, c->u, sizeof(double) * 3 * n))) return r;
+    const double lq[3] = {c->sq.box[0], c->sq.box[1], c->sq.box[2]};
+    launch_species_scatter(itype_dev, n, nsp, P<int32_t>(c->blkoff), nblk, P<double>(c->pos), lq, ndim,
+                           P<double>(c->u), c->stream);
+  }
+  HIPC(c, hipGetLastError());
+  HIPC(c, hipEventRecord(es.ev[1], c->stream));
+
+  // RDF (RDFcomp, rdf.cuf:184-353).
+  if (do_rdf) {
+    RdfPlan& R = c->rdf;
+    FrameGeom g = frame_geometry(ndim, box, R.periodic, &R.rc);
+    PairArgs pa{};
+    pa.ndim = ndim;
+    pa.nsp = nsp;
+    pa.nit = R.nit;
+    pa.lsmax = R.lsmax;
+    for (int k = 0; k < 3; ++k) {
+      pa.lmi[k] = g.lmi[k];
+      pa.half[k] = g.half[k];
+    }
+    pa.side2 = g.side2;
+    pa.inv_dr = static_cast<float>(1.0 / R.deltar);
+    pa.thr = P<double>(c->thr);
+    const int nbins = R.nit * R.lsmax;
+    // Cell grid: side >= cutoff on every axis.
+    const double rc = std::sqrt(g.side2);
+    CellGrid cg{};
+    bool cells_ok = true;
+    int64_t ncell = 1;
+    for (int k = 0; k < 3; ++k) {
+      if (k < ndim) {
+        int nc = static_cast<int>(std::floor(box[k] / (rc * (1.0 + 1e-6))));
+        nc = std::max(1, std::min(nc, ndim == 3 ? 160 : 2048));
+        cg.nc[k] = nc;
+        cg.inv_h[k] = nc / box[k];
+        cg.periodic[k] = periodic[k];
+        if (periodic[k] && nc < 3) cells_ok = false;
+      } else {
+        cg.nc[k] = 1;
+        cg.inv_h[k] = 0.0;
+        cg.periodic[k] = 0;
+      }
+      ncell *= cg.nc[k];
+    }
+    int nblocks;
+    if (cells_ok) {
+      if ((r = ensure(c, c->cellid, sizeof(int32_t) * n))) return r;
+      if ((r = ensure(c, c->cellcount, sizeof(int32_t) * (ncell + 1)))) return r;
+      if ((r = ensure(c, c->cellstart, sizeof(int32_t) * (ncell + 1)))) return r;
+      if ((r = ensure(c, c->cursor, sizeof(int32_t) * ncell))) return r;
+      if ((r = ensure(c, c->spos, sizeof(double) * 3 * n))) return r;
+      if ((r = ensure(c, c->stype, sizeof(int32_t) * n))) return r;
+      HIPC(c, hipMemsetAsync(c->cellcount.p, 0, sizeof(int32_t) * (ncell + 1), c->stream));
+      HIPC(c, hipMemsetAsync(c->cursor.p, 0, sizeof(int32_t) * ncell, c->stream));
+      launch_cell_count(P<double>(c->pos), n, ndim, cg, P<int32_t>(c->cellid), P<int32_t>(c->cellcount),
+                        c->stream);
+      launch_exclusive_scan(P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), ncell + 1, c->stream);
+      launch_cell_scatter(P<double>(c->pos), itype_dev, P<int32_t>(c->cellid), n, P<int32_t>(c->cellstart),
+                          P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype), c->stream);
+      const double mean = static_cast<double>(n) / static_cast<double>(ncell);
+      int W = static_cast<int>(std::lround(mean / 64.0));
+      W = std::max(1, std::min(W, 8));
+      const int64_t ntasks = ncell * W;
+      nblocks = static_cast<int>(std::min<int64_t>((ntasks + 3) / 4, 1024));
+      if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
+      launch_rdf_cells(pa, cg, P<int32_t>(c->cellstart), P<double>(c->spos), P<int32_t>(c->stype), n, W,
+                       nblocks, P<uint32_t>(c->partial), c->stream);
+    } else {
+      const int64_t nb = (n + 63) / 64;
+      nblocks = static_cast<int>(std::min<int64_t>((nb * nb + 3) / 4, 1024));
+      if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
+      launch_rdf_allpairs(pa, P<double>(c->pos), itype_dev, n, nblocks, P<uint32_t>(c->partial), c->stream);
+    }
+    launch_rdf_merge(P<uint32_t>(c->partial), nblocks, nbins, P<int64_t>(c->counts), c->stream);
+    HIPC(c, hipGetLastError());
+    c->rdf_frames += 1;
+    c->last_volume = g.volume;
+  } else {
+    double v = 1.0;
+    for (int k = 0; k < ndim; ++k) v *= box[k];
+    c->last_volume = v;
+  }
+  HIPC(c, hipEventRecord(es.ev[2], c->stream));
+
+  // S(Q) (SQcalc, sq.cuf:58-126).
+  if (do_sq) {
+    SqPlan& S = c->sq;
+    const int ntiles = static_cast<int>(S.tiles.size());
+    if (ntiles > 0) {
+      int ks = static_cast<int>((2048 + ntiles - 1) / ntiles);
+      ks = std::max(1, std::min<int>(ks, static_cast<int>(std::max<int64_t>(1, n / 8192))));
+      c->ksplit = ks;
+      const size_t rho_bytes = sizeof(double) * 2 * static_cast<size_t>(ks) * nsp * ntiles * TILE_VEC;
+      if ((r = ensure(c, c->rho, rho_bytes))) return r;
+      ContractArgs ca{};
+      ca.u = P<double>(c->u);
+      ca.sp_off = P<int32_t>(c->sp_off);
+      ca.tiles = P<int32_t>(c->tiles);
+      ca.ntiles = ntiles;
+      ca.ksplit = ks;
+      ca.nsp = nsp;
+      ca.ndim = ndim;
+      ca.n = n;
+      ca.rho = P<double>(c->rho);
+      launch_sq_contract_f64(ca, c->stream);
+    }
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipEventRecord(es.ev[3], c->stream));
+    if (ntiles > 0)
+      launch_sq_shells(P<double>(c->rho), P<int16_t>(c->tile_shell), ntiles, c->ksplit, nsp, S.p.bsc, S.nqmax,
+                       P<double>(c->sqf), P<double>(c->sqfp), c->stream);
+    launch_sq_list(P<double>(c->u), P<int32_t>(c->sp_off), n, nsp, ndim, P<int32_t>(c->axis_kv),
+                   P<int32_t>(c->axis_shell), static_cast<int>(S.axis_shell.size()), S.p.bsc, S.nqmax,
+                   P<double>(c->sqf), P<double>(c->sqfp), c->stream);
+    HIPC(c, hipGetLastError());
+    c->sq_frames += 1;
+  } else {
+    HIPC(c, hipEventRecord(es.ev[3], c->stream));
+  }
+  HIPC(c, hipEventRecord(es.ev[4], c->stream));
+  es.pending = true;
+  es.has_rdf = do_rdf;
+  es.has_sq = do_sq;
+  c->last_n = n;
+  c->last_nsp = nsp;
+  return TRJ_OK;
+}
+
+int read_ntype(trj_ctx* c, std::vector<int64_t>& ntype) {
+  const int nsp = c->last_nsp;
+  std::vector<int32_t> off(nsp + 1, 0);
+  HIPC(c, hipMemcpyAsync(off.data(), c->sp_off.p, sizeof(int32_t) * (nsp + 1), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  ntype.assign(nsp, 0);
+  for (int s = 0; s < nsp; ++s) ntype[s] = off[s + 1] - off[s];
+  return TRJ_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int trj_abi_version(void) { return TRJ_ABI_VERSION; }
+
+int trj_create(trj_ctx** out, int device) {
+  if (!out) return TRJ_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return TRJ_EDEVICE;
+  if (device < 0 || device >= ndev) return TRJ_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return TRJ_EDEVICE;
+  trj_ctx* c = new trj_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return TRJ_EDEVICE;
+  }
+  for (auto& s : c->evs)
+    for (auto& e : s.ev)
+      if (hipEventCreate(&e) != hipSuccess) {
+        trj_destroy(c);
+        return TRJ_EDEVICE;
+      }
+  for (auto& e : c->pinned_done)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      trj_destroy(c);
+      return TRJ_EDEVICE;
+    }
+  if (ensure(c, c->flags, sizeof(uint32_t)) != TRJ_OK ||
+      hipMemsetAsync(c->flags.p, 0, sizeof(uint32_t), c->stream) != hipSuccess) {
+    trj_destroy(c);
+    return TRJ_ENOMEM;
+  }
+  *out = c;
+  return TRJ_OK;
+}
+
+void trj_destroy(trj_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  DevBuf* bufs[] = {&c->raw,  &c->itype,     &c->pos,       &c->spos,   &c->stype,  &c->cellid,
+                    &c->cellcount, &c->cellstart, &c->cursor, &c->partial, &c->counts, &c->thr,
+                    &c->blk,  &c->blkoff,    &c->sp_off,    &c->u,      &c->rho,    &c->tiles,
+                    &c->tile_shell, &c->sqf, &c->sqfp,      &c->axis_kv, &c->axis_shell, &c->flags};
+  for (DevBuf* b : bufs)
+    if (b->p) hipFree(b->p);
+  for (int k = 0; k < 2; ++k) {
+    if (c->pinned[k]) hipHostFree(c->pinned[k]);
+    if (c->pinned_done[k]) hipEventDestroy(c->pinned_done[k]);
+  }
+  for (auto& s : c->evs)
+    for (auto& e : s.ev)
+      if (e) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* trj_last_error(const trj_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int trj_rdf_init(trj_ctx* c, const trj_rdf_params* p, const double box[3], int32_t* lsmax_out,
+                 int32_t* nit_out) {
+  if (!c || !p) return TRJ_EINVAL;
+  if (p->ndim != 2 && p->ndim != 3) return fail(c, TRJ_EINVAL, "rdf: ndim must be 2 or 3");
+  if (p->nsp < 1 || p->nsp > kNspMax) return fail(c, TRJ_EINVAL, "rdf: nsp must be in 1..10");
+  if (!(p->deltar > 0.0)) return fail(c, TRJ_EINVAL, "rdf: deltar must be positive");
+  if (int r = validate_box(c, p->ndim, box)) return r;
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  RdfPlan R;
+  R.ndim = p->ndim;
+  R.nsp = p->nsp;
+  R.nit = p->nsp * (p->nsp + 1) / 2;
+  R.deltar = p->deltar;
+  for (int k = 0; k < 3; ++k) R.periodic[k] = (k < p->ndim) ? (p->periodic[k] != 0) : 1;
+  R.lsmax = rdf_lsmax(p->ndim, box, p->rcrdf, p->deltar, &R.rc);
+  if (R.lsmax < 1) return fail(c, TRJ_EINVAL, "rdf: lsmax < 1");
+  if (pair_lds_bytes(R.nit, R.lsmax) > kPairLdsLimit)
+    return fail(c, TRJ_ELIMIT,
+                "rdf: g(r) grid needs " + std::to_string(pair_lds_bytes(R.nit, R.lsmax)) +
+                    " bytes of LDS, above 65536; increase deltar or reduce rcrdf (rdf.cuf:61-64)");
+  std::vector<double> thr = rdf_bin_thresholds(R.deltar, R.lsmax);
+  thr.push_back(INFINITY);  // thr[lsmax+2]
+  thr[0] = 0.0;
+  R.thr = thr;
+  if ((r = ensure(c, c->thr, sizeof(double) * thr.size()))) return r;
+  if ((r = ensure(c, c->counts, sizeof(int64_t) * R.nit * R.lsmax))) return r;
+  HIPC(c, hipMemcpyAsync(c->thr.p, thr.data(), sizeof(double) * thr.size(), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemsetAsync(c->counts.p, 0, sizeof(int64_t) * R.nit * R.lsmax, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  c->rdf = R;
+  c->rdf_ready = true;
+  c->rdf_frames = 0;
+  if (lsmax_out) *lsmax_out = R.lsmax;
+  if (nit_out) *nit_out = R.nit;
+  return TRJ_OK;
+}
+
+int trj_sq_init(trj_ctx* c, const trj_sq_params* p, const double box[3], trj_sq_info* info) {
+  if (!c || !p) return TRJ_EINVAL;
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  SqPlan S;
+  std::string err;
+  r = sq_plan(*p, box, &S, &err);
+  if (r) return fail(c, r, err);
+  const int ns = S.nqmax + 1;
+  if ((r = ensure(c, c->sqf, sizeof(double) * ns))) return r;
+  if ((r = ensure(c, c->sqfp, sizeof(double) * ns * S.p.nsp))) return r;
+  if ((r = ensure(c, c->tiles, sizeof(SqTile) * std::max<size_t>(1, S.tiles.size())))) return r;
+  if ((r = ensure(c, c->tile_shell, sizeof(int16_t) * std::max<size_t>(1, S.tile_shell.size())))) return r;
+  if ((r = ensure(c, c->axis_kv, sizeof(int32_t) * std::max<size_t>(1, S.axis_kv.size())))) return r;
+  if ((r = ensure(c, c->axis_shell, sizeof(int32_t) * std::max<size_t>(1, S.axis_shell.size())))) return r;
+  HIPC(c, hipMemsetAsync(c->sqf.p, 0, sizeof(double) * ns, c->stream));
+  HIPC(c, hipMemsetAsync(c->sqfp.p, 0, sizeof(double) * ns * S.p.nsp, c->stream));
+  if (!S.tiles.empty()) {
+    HIPC(c, hipMemcpyAsync(c->tiles.p, S.tiles.data(), sizeof(SqTile) * S.tiles.size(), hipMemcpyHostToDevice,
+                           c->stream));
+    HIPC(c, hipMemcpyAsync(c->tile_shell.p, S.tile_shell.data(), sizeof(int16_t) * S.tile_shell.size(),
+                           hipMemcpyHostToDevice, c->stream));
+  }
+  if (!S.axis_shell.empty()) {
+    HIPC(c, hipMemcpyAsync(c->axis_kv.p, S.axis_kv.data(), sizeof(int32_t) * S.axis_kv.size(),
+                           hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(c->axis_shell.p, S.axis_shell.data(), sizeof(int32_t) * S.axis_shell.size(),
+                           hipMemcpyHostToDevice, c->stream));
+  }
+  HIPC(c, hipStreamSynchronize(c->stream));
+  c->tile_index.clear();
+  for (size_t t = 0; t < S.tiles.size(); ++t)
+    c->tile_index[pack_tile(S.tiles[t].kx0, S.tiles[t].ky0, S.tiles[t].kz0)] = static_cast<int>(t);
+  c->sq = std::move(S);
+  c->sq_ready = true;
+  c->sq_frames = 0;
+  if (info) *info = c->sq.info;
+  return TRJ_OK;
+}
+
+int trj_frame_device(trj_ctx* c, const void* pos_dev, int32_t dtype, int32_t layout, const int32_t* itype_dev,
+                     int64_t n, const double box[3], uint32_t what) {
+  if (!c) return TRJ_EINVAL;
+  if (!pos_dev || !itype_dev) return fail(c, TRJ_EINVAL, "NULL input");
+  hipSetDevice(c->device);
+  return run_frame(c, pos_dev, dtype, layout, itype_dev, n, box, what);
+}
+
+int trj_frame(trj_ctx* c, const void* pos, int32_t dtype, int32_t layout, const int32_t* itype, int64_t n,
+              const double box[3], uint32_t what) {
+  if (!c) return TRJ_EINVAL;
+  if (!pos || !itype) return fail(c, TRJ_EINVAL, "NULL input");
+  if (n < 1) return fail(c, TRJ_EINVAL, "n must be positive");
+  if (dtype != TRJ_F32 && dtype != TRJ_F64) return fail(c, TRJ_EINVAL, "bad dtype");
+  hipSetDevice(c->device);
+  const int ndim = (what & TRJ_WANT_RDF) && c->rdf_ready ? c->rdf.ndim : (c->sq_ready ? c->sq.p.ndim : 3);
+  const size_t pbytes = static_cast<size_t>(n) * ndim * (dtype == TRJ_F32 ? 4 : 8);
+  const size_t tbytes = static_cast<size_t>(n) * sizeof(int32_t);
+  const size_t pbytes_al = (pbytes + 255) & ~size_t(255);
+  int r;
+  if ((r = ensure(c, c->raw, pbytes))) return r;
+  if ((r = ensure(c, c->itype, tbytes))) return r;
+  // Stage through pinned memory; wait until the previous copy out of this
+  // staging buffer has completed.
+  const int k = c->pinned_next;
+  c->pinned_next ^= 1;
+  HIPC(c, hipEventSynchronize(c->pinned_done[k]));
+  if (c->pinned_cap[k] < pbytes_al + tbytes) {
+    if (c->pinned[k]) hipHostFree(c->pinned[k]);
+    c->pinned[k] = nullptr;
+    c->pinned_cap[k] = 0;
+    size_t want = (pbytes_al + tbytes) + (pbytes_al + tbytes) / 8;
+    if (hipHostMalloc(&c->pinned[k], want, hipHostMallocDefault) != hipSuccess)
+      return fail(c, TRJ_ENOMEM, "pinned staging allocation failed");
+    c->pinned_cap[k] = want;
+  }
+  char* st = static_cast<char*>(c->pinned[k]);
+  std::memcpy(st, pos, pbytes);
+  std::memcpy(st + pbytes_al, itype, tbytes);
+  HIPC(c, hipMemcpyAsync(c->raw.p, st, pbytes, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(c->itype.p, st + pbytes_al, tbytes, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipEventRecord(c->pinned_done[k], c->stream));
+  return run_frame(c, c->raw.p, dtype, layout, P<int32_t>(c->itype), n, box, what);
+}
+
+int trj_sync(trj_ctx* c) {
+  if (!c) return TRJ_EINVAL;
+  hipSetDevice(c->device);
+  return sync_all(c);
+}
+
+int trj_rdf_counts(trj_ctx* c, int64_t* counts) {
+  if (!c || !counts) return TRJ_EINVAL;
+  if (!c->rdf_ready) return fail(c, TRJ_ESTATE, "RDF not initialised");
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  HIPC(c, hipMemcpy(counts, c->counts.p, sizeof(int64_t) * c->rdf.nit * c->rdf.lsmax, hipMemcpyDeviceToHost));
+  return TRJ_OK;
+}
+
+int trj_rdf_gmix(trj_ctx* c, int64_t nconf, double* out) {
+  if (!c || !out) return TRJ_EINVAL;
+  if (!c->rdf_ready) return fail(c, TRJ_ESTATE, "RDF not initialised");
+  if (c->rdf_frames == 0) return fail(c, TRJ_ESTATE, "no RDF frame processed");
+  const RdfPlan& R = c->rdf;
+  std::vector<int64_t> counts(static_cast<size_t>(R.nit) * R.lsmax);
+  int r = trj_rdf_counts(c, counts.data());
+  if (r) return r;
+  std::vector<int64_t> ntype;
+  if ((r = read_ntype(c, ntype))) return r;
+  if (nconf <= 0) nconf = c->rdf_frames;
+  // log.f90:484-518
+  const double pi = 3.14159265358979323846;
+  const int ncol = 1 + R.nit;
+  for (int i = 1; i <= R.lsmax - 2; ++i) {
+    const double ri = i * R.deltar, a = ri + R.deltar / 2, b = ri - R.deltar / 2;
+    const double dv = (R.ndim == 3) ? 4.0 * pi * (a * a * a - b * b * b) / 3.0 : pi * (a * a - b * b);
+    double* row = out + static_cast<size_t>(i - 1) * ncol;
+    row[0] = ri;
+    int col = 1;
+    for (int j = 1; j <= R.nsp; ++j)
+      for (int l = j; l <= R.nsp; ++l) {
+        const int ij = (j - 1) * (R.nsp - 1) - (j - 1) * (j - 2) / 2 + l;
+        const double h = static_cast<double>(counts[static_cast<size_t>(ij - 1) * R.lsmax + (i - 1)]);
+        const double fac = (j == l) ? 2.0 : 1.0;  // (j/l + 1), integer division
+        row[col++] = fac * c->last_volume * h /
+                     (dv * static_cast<double>(ntype[l - 1]) * static_cast<double>(ntype[j - 1]) *
+                      static_cast<double>(nconf));
+      }
+  }
+  return TRJ_OK;
+}
+
+int trj_rdf_first_minimum(trj_ctx* c, double* rmin) {
+  if (!c || !rmin) return TRJ_EINVAL;
+  if (!c->rdf_ready || c->rdf_frames == 0) return fail(c, TRJ_ESTATE, "no RDF frame processed");
+  const RdfPlan& R = c->rdf;
+  std::vector<int64_t> counts(static_cast<size_t>(R.nit) * R.lsmax);
+  int r = trj_rdf_counts(c, counts.data());
+  if (r) return r;
+  // rdf.cuf:313-342: running total g over all species pairs, natms^2 normalisation.
+  const double pi = 3.14159265358979323846;
+  const int nrdf = R.lsmax;
+  std::vector<double> g(nrdf + 1, 0.0);
+  const double nat = static_cast<double>(c->last_n);
+  for (int i = 1; i <= nrdf; ++i) {
+    const double ri = i * R.deltar, a = ri + R.deltar / 2, b = ri - R.deltar / 2;
+    const double dv = (R.ndim == 3) ? 4.0 * pi * (a * a * a - b * b * b) / 3.0 : pi * (a * a - b * b);
+    double s = 0.0;
+    for (int ij = 0; ij < R.nit; ++ij) s += static_cast<double>(counts[static_cast<size_t>(ij) * R.lsmax + i - 1]);
+    g[i] = 2.0 * c->last_volume * s / (dv * nat * nat);
+  }
+  // rdf.cuf:755-822
+  int imax = 1;
+  for (int i = 2; i <= nrdf - 1; ++i)
+    if (g[i] > 0.1 && g[i] > g[i - 1] && g[i] > g[i + 1]) {
+      imax = i;
+      break;
+    }
+  if (imax == 1) {
+    double gmax = g[1];
+    for (int i = 2; i <= nrdf; ++i)
+      if (g[i] > gmax) {
+        gmax = g[i];
+        imax = i;
+      }
+  }
+  int imin = imax + 1;
+  for (int i = imax + 1; i <= nrdf - 1; ++i)
+    if (g[i] < g[i - 1] && g[i] < g[i + 1]) {
+      imin = i;
+      break;
+    }
+  if (imin > 0 && imin < nrdf) {
+    const double x1 = R.deltar * (imin - 1), x2 = R.deltar * imin, x3 = R.deltar * (imin + 1);
+    const double y1 = g[imin - 1], y2 = g[imin], y3 = g[imin + 1];
+    const double denom = y1 - 2.0 * y2 + y3;
+    *rmin = (std::fabs(denom) > 1e-12) ? x2 + 0.5 * (y1 - y3) / denom * (x3 - x1) / 2.0 : x2;
+  } else {
+    *rmin = R.deltar * imin;
+  }
+  return TRJ_OK;
+}
+
+int trj_sq_shells(trj_ctx* c, double* sqf, double* sqfp, int64_t* nq) {
+  if (!c) return TRJ_EINVAL;
+  if (!c->sq_ready) return fail(c, TRJ_ESTATE, "S(Q) not initialised");
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  const int ns = c->sq.nqmax + 1;
+  if (sqf) HIPC(c, hipMemcpy(sqf, c->sqf.p, sizeof(double) * ns, hipMemcpyDeviceToHost));
+  if (sqfp) HIPC(c, hipMemcpy(sqfp, c->sqfp.p, sizeof(double) * ns * c->sq.p.nsp, hipMemcpyDeviceToHost));
+  if (nq)
+    for (int s = 0; s < ns; ++s) nq[s] = c->sq.nq[s];
+  return TRJ_OK;
+}
+
+int trj_sq_columns(trj_ctx* c, int64_t nconf, double* cols, int32_t* ncols_out, double* mix) {
+  if (!c || !cols) return TRJ_EINVAL;
+  if (!c->sq_ready || c->sq_frames == 0) return fail(c, TRJ_ESTATE, "no S(Q) frame processed");
+  const SqPlan& S = c->sq;
+  const int nsp = S.p.nsp, ns = S.nqmax + 1;
+  std::vector<double> sqf(ns), sqfp(static_cast<size_t>(ns) * nsp);
+  int r = trj_sq_shells(c, sqf.data(), sqfp.data(), nullptr);
+  if (r) return r;
+  std::vector<int64_t> ntype;
+  if ((r = read_ntype(c, ntype))) return r;
+  if (nconf <= 0) nconf = c->sq_frames;
+  // log.f90:317-401
+  const double N = static_cast<double>(c->last_n);
+  double baver = 0.0;
+  bool bsc_one = true;
+  for (int s = 0; s < nsp; ++s) {
+    baver += static_cast<double>(ntype[s]) * S.p.bsc[s];
+    if (std::fabs(S.p.bsc[s] - 1.0) > 1.0e-4) bsc_one = false;
+  }
+  baver /= N;
+  const bool binary = (nsp == 2 && bsc_one);
+  const int ncols = binary ? 7 : 3;
+  if (ncols_out) *ncols_out = ncols;
+  const double x1 = binary ? ntype[0] / N : 0.0, x2 = binary ? ntype[1] / N : 0.0;
+  for (int i = 1; i <= S.nqmax; ++i) {
+    double* row = cols + static_cast<size_t>(i - 1) * ncols;
+    const double nqi = static_cast<double>(S.nq[i]);
+    const double cn = static_cast<double>(nconf) * nqi;
+    const double snn = (sqf[i] / (baver * baver)) / (N * cn);
+    row[0] = static_cast<double>(static_cast<float>(i) * S.dq);
+    row[1] = snn;
+    if (binary) {
+      const double s11 = x1 * sqfp[i] / (ntype[0] * cn);
+      const double s22 = x2 * sqfp[ns + i] / (ntype[1] * cn);
+      const double s12 = 0.5 * (snn - s11 - s22);
+      row[2] = x2 * x2 * s11 + x1 * x1 * s22 - 2.0 * x1 * x2 * s12;
+      row[3] = s11;
+      row[4] = s22;
+      row[5] = s12;
+      row[6] = nqi;
+    } else {
+      row[2] = nqi;
+    }
+    if (mix) {
+      double* m = mix + static_cast<size_t>(i - 1) * (1 + nsp);
+      m[0] = row[0];
+      for (int s = 0; s < nsp; ++s) m[1 + s] = sqfp[static_cast<size_t>(s) * ns + i] / (ntype[s] * cn);
+    }
+  }
+  return TRJ_OK;
+}
+
+int trj_sq_rho(trj_ctx* c, int64_t nvec, const int32_t* kv, double* rho) {
+  if (!c || (nvec > 0 && (!kv || !rho))) return TRJ_EINVAL;
+  if (!c->sq_ready || c->sq_frames == 0) return fail(c, TRJ_ESTATE, "no S(Q) frame processed");
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  const SqPlan& S = c->sq;
+  const int nsp = S.p.nsp, ntiles = static_cast<int>(S.tiles.size());
+  for (int64_t v = 0; v < nvec; ++v) {
+    const int x = kv[3 * v], y = kv[3 * v + 1], z = kv[3 * v + 2];
+    if (x < S.kmin[0] || y < S.kmin[1] || z < S.kmin[2])
+      return fail(c, TRJ_EINVAL, "vector outside the dense q-box");
+    const int x0 = S.kmin[0] + ((x - S.kmin[0]) / TKX) * TKX;
+    const int y0 = S.kmin[1] + ((y - S.kmin[1]) / TKY) * TKY;
+    const int z0 = S.kmin[2] + ((z - S.kmin[2]) / TKZ) * TKZ;
+    auto it = c->tile_index.find(pack_tile(x0, y0, z0));
+    if (it == c->tile_index.end()) return fail(c, TRJ_EINVAL, "vector not in the dense q-set");
+    const int tile = it->second;
+    const int vi = ((x - x0) * TKY + (y - y0)) * TKZ + (z - z0);
+    if (S.tile_shell[static_cast<size_t>(tile) * TILE_VEC + vi] < 0)
+      return fail(c, TRJ_EINVAL, "vector not in the dense q-set");
+    for (int s = 0; s < nsp; ++s) {
+      double acc[2] = {0.0, 0.0};
+      for (int sp = 0; sp < c->ksplit; ++sp) {
+        double val[2];
+        const size_t off =
+            (((static_cast<size_t>(sp) * nsp + s) * ntiles + tile) * TILE_VEC + vi) * 2 * sizeof(double);
+        HIPC(c, hipMemcpy(val, static_cast<char*>(c->rho.p) + off, sizeof(val), hipMemcpyDeviceToHost));
+        acc[0] += val[0];
+        acc[1] += val[1];
+      }
+      rho[(v * nsp + s) * 2] = acc[0];
+      rho[(v * nsp + s) * 2 + 1] = acc[1];
+    }
+  }
+  return TRJ_OK;
+}
+
+int trj_reset(trj_ctx* c) {
+  if (!c) return TRJ_EINVAL;
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  if (c->rdf_ready)
+    HIPC(c, hipMemsetAsync(c->counts.p, 0, sizeof(int64_t) * c->rdf.nit * c->rdf.lsmax, c->stream));
+  if (c->sq_ready) {
+    const int ns = c->sq.nqmax + 1;
+    HIPC(c, hipMemsetAsync(c->sqf.p, 0, sizeof(double) * ns, c->stream));
+    HIPC(c, hipMemsetAsync(c->sqfp.p, 0, sizeof(double) * ns * c->sq.p.nsp, c->stream));
+  }
+  HIPC(c, hipStreamSynchronize(c->stream));
+  c->rdf_frames = 0;
+  c->sq_frames = 0;
+  return TRJ_OK;
+}
+
+int trj_last_timings(trj_ctx* c, double* ms4) {
+  if (!c || !ms4) return TRJ_EINVAL;
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  for (int k = 0; k < 4; ++k) ms4[k] = c->last_ms[k];
+  return TRJ_OK;
+}
+
+int trj_timing_totals(trj_ctx* c, double* ms4, int64_t* nframes) {
+  if (!c || !ms4) return TRJ_EINVAL;
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  for (int k = 0; k < 4; ++k) ms4[k] = c->tot_ms[k];
+  if (nframes) *nframes = c->tot_frames;
+  return TRJ_OK;
+}
+
+int trj_timing_reset(trj_ctx* c) {
+  if (!c) return TRJ_EINVAL;
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  for (int k = 0; k < 4; ++k) c->tot_ms[k] = 0.0;
+  c->tot_frames = 0;
+  return TRJ_OK;
+}
+
+int64_t trj_device_bytes(const trj_ctx* c) {
+  if (!c) return 0;
+  const DevBuf* bufs[] = {&c->raw,  &c->itype,     &c->pos,       &c->spos,   &c->stype,  &c->cellid,
+                          &c->cellcount, &c->cellstart, &c->cursor, &c->partial, &c->counts, &c->thr,
+                          &c->blk,  &c->blkoff,    &c->sp_off,    &c->u,      &c->rho,    &c->tiles,
+                          &c->tile_shell, &c->sqf, &c->sqfp,      &c->axis_kv, &c->axis_shell, &c->flags};
+  int64_t s = 0;
+  for (const DevBuf* b : bufs) s += static_cast<int64_t>(b->cap);
+  return s;
+}
+
+}  // extern "C"

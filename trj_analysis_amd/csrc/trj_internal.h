@@ -1,0 +1,75 @@
+// trj_internal.h -- shared declarations of the trj_hip library (host planning
+// and kernel launchers).  Not part of the public ABI.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/trj_hip.h"
+
+namespace trj {
+
+constexpr int kNspMax = 10;  // nspmax, src/core/common.cuf:41
+
+// ---------------------------------------------------------------- RDF plan
+struct RdfPlan {
+  int ndim = 3, nsp = 1, nit = 1, lsmax = 0;
+  double deltar = 0.0;
+  double rc = 0.0;       // sticky capped cutoff (netcdf.f90:610-612)
+  int periodic[3] = {1, 1, 1};
+  // Exact bin edges in r^2: thr[m] = smallest double r2 with
+  // nint(sqrt(r2)/deltar) >= m, m = 1..lsmax+1 (thr[0] unused).
+  std::vector<double> thr;
+};
+
+// rdf.cuf:55-58 after the cap of netcdf.f90:610-612.
+int rdf_lsmax(int ndim, const double box[3], double rcrdf, double deltar, double* rc_eff);
+// Thresholds so that bin(r2) = max{m : thr[m] <= r2} equals nint(sqrt(r2)/deltar).
+std::vector<double> rdf_bin_thresholds(double deltar, int lsmax);
+// Smallest d >= 0 with nint(d/L) >= 1 in FP64 (minimum-image switch point).
+double minimage_threshold(double L);
+
+// Per-frame geometry (netcdf.f90:602-622).
+struct FrameGeom {
+  double box[3];
+  double lmi[3];   // minimum-image lengths (sidelv)
+  double half[3];  // minimage_threshold(lmi)
+  double side2;
+  double volume;
+};
+FrameGeom frame_geometry(int ndim, const double box[3], const int periodic[3], double* rc_sticky);
+
+// ---------------------------------------------------------------- S(Q) plan
+// Tile of the dense q-box computed by one workgroup of the contraction:
+// kx in [kx0, kx0+TKX), ky in [ky0, ky0+TKY), kz in [kz0, kz0+TKZ).
+constexpr int TKX = 8, TKY = 8, TKZ = 32;
+constexpr int TILE_VEC = TKX * TKY * TKZ;  // 2048
+
+struct SqTile {
+  int32_t kx0, ky0, kz0, pad;
+};
+
+struct SqPlan {
+  trj_sq_params p{};
+  trj_sq_info info{};
+  float fk[3], dq, qminp, qmin2;
+  int l[3], kext[3];
+  int nqmax = 0, nqmin = 0;
+  double box[3];                     // box the q-grid was built on
+  std::vector<int64_t> nq;           // normalisation counts (0..nqmax)
+  std::vector<int64_t> nq_summed;    // vectors actually summed per shell
+  std::vector<SqTile> tiles;         // active tiles of the dense box
+  std::vector<int16_t> tile_shell;   // [tile][TILE_VEC], -1 = not summed
+  std::vector<int32_t> axis_kv;      // axis vectors (x,y,z) triples
+  std::vector<int32_t> axis_shell;   // their shells
+  int kmin[3], kmax[3];              // inclusive index range of the dense box
+};
+
+// sq.cuf:156-200 (REF) or the half-space grid; fills everything above.
+int sq_plan(const trj_sq_params& p, const double box[3], SqPlan* out, std::string* err);
+
+// ---------------------------------------------------------------- kernels
+struct DevFrame;  // defined in trj_ctx.hip
+
+}  // namespace trj

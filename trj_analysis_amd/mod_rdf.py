@@ -1,0 +1,127 @@
+"""The reference's mod_rdf interface over the trj_hip C ABI.
+
+Same procedure names and argument meaning as src/modules/rdf.cuf and the RDF
+writer of src/core/log.f90, with the module globals (mod_common, mod_input)
+replaced by explicit arguments and one module-level context:
+
+  RDF_init(nsp, ...)                 rdf.cuf:51-122
+  RDFcomp(Nsites, Iconf, r, itype, sidel)   rdf.cuf:184-353
+  rdf_clear()                        rdf.cuf:124-182
+  printrdf(rcl, lsmax, ...)          log.f90:414-596 (gmixsim.dat / gmixsim<k>.dat)
+
+Errors raise trjhip.TrjError (the reference prints and stops, rdf.cuf:241-242).
+Histograms are exact int64 counts (the reference accumulates them in FP32,
+rdf.cuf:306, common.cuf:90).
+"""
+from __future__ import annotations
+
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import trjhip as T
+
+_ctx: Optional[T.Context] = None
+_state: dict = {}
+
+
+def context() -> T.Context:
+    global _ctx
+    if _ctx is None:
+        _ctx = T.Context(int(os.environ.get("TRJ_DEVICE", "0")))
+    return _ctx
+
+
+def RDF_init(nsp: int, *, ndim: int, deltar: float, rcrdf: float, sidel, periodic=(True, True, True),
+             ctx: Optional[T.Context] = None) -> int:
+    """RDF_init (rdf.cuf:51-122): returns lsmax; sidel is the first frame's box."""
+    c = ctx or context()
+    st = c.rdf_init(T.rdf_params(ndim, nsp, deltar, rcrdf, periodic), sidel)
+    _state.update(ctx=c, lsmax=st.lsmax, nit=st.nit, nsp=nsp, ndim=ndim, deltar=deltar,
+                  rmingr=None, nframes=0)
+    return st.lsmax
+
+
+def RDFcomp(Nsites: int, Iconf: int, r: np.ndarray, itype: np.ndarray, sidel, units: str = "lj"):
+    """RDFcomp (rdf.cuf:184-353) for one frame: r is (Nsites, ndim) (or the
+    reference's r(ndim,N) transposed), itype 1..nsp.  On the first frame the
+    suggested rcl of rdf.cuf:344-352 is computed and kept in rdf_rmingr()."""
+    c = _state["ctx"]
+    r = np.asarray(r)
+    if r.shape[0] != Nsites:
+        r = r.T
+    c.frame(np.ascontiguousarray(r), itype, sidel, what=T.TRJ_WANT_RDF)
+    _state["nframes"] += 1
+    if Iconf == 1:
+        _state["rmingr"] = c.rdf_first_minimum()
+
+
+def rdf_rmingr() -> Optional[float]:
+    return _state.get("rmingr")
+
+
+def histomix() -> np.ndarray:
+    """The accumulated histogram, histomix(lsmax, nsp, nsp) as (nsp, nsp, lsmax), symmetric."""
+    c = _state["ctx"]
+    counts = c.rdf_counts()
+    nsp, ls = _state["nsp"], _state["lsmax"]
+    h = np.zeros((nsp, nsp, ls), dtype=np.int64)
+    k = 0
+    for j in range(nsp):
+        for l in range(j, nsp):
+            h[j, l] = counts[k]
+            h[l, j] = counts[k]
+            k += 1
+    return h
+
+
+def rdf_clear():
+    """rdf_clear (rdf.cuf:124-182): drop the accumulators."""
+    c = _state.get("ctx")
+    if c is not None:
+        c.reset()
+    _state.clear()
+
+
+def _fmt_records(values, width, prec, per_record=16):
+    """Fortran '(16fW.D)' with format reversion: 16 values per line."""
+    out = []
+    for k in range(0, len(values), per_record):
+        out.append("".join(f"{v:{width}.{prec}f}" for v in values[k:k + per_record]))
+    return "\n".join(out)
+
+
+def printrdf(rcl: float = 0.0, lsmax: Optional[int] = None, nconf: int = 0, outdir: str = ".") -> list:
+    """printrdf (log.f90:414-596): writes gmixsim.dat (nsp <= 6) or gmixsim<k>.dat
+    and returns the written paths.  nconf <= 0 uses the number of frames processed."""
+    c = _state["ctx"]
+    nsp, nit = _state["nsp"], _state["nit"]
+    g = c.rdf_gmix(nconf)
+    paths = []
+    if nsp <= 6:
+        pairs = [(j, k) for j in range(1, nsp + 1) for k in range(j, nsp + 1)]
+        head = "#       r        " + " " * 9 + "".join(f"g_{j}{k}(r)" + " " * 9 for j, k in pairs)
+        path = os.path.join(outdir, "gmixsim.dat")
+        with open(path, "w") as f:
+            f.write(head + "\n")
+            for row in g:
+                f.write(_fmt_records(list(row), 16, 5) + "\n")
+        paths.append(path)
+    else:
+        full = np.zeros((len(g), nsp, nsp))
+        col = 1
+        for j in range(nsp):
+            for k in range(j, nsp):
+                full[:, j, k] = g[:, col]
+                full[:, k, j] = g[:, col]
+                col += 1
+        for k in range(1, nsp + 1):
+            head = "#       r        " + " " * 9 + "".join(f"g_{k}{j}(r)" + " " * 9 for j in range(1, nsp + 1))
+            path = os.path.join(outdir, f"gmixsim{k}.dat")
+            with open(path, "w") as f:
+                f.write(head + "\n")
+                for i, row in enumerate(g):
+                    f.write(_fmt_records([row[0]] + list(full[i, k - 1]), 16, 5) + "\n")
+            paths.append(path)
+    return paths

@@ -1,0 +1,83 @@
+"""The reference's mod_sq interface over the trj_hip C ABI.
+
+Same procedure names and argument meaning as src/modules/sq.cuf and the S(Q)
+writer of src/core/log.f90:
+
+  sq_init(Nsites, nsp, ...)          sq.cuf:150-369
+  SQcalc(r, itype, sidel)            sq.cuf:58-126 (one frame; accumulates)
+  sq_transfer_gpu_cpu()              sq.cuf:128-148 (returns sqf, sqfp, nq)
+  printSQ(Nsites, ...)               log.f90:317-412 (sq.dat, sqmix.dat)
+  sq_clear()                         sq.cuf:774
+
+Values of the reference's FP32 bookkeeping (dq, nqmax, nqmin, lmax, qmin after
+the lattice adjust) are returned by sq_init as the reference computes them.
+"""
+from __future__ import annotations
+
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import trjhip as T
+from .mod_rdf import context
+
+_state: dict = {}
+
+
+def sq_init(Nsites: int, nsp: int, *, ndim: int, qmax: float, qmin: Optional[float], bsc, sidel,
+            qset: int = T.TRJ_QSET_REF, grid=(0, 0, 0), precision: str = "fp64", flags: int = 0,
+            ctx: Optional[T.Context] = None) -> T.SqInfo:
+    """sq_init (sq.cuf:150-369) on the first frame's box; returns the q-grid info
+    (dq, nqmax, nqmin, lmax, qmin_eff)."""
+    c = ctx or context()
+    info = c.sq_init(T.sq_params(ndim, nsp, qmax, qmin, bsc, qset, grid, precision, flags), sidel)
+    _state.update(ctx=c, info=info, nsp=nsp, Nsites=Nsites, nframes=0)
+    return info
+
+
+def SQcalc(r: np.ndarray, itype: np.ndarray, sidel):
+    """SQcalc (sq.cuf:58-126) for one frame: r (Nsites, ndim), itype 1..nsp."""
+    c = _state["ctx"]
+    r = np.asarray(r)
+    if r.shape[0] != len(itype):
+        r = r.T
+    c.frame(np.ascontiguousarray(r), itype, sidel, what=T.TRJ_WANT_SQ)
+    _state["nframes"] += 1
+
+
+def sq_transfer_gpu_cpu():
+    """sq_transfer_gpu_cpu (sq.cuf:128-148): (sqf[0:nqmax], sqfp[nsp][0:nqmax], nq)."""
+    return _state["ctx"].sq_shells()
+
+
+def sq_clear():
+    """sq_clear (sq.cuf:774): drop the accumulators."""
+    c = _state.get("ctx")
+    if c is not None:
+        c.reset()
+    _state.clear()
+
+
+def printSQ(Nsites: int, nconf: int = 0, outdir: str = ".") -> list:
+    """printSQ (log.f90:317-412): writes sq.dat and sqmix.dat; returns their paths."""
+    c = _state["ctx"]
+    nsp = _state["nsp"]
+    cols, mix = c.sq_columns(nconf)
+    p_sq = os.path.join(outdir, "sq.dat")
+    p_mix = os.path.join(outdir, "sqmix.dat")
+    with open(p_sq, "w") as f:
+        if cols.shape[1] == 7:
+            f.write("#           Q        S_NN(Q)          S_cc(Q)       S_11(Q)        S_22(Q)"
+                    "           S_12(Q)         n(Q)\n")
+            for row in cols:
+                f.write("".join(f"{v:15.7f}" for v in row[:6]) + f"{int(row[6]):12d}\n")
+        else:
+            f.write("#           Q       S_NN(Q)          n(Q)\n")
+            for row in cols:
+                f.write("".join(f"{v:15.7f}" for v in row[:2]) + f"{int(row[2]):12d}\n")
+    with open(p_mix, "w") as f:
+        f.write("#       Q" + " " * 14 + "".join(f"S_{j}{j}(Q)" + " " * 9 for j in range(1, nsp + 1)) + "\n")
+        for row in mix:
+            f.write("".join(f"{v:16.7f}" for v in row) + "\n")
+    return [p_sq, p_mix]

@@ -1,0 +1,319 @@
+"""ctypes binding of libtrjhip.so (the C ABI declared in include/trj_hip.h).
+
+The native library is the only implementation: importing this module never
+falls back to a CPU path, and any call that needs the GPU raises TrjError
+when the library is missing or no device is available.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtrjhip.so")
+
+TRJ_OK = 0
+TRJ_EINVAL, TRJ_EDEVICE, TRJ_ENOMEM, TRJ_ESTATE, TRJ_ERANGE, TRJ_ELIMIT = -1, -2, -3, -4, -5, -6
+TRJ_F32, TRJ_F64 = 0, 1
+TRJ_AOS, TRJ_SOA = 0, 1
+TRJ_WANT_RDF, TRJ_WANT_SQ = 1, 2
+TRJ_QSET_REF, TRJ_QSET_HALFSPACE = 0, 1
+TRJ_PREC_FP64, TRJ_PREC_FP32, TRJ_PREC_FP16X3, TRJ_PREC_BF16X3 = 0, 1, 2, 3
+TRJ_SQ_NQ_CONSISTENT = 1
+
+PRECISIONS = {"fp64": TRJ_PREC_FP64, "fp32": TRJ_PREC_FP32, "fp16x3": TRJ_PREC_FP16X3,
+              "bf16x3": TRJ_PREC_BF16X3}
+
+# Every symbol the header declares (checked by the CPU test suite).
+EXPORTS = [
+    "trj_abi_version", "trj_create", "trj_destroy", "trj_last_error", "trj_rdf_init",
+    "trj_sq_init", "trj_sq_plan", "trj_rdf_plan", "trj_frame", "trj_frame_device", "trj_sync",
+    "trj_rdf_counts", "trj_rdf_gmix", "trj_rdf_first_minimum", "trj_sq_shells", "trj_sq_columns",
+    "trj_sq_rho", "trj_reset", "trj_last_timings", "trj_timing_totals", "trj_timing_reset",
+    "trj_device_bytes",
+]
+
+
+class TrjError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"trj_hip error {code}: {msg}")
+        self.code = code
+
+
+class RdfParams(C.Structure):
+    _fields_ = [("ndim", C.c_int32), ("nsp", C.c_int32), ("deltar", C.c_double),
+                ("rcrdf", C.c_double), ("periodic", C.c_int32 * 3)]
+
+
+class SqParams(C.Structure):
+    _fields_ = [("ndim", C.c_int32), ("nsp", C.c_int32), ("qmax", C.c_double),
+                ("qmin", C.c_double), ("bsc", C.c_double * 10), ("qset", C.c_int32),
+                ("grid", C.c_int32 * 3), ("precision", C.c_int32), ("flags", C.c_uint32)]
+
+
+class SqInfo(C.Structure):
+    _fields_ = [("dq", C.c_double), ("fk", C.c_double * 3), ("qmin_eff", C.c_double),
+                ("nqmax", C.c_int32), ("nqmin", C.c_int32), ("lmax", C.c_int32 * 3),
+                ("nvec_grid", C.c_int64), ("nvec_axis", C.c_int64), ("box_ext", C.c_int64 * 3)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libtrjhip.so (built by trj_analysis_amd.build); raise if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TrjError(TRJ_EDEVICE, f"{LIB_PATH} is not built; run trj_analysis_amd.build.build()")
+        L = C.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    vp, d, i32, i64, u32 = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32), \
+        C.POINTER(C.c_int64), C.c_uint32
+    L.trj_abi_version.restype = C.c_int
+    L.trj_create.argtypes = [C.POINTER(vp), C.c_int]
+    L.trj_destroy.argtypes = [vp]
+    L.trj_destroy.restype = None
+    L.trj_last_error.argtypes = [vp]
+    L.trj_last_error.restype = C.c_char_p
+    L.trj_rdf_init.argtypes = [vp, C.POINTER(RdfParams), d, i32, i32]
+    L.trj_sq_init.argtypes = [vp, C.POINTER(SqParams), d, C.POINTER(SqInfo)]
+    L.trj_sq_plan.argtypes = [C.POINTER(SqParams), d, C.POINTER(SqInfo), i64]
+    L.trj_rdf_plan.argtypes = [C.POINTER(RdfParams), d, i32, d]
+    L.trj_frame.argtypes = [vp, vp, C.c_int32, C.c_int32, i32, C.c_int64, d, u32]
+    L.trj_frame_device.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, C.c_int64, d, u32]
+    L.trj_sync.argtypes = [vp]
+    L.trj_rdf_counts.argtypes = [vp, i64]
+    L.trj_rdf_gmix.argtypes = [vp, C.c_int64, d]
+    L.trj_rdf_first_minimum.argtypes = [vp, d]
+    L.trj_sq_shells.argtypes = [vp, d, d, i64]
+    L.trj_sq_columns.argtypes = [vp, C.c_int64, d, i32, d]
+    L.trj_sq_rho.argtypes = [vp, C.c_int64, i32, d]
+    L.trj_reset.argtypes = [vp]
+    L.trj_last_timings.argtypes = [vp, d]
+    L.trj_timing_totals.argtypes = [vp, d, i64]
+    L.trj_timing_reset.argtypes = [vp]
+    L.trj_device_bytes.argtypes = [vp]
+    L.trj_device_bytes.restype = C.c_int64
+    for name in EXPORTS:
+        if name not in ("trj_destroy", "trj_last_error", "trj_device_bytes"):
+            getattr(L, name).restype = C.c_int
+
+
+def _ptr(a: np.ndarray, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _box3(box) -> np.ndarray:
+    b = np.ones(3, dtype=np.float64)
+    bb = np.asarray(box, dtype=np.float64).ravel()
+    b[: len(bb)] = bb
+    return b
+
+
+def rdf_params(ndim, nsp, deltar, rcrdf, periodic=(True, True, True)) -> RdfParams:
+    p = RdfParams()
+    p.ndim, p.nsp, p.deltar, p.rcrdf = ndim, nsp, float(deltar), float(rcrdf)
+    for k in range(3):
+        p.periodic[k] = int(bool(periodic[k]))
+    return p
+
+
+def sq_params(ndim, nsp, qmax, qmin=None, bsc=None, qset=TRJ_QSET_REF, grid=(0, 0, 0),
+              precision="fp64", flags=0) -> SqParams:
+    p = SqParams()
+    p.ndim, p.nsp, p.qmax = ndim, nsp, float(qmax)
+    p.qmin = float(qmin) if qmin is not None else -1.0
+    b = [1.0] * nsp if bsc is None else list(bsc)
+    for s in range(10):
+        p.bsc[s] = float(b[s]) if s < len(b) else 1.0
+    p.qset = qset
+    for k in range(3):
+        p.grid[k] = int(grid[k])
+    p.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    p.flags = flags
+    return p
+
+
+def sq_plan(p: SqParams, box) -> tuple[SqInfo, np.ndarray]:
+    """Host-only q-set plan (no GPU): info and shell counts nq(0:nqmax)."""
+    info = SqInfo()
+    b = _box3(box)
+    rc = lib().trj_sq_plan(C.byref(p), _ptr(b, C.c_double), C.byref(info), None)
+    if rc != TRJ_OK:
+        raise TrjError(rc, "trj_sq_plan failed")
+    nq = np.zeros(info.nqmax + 1, dtype=np.int64)
+    lib().trj_sq_plan(C.byref(p), _ptr(b, C.c_double), C.byref(info), _ptr(nq, C.c_int64))
+    return info, nq
+
+
+def rdf_plan(p: RdfParams, box) -> tuple[int, float]:
+    b = _box3(box)
+    ls = C.c_int32()
+    rc = C.c_double()
+    r = lib().trj_rdf_plan(C.byref(p), _ptr(b, C.c_double), C.byref(ls), C.byref(rc))
+    if r != TRJ_OK:
+        raise TrjError(r, "trj_rdf_plan failed")
+    return ls.value, rc.value
+
+
+@dataclass
+class RdfState:
+    lsmax: int
+    nit: int
+    nsp: int
+    ndim: int
+
+
+class Context:
+    """One trj_hip context (one HIP stream on one device)."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = C.c_void_p()
+        rc = self._L.trj_create(C.byref(h), device)
+        if rc != TRJ_OK:
+            raise TrjError(rc, f"trj_create(device={device}) failed (no HIP device?)")
+        self._h = h
+        self.rdf: Optional[RdfState] = None
+        self.sq_info: Optional[SqInfo] = None
+        self.sq_nsp = 0
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.trj_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc != TRJ_OK:
+            msg = self._L.trj_last_error(self._h)
+            raise TrjError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    # -- init
+    def rdf_init(self, p: RdfParams, box) -> RdfState:
+        b = _box3(box)
+        ls, nit = C.c_int32(), C.c_int32()
+        self._check(self._L.trj_rdf_init(self._h, C.byref(p), _ptr(b, C.c_double), C.byref(ls),
+                                         C.byref(nit)), "trj_rdf_init")
+        self.rdf = RdfState(ls.value, nit.value, p.nsp, p.ndim)
+        return self.rdf
+
+    def sq_init(self, p: SqParams, box) -> SqInfo:
+        b = _box3(box)
+        info = SqInfo()
+        self._check(self._L.trj_sq_init(self._h, C.byref(p), _ptr(b, C.c_double), C.byref(info)),
+                    "trj_sq_init")
+        self.sq_info = info
+        self.sq_nsp = p.nsp
+        return info
+
+    # -- frames
+    def frame(self, pos: np.ndarray, itype: np.ndarray, box, what=TRJ_WANT_RDF | TRJ_WANT_SQ,
+              layout=TRJ_AOS):
+        pos = np.ascontiguousarray(pos)
+        if pos.dtype not in (np.float32, np.float64):
+            pos = pos.astype(np.float64)
+        dtype = TRJ_F32 if pos.dtype == np.float32 else TRJ_F64
+        it = np.ascontiguousarray(itype, dtype=np.int32)
+        n = len(it)
+        b = _box3(box)
+        self._check(self._L.trj_frame(self._h, pos.ctypes.data_as(C.c_void_p), dtype, layout,
+                                      _ptr(it, C.c_int32), n, _ptr(b, C.c_double), what),
+                    "trj_frame")
+
+    def frame_device(self, pos_ptr: int, dtype: int, layout: int, itype_ptr: int, n: int, box,
+                     what=TRJ_WANT_RDF | TRJ_WANT_SQ):
+        b = _box3(box)
+        self._check(self._L.trj_frame_device(self._h, C.c_void_p(pos_ptr), dtype, layout,
+                                             C.c_void_p(itype_ptr), n, _ptr(b, C.c_double), what),
+                    "trj_frame_device")
+
+    def sync(self):
+        self._check(self._L.trj_sync(self._h), "trj_sync")
+
+    # -- results
+    def rdf_counts(self) -> np.ndarray:
+        r = self.rdf
+        out = np.zeros((r.nit, r.lsmax), dtype=np.int64)
+        self._check(self._L.trj_rdf_counts(self._h, _ptr(out, C.c_int64)), "trj_rdf_counts")
+        return out
+
+    def rdf_gmix(self, nconf: int = 0) -> np.ndarray:
+        r = self.rdf
+        out = np.zeros((max(r.lsmax - 2, 0), 1 + r.nit))
+        self._check(self._L.trj_rdf_gmix(self._h, nconf, _ptr(out, C.c_double)), "trj_rdf_gmix")
+        return out
+
+    def rdf_first_minimum(self) -> float:
+        v = C.c_double()
+        self._check(self._L.trj_rdf_first_minimum(self._h, C.byref(v)), "trj_rdf_first_minimum")
+        return v.value
+
+    def sq_shells(self):
+        ns = self.sq_info.nqmax + 1
+        sqf = np.zeros(ns)
+        sqfp = np.zeros((self.sq_nsp, ns))
+        nq = np.zeros(ns, dtype=np.int64)
+        self._check(self._L.trj_sq_shells(self._h, _ptr(sqf, C.c_double), _ptr(sqfp, C.c_double),
+                                          _ptr(nq, C.c_int64)), "trj_sq_shells")
+        return sqf, sqfp, nq
+
+    def sq_columns(self, nconf: int = 0):
+        nq = self.sq_info.nqmax
+        cols = np.zeros((nq, 7))
+        mix = np.zeros((nq, 1 + self.sq_nsp))
+        nc = C.c_int32()
+        self._check(self._L.trj_sq_columns(self._h, nconf, _ptr(cols, C.c_double), C.byref(nc),
+                                           _ptr(mix, C.c_double)), "trj_sq_columns")
+        cols = cols.reshape(-1)[: nq * nc.value].reshape(nq, nc.value)
+        return cols, mix
+
+    def sq_rho(self, kvecs: Sequence) -> np.ndarray:
+        kv = np.ascontiguousarray(np.asarray(kvecs, dtype=np.int32).reshape(-1, 3))
+        out = np.zeros((len(kv), self.sq_nsp, 2))
+        self._check(self._L.trj_sq_rho(self._h, len(kv), _ptr(kv, C.c_int32), _ptr(out, C.c_double)),
+                    "trj_sq_rho")
+        return out[..., 0] + 1j * out[..., 1]
+
+    def reset(self):
+        self._check(self._L.trj_reset(self._h), "trj_reset")
+
+    def last_timings(self) -> np.ndarray:
+        out = np.zeros(4)
+        self._check(self._L.trj_last_timings(self._h, _ptr(out, C.c_double)), "trj_last_timings")
+        return out
+
+    def timing_totals(self) -> tuple[np.ndarray, int]:
+        out = np.zeros(4)
+        nf = C.c_int64()
+        self._check(self._L.trj_timing_totals(self._h, _ptr(out, C.c_double), C.byref(nf)),
+                    "trj_timing_totals")
+        return out, nf.value
+
+    def timing_reset(self):
+        self._check(self._L.trj_timing_reset(self._h), "trj_timing_reset")
+
+    def device_bytes(self) -> int:
+        return self._L.trj_device_bytes(self._h)
