@@ -1,5 +1,7 @@
 """Extract golden vectors from the reference's shipped example outputs.
 
+Also copies each case's input.nml (data) to inputs/<case>.nml.
+
 Reads (as data, never as code) the n(Q) column of sq.dat and the row count of
 gmixsim.dat in examples/ of the reference checkout, plus the /INPUT_RDF/ and
 /INPUT_SQ/ values of the matching input.nml, and writes nq_golden.json next to
@@ -65,6 +67,11 @@ def main(ref):
             "nq": nq,
             "gmix_rows": grows,
         })
+        # the namelist itself, as a data fixture for the namelist reader
+        nml_dir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "inputs")
+        os.makedirs(nml_dir, exist_ok=True)
+        with open(os.path.join(nml_dir, name.replace("/", "_") + ".nml"), "w") as f:
+            f.write(nml)
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "nq_golden.json")
     with open(path, "w") as f:
         json.dump({"source": "reference examples/*/sq.dat n(Q) column, gmixsim.dat rows",

@@ -124,6 +124,38 @@ def test_lattice_262k_rho_subset():
     orho = O.sq_rho(f.pos, f.itype, 3, f.box, kv)
     err = np.abs(rho - orho)
     print("contract ms", t[1], "max |drho|", err.max(), "info", info.nvec_grid, flush=True)
-    assert err.max() < 2e-11
+    assert err.max() < 5e-12
     # Bragg vector (64,0,0) of the 64^3 lattice is large, (1,0,0) is tiny
     assert np.abs(orho[-2].sum()) > 1e4
+
+
+@pytest.mark.parametrize("prec,tol_rho,tol_shell", [("fp32", 2e-3, 1e-5), ("fp16x3", 2e-3, 1e-5),
+                                                    ("bf16x3", 2e-2, 1e-4)])
+def test_reduced_precision_variants(prec, tol_rho, tol_shell):
+    """FP32 / split-16-bit contraction against the oracle (per vector) and against
+    the FP64 path (per shell)."""
+    rng = np.random.default_rng(12)
+    n, L = 30000, 33.0
+    pos = rng.random((n, 3)) * L
+    it = rng.integers(1, 4, n).astype(np.int32)
+    kv = np.stack([rng.integers(-16, 16, 150), rng.integers(-16, 16, 150), rng.integers(0, 16, 150)], 1)
+    kv = kv[np.any(kv != 0, axis=1)].astype(np.int32)
+    res = {}
+    print("case: precision", prec, flush=True)
+    for p in ["fp64", prec]:
+        with T.Context(0) as ctx:
+            ctx.sq_init(T.sq_params(3, 3, 1e9, bsc=[1.0, 0.6, -0.3], qset=T.TRJ_QSET_HALFSPACE,
+                                    grid=(32, 32, 32), precision=p), [L] * 3)
+            ctx.frame(pos, it, [L] * 3, what=T.TRJ_WANT_SQ)
+            res[p] = (ctx.sq_rho(kv), ctx.sq_shells())
+    orho = O.sq_rho(pos, it, 3, [L] * 3, kv)
+    err64 = np.abs(res["fp64"][0] - orho).max()
+    errp = np.abs(res[prec][0] - orho).max()
+    sq64, sqp = res["fp64"][1][0], res[prec][1][0]
+    m = sq64 > 0
+    shell_rel = np.abs(sqp[m] - sq64[m]) / sq64[m]
+    print(prec, "max|drho| fp64", err64, prec, errp, "shell rel max", shell_rel.max(),
+          "median", np.median(shell_rel), flush=True)
+    assert err64 < 1e-10
+    assert errp < tol_rho
+    assert shell_rel.max() < tol_shell

@@ -69,18 +69,20 @@ void launch_rdf_merge(const uint32_t* partial, int nslices, int nbins, int64_t* 
 
 // S(Q) contraction: rho_s(q) over the active tiles of the dense q-box.
 struct ContractArgs {
-  const double* u;        // species-sorted turns, 3*n (SoA)
+  const double* u;        // species-sorted turns x/L as hi[3][n] then lo[3][n]
   const int32_t* sp_off;  // nsp+1
   const int32_t* tiles;   // 4 ints per tile: kx0, ky0, kz0, pad
   int ntiles, ksplit, nsp, ndim;
   int64_t n;
-  double* rho;            // [split][s][tile][2048][2]
+  double* rho;            // [split][s][tile][tile vectors][2]
 };
 void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s);
+// FP32 MFMA, or FP16 / BF16 hi+lo split with three products (precision =
+// TRJ_PREC_FP32 / FP16X3 / BF16X3); FP32 accumulation folded into FP64.
+void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s);
 // Sum the splits, square, and bin into shells (FP64 atomics).
-void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, int ksplit,
-                      int nsp, const double* bsc, int nqmax, double* sqf, double* sqfp,
-                      hipStream_t s);
+void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, int tile_vec, int ksplit,
+                      int nsp, const double* bsc, int nqmax, double* sqf, double* sqfp, hipStream_t s);
 // Explicit q-vectors (axis directions): direct FP64 phase sums.
 void launch_sq_list(const double* u, const int32_t* sp_off, int64_t n, int nsp, int ndim,
                     const int32_t* kv, const int32_t* shell, int nvec, const double* bsc,

@@ -144,9 +144,16 @@ __global__ __launch_bounds__(256) void k_species_scatter(const int32_t* __restri
   int off = blkoff[static_cast<int64_t>(s) * nblk + blockIdx.x];
   for (int ww = 0; ww < w; ++ww) off += cnt[ww][s];
   off += __popcll(mine & ((1ull << lane) - 1ull));
-  u[off] = pos[i] / lq0;
-  u[n + off] = pos[n + i] / lq1;
-  u[2 * n + off] = (ndim == 3) ? pos[2 * n + i] / lq2 : 0.0;
+  // Turns u = x/L as an unevaluated sum hi + lo: hi rounded, lo = (x - hi*L)/L
+  // with the remainder exact through the FMA.
+  const double lq[3] = {lq0, lq1, lq2};
+  for (int k = 0; k < 3; ++k) {
+    const double x = (k < ndim) ? pos[k * n + i] : 0.0;
+    const double hi = x / lq[k];
+    const double lo = fma(-hi, lq[k], x) / lq[k];
+    u[k * n + off] = hi;
+    u[(3 + k) * n + off] = lo;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_cell_count(const double* __restrict__ pos, int64_t n, int ndim,
@@ -346,20 +353,23 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int KC = 32;            // atoms staged per chunk
 constexpr int kFoldChunks = 32;   // fold MFMA accumulators every KC*32 = 1024 atoms
+// FP64 tile: 8 kx x 8 ky x 32 kz (M = 64 (kx,ky) pairs, N = 32 kz).
+constexpr int TKX = 8, TKY = 8, TKZ = 32, TILE_VEC = TKX * TKY * TKZ;
 
-// Exact-reduction phase in turns: frac(n*u) with the product's rounding error.
-__device__ inline double frac_turns(int nk, double u) {
+// Phase in turns, frac(n*(uh + ul)), reduced exactly: the product's rounding
+// error (FMA) and the low part of u are added after the integer part is gone.
+__device__ inline double frac_turns(int nk, double uh, double ul) {
   const double dn = static_cast<double>(nk);
-  const double hi = dn * u;
-  const double lo = fma(dn, u, -hi);
+  const double hi = dn * uh;
+  const double lo = fma(dn, uh, -hi) + dn * ul;
   return (hi - rint(hi)) + lo;
 }
 
 // e^{2 pi i n u} for n = n0 .. n0+7 into dst (anchor + angle-addition steps).
-__device__ inline void phase8(double u, int n0, double2* dst) {
+__device__ inline void phase8(double uh, double ul, int n0, double2* dst) {
   double s, c, ss, cs;
-  sincospi(2.0 * frac_turns(n0, u), &s, &c);
-  sincospi(2.0 * (u - rint(u)), &ss, &cs);
+  sincospi(2.0 * frac_turns(n0, uh, ul), &s, &c);
+  sincospi(2.0 * frac_turns(1, uh, ul), &ss, &cs);
   double re = c, im = s;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
@@ -381,9 +391,6 @@ __global__ __launch_bounds__(256) void k_sq_contract_f64(ContractArgs a) {
   const int t = threadIdx.x, w = t / kWave, lane = t & (kWave - 1);
   const int i16 = lane & 15, kq = lane >> 4;
   const int m = w * 16 + i16, mx = m / TKY, my = m % TKY;
-  const double* __restrict__ ux = a.u;
-  const double* __restrict__ uy = a.u + a.n;
-  const double* __restrict__ uz = a.u + 2 * a.n;
   // Phase-staging role of this thread: atom at, segment seg (0: x, 1: y, 2-5: z).
   const int st_at = t / 6, st_seg = t % 6;
   for (int s = 0; s < a.nsp; ++s) {
@@ -404,23 +411,22 @@ __global__ __launch_bounds__(256) void k_sq_contract_f64(ContractArgs a) {
       if (t < KC * 6) {
         const int ai = c0 + st_at;
         double2* dst;
-        int n0;
-        const double* src;
+        int n0, ax;
         if (st_seg == 0) {
           dst = phx[st_at];
           n0 = kx0;
-          src = ux;
+          ax = 0;
         } else if (st_seg == 1) {
           dst = phy[st_at];
           n0 = ky0;
-          src = uy;
+          ax = 1;
         } else {
           dst = phz[st_at] + 8 * (st_seg - 2);
           n0 = kz0 + 8 * (st_seg - 2);
-          src = uz;
+          ax = 2;
         }
         if (ai < b1) {
-          phase8(src[ai], n0, dst);
+          phase8(a.u[ax * a.n + ai], a.u[(3 + ax) * a.n + ai], n0, dst);
         } else {
 #pragma unroll
           for (int k = 0; k < 8; ++k) dst[k] = make_double2(0.0, 0.0);
@@ -459,16 +465,263 @@ __global__ __launch_bounds__(256) void k_sq_contract_f64(ContractArgs a) {
       sumR[nb] += accR[nb];
       sumI[nb] += accI[nb];
     }
-    // D layout of v_mfma_f64_16x16x4_f64: lane holds rows 4*(lane/16)+r, column lane%16.
+    // D layout of v_mfma_f64_16x16x4_f64: register r of a lane holds row
+    // 4*r + lane/16, column lane%16 (the FP32/16-bit 16x16 forms hold row
+    // 4*(lane/16) + r instead).
     const size_t base = ((static_cast<size_t>(split) * a.nsp + s) * a.ntiles + tile) * TILE_VEC;
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int mrow = w * 16 + 4 * kq + r;
+        const int mrow = w * 16 + 4 * r + kq;
         const int v = ((mrow / TKY) * TKY + (mrow % TKY)) * TKZ + nb * 16 + i16;
         double2 val = make_double2(sumR[nb][r], sumI[nb][r]);
         reinterpret_cast<double2*>(a.rho)[base + v] = val;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------- reduced precision
+// Tile 64 kx x 4 ky x 32 kz: M = 64 kx (one 16-row block per wavefront),
+// N = 128 (ky,kz) pairs (eight 16-column blocks), K = 32 atoms per chunk.
+// The phase products ey*ez are formed once per workgroup (the B operand all
+// four wavefronts share); ex needs no product.  Phases: exact FP64 reduction
+// of the anchor n0*u and of the step u, FP32 sincospi, FP32 angle addition.
+constexpr int LX = 64, LY = 4, LZ = 32, LN = LY * LZ;
+constexpr int LKC = 32;           // atoms per chunk (K of one 16x16x32 MFMA)
+constexpr int LKP = 40;           // padded row length of the 16-bit operand tables
+constexpr int LKF = LKC + 1;      // padded row length of the FP32 operand tables
+constexpr int kLpFoldChunks = 8;  // fold FP32 accumulators into FP64 every 256 atoms
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+
+template <int PREC>
+struct LpTypes;
+template <>
+struct LpTypes<TRJ_PREC_FP16X3> {
+  typedef _Float16 T;
+  typedef h8 V;
+  __device__ static f4 mfma(V a, V b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+};
+template <>
+struct LpTypes<TRJ_PREC_BF16X3> {
+  typedef __bf16 T;
+  typedef b8 V;
+  __device__ static f4 mfma(V a, V b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+};
+
+// e^{2 pi i n u}, n = n0 .. n0+cnt-1, in FP32 (cnt <= 8).
+__device__ inline void phase_f32(double uh, double ul, int n0, int cnt, float* re, float* im) {
+  float s, c, ss, cs;
+  sincospif(static_cast<float>(2.0 * frac_turns(n0, uh, ul)), &s, &c);
+  sincospif(static_cast<float>(2.0 * frac_turns(1, uh, ul)), &ss, &cs);
+  float r = c, i = s;
+  for (int k = 0; k < cnt; ++k) {
+    re[k] = r;
+    im[k] = i;
+    const float nr = fmaf(r, cs, -(i * ss));
+    const float ni = fmaf(r, ss, i * cs);
+    r = nr;
+    i = ni;
+  }
+}
+
+template <class T>
+__device__ inline void split2(float x, T& hi, T& lo) {
+  hi = static_cast<T>(x);
+  lo = static_cast<T>(x - static_cast<float>(hi));
+}
+
+template <int PREC>
+struct LpShared;
+template <>
+struct LpShared<TRJ_PREC_FP32> {
+  float A[2][LX][LKF];   // re, im of ex
+  float B[2][LN][LKF];   // re, im of ey*ez
+  float2 ey[LKC][LY];
+  float2 ez[LKC][LZ];
+};
+template <int PREC>
+struct LpShared {
+  typedef typename LpTypes<PREC>::T T;
+  __attribute__((aligned(16))) T A[4][LX][LKP];  // re_hi, re_lo, im_hi, im_lo of ex
+  __attribute__((aligned(16))) T B[4][LN][LKP];  // re_hi, re_lo, im_hi, im_lo of ey*ez
+  float2 ey[LKC][LY];
+  float2 ez[LKC][LZ];
+};
+
+template <int PREC>
+__device__ inline void lp_store(LpShared<PREC>& sh, bool isA, int row, int a, float re, float im) {
+  if constexpr (PREC == TRJ_PREC_FP32) {
+    if (isA) {
+      sh.A[0][row][a] = re;
+      sh.A[1][row][a] = im;
+    } else {
+      sh.B[0][row][a] = re;
+      sh.B[1][row][a] = im;
+    }
+  } else {
+    typedef typename LpTypes<PREC>::T T;
+    T rh, rl, ih, il;
+    split2(re, rh, rl);
+    split2(im, ih, il);
+    if (isA) {
+      sh.A[0][row][a] = rh;
+      sh.A[1][row][a] = rl;
+      sh.A[2][row][a] = ih;
+      sh.A[3][row][a] = il;
+    } else {
+      sh.B[0][row][a] = rh;
+      sh.B[1][row][a] = rl;
+      sh.B[2][row][a] = ih;
+      sh.B[3][row][a] = il;
+    }
+  }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_sq_contract_lp(ContractArgs a) {
+  __shared__ LpShared<PREC> sh;
+  const int tile = blockIdx.x % a.ntiles;
+  const int split = blockIdx.x / a.ntiles;
+  const int kx0 = a.tiles[4 * tile], ky0 = a.tiles[4 * tile + 1], kz0 = a.tiles[4 * tile + 2];
+  const int t = threadIdx.x, w = t / kWave, lane = t & (kWave - 1);
+  const int i16 = lane & 15, kq = lane >> 4;
+  const double* __restrict__ ux = a.u;
+  const double* __restrict__ uy = a.u + a.n;
+  const double* __restrict__ uz = a.u + 2 * a.n;
+  const double* __restrict__ lx = a.u + 3 * a.n;
+  const double* __restrict__ ly = a.u + 4 * a.n;
+  const double* __restrict__ lz = a.u + 5 * a.n;
+  for (int s = 0; s < a.nsp; ++s) {
+    const int a0 = a.sp_off[s], a1 = a.sp_off[s + 1];
+    const int64_t len = a1 - a0;
+    const int b0 = a0 + static_cast<int>(len * split / a.ksplit);
+    const int b1 = a0 + static_cast<int>(len * (split + 1) / a.ksplit);
+    f4 accR[8], accI[8];
+    d4 sumR[8], sumI[8];
+    for (int nb = 0; nb < 8; ++nb) {
+      accR[nb] = f4{0.f, 0.f, 0.f, 0.f};
+      accI[nb] = accR[nb];
+      sumR[nb] = d4{0.0, 0.0, 0.0, 0.0};
+      sumI[nb] = sumR[nb];
+    }
+    int chunks = 0;
+    for (int c0 = b0; c0 < b1; c0 += LKC) {
+      __syncthreads();
+      {  // ex: 8 segments of 8 kx per atom, one per thread
+        const int at = t >> 3, seg = t & 7, ai = c0 + at;
+        float re[8], im[8];
+        if (ai < b1) {
+          phase_f32(ux[ai], lx[ai], kx0 + 8 * seg, 8, re, im);
+        } else {
+          for (int k = 0; k < 8; ++k) re[k] = im[k] = 0.f;
+        }
+        for (int k = 0; k < 8; ++k) lp_store<PREC>(sh, true, 8 * seg + k, at, re[k], im[k]);
+      }
+      if (t < LKC * 5) {  // ey (4 values) and ez (4 segments of 8) per atom
+        const int at = t / 5, sub = t % 5, ai = c0 + at;
+        float re[8], im[8];
+        const int cnt = (sub == 0) ? LY : 8;
+        if (ai < b1) {
+          if (sub == 0)
+            phase_f32(uy[ai], ly[ai], ky0, LY, re, im);
+          else
+            phase_f32(uz[ai], lz[ai], kz0 + 8 * (sub - 1), 8, re, im);
+        } else {
+          for (int k = 0; k < 8; ++k) re[k] = im[k] = 0.f;
+        }
+        for (int k = 0; k < cnt; ++k) {
+          if (sub == 0)
+            sh.ey[at][k] = make_float2(re[k], im[k]);
+          else
+            sh.ez[at][8 * (sub - 1) + k] = make_float2(re[k], im[k]);
+        }
+      }
+      __syncthreads();
+      {  // B = ey*ez: 128 (ky,kz) x 32 atoms, 16 per thread
+        const int n = t & (LN - 1), ab = (t >> 7) * 16;
+        const int ky = n / LZ, kz = n % LZ;
+        for (int k = 0; k < 16; ++k) {
+          const float2 y = sh.ey[ab + k][ky], z = sh.ez[ab + k][kz];
+          lp_store<PREC>(sh, false, n, ab + k, fmaf(y.x, z.x, -(y.y * z.y)), fmaf(y.x, z.y, y.y * z.x));
+        }
+      }
+      __syncthreads();
+      const int row = 16 * w + i16;
+      if constexpr (PREC == TRJ_PREC_FP32) {
+#pragma unroll
+        for (int kk = 0; kk < LKC / 4; ++kk) {
+          const int at = 4 * kk + kq;
+          const float ar = sh.A[0][row][at], ai = sh.A[1][row][at];
+#pragma unroll
+          for (int nb = 0; nb < 8; ++nb) {
+            const float br = sh.B[0][16 * nb + i16][at], bi = sh.B[1][16 * nb + i16][at];
+            accR[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, accR[nb], 0, 0, 0);
+            accR[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, bi, accR[nb], 0, 0, 0);
+            accI[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, accI[nb], 0, 0, 0);
+            accI[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, accI[nb], 0, 0, 0);
+          }
+        }
+      } else {
+        typedef LpTypes<PREC> LT;
+        typedef typename LT::V V;
+        const V arh = *reinterpret_cast<const V*>(&sh.A[0][row][8 * kq]);
+        const V arl = *reinterpret_cast<const V*>(&sh.A[1][row][8 * kq]);
+        const V aih = *reinterpret_cast<const V*>(&sh.A[2][row][8 * kq]);
+        const V ail = *reinterpret_cast<const V*>(&sh.A[3][row][8 * kq]);
+        const V naih = -aih, nail = -ail;
+#pragma unroll
+        for (int nb = 0; nb < 8; ++nb) {
+          const int col = 16 * nb + i16;
+          const V brh = *reinterpret_cast<const V*>(&sh.B[0][col][8 * kq]);
+          const V brl = *reinterpret_cast<const V*>(&sh.B[1][col][8 * kq]);
+          const V bih = *reinterpret_cast<const V*>(&sh.B[2][col][8 * kq]);
+          const V bil = *reinterpret_cast<const V*>(&sh.B[3][col][8 * kq]);
+          f4 r = accR[nb], i = accI[nb];
+          r = LT::mfma(arh, brh, r);
+          r = LT::mfma(arh, brl, r);
+          r = LT::mfma(arl, brh, r);
+          r = LT::mfma(naih, bih, r);
+          r = LT::mfma(naih, bil, r);
+          r = LT::mfma(nail, bih, r);
+          i = LT::mfma(arh, bih, i);
+          i = LT::mfma(arh, bil, i);
+          i = LT::mfma(arl, bih, i);
+          i = LT::mfma(aih, brh, i);
+          i = LT::mfma(aih, brl, i);
+          i = LT::mfma(ail, brh, i);
+          accR[nb] = r;
+          accI[nb] = i;
+        }
+      }
+      if (++chunks == kLpFoldChunks) {
+        chunks = 0;
+#pragma unroll
+        for (int nb = 0; nb < 8; ++nb) {
+          for (int r = 0; r < 4; ++r) {
+            sumR[nb][r] += static_cast<double>(accR[nb][r]);
+            sumI[nb][r] += static_cast<double>(accI[nb][r]);
+          }
+          accR[nb] = f4{0.f, 0.f, 0.f, 0.f};
+          accI[nb] = accR[nb];
+        }
+      }
+    }
+    const size_t base = ((static_cast<size_t>(split) * a.nsp + s) * a.ntiles + tile) * (LX * LN);
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int kxl = 16 * w + 4 * kq + r;
+        const int n = 16 * nb + i16;
+        const int v = (kxl * LY + n / LZ) * LZ + (n % LZ);
+        const double re = sumR[nb][r] + static_cast<double>(accR[nb][r]);
+        const double im = sumI[nb][r] + static_cast<double>(accI[nb][r]);
+        reinterpret_cast<double2*>(a.rho)[base + v] = make_double2(re, im);
       }
     }
   }
@@ -480,7 +733,7 @@ struct Bsc {
 
 __global__ __launch_bounds__(256) void k_sq_shells(const double* __restrict__ rho,
                                                    const int16_t* __restrict__ tile_shell, int ntiles,
-                                                   int ksplit, int nsp, Bsc bsc, int nqmax,
+                                                   int tile_vec, int ksplit, int nsp, Bsc bsc, int nqmax,
                                                    double* __restrict__ sqf, double* __restrict__ sqfp,
                                                    int use_lds) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -490,7 +743,7 @@ __global__ __launch_bounds__(256) void k_sq_shells(const double* __restrict__ rh
     for (int b = threadIdx.x; b < (1 + nsp) * ns; b += 256) acc[b] = 0.0;
     __syncthreads();
   }
-  const int64_t total = static_cast<int64_t>(ntiles) * TILE_VEC;
+  const int64_t total = static_cast<int64_t>(ntiles) * tile_vec;
   const double2* r2 = reinterpret_cast<const double2*>(rho);
   for (int64_t e = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; e < total;
        e += static_cast<int64_t>(gridDim.x) * 256) {
@@ -546,8 +799,8 @@ __global__ __launch_bounds__(256) void k_sq_list(const double* __restrict__ u,
   for (int s = 0; s < nsp; ++s) {
     double pr = 0.0, pi = 0.0;
     for (int j = sp_off[s] + threadIdx.x; j < sp_off[s + 1]; j += 256) {
-      double f = frac_turns(kx, u[j]) + frac_turns(ky, u[n + j]);
-      if (ndim == 3) f += frac_turns(kz, u[2 * n + j]);
+      double f = frac_turns(kx, u[j], u[3 * n + j]) + frac_turns(ky, u[n + j], u[4 * n + j]);
+      if (ndim == 3) f += frac_turns(kz, u[2 * n + j], u[5 * n + j]);
       f -= rint(f);
       double sn, cn;
       sincospi(2.0 * f, &sn, &cn);
@@ -665,18 +918,29 @@ void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_sq_contract_f64, dim3(a.ntiles * a.ksplit), dim3(256), 0, s, a);
 }
 
-void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, int ksplit, int nsp,
-                      const double* bsc, int nqmax, double* sqf, double* sqfp, hipStream_t s) {
+void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s) {
+  if (a.ntiles <= 0) return;
+  const dim3 grid(a.ntiles * a.ksplit), block(256);
+  if (precision == TRJ_PREC_FP32)
+    hipLaunchKernelGGL(k_sq_contract_lp<TRJ_PREC_FP32>, grid, block, 0, s, a);
+  else if (precision == TRJ_PREC_FP16X3)
+    hipLaunchKernelGGL(k_sq_contract_lp<TRJ_PREC_FP16X3>, grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL(k_sq_contract_lp<TRJ_PREC_BF16X3>, grid, block, 0, s, a);
+}
+
+void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, int tile_vec, int ksplit,
+                      int nsp, const double* bsc, int nqmax, double* sqf, double* sqfp, hipStream_t s) {
   if (ntiles <= 0) return;
   Bsc b{};
   for (int i = 0; i < nsp; ++i) b.b[i] = bsc[i];
   const size_t lds = static_cast<size_t>(1 + nsp) * (nqmax + 1) * sizeof(double);
   const int use_lds = lds <= 48 * 1024;
-  const int64_t total = static_cast<int64_t>(ntiles) * TILE_VEC;
+  const int64_t total = static_cast<int64_t>(ntiles) * tile_vec;
   int64_t g = (total + 255) / 256;
   if (g > 1024) g = 1024;
   hipLaunchKernelGGL(k_sq_shells, dim3(static_cast<unsigned>(g)), dim3(256), use_lds ? lds : 0, s, rho,
-                     tile_shell, ntiles, ksplit, nsp, b, nqmax, sqf, sqfp, use_lds);
+                     tile_shell, ntiles, tile_vec, ksplit, nsp, b, nqmax, sqf, sqfp, use_lds);
 }
 
 void launch_sq_list(const double* u, const int32_t* sp_off, int64_t n, int nsp, int ndim, const int32_t* kv,

@@ -287,8 +287,14 @@ int sq_plan(const trj_sq_params& p_in, const double box[3], SqPlan* out, std::st
     dq64 = std::fmin(fk64[0], fk64[1]);
     if (ndim == 3) dq64 = std::fmin(dq64, fk64[2]);
   }
+  if (p.precision < TRJ_PREC_FP64 || p.precision > TRJ_PREC_BF16X3) {
+    *err = "sq: unknown precision";
+    return TRJ_EINVAL;
+  }
+  P.tk = tile_shape(p.precision);
+  const int TKX = P.tk.tx, TKY = P.tk.ty, TKZ = P.tk.tz;
   if (P.kmax[0] >= P.kmin[0] && P.kmax[1] >= P.kmin[1] && P.kmax[2] >= P.kmin[2]) {
-    std::vector<int16_t> sh(TILE_VEC);
+    std::vector<int16_t> sh(P.tk.vec());
     for (int x0 = P.kmin[0]; x0 <= P.kmax[0]; x0 += TKX)
       for (int y0 = P.kmin[1]; y0 <= P.kmax[1]; y0 += TKY)
         for (int z0 = P.kmin[2]; z0 <= P.kmax[2]; z0 += TKZ) {

@@ -221,7 +221,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
   if (do_sq) {
     // Species-contiguous turns u = x / Lq for the contraction (Lq: the box the
     // q-grid was built on, sq.cuf:189).
-    if ((r = ensure(c, c->u, sizeof(double) * 3 * n))) return r;
+    if ((r = ensure(c, c->u, sizeof(double) * 6 * n))) return r;
     const double lq[3] = {c->sq.box[0], c->sq.box[1], c->sq.box[2]};
     launch_species_scatter(itype_dev, n, nsp, P<int32_t>(c->blkoff), nblk, P<double>(c->pos), lq, ndim,
                            P<double>(c->u), c->stream);
@@ -314,7 +314,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
       int ks = static_cast<int>((2048 + ntiles - 1) / ntiles);
       ks = std::max(1, std::min<int>(ks, static_cast<int>(std::max<int64_t>(1, n / 8192))));
       c->ksplit = ks;
-      const size_t rho_bytes = sizeof(double) * 2 * static_cast<size_t>(ks) * nsp * ntiles * TILE_VEC;
+      const size_t rho_bytes = sizeof(double) * 2 * static_cast<size_t>(ks) * nsp * ntiles * S.tk.vec();
       if ((r = ensure(c, c->rho, rho_bytes))) return r;
       ContractArgs ca{};
       ca.u = P<double>(c->u);
@@ -326,12 +326,16 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
       ca.ndim = ndim;
       ca.n = n;
       ca.rho = P<double>(c->rho);
-      launch_sq_contract_f64(ca, c->stream);
+      if (S.p.precision == TRJ_PREC_FP64)
+        launch_sq_contract_f64(ca, c->stream);
+      else
+        launch_sq_contract_lp(ca, S.p.precision, c->stream);
     }
     HIPC(c, hipGetLastError());
     HIPC(c, hipEventRecord(es.ev[3], c->stream));
     if (ntiles > 0)
-      launch_sq_shells(P<double>(c->rho), P<int16_t>(c->tile_shell), ntiles, c->ksplit, nsp, S.p.bsc, S.nqmax,
+      launch_sq_shells(P<double>(c->rho), P<int16_t>(c->tile_shell), ntiles, S.tk.vec(), c->ksplit, nsp, S.p.bsc,
+                       S.nqmax,
                        P<double>(c->sqf), P<double>(c->sqfp), c->stream);
     launch_sq_list(P<double>(c->u), P<int32_t>(c->sp_off), n, nsp, ndim, P<int32_t>(c->axis_kv),
                    P<int32_t>(c->axis_shell), static_cast<int>(S.axis_shell.size()), S.p.bsc, S.nqmax,
@@ -726,6 +730,7 @@ int trj_sq_rho(trj_ctx* c, int64_t nvec, const int32_t* kv, double* rho) {
     const int x = kv[3 * v], y = kv[3 * v + 1], z = kv[3 * v + 2];
     if (x < S.kmin[0] || y < S.kmin[1] || z < S.kmin[2])
       return fail(c, TRJ_EINVAL, "vector outside the dense q-box");
+    const int TKX = S.tk.tx, TKY = S.tk.ty, TKZ = S.tk.tz, TILE_VEC = S.tk.vec();
     const int x0 = S.kmin[0] + ((x - S.kmin[0]) / TKX) * TKX;
     const int y0 = S.kmin[1] + ((y - S.kmin[1]) / TKY) * TKY;
     const int z0 = S.kmin[2] + ((z - S.kmin[2]) / TKZ) * TKZ;

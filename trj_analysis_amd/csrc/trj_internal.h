@@ -42,9 +42,18 @@ FrameGeom frame_geometry(int ndim, const double box[3], const int periodic[3], d
 
 // ---------------------------------------------------------------- S(Q) plan
 // Tile of the dense q-box computed by one workgroup of the contraction:
-// kx in [kx0, kx0+TKX), ky in [ky0, ky0+TKY), kz in [kz0, kz0+TKZ).
-constexpr int TKX = 8, TKY = 8, TKZ = 32;
-constexpr int TILE_VEC = TKX * TKY * TKZ;  // 2048
+// kx in [kx0, kx0+tx), ky in [ky0, ky0+ty), kz in [kz0, kz0+tz); vectors are
+// numbered (i*ty + j)*tz + k inside a tile (kz fastest).
+struct TileShape {
+  int tx, ty, tz;
+  int vec() const { return tx * ty * tz; }
+};
+// FP64: M = 64 (kx,ky) pairs x N = 32 kz per workgroup.  FP32 / split 16-bit:
+// M = 64 kx x N = 128 (ky,kz) pairs, so the phase products ey*ez form the
+// operand shared by all four wavefronts.
+inline TileShape tile_shape(int precision) {
+  return precision == TRJ_PREC_FP64 ? TileShape{8, 8, 32} : TileShape{64, 4, 32};
+}
 
 struct SqTile {
   int32_t kx0, ky0, kz0, pad;
@@ -59,8 +68,9 @@ struct SqPlan {
   double box[3];                     // box the q-grid was built on
   std::vector<int64_t> nq;           // normalisation counts (0..nqmax)
   std::vector<int64_t> nq_summed;    // vectors actually summed per shell
+  TileShape tk{8, 8, 32};            // tile shape of the contraction kernel
   std::vector<SqTile> tiles;         // active tiles of the dense box
-  std::vector<int16_t> tile_shell;   // [tile][TILE_VEC], -1 = not summed
+  std::vector<int16_t> tile_shell;   // [tile][tk.vec()], -1 = not summed
   std::vector<int32_t> axis_kv;      // axis vectors (x,y,z) triples
   std::vector<int32_t> axis_shell;   // their shells
   int kmin[3], kmax[3];              // inclusive index range of the dense box
