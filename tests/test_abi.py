@@ -59,3 +59,16 @@ def test_no_gpu_context_fails_loudly():
     import pytest
     with pytest.raises(T.TrjError):
         T.Context(0)
+
+
+def test_plan_large_qmax_is_clamped():
+    """qmax/dq above the int range: half-space clamps to the grid's largest shell,
+    the reference q-set reports a limit error instead of overflowing."""
+    from trj_analysis_amd import trjhip as T
+    import pytest
+    L = (1e6 / 0.85) ** (1 / 3)
+    info, nq = T.sq_plan(T.sq_params(3, 3, 1e9, qset=T.TRJ_QSET_HALFSPACE, grid=(64, 64, 64)), [L] * 3)
+    assert info.nvec_grid == 64 * 64 * 32 - 1 and nq.sum() == info.nvec_grid and info.nqmax == 55
+    with pytest.raises(T.TrjError) as e:
+        T.sq_plan(T.sq_params(3, 1, 1e9), [L] * 3)
+    assert e.value.code == T.TRJ_ELIMIT

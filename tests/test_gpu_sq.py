@@ -124,7 +124,8 @@ def test_lattice_262k_rho_subset():
     orho = O.sq_rho(f.pos, f.itype, 3, f.box, kv)
     err = np.abs(rho - orho)
     print("contract ms", t[1], "max |drho|", err.max(), "info", info.nvec_grid, flush=True)
-    assert err.max() < 5e-12
+    # absolute 5e-12 plus 1e-15 relative (the (64,0,0) Bragg term has |rho| ~ 9e4)
+    assert np.all(err < 5e-12 + 1e-15 * np.abs(orho))
     # Bragg vector (64,0,0) of the 64^3 lattice is large, (1,0,0) is tiny
     assert np.abs(orho[-2].sum()) > 1e4
 

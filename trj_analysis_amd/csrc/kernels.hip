@@ -15,8 +15,9 @@
 // v_mfma_f64_16x16x4_f64 (4 real products per complex product).  Phases come
 // from an exact FP64 reduction of n*u (u = x/L in turns, with the FMA error
 // term) and sincospi, extended by an 8-step angle-addition recurrence.
-// MFMA accumulators are folded into FP64 sums every 1024 atoms (blocked
-// summation keeps the rounding error of 1e5-atom sums near 1e-13).
+// MFMA accumulators are folded every 32 atoms into FP64 sums with TwoSum
+// compensation, so the rounding error of a 1e5-atom sum stays near that of
+// one 32-term block even when long runs of atoms add coherently.
 //
 // Compiled with -ffp-contract=off: every FP64 operation of the RDF pair rule
 // rounds once, in the order written, like the oracle.
@@ -352,7 +353,6 @@ __global__ __launch_bounds__(256) void k_rdf_merge(const uint32_t* __restrict__ 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int KC = 32;            // atoms staged per chunk
-constexpr int kFoldChunks = 32;   // fold MFMA accumulators every KC*32 = 1024 atoms
 // FP64 tile: 8 kx x 8 ky x 32 kz (M = 64 (kx,ky) pairs, N = 32 kz).
 constexpr int TKX = 8, TKY = 8, TKZ = 32, TILE_VEC = TKX * TKY * TKZ;
 
@@ -381,6 +381,15 @@ __device__ inline void phase8(double uh, double ul, int n0, double2* dst) {
   }
 }
 
+// s + c += x with Knuth's TwoSum: s keeps the rounded sum, c the exact error.
+__device__ inline void two_sum_acc(double& s, double& c, double x) {
+  const double t = s + x;
+  const double bp = t - s;
+  const double e = (s - (t - bp)) + (x - bp);
+  s = t;
+  c += e;
+}
+
 __global__ __launch_bounds__(256) void k_sq_contract_f64(ContractArgs a) {
   __shared__ double2 phx[KC][TKX];
   __shared__ double2 phy[KC][TKY];
@@ -398,15 +407,15 @@ __global__ __launch_bounds__(256) void k_sq_contract_f64(ContractArgs a) {
     const int64_t len = a1 - a0;
     const int b0 = a0 + static_cast<int>(len * split / a.ksplit);
     const int b1 = a0 + static_cast<int>(len * (split + 1) / a.ksplit);
-    d4 accR[2], accI[2], sumR[2], sumI[2];
-    for (int nb = 0; nb < 2; ++nb) {
-      accR[nb] = d4{0.0, 0.0, 0.0, 0.0};
-      accI[nb] = accR[nb];
-      sumR[nb] = accR[nb];
-      sumI[nb] = accR[nb];
-    }
-    int chunks = 0;
+    d4 accR[2], accI[2];
+    double sumR[2][4], sumI[2][4], cmpR[2][4], cmpI[2][4];
+    for (int nb = 0; nb < 2; ++nb)
+      for (int r = 0; r < 4; ++r) sumR[nb][r] = sumI[nb][r] = cmpR[nb][r] = cmpI[nb][r] = 0.0;
     for (int c0 = b0; c0 < b1; c0 += KC) {
+      for (int nb = 0; nb < 2; ++nb) {
+        accR[nb] = d4{0.0, 0.0, 0.0, 0.0};
+        accI[nb] = accR[nb];
+      }
       __syncthreads();
       if (t < KC * 6) {
         const int ai = c0 + st_at;
@@ -451,20 +460,17 @@ __global__ __launch_bounds__(256) void k_sq_contract_f64(ContractArgs a) {
         accI[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bz1.y, accI[1], 0, 0, 0);
         accI[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bz1.x, accI[1], 0, 0, 0);
       }
-      if (++chunks == kFoldChunks) {
-        chunks = 0;
-        for (int nb = 0; nb < 2; ++nb) {
-          sumR[nb] += accR[nb];
-          sumI[nb] += accI[nb];
-          accR[nb] = d4{0.0, 0.0, 0.0, 0.0};
-          accI[nb] = accR[nb];
+      for (int nb = 0; nb < 2; ++nb)
+        for (int r = 0; r < 4; ++r) {
+          two_sum_acc(sumR[nb][r], cmpR[nb][r], accR[nb][r]);
+          two_sum_acc(sumI[nb][r], cmpI[nb][r], accI[nb][r]);
         }
+    }
+    for (int nb = 0; nb < 2; ++nb)
+      for (int r = 0; r < 4; ++r) {
+        sumR[nb][r] += cmpR[nb][r];
+        sumI[nb][r] += cmpI[nb][r];
       }
-    }
-    for (int nb = 0; nb < 2; ++nb) {
-      sumR[nb] += accR[nb];
-      sumI[nb] += accI[nb];
-    }
     // D layout of v_mfma_f64_16x16x4_f64: register r of a lane holds row
     // 4*r + lane/16, column lane%16 (the FP32/16-bit 16x16 forms hold row
     // 4*(lane/16) + r instead).

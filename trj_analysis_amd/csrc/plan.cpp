@@ -161,7 +161,12 @@ int sq_plan(const trj_sq_params& p_in, const double box[3], SqPlan* out, std::st
     for (int k = 0; k < 3; ++k) P.fk[k] = (2.0f * kPiF) / side[k];
     P.dq = std::fmin(P.fk[0], P.fk[1]);
     if (ndim == 3) P.dq = std::fmin(P.dq, P.fk[2]);
-    P.nqmax = nint_f(static_cast<float>(p.qmax) / P.dq);
+    const float nqf = static_cast<float>(p.qmax) / P.dq;
+    if (!(nqf < 32767.0f)) {
+      *err = "sq: qmax/dq above 32767 shells is not supported";
+      return TRJ_ELIMIT;
+    }
+    P.nqmax = nint_f(nqf);
     float qminf = static_cast<float>(p.qmin);
     P.l[0] = nint_f(qminf / P.fk[0]);
     P.l[1] = nint_f(qminf / P.fk[1]);
@@ -269,7 +274,8 @@ int sq_plan(const trj_sq_params& p_in, const double box[3], SqPlan* out, std::st
           double qa = a * fk64[0], qb = b * fk64[1], qc = c * fk64[2];
           maxshell = std::max(maxshell, static_cast<int>(nint_d(std::sqrt(qa * qa + qb * qb + qc * qc) / dq64)));
         }
-    P.nqmax = std::min(maxshell, static_cast<int>(nint_d(p.qmax / dq64)));
+    const double qcap = std::floor(p.qmax / dq64 + 0.5);  // nint without int overflow
+    P.nqmax = (qcap < static_cast<double>(maxshell)) ? static_cast<int>(qcap) : maxshell;
     P.nqmin = 0;
     P.qminp = 0.0f;
     P.qmin2 = 0.0f;
