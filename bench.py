@@ -163,6 +163,12 @@ def main():
 
     fps = args.steps / elapsed
     flops = 8.0 * n * info.nvec_grid
+    kname = "k_sq_contract_f64" if args.precision == "fp64" else f"k_sq_contract_lp<{args.precision}>"
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(f"{args.workload}/{args.precision}/{kname}")
     achieved = flops / (kernel_ms["sq_contract"] * 1e-3)
     peak = PEAK[args.precision]
     out = {
@@ -183,11 +189,11 @@ def main():
                    "lsmax": st.lsmax, "precision": args.precision, "inputs": "HBM-resident float64"},
         "pairs_per_s": pairs / elapsed,
         "q_atom_terms_per_s": nvec * n * args.steps / elapsed,
-        "latency_ms_p50": float(np.percentile(lat, 50)),
-        "latency_ms_p99": float(np.percentile(lat, 99)),
+        "latency_ms_p50": float(np.percentile(lat, 50)) if lat else None,
+        "latency_ms_p99": float(np.percentile(lat, 99)) if lat else None,
         "kernel_ms": kernel_ms,
-        "roofline": {"bound": "mfma", "kernel": "k_sq_contract_f64", "achieved": achieved / 1e12,
-                     "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+        "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved / 1e12,
+                     "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                      "algorithmic": f"8*N*Nq = {flops:.4e} flop per launch (N={n}, Nq={info.nvec_grid})"},
     }
     ctx.close()

@@ -83,9 +83,11 @@ void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s);
 // Sum the splits, square, and bin into shells (FP64 atomics).
 void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, int tile_vec, int ksplit,
                       int nsp, const double* bsc, int nqmax, double* sqf, double* sqfp, hipStream_t s);
-// Explicit q-vectors (axis directions): direct FP64 phase sums.
+// Explicit q-vectors (axis directions): direct FP64 phase sums over atom
+// blocks, combined in a fixed order; part needs sq_list_scratch_bytes().
+size_t sq_list_scratch_bytes(int nvec, int nsp);
 void launch_sq_list(const double* u, const int32_t* sp_off, int64_t n, int nsp, int ndim,
                     const int32_t* kv, const int32_t* shell, int nvec, const double* bsc,
-                    int nqmax, double* sqf, double* sqfp, hipStream_t s);
+                    int nqmax, double* part, double* sqf, double* sqfp, hipStream_t s);
 
 }  // namespace trj

@@ -339,13 +339,20 @@ __global__ __launch_bounds__(256) void k_rdf_allpairs(PairArgs a, const double* 
   hist_flush(a, hist, partial);
 }
 
+// Slice group blockIdx.y sums its share of the per-workgroup slices, then adds
+// into the int64 totals (integer atomics: the result is order independent).
+constexpr int kMergeGroups = 32;
+
 __global__ __launch_bounds__(256) void k_rdf_merge(const uint32_t* __restrict__ partial, int nslices,
                                                    int nbins, int64_t* __restrict__ counts) {
   const int b = blockIdx.x * 256 + threadIdx.x;
   if (b >= nbins) return;
+  const int g = blockIdx.y;
+  const int s0 = static_cast<int>(static_cast<int64_t>(nslices) * g / kMergeGroups);
+  const int s1 = static_cast<int>(static_cast<int64_t>(nslices) * (g + 1) / kMergeGroups);
   uint64_t s = 0;
-  for (int sl = 0; sl < nslices; ++sl) s += partial[static_cast<int64_t>(sl) * nbins + b];
-  counts[b] += static_cast<int64_t>(s);
+  for (int sl = s0; sl < s1; ++sl) s += partial[static_cast<int64_t>(sl) * nbins + b];
+  if (s) atomicAdd(reinterpret_cast<unsigned long long*>(&counts[b]), static_cast<unsigned long long>(s));
 }
 
 // ------------------------------------------------------------------ S(Q)
@@ -791,20 +798,23 @@ __global__ __launch_bounds__(256) void k_sq_shells(const double* __restrict__ rh
   }
 }
 
+// Explicit q-vectors: block (v, b) sums the phases of the atoms of every
+// species in its 1/nb share; partial sums are combined in a fixed order by
+// k_sq_list_finish (deterministic), which then bins |rho|^2 into the shells.
+constexpr int kListBlocks = 64;
+
 __global__ __launch_bounds__(256) void k_sq_list(const double* __restrict__ u,
                                                  const int32_t* __restrict__ sp_off, int64_t n, int nsp,
                                                  int ndim, const int32_t* __restrict__ kv,
-                                                 const int32_t* __restrict__ shell, Bsc bsc, int nqmax,
-                                                 double* __restrict__ sqf, double* __restrict__ sqfp) {
+                                                 double* __restrict__ part) {
   __shared__ double red[2][256];
-  const int v = blockIdx.x;
+  const int v = blockIdx.x, b = blockIdx.y;
   const int kx = kv[3 * v], ky = kv[3 * v + 1], kz = kv[3 * v + 2];
-  const int sh = shell[v];
-  const int ns = nqmax + 1;
-  double tr = 0.0, ti = 0.0;
   for (int s = 0; s < nsp; ++s) {
+    const int64_t a0 = sp_off[s], a1 = sp_off[s + 1], len = a1 - a0;
+    const int64_t j0 = a0 + len * b / kListBlocks, j1 = a0 + len * (b + 1) / kListBlocks;
     double pr = 0.0, pi = 0.0;
-    for (int j = sp_off[s] + threadIdx.x; j < sp_off[s + 1]; j += 256) {
+    for (int64_t j = j0 + threadIdx.x; j < j1; j += 256) {
       double f = frac_turns(kx, u[j], u[3 * n + j]) + frac_turns(ky, u[n + j], u[4 * n + j]);
       if (ndim == 3) f += frac_turns(kz, u[2 * n + j], u[5 * n + j]);
       f -= rint(f);
@@ -824,14 +834,33 @@ __global__ __launch_bounds__(256) void k_sq_list(const double* __restrict__ u,
       __syncthreads();
     }
     if (threadIdx.x == 0) {
-      const double rr = red[0][0], ri = red[1][0];
-      atomicAdd(&sqfp[static_cast<int64_t>(s) * ns + sh], rr * rr + ri * ri);
-      tr += bsc.b[s] * rr;
-      ti += bsc.b[s] * ri;
+      double* o = part + ((static_cast<int64_t>(v) * kListBlocks + b) * nsp + s) * 2;
+      o[0] = red[0][0];
+      o[1] = red[1][0];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicAdd(&sqf[sh], tr * tr + ti * ti);
+}
+
+__global__ void k_sq_list_finish(const double* __restrict__ part, const int32_t* __restrict__ shell, int nvec,
+                                 int nsp, Bsc bsc, int nqmax, double* __restrict__ sqf,
+                                 double* __restrict__ sqfp) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nvec) return;
+  const int sh = shell[v], ns = nqmax + 1;
+  double tr = 0.0, ti = 0.0;
+  for (int s = 0; s < nsp; ++s) {
+    double rr = 0.0, ri = 0.0;
+    for (int b = 0; b < kListBlocks; ++b) {
+      const double* o = part + ((static_cast<int64_t>(v) * kListBlocks + b) * nsp + s) * 2;
+      rr += o[0];
+      ri += o[1];
+    }
+    atomicAdd(&sqfp[static_cast<int64_t>(s) * ns + sh], rr * rr + ri * ri);
+    tr += bsc.b[s] * rr;
+    ti += bsc.b[s] * ri;
+  }
+  atomicAdd(&sqf[sh], tr * tr + ti * ti);
 }
 
 inline unsigned grid1d(int64_t n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
@@ -915,8 +944,8 @@ void launch_rdf_allpairs(const PairArgs& a, const double* pos, const int32_t* it
 }
 
 void launch_rdf_merge(const uint32_t* partial, int nslices, int nbins, int64_t* counts, hipStream_t s) {
-  hipLaunchKernelGGL(k_rdf_merge, dim3(grid1d(nbins, 256)), dim3(256), 0, s, partial, nslices, nbins,
-                     counts);
+  hipLaunchKernelGGL(k_rdf_merge, dim3(grid1d(nbins, 256), kMergeGroups), dim3(256), 0, s, partial, nslices,
+                     nbins, counts);
 }
 
 void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s) {
@@ -949,13 +978,18 @@ void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, 
                      tile_shell, ntiles, tile_vec, ksplit, nsp, b, nqmax, sqf, sqfp, use_lds);
 }
 
+size_t sq_list_scratch_bytes(int nvec, int nsp) {
+  return sizeof(double) * 2 * static_cast<size_t>(nvec) * kListBlocks * nsp;
+}
+
 void launch_sq_list(const double* u, const int32_t* sp_off, int64_t n, int nsp, int ndim, const int32_t* kv,
-                    const int32_t* shell, int nvec, const double* bsc, int nqmax, double* sqf, double* sqfp,
-                    hipStream_t s) {
+                    const int32_t* shell, int nvec, const double* bsc, int nqmax, double* part, double* sqf,
+                    double* sqfp, hipStream_t s) {
   if (nvec <= 0) return;
   Bsc b{};
   for (int i = 0; i < nsp; ++i) b.b[i] = bsc[i];
-  hipLaunchKernelGGL(k_sq_list, dim3(nvec), dim3(256), 0, s, u, sp_off, n, nsp, ndim, kv, shell, b, nqmax,
+  hipLaunchKernelGGL(k_sq_list, dim3(nvec, kListBlocks), dim3(256), 0, s, u, sp_off, n, nsp, ndim, kv, part);
+  hipLaunchKernelGGL(k_sq_list_finish, dim3((nvec + 63) / 64), dim3(64), 0, s, part, shell, nvec, nsp, b, nqmax,
                      sqf, sqfp);
 }
 

@@ -40,7 +40,7 @@ struct trj_ctx {
   std::string err;
   // device buffers
   DevBuf raw, itype, pos, spos, stype, cellid, cellcount, cellstart, cursor, partial, counts, thr, blk,
-      blkoff, sp_off, u, rho, tiles, tile_shell, sqf, sqfp, axis_kv, axis_shell, flags;
+      blkoff, sp_off, u, rho, tiles, tile_shell, sqf, sqfp, axis_kv, axis_shell, listpart, flags;
   // pinned staging for host inputs (double-buffered)
   void* pinned[2] = {nullptr, nullptr};
   size_t pinned_cap[2] = {0, 0};
@@ -337,9 +337,11 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
       launch_sq_shells(P<double>(c->rho), P<int16_t>(c->tile_shell), ntiles, S.tk.vec(), c->ksplit, nsp, S.p.bsc,
                        S.nqmax,
                        P<double>(c->sqf), P<double>(c->sqfp), c->stream);
+    const int nax = static_cast<int>(S.axis_shell.size());
+    if ((r = ensure(c, c->listpart, sq_list_scratch_bytes(nax, nsp)))) return r;
     launch_sq_list(P<double>(c->u), P<int32_t>(c->sp_off), n, nsp, ndim, P<int32_t>(c->axis_kv),
-                   P<int32_t>(c->axis_shell), static_cast<int>(S.axis_shell.size()), S.p.bsc, S.nqmax,
-                   P<double>(c->sqf), P<double>(c->sqfp), c->stream);
+                   P<int32_t>(c->axis_shell), nax, S.p.bsc, S.nqmax, P<double>(c->listpart), P<double>(c->sqf),
+                   P<double>(c->sqfp), c->stream);
     HIPC(c, hipGetLastError());
     c->sq_frames += 1;
   } else {
@@ -411,7 +413,8 @@ void trj_destroy(trj_ctx* c) {
   DevBuf* bufs[] = {&c->raw,  &c->itype,     &c->pos,       &c->spos,   &c->stype,  &c->cellid,
                     &c->cellcount, &c->cellstart, &c->cursor, &c->partial, &c->counts, &c->thr,
                     &c->blk,  &c->blkoff,    &c->sp_off,    &c->u,      &c->rho,    &c->tiles,
-                    &c->tile_shell, &c->sqf, &c->sqfp,      &c->axis_kv, &c->axis_shell, &c->flags};
+                    &c->tile_shell, &c->sqf, &c->sqfp,      &c->axis_kv, &c->axis_shell, &c->listpart,
+                    &c->flags};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   for (int k = 0; k < 2; ++k) {
@@ -809,7 +812,8 @@ int64_t trj_device_bytes(const trj_ctx* c) {
   const DevBuf* bufs[] = {&c->raw,  &c->itype,     &c->pos,       &c->spos,   &c->stype,  &c->cellid,
                           &c->cellcount, &c->cellstart, &c->cursor, &c->partial, &c->counts, &c->thr,
                           &c->blk,  &c->blkoff,    &c->sp_off,    &c->u,      &c->rho,    &c->tiles,
-                          &c->tile_shell, &c->sqf, &c->sqfp,      &c->axis_kv, &c->axis_shell, &c->flags};
+                          &c->tile_shell, &c->sqf, &c->sqfp,      &c->axis_kv, &c->axis_shell, &c->listpart,
+                          &c->flags};
   int64_t s = 0;
   for (const DevBuf* b : bufs) s += static_cast<int64_t>(b->cap);
   return s;
