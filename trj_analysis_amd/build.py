@@ -44,5 +44,38 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FORTRAN = ["trj_hip.f90", "trj_hip_modules.f90"]
+FPROGRAMS = ["example_driver.f90", "abi_check.f90"]
+FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran(verbose: bool = False) -> list:
+    """ISO_C_BINDING modules and the two Fortran programs (amdflang), linked
+    against libtrjhip.so with an rpath relative to the executable."""
+    fdir = os.path.join(HERE, "fortran")
+    objdir = os.path.join(HERE, "_build", "fortran")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src in FORTRAN:
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        cmd = [FLANG, "-O2", "-fPIC", "-J", objdir, "-c", os.path.join(fdir, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    out = []
+    for prog in FPROGRAMS:
+        exe = os.path.join(fdir, os.path.splitext(prog)[0])
+        cmd = [FLANG, "-O2", "-J", objdir, os.path.join(fdir, prog)] + objs + \
+            ["-L" + HERE, "-ltrjhip", "-Wl,-rpath,$ORIGIN/..", "-o", exe + ".tmp"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_fortran(verbose="-v" in sys.argv))

@@ -1,0 +1,157 @@
+! trj_hip.f90 -- ISO_C_BINDING interface to the trj_hip C ABI (include/trj_hip.h).
+!
+! The derived types mirror the C structs field for field (bind(C) gives them
+! the C layout); every function returns TRJ_OK (0) or a negative TRJ_E* code.
+module trj_hip
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: TRJ_OK = 0
+  integer(c_int32_t), parameter :: TRJ_F32 = 0, TRJ_F64 = 1
+  integer(c_int32_t), parameter :: TRJ_AOS = 0, TRJ_SOA = 1
+  integer(c_int32_t), parameter :: TRJ_WANT_RDF = 1, TRJ_WANT_SQ = 2
+  integer(c_int32_t), parameter :: TRJ_QSET_REF = 0, TRJ_QSET_HALFSPACE = 1
+  integer(c_int32_t), parameter :: TRJ_PREC_FP64 = 0, TRJ_PREC_FP32 = 1, &
+                                   TRJ_PREC_FP16X3 = 2, TRJ_PREC_BF16X3 = 3
+  integer(c_int32_t), parameter :: TRJ_SQ_NQ_CONSISTENT = 1
+
+  type, bind(C) :: trj_rdf_params
+    integer(c_int32_t) :: ndim, nsp
+    real(c_double) :: deltar, rcrdf
+    integer(c_int32_t) :: periodic(3)
+  end type
+
+  type, bind(C) :: trj_sq_params
+    integer(c_int32_t) :: ndim, nsp
+    real(c_double) :: qmax, qmin
+    real(c_double) :: bsc(10)
+    integer(c_int32_t) :: qset
+    integer(c_int32_t) :: grid(3)
+    integer(c_int32_t) :: precision
+    integer(c_int32_t) :: flags
+  end type
+
+  type, bind(C) :: trj_sq_info
+    real(c_double) :: dq, fk(3), qmin_eff
+    integer(c_int32_t) :: nqmax, nqmin, lmax(3)
+    integer(c_int64_t) :: nvec_grid, nvec_axis, box_ext(3)
+  end type
+
+  interface
+    integer(c_int) function trj_abi_version() bind(C, name="trj_abi_version")
+      import :: c_int
+    end function
+    integer(c_int) function trj_create(ctx, device) bind(C, name="trj_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int), value :: device
+    end function
+    subroutine trj_destroy(ctx) bind(C, name="trj_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine
+    type(c_ptr) function trj_last_error(ctx) bind(C, name="trj_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function trj_rdf_init(ctx, p, box, lsmax, nit) bind(C, name="trj_rdf_init")
+      import :: c_int, c_ptr, c_double, c_int32_t, trj_rdf_params
+      type(c_ptr), value :: ctx
+      type(trj_rdf_params), intent(in) :: p
+      real(c_double), intent(in) :: box(3)
+      integer(c_int32_t), intent(out) :: lsmax, nit
+    end function
+    integer(c_int) function trj_sq_init(ctx, p, box, info) bind(C, name="trj_sq_init")
+      import :: c_int, c_ptr, c_double, trj_sq_params, trj_sq_info
+      type(c_ptr), value :: ctx
+      type(trj_sq_params), intent(in) :: p
+      real(c_double), intent(in) :: box(3)
+      type(trj_sq_info), intent(out) :: info
+    end function
+    integer(c_int) function trj_frame(ctx, pos, dtype, layout, itype, n, box, what) bind(C, name="trj_frame")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: pos
+      integer(c_int32_t), value :: dtype, layout
+      integer(c_int32_t), intent(in) :: itype(*)
+      integer(c_int64_t), value :: n
+      real(c_double), intent(in) :: box(3)
+      integer(c_int32_t), value :: what
+    end function
+    integer(c_int) function trj_sync(ctx) bind(C, name="trj_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function trj_rdf_counts(ctx, counts) bind(C, name="trj_rdf_counts")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: counts(*)
+    end function
+    integer(c_int) function trj_rdf_gmix(ctx, nconf, out) bind(C, name="trj_rdf_gmix")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nconf
+      real(c_double), intent(out) :: out(*)
+    end function
+    integer(c_int) function trj_rdf_first_minimum(ctx, rmin) bind(C, name="trj_rdf_first_minimum")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: rmin
+    end function
+    integer(c_int) function trj_sq_shells(ctx, sqf, sqfp, nq) bind(C, name="trj_sq_shells")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: sqf(*), sqfp(*)
+      integer(c_int64_t), intent(out) :: nq(*)
+    end function
+    integer(c_int) function trj_sq_columns(ctx, nconf, cols, ncols, mix) bind(C, name="trj_sq_columns")
+      import :: c_int, c_ptr, c_int64_t, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nconf
+      real(c_double), intent(out) :: cols(*)
+      integer(c_int32_t), intent(out) :: ncols
+      real(c_double), intent(out) :: mix(*)
+    end function
+    integer(c_int) function trj_sq_plan(p, box, info, nq) bind(C, name="trj_sq_plan")
+      import :: c_int, c_double, c_int64_t, trj_sq_params, trj_sq_info
+      type(trj_sq_params), intent(in) :: p
+      real(c_double), intent(in) :: box(3)
+      type(trj_sq_info), intent(out) :: info
+      integer(c_int64_t), intent(out) :: nq(*)
+    end function
+    integer(c_int) function trj_rdf_plan(p, box, lsmax, rc_eff) bind(C, name="trj_rdf_plan")
+      import :: c_int, c_double, c_int32_t, trj_rdf_params
+      type(trj_rdf_params), intent(in) :: p
+      real(c_double), intent(in) :: box(3)
+      integer(c_int32_t), intent(out) :: lsmax
+      real(c_double), intent(out) :: rc_eff
+    end function
+    integer(c_int) function trj_reset(ctx) bind(C, name="trj_reset")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+  end interface
+
+contains
+
+  ! Print the context's last error and stop, as the reference does on a failed
+  ! kernel (printcudaerror + stop, rdf.cuf:241-242).
+  subroutine trj_check(ctx, rc, what)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    integer :: k
+    if (rc == TRJ_OK) return
+    write (*, '(" *** trj_hip error ",i0," in ",a,": ")', advance='no') rc, what
+    if (c_associated(ctx)) then
+      call c_f_pointer(trj_last_error(ctx), msg, [1024])
+      do k = 1, 1024
+        if (msg(k) == c_null_char) exit
+        write (*, '(a)', advance='no') msg(k)
+      end do
+    end if
+    write (*, *)
+    stop 1
+  end subroutine
+end module trj_hip
