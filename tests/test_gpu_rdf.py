@@ -142,3 +142,16 @@ def test_deterministic_repeat():
             ctx.frame(f.pos, f.itype, f.box, what=T.TRJ_WANT_RDF)
             res.append(ctx.rdf_counts())
     np.testing.assert_array_equal(res[0], res[1])
+
+
+@pytest.mark.parametrize("L,rc,n", [(30.0, 5.0, 20000), (26.0, 5.0, 12000), (52.0, 4.0, 60000)])
+def test_cell_shift_path_bit_exact(L, rc, n):
+    """Boxes with >= 5 cells per axis take the neighbour-cell image shift instead
+    of the per-pair rule; unwrapped input, three species."""
+    rng = np.random.default_rng(int(L * 10))
+    pos = rng.random((n, 3)) * L * 3.0 - L  # spans three periods
+    it = rng.integers(1, 4, n).astype(np.int32)
+    print("case: cell shift", L, rc, n, flush=True)
+    g, _ = _gpu_counts([(pos, it, [L] * 3)], 3, 3, 0.01, rc)
+    o = _oracle_counts([(pos, it, [L] * 3)], 3, 3, 0.01, rc, method="cells")
+    np.testing.assert_array_equal(g, o)

@@ -50,6 +50,10 @@ void launch_cell_scatter(const double* pos, const int32_t* itype, const int32_t*
 
 struct PairArgs {
   int ndim, nsp, nit, lsmax;
+  // Axis k takes the image shift of the neighbour cell (no per-pair test):
+  // periodic with >= 5 cells, where every pair between two cells has the same
+  // nearest image and the FP64 rule d - L*nint(d/L) picks exactly that shift.
+  int cellshift[3];
   double lmi[3], half[3];
   double side2;
   float inv_dr;

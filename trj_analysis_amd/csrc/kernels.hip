@@ -197,13 +197,14 @@ __device__ inline int pair_index(int a, int b, int nsp) {
 }
 
 // nint(sqrt(r2)/deltar) through the exact edges thr[0..lsmax+2]
-// (thr[0] = 0, thr[lsmax+2] = +inf).
+// (thr[0] = 0, thr[lsmax+2] = +inf).  The float estimate is within one bin of
+// the exact one (its relative error is ~1e-7 and r/deltar < 2^15), so one
+// comparison against each neighbouring edge corrects it.
 __device__ inline int bin_of(double r2, const double* thr, float inv_dr, int lsmax) {
   int k = static_cast<int>(sqrtf(static_cast<float>(r2)) * inv_dr + 0.5f);
   k = k > lsmax + 1 ? lsmax + 1 : k;
-  while (k <= lsmax && r2 >= thr[k + 1]) ++k;
-  while (k > 0 && r2 < thr[k]) --k;
-  return k;
+  const double lo = thr[k], hi = thr[k + 1];
+  return k + (r2 >= hi ? 1 : 0) - (r2 < lo ? 1 : 0);
 }
 
 __device__ inline double min_image(double d, double L, double h) {
@@ -214,6 +215,18 @@ struct Stencil {
   int n;
   int off[14][3];
 };
+
+__device__ inline void bin_pair(const PairArgs& a, const double* thr, uint32_t* hist, bool ok, double r2,
+                                int ti, int tj) {
+  if (ok && r2 < a.side2) {
+    const int k = bin_of(r2, thr, a.inv_dr, a.lsmax);
+    if (k >= 1 && k <= a.lsmax) {
+      const int lo = ti < tj ? ti : tj, hi = ti < tj ? tj : ti;
+      const int ij = pair_index(lo, hi, a.nsp);
+      atomicAdd(&hist[(ij - 1) * a.lsmax + (k - 1)], 1u);
+    }
+  }
+}
 
 __device__ inline void pair_accumulate(const PairArgs& a, const double* thr, uint32_t* hist,
                                        bool ok, double xi, double yi, double zi, int ti, double xj,
@@ -251,6 +264,16 @@ __device__ inline void hist_flush(const PairArgs& a, const uint32_t* hist, uint3
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) out[b] = hist[b];
 }
 
+// Broadcast lane l's value to the whole wave (result is wave-uniform, held in
+// scalar registers).
+__device__ inline double readlane_d(double v, int l) {
+  const unsigned long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffull), l);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+template <bool SHIFT>
 __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stencil st,
                                                    const int32_t* __restrict__ start,
                                                    const double* __restrict__ spos,
@@ -283,19 +306,52 @@ __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stenc
       const bool oki = vi && static_cast<unsigned>(ti - 1) < static_cast<unsigned>(a.nsp);
       for (int o = 0; o < st.n; ++o) {
         int nx = cx + st.off[o][0], ny = cy + st.off[o][1], nz = cz + st.off[o][2];
+        // Image shift of a wrapped neighbour, the L*nint(d/L) the FP64 rule
+        // subtracts: a neighbour below cell 0 is the top cell (d ~ +L, subtract
+        // +L); one above the top cell is cell 0 (d ~ -L, subtract -L).
+        const double shx = nx < 0 ? a.lmi[0] : (nx >= ncx ? -a.lmi[0] : 0.0);
+        const double shy = ny < 0 ? a.lmi[1] : (ny >= ncy ? -a.lmi[1] : 0.0);
+        const double shz = nz < 0 ? a.lmi[2] : (nz >= ncz ? -a.lmi[2] : 0.0);
         if (g.periodic[0]) nx = (nx + ncx) % ncx; else if (nx < 0 || nx >= ncx) continue;
         if (g.periodic[1]) ny = (ny + ncy) % ncy; else if (ny < 0 || ny >= ncy) continue;
         if (g.periodic[2]) nz = (nz + ncz) % ncz; else if (nz < 0 || nz >= ncz) continue;
         const int nb = (nz * ncy + ny) * ncx + nx;
-        int js = start[nb];
-        const int je = start[nb + 1];
         const bool self = (o == 0);
-        if (self) js = cs + ch * kWave + 1;
-        for (int j = js; j < je; ++j) {
-          const int tj = stype[j];
-          const bool ok = oki && (!self || j > gi) &&
-                          static_cast<unsigned>(tj - 1) < static_cast<unsigned>(a.nsp);
-          pair_accumulate(a, thr, hist, ok, xi, yi, zi, ti, sx[j], sy[j], sz[j], tj);
+        const int js = self ? cs + ch * kWave + 1 : start[nb];
+        const int je = start[nb + 1];
+        // 64 neighbours per pass: one coalesced load per lane, then a
+        // broadcast per pair through the scalar registers.
+        for (int jb = js; jb < je; jb += kWave) {
+          const int jl = jb + lane;
+          const bool vl = jl < je;
+          const double xl = vl ? sx[jl] : 0.0;
+          const double yl = vl ? sy[jl] : 0.0;
+          const double zl = vl ? sz[jl] : 0.0;
+          const int tl = vl ? stype[jl] : 0;
+          const int cnt = (je - jb) < kWave ? (je - jb) : kWave;
+          for (int jj = 0; jj < cnt; ++jj) {
+            const int j = jb + jj;
+            const int tj = __builtin_amdgcn_readlane(tl, jj);
+            const bool ok = oki && (!self || j > gi) &&
+                            static_cast<unsigned>(tj - 1) < static_cast<unsigned>(a.nsp);
+            if constexpr (SHIFT) {
+              // d - L*nint(d/L) == d - s for every pair of these two cells
+              // (s = -shift, exact: L*1 = L); no per-pair test.
+              const double dx = (readlane_d(xl, jj) - xi) - shx;
+              const double dy = (readlane_d(yl, jj) - yi) - shy;
+              double r2;
+              if (a.ndim == 3) {
+                const double dz = (readlane_d(zl, jj) - zi) - shz;
+                r2 = dx * dx + dy * dy + dz * dz;
+              } else {
+                r2 = dx * dx + dy * dy;
+              }
+              bin_pair(a, thr, hist, ok, r2, ti, tj);
+            } else {
+              pair_accumulate(a, thr, hist, ok, xi, yi, zi, ti, readlane_d(xl, jj), readlane_d(yl, jj),
+                              readlane_d(zl, jj), tj);
+            }
+          }
         }
       }
     }
@@ -505,7 +561,6 @@ constexpr int LX = 64, LY = 4, LZ = 32, LN = LY * LZ;
 constexpr int LKC = 32;           // atoms per chunk (K of one 16x16x32 MFMA)
 constexpr int LKP = 40;           // padded row length of the 16-bit operand tables
 constexpr int LKF = LKC + 1;      // padded row length of the FP32 operand tables
-constexpr int kLpFoldChunks = 8;  // fold FP32 accumulators into FP64 every 256 atoms
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
@@ -566,32 +621,41 @@ struct LpShared {
   float2 ez[LKC][LZ];
 };
 
+// Store 8 consecutive-atom values of one operand row: FP32 as two float4, the
+// 16-bit forms as hi and lo vectors of 8 (one 16-byte store each).
 template <int PREC>
-__device__ inline void lp_store(LpShared<PREC>& sh, bool isA, int row, int a, float re, float im) {
+__device__ inline void lp_store8(LpShared<PREC>& sh, bool isA, int row, int a0, const float* re,
+                                 const float* im) {
   if constexpr (PREC == TRJ_PREC_FP32) {
-    if (isA) {
-      sh.A[0][row][a] = re;
-      sh.A[1][row][a] = im;
-    } else {
-      sh.B[0][row][a] = re;
-      sh.B[1][row][a] = im;
+    for (int k = 0; k < 8; ++k) {
+      if (isA) {
+        sh.A[0][row][a0 + k] = re[k];
+        sh.A[1][row][a0 + k] = im[k];
+      } else {
+        sh.B[0][row][a0 + k] = re[k];
+        sh.B[1][row][a0 + k] = im[k];
+      }
     }
   } else {
-    typedef typename LpTypes<PREC>::T T;
-    T rh, rl, ih, il;
-    split2(re, rh, rl);
-    split2(im, ih, il);
-    if (isA) {
-      sh.A[0][row][a] = rh;
-      sh.A[1][row][a] = rl;
-      sh.A[2][row][a] = ih;
-      sh.A[3][row][a] = il;
-    } else {
-      sh.B[0][row][a] = rh;
-      sh.B[1][row][a] = rl;
-      sh.B[2][row][a] = ih;
-      sh.B[3][row][a] = il;
+    typedef LpTypes<PREC> LT;
+    typedef typename LT::T T;
+    typedef typename LT::V V;
+    V rh, rl, ih, il;
+    for (int k = 0; k < 8; ++k) {
+      T h, l;
+      split2(re[k], h, l);
+      rh[k] = h;
+      rl[k] = l;
+      split2(im[k], h, l);
+      ih[k] = h;
+      il[k] = l;
     }
+    T* base = isA ? &sh.A[0][row][a0] : &sh.B[0][row][a0];
+    const size_t plane = isA ? sizeof(sh.A[0]) / sizeof(T) : sizeof(sh.B[0]) / sizeof(T);
+    *reinterpret_cast<V*>(base) = rh;
+    *reinterpret_cast<V*>(base + plane) = rl;
+    *reinterpret_cast<V*>(base + 2 * plane) = ih;
+    *reinterpret_cast<V*>(base + 3 * plane) = il;
   }
 }
 
@@ -614,29 +678,48 @@ __global__ __launch_bounds__(256) void k_sq_contract_lp(ContractArgs a) {
     const int64_t len = a1 - a0;
     const int b0 = a0 + static_cast<int>(len * split / a.ksplit);
     const int b1 = a0 + static_cast<int>(len * (split + 1) / a.ksplit);
+    // FP32 accumulation over this workgroup's atoms (at most 8192 per species,
+    // set by the host through ksplit); the K-splits are summed in FP64.
     f4 accR[8], accI[8];
-    d4 sumR[8], sumI[8];
     for (int nb = 0; nb < 8; ++nb) {
       accR[nb] = f4{0.f, 0.f, 0.f, 0.f};
       accI[nb] = accR[nb];
-      sumR[nb] = d4{0.0, 0.0, 0.0, 0.0};
-      sumI[nb] = sumR[nb];
     }
-    int chunks = 0;
     for (int c0 = b0; c0 < b1; c0 += LKC) {
       __syncthreads();
-      {  // ex: 8 segments of 8 kx per atom, one per thread
-        const int at = t >> 3, seg = t & 7, ai = c0 + at;
-        float re[8], im[8];
-        if (ai < b1) {
-          phase_f32(ux[ai], lx[ai], kx0 + 8 * seg, 8, re, im);
-        } else {
-          for (int k = 0; k < 8; ++k) re[k] = im[k] = 0.f;
+      if (t < 64) {
+        // ex: 8 kx x 4 atoms per thread (8 segments x 8 atom quads).
+        const int seg = t & 7, aq = (t >> 3) * 4;
+        float re[4][8], im[4][8];
+        for (int q = 0; q < 4; ++q) {
+          const int ai = c0 + aq + q;
+          if (ai < b1) {
+            phase_f32(ux[ai], lx[ai], kx0 + 8 * seg, 8, re[q], im[q]);
+          } else {
+            for (int k = 0; k < 8; ++k) re[q][k] = im[q][k] = 0.f;
+          }
         }
-        for (int k = 0; k < 8; ++k) lp_store<PREC>(sh, true, 8 * seg + k, at, re[k], im[k]);
-      }
-      if (t < LKC * 5) {  // ey (4 values) and ez (4 segments of 8) per atom
-        const int at = t / 5, sub = t % 5, ai = c0 + at;
+        for (int k = 0; k < 8; ++k) {
+          const int row = 8 * seg + k;
+          for (int q = 0; q < 4; ++q) {
+            if constexpr (PREC == TRJ_PREC_FP32) {
+              sh.A[0][row][aq + q] = re[q][k];
+              sh.A[1][row][aq + q] = im[q][k];
+            } else {
+              typedef typename LpTypes<PREC>::T T;
+              T h, l;
+              split2(re[q][k], h, l);
+              sh.A[0][row][aq + q] = h;
+              sh.A[1][row][aq + q] = l;
+              split2(im[q][k], h, l);
+              sh.A[2][row][aq + q] = h;
+              sh.A[3][row][aq + q] = l;
+            }
+          }
+        }
+      } else if (t < 64 + LKC * 5) {
+        // ey (4 values) and ez (4 segments of 8) per atom.
+        const int tt = t - 64, at = tt / 5, sub = tt % 5, ai = c0 + at;
         float re[8], im[8];
         const int cnt = (sub == 0) ? LY : 8;
         if (ai < b1) {
@@ -655,12 +738,17 @@ __global__ __launch_bounds__(256) void k_sq_contract_lp(ContractArgs a) {
         }
       }
       __syncthreads();
-      {  // B = ey*ez: 128 (ky,kz) x 32 atoms, 16 per thread
+      {  // B = ey*ez: one (ky,kz) row x 16 atoms per thread, stored 8 atoms at a time.
         const int n = t & (LN - 1), ab = (t >> 7) * 16;
         const int ky = n / LZ, kz = n % LZ;
-        for (int k = 0; k < 16; ++k) {
-          const float2 y = sh.ey[ab + k][ky], z = sh.ez[ab + k][kz];
-          lp_store<PREC>(sh, false, n, ab + k, fmaf(y.x, z.x, -(y.y * z.y)), fmaf(y.x, z.y, y.y * z.x));
+        for (int h = 0; h < 2; ++h) {
+          float re[8], im[8];
+          for (int k = 0; k < 8; ++k) {
+            const float2 y = sh.ey[ab + 8 * h + k][ky], z = sh.ez[ab + 8 * h + k][kz];
+            re[k] = fmaf(y.x, z.x, -(y.y * z.y));
+            im[k] = fmaf(y.x, z.y, y.y * z.x);
+          }
+          lp_store8<PREC>(sh, false, n, ab + 8 * h, re, im);
         }
       }
       __syncthreads();
@@ -674,8 +762,8 @@ __global__ __launch_bounds__(256) void k_sq_contract_lp(ContractArgs a) {
           for (int nb = 0; nb < 8; ++nb) {
             const float br = sh.B[0][16 * nb + i16][at], bi = sh.B[1][16 * nb + i16][at];
             accR[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, accR[nb], 0, 0, 0);
-            accR[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, bi, accR[nb], 0, 0, 0);
             accI[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, accI[nb], 0, 0, 0);
+            accR[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, bi, accR[nb], 0, 0, 0);
             accI[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, accI[nb], 0, 0, 0);
           }
         }
@@ -688,39 +776,45 @@ __global__ __launch_bounds__(256) void k_sq_contract_lp(ContractArgs a) {
         const V ail = *reinterpret_cast<const V*>(&sh.A[3][row][8 * kq]);
         const V naih = -aih, nail = -ail;
 #pragma unroll
-        for (int nb = 0; nb < 8; ++nb) {
-          const int col = 16 * nb + i16;
-          const V brh = *reinterpret_cast<const V*>(&sh.B[0][col][8 * kq]);
-          const V brl = *reinterpret_cast<const V*>(&sh.B[1][col][8 * kq]);
-          const V bih = *reinterpret_cast<const V*>(&sh.B[2][col][8 * kq]);
-          const V bil = *reinterpret_cast<const V*>(&sh.B[3][col][8 * kq]);
-          f4 r = accR[nb], i = accI[nb];
-          r = LT::mfma(arh, brh, r);
-          r = LT::mfma(arh, brl, r);
-          r = LT::mfma(arl, brh, r);
-          r = LT::mfma(naih, bih, r);
-          r = LT::mfma(naih, bil, r);
-          r = LT::mfma(nail, bih, r);
-          i = LT::mfma(arh, bih, i);
-          i = LT::mfma(arh, bil, i);
-          i = LT::mfma(arl, bih, i);
-          i = LT::mfma(aih, brh, i);
-          i = LT::mfma(aih, brl, i);
-          i = LT::mfma(ail, brh, i);
-          accR[nb] = r;
-          accI[nb] = i;
-        }
-      }
-      if (++chunks == kLpFoldChunks) {
-        chunks = 0;
-#pragma unroll
-        for (int nb = 0; nb < 8; ++nb) {
-          for (int r = 0; r < 4; ++r) {
-            sumR[nb][r] += static_cast<double>(accR[nb][r]);
-            sumI[nb][r] += static_cast<double>(accI[nb][r]);
+        for (int nb = 0; nb < 8; nb += 2) {
+          V brh[2], brl[2], bih[2], bil[2];
+          for (int e = 0; e < 2; ++e) {
+            const int col = 16 * (nb + e) + i16;
+            brh[e] = *reinterpret_cast<const V*>(&sh.B[0][col][8 * kq]);
+            brl[e] = *reinterpret_cast<const V*>(&sh.B[1][col][8 * kq]);
+            bih[e] = *reinterpret_cast<const V*>(&sh.B[2][col][8 * kq]);
+            bil[e] = *reinterpret_cast<const V*>(&sh.B[3][col][8 * kq]);
           }
-          accR[nb] = f4{0.f, 0.f, 0.f, 0.f};
-          accI[nb] = accR[nb];
+          // four independent accumulators, interleaved
+          f4 r0 = accR[nb], i0 = accI[nb], r1 = accR[nb + 1], i1 = accI[nb + 1];
+          r0 = LT::mfma(arh, brh[0], r0);
+          i0 = LT::mfma(arh, bih[0], i0);
+          r1 = LT::mfma(arh, brh[1], r1);
+          i1 = LT::mfma(arh, bih[1], i1);
+          r0 = LT::mfma(arh, brl[0], r0);
+          i0 = LT::mfma(arh, bil[0], i0);
+          r1 = LT::mfma(arh, brl[1], r1);
+          i1 = LT::mfma(arh, bil[1], i1);
+          r0 = LT::mfma(arl, brh[0], r0);
+          i0 = LT::mfma(arl, bih[0], i0);
+          r1 = LT::mfma(arl, brh[1], r1);
+          i1 = LT::mfma(arl, bih[1], i1);
+          r0 = LT::mfma(naih, bih[0], r0);
+          i0 = LT::mfma(aih, brh[0], i0);
+          r1 = LT::mfma(naih, bih[1], r1);
+          i1 = LT::mfma(aih, brh[1], i1);
+          r0 = LT::mfma(naih, bil[0], r0);
+          i0 = LT::mfma(aih, brl[0], i0);
+          r1 = LT::mfma(naih, bil[1], r1);
+          i1 = LT::mfma(aih, brl[1], i1);
+          r0 = LT::mfma(nail, bih[0], r0);
+          i0 = LT::mfma(ail, brh[0], i0);
+          r1 = LT::mfma(nail, bih[1], r1);
+          i1 = LT::mfma(ail, brh[1], i1);
+          accR[nb] = r0;
+          accI[nb] = i0;
+          accR[nb + 1] = r1;
+          accI[nb + 1] = i1;
         }
       }
     }
@@ -732,9 +826,8 @@ __global__ __launch_bounds__(256) void k_sq_contract_lp(ContractArgs a) {
         const int kxl = 16 * w + 4 * kq + r;
         const int n = 16 * nb + i16;
         const int v = (kxl * LY + n / LZ) * LZ + (n % LZ);
-        const double re = sumR[nb][r] + static_cast<double>(accR[nb][r]);
-        const double im = sumI[nb][r] + static_cast<double>(accI[nb][r]);
-        reinterpret_cast<double2*>(a.rho)[base + v] = make_double2(re, im);
+        reinterpret_cast<double2*>(a.rho)[base + v] =
+            make_double2(static_cast<double>(accR[nb][r]), static_cast<double>(accI[nb][r]));
       }
     }
   }
@@ -932,8 +1025,14 @@ void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start
   st.n = k;
   const int64_t ncell = static_cast<int64_t>(g.nc[0]) * g.nc[1] * g.nc[2];
   const int64_t ntasks = ncell * waves_per_cell;
-  hipLaunchKernelGGL(k_rdf_cells, dim3(nblocks), dim3(256), pair_lds_bytes(a.nit, a.lsmax), s, a, g, st,
-                     start, spos, stype, n, waves_per_cell, ntasks, partial);
+  bool shift = true;
+  for (int k = 0; k < a.ndim; ++k) shift = shift && a.cellshift[k];
+  if (shift)
+    hipLaunchKernelGGL(k_rdf_cells<true>, dim3(nblocks), dim3(256), pair_lds_bytes(a.nit, a.lsmax), s, a, g,
+                       st, start, spos, stype, n, waves_per_cell, ntasks, partial);
+  else
+    hipLaunchKernelGGL(k_rdf_cells<false>, dim3(nblocks), dim3(256), pair_lds_bytes(a.nit, a.lsmax), s, a, g,
+                       st, start, spos, stype, n, waves_per_cell, ntasks, partial);
 }
 
 void launch_rdf_allpairs(const PairArgs& a, const double* pos, const int32_t* itype, int64_t n,

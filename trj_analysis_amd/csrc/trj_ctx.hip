@@ -259,6 +259,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
         cg.inv_h[k] = nc / box[k];
         cg.periodic[k] = periodic[k];
         if (periodic[k] && nc < 3) cells_ok = false;
+        pa.cellshift[k] = (periodic[k] && nc >= 5) ? 1 : 0;
       } else {
         cg.nc[k] = 1;
         cg.inv_h[k] = 0.0;
@@ -285,7 +286,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
       int W = static_cast<int>(std::lround(mean / 64.0));
       W = std::max(1, std::min(W, 8));
       const int64_t ntasks = ncell * W;
-      nblocks = static_cast<int>(std::min<int64_t>((ntasks + 3) / 4, 1024));
+      nblocks = static_cast<int>(std::min<int64_t>((ntasks + 3) / 4, 2048));
       if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
       launch_rdf_cells(pa, cg, P<int32_t>(c->cellstart), P<double>(c->spos), P<int32_t>(c->stype), n, W,
                        nblocks, P<uint32_t>(c->partial), c->stream);
@@ -311,8 +312,12 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     SqPlan& S = c->sq;
     const int ntiles = static_cast<int>(S.tiles.size());
     if (ntiles > 0) {
+      // K-splits: enough workgroups to fill the device, and for the FP32 /
+      // 16-bit kernels at most 8192 atoms per workgroup (the FP32 accumulator
+      // span; the splits are summed in FP64).
       int ks = static_cast<int>((2048 + ntiles - 1) / ntiles);
       ks = std::max(1, std::min<int>(ks, static_cast<int>(std::max<int64_t>(1, n / 8192))));
+      if (S.p.precision != TRJ_PREC_FP64) ks = std::max<int>(ks, static_cast<int>((n + 8191) / 8192));
       c->ksplit = ks;
       const size_t rho_bytes = sizeof(double) * 2 * static_cast<size_t>(ks) * nsp * ntiles * S.tk.vec();
       if ((r = ensure(c, c->rho, rho_bytes))) return r;
