@@ -93,13 +93,92 @@ def cpu_baseline(wl, frame, n_in_range_pairs: float, nq_terms: int, budget_s: fl
     }
 
 
+def bench_batch(args):
+    """uneven_batch: frames of N log-uniform in [1e4, 1e6], 2-4 species, float32,
+    rcrdf=4, deltar=0.01; per-frame pair histograms.  Timed: the batched
+    graph-replayed path (autotuned on the first frames) against the
+    one-frame-per-launch path on the same frames; results compared bit for bit."""
+    import torch
+    from trj_analysis_amd import trjhip as T
+    from trj_analysis_amd import workloads as W
+
+    nf = args.batch_frames
+    log(f"generating {nf} uneven frames")
+    frames = [W.uneven_frame(1000 + k) for k in range(nf)]
+    natoms = sum(len(f.itype) for f in frames)
+    pos = torch.from_numpy(np.concatenate([f.pos for f in frames]).astype(np.float32)).cuda()
+    it = torch.from_numpy(np.concatenate([f.itype for f in frames]).astype(np.int32)).cuda()
+    offs = np.concatenate([[0], np.cumsum([len(f.itype) for f in frames])])
+    torch.cuda.synchronize()
+    wl = W.UNEVEN_BATCH
+    fa = T.batch_frames((len(f.itype), f.nsp, f.box, f.periodic) for f in frames)
+
+    # one frame per launch (the per-frame GPU path), each frame its own request
+    ctx = T.Context(0)
+    plain = []
+    def per_frame(collect):
+        for k, f in enumerate(frames):
+            ctx.rdf_init(T.rdf_params(3, f.nsp, wl.deltar, wl.rcrdf, f.periodic), f.box)
+            ctx.frame_device(pos.data_ptr() + 12 * int(offs[k]), T.TRJ_F32, T.TRJ_AOS,
+                             it.data_ptr() + 4 * int(offs[k]), len(f.itype), f.box, T.TRJ_WANT_RDF)
+            c = ctx.rdf_counts()
+            if collect:
+                plain.append(c)
+    per_frame(False)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    per_frame(True)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    ctx.close()
+
+    b = T.Batch(T.batch_params(3, wl.deltar, wl.rcrdf, use_graph=True))
+    chosen = b.autotune(fa, pos.data_ptr(), T.TRJ_F32, it.data_ptr())
+    log(f"autotuned: frames_per_launch={chosen.frames_per_launch} workgroups={chosen.workgroups} "
+        f"waves_per_cell={chosen.waves_per_cell}")
+    b.run(fa, pos.data_ptr(), T.TRJ_F32, it.data_ptr(), want_hist=False)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    got, gpu_ms = b.run(fa, pos.data_ptr(), T.TRJ_F32, it.data_ptr())
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    exact = all(np.array_equal(got[k], plain[k]) for k in range(nf))
+    pairs = sum(int(c.sum()) for c in plain)
+    nog = T.Batch(T.batch_params(3, wl.deltar, wl.rcrdf, chosen.frames_per_launch, chosen.waves_per_cell,
+                                 chosen.workgroups, use_graph=False))
+    nog.run(fa, pos.data_ptr(), T.TRJ_F32, it.data_ptr(), want_hist=False)
+    t0 = time.perf_counter()
+    nog.run(fa, pos.data_ptr(), T.TRJ_F32, it.data_ptr(), want_hist=False)
+    t_nograph = time.perf_counter() - t0
+    out = {
+        "metric": METRIC, "value": nf / t_batch, "unit": "frames/s", "n_gpus": 1, "steps": nf,
+        "warmup": 1, "ms_per_step": t_batch * 1e3 / nf, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "uneven_batch", "frames": nf, "atoms": natoms, "rcrdf": wl.rcrdf,
+                   "deltar": wl.deltar, "inputs": "HBM-resident float32",
+                   "autotuned": {"frames_per_launch": chosen.frames_per_launch,
+                                 "workgroups": chosen.workgroups,
+                                 "waves_per_cell": chosen.waves_per_cell}},
+        "pairs_per_s": pairs / t_batch,
+        "plain_frames_per_s": nf / t_plain,
+        "speedup_vs_plain": t_plain / t_batch,
+        "no_graph_frames_per_s": nf / t_nograph,
+        "batch_gpu_ms": gpu_ms,
+        "bit_exact_vs_plain": exact,
+    }
+    b.close()
+    nog.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="lattice_262k",
-                    choices=["lattice_262k", "lj_binary_4k", "uniform_1m"])
+                    choices=["lattice_262k", "lj_binary_4k", "uniform_1m", "uneven_batch"])
+    ap.add_argument("--batch-frames", type=int, default=512)
     ap.add_argument("--precision", default="fp64", choices=list(PEAK))
     ap.add_argument("--latency-frames", type=int, default=10)
     ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0)
@@ -109,6 +188,10 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "; run it with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "uneven_batch":
+        bench_batch(args)
+        return
 
     import torch
     from trj_analysis_amd import trjhip as T

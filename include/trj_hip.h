@@ -203,6 +203,53 @@ int trj_timing_reset(trj_ctx* ctx);
 /* Device memory the context holds, in bytes. */
 int64_t trj_device_bytes(const trj_ctx* ctx);
 
+/* ------------------------------------------------------------------------
+ * Batched frames: many independent frames of uneven size and species count,
+ * each with its own pair histogram (the frame loop of trj_analysis.f90:173-242
+ * with RDFcomp per frame, rdf.cuf:184-353), run as groups of frames in
+ * fixed-grid persistent kernels.  With use_graph the per-group launch
+ * sequence is captured once into a HIP graph and replayed for every group.
+ * Inputs are concatenated in device memory: positions AoS (stride ndim,
+ * TRJ_F32 or TRJ_F64) frame after frame, species 1..nsp_f likewise. */
+typedef struct trj_batch trj_batch;
+
+typedef struct {
+  int64_t n;            /* atoms of this frame */
+  int32_t nsp;          /* species of this frame, 1..10 */
+  int32_t periodic[3];
+  double box[3];
+} trj_batch_frame;
+
+typedef struct {
+  int32_t ndim;               /* 2 or 3, same for all frames */
+  double deltar, rcrdf;       /* /INPUT_RDF/, same for all frames */
+  int32_t frames_per_launch;  /* frames per group; 0 = 64 */
+  int32_t waves_per_cell;     /* wavefronts per home cell; 0 = from the mean occupancy */
+  int32_t workgroups;         /* persistent grid; 0 = 1024 */
+  int32_t use_graph;          /* 1: capture the group launch sequence once and replay it */
+} trj_batch_params;
+
+int trj_batch_create(trj_batch** out, int device, const trj_batch_params* p);
+void trj_batch_destroy(trj_batch* b);
+const char* trj_batch_last_error(const trj_batch* b);
+
+/* Host-only: lsmax and nit of each frame and the offset of its histogram
+ * (int64, nit x lsmax, histomix_d(lsmax,nit) order) in the output; returns
+ * the total number of int64 values through *total. */
+int trj_batch_layout(const trj_batch_params* p, int32_t nframes, const trj_batch_frame* fr,
+                     int32_t* lsmax, int32_t* nit, int64_t* hist_off, int64_t* total);
+
+/* Run all frames; hist_host receives every frame's counts at hist_off.
+ * gpu_ms (may be NULL) receives the device time of the whole run. */
+int trj_batch_run(trj_batch* b, int32_t nframes, const trj_batch_frame* fr, const void* pos_dev,
+                  int32_t dtype, const int32_t* itype_dev, int64_t* hist_host, double* gpu_ms);
+
+/* Project autotuner: times candidate (frames_per_launch, workgroups,
+ * waves_per_cell) settings on the first frames given (at most 128), keeps
+ * the fastest in the batch object and reports it through chosen. */
+int trj_batch_autotune(trj_batch* b, int32_t nframes, const trj_batch_frame* fr, const void* pos_dev,
+                       int32_t dtype, const int32_t* itype_dev, trj_batch_params* chosen);
+
 #ifdef __cplusplus
 }
 #endif

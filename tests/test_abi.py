@@ -72,3 +72,11 @@ def test_plan_large_qmax_is_clamped():
     with pytest.raises(T.TrjError) as e:
         T.sq_plan(T.sq_params(3, 1, 1e9), [L] * 3)
     assert e.value.code == T.TRJ_ELIMIT
+
+
+def test_batch_layout_host_only():
+    from trj_analysis_amd import trjhip as T
+    fa = T.batch_frames([(1000, 2, [20.0] * 3, (1, 1, 1)), (5000, 4, [30.0] * 3, (1, 1, 0))])
+    ls, nit, off, tot = T.batch_layout(T.batch_params(3, 0.01, 4.0), fa)
+    assert list(ls) == [401, 401] and list(nit) == [3, 10]
+    assert list(off) == [0, 3 * 401] and tot == 13 * 401

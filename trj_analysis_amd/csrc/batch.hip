@@ -1,0 +1,420 @@
+// batch.hip -- the batched-frames API of the trj_hip library (trj_batch_*):
+// per-frame planning, grouping into launch groups, device descriptors, and
+// graph capture / replay of the fixed-grid per-group launch sequence.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/trj_hip.h"
+#include "kernels.h"
+#include "trj_internal.h"
+
+using namespace trj;
+
+namespace {
+
+struct Buf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+struct FramePlan {
+  int lsmax, nit, ncell, W;
+  int64_t ntasks, hist_off;
+  BatchFrameDesc d;
+};
+
+}  // namespace
+
+struct trj_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  trj_batch_params p{};
+  std::string err;
+  Buf hdr, fr, pos, spos, stype, cellid, count, start, cursor, blocksum, thr, hist, flags;
+  void* pin[2] = {nullptr, nullptr};
+  size_t pin_cap[2] = {0, 0};
+  hipEvent_t pin_done[2] = {nullptr, nullptr};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int thr_lsmax = -1;
+  double thr_deltar = 0.0;
+  // captured group sequence
+  hipGraphExec_t gexec = nullptr;
+  BatchDev gdev{};
+  int g_nblocks = 0;
+  size_t g_lds = 0;
+};
+
+namespace {
+
+int bfail(trj_batch* b, int code, const std::string& m) {
+  if (b) b->err = m;
+  return code;
+}
+
+#define BHIP(b, call)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) return bfail((b), TRJ_EDEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+int bensure(trj_batch* b, Buf& x, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (x.cap >= bytes) return TRJ_OK;
+  BHIP(b, hipStreamSynchronize(b->stream));
+  if (x.p) (void)hipFree(x.p);
+  x.p = nullptr;
+  x.cap = 0;
+  const size_t want = bytes + bytes / 4;
+  if (hipMalloc(&x.p, want) != hipSuccess) return bfail(b, TRJ_ENOMEM, "batch: device allocation failed");
+  x.cap = want;
+  if (b->gexec) {  // pointers changed: the captured sequence is stale
+    (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+  }
+  return TRJ_OK;
+}
+
+int plan_frames(const trj_batch_params& p, int nframes, const trj_batch_frame* fr, std::vector<FramePlan>& out,
+                int64_t* total, std::string* err) {
+  if (p.ndim != 2 && p.ndim != 3) {
+    *err = "batch: ndim must be 2 or 3";
+    return TRJ_EINVAL;
+  }
+  if (!(p.deltar > 0.0)) {
+    *err = "batch: deltar must be positive";
+    return TRJ_EINVAL;
+  }
+  out.assign(nframes, FramePlan{});
+  int64_t off = 0;
+  for (int f = 0; f < nframes; ++f) {
+    const trj_batch_frame& F = fr[f];
+    FramePlan& P = out[f];
+    if (F.n < 1 || F.nsp < 1 || F.nsp > kNspMax) {
+      *err = "batch: frame " + std::to_string(f) + " has n < 1 or nsp outside 1..10";
+      return TRJ_EINVAL;
+    }
+    double box[3] = {F.box[0], F.box[1], p.ndim == 3 ? F.box[2] : 1.0};
+    for (int k = 0; k < p.ndim; ++k)
+      if (!(box[k] > 0.0)) {
+        *err = "batch: frame " + std::to_string(f) + " has a non-positive box length";
+        return TRJ_EINVAL;
+      }
+    double rc = 0.0;
+    P.lsmax = rdf_lsmax(p.ndim, box, p.rcrdf, p.deltar, &rc);
+    P.nit = F.nsp * (F.nsp + 1) / 2;
+    if (pair_lds_bytes(P.nit, P.lsmax) > 64 * 1024) {
+      *err = "batch: frame " + std::to_string(f) + " histogram exceeds 64 KiB of LDS";
+      return TRJ_ELIMIT;
+    }
+    int per[3] = {F.periodic[0] != 0, F.periodic[1] != 0, F.periodic[2] != 0};
+    double rc_sticky = p.rcrdf;  // frames are independent: the cap applies per frame
+    FrameGeom g = frame_geometry(p.ndim, box, per, &rc_sticky);
+    BatchFrameDesc& D = P.d;
+    std::memset(&D, 0, sizeof(D));
+    const double rcut = std::sqrt(g.side2);
+    P.ncell = 1;
+    for (int k = 0; k < 3; ++k) {
+      if (k < p.ndim) {
+        int nc = static_cast<int>(std::floor(box[k] / (rcut * (1.0 + 1e-6))));
+        nc = std::max(1, std::min(nc, p.ndim == 3 ? 160 : 2048));
+        if (per[k] && nc < 3) {
+          *err = "batch: frame " + std::to_string(f) +
+                 " has fewer than 3 cells on a periodic axis; use the per-frame path (trj_frame)";
+          return TRJ_EINVAL;
+        }
+        D.nc[k] = nc;
+        D.inv_h[k] = nc / box[k];
+        D.periodic[k] = per[k];
+        D.cellshift[k] = (per[k] && nc >= 5) ? 1 : 0;
+      } else {
+        D.nc[k] = 1;
+        D.inv_h[k] = 0.0;
+      }
+      D.box[k] = box[k];
+      D.lmi[k] = g.lmi[k];
+      D.half[k] = g.half[k];
+      P.ncell *= D.nc[k];
+    }
+    const double mean = static_cast<double>(F.n) / P.ncell;
+    P.W = p.waves_per_cell > 0 ? p.waves_per_cell
+                               : std::max(1, std::min(8, static_cast<int>(std::lround(mean / 64.0))));
+    P.ntasks = static_cast<int64_t>(P.ncell) * P.W;
+    D.n = static_cast<int32_t>(F.n);
+    D.nsp = F.nsp;
+    D.nit = P.nit;
+    D.lsmax = P.lsmax;
+    D.ncell = P.ncell;
+    D.W = P.W;
+    D.side2 = g.side2;
+    D.inv_dr = static_cast<float>(1.0 / p.deltar);
+    P.hist_off = off;
+    D.hist_off = off;
+    off += static_cast<int64_t>(P.nit) * P.lsmax;
+  }
+  *total = off;
+  return TRJ_OK;
+}
+
+int run_groups(trj_batch* b, const trj_batch_params& p, int nframes, const trj_batch_frame* fr,
+               const void* pos_dev, int dtype, const int32_t* itype_dev, int64_t* hist_host, double* gpu_ms) {
+  std::vector<FramePlan> plan;
+  int64_t total = 0;
+  int r = plan_frames(p, nframes, fr, plan, &total, &b->err);
+  if (r) return r;
+  if (dtype != TRJ_F32 && dtype != TRJ_F64) return bfail(b, TRJ_EINVAL, "batch: bad dtype");
+  const int F = p.frames_per_launch > 0 ? p.frames_per_launch : 64;
+  const int G = p.workgroups > 0 ? p.workgroups : 1024;
+  // capacities over all groups
+  std::vector<int64_t> atom_off(nframes + 1, 0);
+  for (int f = 0; f < nframes; ++f) atom_off[f + 1] = atom_off[f] + fr[f].n;
+  int64_t max_atoms = 1, max_cells = 1;
+  int max_lsmax = 1;
+  size_t lds = 0;
+  for (int g0 = 0; g0 < nframes; g0 += F) {
+    const int g1 = std::min(nframes, g0 + F);
+    int64_t na = 0, nc = 0;
+    for (int f = g0; f < g1; ++f) {
+      na += fr[f].n;
+      nc += plan[f].ncell;
+      max_lsmax = std::max(max_lsmax, plan[f].lsmax);
+      lds = std::max(lds, pair_lds_bytes(plan[f].nit, plan[f].lsmax));
+    }
+    max_atoms = std::max(max_atoms, na);
+    max_cells = std::max(max_cells, nc);
+  }
+  if (max_atoms > (int64_t(1) << 31) - 1024 || max_cells > (int64_t(1) << 30))
+    return bfail(b, TRJ_ELIMIT, "batch: a launch group exceeds 2^31 atoms; lower frames_per_launch");
+  if ((r = bensure(b, b->hdr, sizeof(BatchHeader)))) return r;
+  if ((r = bensure(b, b->fr, sizeof(BatchFrameDesc) * F))) return r;
+  if ((r = bensure(b, b->pos, sizeof(double) * 3 * max_atoms))) return r;
+  if ((r = bensure(b, b->spos, sizeof(double) * 3 * max_atoms))) return r;
+  if ((r = bensure(b, b->stype, sizeof(int32_t) * max_atoms))) return r;
+  if ((r = bensure(b, b->cellid, sizeof(int32_t) * max_atoms))) return r;
+  if ((r = bensure(b, b->count, sizeof(int32_t) * (max_cells + 1)))) return r;
+  if ((r = bensure(b, b->start, sizeof(int32_t) * (max_cells + 1)))) return r;
+  if ((r = bensure(b, b->cursor, sizeof(int32_t) * max_cells))) return r;
+  if ((r = bensure(b, b->blocksum, sizeof(int32_t) * (G + 1)))) return r;
+  if ((r = bensure(b, b->hist, sizeof(int64_t) * std::max<int64_t>(1, total)))) return r;
+  if ((r = bensure(b, b->flags, sizeof(uint32_t)))) return r;
+  if (b->thr_lsmax < max_lsmax || b->thr_deltar != p.deltar) {
+    std::vector<double> thr = rdf_bin_thresholds(p.deltar, max_lsmax);
+    thr.push_back(INFINITY);
+    thr[0] = 0.0;
+    if ((r = bensure(b, b->thr, sizeof(double) * thr.size()))) return r;
+    BHIP(b, hipMemcpyAsync(b->thr.p, thr.data(), sizeof(double) * thr.size(), hipMemcpyHostToDevice, b->stream));
+    b->thr_lsmax = max_lsmax;
+    b->thr_deltar = p.deltar;
+  }
+  // Frames of different lsmax share the edges: thr[m] depends on m and deltar only.
+  BatchDev d{};
+  d.hdr = static_cast<const BatchHeader*>(b->hdr.p);
+  d.fr = static_cast<const BatchFrameDesc*>(b->fr.p);
+  d.pos_in = pos_dev;
+  d.itype_in = itype_dev;
+  d.dtype = dtype;
+  d.ndim = p.ndim;
+  d.pos = static_cast<double*>(b->pos.p);
+  d.spos = static_cast<double*>(b->spos.p);
+  d.stype = static_cast<int32_t*>(b->stype.p);
+  d.cellid = static_cast<int32_t*>(b->cellid.p);
+  d.count = static_cast<int32_t*>(b->count.p);
+  d.start = static_cast<int32_t*>(b->start.p);
+  d.cursor = static_cast<int32_t*>(b->cursor.p);
+  d.blocksum = static_cast<int32_t*>(b->blocksum.p);
+  d.cap_atoms = max_atoms;
+  d.thr = static_cast<const double*>(b->thr.p);
+  d.hist = static_cast<int64_t*>(b->hist.p);
+  d.flags = static_cast<uint32_t*>(b->flags.p);
+  BHIP(b, hipMemsetAsync(b->hist.p, 0, sizeof(int64_t) * std::max<int64_t>(1, total), b->stream));
+  BHIP(b, hipMemsetAsync(b->flags.p, 0, sizeof(uint32_t), b->stream));
+  const bool graph = p.use_graph != 0;
+  const bool same = b->gexec && std::memcmp(&b->gdev, &d, sizeof(d)) == 0 && b->g_nblocks == G && b->g_lds >= lds;
+  if (graph && !same) {
+    if (b->gexec) {
+      (void)hipGraphExecDestroy(b->gexec);
+      b->gexec = nullptr;
+    }
+    hipGraph_t gr = nullptr;
+    BHIP(b, hipStreamSynchronize(b->stream));
+    BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    launch_batch_group(d, G, lds, b->stream);
+    BHIP(b, hipStreamEndCapture(b->stream, &gr));
+    hipError_t e = hipGraphInstantiate(&b->gexec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) return bfail(b, TRJ_EDEVICE, std::string("batch graph: ") + hipGetErrorString(e));
+    b->gdev = d;
+    b->g_nblocks = G;
+    b->g_lds = lds;
+  }
+  BHIP(b, hipEventRecord(b->ev0, b->stream));
+  int k = 0;
+  for (int g0 = 0; g0 < nframes; g0 += F, k ^= 1) {
+    const int g1 = std::min(nframes, g0 + F);
+    const int nf = g1 - g0;
+    const size_t bytes = sizeof(BatchHeader) + sizeof(BatchFrameDesc) * nf;
+    BHIP(b, hipEventSynchronize(b->pin_done[k]));
+    if (b->pin_cap[k] < bytes) {
+      if (b->pin[k]) (void)hipHostFree(b->pin[k]);
+      b->pin[k] = nullptr;
+      if (hipHostMalloc(&b->pin[k], sizeof(BatchHeader) + sizeof(BatchFrameDesc) * F, hipHostMallocDefault) !=
+          hipSuccess)
+        return bfail(b, TRJ_ENOMEM, "batch: pinned allocation failed");
+      b->pin_cap[k] = sizeof(BatchHeader) + sizeof(BatchFrameDesc) * F;
+    }
+    BatchHeader* H = static_cast<BatchHeader*>(b->pin[k]);
+    BatchFrameDesc* D = reinterpret_cast<BatchFrameDesc*>(static_cast<char*>(b->pin[k]) + sizeof(BatchHeader));
+    int64_t la = 0, lt = 0;
+    int lc = 0;
+    for (int f = g0; f < g1; ++f) {
+      BatchFrameDesc x = plan[f].d;
+      x.atom_off = atom_off[f];
+      x.local_off = la;
+      x.task_off = lt;
+      x.cell_off = lc;
+      D[f - g0] = x;
+      la += fr[f].n;
+      lt += plan[f].ntasks;
+      lc += plan[f].ncell;
+    }
+    H->nframes = nf;
+    H->ncells = lc;
+    H->natoms = la;
+    H->ntasks = lt;
+    BHIP(b, hipMemcpyAsync(b->hdr.p, H, sizeof(BatchHeader), hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMemcpyAsync(b->fr.p, D, sizeof(BatchFrameDesc) * nf, hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipEventRecord(b->pin_done[k], b->stream));
+    if (graph)
+      BHIP(b, hipGraphLaunch(b->gexec, b->stream));
+    else
+      launch_batch_group(d, G, lds, b->stream);
+    BHIP(b, hipGetLastError());
+  }
+  BHIP(b, hipEventRecord(b->ev1, b->stream));
+  uint32_t flags = 0;
+  BHIP(b, hipMemcpyAsync(&flags, b->flags.p, sizeof(flags), hipMemcpyDeviceToHost, b->stream));
+  if (hist_host)
+    BHIP(b, hipMemcpyAsync(hist_host, b->hist.p, sizeof(int64_t) * total, hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  if (gpu_ms) {
+    float ms = 0.f;
+    BHIP(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    *gpu_ms = ms;
+  }
+  if (flags & kFlagType) return bfail(b, TRJ_EINVAL, "batch: species types outside 1..nsp");
+  if (flags & kFlagRange) return bfail(b, TRJ_ERANGE, "batch: non-finite or out-of-range coordinates");
+  return TRJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int trj_batch_create(trj_batch** out, int device, const trj_batch_params* p) {
+  if (!out || !p) return TRJ_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return TRJ_EDEVICE;
+  if (device < 0 || device >= ndev) return TRJ_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return TRJ_EDEVICE;
+  trj_batch* b = new trj_batch();
+  b->device = device;
+  b->p = *p;
+  bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
+  ok = ok && hipEventCreate(&b->ev0) == hipSuccess && hipEventCreate(&b->ev1) == hipSuccess;
+  for (auto& e : b->pin_done) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    trj_batch_destroy(b);
+    return TRJ_EDEVICE;
+  }
+  *out = b;
+  return TRJ_OK;
+}
+
+void trj_batch_destroy(trj_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  Buf* bufs[] = {&b->hdr, &b->fr, &b->pos, &b->spos, &b->stype, &b->cellid, &b->count,
+                 &b->start, &b->cursor, &b->blocksum, &b->thr, &b->hist, &b->flags};
+  for (Buf* x : bufs)
+    if (x->p) (void)hipFree(x->p);
+  for (int k = 0; k < 2; ++k) {
+    if (b->pin[k]) (void)hipHostFree(b->pin[k]);
+    if (b->pin_done[k]) (void)hipEventDestroy(b->pin_done[k]);
+  }
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char* trj_batch_last_error(const trj_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int trj_batch_layout(const trj_batch_params* p, int32_t nframes, const trj_batch_frame* fr, int32_t* lsmax,
+                     int32_t* nit, int64_t* hist_off, int64_t* total) {
+  if (!p || !fr || nframes < 0) return TRJ_EINVAL;
+  std::vector<FramePlan> plan;
+  int64_t tot = 0;
+  std::string err;
+  int r = plan_frames(*p, nframes, fr, plan, &tot, &err);
+  if (r) return r;
+  for (int f = 0; f < nframes; ++f) {
+    if (lsmax) lsmax[f] = plan[f].lsmax;
+    if (nit) nit[f] = plan[f].nit;
+    if (hist_off) hist_off[f] = plan[f].hist_off;
+  }
+  if (total) *total = tot;
+  return TRJ_OK;
+}
+
+int trj_batch_run(trj_batch* b, int32_t nframes, const trj_batch_frame* fr, const void* pos_dev, int32_t dtype,
+                  const int32_t* itype_dev, int64_t* hist_host, double* gpu_ms) {
+  if (!b) return TRJ_EINVAL;
+  if (nframes < 1 || !fr || !pos_dev || !itype_dev) return bfail(b, TRJ_EINVAL, "batch: empty or NULL input");
+  (void)hipSetDevice(b->device);
+  return run_groups(b, b->p, nframes, fr, pos_dev, dtype, itype_dev, hist_host, gpu_ms);
+}
+
+int trj_batch_autotune(trj_batch* b, int32_t nframes, const trj_batch_frame* fr, const void* pos_dev,
+                       int32_t dtype, const int32_t* itype_dev, trj_batch_params* chosen) {
+  if (!b) return TRJ_EINVAL;
+  if (nframes < 1 || !fr || !pos_dev || !itype_dev) return bfail(b, TRJ_EINVAL, "batch: empty or NULL input");
+  (void)hipSetDevice(b->device);
+  const int ns = std::min<int>(nframes, 128);
+  const int fpl[] = {16, 64, 128};
+  const int wgs[] = {1024, 2048};
+  const int wpc[] = {0, 1, 2};
+  double best = 1e300;
+  trj_batch_params bp = b->p;
+  for (int f : fpl) {
+    if (f > ns && f != fpl[0]) continue;
+    for (int g : wgs)
+      for (int w : wpc) {
+        trj_batch_params t = b->p;
+        t.frames_per_launch = f;
+        t.workgroups = g;
+        t.waves_per_cell = w;
+        double ms = 0.0;
+        int r = run_groups(b, t, ns, fr, pos_dev, dtype, itype_dev, nullptr, &ms);  // warm (captures)
+        if (r) return r;
+        r = run_groups(b, t, ns, fr, pos_dev, dtype, itype_dev, nullptr, &ms);
+        if (r) return r;
+        if (ms < best) {
+          best = ms;
+          bp = t;
+        }
+      }
+  }
+  b->p = bp;
+  if (chosen) *chosen = bp;
+  return TRJ_OK;
+}
+
+}  // extern "C"

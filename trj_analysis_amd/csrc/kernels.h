@@ -94,4 +94,47 @@ void launch_sq_list(const double* u, const int32_t* sp_off, int64_t n, int nsp, 
                     const int32_t* kv, const int32_t* shell, int nvec, const double* bsc,
                     int nqmax, double* part, double* sqf, double* sqfp, hipStream_t s);
 
+// ------------------------------------------------------------------ batches
+// Frames of one launch group, concatenated.  Device-side descriptors drive
+// fixed-grid kernels, so one captured launch sequence serves every group.
+struct BatchFrameDesc {
+  int64_t atom_off;   // first atom in the caller's concatenated input
+  int64_t local_off;  // first atom in the group's arrays
+  int64_t task_off;   // first pair task in the group
+  int64_t hist_off;   // first int64 of this frame's histogram in the output
+  int32_t n, nsp, nit, lsmax;
+  int32_t cell_off, ncell, W, pad;
+  int32_t nc[3], periodic[3], cellshift[3], pad2;
+  double box[3], inv_h[3], lmi[3], half[3];
+  double side2;
+  float inv_dr;
+  float pad3;
+};
+struct BatchHeader {
+  int32_t nframes, ncells;
+  int64_t natoms, ntasks;
+};
+struct BatchDev {
+  const BatchHeader* hdr;
+  const BatchFrameDesc* fr;
+  const void* pos_in;      // caller's positions (AoS, stride ndim), dtype below
+  const int32_t* itype_in;
+  int dtype, ndim;
+  double* pos;             // group-local FP64 SoA, 3 * max atoms
+  double* spos;            // cell-sorted
+  int32_t* stype;
+  int32_t* cellid;
+  int32_t* count;          // ncells + 1
+  int32_t* start;          // ncells + 1
+  int32_t* cursor;         // ncells
+  int32_t* blocksum;       // scan scratch, nblocks + 1
+  int64_t cap_atoms;       // array capacities (checked on the host)
+  const double* thr;       // bin edges, lsmax_max + 3
+  int64_t* hist;           // all frames' histograms
+  uint32_t* flags;
+};
+// Launch the whole per-group sequence on stream s with grid nblocks (all
+// kernels are grid-stride loops; nothing depends on the group's sizes).
+void launch_batch_group(const BatchDev& d, int nblocks, size_t pair_lds, hipStream_t s);
+
 }  // namespace trj

@@ -956,6 +956,289 @@ __global__ void k_sq_list_finish(const double* __restrict__ part, const int32_t*
   atomicAdd(&sqf[sh], tr * tr + ti * ti);
 }
 
+// ------------------------------------------------------------------ batches
+
+__device__ inline int frame_of_atom(const BatchDev& d, int nframes, int64_t i) {
+  int lo = 0, hi = nframes - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d.fr[mid].local_off <= i) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ inline int frame_of_task(const BatchFrameDesc* fr, int nframes, int64_t t) {
+  int lo = 0, hi = nframes - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (fr[mid].task_off <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Zero the cell counters and cursors (a fill kernel: graph-safe).
+__global__ __launch_bounds__(256) void bk_zero(BatchDev d) {
+  const int64_t nc = d.hdr->ncells;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i <= nc;
+       i += static_cast<int64_t>(gridDim.x) * 256) {
+    d.count[i] = 0;
+    if (i < nc) d.cursor[i] = 0;
+  }
+}
+
+// Wrap (FP64, as k_ingest) and count cells, all frames of the group.
+__global__ __launch_bounds__(256) void bk_ingest(BatchDev d) {
+  const int nframes = d.hdr->nframes;
+  const int64_t na = d.hdr->natoms;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < na;
+       i += static_cast<int64_t>(gridDim.x) * 256) {
+    const int f = frame_of_atom(d, nframes, i);
+    const BatchFrameDesc& F = d.fr[f];
+    const int64_t src = F.atom_off + (i - F.local_off);
+    double c[3] = {0.0, 0.0, 0.0};
+    uint32_t bad = 0;
+    int cc[3] = {0, 0, 0};
+    for (int k = 0; k < d.ndim; ++k) {
+      const int64_t idx = src * d.ndim + k;
+      c[k] = (d.dtype == TRJ_F32) ? static_cast<double>(static_cast<const float*>(d.pos_in)[idx])
+                                  : static_cast<const double*>(d.pos_in)[idx];
+      if (!isfinite(c[k])) {
+        bad |= kFlagRange;
+        c[k] = 0.0;
+      } else if (F.periodic[k]) {
+        c[k] = wrap_periodic(c[k], F.box[k]);
+      } else if (c[k] < -0.25 * F.box[k] || c[k] >= 1.25 * F.box[k]) {
+        bad |= kFlagRange;
+      }
+      int ck = static_cast<int>(floor(c[k] * F.inv_h[k]));
+      cc[k] = ck < 0 ? 0 : (ck >= F.nc[k] ? F.nc[k] - 1 : ck);
+    }
+    const int t = d.itype_in[src];
+    if (t < 1 || t > F.nsp) bad |= kFlagType;
+    if (bad) atomicOr(d.flags, bad);
+    d.pos[i] = c[0];
+    d.pos[d.cap_atoms + i] = c[1];
+    d.pos[2 * d.cap_atoms + i] = c[2];
+    const int id = F.cell_off + (cc[2] * F.nc[1] + cc[1]) * F.nc[0] + cc[0];
+    d.cellid[i] = id;
+    atomicAdd(&d.count[id], 1);
+  }
+}
+
+// Exclusive scan of count[0..ncells] into start, three phases over a fixed
+// grid: per-block sums, one block scanning those, per-block rescans.
+__device__ inline void block_chunk(int64_t total, int b, int nb, int64_t* lo, int64_t* hi) {
+  *lo = total * b / nb;
+  *hi = total * (b + 1) / nb;
+}
+
+__global__ __launch_bounds__(256) void bk_scan_reduce(BatchDev d) {
+  __shared__ int32_t red[256];
+  int64_t lo, hi;
+  block_chunk(static_cast<int64_t>(d.hdr->ncells) + 1, blockIdx.x, gridDim.x, &lo, &hi);
+  int32_t s = 0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += 256) s += d.count[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (static_cast<int>(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) d.blocksum[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(1024) void bk_scan_blocks(BatchDev d, int nb) {
+  __shared__ int32_t tmp[1024];
+  __shared__ int32_t carry;
+  const int t = threadIdx.x;
+  if (t == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += 1024) {
+    const int i = base + t;
+    const int32_t v = i < nb ? d.blocksum[i] : 0;
+    tmp[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int32_t x = t >= off ? tmp[t - off] : 0;
+      __syncthreads();
+      tmp[t] += x;
+      __syncthreads();
+    }
+    if (i < nb) d.blocksum[i] = carry + tmp[t] - v;
+    __syncthreads();
+    if (t == 1023) carry += tmp[1023];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void bk_scan_apply(BatchDev d) {
+  __shared__ int32_t tmp[256];
+  __shared__ int32_t carry;
+  int64_t lo, hi;
+  block_chunk(static_cast<int64_t>(d.hdr->ncells) + 1, blockIdx.x, gridDim.x, &lo, &hi);
+  const int t = threadIdx.x;
+  if (t == 0) carry = d.blocksum[blockIdx.x];
+  __syncthreads();
+  for (int64_t base = lo; base < hi; base += 256) {
+    const int64_t i = base + t;
+    const int32_t v = i < hi ? d.count[i] : 0;
+    tmp[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      const int32_t x = t >= off ? tmp[t - off] : 0;
+      __syncthreads();
+      tmp[t] += x;
+      __syncthreads();
+    }
+    if (i < hi) d.start[i] = carry + tmp[t] - v;
+    __syncthreads();
+    if (t == 255) carry += tmp[255];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void bk_scatter(BatchDev d) {
+  const int64_t na = d.hdr->natoms;
+  const int nframes = d.hdr->nframes;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < na;
+       i += static_cast<int64_t>(gridDim.x) * 256) {
+    const int c = d.cellid[i];
+    const int slot = d.start[c] + atomicAdd(&d.cursor[c], 1);
+    const int f = frame_of_atom(d, nframes, i);
+    const BatchFrameDesc& F = d.fr[f];
+    d.spos[slot] = d.pos[i];
+    d.spos[d.cap_atoms + slot] = d.pos[d.cap_atoms + i];
+    d.spos[2 * d.cap_atoms + slot] = d.pos[2 * d.cap_atoms + i];
+    d.stype[slot] = d.itype_in[F.atom_off + (i - F.local_off)];
+  }
+}
+
+// Pair tasks of all frames: workgroup b owns the contiguous task range
+// [b*T/G, (b+1)*T/G) and walks it frame segment by frame segment, flushing
+// its LDS histogram into the frame's int64 histogram at each segment end.
+__global__ __launch_bounds__(256) void bk_pairs(BatchDev d) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nframes = d.hdr->nframes;
+  const int64_t T = d.hdr->ntasks;
+  int64_t t0, t1;
+  block_chunk(T, blockIdx.x, gridDim.x, &t0, &t1);
+  if (t0 >= t1) return;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const double* __restrict__ sx = d.spos;
+  const double* __restrict__ sy = d.spos + d.cap_atoms;
+  const double* __restrict__ sz = d.spos + 2 * d.cap_atoms;
+  int f = frame_of_task(d.fr, nframes, t0);
+  Stencil st{};
+  {
+    int k = 1;
+    const int zr = (d.ndim == 3) ? 1 : 0;
+    for (int dz = -zr; dz <= zr; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx)
+          if ((dz > 0) || (dz == 0 && dy > 0) || (dz == 0 && dy == 0 && dx > 0)) {
+            st.off[k][0] = dx;
+            st.off[k][1] = dy;
+            st.off[k][2] = dz;
+            ++k;
+          }
+    st.n = k;
+  }
+  for (int64_t seg0 = t0; seg0 < t1; ++f) {
+    const BatchFrameDesc& F = d.fr[f];
+    const int64_t seg1 = (f + 1 < nframes && d.fr[f + 1].task_off < t1) ? d.fr[f + 1].task_off : t1;
+    PairArgs a{};
+    a.ndim = d.ndim;
+    a.nsp = F.nsp;
+    a.nit = F.nit;
+    a.lsmax = F.lsmax;
+    for (int k = 0; k < 3; ++k) {
+      a.lmi[k] = F.lmi[k];
+      a.half[k] = F.half[k];
+      a.cellshift[k] = F.cellshift[k];
+    }
+    a.side2 = F.side2;
+    a.inv_dr = F.inv_dr;
+    const int nbins = a.nit * a.lsmax;
+    uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
+    double* thr = reinterpret_cast<double*>(smem + ((static_cast<size_t>(nbins) * 4 + 15) & ~size_t(15)));
+    a.thr = d.thr;
+    __syncthreads();
+    hist_setup(a, hist, thr);
+    const int ncx = F.nc[0], ncy = F.nc[1], ncz = F.nc[2];
+    bool shift = true;
+    for (int k = 0; k < d.ndim; ++k) shift = shift && F.cellshift[k];
+    for (int64_t task = seg0 + wave; task < seg1; task += 4) {
+      const int64_t lt = task - F.task_off;
+      const int cellL = static_cast<int>(lt / F.W);
+      const int ch0 = static_cast<int>(lt % F.W);
+      const int cell = F.cell_off + cellL;
+      const int cs = d.start[cell], ce = d.start[cell + 1];
+      const int cx = cellL % ncx, cy = (cellL / ncx) % ncy, cz = cellL / (ncx * ncy);
+      for (int ch = ch0; cs + ch * kWave < ce; ch += F.W) {
+        const int gi = cs + ch * kWave + lane;
+        const bool vi = gi < ce;
+        const double xi = vi ? sx[gi] : 0.0;
+        const double yi = vi ? sy[gi] : 0.0;
+        const double zi = vi ? sz[gi] : 0.0;
+        const int ti = vi ? d.stype[gi] : 0;
+        const bool oki = vi && static_cast<unsigned>(ti - 1) < static_cast<unsigned>(a.nsp);
+        for (int o = 0; o < st.n; ++o) {
+          int nx = cx + st.off[o][0], ny = cy + st.off[o][1], nz = cz + st.off[o][2];
+          const double shx = nx < 0 ? a.lmi[0] : (nx >= ncx ? -a.lmi[0] : 0.0);
+          const double shy = ny < 0 ? a.lmi[1] : (ny >= ncy ? -a.lmi[1] : 0.0);
+          const double shz = nz < 0 ? a.lmi[2] : (nz >= ncz ? -a.lmi[2] : 0.0);
+          if (F.periodic[0]) nx = (nx + ncx) % ncx; else if (nx < 0 || nx >= ncx) continue;
+          if (F.periodic[1]) ny = (ny + ncy) % ncy; else if (ny < 0 || ny >= ncy) continue;
+          if (d.ndim == 3) {
+            if (F.periodic[2]) nz = (nz + ncz) % ncz; else if (nz < 0 || nz >= ncz) continue;
+          }
+          const int nb = F.cell_off + (nz * ncy + ny) * ncx + nx;
+          const bool self = (o == 0);
+          const int js = self ? cs + ch * kWave + 1 : d.start[nb];
+          const int je = d.start[nb + 1];
+          for (int jb = js; jb < je; jb += kWave) {
+            const int jl = jb + lane;
+            const bool vl = jl < je;
+            const double xl = vl ? sx[jl] : 0.0;
+            const double yl = vl ? sy[jl] : 0.0;
+            const double zl = vl ? sz[jl] : 0.0;
+            const int tl = vl ? d.stype[jl] : 0;
+            const int cnt = (je - jb) < kWave ? (je - jb) : kWave;
+            for (int jj = 0; jj < cnt; ++jj) {
+              const int j = jb + jj;
+              const int tj = __builtin_amdgcn_readlane(tl, jj);
+              const bool ok = oki && (!self || j > gi) &&
+                              static_cast<unsigned>(tj - 1) < static_cast<unsigned>(a.nsp);
+              if (shift) {
+                const double dx = (readlane_d(xl, jj) - xi) - shx;
+                const double dy = (readlane_d(yl, jj) - yi) - shy;
+                double r2;
+                if (a.ndim == 3) {
+                  const double dz = (readlane_d(zl, jj) - zi) - shz;
+                  r2 = dx * dx + dy * dy + dz * dz;
+                } else {
+                  r2 = dx * dx + dy * dy;
+                }
+                bin_pair(a, thr, hist, ok, r2, ti, tj);
+              } else {
+                pair_accumulate(a, thr, hist, ok, xi, yi, zi, ti, readlane_d(xl, jj), readlane_d(yl, jj),
+                                readlane_d(zl, jj), tj);
+              }
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    int64_t* out = d.hist + F.hist_off;
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
+      const uint32_t v = hist[b];
+      if (v) atomicAdd(reinterpret_cast<unsigned long long*>(&out[b]), static_cast<unsigned long long>(v));
+    }
+    seg0 = seg1;
+  }
+}
+
 inline unsigned grid1d(int64_t n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
 
 }  // namespace
@@ -1090,6 +1373,16 @@ void launch_sq_list(const double* u, const int32_t* sp_off, int64_t n, int nsp, 
   hipLaunchKernelGGL(k_sq_list, dim3(nvec, kListBlocks), dim3(256), 0, s, u, sp_off, n, nsp, ndim, kv, part);
   hipLaunchKernelGGL(k_sq_list_finish, dim3((nvec + 63) / 64), dim3(64), 0, s, part, shell, nvec, nsp, b, nqmax,
                      sqf, sqfp);
+}
+
+void launch_batch_group(const BatchDev& d, int nblocks, size_t pair_lds, hipStream_t s) {
+  hipLaunchKernelGGL(bk_zero, dim3(nblocks), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(bk_ingest, dim3(nblocks), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(bk_scan_reduce, dim3(nblocks), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(bk_scan_blocks, dim3(1), dim3(1024), 0, s, d, nblocks);
+  hipLaunchKernelGGL(bk_scan_apply, dim3(nblocks), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(bk_scatter, dim3(nblocks), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(bk_pairs, dim3(nblocks), dim3(256), pair_lds, s, d);
 }
 
 }  // namespace trj

@@ -34,7 +34,8 @@ EXPORTS = [
     "trj_sq_init", "trj_sq_plan", "trj_rdf_plan", "trj_frame", "trj_frame_device", "trj_sync",
     "trj_rdf_counts", "trj_rdf_gmix", "trj_rdf_first_minimum", "trj_sq_shells", "trj_sq_columns",
     "trj_sq_rho", "trj_reset", "trj_last_timings", "trj_timing_totals", "trj_timing_reset",
-    "trj_device_bytes",
+    "trj_device_bytes", "trj_batch_create", "trj_batch_destroy", "trj_batch_last_error",
+    "trj_batch_layout", "trj_batch_run", "trj_batch_autotune",
 ]
 
 
@@ -59,6 +60,17 @@ class SqInfo(C.Structure):
     _fields_ = [("dq", C.c_double), ("fk", C.c_double * 3), ("qmin_eff", C.c_double),
                 ("nqmax", C.c_int32), ("nqmin", C.c_int32), ("lmax", C.c_int32 * 3),
                 ("nvec_grid", C.c_int64), ("nvec_axis", C.c_int64), ("box_ext", C.c_int64 * 3)]
+
+
+class BatchFrame(C.Structure):
+    _fields_ = [("n", C.c_int64), ("nsp", C.c_int32), ("periodic", C.c_int32 * 3),
+                ("box", C.c_double * 3)]
+
+
+class BatchParams(C.Structure):
+    _fields_ = [("ndim", C.c_int32), ("deltar", C.c_double), ("rcrdf", C.c_double),
+                ("frames_per_launch", C.c_int32), ("waves_per_cell", C.c_int32),
+                ("workgroups", C.c_int32), ("use_graph", C.c_int32)]
 
 
 _lib = None
@@ -104,8 +116,20 @@ def _declare(L):
     L.trj_timing_reset.argtypes = [vp]
     L.trj_device_bytes.argtypes = [vp]
     L.trj_device_bytes.restype = C.c_int64
+    L.trj_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(BatchParams)]
+    L.trj_batch_destroy.argtypes = [vp]
+    L.trj_batch_destroy.restype = None
+    L.trj_batch_last_error.argtypes = [vp]
+    L.trj_batch_last_error.restype = C.c_char_p
+    L.trj_batch_layout.argtypes = [C.POINTER(BatchParams), C.c_int32, C.POINTER(BatchFrame), i32, i32,
+                                   i64, i64]
+    L.trj_batch_run.argtypes = [vp, C.c_int32, C.POINTER(BatchFrame), vp, C.c_int32, vp, i64, d]
+    L.trj_batch_autotune.argtypes = [vp, C.c_int32, C.POINTER(BatchFrame), vp, C.c_int32, vp,
+                                     C.POINTER(BatchParams)]
+    no_int = ("trj_destroy", "trj_last_error", "trj_device_bytes", "trj_batch_destroy",
+              "trj_batch_last_error")
     for name in EXPORTS:
-        if name not in ("trj_destroy", "trj_last_error", "trj_device_bytes"):
+        if name not in no_int:
             getattr(L, name).restype = C.c_int
 
 
@@ -317,3 +341,88 @@ class Context:
 
     def device_bytes(self) -> int:
         return self._L.trj_device_bytes(self._h)
+
+
+def batch_params(ndim, deltar, rcrdf, frames_per_launch=0, waves_per_cell=0, workgroups=0,
+                 use_graph=True) -> BatchParams:
+    p = BatchParams()
+    p.ndim, p.deltar, p.rcrdf = ndim, float(deltar), float(rcrdf)
+    p.frames_per_launch, p.waves_per_cell, p.workgroups = frames_per_launch, waves_per_cell, workgroups
+    p.use_graph = int(bool(use_graph))
+    return p
+
+
+def batch_frames(frames) -> "C.Array":
+    """frames: iterable of (n, nsp, box, periodic)."""
+    frames = list(frames)
+    arr = (BatchFrame * len(frames))()
+    for k, (n, nsp, box, per) in enumerate(frames):
+        arr[k].n, arr[k].nsp = int(n), int(nsp)
+        b = _box3(box)
+        for a in range(3):
+            arr[k].box[a] = float(b[a])
+            arr[k].periodic[a] = int(bool(per[a]))
+    return arr
+
+
+def batch_layout(p: BatchParams, frames_arr):
+    nf = len(frames_arr)
+    ls = np.zeros(nf, dtype=np.int32)
+    nit = np.zeros(nf, dtype=np.int32)
+    off = np.zeros(nf, dtype=np.int64)
+    tot = C.c_int64()
+    r = lib().trj_batch_layout(C.byref(p), nf, frames_arr, _ptr(ls, C.c_int32), _ptr(nit, C.c_int32),
+                               _ptr(off, C.c_int64), C.byref(tot))
+    if r != TRJ_OK:
+        raise TrjError(r, "trj_batch_layout failed")
+    return ls, nit, off, tot.value
+
+
+class Batch:
+    """Batched frames of uneven size (trj_batch_*): per-frame pair histograms."""
+
+    def __init__(self, p: BatchParams, device: int = 0):
+        self._L = lib()
+        h = C.c_void_p()
+        rc = self._L.trj_batch_create(C.byref(h), device, C.byref(p))
+        if rc != TRJ_OK:
+            raise TrjError(rc, "trj_batch_create failed")
+        self._h = h
+        self.p = p
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.trj_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != TRJ_OK:
+            msg = self._L.trj_batch_last_error(self._h)
+            raise TrjError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def run(self, frames_arr, pos_ptr: int, dtype: int, itype_ptr: int, want_hist=True):
+        ls, nit, off, tot = batch_layout(self.p, frames_arr)
+        hist = np.zeros(max(tot, 1), dtype=np.int64) if want_hist else None
+        ms = C.c_double()
+        self._check(self._L.trj_batch_run(self._h, len(frames_arr), frames_arr, C.c_void_p(pos_ptr), dtype,
+                                          C.c_void_p(itype_ptr),
+                                          _ptr(hist, C.c_int64) if want_hist else None, C.byref(ms)),
+                    "trj_batch_run")
+        out = None
+        if want_hist:
+            out = [hist[off[k]:off[k] + nit[k] * ls[k]].reshape(nit[k], ls[k]) for k in range(len(off))]
+        return out, ms.value
+
+    def autotune(self, frames_arr, pos_ptr: int, dtype: int, itype_ptr: int) -> BatchParams:
+        chosen = BatchParams()
+        self._check(self._L.trj_batch_autotune(self._h, len(frames_arr), frames_arr, C.c_void_p(pos_ptr),
+                                               dtype, C.c_void_p(itype_ptr), C.byref(chosen)),
+                    "trj_batch_autotune")
+        self.p = chosen
+        return chosen
