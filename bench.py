@@ -171,14 +171,53 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_serving(args):
+    """serving: Poisson streams of RDF + S(Q) requests (bulk or z-slab frames, N
+    log-uniform in [1e4, 2e6], 1-4 species, float32, 64^3 half-space q-grid,
+    FP16x3 contraction) at two offered rates; the continuous-batching server
+    against the plain per-request path on the same arrivals."""
+    from trj_analysis_amd import serve as S
+    out_rates = {}
+    value = None
+    for rate in args.serve_rates:
+        jobs = S.make_jobs(args.serve_requests, rate=rate, seed=11, precision="fp16x3")
+        log(f"rate {rate}/s: {len(jobs)} requests, mean N {np.mean([len(j.frame.itype) for j in jobs]):.0f}")
+        S.serve_stream(jobs[:4], slots=args.serve_slots)  # warm
+        lat_s, res_s, span_s = S.serve_stream(jobs, slots=args.serve_slots)
+        lat_q, res_q, span_q = S.serve_sequential(jobs)
+        exact = all(np.array_equal(a[0], b[0]) for a, b in zip(res_s, res_q))
+        rel = max(float(np.max(np.abs(a[1] - b[1]) / np.maximum(np.abs(b[1]), 1e-300)))
+                  for a, b in zip(res_s, res_q))
+        out_rates[str(rate)] = {
+            "server": {"throughput_req_s": len(jobs) / span_s, "p50_ms": 1e3 * float(np.percentile(lat_s, 50)),
+                       "p99_ms": 1e3 * float(np.percentile(lat_s, 99))},
+            "per_request": {"throughput_req_s": len(jobs) / span_q,
+                            "p50_ms": 1e3 * float(np.percentile(lat_q, 50)),
+                            "p99_ms": 1e3 * float(np.percentile(lat_q, 99))},
+            "counts_bit_exact": exact, "sq_max_rel_diff": rel,
+        }
+        log(json.dumps(out_rates[str(rate)]))
+        value = len(jobs) / span_s
+    out = {"metric": METRIC, "value": value, "unit": "requests/s", "n_gpus": 1,
+           "steps": args.serve_requests, "warmup": 4, "ms_per_step": 1e3 / value, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "fp16x3", "data": "synthetic",
+           "config": {"workload": "serving", "slots": args.serve_slots, "rates": args.serve_rates,
+                      "rcrdf": S.RCRDF, "deltar": S.DELTAR, "q_grid": list(S.GRID)},
+           "rates": out_rates}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="lattice_262k",
-                    choices=["lattice_262k", "lj_binary_4k", "uniform_1m", "uneven_batch"])
+                    choices=["lattice_262k", "lj_binary_4k", "uniform_1m", "uneven_batch", "serving"])
     ap.add_argument("--batch-frames", type=int, default=512)
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[80.0, 240.0])
+    ap.add_argument("--serve-requests", type=int, default=150)
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--precision", default="fp64", choices=list(PEAK))
     ap.add_argument("--latency-frames", type=int, default=10)
     ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0)
@@ -191,6 +230,9 @@ def main():
 
     if args.workload == "uneven_batch":
         bench_batch(args)
+        return
+    if args.workload == "serving":
+        bench_serving(args)
         return
 
     import torch

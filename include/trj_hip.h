@@ -250,6 +250,42 @@ int trj_batch_run(trj_batch* b, int32_t nframes, const trj_batch_frame* fr, cons
 int trj_batch_autotune(trj_batch* b, int32_t nframes, const trj_batch_frame* fr, const void* pos_dev,
                        int32_t dtype, const int32_t* itype_dev, trj_batch_params* chosen);
 
+/* ------------------------------------------------------------------------
+ * Request server: continuous batching of independent analysis requests
+ * (RDF and/or S(Q) of one frame each).  Each slot is a context with its own
+ * stream and its own worker thread.  A request waits in a FIFO until a slot
+ * is free and its estimated device footprint fits in the budget next to the
+ * requests already running (a request alone is always admitted); its host
+ * preparation and device work then overlap with the other slots'.  Host
+ * arrays of a request must stay valid until trj_srv_poll reports it done. */
+typedef struct trj_srv trj_srv;
+
+typedef struct {
+  int64_t n;
+  const void* pos;      /* host, AoS stride ndim (rdf.ndim or sq.ndim) */
+  int32_t dtype;        /* TRJ_F32 / TRJ_F64 */
+  const int32_t* itype; /* host, 1..nsp */
+  double box[3];
+  uint32_t what;        /* TRJ_WANT_RDF | TRJ_WANT_SQ */
+  trj_rdf_params rdf;
+  trj_sq_params sq;
+} trj_request;
+
+int trj_srv_create(trj_srv** out, int device, int32_t slots, int64_t hbm_budget_bytes);
+void trj_srv_destroy(trj_srv* s);
+const char* trj_srv_last_error(const trj_srv* s);
+/* Host-only: device bytes a request needs while it runs (admission estimate). */
+int64_t trj_request_footprint(const trj_request* r);
+/* Queue a request; *ticket identifies it.  Admits queued requests whose turn has come. */
+int trj_srv_submit(trj_srv* s, const trj_request* r, int64_t* ticket);
+/* Admit what fits; report completion of ticket without blocking (*done = 0/1). */
+int trj_srv_poll(trj_srv* s, int64_t ticket, int32_t* done);
+/* Results of a completed request (any pointer may be NULL); releases its slot.
+ * counts: nit x lsmax (RDF); sqf, sqfp, nq as trj_sq_shells (S(Q)). */
+int trj_srv_result(trj_srv* s, int64_t ticket, int64_t* counts, double* sqf, double* sqfp, int64_t* nq);
+/* Requests queued, running, and device bytes reserved by running requests. */
+int trj_srv_stats(const trj_srv* s, int32_t* queued, int32_t* running, int64_t* reserved_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -312,12 +312,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     SqPlan& S = c->sq;
     const int ntiles = static_cast<int>(S.tiles.size());
     if (ntiles > 0) {
-      // K-splits: enough workgroups to fill the device, and for the FP32 /
-      // 16-bit kernels at most 8192 atoms per workgroup (the FP32 accumulator
-      // span; the splits are summed in FP64).
-      int ks = static_cast<int>((2048 + ntiles - 1) / ntiles);
-      ks = std::max(1, std::min<int>(ks, static_cast<int>(std::max<int64_t>(1, n / 8192))));
-      if (S.p.precision != TRJ_PREC_FP64) ks = std::max<int>(ks, static_cast<int>((n + 8191) / 8192));
+      const int ks = contraction_splits(ntiles, n, S.p.precision);
       c->ksplit = ks;
       const size_t rho_bytes = sizeof(double) * 2 * static_cast<size_t>(ks) * nsp * ntiles * S.tk.vec();
       if ((r = ensure(c, c->rho, rho_bytes))) return r;
@@ -360,6 +355,24 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
   c->last_nsp = nsp;
   return TRJ_OK;
 }
+
+}  // namespace
+
+namespace trj {
+hipStream_t context_stream(trj_ctx* c) { return c->stream; }
+
+// K-splits of the contraction: enough workgroups to fill the device, and for
+// the FP32 / 16-bit kernels at most 8192 atoms per workgroup (the FP32
+// accumulator span; the splits are summed in FP64).
+int contraction_splits(int ntiles, int64_t n, int precision) {
+  int ks = static_cast<int>((2048 + ntiles - 1) / ntiles);
+  ks = std::max(1, std::min<int>(ks, static_cast<int>(std::max<int64_t>(1, n / 8192))));
+  if (precision != TRJ_PREC_FP64) ks = std::max<int>(ks, static_cast<int>((n + 8191) / 8192));
+  return ks;
+}
+}  // namespace trj
+
+namespace {
 
 int read_ntype(trj_ctx* c, std::vector<int64_t>& ntype) {
   const int nsp = c->last_nsp;

@@ -79,6 +79,9 @@ struct SqPlan {
 // sq.cuf:156-200 (REF) or the half-space grid; fills everything above.
 int sq_plan(const trj_sq_params& p, const double box[3], SqPlan* out, std::string* err);
 
+// K-splits of the S(Q) contraction for ntiles tiles and n atoms.
+int contraction_splits(int ntiles, int64_t n, int precision);
+
 // ---------------------------------------------------------------- kernels
 struct DevFrame;  // defined in trj_ctx.hip
 

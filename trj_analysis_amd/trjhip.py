@@ -35,7 +35,9 @@ EXPORTS = [
     "trj_rdf_counts", "trj_rdf_gmix", "trj_rdf_first_minimum", "trj_sq_shells", "trj_sq_columns",
     "trj_sq_rho", "trj_reset", "trj_last_timings", "trj_timing_totals", "trj_timing_reset",
     "trj_device_bytes", "trj_batch_create", "trj_batch_destroy", "trj_batch_last_error",
-    "trj_batch_layout", "trj_batch_run", "trj_batch_autotune",
+    "trj_batch_layout", "trj_batch_run", "trj_batch_autotune", "trj_srv_create", "trj_srv_destroy",
+    "trj_srv_last_error", "trj_request_footprint", "trj_srv_submit", "trj_srv_poll", "trj_srv_result",
+    "trj_srv_stats",
 ]
 
 
@@ -71,6 +73,11 @@ class BatchParams(C.Structure):
     _fields_ = [("ndim", C.c_int32), ("deltar", C.c_double), ("rcrdf", C.c_double),
                 ("frames_per_launch", C.c_int32), ("waves_per_cell", C.c_int32),
                 ("workgroups", C.c_int32), ("use_graph", C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("n", C.c_int64), ("pos", C.c_void_p), ("dtype", C.c_int32), ("itype", C.c_void_p),
+                ("box", C.c_double * 3), ("what", C.c_uint32), ("rdf", RdfParams), ("sq", SqParams)]
 
 
 _lib = None
@@ -126,8 +133,19 @@ def _declare(L):
     L.trj_batch_run.argtypes = [vp, C.c_int32, C.POINTER(BatchFrame), vp, C.c_int32, vp, i64, d]
     L.trj_batch_autotune.argtypes = [vp, C.c_int32, C.POINTER(BatchFrame), vp, C.c_int32, vp,
                                      C.POINTER(BatchParams)]
+    L.trj_srv_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int32, C.c_int64]
+    L.trj_srv_destroy.argtypes = [vp]
+    L.trj_srv_destroy.restype = None
+    L.trj_srv_last_error.argtypes = [vp]
+    L.trj_srv_last_error.restype = C.c_char_p
+    L.trj_request_footprint.argtypes = [C.POINTER(Request)]
+    L.trj_request_footprint.restype = C.c_int64
+    L.trj_srv_submit.argtypes = [vp, C.POINTER(Request), i64]
+    L.trj_srv_poll.argtypes = [vp, C.c_int64, i32]
+    L.trj_srv_result.argtypes = [vp, C.c_int64, i64, d, d, i64]
+    L.trj_srv_stats.argtypes = [vp, i32, i32, i64]
     no_int = ("trj_destroy", "trj_last_error", "trj_device_bytes", "trj_batch_destroy",
-              "trj_batch_last_error")
+              "trj_batch_last_error", "trj_srv_destroy", "trj_srv_last_error", "trj_request_footprint")
     for name in EXPORTS:
         if name not in no_int:
             getattr(L, name).restype = C.c_int
@@ -426,3 +444,94 @@ class Batch:
                     "trj_batch_autotune")
         self.p = chosen
         return chosen
+
+
+def request(pos: np.ndarray, itype: np.ndarray, box, what=TRJ_WANT_RDF | TRJ_WANT_SQ, rdf=None,
+            sq=None) -> "tuple[Request, tuple]":
+    """A trj_request over host arrays; returns (request, keepalive)."""
+    pos = np.ascontiguousarray(pos)
+    if pos.dtype not in (np.float32, np.float64):
+        pos = pos.astype(np.float64)
+    it = np.ascontiguousarray(itype, dtype=np.int32)
+    r = Request()
+    r.n = len(it)
+    r.pos = pos.ctypes.data
+    r.dtype = TRJ_F32 if pos.dtype == np.float32 else TRJ_F64
+    r.itype = it.ctypes.data
+    b = _box3(box)
+    for k in range(3):
+        r.box[k] = float(b[k])
+    r.what = what
+    if rdf is not None:
+        r.rdf = rdf
+    if sq is not None:
+        r.sq = sq
+    return r, (pos, it)
+
+
+def request_footprint(r: Request) -> int:
+    return lib().trj_request_footprint(C.byref(r))
+
+
+class Server:
+    """Continuous-batching request server (trj_srv_*)."""
+
+    def __init__(self, slots: int = 4, budget_bytes: int = 0, device: int = 0):
+        self._L = lib()
+        h = C.c_void_p()
+        rc = self._L.trj_srv_create(C.byref(h), device, slots, budget_bytes)
+        if rc != TRJ_OK:
+            raise TrjError(rc, "trj_srv_create failed")
+        self._h = h
+        self._keep = {}
+        self._req = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.trj_srv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != TRJ_OK:
+            msg = self._L.trj_srv_last_error(self._h)
+            raise TrjError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def submit(self, req: Request, keepalive=None) -> int:
+        t = C.c_int64()
+        self._check(self._L.trj_srv_submit(self._h, C.byref(req), C.byref(t)), "trj_srv_submit")
+        self._keep[t.value] = (req, keepalive)
+        self._req[t.value] = req
+        return t.value
+
+    def poll(self, ticket: int) -> bool:
+        d = C.c_int32()
+        self._check(self._L.trj_srv_poll(self._h, ticket, C.byref(d)), "trj_srv_poll")
+        return bool(d.value)
+
+    def result(self, ticket: int, lsmax: int = 0, nit: int = 0, nqmax: int = -1, nsp: int = 0):
+        counts = np.zeros((nit, lsmax), dtype=np.int64) if lsmax > 0 else None
+        sqf = sqfp = nq = None
+        if nqmax >= 0:
+            sqf = np.zeros(nqmax + 1)
+            sqfp = np.zeros((nsp, nqmax + 1))
+            nq = np.zeros(nqmax + 1, dtype=np.int64)
+        self._check(self._L.trj_srv_result(self._h, ticket,
+                                           _ptr(counts, C.c_int64) if counts is not None else None,
+                                           _ptr(sqf, C.c_double) if sqf is not None else None,
+                                           _ptr(sqfp, C.c_double) if sqfp is not None else None,
+                                           _ptr(nq, C.c_int64) if nq is not None else None),
+                    "trj_srv_result")
+        self._keep.pop(ticket, None)
+        self._req.pop(ticket, None)
+        return counts, sqf, sqfp, nq
+
+    def stats(self):
+        q, r, b = C.c_int32(), C.c_int32(), C.c_int64()
+        self._check(self._L.trj_srv_stats(self._h, C.byref(q), C.byref(r), C.byref(b)), "trj_srv_stats")
+        return q.value, r.value, b.value
