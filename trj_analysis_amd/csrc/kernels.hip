@@ -15,9 +15,9 @@
 // v_mfma_f64_16x16x4_f64 (4 real products per complex product).  Phases come
 // from an exact FP64 reduction of n*u (u = x/L in turns, with the FMA error
 // term) and sincospi, extended by an 8-step angle-addition recurrence.
-// MFMA accumulators are folded every 32 atoms into FP64 sums with TwoSum
+// MFMA accumulators are folded every 128 atoms into FP64 sums with TwoSum
 // compensation, so the rounding error of a 1e5-atom sum stays near that of
-// one 32-term block even when long runs of atoms add coherently.
+// one 128-term block even when long runs of atoms add coherently.
 //
 // Compiled with -ffp-contract=off: every FP64 operation of the RDF pair rule
 // rounds once, in the order written, like the oracle.
@@ -416,6 +416,7 @@ __global__ __launch_bounds__(256) void k_rdf_merge(const uint32_t* __restrict__ 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int KC = 32;            // atoms staged per chunk
+constexpr int kFoldChunks = 4;    // fold MFMA accumulators every 4 chunks (128 atoms)
 // FP64 tile: 8 kx x 8 ky x 32 kz (M = 64 (kx,ky) pairs, N = 32 kz).
 constexpr int TKX = 8, TKY = 8, TKZ = 32, TILE_VEC = TKX * TKY * TKZ;
 
@@ -474,11 +475,12 @@ __global__ __launch_bounds__(256) void k_sq_contract_f64(ContractArgs a) {
     double sumR[2][4], sumI[2][4], cmpR[2][4], cmpI[2][4];
     for (int nb = 0; nb < 2; ++nb)
       for (int r = 0; r < 4; ++r) sumR[nb][r] = sumI[nb][r] = cmpR[nb][r] = cmpI[nb][r] = 0.0;
+    for (int nb = 0; nb < 2; ++nb) {
+      accR[nb] = d4{0.0, 0.0, 0.0, 0.0};
+      accI[nb] = accR[nb];
+    }
+    int chunks = 0;
     for (int c0 = b0; c0 < b1; c0 += KC) {
-      for (int nb = 0; nb < 2; ++nb) {
-        accR[nb] = d4{0.0, 0.0, 0.0, 0.0};
-        accI[nb] = accR[nb];
-      }
       __syncthreads();
       if (t < KC * 6) {
         const int ai = c0 + st_at;
@@ -523,11 +525,17 @@ __global__ __launch_bounds__(256) void k_sq_contract_f64(ContractArgs a) {
         accI[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bz1.y, accI[1], 0, 0, 0);
         accI[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bz1.x, accI[1], 0, 0, 0);
       }
-      for (int nb = 0; nb < 2; ++nb)
-        for (int r = 0; r < 4; ++r) {
-          two_sum_acc(sumR[nb][r], cmpR[nb][r], accR[nb][r]);
-          two_sum_acc(sumI[nb][r], cmpI[nb][r], accI[nb][r]);
+      if (++chunks == kFoldChunks || c0 + KC >= b1) {
+        chunks = 0;
+        for (int nb = 0; nb < 2; ++nb) {
+          for (int r = 0; r < 4; ++r) {
+            two_sum_acc(sumR[nb][r], cmpR[nb][r], accR[nb][r]);
+            two_sum_acc(sumI[nb][r], cmpI[nb][r], accI[nb][r]);
+          }
+          accR[nb] = d4{0.0, 0.0, 0.0, 0.0};
+          accI[nb] = accR[nb];
         }
+      }
     }
     for (int nb = 0; nb < 2; ++nb)
       for (int r = 0; r < 4; ++r) {
