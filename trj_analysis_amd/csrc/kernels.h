@@ -26,7 +26,8 @@ struct IngestArgs {
 void launch_ingest(const IngestArgs& a, hipStream_t s);
 
 // Stable partition of atoms by species: per-block counts (species-major), a
-// scan, then a scatter of the S(Q) turns u = x / Lq into species order.
+// scan, then a scatter of the S(Q) turns u = x / Lq (hi + lo) and of the unit
+// steps e^{2 pi i u} into species order: u holds 12 * n doubles.
 void launch_species_count(const int32_t* itype, int64_t n, int nsp, int32_t* blk, int nblk,
                           hipStream_t s);
 void launch_exclusive_scan(const int32_t* in, int32_t* out, int64_t count, hipStream_t s);
@@ -73,7 +74,8 @@ void launch_rdf_merge(const uint32_t* partial, int nslices, int nbins, int64_t* 
 
 // S(Q) contraction: rho_s(q) over the active tiles of the dense q-box.
 struct ContractArgs {
-  const double* u;        // species-sorted turns x/L as hi[3][n] then lo[3][n]
+  const double* u;        // species-sorted turns x/L as hi[3][n], lo[3][n], then
+                          // e^{2 pi i u} as cos[3][n], sin[3][n]
   const int32_t* sp_off;  // nsp+1
   const int32_t* tiles;   // 4 ints per tile: kx0, ky0, kz0, pad
   int ntiles, ksplit, nsp, ndim;

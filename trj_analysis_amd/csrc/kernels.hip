@@ -124,6 +124,15 @@ __global__ void k_species_offsets(const int32_t* __restrict__ blkoff, const int3
   sp_off[nsp] = blkoff[last] + blk[last];
 }
 
+// Phase in turns, frac(n*(uh + ul)), reduced exactly: the product's rounding
+// error (FMA) and the low part of u are added after the integer part is gone.
+__device__ inline double frac_turns(int nk, double uh, double ul) {
+  const double dn = static_cast<double>(nk);
+  const double hi = dn * uh;
+  const double lo = fma(dn, uh, -hi) + dn * ul;
+  return (hi - rint(hi)) + lo;
+}
+
 __global__ __launch_bounds__(256) void k_species_scatter(const int32_t* __restrict__ itype, int64_t n,
                                                          int nsp, const int32_t* __restrict__ blkoff,
                                                          int nblk, const double* __restrict__ pos,
@@ -146,7 +155,8 @@ __global__ __launch_bounds__(256) void k_species_scatter(const int32_t* __restri
   for (int ww = 0; ww < w; ++ww) off += cnt[ww][s];
   off += __popcll(mine & ((1ull << lane) - 1ull));
   // Turns u = x/L as an unevaluated sum hi + lo: hi rounded, lo = (x - hi*L)/L
-  // with the remainder exact through the FMA.
+  // with the remainder exact through the FMA; then the unit step e^{2 pi i u}
+  // of the phase recurrences as cos, sin.
   const double lq[3] = {lq0, lq1, lq2};
   for (int k = 0; k < 3; ++k) {
     const double x = (k < ndim) ? pos[k * n + i] : 0.0;
@@ -154,6 +164,10 @@ __global__ __launch_bounds__(256) void k_species_scatter(const int32_t* __restri
     const double lo = fma(-hi, lq[k], x) / lq[k];
     u[k * n + off] = hi;
     u[(3 + k) * n + off] = lo;
+    double sn, cs;
+    sincospi(2.0 * frac_turns(1, hi, lo), &sn, &cs);
+    u[(6 + k) * n + off] = cs;
+    u[(9 + k) * n + off] = sn;
   }
 }
 
@@ -416,24 +430,20 @@ __global__ __launch_bounds__(256) void k_rdf_merge(const uint32_t* __restrict__ 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int KC = 32;            // atoms staged per chunk
-constexpr int kFoldChunks = 4;    // fold MFMA accumulators every 4 chunks (128 atoms)
+constexpr int kFoldChunks = 8;    // fold MFMA accumulators every 8 chunks (256 atoms)
 // FP64 tile: 8 kx x 8 ky x 32 kz (M = 64 (kx,ky) pairs, N = 32 kz).
 constexpr int TKX = 8, TKY = 8, TKZ = 32, TILE_VEC = TKX * TKY * TKZ;
+// LDS record of one atom's phases: x at 0..7, y at 8..15, z at 16..47.  The
+// stride of 49 slots (16 B each) puts consecutive atoms one slot apart in the
+// bank cycle, so staging writes (lane = atom) and operand reads are
+// conflict-free.
+constexpr int kRec = 49;
 
-// Phase in turns, frac(n*(uh + ul)), reduced exactly: the product's rounding
-// error (FMA) and the low part of u are added after the integer part is gone.
-__device__ inline double frac_turns(int nk, double uh, double ul) {
-  const double dn = static_cast<double>(nk);
-  const double hi = dn * uh;
-  const double lo = fma(dn, uh, -hi) + dn * ul;
-  return (hi - rint(hi)) + lo;
-}
-
-// e^{2 pi i n u} for n = n0 .. n0+7 into dst (anchor + angle-addition steps).
-__device__ inline void phase8(double uh, double ul, int n0, double2* dst) {
-  double s, c, ss, cs;
+// e^{2 pi i n u} for n = n0 .. n0+7 into dst: the anchor n0, then
+// angle-addition steps by e^{2 pi i u} = cs + i ss.
+__device__ inline void phase8(double uh, double ul, int n0, double cs, double ss, double2* dst) {
+  double s, c;
   sincospi(2.0 * frac_turns(n0, uh, ul), &s, &c);
-  sincospi(2.0 * frac_turns(1, uh, ul), &ss, &cs);
   double re = c, im = s;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
@@ -445,103 +455,135 @@ __device__ inline void phase8(double uh, double ul, int n0, double2* dst) {
   }
 }
 
-// s + c += x with Knuth's TwoSum: s keeps the rounded sum, c the exact error.
-__device__ inline void two_sum_acc(double& s, double& c, double x) {
+// s + c += x with Knuth's TwoSum: s keeps the rounded sum, c the error, kept
+// in FP32 (it only has to hold the few low-order bits lost per fold).
+__device__ inline void two_sum_acc(double& s, float& c, double x) {
   const double t = s + x;
   const double bp = t - s;
   const double e = (s - (t - bp)) + (x - bp);
   s = t;
-  c += e;
+  c += static_cast<float>(e);
 }
 
-__global__ __launch_bounds__(256) void k_sq_contract_f64(ContractArgs a) {
-  __shared__ double2 phx[KC][TKX];
-  __shared__ double2 phy[KC][TKY];
-  __shared__ double2 phz[KC][TKZ];
+// One 32-atom chunk of MFMAs for one wave: rows w*16.. of the tile, the
+// 16-wide kz column blocks selected by CB (bit 0: kz 0..15, bit 1: 16..31).
+// Complex products in three real ones: t1 = Ar Br, t2 = Ai Bi,
+// t3 = (Ar + Ai)(Br + Bi); Re = t1 - t2, Im = t3 - t1 - t2.
+template <int CB>
+__device__ inline void contract_chunk_f64(const double2 (*ph)[kRec], int kq, int mx, int my, int i16,
+                                          d4 (&t1)[2], d4 (&t2)[2], d4 (&t3)[2]) {
+#pragma unroll
+  for (int kk = 0; kk < KC / 4; ++kk) {
+    const double2* rec = ph[kk * 4 + kq];
+    const double2 ex = rec[mx];
+    const double2 ey = rec[8 + my];
+    const double ar = fma(ex.x, ey.x, -(ex.y * ey.y));
+    const double ai = fma(ex.x, ey.y, ex.y * ey.x);
+    const double as = ar + ai;
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+      if (!((CB >> nb) & 1)) continue;
+      const double2 bz = rec[16 + 16 * nb + i16];
+      t1[nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bz.x, t1[nb], 0, 0, 0);
+      t2[nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bz.y, t2[nb], 0, 0, 0);
+      t3[nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(as, bz.x + bz.y, t3[nb], 0, 0, 0);
+    }
+  }
+}
+
+// rho_s(q) for one tile and one atom split: a complex GEMM over atoms whose
+// operands are generated per 32-atom chunk in LDS (double-buffered: the next
+// chunk is staged while this one is contracted).  Waves 0-2 stage one
+// 8-value segment per lane (lane = atom; wave 0: x | y, waves 1-2: z); the
+// next chunk's turns are prefetched into registers a chunk ahead.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_sq_contract_f64(
+    ContractArgs a) {
+  __shared__ double2 ph[2][KC][kRec];
   const int tile = blockIdx.x % a.ntiles;
   const int split = blockIdx.x / a.ntiles;
   const int kx0 = a.tiles[4 * tile], ky0 = a.tiles[4 * tile + 1], kz0 = a.tiles[4 * tile + 2];
   const int t = threadIdx.x, w = t / kWave, lane = t & (kWave - 1);
   const int i16 = lane & 15, kq = lane >> 4;
   const int m = w * 16 + i16, mx = m / TKY, my = m % TKY;
-  // Phase-staging role of this thread: atom at, segment seg (0: x, 1: y, 2-5: z).
-  const int st_at = t / 6, st_seg = t % 6;
+  // Column blocks of this wave's 16 rows that hold summed vectors (uniform).
+  const int cb = (a.tiles[4 * tile + 3] >> (2 * w)) & 3;
+  // Staging role: atom st_at, segment st_seg (0: x, 1: y, 2-5: z 8-blocks).
+  const bool stager = t < KC * 6;
+  const int st_at = t % KC, st_seg = t / KC;
+  const int st_ax = st_seg < 2 ? st_seg : 2;
+  const int st_n0 = st_seg == 0 ? kx0 : (st_seg == 1 ? ky0 : kz0 + 8 * (st_seg - 2));
+  const int st_slot = st_seg < 2 ? 8 * st_seg : 16 + 8 * (st_seg - 2);
   for (int s = 0; s < a.nsp; ++s) {
     const int a0 = a.sp_off[s], a1 = a.sp_off[s + 1];
     const int64_t len = a1 - a0;
     const int b0 = a0 + static_cast<int>(len * split / a.ksplit);
     const int b1 = a0 + static_cast<int>(len * (split + 1) / a.ksplit);
-    d4 accR[2], accI[2];
-    double sumR[2][4], sumI[2][4], cmpR[2][4], cmpI[2][4];
-    for (int nb = 0; nb < 2; ++nb)
-      for (int r = 0; r < 4; ++r) sumR[nb][r] = sumI[nb][r] = cmpR[nb][r] = cmpI[nb][r] = 0.0;
+    d4 t1[2], t2[2], t3[2];
+    double sumR[2][4], sumI[2][4];
+    float cmpR[2][4], cmpI[2][4];
     for (int nb = 0; nb < 2; ++nb) {
-      accR[nb] = d4{0.0, 0.0, 0.0, 0.0};
-      accI[nb] = accR[nb];
-    }
-    int chunks = 0;
-    for (int c0 = b0; c0 < b1; c0 += KC) {
-      __syncthreads();
-      if (t < KC * 6) {
-        const int ai = c0 + st_at;
-        double2* dst;
-        int n0, ax;
-        if (st_seg == 0) {
-          dst = phx[st_at];
-          n0 = kx0;
-          ax = 0;
-        } else if (st_seg == 1) {
-          dst = phy[st_at];
-          n0 = ky0;
-          ax = 1;
-        } else {
-          dst = phz[st_at] + 8 * (st_seg - 2);
-          n0 = kz0 + 8 * (st_seg - 2);
-          ax = 2;
-        }
-        if (ai < b1) {
-          phase8(a.u[ax * a.n + ai], a.u[(3 + ax) * a.n + ai], n0, dst);
-        } else {
-#pragma unroll
-          for (int k = 0; k < 8; ++k) dst[k] = make_double2(0.0, 0.0);
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int kk = 0; kk < KC / 4; ++kk) {
-        const int at = kk * 4 + kq;
-        const double2 ex = phx[at][mx];
-        const double2 ey = phy[at][my];
-        const double ar = fma(ex.x, ey.x, -(ex.y * ey.y));
-        const double ai = fma(ex.x, ey.y, ex.y * ey.x);
-        const double2 bz0 = phz[at][i16];
-        const double2 bz1 = phz[at][16 + i16];
-        accR[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bz0.x, accR[0], 0, 0, 0);
-        accR[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bz0.y, accR[0], 0, 0, 0);
-        accI[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bz0.y, accI[0], 0, 0, 0);
-        accI[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bz0.x, accI[0], 0, 0, 0);
-        accR[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bz1.x, accR[1], 0, 0, 0);
-        accR[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bz1.y, accR[1], 0, 0, 0);
-        accI[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bz1.y, accI[1], 0, 0, 0);
-        accI[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bz1.x, accI[1], 0, 0, 0);
-      }
-      if (++chunks == kFoldChunks || c0 + KC >= b1) {
-        chunks = 0;
-        for (int nb = 0; nb < 2; ++nb) {
-          for (int r = 0; r < 4; ++r) {
-            two_sum_acc(sumR[nb][r], cmpR[nb][r], accR[nb][r]);
-            two_sum_acc(sumI[nb][r], cmpI[nb][r], accI[nb][r]);
-          }
-          accR[nb] = d4{0.0, 0.0, 0.0, 0.0};
-          accI[nb] = accR[nb];
-        }
-      }
-    }
-    for (int nb = 0; nb < 2; ++nb)
+      t1[nb] = d4{0.0, 0.0, 0.0, 0.0};
+      t2[nb] = t1[nb];
+      t3[nb] = t1[nb];
       for (int r = 0; r < 4; ++r) {
-        sumR[nb][r] += cmpR[nb][r];
-        sumI[nb][r] += cmpI[nb][r];
+        sumR[nb][r] = sumI[nb][r] = 0.0;
+        cmpR[nb][r] = cmpI[nb][r] = 0.0f;
       }
+    }
+    double uh = 0.0, ul = 0.0, uc = 1.0, us = 0.0;
+    if (stager && b0 + st_at < b1) {
+      uh = a.u[st_ax * a.n + b0 + st_at];
+      ul = a.u[(3 + st_ax) * a.n + b0 + st_at];
+      uc = a.u[(6 + st_ax) * a.n + b0 + st_at];
+      us = a.u[(9 + st_ax) * a.n + b0 + st_at];
+    }
+    // Write chunk c0's phases into buffer buf from the prefetched turns, then
+    // prefetch the turns of chunk c0 + KC.
+    auto stage = [&](int buf, int c0) {
+      if (!stager) return;
+      double2* dst = ph[buf][st_at] + st_slot;
+      if (c0 + st_at < b1) {
+        phase8(uh, ul, st_n0, uc, us, dst);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) dst[k] = make_double2(0.0, 0.0);
+      }
+      const int nx = c0 + KC + st_at;
+      if (nx < b1) {
+        uh = a.u[st_ax * a.n + nx];
+        ul = a.u[(3 + st_ax) * a.n + nx];
+        uc = a.u[(6 + st_ax) * a.n + nx];
+        us = a.u[(9 + st_ax) * a.n + nx];
+      }
+    };
+    __syncthreads();  // the previous species' readers are done with both buffers
+    if (b0 < b1) stage(0, b0);
+    __syncthreads();
+    int it = 0;
+    for (int g0 = b0; g0 < b1; g0 += KC * kFoldChunks) {
+      const int g1 = min(b1, g0 + KC * kFoldChunks);
+      for (int c0 = g0; c0 < g1; c0 += KC, ++it) {
+        const int cur = it & 1;
+        if (c0 + KC < b1) stage(cur ^ 1, c0 + KC);
+        if (cb == 3)
+          contract_chunk_f64<3>(ph[cur], kq, mx, my, i16, t1, t2, t3);
+        else if (cb == 1)
+          contract_chunk_f64<1>(ph[cur], kq, mx, my, i16, t1, t2, t3);
+        else if (cb == 2)
+          contract_chunk_f64<2>(ph[cur], kq, mx, my, i16, t1, t2, t3);
+        __syncthreads();
+      }
+      for (int nb = 0; nb < 2; ++nb) {
+        if (!((cb >> nb) & 1)) continue;
+        for (int r = 0; r < 4; ++r) {
+          two_sum_acc(sumR[nb][r], cmpR[nb][r], t1[nb][r] - t2[nb][r]);
+          two_sum_acc(sumI[nb][r], cmpI[nb][r], t3[nb][r] - t1[nb][r] - t2[nb][r]);
+        }
+        t1[nb] = d4{0.0, 0.0, 0.0, 0.0};
+        t2[nb] = t1[nb];
+        t3[nb] = t1[nb];
+      }
+    }
     // D layout of v_mfma_f64_16x16x4_f64: register r of a lane holds row
     // 4*r + lane/16, column lane%16 (the FP32/16-bit 16x16 forms hold row
     // 4*(lane/16) + r instead).
@@ -552,7 +594,8 @@ __global__ __launch_bounds__(256) void k_sq_contract_f64(ContractArgs a) {
       for (int r = 0; r < 4; ++r) {
         const int mrow = w * 16 + 4 * r + kq;
         const int v = ((mrow / TKY) * TKY + (mrow % TKY)) * TKZ + nb * 16 + i16;
-        double2 val = make_double2(sumR[nb][r], sumI[nb][r]);
+        const double2 val = make_double2(sumR[nb][r] + static_cast<double>(cmpR[nb][r]),
+                                         sumI[nb][r] + static_cast<double>(cmpI[nb][r]));
         reinterpret_cast<double2*>(a.rho)[base + v] = val;
       }
     }

@@ -305,6 +305,7 @@ int sq_plan(const trj_sq_params& p_in, const double box[3], SqPlan* out, std::st
       for (int y0 = P.kmin[1]; y0 <= P.kmax[1]; y0 += TKY)
         for (int z0 = P.kmin[2]; z0 <= P.kmax[2]; z0 += TKZ) {
           int count = 0;
+          uint32_t blocks = 0;
           for (int i = 0; i < TKX; ++i)
             for (int j = 0; j < TKY; ++j)
               for (int k = 0; k < TKZ; ++k) {
@@ -324,10 +325,11 @@ int sq_plan(const trj_sq_params& p_in, const double box[3], SqPlan* out, std::st
                 if (s >= 0) {
                   ++count;
                   P.nq_summed[s] += 1;
+                  blocks |= 1u << (((i * TKY + j) / 16) * (TKZ / 16) + k / 16);
                 }
               }
           if (count > 0) {
-            P.tiles.push_back(SqTile{x0, y0, z0, 0});
+            P.tiles.push_back(SqTile{x0, y0, z0, static_cast<int32_t>(blocks)});
             P.tile_shell.insert(P.tile_shell.end(), sh.begin(), sh.end());
           }
         }

@@ -221,7 +221,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
   if (do_sq) {
     // Species-contiguous turns u = x / Lq for the contraction (Lq: the box the
     // q-grid was built on, sq.cuf:189).
-    if ((r = ensure(c, c->u, sizeof(double) * 6 * n))) return r;
+    if ((r = ensure(c, c->u, sizeof(double) * 12 * n))) return r;
     const double lq[3] = {c->sq.box[0], c->sq.box[1], c->sq.box[2]};
     launch_species_scatter(itype_dev, n, nsp, P<int32_t>(c->blkoff), nblk, P<double>(c->pos), lq, ndim,
                            P<double>(c->u), c->stream);

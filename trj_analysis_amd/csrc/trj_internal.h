@@ -56,7 +56,10 @@ inline TileShape tile_shape(int precision) {
 }
 
 struct SqTile {
-  int32_t kx0, ky0, kz0, pad;
+  // blocks: bit (rb * (tz / 16) + cb) is set when row block rb (16 rows of the
+  // tile's (kx, ky) rows, kx major) and 16-wide kz column block cb hold at
+  // least one summed vector.
+  int32_t kx0, ky0, kz0, blocks;
 };
 
 struct SqPlan {
