@@ -133,8 +133,8 @@ int64_t trj_request_footprint(const trj_request* r) {
     if (trj_sq_plan(&r->sq, r->box, &info, nullptr) == TRJ_OK) {
       const trj::TileShape tk = trj::tile_shape(r->sq.precision);
       const int64_t tiles = (info.nvec_grid + tk.vec() - 1) / tk.vec() + 1;
-      const int ks = trj::contraction_splits(static_cast<int>(tiles), n, r->sq.precision);
-      b += int64_t(ks) * r->sq.nsp * tiles * tk.vec() * 16 + n * 48;
+      const int ks = trj::contraction_splits(static_cast<int>(tiles), n, r->sq.precision, r->sq.nsp);
+      b += int64_t(ks) * r->sq.nsp * tiles * tk.vec() * 16 + n * 96;
     }
   }
   return b;

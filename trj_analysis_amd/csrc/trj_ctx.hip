@@ -312,7 +312,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     SqPlan& S = c->sq;
     const int ntiles = static_cast<int>(S.tiles.size());
     if (ntiles > 0) {
-      const int ks = contraction_splits(ntiles, n, S.p.precision);
+      const int ks = contraction_splits(ntiles, n, S.p.precision, nsp);
       c->ksplit = ks;
       const size_t rho_bytes = sizeof(double) * 2 * static_cast<size_t>(ks) * nsp * ntiles * S.tk.vec();
       if ((r = ensure(c, c->rho, rho_bytes))) return r;
@@ -364,11 +364,19 @@ hipStream_t context_stream(trj_ctx* c) { return c->stream; }
 // K-splits of the contraction: enough workgroups to fill the device, and for
 // the FP32 / 16-bit kernels at most 8192 atoms per workgroup (the FP32
 // accumulator span; the splits are summed in FP64).
-int contraction_splits(int ntiles, int64_t n, int precision) {
+int contraction_splits(int ntiles, int64_t n, int precision, int nsp) {
+  // At least 2048 workgroups when the atoms allow it, and atom ranges short
+  // enough that the workgroups running at one time share their turns in L2
+  // (32768 atoms x 96 B for FP64; 8192 for the FP32-accumulating kernels,
+  // which fold into FP64 per split).  The partial rho buffer (one per split)
+  // is kept within 4 GiB.
   int ks = static_cast<int>((2048 + ntiles - 1) / ntiles);
   ks = std::max(1, std::min<int>(ks, static_cast<int>(std::max<int64_t>(1, n / 8192))));
-  if (precision != TRJ_PREC_FP64) ks = std::max<int>(ks, static_cast<int>((n + 8191) / 8192));
-  return ks;
+  const int64_t span = precision == TRJ_PREC_FP64 ? 32768 : 8192;
+  ks = std::max<int>(ks, static_cast<int>((n + span - 1) / span));
+  const int64_t per_split = int64_t(16) * std::max(1, nsp) * std::max(1, ntiles) * tile_shape(precision).vec();
+  const int64_t cap = std::max<int64_t>(1, (int64_t(4) << 30) / per_split);
+  return static_cast<int>(std::min<int64_t>(ks, cap));
 }
 }  // namespace trj
 
