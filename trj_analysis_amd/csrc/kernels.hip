@@ -513,6 +513,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const int st_ax = st_seg < 2 ? st_seg : 2;
   const int st_n0 = st_seg == 0 ? kx0 : (st_seg == 1 ? ky0 : kz0 + 8 * (st_seg - 2));
   const int st_slot = st_seg < 2 ? 8 * st_seg : 16 + 8 * (st_seg - 2);
+  // Turn planes as uniform bases plus one 32-bit lane offset (3 n < 2^32 is
+  // checked by the launcher).
+  const double* u_hi = a.u;
+  const double* u_lo = a.u + 3 * a.n;
+  const double* u_c = a.u + 6 * a.n;
+  const double* u_s = a.u + 9 * a.n;
+  const uint32_t st_base = static_cast<uint32_t>(st_ax) * static_cast<uint32_t>(a.n);
   for (int s = 0; s < a.nsp; ++s) {
     const int a0 = a.sp_off[s], a1 = a.sp_off[s + 1];
     const int64_t len = a1 - a0;
@@ -532,10 +539,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
     double uh = 0.0, ul = 0.0, uc = 1.0, us = 0.0;
     if (stager && b0 + st_at < b1) {
-      uh = a.u[st_ax * a.n + b0 + st_at];
-      ul = a.u[(3 + st_ax) * a.n + b0 + st_at];
-      uc = a.u[(6 + st_ax) * a.n + b0 + st_at];
-      us = a.u[(9 + st_ax) * a.n + b0 + st_at];
+      const uint32_t o = st_base + static_cast<uint32_t>(b0 + st_at);
+      uh = u_hi[o];
+      ul = u_lo[o];
+      uc = u_c[o];
+      us = u_s[o];
     }
     // Write chunk c0's phases into buffer buf from the prefetched turns, then
     // prefetch the turns of chunk c0 + KC.
@@ -550,10 +558,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       }
       const int nx = c0 + KC + st_at;
       if (nx < b1) {
-        uh = a.u[st_ax * a.n + nx];
-        ul = a.u[(3 + st_ax) * a.n + nx];
-        uc = a.u[(6 + st_ax) * a.n + nx];
-        us = a.u[(9 + st_ax) * a.n + nx];
+        const uint32_t o = st_base + static_cast<uint32_t>(nx);
+        uh = u_hi[o];
+        ul = u_lo[o];
+        uc = u_c[o];
+        us = u_s[o];
       }
     };
     __syncthreads();  // the previous species' readers are done with both buffers
@@ -587,16 +596,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     // D layout of v_mfma_f64_16x16x4_f64: register r of a lane holds row
     // 4*r + lane/16, column lane%16 (the FP32/16-bit 16x16 forms hold row
     // 4*(lane/16) + r instead).
-    const size_t base = ((static_cast<size_t>(split) * a.nsp + s) * a.ntiles + tile) * TILE_VEC;
+    // Vector index within the tile: row mrow = w*16 + 4r + kq (= kx*8 + ky),
+    // column nb*16 + i16 (kz), i.e. mrow * TKZ + nb * 16 + i16.
+    double2* out = reinterpret_cast<double2*>(a.rho) +
+                   ((static_cast<size_t>(split) * a.nsp + s) * a.ntiles + tile) * TILE_VEC +
+                   ((w * 16 + kq) * TKZ + i16);
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int mrow = w * 16 + 4 * r + kq;
-        const int v = ((mrow / TKY) * TKY + (mrow % TKY)) * TKZ + nb * 16 + i16;
-        const double2 val = make_double2(sumR[nb][r] + static_cast<double>(cmpR[nb][r]),
-                                         sumI[nb][r] + static_cast<double>(cmpI[nb][r]));
-        reinterpret_cast<double2*>(a.rho)[base + v] = val;
+        out[4 * r * TKZ + nb * 16] = make_double2(sumR[nb][r] + static_cast<double>(cmpR[nb][r]),
+                                                  sumI[nb][r] + static_cast<double>(cmpI[nb][r]));
       }
     }
   }

@@ -176,6 +176,9 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     return fail(c, TRJ_EINVAL, "RDF and S(Q) were initialised with different ndim/nsp");
   if (n < 1) return fail(c, TRJ_EINVAL, "n must be positive");
   if (n > (int64_t(1) << 31) - 1024) return fail(c, TRJ_ELIMIT, "n above 2^31 is not supported");
+  // The S(Q) kernels address the turn planes with 32-bit offsets up to 3 n.
+  if (do_sq && n > int64_t(0xFFFFFFFFu) / 3)
+    return fail(c, TRJ_ELIMIT, "S(Q) with n above 1431655765 is not supported");
   if (dtype != TRJ_F32 && dtype != TRJ_F64) return fail(c, TRJ_EINVAL, "bad dtype");
   if (layout != TRJ_AOS && layout != TRJ_SOA) return fail(c, TRJ_EINVAL, "bad layout");
   if (int r = validate_box(c, ndim, box_in)) return r;
