@@ -894,6 +894,218 @@ __global__ __launch_bounds__(256) void k_sq_contract_lp(ContractArgs a) {
   }
 }
 
+// ---------------------------------------------------- 16-bit split kernels
+// Same tile and GEMM as k_sq_contract_lp (M = 64 kx, one 16-row block per
+// wavefront; N = 128 (ky,kz) pairs; K = 32 atoms per chunk), organised so that
+// the VALU work and the MFMAs of different wavefronts overlap:
+//  * each chunk's phases are staged once as FP32 complex records (ex[64],
+//    ey[4], ez[32] per atom) in LDS; a lane computes one 8-value segment: FP64
+//    reduction of the anchor, FP32 sincospi, FP32 angle addition by the
+//    frame's FP64 unit step;
+//  * the B operand ey*ez (shared by the four wavefronts) is formed and split
+//    into hi + lo once per chunk; each wavefront splits its own 16 rows of ex
+//    straight into registers (no A table in LDS);
+//  * chunk c+1 is staged while the MFMAs of chunk c run.
+constexpr int kLRec = 102;  // record stride in float2 (100 used): lane = atom
+                            // writes and lane = row reads are conflict-free
+
+typedef float f8 __attribute__((ext_vector_type(8)));
+
+// hi = round(x), lo = round(x - hi), eight values at a time (packed converts).
+template <class V>
+__device__ inline void split8(const f8& x, V& hi, V& lo) {
+  hi = __builtin_convertvector(x, V);
+  const f8 back = __builtin_convertvector(hi, f8);
+  lo = __builtin_convertvector(x - back, V);
+}
+
+// e^{2 pi i n u}, n = n0 .. n0+cnt-1, into dst as FP32 complex: anchor by
+// FP64 reduction and FP32 sincospi, steps by the FP64 unit step (c, s).
+__device__ inline void phase_rec(double uh, double ul, double c, double s, int n0, int cnt, float2* dst) {
+  float sn, cs;
+  sincospif(static_cast<float>(2.0 * frac_turns(n0, uh, ul)), &sn, &cs);
+  const float sc = static_cast<float>(c), ss = static_cast<float>(s);
+  float re = cs, im = sn;
+  for (int k = 0; k < cnt; ++k) {
+    dst[k] = make_float2(re, im);
+    const float nr = fmaf(re, sc, -(im * ss));
+    const float ni = fmaf(re, ss, im * sc);
+    re = nr;
+    im = ni;
+  }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sq_contract_lp16(
+    ContractArgs a) {
+  typedef LpTypes<PREC> LT;
+  typedef typename LT::T T;
+  typedef typename LT::V V;
+  __shared__ float2 rec[LKC][kLRec];                   // x at 0..63, y at 64..67, z at 68..99
+  __shared__ __attribute__((aligned(16))) T B[4][LN][LKP];  // re_hi, re_lo, im_hi, im_lo of ey*ez
+  const int tile = blockIdx.x % a.ntiles;
+  const int split = blockIdx.x / a.ntiles;
+  const int kx0 = a.tiles[4 * tile], ky0 = a.tiles[4 * tile + 1], kz0 = a.tiles[4 * tile + 2];
+  const int t = threadIdx.x, w = t / kWave, lane = t & (kWave - 1);
+  const int i16 = lane & 15, kq = lane >> 4;
+  // Staging: every lane writes ex segment t/32 (8 kx) of atom t%32; lanes
+  // 0-127 also write ez segment t/32 (8 kz), lanes 128-159 ey (4 ky).
+  const int sa = t % LKC, xseg = t / LKC;
+  const int task2 = t < 128 ? 2 : (t < 160 ? 1 : -1);  // axis of the second segment
+  const int n0x = kx0 + 8 * xseg;
+  const int n02 = task2 == 2 ? kz0 + 8 * xseg : ky0;
+  float2* const dx = rec[sa] + 8 * xseg;
+  float2* const d2 = task2 == 2 ? rec[sa] + 68 + 8 * xseg : rec[sa] + 64;
+  const double* u_hi = a.u;
+  const double* u_lo = a.u + 3 * a.n;
+  const double* u_c = a.u + 6 * a.n;
+  const double* u_s = a.u + 9 * a.n;
+  const uint32_t n32 = static_cast<uint32_t>(a.n);
+  const uint32_t o2 = static_cast<uint32_t>(task2 < 0 ? 0 : task2) * n32;
+  // Products: column n = (ky, kz), atoms 16*ph .. 16*ph+15.
+  const int pn = t & (LN - 1), ph = t >> 7;
+  const int pky = pn / LZ, pkz = pn % LZ;
+  for (int s = 0; s < a.nsp; ++s) {
+    const int a0 = a.sp_off[s], a1 = a.sp_off[s + 1];
+    const int64_t len = a1 - a0;
+    const int b0 = a0 + static_cast<int>(len * split / a.ksplit);
+    const int b1 = a0 + static_cast<int>(len * (split + 1) / a.ksplit);
+    // FP32 accumulation over this workgroup's atoms (at most 8192 per species,
+    // set by the host through ksplit); the K-splits are summed in FP64.
+    f4 accR[8], accI[8];
+    for (int nb = 0; nb < 8; ++nb) {
+      accR[nb] = f4{0.f, 0.f, 0.f, 0.f};
+      accI[nb] = accR[nb];
+    }
+    double xh = 0.0, xl = 0.0, xc = 1.0, xs = 0.0, vh = 0.0, vl = 0.0, vc = 1.0, vs = 0.0;
+    auto fetch = [&](int c0) {
+      const int ai = c0 + sa;
+      if (ai < b1) {
+        const uint32_t o = static_cast<uint32_t>(ai);
+        xh = u_hi[o];
+        xl = u_lo[o];
+        xc = u_c[o];
+        xs = u_s[o];
+        if (task2 >= 0) {
+          vh = u_hi[o2 + o];
+          vl = u_lo[o2 + o];
+          vc = u_c[o2 + o];
+          vs = u_s[o2 + o];
+        }
+      }
+    };
+    // Records of chunk c0 from the prefetched turns; then prefetch c0 + LKC.
+    auto stage = [&](int c0) {
+      if (c0 + sa < b1) {
+        phase_rec(xh, xl, xc, xs, n0x, 8, dx);
+        if (task2 >= 0) phase_rec(vh, vl, vc, vs, n02, task2 == 2 ? 8 : LY, d2);
+      } else {
+        for (int k = 0; k < 8; ++k) dx[k] = make_float2(0.f, 0.f);
+        if (task2 >= 0)
+          for (int k = 0; k < (task2 == 2 ? 8 : LY); ++k) d2[k] = make_float2(0.f, 0.f);
+      }
+      fetch(c0 + LKC);
+    };
+    __syncthreads();  // the previous species' readers are done with rec and B
+    fetch(b0);
+    if (b0 < b1) stage(b0);
+    __syncthreads();
+    for (int c0 = b0; c0 < b1; c0 += LKC) {
+      // B = ey*ez for column pn and 16 atoms, split, 8 atoms per 16-byte store.
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        f8 re, im;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const float2* r = rec[16 * ph + 8 * g + k];
+          const float2 y = r[64 + pky], z = r[68 + pkz];
+          re[k] = fmaf(y.x, z.x, -(y.y * z.y));
+          im[k] = fmaf(y.x, z.y, y.y * z.x);
+        }
+        V rh, rl, ih, il;
+        split8(re, rh, rl);
+        split8(im, ih, il);
+        const int ab = 16 * ph + 8 * g;
+        *reinterpret_cast<V*>(&B[0][pn][ab]) = rh;
+        *reinterpret_cast<V*>(&B[1][pn][ab]) = rl;
+        *reinterpret_cast<V*>(&B[2][pn][ab]) = ih;
+        *reinterpret_cast<V*>(&B[3][pn][ab]) = il;
+      }
+      // A = ex for this wavefront's row 16w + i16, atoms 8kq .. 8kq+7.
+      V arh, arl, aih, ail;
+      {
+        f8 re, im;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const float2 e = rec[8 * kq + k][16 * w + i16];
+          re[k] = e.x;
+          im[k] = e.y;
+        }
+        split8(re, arh, arl);
+        split8(im, aih, ail);
+      }
+      __syncthreads();  // B complete; rec free for the next chunk
+      if (c0 + LKC < b1) stage(c0 + LKC);
+      const V naih = -aih, nail = -ail;
+#pragma unroll
+      for (int nb = 0; nb < 8; nb += 2) {
+        V brh[2], brl[2], bih[2], bil[2];
+        for (int e = 0; e < 2; ++e) {
+          const int col = 16 * (nb + e) + i16;
+          brh[e] = *reinterpret_cast<const V*>(&B[0][col][8 * kq]);
+          brl[e] = *reinterpret_cast<const V*>(&B[1][col][8 * kq]);
+          bih[e] = *reinterpret_cast<const V*>(&B[2][col][8 * kq]);
+          bil[e] = *reinterpret_cast<const V*>(&B[3][col][8 * kq]);
+        }
+        f4 r0 = accR[nb], i0 = accI[nb], r1 = accR[nb + 1], i1 = accI[nb + 1];
+        r0 = LT::mfma(arh, brh[0], r0);
+        i0 = LT::mfma(arh, bih[0], i0);
+        r1 = LT::mfma(arh, brh[1], r1);
+        i1 = LT::mfma(arh, bih[1], i1);
+        r0 = LT::mfma(arh, brl[0], r0);
+        i0 = LT::mfma(arh, bil[0], i0);
+        r1 = LT::mfma(arh, brl[1], r1);
+        i1 = LT::mfma(arh, bil[1], i1);
+        r0 = LT::mfma(arl, brh[0], r0);
+        i0 = LT::mfma(arl, bih[0], i0);
+        r1 = LT::mfma(arl, brh[1], r1);
+        i1 = LT::mfma(arl, bih[1], i1);
+        r0 = LT::mfma(naih, bih[0], r0);
+        i0 = LT::mfma(aih, brh[0], i0);
+        r1 = LT::mfma(naih, bih[1], r1);
+        i1 = LT::mfma(aih, brh[1], i1);
+        r0 = LT::mfma(naih, bil[0], r0);
+        i0 = LT::mfma(aih, brl[0], i0);
+        r1 = LT::mfma(naih, bil[1], r1);
+        i1 = LT::mfma(aih, brl[1], i1);
+        r0 = LT::mfma(nail, bih[0], r0);
+        i0 = LT::mfma(ail, brh[0], i0);
+        r1 = LT::mfma(nail, bih[1], r1);
+        i1 = LT::mfma(ail, brh[1], i1);
+        accR[nb] = r0;
+        accI[nb] = i0;
+        accR[nb + 1] = r1;
+        accI[nb + 1] = i1;
+      }
+      __syncthreads();  // rec of the next chunk complete; B free
+    }
+    // D layout of the 16x16 FP32-output MFMAs: register r holds row
+    // 4*(lane/16) + r, column lane%16.
+    const size_t base = ((static_cast<size_t>(split) * a.nsp + s) * a.ntiles + tile) * (LX * LN);
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int kxl = 16 * w + 4 * kq + r;
+        const int n = 16 * nb + i16;
+        const int v = (kxl * LY + n / LZ) * LZ + (n % LZ);
+        reinterpret_cast<double2*>(a.rho)[base + v] =
+            make_double2(static_cast<double>(accR[nb][r]), static_cast<double>(accI[nb][r]));
+      }
+    }
+  }
+}
+
 struct Bsc {
   double b[kNspMax];
 };
@@ -1402,9 +1614,9 @@ void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s) 
   if (precision == TRJ_PREC_FP32)
     hipLaunchKernelGGL(k_sq_contract_lp<TRJ_PREC_FP32>, grid, block, 0, s, a);
   else if (precision == TRJ_PREC_FP16X3)
-    hipLaunchKernelGGL(k_sq_contract_lp<TRJ_PREC_FP16X3>, grid, block, 0, s, a);
+    hipLaunchKernelGGL(k_sq_contract_lp16<TRJ_PREC_FP16X3>, grid, block, 0, s, a);
   else
-    hipLaunchKernelGGL(k_sq_contract_lp<TRJ_PREC_BF16X3>, grid, block, 0, s, a);
+    hipLaunchKernelGGL(k_sq_contract_lp16<TRJ_PREC_BF16X3>, grid, block, 0, s, a);
 }
 
 void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, int tile_vec, int ksplit,
