@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=150)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--precision", default="fp64", choices=list(PEAK))
+    ap.add_argument("--pipeline", default="serial", choices=["serial", "overlap"],
+                    help="overlap: the RDF runs on its own stream beside the S(Q) contraction")
     ap.add_argument("--latency-frames", type=int, default=10)
     ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -248,6 +250,7 @@ def main():
     torch.cuda.synchronize()
 
     ctx = T.Context(0)
+    ctx.set_pipeline(T.TRJ_PIPE_OVERLAP if args.pipeline == "overlap" else T.TRJ_PIPE_SERIAL)
     box = frames[0].box
     st = ctx.rdf_init(T.rdf_params(3, wl.nsp, wl.deltar, wl.rcrdf), box)
     qset = T.TRJ_QSET_HALFSPACE if wl.qset == "halfspace" else T.TRJ_QSET_REF
@@ -311,7 +314,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "n_atoms": n, "nsp": wl.nsp, "rcrdf": wl.rcrdf,
                    "deltar": wl.deltar, "qmax": wl.qmax, "qset": wl.qset, "q_vectors": nvec,
-                   "lsmax": st.lsmax, "precision": args.precision, "inputs": "HBM-resident float64"},
+                   "lsmax": st.lsmax, "precision": args.precision, "pipeline": args.pipeline,
+                   "inputs": "HBM-resident float64"},
         "pairs_per_s": pairs / elapsed,
         "q_atom_terms_per_s": nvec * n * args.steps / elapsed,
         "latency_ms_p50": float(np.percentile(lat, 50)) if lat else None,

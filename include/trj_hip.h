@@ -203,6 +203,16 @@ int trj_timing_reset(trj_ctx* ctx);
 /* Device memory the context holds, in bytes. */
 int64_t trj_device_bytes(const trj_ctx* ctx);
 
+/* Frame pipeline of a context.  TRJ_PIPE_SERIAL (the default): one stream; a
+ * frame runs ingest, the RDF, then S(Q), as the reference runs RDFcomp then
+ * SQcalc per frame (trj_analysis.f90:173-242).  TRJ_PIPE_OVERLAP: for frames
+ * that ask for both, the RDF runs on a second stream of the context,
+ * concurrently with the S(Q) contraction; the frame completes when both have.
+ * Results are identical in both modes.  Synchronises. */
+#define TRJ_PIPE_SERIAL 0
+#define TRJ_PIPE_OVERLAP 1
+int trj_set_pipeline(trj_ctx* ctx, int mode);
+
 /* ------------------------------------------------------------------------
  * Batched frames: many independent frames of uneven size and species count,
  * each with its own pair histogram (the frame loop of trj_analysis.f90:173-242

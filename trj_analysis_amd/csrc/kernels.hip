@@ -616,8 +616,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // Tile 64 kx x 4 ky x 32 kz: M = 64 kx (one 16-row block per wavefront),
 // N = 128 (ky,kz) pairs (eight 16-column blocks), K = 32 atoms per chunk.
 // The phase products ey*ez are formed once per workgroup (the B operand all
-// four wavefronts share); ex needs no product.  Phases: exact FP64 reduction
-// of the anchor n0*u and of the step u, FP32 sincospi, FP32 angle addition.
+// four wavefronts share); ex needs no product.
+//
+// FP32 kernel (v_mfma_f32_16x16x4_f32, MFMA-bound): phases by exact FP64
+// reduction of the anchor n0*u and of the step u, FP32 sincospi, FP32 angle
+// addition; FP32 operand tables in LDS.
 constexpr int LX = 64, LY = 4, LZ = 32, LN = LY * LZ;
 constexpr int LKC = 32;           // atoms per chunk (K of one 16x16x32 MFMA)
 constexpr int LKP = 40;           // padded row length of the 16-bit operand tables
@@ -658,71 +661,15 @@ __device__ inline void phase_f32(double uh, double ul, int n0, int cnt, float* r
   }
 }
 
-template <class T>
-__device__ inline void split2(float x, T& hi, T& lo) {
-  hi = static_cast<T>(x);
-  lo = static_cast<T>(x - static_cast<float>(hi));
-}
-
-template <int PREC>
-struct LpShared;
-template <>
-struct LpShared<TRJ_PREC_FP32> {
+struct LpShared32 {
   float A[2][LX][LKF];   // re, im of ex
   float B[2][LN][LKF];   // re, im of ey*ez
   float2 ey[LKC][LY];
   float2 ez[LKC][LZ];
 };
-template <int PREC>
-struct LpShared {
-  typedef typename LpTypes<PREC>::T T;
-  __attribute__((aligned(16))) T A[4][LX][LKP];  // re_hi, re_lo, im_hi, im_lo of ex
-  __attribute__((aligned(16))) T B[4][LN][LKP];  // re_hi, re_lo, im_hi, im_lo of ey*ez
-  float2 ey[LKC][LY];
-  float2 ez[LKC][LZ];
-};
 
-// Store 8 consecutive-atom values of one operand row: FP32 as two float4, the
-// 16-bit forms as hi and lo vectors of 8 (one 16-byte store each).
-template <int PREC>
-__device__ inline void lp_store8(LpShared<PREC>& sh, bool isA, int row, int a0, const float* re,
-                                 const float* im) {
-  if constexpr (PREC == TRJ_PREC_FP32) {
-    for (int k = 0; k < 8; ++k) {
-      if (isA) {
-        sh.A[0][row][a0 + k] = re[k];
-        sh.A[1][row][a0 + k] = im[k];
-      } else {
-        sh.B[0][row][a0 + k] = re[k];
-        sh.B[1][row][a0 + k] = im[k];
-      }
-    }
-  } else {
-    typedef LpTypes<PREC> LT;
-    typedef typename LT::T T;
-    typedef typename LT::V V;
-    V rh, rl, ih, il;
-    for (int k = 0; k < 8; ++k) {
-      T h, l;
-      split2(re[k], h, l);
-      rh[k] = h;
-      rl[k] = l;
-      split2(im[k], h, l);
-      ih[k] = h;
-      il[k] = l;
-    }
-    T* base = isA ? &sh.A[0][row][a0] : &sh.B[0][row][a0];
-    const size_t plane = isA ? sizeof(sh.A[0]) / sizeof(T) : sizeof(sh.B[0]) / sizeof(T);
-    *reinterpret_cast<V*>(base) = rh;
-    *reinterpret_cast<V*>(base + plane) = rl;
-    *reinterpret_cast<V*>(base + 2 * plane) = ih;
-    *reinterpret_cast<V*>(base + 3 * plane) = il;
-  }
-}
-
-template <int PREC>
-__global__ __launch_bounds__(256) void k_sq_contract_lp(ContractArgs a) {
-  __shared__ LpShared<PREC> sh;
+__global__ __launch_bounds__(256) void k_sq_contract_f32(ContractArgs a) {
+  __shared__ LpShared32 sh;
   const int tile = blockIdx.x % a.ntiles;
   const int split = blockIdx.x / a.ntiles;
   const int kx0 = a.tiles[4 * tile], ky0 = a.tiles[4 * tile + 1], kz0 = a.tiles[4 * tile + 2];
@@ -763,19 +710,8 @@ __global__ __launch_bounds__(256) void k_sq_contract_lp(ContractArgs a) {
         for (int k = 0; k < 8; ++k) {
           const int row = 8 * seg + k;
           for (int q = 0; q < 4; ++q) {
-            if constexpr (PREC == TRJ_PREC_FP32) {
-              sh.A[0][row][aq + q] = re[q][k];
-              sh.A[1][row][aq + q] = im[q][k];
-            } else {
-              typedef typename LpTypes<PREC>::T T;
-              T h, l;
-              split2(re[q][k], h, l);
-              sh.A[0][row][aq + q] = h;
-              sh.A[1][row][aq + q] = l;
-              split2(im[q][k], h, l);
-              sh.A[2][row][aq + q] = h;
-              sh.A[3][row][aq + q] = l;
-            }
+            sh.A[0][row][aq + q] = re[q][k];
+            sh.A[1][row][aq + q] = im[q][k];
           }
         }
       } else if (t < 64 + LKC * 5) {
@@ -809,73 +745,25 @@ __global__ __launch_bounds__(256) void k_sq_contract_lp(ContractArgs a) {
             re[k] = fmaf(y.x, z.x, -(y.y * z.y));
             im[k] = fmaf(y.x, z.y, y.y * z.x);
           }
-          lp_store8<PREC>(sh, false, n, ab + 8 * h, re, im);
+          for (int k = 0; k < 8; ++k) {
+            sh.B[0][n][ab + 8 * h + k] = re[k];
+            sh.B[1][n][ab + 8 * h + k] = im[k];
+          }
         }
       }
       __syncthreads();
       const int row = 16 * w + i16;
-      if constexpr (PREC == TRJ_PREC_FP32) {
 #pragma unroll
-        for (int kk = 0; kk < LKC / 4; ++kk) {
-          const int at = 4 * kk + kq;
-          const float ar = sh.A[0][row][at], ai = sh.A[1][row][at];
+      for (int kk = 0; kk < LKC / 4; ++kk) {
+        const int at = 4 * kk + kq;
+        const float ar = sh.A[0][row][at], ai = sh.A[1][row][at];
 #pragma unroll
-          for (int nb = 0; nb < 8; ++nb) {
-            const float br = sh.B[0][16 * nb + i16][at], bi = sh.B[1][16 * nb + i16][at];
-            accR[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, accR[nb], 0, 0, 0);
-            accI[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, accI[nb], 0, 0, 0);
-            accR[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, bi, accR[nb], 0, 0, 0);
-            accI[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, accI[nb], 0, 0, 0);
-          }
-        }
-      } else {
-        typedef LpTypes<PREC> LT;
-        typedef typename LT::V V;
-        const V arh = *reinterpret_cast<const V*>(&sh.A[0][row][8 * kq]);
-        const V arl = *reinterpret_cast<const V*>(&sh.A[1][row][8 * kq]);
-        const V aih = *reinterpret_cast<const V*>(&sh.A[2][row][8 * kq]);
-        const V ail = *reinterpret_cast<const V*>(&sh.A[3][row][8 * kq]);
-        const V naih = -aih, nail = -ail;
-#pragma unroll
-        for (int nb = 0; nb < 8; nb += 2) {
-          V brh[2], brl[2], bih[2], bil[2];
-          for (int e = 0; e < 2; ++e) {
-            const int col = 16 * (nb + e) + i16;
-            brh[e] = *reinterpret_cast<const V*>(&sh.B[0][col][8 * kq]);
-            brl[e] = *reinterpret_cast<const V*>(&sh.B[1][col][8 * kq]);
-            bih[e] = *reinterpret_cast<const V*>(&sh.B[2][col][8 * kq]);
-            bil[e] = *reinterpret_cast<const V*>(&sh.B[3][col][8 * kq]);
-          }
-          // four independent accumulators, interleaved
-          f4 r0 = accR[nb], i0 = accI[nb], r1 = accR[nb + 1], i1 = accI[nb + 1];
-          r0 = LT::mfma(arh, brh[0], r0);
-          i0 = LT::mfma(arh, bih[0], i0);
-          r1 = LT::mfma(arh, brh[1], r1);
-          i1 = LT::mfma(arh, bih[1], i1);
-          r0 = LT::mfma(arh, brl[0], r0);
-          i0 = LT::mfma(arh, bil[0], i0);
-          r1 = LT::mfma(arh, brl[1], r1);
-          i1 = LT::mfma(arh, bil[1], i1);
-          r0 = LT::mfma(arl, brh[0], r0);
-          i0 = LT::mfma(arl, bih[0], i0);
-          r1 = LT::mfma(arl, brh[1], r1);
-          i1 = LT::mfma(arl, bih[1], i1);
-          r0 = LT::mfma(naih, bih[0], r0);
-          i0 = LT::mfma(aih, brh[0], i0);
-          r1 = LT::mfma(naih, bih[1], r1);
-          i1 = LT::mfma(aih, brh[1], i1);
-          r0 = LT::mfma(naih, bil[0], r0);
-          i0 = LT::mfma(aih, brl[0], i0);
-          r1 = LT::mfma(naih, bil[1], r1);
-          i1 = LT::mfma(aih, brl[1], i1);
-          r0 = LT::mfma(nail, bih[0], r0);
-          i0 = LT::mfma(ail, brh[0], i0);
-          r1 = LT::mfma(nail, bih[1], r1);
-          i1 = LT::mfma(ail, brh[1], i1);
-          accR[nb] = r0;
-          accI[nb] = i0;
-          accR[nb + 1] = r1;
-          accI[nb + 1] = i1;
+        for (int nb = 0; nb < 8; ++nb) {
+          const float br = sh.B[0][16 * nb + i16][at], bi = sh.B[1][16 * nb + i16][at];
+          accR[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, accR[nb], 0, 0, 0);
+          accI[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, accI[nb], 0, 0, 0);
+          accR[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, bi, accR[nb], 0, 0, 0);
+          accI[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, accI[nb], 0, 0, 0);
         }
       }
     }
@@ -1612,7 +1500,7 @@ void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s) 
   if (a.ntiles <= 0) return;
   const dim3 grid(a.ntiles * a.ksplit), block(256);
   if (precision == TRJ_PREC_FP32)
-    hipLaunchKernelGGL(k_sq_contract_lp<TRJ_PREC_FP32>, grid, block, 0, s, a);
+    hipLaunchKernelGGL(k_sq_contract_f32, grid, block, 0, s, a);
   else if (precision == TRJ_PREC_FP16X3)
     hipLaunchKernelGGL(k_sq_contract_lp16<TRJ_PREC_FP16X3>, grid, block, 0, s, a);
   else

@@ -23,10 +23,13 @@ struct DevBuf {
   size_t cap = 0;
 };
 
+// Per-frame timing events: [0] start, [1] ingest done, [2] S(Q) start (after
+// the RDF when serial), [3] contraction done, [4] frame done; with the RDF on
+// its own stream, [5] / [6] bracket it there.
 struct EventSet {
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   bool pending = false;
-  bool has_rdf = false, has_sq = false;
+  bool has_rdf = false, has_sq = false, overlapped = false;
 };
 
 constexpr int kEventRing = 8;
@@ -37,6 +40,10 @@ constexpr size_t kPairLdsLimit = 64 * 1024;
 struct trj_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // TRJ_PIPE_OVERLAP: the RDF of frames that also ask for S(Q) runs on
+  // rdf_stream; the frame's last event on `stream` waits for it.
+  int pipe = TRJ_PIPE_SERIAL;
+  hipStream_t rdf_stream = nullptr;
   std::string err;
   // device buffers
   DevBuf raw, itype, pos, spos, stype, cellid, cellcount, cellstart, cursor, partial, counts, thr, blk,
@@ -112,7 +119,10 @@ void harvest(trj_ctx* c, EventSet& s) {
   hipEventSynchronize(s.ev[4]);
   float ms[4] = {0, 0, 0, 0};
   hipEventElapsedTime(&ms[3], s.ev[0], s.ev[1]);  // ingest
-  hipEventElapsedTime(&ms[0], s.ev[1], s.ev[2]);  // RDF
+  if (s.overlapped)
+    hipEventElapsedTime(&ms[0], s.ev[5], s.ev[6]);  // RDF on its stream
+  else
+    hipEventElapsedTime(&ms[0], s.ev[1], s.ev[2]);  // RDF
   hipEventElapsedTime(&ms[1], s.ev[2], s.ev[3]);  // S(Q) contraction
   hipEventElapsedTime(&ms[2], s.ev[3], s.ev[4]);  // S(Q) shells + axis vectors
   for (int k = 0; k < 4; ++k) {
@@ -232,7 +242,14 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
   HIPC(c, hipGetLastError());
   HIPC(c, hipEventRecord(es.ev[1], c->stream));
 
-  // RDF (RDFcomp, rdf.cuf:184-353).
+  // RDF (RDFcomp, rdf.cuf:184-353), on its own stream when overlapped with
+  // the S(Q) contraction (it reads pos, itype and its own buffers only).
+  const bool overlap = c->pipe == TRJ_PIPE_OVERLAP && do_rdf && do_sq;
+  hipStream_t rs = overlap ? c->rdf_stream : c->stream;
+  if (overlap) {
+    HIPC(c, hipStreamWaitEvent(rs, es.ev[1], 0));
+    HIPC(c, hipEventRecord(es.ev[5], rs));
+  }
   if (do_rdf) {
     RdfPlan& R = c->rdf;
     FrameGeom g = frame_geometry(ndim, box, R.periodic, &R.rc);
@@ -278,13 +295,13 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
       if ((r = ensure(c, c->cursor, sizeof(int32_t) * ncell))) return r;
       if ((r = ensure(c, c->spos, sizeof(double) * 3 * n))) return r;
       if ((r = ensure(c, c->stype, sizeof(int32_t) * n))) return r;
-      HIPC(c, hipMemsetAsync(c->cellcount.p, 0, sizeof(int32_t) * (ncell + 1), c->stream));
-      HIPC(c, hipMemsetAsync(c->cursor.p, 0, sizeof(int32_t) * ncell, c->stream));
+      HIPC(c, hipMemsetAsync(c->cellcount.p, 0, sizeof(int32_t) * (ncell + 1), rs));
+      HIPC(c, hipMemsetAsync(c->cursor.p, 0, sizeof(int32_t) * ncell, rs));
       launch_cell_count(P<double>(c->pos), n, ndim, cg, P<int32_t>(c->cellid), P<int32_t>(c->cellcount),
-                        c->stream);
-      launch_exclusive_scan(P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), ncell + 1, c->stream);
+                        rs);
+      launch_exclusive_scan(P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), ncell + 1, rs);
       launch_cell_scatter(P<double>(c->pos), itype_dev, P<int32_t>(c->cellid), n, P<int32_t>(c->cellstart),
-                          P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype), c->stream);
+                          P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype), rs);
       const double mean = static_cast<double>(n) / static_cast<double>(ncell);
       int W = static_cast<int>(std::lround(mean / 64.0));
       W = std::max(1, std::min(W, 8));
@@ -292,14 +309,14 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
       nblocks = static_cast<int>(std::min<int64_t>((ntasks + 3) / 4, 2048));
       if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
       launch_rdf_cells(pa, cg, P<int32_t>(c->cellstart), P<double>(c->spos), P<int32_t>(c->stype), n, W,
-                       nblocks, P<uint32_t>(c->partial), c->stream);
+                       nblocks, P<uint32_t>(c->partial), rs);
     } else {
       const int64_t nb = (n + 63) / 64;
       nblocks = static_cast<int>(std::min<int64_t>((nb * nb + 3) / 4, 1024));
       if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
-      launch_rdf_allpairs(pa, P<double>(c->pos), itype_dev, n, nblocks, P<uint32_t>(c->partial), c->stream);
+      launch_rdf_allpairs(pa, P<double>(c->pos), itype_dev, n, nblocks, P<uint32_t>(c->partial), rs);
     }
-    launch_rdf_merge(P<uint32_t>(c->partial), nblocks, nbins, P<int64_t>(c->counts), c->stream);
+    launch_rdf_merge(P<uint32_t>(c->partial), nblocks, nbins, P<int64_t>(c->counts), rs);
     HIPC(c, hipGetLastError());
     c->rdf_frames += 1;
     c->last_volume = g.volume;
@@ -308,6 +325,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     for (int k = 0; k < ndim; ++k) v *= box[k];
     c->last_volume = v;
   }
+  if (overlap) HIPC(c, hipEventRecord(es.ev[6], rs));
   HIPC(c, hipEventRecord(es.ev[2], c->stream));
 
   // S(Q) (SQcalc, sq.cuf:58-126).
@@ -350,10 +368,12 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
   } else {
     HIPC(c, hipEventRecord(es.ev[3], c->stream));
   }
+  if (overlap) HIPC(c, hipStreamWaitEvent(c->stream, es.ev[6], 0));
   HIPC(c, hipEventRecord(es.ev[4], c->stream));
   es.pending = true;
   es.has_rdf = do_rdf;
   es.has_sq = do_sq;
+  es.overlapped = overlap;
   c->last_n = n;
   c->last_nsp = nsp;
   return TRJ_OK;
@@ -364,19 +384,18 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
 namespace trj {
 hipStream_t context_stream(trj_ctx* c) { return c->stream; }
 
-// K-splits of the contraction: enough workgroups to fill the device, and for
-// the FP32 / 16-bit kernels at most 8192 atoms per workgroup (the FP32
-// accumulator span; the splits are summed in FP64).
+// K-splits of the contraction (the splits are summed in FP64 by the shells
+// kernel): at least 2048 workgroups when the atoms allow it; for FP64, atom
+// ranges of at most 32768 so that the workgroups running at one time share
+// their turns (96 B per atom) in L2, with the partial rho buffer (one per
+// split) kept within 4 GiB; for the FP32-accumulating kernels at most 8192
+// atoms per workgroup (their accuracy bound, not capped).
 int contraction_splits(int ntiles, int64_t n, int precision, int nsp) {
-  // At least 2048 workgroups when the atoms allow it, and atom ranges short
-  // enough that the workgroups running at one time share their turns in L2
-  // (32768 atoms x 96 B for FP64; 8192 for the FP32-accumulating kernels,
-  // which fold into FP64 per split).  The partial rho buffer (one per split)
-  // is kept within 4 GiB.
   int ks = static_cast<int>((2048 + ntiles - 1) / ntiles);
   ks = std::max(1, std::min<int>(ks, static_cast<int>(std::max<int64_t>(1, n / 8192))));
   const int64_t span = precision == TRJ_PREC_FP64 ? 32768 : 8192;
   ks = std::max<int>(ks, static_cast<int>((n + span - 1) / span));
+  if (precision != TRJ_PREC_FP64) return ks;
   const int64_t per_split = int64_t(16) * std::max(1, nsp) * std::max(1, ntiles) * tile_shape(precision).vec();
   const int64_t cap = std::max<int64_t>(1, (int64_t(4) << 30) / per_split);
   return static_cast<int>(std::min<int64_t>(ks, cap));
@@ -453,11 +472,24 @@ void trj_destroy(trj_ctx* c) {
   for (auto& s : c->evs)
     for (auto& e : s.ev)
       if (e) hipEventDestroy(e);
+  if (c->rdf_stream) hipStreamDestroy(c->rdf_stream);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
 
 const char* trj_last_error(const trj_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int trj_set_pipeline(trj_ctx* c, int mode) {
+  if (!c) return TRJ_EINVAL;
+  if (mode != TRJ_PIPE_SERIAL && mode != TRJ_PIPE_OVERLAP) return fail(c, TRJ_EINVAL, "unknown pipeline mode");
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  if (mode == TRJ_PIPE_OVERLAP && !c->rdf_stream)
+    HIPC(c, hipStreamCreateWithFlags(&c->rdf_stream, hipStreamNonBlocking));
+  c->pipe = mode;
+  return TRJ_OK;
+}
 
 int trj_rdf_init(trj_ctx* c, const trj_rdf_params* p, const double box[3], int32_t* lsmax_out,
                  int32_t* nit_out) {

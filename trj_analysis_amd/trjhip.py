@@ -24,6 +24,7 @@ TRJ_WANT_RDF, TRJ_WANT_SQ = 1, 2
 TRJ_QSET_REF, TRJ_QSET_HALFSPACE = 0, 1
 TRJ_PREC_FP64, TRJ_PREC_FP32, TRJ_PREC_FP16X3, TRJ_PREC_BF16X3 = 0, 1, 2, 3
 TRJ_SQ_NQ_CONSISTENT = 1
+TRJ_PIPE_SERIAL, TRJ_PIPE_OVERLAP = 0, 1
 
 PRECISIONS = {"fp64": TRJ_PREC_FP64, "fp32": TRJ_PREC_FP32, "fp16x3": TRJ_PREC_FP16X3,
               "bf16x3": TRJ_PREC_BF16X3}
@@ -34,7 +35,7 @@ EXPORTS = [
     "trj_sq_init", "trj_sq_plan", "trj_rdf_plan", "trj_frame", "trj_frame_device", "trj_sync",
     "trj_rdf_counts", "trj_rdf_gmix", "trj_rdf_first_minimum", "trj_sq_shells", "trj_sq_columns",
     "trj_sq_rho", "trj_reset", "trj_last_timings", "trj_timing_totals", "trj_timing_reset",
-    "trj_device_bytes", "trj_batch_create", "trj_batch_destroy", "trj_batch_last_error",
+    "trj_device_bytes", "trj_set_pipeline", "trj_batch_create", "trj_batch_destroy", "trj_batch_last_error",
     "trj_batch_layout", "trj_batch_run", "trj_batch_autotune", "trj_srv_create", "trj_srv_destroy",
     "trj_srv_last_error", "trj_request_footprint", "trj_srv_submit", "trj_srv_poll", "trj_srv_result",
     "trj_srv_stats",
@@ -123,6 +124,7 @@ def _declare(L):
     L.trj_timing_reset.argtypes = [vp]
     L.trj_device_bytes.argtypes = [vp]
     L.trj_device_bytes.restype = C.c_int64
+    L.trj_set_pipeline.argtypes = [vp, C.c_int]
     L.trj_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(BatchParams)]
     L.trj_batch_destroy.argtypes = [vp]
     L.trj_batch_destroy.restype = None
@@ -359,6 +361,10 @@ class Context:
 
     def device_bytes(self) -> int:
         return self._L.trj_device_bytes(self._h)
+
+    def set_pipeline(self, mode: int):
+        """TRJ_PIPE_SERIAL or TRJ_PIPE_OVERLAP (RDF on its own stream beside S(Q))."""
+        self._check(self._L.trj_set_pipeline(self._h, int(mode)), "trj_set_pipeline")
 
 
 def batch_params(ndim, deltar, rcrdf, frames_per_launch=0, waves_per_cell=0, workgroups=0,
