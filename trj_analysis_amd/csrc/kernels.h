@@ -61,6 +61,8 @@ struct PairArgs {
   const double* thr;  // lsmax+3 edges
 };
 size_t pair_lds_bytes(int nit, int lsmax);
+// LDS of the cell-list pair kernel: pair_lds_bytes plus its hit rings.
+size_t cells_lds_bytes(int nit, int lsmax);
 // Cell-list pairs (half stencil); every periodic axis needs nc >= 3.
 void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start,
                       const double* spos, const int32_t* stype, int64_t n, int waves_per_cell,

@@ -287,7 +287,96 @@ __device__ inline double readlane_d(double v, int l) {
   return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
 
-template <bool SHIFT>
+// In-range pairs are not binned where they are found: a wavefront appends
+// (r2, types) to its own ring in LDS (ballot + mbcnt) and bins 64 of them at
+// a time with every lane busy, so the binning and the LDS atomic cost one
+// pass per 64 hits instead of a masked pass per candidate.
+constexpr int kHitQ = 128;  // ring entries per wavefront (holds < 64 + 64)
+
+
+// Bin entries head .. head+cnt-1 (cnt <= 64) of a wavefront's ring, one per lane.
+__device__ inline void bin_hits(const PairArgs& a, const double* thr, uint32_t* hist, const double* qr2,
+                                const int* qtt, uint32_t head, int cnt, int lane) {
+  if (lane < cnt) {
+    const uint32_t e = (head + lane) & (kHitQ - 1);
+    const double r2 = qr2[e];
+    const int tt = qtt[e];
+    const int ti = tt >> 8, tj = tt & 255;
+    const int k = bin_of(r2, thr, a.inv_dr, a.lsmax);
+    if (k >= 1 && k <= a.lsmax) {
+      const int lo = ti < tj ? ti : tj, hi = ti < tj ? tj : ti;
+      atomicAdd(&hist[(pair_index(lo, hi, a.nsp) - 1) * a.lsmax + (k - 1)], 1u);
+    }
+  }
+}
+
+// Candidates of one neighbour cell for the 64 home atoms of a wavefront.
+// Atoms outside the cell ranges or with types outside 1..nsp carry NaN
+// coordinates, so r2 < side2 fails for them without a separate test.
+template <bool SHIFT, bool D3, bool SELF, bool IMG>
+__device__ inline void rdf_neighbours(const PairArgs& a, const double* thr, uint32_t* hist, double* qr2, int* qtt,
+                                      uint32_t& head, uint32_t& tail, int lane, int js, int je, int gi,
+                                      double xi, double yi, double zi, int ti8, double shx, double shy,
+                                      double shz, const double* __restrict__ sx, const double* __restrict__ sy,
+                                      const double* __restrict__ sz, const int32_t* __restrict__ stype) {
+  const double nan = __builtin_nan("");
+  for (int jb = js; jb < je; jb += kWave) {
+    const int jl = jb + lane;
+    int tl = jl < je ? stype[jl] : 0;
+    const bool vl = static_cast<unsigned>(tl - 1) < static_cast<unsigned>(a.nsp);
+    tl = vl ? tl : 0;
+    const double xl = vl ? sx[jl] : nan;
+    const double yl = vl ? sy[jl] : nan;
+    const double zl = (D3 && vl) ? sz[jl] : nan;
+    const int cnt = (je - jb) < kWave ? (je - jb) : kWave;
+    for (int jj = 0; jj < cnt; ++jj) {
+      double r2;
+      if constexpr (SHIFT) {
+        // d - L*nint(d/L) == d - s for every pair of these two cells
+        // (s = -shift, exact: L*1 = L); no per-pair test.
+        // (IMG false: all shifts are 0 and d - 0 == d exactly.)
+        double dx = readlane_d(xl, jj) - xi, dy = readlane_d(yl, jj) - yi;
+        if constexpr (IMG) {
+          dx = dx - shx;
+          dy = dy - shy;
+        }
+        r2 = dx * dx + dy * dy;
+        if constexpr (D3) {
+          double dz = readlane_d(zl, jj) - zi;
+          if constexpr (IMG) dz = dz - shz;
+          r2 = r2 + dz * dz;
+        }
+      } else {
+        const double dx = min_image(readlane_d(xl, jj) - xi, a.lmi[0], a.half[0]);
+        const double dy = min_image(readlane_d(yl, jj) - yi, a.lmi[1], a.half[1]);
+        r2 = dx * dx + dy * dy;
+        if constexpr (D3) {
+          const double dz = min_image(readlane_d(zl, jj) - zi, a.lmi[2], a.half[2]);
+          r2 = r2 + dz * dz;
+        }
+      }
+      bool hit = r2 < a.side2;
+      if constexpr (SELF) hit = hit && (jb + jj > gi);
+      const unsigned long long m = __ballot(hit);
+      if (hit) {
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+        const uint32_t e = (tail + below) & (kHitQ - 1);
+        qr2[e] = r2;
+        qtt[e] = ti8 | __builtin_amdgcn_readlane(tl, jj);
+      }
+      tail += static_cast<uint32_t>(__popcll(m));
+      if (tail - head >= kWave) {
+        __builtin_amdgcn_wave_barrier();
+        bin_hits(a, thr, hist, qr2, qtt, head, kWave, lane);
+        __builtin_amdgcn_wave_barrier();
+        head += kWave;
+      }
+    }
+  }
+}
+
+template <bool SHIFT, bool D3>
 __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stencil st,
                                                    const int32_t* __restrict__ start,
                                                    const double* __restrict__ spos,
@@ -298,26 +387,34 @@ __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stenc
   const int nbins = a.nit * a.lsmax;
   uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
   double* thr = reinterpret_cast<double*>(smem + ((static_cast<size_t>(nbins) * 4 + 15) & ~size_t(15)));
+  // Same layout as cells_lds_bytes: hist, thr, then the rings.
+  const size_t qoff = (((static_cast<size_t>(nbins) * 4 + 15) & ~size_t(15)) +
+                       static_cast<size_t>(a.lsmax + 3) * 8 + 15) & ~size_t(15);
   hist_setup(a, hist, thr);
   const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  double* qr2 = reinterpret_cast<double*>(smem + qoff) + wave * kHitQ;
+  int* qtt = reinterpret_cast<int*>(smem + qoff + 4 * kHitQ * sizeof(double)) + wave * kHitQ;
+  uint32_t head = 0, tail = 0;  // wave-uniform ring counters
   const double* __restrict__ sx = spos;
   const double* __restrict__ sy = spos + n;
   const double* __restrict__ sz = spos + 2 * n;
   const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
+  const double nan = __builtin_nan("");
   for (int64_t task = static_cast<int64_t>(blockIdx.x) * 4 + wave; task < ntasks;
        task += static_cast<int64_t>(gridDim.x) * 4) {
     const int cell = static_cast<int>(task / W);
     const int ch0 = static_cast<int>(task % W);
-    const int cs = start[cell], ce = start[cell + 1];
+    const int cs = __builtin_amdgcn_readfirstlane(start[cell]);
+    const int ce = __builtin_amdgcn_readfirstlane(start[cell + 1]);
     const int cx = cell % ncx, cy = (cell / ncx) % ncy, cz = cell / (ncx * ncy);
     for (int ch = ch0; cs + ch * kWave < ce; ch += W) {
       const int gi = cs + ch * kWave + lane;
-      const bool vi = gi < ce;
-      const double xi = vi ? sx[gi] : 0.0;
-      const double yi = vi ? sy[gi] : 0.0;
-      const double zi = vi ? sz[gi] : 0.0;
-      const int ti = vi ? stype[gi] : 0;
-      const bool oki = vi && static_cast<unsigned>(ti - 1) < static_cast<unsigned>(a.nsp);
+      int ti = gi < ce ? stype[gi] : 0;
+      const bool oki = static_cast<unsigned>(ti - 1) < static_cast<unsigned>(a.nsp);
+      const double xi = oki ? sx[gi] : nan;
+      const double yi = oki ? sy[gi] : nan;
+      const double zi = (D3 && oki) ? sz[gi] : nan;
+      const int ti8 = (oki ? ti : 0) << 8;
       for (int o = 0; o < st.n; ++o) {
         int nx = cx + st.off[o][0], ny = cy + st.off[o][1], nz = cz + st.off[o][2];
         // Image shift of a wrapped neighbour, the L*nint(d/L) the FP64 rule
@@ -330,46 +427,26 @@ __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stenc
         if (g.periodic[1]) ny = (ny + ncy) % ncy; else if (ny < 0 || ny >= ncy) continue;
         if (g.periodic[2]) nz = (nz + ncz) % ncz; else if (nz < 0 || nz >= ncz) continue;
         const int nb = (nz * ncy + ny) * ncx + nx;
-        const bool self = (o == 0);
-        const int js = self ? cs + ch * kWave + 1 : start[nb];
-        const int je = start[nb + 1];
         // 64 neighbours per pass: one coalesced load per lane, then a
         // broadcast per pair through the scalar registers.
-        for (int jb = js; jb < je; jb += kWave) {
-          const int jl = jb + lane;
-          const bool vl = jl < je;
-          const double xl = vl ? sx[jl] : 0.0;
-          const double yl = vl ? sy[jl] : 0.0;
-          const double zl = vl ? sz[jl] : 0.0;
-          const int tl = vl ? stype[jl] : 0;
-          const int cnt = (je - jb) < kWave ? (je - jb) : kWave;
-          for (int jj = 0; jj < cnt; ++jj) {
-            const int j = jb + jj;
-            const int tj = __builtin_amdgcn_readlane(tl, jj);
-            const bool ok = oki && (!self || j > gi) &&
-                            static_cast<unsigned>(tj - 1) < static_cast<unsigned>(a.nsp);
-            if constexpr (SHIFT) {
-              // d - L*nint(d/L) == d - s for every pair of these two cells
-              // (s = -shift, exact: L*1 = L); no per-pair test.
-              const double dx = (readlane_d(xl, jj) - xi) - shx;
-              const double dy = (readlane_d(yl, jj) - yi) - shy;
-              double r2;
-              if (a.ndim == 3) {
-                const double dz = (readlane_d(zl, jj) - zi) - shz;
-                r2 = dx * dx + dy * dy + dz * dz;
-              } else {
-                r2 = dx * dx + dy * dy;
-              }
-              bin_pair(a, thr, hist, ok, r2, ti, tj);
-            } else {
-              pair_accumulate(a, thr, hist, ok, xi, yi, zi, ti, readlane_d(xl, jj), readlane_d(yl, jj),
-                              readlane_d(zl, jj), tj);
-            }
-          }
+        if (o == 0) {
+          rdf_neighbours<SHIFT, D3, true, false>(a, thr, hist, qr2, qtt, head, tail, lane, cs + ch * kWave + 1,
+                                                 ce, gi, xi, yi, zi, ti8, 0.0, 0.0, 0.0, sx, sy, sz, stype);
+        } else {
+          const int js = __builtin_amdgcn_readfirstlane(start[nb]);
+          const int je = __builtin_amdgcn_readfirstlane(start[nb + 1]);
+          if (shx != 0.0 || shy != 0.0 || shz != 0.0)
+            rdf_neighbours<SHIFT, D3, false, true>(a, thr, hist, qr2, qtt, head, tail, lane, js, je, gi, xi, yi,
+                                                   zi, ti8, shx, shy, shz, sx, sy, sz, stype);
+          else
+            rdf_neighbours<SHIFT, D3, false, false>(a, thr, hist, qr2, qtt, head, tail, lane, js, je, gi, xi, yi,
+                                                    zi, ti8, 0.0, 0.0, 0.0, sx, sy, sz, stype);
         }
       }
     }
   }
+  __builtin_amdgcn_wave_barrier();
+  bin_hits(a, thr, hist, qr2, qtt, head, static_cast<int>(tail - head), lane);
   hist_flush(a, hist, partial);
 }
 
@@ -1448,6 +1525,11 @@ size_t pair_lds_bytes(int nit, int lsmax) {
   return ((static_cast<size_t>(nit) * lsmax * 4 + 15) & ~size_t(15)) + static_cast<size_t>(lsmax + 3) * 8;
 }
 
+size_t cells_lds_bytes(int nit, int lsmax) {
+  const size_t base = (pair_lds_bytes(nit, lsmax) + 15) & ~size_t(15);
+  return base + static_cast<size_t>(4) * kHitQ * (sizeof(double) + sizeof(int));
+}
+
 void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start, const double* spos,
                       const int32_t* stype, int64_t n, int waves_per_cell, int nblocks,
                       uint32_t* partial, hipStream_t s) {
@@ -1471,12 +1553,22 @@ void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start
   const int64_t ntasks = ncell * waves_per_cell;
   bool shift = true;
   for (int k = 0; k < a.ndim; ++k) shift = shift && a.cellshift[k];
-  if (shift)
-    hipLaunchKernelGGL(k_rdf_cells<true>, dim3(nblocks), dim3(256), pair_lds_bytes(a.nit, a.lsmax), s, a, g,
-                       st, start, spos, stype, n, waves_per_cell, ntasks, partial);
-  else
-    hipLaunchKernelGGL(k_rdf_cells<false>, dim3(nblocks), dim3(256), pair_lds_bytes(a.nit, a.lsmax), s, a, g,
-                       st, start, spos, stype, n, waves_per_cell, ntasks, partial);
+  const size_t lds = cells_lds_bytes(a.nit, a.lsmax);
+  if (a.ndim == 3) {
+    if (shift)
+      hipLaunchKernelGGL((k_rdf_cells<true, true>), dim3(nblocks), dim3(256), lds, s, a, g, st, start, spos, stype,
+                         n, waves_per_cell, ntasks, partial);
+    else
+      hipLaunchKernelGGL((k_rdf_cells<false, true>), dim3(nblocks), dim3(256), lds, s, a, g, st, start, spos,
+                         stype, n, waves_per_cell, ntasks, partial);
+  } else {
+    if (shift)
+      hipLaunchKernelGGL((k_rdf_cells<true, false>), dim3(nblocks), dim3(256), lds, s, a, g, st, start, spos,
+                         stype, n, waves_per_cell, ntasks, partial);
+    else
+      hipLaunchKernelGGL((k_rdf_cells<false, false>), dim3(nblocks), dim3(256), lds, s, a, g, st, start, spos,
+                         stype, n, waves_per_cell, ntasks, partial);
+  }
 }
 
 void launch_rdf_allpairs(const PairArgs& a, const double* pos, const int32_t* itype, int64_t n,

@@ -509,9 +509,9 @@ int trj_rdf_init(trj_ctx* c, const trj_rdf_params* p, const double box[3], int32
   for (int k = 0; k < 3; ++k) R.periodic[k] = (k < p->ndim) ? (p->periodic[k] != 0) : 1;
   R.lsmax = rdf_lsmax(p->ndim, box, p->rcrdf, p->deltar, &R.rc);
   if (R.lsmax < 1) return fail(c, TRJ_EINVAL, "rdf: lsmax < 1");
-  if (pair_lds_bytes(R.nit, R.lsmax) > kPairLdsLimit)
+  if (cells_lds_bytes(R.nit, R.lsmax) > kPairLdsLimit)
     return fail(c, TRJ_ELIMIT,
-                "rdf: g(r) grid needs " + std::to_string(pair_lds_bytes(R.nit, R.lsmax)) +
+                "rdf: g(r) grid needs " + std::to_string(cells_lds_bytes(R.nit, R.lsmax)) +
                     " bytes of LDS, above 65536; increase deltar or reduce rcrdf (rdf.cuf:61-64)");
   std::vector<double> thr = rdf_bin_thresholds(R.deltar, R.lsmax);
   thr.push_back(INFINITY);  // thr[lsmax+2]
