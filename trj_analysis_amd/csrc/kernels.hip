@@ -883,6 +883,25 @@ __device__ inline void split8(const f8& x, V& hi, V& lo) {
   const f8 back = __builtin_convertvector(hi, f8);
   lo = __builtin_convertvector(x - back, V);
 }
+// BF16: the FP32 value of a packed pair's halves is a bit shift or mask, so hi
+// is widened by bit operations instead of a second rounding per element.
+template <>
+__device__ inline void split8<b8>(const f8& x, b8& hi, b8& lo) {
+  typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  uint32_t ph[4], pl[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const b2 h = __builtin_convertvector(f2{x[2 * k], x[2 * k + 1]}, b2);
+    const uint32_t p = __builtin_bit_cast(uint32_t, h);
+    const float b0 = __uint_as_float(p << 16), b1 = __uint_as_float(p & 0xffff0000u);
+    const b2 l = __builtin_convertvector(f2{x[2 * k] - b0, x[2 * k + 1] - b1}, b2);
+    ph[k] = p;
+    pl[k] = __builtin_bit_cast(uint32_t, l);
+  }
+  hi = __builtin_bit_cast(b8, ph);
+  lo = __builtin_bit_cast(b8, pl);
+}
 
 // e^{2 pi i n u}, n = n0 .. n0+cnt-1, into dst as FP32 complex: anchor by
 // FP64 reduction and FP32 sincospi, steps by the FP64 unit step (c, s).
