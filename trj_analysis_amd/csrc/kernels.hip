@@ -903,13 +903,36 @@ __device__ inline void split8<b8>(const f8& x, b8& hi, b8& lo) {
   lo = __builtin_bit_cast(b8, pl);
 }
 
-// e^{2 pi i n u}, n = n0 .. n0+cnt-1, into dst as FP32 complex: anchor by
-// FP64 reduction and FP32 sincospi, steps by the FP64 unit step (c, s).
+// e^{2 pi i f} in FP32 for a turn fraction f (|f| <= 1/2, FP64): exact
+// quarter-turn reduction in FP64, Taylor polynomials of sin and cos on
+// [-pi/4, pi/4] (truncation below 3e-8), rotation by the quarter turns.
+__device__ inline float2 cis_turns_f32(double f) {
+  const double t = 4.0 * f;  // quarter turns (exact), |t| <= 2
+  const double q = rint(t);
+  const float x = static_cast<float>(t - q) * 1.57079632679489662f;  // |x| <= pi/4
+  const float x2 = x * x;
+  // sin x = x - x^3/3! + x^5/5! - x^7/7! + x^9/9!
+  float sp = fmaf(x2, 2.75573192e-6f, -1.98412698e-4f);
+  sp = fmaf(x2, sp, 8.33333333e-3f);
+  sp = fmaf(x2, sp, -1.66666667e-1f);
+  const float sn = fmaf(x * x2, sp, x);
+  // cos x = 1 - x^2/2! + x^4/4! - x^6/6! + x^8/8!
+  float cp = fmaf(x2, 2.48015873e-5f, -1.38888889e-3f);
+  cp = fmaf(x2, cp, 4.16666667e-2f);
+  cp = fmaf(x2, cp, -0.5f);
+  const float cs = fmaf(x2, cp, 1.0f);
+  // times i^q: q = 1 -> (-sn, cs), 2 -> (-cs, -sn), 3 -> (sn, -cs)
+  const int qi = static_cast<int>(q) & 3;
+  const float re = (qi & 1) ? sn : cs, im = (qi & 1) ? cs : sn;
+  return make_float2(((qi + 1) & 2) ? -re : re, (qi & 2) ? -im : im);
+}
+
+// e^{2 pi i n u}, n = n0 .. n0+cnt-1, into dst as FP32 complex: the anchor
+// from the FP64-reduced turns, steps by the FP64 unit step (c, s).
 __device__ inline void phase_rec(double uh, double ul, double c, double s, int n0, int cnt, float2* dst) {
-  float sn, cs;
-  sincospif(static_cast<float>(2.0 * frac_turns(n0, uh, ul)), &sn, &cs);
+  const float2 an = cis_turns_f32(frac_turns(n0, uh, ul));
   const float sc = static_cast<float>(c), ss = static_cast<float>(s);
-  float re = cs, im = sn;
+  float re = an.x, im = an.y;
   for (int k = 0; k < cnt; ++k) {
     dst[k] = make_float2(re, im);
     const float nr = fmaf(re, sc, -(im * ss));
