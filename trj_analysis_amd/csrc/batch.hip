@@ -108,7 +108,7 @@ int plan_frames(const trj_batch_params& p, int nframes, const trj_batch_frame* f
     double rc = 0.0;
     P.lsmax = rdf_lsmax(p.ndim, box, p.rcrdf, p.deltar, &rc);
     P.nit = F.nsp * (F.nsp + 1) / 2;
-    if (pair_lds_bytes(P.nit, P.lsmax) > 64 * 1024) {
+    if (cells_lds_bytes(P.nit, P.lsmax) > 64 * 1024) {
       *err = "batch: frame " + std::to_string(f) + " histogram exceeds 64 KiB of LDS";
       return TRJ_ELIMIT;
     }

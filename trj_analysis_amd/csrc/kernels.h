@@ -139,6 +139,8 @@ struct BatchDev {
 };
 // Launch the whole per-group sequence on stream s with grid nblocks (all
 // kernels are grid-stride loops; nothing depends on the group's sizes).
+// pair_lds: the largest pair_lds_bytes of the group's frames (the pair kernel
+// adds its hit rings, cells_lds_bytes - pair_lds_bytes).
 void launch_batch_group(const BatchDev& d, int nblocks, size_t pair_lds, hipStream_t s);
 
 }  // namespace trj

@@ -230,18 +230,6 @@ struct Stencil {
   int off[14][3];
 };
 
-__device__ inline void bin_pair(const PairArgs& a, const double* thr, uint32_t* hist, bool ok, double r2,
-                                int ti, int tj) {
-  if (ok && r2 < a.side2) {
-    const int k = bin_of(r2, thr, a.inv_dr, a.lsmax);
-    if (k >= 1 && k <= a.lsmax) {
-      const int lo = ti < tj ? ti : tj, hi = ti < tj ? tj : ti;
-      const int ij = pair_index(lo, hi, a.nsp);
-      atomicAdd(&hist[(ij - 1) * a.lsmax + (k - 1)], 1u);
-    }
-  }
-}
-
 __device__ inline void pair_accumulate(const PairArgs& a, const double* thr, uint32_t* hist,
                                        bool ok, double xi, double yi, double zi, int ti, double xj,
                                        double yj, double zj, int tj) {
@@ -376,6 +364,58 @@ __device__ inline void rdf_neighbours(const PairArgs& a, const double* thr, uint
   }
 }
 
+// One chunk of up to 64 atoms of home cell cellL (cells numbered from
+// cell_off in `start`) against the half stencil; hits go to the wavefront's
+// ring (qr2, qtt, head, tail).
+template <bool SHIFT, bool D3>
+__device__ inline void rdf_home_chunk(const PairArgs& a, const double* thr, uint32_t* hist, double* qr2, int* qtt,
+                                      uint32_t& head, uint32_t& tail, int lane, const Stencil& st, const int nc[3],
+                                      const int periodic[3], int cell_off, int cellL, int ch, int cs, int ce,
+                                      const int32_t* __restrict__ start, const double* __restrict__ sx,
+                                      const double* __restrict__ sy, const double* __restrict__ sz,
+                                      const int32_t* __restrict__ stype) {
+  const double nan = __builtin_nan("");
+  const int ncx = nc[0], ncy = nc[1], ncz = nc[2];
+  const int cx = cellL % ncx, cy = (cellL / ncx) % ncy, cz = cellL / (ncx * ncy);
+  const int gi = cs + ch * kWave + lane;
+  const int ti = gi < ce ? stype[gi] : 0;
+  const bool oki = static_cast<unsigned>(ti - 1) < static_cast<unsigned>(a.nsp);
+  const double xi = oki ? sx[gi] : nan;
+  const double yi = oki ? sy[gi] : nan;
+  const double zi = (D3 && oki) ? sz[gi] : nan;
+  const int ti8 = (oki ? ti : 0) << 8;
+  for (int o = 0; o < st.n; ++o) {
+    int nx = cx + st.off[o][0], ny = cy + st.off[o][1], nz = cz + st.off[o][2];
+    // Image shift of a wrapped neighbour, the L*nint(d/L) the FP64 rule
+    // subtracts: a neighbour below cell 0 is the top cell (d ~ +L, subtract
+    // +L); one above the top cell is cell 0 (d ~ -L, subtract -L).
+    const double shx = nx < 0 ? a.lmi[0] : (nx >= ncx ? -a.lmi[0] : 0.0);
+    const double shy = ny < 0 ? a.lmi[1] : (ny >= ncy ? -a.lmi[1] : 0.0);
+    const double shz = D3 ? (nz < 0 ? a.lmi[2] : (nz >= ncz ? -a.lmi[2] : 0.0)) : 0.0;
+    if (periodic[0]) nx = (nx + ncx) % ncx; else if (nx < 0 || nx >= ncx) continue;
+    if (periodic[1]) ny = (ny + ncy) % ncy; else if (ny < 0 || ny >= ncy) continue;
+    if (D3) {
+      if (periodic[2]) nz = (nz + ncz) % ncz; else if (nz < 0 || nz >= ncz) continue;
+    }
+    const int nb = cell_off + (nz * ncy + ny) * ncx + nx;
+    // 64 neighbours per pass: one coalesced load per lane, then a broadcast
+    // per pair through the scalar registers.
+    if (o == 0) {
+      rdf_neighbours<SHIFT, D3, true, false>(a, thr, hist, qr2, qtt, head, tail, lane, cs + ch * kWave + 1, ce,
+                                             gi, xi, yi, zi, ti8, 0.0, 0.0, 0.0, sx, sy, sz, stype);
+    } else {
+      const int js = __builtin_amdgcn_readfirstlane(start[nb]);
+      const int je = __builtin_amdgcn_readfirstlane(start[nb + 1]);
+      if (shx != 0.0 || shy != 0.0 || shz != 0.0)
+        rdf_neighbours<SHIFT, D3, false, true>(a, thr, hist, qr2, qtt, head, tail, lane, js, je, gi, xi, yi, zi,
+                                               ti8, shx, shy, shz, sx, sy, sz, stype);
+      else
+        rdf_neighbours<SHIFT, D3, false, false>(a, thr, hist, qr2, qtt, head, tail, lane, js, je, gi, xi, yi, zi,
+                                                ti8, 0.0, 0.0, 0.0, sx, sy, sz, stype);
+    }
+  }
+}
+
 template <bool SHIFT, bool D3>
 __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stencil st,
                                                    const int32_t* __restrict__ start,
@@ -398,52 +438,15 @@ __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stenc
   const double* __restrict__ sx = spos;
   const double* __restrict__ sy = spos + n;
   const double* __restrict__ sz = spos + 2 * n;
-  const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
-  const double nan = __builtin_nan("");
   for (int64_t task = static_cast<int64_t>(blockIdx.x) * 4 + wave; task < ntasks;
        task += static_cast<int64_t>(gridDim.x) * 4) {
     const int cell = static_cast<int>(task / W);
     const int ch0 = static_cast<int>(task % W);
     const int cs = __builtin_amdgcn_readfirstlane(start[cell]);
     const int ce = __builtin_amdgcn_readfirstlane(start[cell + 1]);
-    const int cx = cell % ncx, cy = (cell / ncx) % ncy, cz = cell / (ncx * ncy);
-    for (int ch = ch0; cs + ch * kWave < ce; ch += W) {
-      const int gi = cs + ch * kWave + lane;
-      int ti = gi < ce ? stype[gi] : 0;
-      const bool oki = static_cast<unsigned>(ti - 1) < static_cast<unsigned>(a.nsp);
-      const double xi = oki ? sx[gi] : nan;
-      const double yi = oki ? sy[gi] : nan;
-      const double zi = (D3 && oki) ? sz[gi] : nan;
-      const int ti8 = (oki ? ti : 0) << 8;
-      for (int o = 0; o < st.n; ++o) {
-        int nx = cx + st.off[o][0], ny = cy + st.off[o][1], nz = cz + st.off[o][2];
-        // Image shift of a wrapped neighbour, the L*nint(d/L) the FP64 rule
-        // subtracts: a neighbour below cell 0 is the top cell (d ~ +L, subtract
-        // +L); one above the top cell is cell 0 (d ~ -L, subtract -L).
-        const double shx = nx < 0 ? a.lmi[0] : (nx >= ncx ? -a.lmi[0] : 0.0);
-        const double shy = ny < 0 ? a.lmi[1] : (ny >= ncy ? -a.lmi[1] : 0.0);
-        const double shz = nz < 0 ? a.lmi[2] : (nz >= ncz ? -a.lmi[2] : 0.0);
-        if (g.periodic[0]) nx = (nx + ncx) % ncx; else if (nx < 0 || nx >= ncx) continue;
-        if (g.periodic[1]) ny = (ny + ncy) % ncy; else if (ny < 0 || ny >= ncy) continue;
-        if (g.periodic[2]) nz = (nz + ncz) % ncz; else if (nz < 0 || nz >= ncz) continue;
-        const int nb = (nz * ncy + ny) * ncx + nx;
-        // 64 neighbours per pass: one coalesced load per lane, then a
-        // broadcast per pair through the scalar registers.
-        if (o == 0) {
-          rdf_neighbours<SHIFT, D3, true, false>(a, thr, hist, qr2, qtt, head, tail, lane, cs + ch * kWave + 1,
-                                                 ce, gi, xi, yi, zi, ti8, 0.0, 0.0, 0.0, sx, sy, sz, stype);
-        } else {
-          const int js = __builtin_amdgcn_readfirstlane(start[nb]);
-          const int je = __builtin_amdgcn_readfirstlane(start[nb + 1]);
-          if (shx != 0.0 || shy != 0.0 || shz != 0.0)
-            rdf_neighbours<SHIFT, D3, false, true>(a, thr, hist, qr2, qtt, head, tail, lane, js, je, gi, xi, yi,
-                                                   zi, ti8, shx, shy, shz, sx, sy, sz, stype);
-          else
-            rdf_neighbours<SHIFT, D3, false, false>(a, thr, hist, qr2, qtt, head, tail, lane, js, je, gi, xi, yi,
-                                                    zi, ti8, 0.0, 0.0, 0.0, sx, sy, sz, stype);
-        }
-      }
-    }
+    for (int ch = ch0; cs + ch * kWave < ce; ch += W)
+      rdf_home_chunk<SHIFT, D3>(a, thr, hist, qr2, qtt, head, tail, lane, st, g.nc, g.periodic, 0, cell, ch, cs,
+                                ce, start, sx, sy, sz, stype);
   }
   __builtin_amdgcn_wave_barrier();
   bin_hits(a, thr, hist, qr2, qtt, head, static_cast<int>(tail - head), lane);
@@ -1396,6 +1399,10 @@ __global__ __launch_bounds__(256) void bk_scatter(BatchDev d) {
 // Pair tasks of all frames: workgroup b owns the contiguous task range
 // [b*T/G, (b+1)*T/G) and walks it frame segment by frame segment, flushing
 // its LDS histogram into the frame's int64 histogram at each segment end.
+// Pairs of every frame of the group.  LDS: the four wavefront hit rings
+// first (fixed size), then the frame's histogram and bin edges.
+constexpr size_t kRingBytes = static_cast<size_t>(4) * kHitQ * (sizeof(double) + sizeof(int));
+
 __global__ __launch_bounds__(256) void bk_pairs(BatchDev d) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int nframes = d.hdr->nframes;
@@ -1404,6 +1411,9 @@ __global__ __launch_bounds__(256) void bk_pairs(BatchDev d) {
   block_chunk(T, blockIdx.x, gridDim.x, &t0, &t1);
   if (t0 >= t1) return;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  double* qr2 = reinterpret_cast<double*>(smem) + wave * kHitQ;
+  int* qtt = reinterpret_cast<int*>(smem + 4 * kHitQ * sizeof(double)) + wave * kHitQ;
+  unsigned char* hsm = smem + kRingBytes;
   const double* __restrict__ sx = d.spos;
   const double* __restrict__ sy = d.spos + d.cap_atoms;
   const double* __restrict__ sz = d.spos + 2 * d.cap_atoms;
@@ -1439,76 +1449,41 @@ __global__ __launch_bounds__(256) void bk_pairs(BatchDev d) {
     a.side2 = F.side2;
     a.inv_dr = F.inv_dr;
     const int nbins = a.nit * a.lsmax;
-    uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
-    double* thr = reinterpret_cast<double*>(smem + ((static_cast<size_t>(nbins) * 4 + 15) & ~size_t(15)));
+    uint32_t* hist = reinterpret_cast<uint32_t*>(hsm);
+    double* thr = reinterpret_cast<double*>(hsm + ((static_cast<size_t>(nbins) * 4 + 15) & ~size_t(15)));
     a.thr = d.thr;
     __syncthreads();
     hist_setup(a, hist, thr);
-    const int ncx = F.nc[0], ncy = F.nc[1], ncz = F.nc[2];
     bool shift = true;
     for (int k = 0; k < d.ndim; ++k) shift = shift && F.cellshift[k];
+    uint32_t head = 0, tail = 0;
     for (int64_t task = seg0 + wave; task < seg1; task += 4) {
       const int64_t lt = task - F.task_off;
       const int cellL = static_cast<int>(lt / F.W);
       const int ch0 = static_cast<int>(lt % F.W);
       const int cell = F.cell_off + cellL;
-      const int cs = d.start[cell], ce = d.start[cell + 1];
-      const int cx = cellL % ncx, cy = (cellL / ncx) % ncy, cz = cellL / (ncx * ncy);
+      const int cs = __builtin_amdgcn_readfirstlane(d.start[cell]);
+      const int ce = __builtin_amdgcn_readfirstlane(d.start[cell + 1]);
       for (int ch = ch0; cs + ch * kWave < ce; ch += F.W) {
-        const int gi = cs + ch * kWave + lane;
-        const bool vi = gi < ce;
-        const double xi = vi ? sx[gi] : 0.0;
-        const double yi = vi ? sy[gi] : 0.0;
-        const double zi = vi ? sz[gi] : 0.0;
-        const int ti = vi ? d.stype[gi] : 0;
-        const bool oki = vi && static_cast<unsigned>(ti - 1) < static_cast<unsigned>(a.nsp);
-        for (int o = 0; o < st.n; ++o) {
-          int nx = cx + st.off[o][0], ny = cy + st.off[o][1], nz = cz + st.off[o][2];
-          const double shx = nx < 0 ? a.lmi[0] : (nx >= ncx ? -a.lmi[0] : 0.0);
-          const double shy = ny < 0 ? a.lmi[1] : (ny >= ncy ? -a.lmi[1] : 0.0);
-          const double shz = nz < 0 ? a.lmi[2] : (nz >= ncz ? -a.lmi[2] : 0.0);
-          if (F.periodic[0]) nx = (nx + ncx) % ncx; else if (nx < 0 || nx >= ncx) continue;
-          if (F.periodic[1]) ny = (ny + ncy) % ncy; else if (ny < 0 || ny >= ncy) continue;
-          if (d.ndim == 3) {
-            if (F.periodic[2]) nz = (nz + ncz) % ncz; else if (nz < 0 || nz >= ncz) continue;
-          }
-          const int nb = F.cell_off + (nz * ncy + ny) * ncx + nx;
-          const bool self = (o == 0);
-          const int js = self ? cs + ch * kWave + 1 : d.start[nb];
-          const int je = d.start[nb + 1];
-          for (int jb = js; jb < je; jb += kWave) {
-            const int jl = jb + lane;
-            const bool vl = jl < je;
-            const double xl = vl ? sx[jl] : 0.0;
-            const double yl = vl ? sy[jl] : 0.0;
-            const double zl = vl ? sz[jl] : 0.0;
-            const int tl = vl ? d.stype[jl] : 0;
-            const int cnt = (je - jb) < kWave ? (je - jb) : kWave;
-            for (int jj = 0; jj < cnt; ++jj) {
-              const int j = jb + jj;
-              const int tj = __builtin_amdgcn_readlane(tl, jj);
-              const bool ok = oki && (!self || j > gi) &&
-                              static_cast<unsigned>(tj - 1) < static_cast<unsigned>(a.nsp);
-              if (shift) {
-                const double dx = (readlane_d(xl, jj) - xi) - shx;
-                const double dy = (readlane_d(yl, jj) - yi) - shy;
-                double r2;
-                if (a.ndim == 3) {
-                  const double dz = (readlane_d(zl, jj) - zi) - shz;
-                  r2 = dx * dx + dy * dy + dz * dz;
-                } else {
-                  r2 = dx * dx + dy * dy;
-                }
-                bin_pair(a, thr, hist, ok, r2, ti, tj);
-              } else {
-                pair_accumulate(a, thr, hist, ok, xi, yi, zi, ti, readlane_d(xl, jj), readlane_d(yl, jj),
-                                readlane_d(zl, jj), tj);
-              }
-            }
-          }
+        if (d.ndim == 3) {
+          if (shift)
+            rdf_home_chunk<true, true>(a, thr, hist, qr2, qtt, head, tail, lane, st, F.nc, F.periodic, F.cell_off,
+                                       cellL, ch, cs, ce, d.start, sx, sy, sz, d.stype);
+          else
+            rdf_home_chunk<false, true>(a, thr, hist, qr2, qtt, head, tail, lane, st, F.nc, F.periodic, F.cell_off,
+                                        cellL, ch, cs, ce, d.start, sx, sy, sz, d.stype);
+        } else {
+          if (shift)
+            rdf_home_chunk<true, false>(a, thr, hist, qr2, qtt, head, tail, lane, st, F.nc, F.periodic, F.cell_off,
+                                        cellL, ch, cs, ce, d.start, sx, sy, sz, d.stype);
+          else
+            rdf_home_chunk<false, false>(a, thr, hist, qr2, qtt, head, tail, lane, st, F.nc, F.periodic,
+                                         F.cell_off, cellL, ch, cs, ce, d.start, sx, sy, sz, d.stype);
         }
       }
     }
+    __builtin_amdgcn_wave_barrier();
+    bin_hits(a, thr, hist, qr2, qtt, head, static_cast<int>(tail - head), lane);
     __syncthreads();
     int64_t* out = d.hist + F.hist_off;
     for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
@@ -1677,7 +1652,7 @@ void launch_batch_group(const BatchDev& d, int nblocks, size_t pair_lds, hipStre
   hipLaunchKernelGGL(bk_scan_blocks, dim3(1), dim3(1024), 0, s, d, nblocks);
   hipLaunchKernelGGL(bk_scan_apply, dim3(nblocks), dim3(256), 0, s, d);
   hipLaunchKernelGGL(bk_scatter, dim3(nblocks), dim3(256), 0, s, d);
-  hipLaunchKernelGGL(bk_pairs, dim3(nblocks), dim3(256), pair_lds, s, d);
+  hipLaunchKernelGGL(bk_pairs, dim3(nblocks), dim3(256), kRingBytes + pair_lds, s, d);
 }
 
 }  // namespace trj
