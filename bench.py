@@ -299,6 +299,11 @@ def main():
             traffic = json.load(f).get(f"{args.workload}/{args.precision}/{kname}")
     achieved = flops / (kernel_ms["sq_contract"] * 1e-3)
     peak = PEAK[args.precision]
+    # RDF: in-range pairs per second of the RDF stage against the device's
+    # measured LDS-atomic rate (one histogram add per in-range pair).
+    lds_rate = T.lds_atomic_rate(0)
+    pairs_frame = pairs / args.steps
+    rdf_rate = pairs_frame / (kernel_ms["rdf"] * 1e-3) if kernel_ms["rdf"] > 0 else 0.0
     out = {
         "metric": METRIC,
         "value": fps,
@@ -324,6 +329,11 @@ def main():
         "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved / 1e12,
                      "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                      "algorithmic": f"8*N*Nq = {flops:.4e} flop per launch (N={n}, Nq={info.nvec_grid})"},
+        "rdf_roofline": {"bound": "lds_atomic", "achieved": rdf_rate, "peak": lds_rate, "unit": "pairs/s",
+                         "frac": rdf_rate / lds_rate, "pairs_per_frame": pairs_frame,
+                         "note": "in-range pairs of the RDF stage (cell build + pairs + merge) per second "
+                                 "against the measured LDS-atomic rate (trj_lds_atomic_rate); the pair "
+                                 "kernel is bound by its candidate loop (VALU), not by the atomics"},
     }
     ctx.close()
     if not args.no_cpu_baseline:

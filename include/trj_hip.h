@@ -213,6 +213,27 @@ int64_t trj_device_bytes(const trj_ctx* ctx);
 #define TRJ_PIPE_OVERLAP 1
 int trj_set_pipeline(trj_ctx* ctx, int mode);
 
+/* Accumulated results of a context -- RDF counts, S(Q) shell sums, frame counts,
+ * and the last frame's volume and species counts that trj_rdf_gmix /
+ * trj_sq_columns normalise with -- as a byte blob, for checkpoint and resume
+ * and for merging runs over different frames (the reference keeps these in
+ * module globals, histomix / sqf / sqfp, rdf.cuf:301-310, sq.cuf:128-148,
+ * and has no save).  trj_state_bytes gives the size; trj_state_load checks
+ * that the blob belongs to the same RDF / S(Q) plans and either replaces the
+ * context's accumulators (TRJ_STATE_REPLACE) or adds to them
+ * (TRJ_STATE_ADD).  Both synchronise. */
+#define TRJ_STATE_REPLACE 0
+#define TRJ_STATE_ADD 1
+int64_t trj_state_bytes(const trj_ctx* ctx);
+int trj_state_save(trj_ctx* ctx, void* buf, int64_t bytes);
+int trj_state_load(trj_ctx* ctx, const void* buf, int64_t bytes, int mode);
+
+/* Measurement probe, no counterpart in the reference: the device's LDS-atomic
+ * rate (uint32 adds per second, all CUs, 64-lane waves adding to random bins
+ * of a 4096-bin histogram), the ceiling the RDF's per-pair histogram update
+ * is priced against.  Runs a short kernel on the default stream of `device`. */
+int trj_lds_atomic_rate(int device, double* per_s);
+
 /* ------------------------------------------------------------------------
  * Batched frames: many independent frames of uneven size and species count,
  * each with its own pair histogram (the frame loop of trj_analysis.f90:173-242

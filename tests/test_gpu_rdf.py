@@ -155,3 +155,10 @@ def test_cell_shift_path_bit_exact(L, rc, n):
     g, _ = _gpu_counts([(pos, it, [L] * 3)], 3, 3, 0.01, rc)
     o = _oracle_counts([(pos, it, [L] * 3)], 3, 3, 0.01, rc, method="cells")
     np.testing.assert_array_equal(g, o)
+
+
+def test_lds_atomic_probe():
+    print("case: LDS-atomic rate probe", flush=True)
+    r = T.lds_atomic_rate(0)
+    print("LDS atomics/s", r, flush=True)
+    assert 1e11 < r < 1e15

@@ -98,6 +98,10 @@ void launch_sq_list(const double* u, const int32_t* sp_off, int64_t n, int nsp, 
                     const int32_t* kv, const int32_t* shell, int nvec, const double* bsc,
                     int nqmax, double* part, double* sqf, double* sqfp, hipStream_t s);
 
+// LDS-atomic throughput probe (random bins of a 4096-bin uint32 histogram);
+// returns the number of atomics the launch performs; *out gets their sum.
+int64_t launch_lds_atomics(int nblocks, int iters, uint32_t* out, hipStream_t s);
+
 // ------------------------------------------------------------------ batches
 // Frames of one launch group, concatenated.  Device-side descriptors drive
 // fixed-grid kernels, so one captured launch sequence serves every group.

@@ -1494,6 +1494,34 @@ __global__ __launch_bounds__(256) void bk_pairs(BatchDev d) {
   }
 }
 
+// LDS-atomic throughput probe: every lane adds 1 to 16 pseudo-random bins of
+// a 4096-bin uint32 histogram in LDS, `iters` times over (the bins are drawn
+// once, so the loop issues almost nothing but ds_add_u32).
+constexpr int kProbeBins = 4096;
+
+__global__ __launch_bounds__(256) void k_lds_atomics(int iters, uint32_t* __restrict__ out) {
+  __shared__ uint32_t h[kProbeBins];
+  for (int b = threadIdx.x; b < kProbeBins; b += 256) h[b] = 0u;
+  __syncthreads();
+  uint32_t x = (blockIdx.x * 256u + threadIdx.x) * 2654435761u + 0x9e3779b9u;
+  uint32_t off[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    off[k] = x & (kProbeBins - 1);
+  }
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) atomicAdd(&h[off[k]], 1u);
+  }
+  __syncthreads();
+  uint32_t sum = 0;
+  for (int b = threadIdx.x; b < kProbeBins; b += 256) sum += h[b];
+  atomicAdd(out, sum);
+}
+
 inline unsigned grid1d(int64_t n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
 
 }  // namespace
@@ -1643,6 +1671,11 @@ void launch_sq_list(const double* u, const int32_t* sp_off, int64_t n, int nsp, 
   hipLaunchKernelGGL(k_sq_list, dim3(nvec, kListBlocks), dim3(256), 0, s, u, sp_off, n, nsp, ndim, kv, part);
   hipLaunchKernelGGL(k_sq_list_finish, dim3((nvec + 63) / 64), dim3(64), 0, s, part, shell, nvec, nsp, b, nqmax,
                      sqf, sqfp);
+}
+
+int64_t launch_lds_atomics(int nblocks, int iters, uint32_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_lds_atomics, dim3(nblocks), dim3(256), 0, s, iters, out);
+  return static_cast<int64_t>(nblocks) * 256 * 16 * iters;
 }
 
 void launch_batch_group(const BatchDev& d, int nblocks, size_t pair_lds, hipStream_t s) {

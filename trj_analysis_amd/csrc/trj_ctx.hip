@@ -839,6 +839,200 @@ int trj_reset(trj_ctx* c) {
   return TRJ_OK;
 }
 
+// ---------------------------------------------------------------- state blob
+// Accumulated results of a context: RDF counts, S(Q) shell sums, frame counts
+// and the last frame's volume and species counts (what the normalisations
+// read), behind a header that identifies the plans they belong to.
+
+namespace {
+struct StateHeader {
+  char magic[8];
+  int32_t version, has_rdf, has_sq, nsp;
+  int32_t rdf_ndim, nit, lsmax, rdf_periodic[3];
+  double deltar, rc;
+  int32_t sq_ndim, nqmax, qset, pad0;
+  double dq, qbox[3];
+  int64_t nvec_grid, nvec_axis;
+  int64_t rdf_frames, sq_frames, last_n;
+  double last_volume;
+  int32_t last_nsp, pad1;
+  int64_t ntype[kNspMax];
+};
+constexpr char kStateMagic[8] = {'T', 'R', 'J', 'S', 'T', 'A', 'T', 'E'};
+
+int64_t state_payload(const trj_ctx* c) {
+  int64_t b = 0;
+  if (c->rdf_ready) b += int64_t(8) * c->rdf.nit * c->rdf.lsmax;
+  if (c->sq_ready) b += int64_t(8) * (c->sq.nqmax + 1) * (1 + c->sq.p.nsp);
+  return b;
+}
+
+StateHeader state_header(const trj_ctx* c) {
+  StateHeader h{};
+  std::memcpy(h.magic, kStateMagic, 8);
+  h.version = 1;
+  h.has_rdf = c->rdf_ready;
+  h.has_sq = c->sq_ready;
+  h.nsp = c->rdf_ready ? c->rdf.nsp : (c->sq_ready ? c->sq.p.nsp : 0);
+  if (c->rdf_ready) {
+    h.rdf_ndim = c->rdf.ndim;
+    h.nit = c->rdf.nit;
+    h.lsmax = c->rdf.lsmax;
+    for (int k = 0; k < 3; ++k) h.rdf_periodic[k] = c->rdf.periodic[k];
+    h.deltar = c->rdf.deltar;
+    h.rc = c->rdf.rc;
+  }
+  if (c->sq_ready) {
+    h.sq_ndim = c->sq.p.ndim;
+    h.nqmax = c->sq.nqmax;
+    h.qset = c->sq.p.qset;
+    h.dq = c->sq.info.dq;
+    for (int k = 0; k < 3; ++k) h.qbox[k] = c->sq.box[k];
+    h.nvec_grid = c->sq.info.nvec_grid;
+    h.nvec_axis = c->sq.info.nvec_axis;
+  }
+  h.rdf_frames = c->rdf_frames;
+  h.sq_frames = c->sq_frames;
+  h.last_n = c->last_n;
+  h.last_volume = c->last_volume;
+  h.last_nsp = c->last_nsp;
+  return h;
+}
+}  // namespace
+
+int64_t trj_state_bytes(const trj_ctx* c) {
+  if (!c) return TRJ_EINVAL;
+  return static_cast<int64_t>(sizeof(StateHeader)) + state_payload(c);
+}
+
+int trj_state_save(trj_ctx* c, void* buf, int64_t bytes) {
+  if (!c || !buf) return TRJ_EINVAL;
+  if (bytes < trj_state_bytes(c)) return fail(c, TRJ_EINVAL, "state buffer too small");
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  StateHeader h = state_header(c);
+  if (c->last_nsp > 0 && (c->rdf_frames > 0 || c->sq_frames > 0)) {
+    std::vector<int64_t> nt;
+    if ((r = read_ntype(c, nt))) return r;
+    for (int k = 0; k < c->last_nsp; ++k) h.ntype[k] = nt[k];
+  }
+  unsigned char* p = static_cast<unsigned char*>(buf);
+  std::memcpy(p, &h, sizeof(h));
+  p += sizeof(h);
+  if (c->rdf_ready) {
+    const size_t b = sizeof(int64_t) * c->rdf.nit * c->rdf.lsmax;
+    HIPC(c, hipMemcpy(p, c->counts.p, b, hipMemcpyDeviceToHost));
+    p += b;
+  }
+  if (c->sq_ready) {
+    const size_t ns = c->sq.nqmax + 1;
+    HIPC(c, hipMemcpy(p, c->sqf.p, sizeof(double) * ns, hipMemcpyDeviceToHost));
+    p += sizeof(double) * ns;
+    HIPC(c, hipMemcpy(p, c->sqfp.p, sizeof(double) * ns * c->sq.p.nsp, hipMemcpyDeviceToHost));
+  }
+  return TRJ_OK;
+}
+
+int trj_state_load(trj_ctx* c, const void* buf, int64_t bytes, int mode) {
+  if (!c || !buf) return TRJ_EINVAL;
+  if (mode != TRJ_STATE_REPLACE && mode != TRJ_STATE_ADD) return fail(c, TRJ_EINVAL, "unknown state load mode");
+  if (bytes < static_cast<int64_t>(sizeof(StateHeader))) return fail(c, TRJ_EINVAL, "state blob too short");
+  StateHeader h;
+  std::memcpy(&h, buf, sizeof(h));
+  if (std::memcmp(h.magic, kStateMagic, 8) != 0 || h.version != 1)
+    return fail(c, TRJ_EINVAL, "not a trj_hip state blob (or another version)");
+  const StateHeader m = state_header(c);
+  const bool same = h.has_rdf == m.has_rdf && h.has_sq == m.has_sq && h.nsp == m.nsp && h.rdf_ndim == m.rdf_ndim &&
+                    h.nit == m.nit && h.lsmax == m.lsmax && h.deltar == m.deltar && h.rc == m.rc &&
+                    h.rdf_periodic[0] == m.rdf_periodic[0] && h.rdf_periodic[1] == m.rdf_periodic[1] &&
+                    h.rdf_periodic[2] == m.rdf_periodic[2] && h.sq_ndim == m.sq_ndim && h.nqmax == m.nqmax &&
+                    h.qset == m.qset && h.dq == m.dq && h.qbox[0] == m.qbox[0] && h.qbox[1] == m.qbox[1] &&
+                    h.qbox[2] == m.qbox[2] && h.nvec_grid == m.nvec_grid && h.nvec_axis == m.nvec_axis;
+  if (!same) return fail(c, TRJ_EINVAL, "state blob belongs to other RDF / S(Q) plans than this context's");
+  if (bytes < trj_state_bytes(c)) return fail(c, TRJ_EINVAL, "state blob too short");
+  if (h.last_nsp < 0 || h.last_nsp > kNspMax) return fail(c, TRJ_EINVAL, "corrupt state blob");
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  const unsigned char* p = static_cast<const unsigned char*>(buf) + sizeof(h);
+  const bool add = mode == TRJ_STATE_ADD;
+  if (c->rdf_ready) {
+    const size_t nb = static_cast<size_t>(c->rdf.nit) * c->rdf.lsmax;
+    std::vector<int64_t> v(nb), cur(add ? nb : 0);
+    std::memcpy(v.data(), p, sizeof(int64_t) * nb);
+    p += sizeof(int64_t) * nb;
+    if (add) {
+      HIPC(c, hipMemcpy(cur.data(), c->counts.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < nb; ++k) v[k] += cur[k];
+    }
+    HIPC(c, hipMemcpy(c->counts.p, v.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice));
+  }
+  if (c->sq_ready) {
+    const size_t nv = static_cast<size_t>(c->sq.nqmax + 1) * (1 + c->sq.p.nsp);
+    std::vector<double> v(nv), cur(add ? nv : 0);
+    std::memcpy(v.data(), p, sizeof(double) * nv);
+    if (add) {
+      const size_t ns = c->sq.nqmax + 1;
+      HIPC(c, hipMemcpy(cur.data(), c->sqf.p, sizeof(double) * ns, hipMemcpyDeviceToHost));
+      HIPC(c, hipMemcpy(cur.data() + ns, c->sqfp.p, sizeof(double) * (nv - ns), hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < nv; ++k) v[k] += cur[k];
+    }
+    const size_t ns = c->sq.nqmax + 1;
+    HIPC(c, hipMemcpy(c->sqf.p, v.data(), sizeof(double) * ns, hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(c->sqfp.p, v.data() + ns, sizeof(double) * (nv - ns), hipMemcpyHostToDevice));
+  }
+  // The normalisation inputs: the loaded ones unless this context has
+  // processed frames of its own and is adding.
+  const bool own = add && (c->rdf_frames > 0 || c->sq_frames > 0);
+  c->rdf_frames = (add ? c->rdf_frames : 0) + h.rdf_frames;
+  c->sq_frames = (add ? c->sq_frames : 0) + h.sq_frames;
+  if (!own && h.last_nsp > 0) {
+    c->last_n = h.last_n;
+    c->last_volume = h.last_volume;
+    c->last_nsp = h.last_nsp;
+    std::vector<int32_t> off(h.last_nsp + 1, 0);
+    for (int k = 0; k < h.last_nsp; ++k) off[k + 1] = off[k] + static_cast<int32_t>(h.ntype[k]);
+    if ((r = ensure(c, c->sp_off, sizeof(int32_t) * off.size()))) return r;
+    HIPC(c, hipMemcpy(c->sp_off.p, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice));
+  }
+  return TRJ_OK;
+}
+
+int trj_lds_atomic_rate(int device, double* per_s) {
+  if (!per_s) return TRJ_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return TRJ_EDEVICE;
+  if (hipSetDevice(device) != hipSuccess) return TRJ_EDEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return TRJ_EDEVICE;
+  uint32_t* out = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = TRJ_EDEVICE;
+  if (hipMalloc(&out, sizeof(uint32_t)) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+      hipEventCreate(&e1) == hipSuccess) {
+    const int nblocks = prop.multiProcessorCount * 8;
+    const int iters = 2048;
+    hipMemset(out, 0, sizeof(uint32_t));
+    launch_lds_atomics(nblocks, 64, out, nullptr);  // warm-up
+    hipEventRecord(e0, nullptr);
+    const int64_t total = launch_lds_atomics(nblocks, iters, out, nullptr);
+    hipEventRecord(e1, nullptr);
+    if (hipEventSynchronize(e1) == hipSuccess) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      if (ms > 0.f) {
+        *per_s = static_cast<double>(total) / (ms * 1e-3);
+        rc = TRJ_OK;
+      }
+    }
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (out) hipFree(out);
+  return rc;
+}
+
 int trj_last_timings(trj_ctx* c, double* ms4) {
   if (!c || !ms4) return TRJ_EINVAL;
   hipSetDevice(c->device);

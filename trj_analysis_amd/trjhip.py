@@ -25,6 +25,7 @@ TRJ_QSET_REF, TRJ_QSET_HALFSPACE = 0, 1
 TRJ_PREC_FP64, TRJ_PREC_FP32, TRJ_PREC_FP16X3, TRJ_PREC_BF16X3 = 0, 1, 2, 3
 TRJ_SQ_NQ_CONSISTENT = 1
 TRJ_PIPE_SERIAL, TRJ_PIPE_OVERLAP = 0, 1
+TRJ_STATE_REPLACE, TRJ_STATE_ADD = 0, 1
 
 PRECISIONS = {"fp64": TRJ_PREC_FP64, "fp32": TRJ_PREC_FP32, "fp16x3": TRJ_PREC_FP16X3,
               "bf16x3": TRJ_PREC_BF16X3}
@@ -35,7 +36,9 @@ EXPORTS = [
     "trj_sq_init", "trj_sq_plan", "trj_rdf_plan", "trj_frame", "trj_frame_device", "trj_sync",
     "trj_rdf_counts", "trj_rdf_gmix", "trj_rdf_first_minimum", "trj_sq_shells", "trj_sq_columns",
     "trj_sq_rho", "trj_reset", "trj_last_timings", "trj_timing_totals", "trj_timing_reset",
-    "trj_device_bytes", "trj_set_pipeline", "trj_batch_create", "trj_batch_destroy", "trj_batch_last_error",
+    "trj_device_bytes", "trj_set_pipeline", "trj_state_bytes", "trj_state_save", "trj_state_load",
+    "trj_lds_atomic_rate",
+    "trj_batch_create", "trj_batch_destroy", "trj_batch_last_error",
     "trj_batch_layout", "trj_batch_run", "trj_batch_autotune", "trj_srv_create", "trj_srv_destroy",
     "trj_srv_last_error", "trj_request_footprint", "trj_srv_submit", "trj_srv_poll", "trj_srv_result",
     "trj_srv_stats",
@@ -125,6 +128,11 @@ def _declare(L):
     L.trj_device_bytes.argtypes = [vp]
     L.trj_device_bytes.restype = C.c_int64
     L.trj_set_pipeline.argtypes = [vp, C.c_int]
+    L.trj_state_bytes.argtypes = [vp]
+    L.trj_state_bytes.restype = C.c_int64
+    L.trj_state_save.argtypes = [vp, vp, C.c_int64]
+    L.trj_state_load.argtypes = [vp, vp, C.c_int64, C.c_int]
+    L.trj_lds_atomic_rate.argtypes = [C.c_int, d]
     L.trj_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(BatchParams)]
     L.trj_batch_destroy.argtypes = [vp]
     L.trj_batch_destroy.restype = None
@@ -147,10 +155,20 @@ def _declare(L):
     L.trj_srv_result.argtypes = [vp, C.c_int64, i64, d, d, i64]
     L.trj_srv_stats.argtypes = [vp, i32, i32, i64]
     no_int = ("trj_destroy", "trj_last_error", "trj_device_bytes", "trj_batch_destroy",
-              "trj_batch_last_error", "trj_srv_destroy", "trj_srv_last_error", "trj_request_footprint")
+              "trj_batch_last_error", "trj_srv_destroy", "trj_srv_last_error", "trj_request_footprint",
+              "trj_state_bytes")
     for name in EXPORTS:
         if name not in no_int:
             getattr(L, name).restype = C.c_int
+
+
+def lds_atomic_rate(device: int = 0) -> float:
+    """LDS-atomic adds per second of the device (trj_lds_atomic_rate)."""
+    out = C.c_double(0.0)
+    rc = lib().trj_lds_atomic_rate(int(device), C.byref(out))
+    if rc != TRJ_OK:
+        raise TrjError(rc, "trj_lds_atomic_rate")
+    return out.value
 
 
 def _ptr(a: np.ndarray, ct):
@@ -361,6 +379,20 @@ class Context:
 
     def device_bytes(self) -> int:
         return self._L.trj_device_bytes(self._h)
+
+    def state_save(self) -> bytes:
+        """The context's accumulated results as bytes (trj_state_save)."""
+        nb = self._L.trj_state_bytes(self._h)
+        if nb < 0:
+            raise TrjError(nb, "trj_state_bytes")
+        buf = C.create_string_buffer(nb)
+        self._check(self._L.trj_state_save(self._h, buf, nb), "trj_state_save")
+        return buf.raw
+
+    def state_load(self, blob: bytes, mode: int = TRJ_STATE_REPLACE):
+        """Replace (TRJ_STATE_REPLACE) or add to (TRJ_STATE_ADD) the accumulators."""
+        buf = C.create_string_buffer(bytes(blob), len(blob))
+        self._check(self._L.trj_state_load(self._h, buf, len(blob), int(mode)), "trj_state_load")
 
     def set_pipeline(self, mode: int):
         """TRJ_PIPE_SERIAL or TRJ_PIPE_OVERLAP (RDF on its own stream beside S(Q))."""
