@@ -14,6 +14,8 @@ module trj_hip
   integer(c_int32_t), parameter :: TRJ_PREC_FP64 = 0, TRJ_PREC_FP32 = 1, &
                                    TRJ_PREC_FP16X3 = 2, TRJ_PREC_BF16X3 = 3
   integer(c_int32_t), parameter :: TRJ_SQ_NQ_CONSISTENT = 1
+  integer(c_int), parameter :: TRJ_PIPE_SERIAL = 0, TRJ_PIPE_OVERLAP = 1
+  integer(c_int), parameter :: TRJ_STATE_REPLACE = 0, TRJ_STATE_ADD = 1
 
   type, bind(C) :: trj_rdf_params
     integer(c_int32_t) :: ndim, nsp
@@ -129,6 +131,28 @@ module trj_hip
     integer(c_int) function trj_reset(ctx) bind(C, name="trj_reset")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function trj_set_pipeline(ctx, mode) bind(C, name="trj_set_pipeline")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
+    end function
+    integer(c_int64_t) function trj_state_bytes(ctx) bind(C, name="trj_state_bytes")
+      import :: c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function trj_state_save(ctx, buf, bytes) bind(C, name="trj_state_save")
+      import :: c_int, c_ptr, c_int64_t, c_int8_t
+      type(c_ptr), value :: ctx
+      integer(c_int8_t), intent(out) :: buf(*)
+      integer(c_int64_t), value :: bytes
+    end function
+    integer(c_int) function trj_state_load(ctx, buf, bytes, mode) bind(C, name="trj_state_load")
+      import :: c_int, c_ptr, c_int64_t, c_int8_t
+      type(c_ptr), value :: ctx
+      integer(c_int8_t), intent(in) :: buf(*)
+      integer(c_int64_t), value :: bytes
+      integer(c_int), value :: mode
     end function
   end interface
 
