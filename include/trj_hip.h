@@ -235,6 +235,44 @@ int trj_state_load(trj_ctx* ctx, const void* buf, int64_t bytes, int mode);
 int trj_lds_atomic_rate(int device, double* per_s);
 
 /* ------------------------------------------------------------------------
+ * NetCDF trajectory input (host only): LAMMPS `dump netcdf` files in the
+ * classic NetCDF formats (CDF-1, CDF-2 64-bit offset, CDF-5 64-bit data),
+ * read without the NetCDF library.  Replaces the nf90_* reads of read_nc_cfg
+ * (netcdf.f90:173-550) for coordinates(frame,atom,spatial), type(frame,atom),
+ * cell_origin and cell_lengths(frame,cell_spatial).  netCDF-4 (HDF5) files
+ * are refused; `nccopy -k cdf5` converts them. */
+typedef struct trj_nc trj_nc;
+typedef struct {
+  int64_t nframes, natoms;
+  int32_t version;      /* 1, 2 or 5 */
+  int32_t coord_dtype;  /* TRJ_F32 / TRJ_F64, as stored */
+  int32_t has_type, has_cell_origin, has_cell_lengths, has_time;
+} trj_nc_file_info;
+/* On failure *out is NULL and errbuf (errlen bytes, may be NULL) holds the reason. */
+int trj_nc_open(trj_nc** out, const char* path, char* errbuf, int32_t errlen);
+void trj_nc_close(trj_nc* h);
+const char* trj_nc_last_error(const trj_nc* h);
+int trj_nc_info(const trj_nc* h, trj_nc_file_info* info);
+/* Frame `frame` (0-based) as stored: pos AoS [natoms][3], type[natoms] (may
+ * be NULL), cell origin and lengths (zeros when the file has none). */
+int trj_nc_read(trj_nc* h, int64_t frame, double* pos, int32_t* type, double origin[3], double lengths[3]);
+/* time(frame) of frame `frame` (0-based) as stored; the reference reads it
+ * into the integer nstep and skips frames whose nstep is 0 (netcdf.f90:432-436,
+ * trj_analysis.f90:153-157, 196-198). */
+int trj_nc_time(trj_nc* h, int64_t frame, double* t);
+/* Frame `frame` as read_nc_cfg / select_ncdfinput hand it on (netcdf.f90:
+ * 317-357, 456-531, 582-650): the LAMMPS types sp_types[0..nsp-1] are the
+ * species, numbered 1..nsp in order of first appearance in the first frame
+ * read (the map is then fixed); atoms of other types are dropped.  periodic
+ * is in/out like mod_common's: in, the /INPUT/ periodic flags -- those axes
+ * get cell_origin subtracted; out, cleared where the cell length is zero,
+ * which gives the axis the length span + 8.  Non-periodic coordinates are
+ * translated to sit centred in [0, L) (to start at 4 for span + 8).  Output:
+ * pos_soa x[n] y[n] z[n] and itype[n] (capacity natoms each), n, box. */
+int trj_nc_frame_ref(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_types, double* pos_soa,
+                     int32_t* itype, int64_t* n_out, double box[3], int32_t periodic[3]);
+
+/* ------------------------------------------------------------------------
  * Batched frames: many independent frames of uneven size and species count,
  * each with its own pair histogram (the frame loop of trj_analysis.f90:173-242
  * with RDFcomp per frame, rdf.cuf:184-353), run as groups of frames in

@@ -12,10 +12,41 @@ def test_fortran_abi_check():
     assert os.path.exists(EXE), "run __graft_entry__.build() first (amdflang)"
     out = subprocess.run([EXE], capture_output=True, text=True, check=True).stdout.split("\n")
     vals = {line.split()[0]: line.split()[1:] for line in out if line.strip()}
-    assert vals["sizes"] == ["40", "128", "104"]
+    assert vals["sizes"] == ["40", "128", "104", "40"]
     assert vals["abi"] == ["1"]
     assert vals["plan"] == ["67", "14", "14"]
     with open(os.path.join(ROOT, "tests", "golden", "nq_golden.json")) as f:
         lj = [c for c in json.load(f)["cases"] if c["name"] == "3D/LJmix"][0]
     assert [int(v) for v in vals["nq"]] == lj["nq"][:16]
     assert int(vals["rdf"][0]) - 2 == lj["gmix_rows"]
+
+
+def test_fortran_netcdf(tmp_path):
+    """trj_nc_* through the Fortran interfaces: frame 0 of a CDF-2 file with
+    types 2 and 1 (first appearance 2, so type 2 is species 1), origin
+    subtracted, z written with zero length (non-periodic, starts at 4)."""
+    from scipy.io import netcdf_file
+    import numpy as np
+
+    path = str(tmp_path / "f.nc")
+    xyz = np.array([[1.5, 2.5, -3.0], [4.0, 5.0, 6.0], [7.0, 1.0, 2.0]], dtype=np.float32)
+    with netcdf_file(path, "w", version=2) as nc:
+        nc.createDimension("frame", None)
+        nc.createDimension("atom", 3)
+        nc.createDimension("spatial", 3)
+        nc.createDimension("cell_spatial", 3)
+        t = nc.createVariable("time", "d", ("frame",))
+        co = nc.createVariable("cell_origin", "f", ("frame", "cell_spatial"))
+        cl = nc.createVariable("cell_lengths", "f", ("frame", "cell_spatial"))
+        ty = nc.createVariable("type", "i", ("frame", "atom"))
+        c = nc.createVariable("coordinates", "f", ("frame", "atom", "spatial"))
+        for f in range(2):
+            t[f] = 250.0 * f
+            co[f] = [0.5, -1.0, 0.0]
+            cl[f] = [10.0, 10.0, 0.0]
+            ty[f] = [2, 1, 2]
+            c[f] = xyz
+    out = subprocess.run([EXE, path], capture_output=True, text=True, check=True).stdout.split("\n")
+    vals = {line.split()[0]: line.split()[1:] for line in out if line.strip()}
+    assert vals["nc"] == ["2", "2", "3", "1", "1", "1", "0", "250.00"]
+    assert [float(v) for v in vals["ncpos"]] == [1.0, 3.5, -3.0 + 4.0 + 3.0]

@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libtrjhip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TRJ_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "trj_ctx.hip", "batch.hip", "serve.hip", "plan.cpp"]
+SOURCES = ["kernels.hip", "trj_ctx.hip", "batch.hip", "serve.hip", "plan.cpp", "ncread.cpp"]
 # -ffp-contract=off: the RDF pair rule and the FP32 q-set bookkeeping must
 # round once per operation, as written.  Packed-FP32 instructions are disabled.
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-result", "-Wno-unused-value"]

@@ -39,6 +39,13 @@ module trj_hip
     integer(c_int64_t) :: nvec_grid, nvec_axis, box_ext(3)
   end type
 
+  ! NetCDF trajectory (classic formats); the handle is an opaque type(c_ptr)
+  type, bind(C) :: trj_nc_file_info
+    integer(c_int64_t) :: nframes, natoms
+    integer(c_int32_t) :: version, coord_dtype
+    integer(c_int32_t) :: has_type, has_cell_origin, has_cell_lengths, has_time
+  end type
+
   interface
     integer(c_int) function trj_abi_version() bind(C, name="trj_abi_version")
       import :: c_int
@@ -153,6 +160,56 @@ module trj_hip
       integer(c_int8_t), intent(in) :: buf(*)
       integer(c_int64_t), value :: bytes
       integer(c_int), value :: mode
+    end function
+    ! path: a C string (trim(name)//c_null_char)
+    integer(c_int) function trj_nc_open(h, path, errbuf, errlen) bind(C, name="trj_nc_open")
+      import :: c_int, c_ptr, c_char, c_int32_t
+      type(c_ptr), intent(out) :: h
+      character(kind=c_char), intent(in) :: path(*)
+      character(kind=c_char), intent(out) :: errbuf(*)
+      integer(c_int32_t), value :: errlen
+    end function
+    subroutine trj_nc_close(h) bind(C, name="trj_nc_close")
+      import :: c_ptr
+      type(c_ptr), value :: h
+    end subroutine
+    type(c_ptr) function trj_nc_last_error(h) bind(C, name="trj_nc_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function trj_nc_info(h, info) bind(C, name="trj_nc_info")
+      import :: c_int, c_ptr, trj_nc_file_info
+      type(c_ptr), value :: h
+      type(trj_nc_file_info), intent(out) :: info
+    end function
+    integer(c_int) function trj_nc_time(h, frame, t) bind(C, name="trj_nc_time")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: h
+      integer(c_int64_t), value :: frame
+      real(c_double), intent(out) :: t
+    end function
+    ! frame is 0-based; pos(3*natoms) as stored (AoS); type(natoms)
+    integer(c_int) function trj_nc_read(h, frame, pos, type, origin, lengths) bind(C, name="trj_nc_read")
+      import :: c_int, c_ptr, c_int64_t, c_double, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int64_t), value :: frame
+      real(c_double), intent(out) :: pos(*)
+      integer(c_int32_t), intent(out) :: type(*)
+      real(c_double), intent(out) :: origin(3), lengths(3)
+    end function
+    ! pos_soa(natoms*3): x(1:n) y(1:n) z(1:n); periodic in: /INPUT/ flags, out: updated
+    integer(c_int) function trj_nc_frame_ref(h, frame, nsp, sp_types, pos_soa, itype, n, box, periodic) &
+        bind(C, name="trj_nc_frame_ref")
+      import :: c_int, c_ptr, c_int64_t, c_double, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int64_t), value :: frame
+      integer(c_int32_t), value :: nsp
+      integer(c_int32_t), intent(in) :: sp_types(*)
+      real(c_double), intent(out) :: pos_soa(*)
+      integer(c_int32_t), intent(out) :: itype(*)
+      integer(c_int64_t), intent(out) :: n
+      real(c_double), intent(out) :: box(3)
+      integer(c_int32_t), intent(inout) :: periodic(3)
     end function
   end interface
 

@@ -1,8 +1,10 @@
 """Reader of the reference's Fortran namelists for the structural path.
 
 Restates the parts of src/io/input.f90 the hot path uses:
-  /INPUT/      ndim, nsp, nthread, periodic, ncfs_from_to, rdf_sq_cl_dyn_sqw_conf_ord
-               (input.f90:64-67; nthread defaults to 64, input.f90:51)
+  /INPUT/      trj_input_file, ndim, nsp, nthread, periodic, ncfs_from_to,
+               rdf_sq_cl_dyn_sqw_conf_ord (input.f90:64-67; nthread defaults
+               to 64, input.f90:51)
+  /INPUT_SP/   sp_types_selected (input.f90:68, 100-102; zeros where unset)
   /INPUT_RDF/  deltar, rcrdf, nrandom (input.f90:69); deltar = -1 sentinel (:143)
   /INPUT_SQ/   qmax, qmin, bsc (input.f90:70); qmax = -1 sentinel (:147),
                bsc defaults to 1.0 per species (:106-107)
@@ -70,6 +72,7 @@ def parse_groups(text: str) -> Dict[str, Dict[str, List]]:
 
 @dataclass
 class StructuralInput:
+    trj_input_file: str = ""
     ndim: int = 3
     nsp: int = 1
     nthread: int = 64
@@ -82,6 +85,7 @@ class StructuralInput:
     qmax: float = -1.0
     qmin: Optional[float] = None
     bsc: List[float] = field(default_factory=list)
+    sp_types_selected: List[int] = field(default_factory=list)
 
     @property
     def run_rdf(self) -> bool:
@@ -97,6 +101,8 @@ def read_input(text: str) -> StructuralInput:
     g = parse_groups(text)
     inp = StructuralInput()
     i = g.get("input", {})
+    if "trj_input_file" in i:
+        inp.trj_input_file = str(i["trj_input_file"][0])
     if "ndim" in i:
         inp.ndim = int(i["ndim"][0])
     if "nsp" in i:
@@ -115,6 +121,9 @@ def read_input(text: str) -> StructuralInput:
         f = [bool(v) for v in i["rdf_sq_cl_dyn_sqw_conf_ord"]] + [False] * 7
         inp.flags = tuple(f[:7])
     inp.bsc = [1.0] * inp.nsp
+    sp = g.get("input_sp", {})
+    sel = [int(x) for x in sp.get("sp_types_selected", [])]
+    inp.sp_types_selected = (sel + [0] * inp.nsp)[: inp.nsp]
     fl = inp.flags
     if fl[0] or fl[2] or fl[5]:
         r = g.get("input_rdf", {})

@@ -37,7 +37,8 @@ EXPORTS = [
     "trj_rdf_counts", "trj_rdf_gmix", "trj_rdf_first_minimum", "trj_sq_shells", "trj_sq_columns",
     "trj_sq_rho", "trj_reset", "trj_last_timings", "trj_timing_totals", "trj_timing_reset",
     "trj_device_bytes", "trj_set_pipeline", "trj_state_bytes", "trj_state_save", "trj_state_load",
-    "trj_lds_atomic_rate",
+    "trj_lds_atomic_rate", "trj_nc_open", "trj_nc_close", "trj_nc_last_error", "trj_nc_info", "trj_nc_read",
+    "trj_nc_time", "trj_nc_frame_ref",
     "trj_batch_create", "trj_batch_destroy", "trj_batch_last_error",
     "trj_batch_layout", "trj_batch_run", "trj_batch_autotune", "trj_srv_create", "trj_srv_destroy",
     "trj_srv_last_error", "trj_request_footprint", "trj_srv_submit", "trj_srv_poll", "trj_srv_result",
@@ -133,6 +134,15 @@ def _declare(L):
     L.trj_state_save.argtypes = [vp, vp, C.c_int64]
     L.trj_state_load.argtypes = [vp, vp, C.c_int64, C.c_int]
     L.trj_lds_atomic_rate.argtypes = [C.c_int, d]
+    L.trj_nc_open.argtypes = [C.POINTER(vp), C.c_char_p, C.c_char_p, C.c_int32]
+    L.trj_nc_close.argtypes = [vp]
+    L.trj_nc_close.restype = None
+    L.trj_nc_last_error.argtypes = [vp]
+    L.trj_nc_last_error.restype = C.c_char_p
+    L.trj_nc_info.argtypes = [vp, C.POINTER(NcFileInfo)]
+    L.trj_nc_read.argtypes = [vp, C.c_int64, d, i32, d, d]
+    L.trj_nc_time.argtypes = [vp, C.c_int64, d]
+    L.trj_nc_frame_ref.argtypes = [vp, C.c_int64, C.c_int32, i32, d, i32, i64, d, i32]
     L.trj_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(BatchParams)]
     L.trj_batch_destroy.argtypes = [vp]
     L.trj_batch_destroy.restype = None
@@ -156,10 +166,86 @@ def _declare(L):
     L.trj_srv_stats.argtypes = [vp, i32, i32, i64]
     no_int = ("trj_destroy", "trj_last_error", "trj_device_bytes", "trj_batch_destroy",
               "trj_batch_last_error", "trj_srv_destroy", "trj_srv_last_error", "trj_request_footprint",
-              "trj_state_bytes")
+              "trj_state_bytes", "trj_nc_close", "trj_nc_last_error")
     for name in EXPORTS:
         if name not in no_int:
             getattr(L, name).restype = C.c_int
+
+
+class NcFileInfo(C.Structure):
+    _fields_ = [("nframes", C.c_int64), ("natoms", C.c_int64), ("version", C.c_int32),
+                ("coord_dtype", C.c_int32), ("has_type", C.c_int32), ("has_cell_origin", C.c_int32),
+                ("has_cell_lengths", C.c_int32), ("has_time", C.c_int32)]
+
+
+class NcTrajectory:
+    """A LAMMPS NetCDF trajectory (classic formats) through trj_nc_*: frames as
+    stored (read) or as the reference's read_nc_cfg hands them on (frame_ref)."""
+
+    def __init__(self, path: str):
+        self._L = lib()
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = self._L.trj_nc_open(C.byref(h), os.fsencode(path), err, 512)
+        if rc != TRJ_OK:
+            raise TrjError(rc, f"trj_nc_open: {err.value.decode()}")
+        self._h = h
+        info = NcFileInfo()
+        self._check(self._L.trj_nc_info(self._h, C.byref(info)), "trj_nc_info")
+        self.nframes, self.natoms, self.version = info.nframes, info.natoms, info.version
+        self.coord_dtype = info.coord_dtype
+        self.has_type = bool(info.has_type)
+        self.has_cell_origin = bool(info.has_cell_origin)
+        self.has_cell_lengths = bool(info.has_cell_lengths)
+        self.has_time = bool(info.has_time)
+
+    def _check(self, rc, what):
+        if rc != TRJ_OK:
+            raise TrjError(rc, f"{what}: {self._L.trj_nc_last_error(self._h).decode()}")
+
+    def read(self, frame: int):
+        pos = np.zeros((self.natoms, 3))
+        typ = np.zeros(self.natoms, dtype=np.int32)
+        org, cell = np.zeros(3), np.zeros(3)
+        self._check(self._L.trj_nc_read(self._h, int(frame), _ptr(pos, C.c_double),
+                                        _ptr(typ, C.c_int32) if self.has_type else None,
+                                        _ptr(org, C.c_double), _ptr(cell, C.c_double)), "trj_nc_read")
+        return pos, (typ if self.has_type else None), org, cell
+
+    def time(self, frame: int) -> float:
+        """time(frame) as stored (the reference's nstep is its integer part)."""
+        t = C.c_double(0.0)
+        self._check(self._L.trj_nc_time(self._h, int(frame), C.byref(t)), "trj_nc_time")
+        return t.value
+
+    def frame_ref(self, frame: int, sp_types, periodic=(True, True, True)):
+        """(pos SoA (3, n), itype (n,), box (3,), periodic (3,)) as read_nc_cfg hands
+        them on; periodic in is the /INPUT/ flags, out has zero-length axes cleared."""
+        st = np.ascontiguousarray(np.asarray(sp_types, dtype=np.int32))
+        pos = np.empty(3 * self.natoms)
+        it = np.empty(self.natoms, dtype=np.int32)
+        n = C.c_int64()
+        box = np.zeros(3)
+        per = np.array([1 if p else 0 for p in periodic][:3], dtype=np.int32)
+        self._check(self._L.trj_nc_frame_ref(self._h, int(frame), len(st), _ptr(st, C.c_int32),
+                                             _ptr(pos, C.c_double), _ptr(it, C.c_int32), C.byref(n),
+                                             _ptr(box, C.c_double), _ptr(per, C.c_int32)), "trj_nc_frame_ref")
+        m = n.value
+        return pos[:3 * m].reshape(3, m), it[:m], box, per
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.trj_nc_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()
 
 
 def lds_atomic_rate(device: int = 0) -> float:
