@@ -703,7 +703,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // addition; FP32 operand tables in LDS.
 constexpr int LX = 64, LY = 4, LZ = 32, LN = LY * LZ;
 constexpr int LKC = 32;           // atoms per chunk (K of one 16x16x32 MFMA)
-constexpr int LKP = 40;           // padded row length of the 16-bit operand tables
 constexpr int LKF = LKC + 1;      // padded row length of the FP32 operand tables
 
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -863,20 +862,19 @@ __global__ __launch_bounds__(256) void k_sq_contract_f32(ContractArgs a) {
 }
 
 // ---------------------------------------------------- 16-bit split kernels
-// Same tile and GEMM as k_sq_contract_lp (M = 64 kx, one 16-row block per
-// wavefront; N = 128 (ky,kz) pairs; K = 32 atoms per chunk), organised so that
-// the VALU work and the MFMAs of different wavefronts overlap:
-//  * each chunk's phases are staged once as FP32 complex records (ex[64],
-//    ey[4], ez[32] per atom) in LDS; a lane computes one 8-value segment: FP64
-//    reduction of the anchor, FP32 sincospi, FP32 angle addition by the
-//    frame's FP64 unit step;
+// Same tile and GEMM as k_sq_contract_f32 (M = 64 kx, one 16-row block per
+// wavefront; N = 128 (ky,kz) pairs; K = 32 atoms per chunk), organised so
+// that the VALU work and the MFMAs of different wavefronts overlap:
+//  * each chunk's phases are staged once as FP32 complex records in LDS,
+//    value-major (row = kx, kz or ky value, column = atom); every lane runs
+//    the same code: one 16-value segment (FP64 reduction of the anchor, FP32
+//    polynomial, FP32 angle addition), branch-free (atoms past the end give
+//    zero ex), so the staging of chunk c+1 is scheduled among the MFMAs of c;
 //  * the B operand ey*ez (shared by the four wavefronts) is formed and split
-//    into hi + lo once per chunk; each wavefront splits its own 16 rows of ex
-//    straight into registers (no A table in LDS);
-//  * chunk c+1 is staged while the MFMAs of chunk c run.
-constexpr int kLRec = 102;  // record stride in float2 (100 used): lane = atom
-                            // writes and lane = row reads are conflict-free
-
+//    into hi + lo once per chunk, stored 8 atoms per 16-byte entry so that the
+//    MFMA operand loads are contiguous (no padding); each wavefront splits its
+//    own 16 rows of ex straight into registers;
+//  * the -Im*Im products accumulate separately (no operand negation).
 typedef float f8 __attribute__((ext_vector_type(8)));
 
 // hi = round(x), lo = round(x - hi), eight values at a time (packed converts).
@@ -930,14 +928,18 @@ __device__ inline float2 cis_turns_f32(double f) {
   return make_float2(((qi + 1) & 2) ? -re : re, (qi & 2) ? -im : im);
 }
 
-// e^{2 pi i n u}, n = n0 .. n0+cnt-1, into dst as FP32 complex: the anchor
-// from the FP64-reduced turns, steps by the FP64 unit step (c, s).
-__device__ inline void phase_rec(double uh, double ul, double c, double s, int n0, int cnt, float2* dst) {
+// e^{2 pi i n u} for n = n0 .. n0+15, times m (1, or 0 for an atom past the
+// end), into dst[k * kRS] as FP32 complex: the anchor from the FP64-reduced
+// turns, then steps by the FP64 unit step rounded to FP32 (16 steps add at
+// most ~2e-6 relative error, below the 16-bit operands' own).
+constexpr int kRS = 34;  // row stride of the value-major records (float2): 32 atoms + 2
+__device__ inline void phase_rec16(double uh, double ul, double c, double s, int n0, float m, float2* dst) {
   const float2 an = cis_turns_f32(frac_turns(n0, uh, ul));
   const float sc = static_cast<float>(c), ss = static_cast<float>(s);
-  float re = an.x, im = an.y;
-  for (int k = 0; k < cnt; ++k) {
-    dst[k] = make_float2(re, im);
+  float re = an.x * m, im = an.y * m;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    dst[k * kRS] = make_float2(re, im);
     const float nr = fmaf(re, sc, -(im * ss));
     const float ni = fmaf(re, ss, im * sc);
     re = nr;
@@ -949,29 +951,27 @@ template <int PREC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sq_contract_lp16(
     ContractArgs a) {
   typedef LpTypes<PREC> LT;
-  typedef typename LT::T T;
   typedef typename LT::V V;
-  __shared__ float2 rec[LKC][kLRec];                   // x at 0..63, y at 64..67, z at 68..99
-  __shared__ __attribute__((aligned(16))) T B[4][LN][LKP];  // re_hi, re_lo, im_hi, im_lo of ey*ez
+  __shared__ __attribute__((aligned(16))) float2 rec[112][kRS];  // value-major: x 0..63, z 64..95, y 96..111
+  __shared__ __attribute__((aligned(16))) V B[4][LKC / 8][LN];  // re_hi, re_lo, im_hi, im_lo of ey*ez; 8 atoms per entry
   const int tile = blockIdx.x % a.ntiles;
   const int split = blockIdx.x / a.ntiles;
   const int kx0 = a.tiles[4 * tile], ky0 = a.tiles[4 * tile + 1], kz0 = a.tiles[4 * tile + 2];
   const int t = threadIdx.x, w = t / kWave, lane = t & (kWave - 1);
   const int i16 = lane & 15, kq = lane >> 4;
-  // Staging: every lane writes ex segment t/32 (8 kx) of atom t%32; lanes
-  // 0-127 also write ez segment t/32 (8 kz), lanes 128-159 ey (4 ky).
-  const int sa = t % LKC, xseg = t / LKC;
-  const int task2 = t < 128 ? 2 : (t < 160 ? 1 : -1);  // axis of the second segment
-  const int n0x = kx0 + 8 * xseg;
-  const int n02 = task2 == 2 ? kz0 + 8 * xseg : ky0;
-  float2* const dx = rec[sa] + 8 * xseg;
-  float2* const d2 = task2 == 2 ? rec[sa] + 68 + 8 * xseg : rec[sa] + 64;
-  const double* u_hi = a.u;
-  const double* u_lo = a.u + 3 * a.n;
-  const double* u_c = a.u + 6 * a.n;
-  const double* u_s = a.u + 9 * a.n;
+  // Staging: one 16-value segment per lane, same code for every lane.
+  // t < 128: ex segment t/32 of atom t%32; t < 192: ez segment (t-128)/32;
+  // t >= 192: ey (16 values from ky0, 4 used; lanes 224-255 repeat 192-223).
+  const int sa = t % LKC;
+  const int axis = t < 128 ? 0 : (t < 192 ? 2 : 1);
+  const int seg = t < 128 ? t / 32 : (t < 192 ? (t - 128) / 32 : 0);
+  const int sn0 = (axis == 0 ? kx0 : (axis == 2 ? kz0 : ky0)) + 16 * seg;
+  float2* const dst = &rec[axis == 0 ? 16 * seg : (axis == 2 ? 64 + 16 * seg : 96)][sa];
   const uint32_t n32 = static_cast<uint32_t>(a.n);
-  const uint32_t o2 = static_cast<uint32_t>(task2 < 0 ? 0 : task2) * n32;
+  const double* u_hi = a.u + axis * n32;
+  const double* u_lo = a.u + (3 + axis) * n32;
+  const double* u_c = a.u + (6 + axis) * n32;
+  const double* u_s = a.u + (9 + axis) * n32;
   // Products: column n = (ky, kz), atoms 16*ph .. 16*ph+15.
   const int pn = t & (LN - 1), ph = t >> 7;
   const int pky = pn / LZ, pkz = pn % LZ;
@@ -980,94 +980,98 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int64_t len = a1 - a0;
     const int b0 = a0 + static_cast<int>(len * split / a.ksplit);
     const int b1 = a0 + static_cast<int>(len * (split + 1) / a.ksplit);
+    if (b0 >= b1) {
+      // no atoms of this species in this split: zero partials
+      const size_t base = ((static_cast<size_t>(split) * a.nsp + s) * a.ntiles + tile) * (LX * LN);
+      for (int v = t; v < LX * LN; v += 256) reinterpret_cast<double2*>(a.rho)[base + v] = make_double2(0.0, 0.0);
+      continue;
+    }
     // FP32 accumulation over this workgroup's atoms (at most 8192 per species,
-    // set by the host through ksplit); the K-splits are summed in FP64.
-    f4 accR[8], accI[8];
+    // set by the host through ksplit); the K-splits are summed in FP64.  The
+    // -Im*Im terms go to their own accumulators (no operand negation).
+    f4 accR[8], accI[8], accM[8];
     for (int nb = 0; nb < 8; ++nb) {
       accR[nb] = f4{0.f, 0.f, 0.f, 0.f};
       accI[nb] = accR[nb];
+      accM[nb] = accR[nb];
     }
-    double xh = 0.0, xl = 0.0, xc = 1.0, xs = 0.0, vh = 0.0, vl = 0.0, vc = 1.0, vs = 0.0;
+    double xh, xl, xc, xs;
+    float xm;
     auto fetch = [&](int c0) {
       const int ai = c0 + sa;
-      if (ai < b1) {
-        const uint32_t o = static_cast<uint32_t>(ai);
-        xh = u_hi[o];
-        xl = u_lo[o];
-        xc = u_c[o];
-        xs = u_s[o];
-        if (task2 >= 0) {
-          vh = u_hi[o2 + o];
-          vl = u_lo[o2 + o];
-          vc = u_c[o2 + o];
-          vs = u_s[o2 + o];
-        }
-      }
-    };
-    // Records of chunk c0 from the prefetched turns; then prefetch c0 + LKC.
-    auto stage = [&](int c0) {
-      if (c0 + sa < b1) {
-        phase_rec(xh, xl, xc, xs, n0x, 8, dx);
-        if (task2 >= 0) phase_rec(vh, vl, vc, vs, n02, task2 == 2 ? 8 : LY, d2);
-      } else {
-        for (int k = 0; k < 8; ++k) dx[k] = make_float2(0.f, 0.f);
-        if (task2 >= 0)
-          for (int k = 0; k < (task2 == 2 ? 8 : LY); ++k) d2[k] = make_float2(0.f, 0.f);
-      }
-      fetch(c0 + LKC);
+      const uint32_t o = static_cast<uint32_t>(ai < b1 ? ai : b1 - 1);
+      xh = u_hi[o];
+      xl = u_lo[o];
+      xc = u_c[o];
+      xs = u_s[o];
+      xm = (axis != 0 || ai < b1) ? 1.f : 0.f;
     };
     __syncthreads();  // the previous species' readers are done with rec and B
     fetch(b0);
-    if (b0 < b1) stage(b0);
+    phase_rec16(xh, xl, xc, xs, sn0, xm, dst);
+    fetch(b0 + LKC);
     __syncthreads();
     for (int c0 = b0; c0 < b1; c0 += LKC) {
       // B = ey*ez for column pn and 16 atoms, split, 8 atoms per 16-byte store.
 #pragma unroll
       for (int g = 0; g < 2; ++g) {
         f8 re, im;
+        const float4* yr = reinterpret_cast<const float4*>(&rec[96 + pky][16 * ph + 8 * g]);
+        const float4* zr = reinterpret_cast<const float4*>(&rec[64 + pkz][16 * ph + 8 * g]);
+        float4 yv[4], zv[4];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          const float2* r = rec[16 * ph + 8 * g + k];
-          const float2 y = r[64 + pky], z = r[68 + pkz];
-          re[k] = fmaf(y.x, z.x, -(y.y * z.y));
-          im[k] = fmaf(y.x, z.y, y.y * z.x);
+        for (int k = 0; k < 4; ++k) {
+          yv[k] = yr[k];
+          zv[k] = zr[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          re[2 * k] = fmaf(yv[k].x, zv[k].x, -(yv[k].y * zv[k].y));
+          im[2 * k] = fmaf(yv[k].x, zv[k].y, yv[k].y * zv[k].x);
+          re[2 * k + 1] = fmaf(yv[k].z, zv[k].z, -(yv[k].w * zv[k].w));
+          im[2 * k + 1] = fmaf(yv[k].z, zv[k].w, yv[k].w * zv[k].z);
         }
         V rh, rl, ih, il;
         split8(re, rh, rl);
         split8(im, ih, il);
-        const int ab = 16 * ph + 8 * g;
-        *reinterpret_cast<V*>(&B[0][pn][ab]) = rh;
-        *reinterpret_cast<V*>(&B[1][pn][ab]) = rl;
-        *reinterpret_cast<V*>(&B[2][pn][ab]) = ih;
-        *reinterpret_cast<V*>(&B[3][pn][ab]) = il;
+        const int oc = 2 * ph + g;
+        B[0][oc][pn] = rh;
+        B[1][oc][pn] = rl;
+        B[2][oc][pn] = ih;
+        B[3][oc][pn] = il;
       }
       // A = ex for this wavefront's row 16w + i16, atoms 8kq .. 8kq+7.
       V arh, arl, aih, ail;
       {
         f8 re, im;
+        const float4* er = reinterpret_cast<const float4*>(&rec[16 * w + i16][8 * kq]);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          const float2 e = rec[8 * kq + k][16 * w + i16];
-          re[k] = e.x;
-          im[k] = e.y;
+        for (int k = 0; k < 4; ++k) {
+          const float4 e = er[k];
+          re[2 * k] = e.x;
+          im[2 * k] = e.y;
+          re[2 * k + 1] = e.z;
+          im[2 * k + 1] = e.w;
         }
         split8(re, arh, arl);
         split8(im, aih, ail);
       }
       __syncthreads();  // B complete; rec free for the next chunk
-      if (c0 + LKC < b1) stage(c0 + LKC);
-      const V naih = -aih, nail = -ail;
+      // Records of the next chunk (zeros past the end), prefetch of the one after.
+      phase_rec16(xh, xl, xc, xs, sn0, xm, dst);
+      fetch(c0 + 2 * LKC);
 #pragma unroll
       for (int nb = 0; nb < 8; nb += 2) {
         V brh[2], brl[2], bih[2], bil[2];
         for (int e = 0; e < 2; ++e) {
           const int col = 16 * (nb + e) + i16;
-          brh[e] = *reinterpret_cast<const V*>(&B[0][col][8 * kq]);
-          brl[e] = *reinterpret_cast<const V*>(&B[1][col][8 * kq]);
-          bih[e] = *reinterpret_cast<const V*>(&B[2][col][8 * kq]);
-          bil[e] = *reinterpret_cast<const V*>(&B[3][col][8 * kq]);
+          brh[e] = B[0][kq][col];
+          brl[e] = B[1][kq][col];
+          bih[e] = B[2][kq][col];
+          bil[e] = B[3][kq][col];
         }
         f4 r0 = accR[nb], i0 = accI[nb], r1 = accR[nb + 1], i1 = accI[nb + 1];
+        f4 m0 = accM[nb], m1 = accM[nb + 1];
         r0 = LT::mfma(arh, brh[0], r0);
         i0 = LT::mfma(arh, bih[0], i0);
         r1 = LT::mfma(arh, brh[1], r1);
@@ -1080,22 +1084,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         i0 = LT::mfma(arl, bih[0], i0);
         r1 = LT::mfma(arl, brh[1], r1);
         i1 = LT::mfma(arl, bih[1], i1);
-        r0 = LT::mfma(naih, bih[0], r0);
+        m0 = LT::mfma(aih, bih[0], m0);
         i0 = LT::mfma(aih, brh[0], i0);
-        r1 = LT::mfma(naih, bih[1], r1);
+        m1 = LT::mfma(aih, bih[1], m1);
         i1 = LT::mfma(aih, brh[1], i1);
-        r0 = LT::mfma(naih, bil[0], r0);
+        m0 = LT::mfma(aih, bil[0], m0);
         i0 = LT::mfma(aih, brl[0], i0);
-        r1 = LT::mfma(naih, bil[1], r1);
+        m1 = LT::mfma(aih, bil[1], m1);
         i1 = LT::mfma(aih, brl[1], i1);
-        r0 = LT::mfma(nail, bih[0], r0);
+        m0 = LT::mfma(ail, bih[0], m0);
         i0 = LT::mfma(ail, brh[0], i0);
-        r1 = LT::mfma(nail, bih[1], r1);
+        m1 = LT::mfma(ail, bih[1], m1);
         i1 = LT::mfma(ail, brh[1], i1);
         accR[nb] = r0;
         accI[nb] = i0;
         accR[nb + 1] = r1;
         accI[nb + 1] = i1;
+        accM[nb] = m0;
+        accM[nb + 1] = m1;
       }
       __syncthreads();  // rec of the next chunk complete; B free
     }
@@ -1110,7 +1116,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         const int n = 16 * nb + i16;
         const int v = (kxl * LY + n / LZ) * LZ + (n % LZ);
         reinterpret_cast<double2*>(a.rho)[base + v] =
-            make_double2(static_cast<double>(accR[nb][r]), static_cast<double>(accI[nb][r]));
+            make_double2(static_cast<double>(accR[nb][r]) - static_cast<double>(accM[nb][r]),
+                         static_cast<double>(accI[nb][r]));
       }
     }
   }
