@@ -36,6 +36,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "frames/s + pairs/s (RDF), q·atom terms/s (S(Q)); p50/p99 frame latency"
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "fp16x3": 2.5e15 / 3, "bf16x3": 2.5e15 / 3}
+# Nominal dense rate of the MFMA instruction each path issues (PEAK divides the
+# 16-bit one by the three products of a split-precision real product).
+NOMINAL_MFMA = {"fp64": 78.6e12, "fp16x3": 2.5e15, "bf16x3": 2.5e15}
 
 
 def log(*a):
@@ -299,6 +302,15 @@ def main():
             traffic = json.load(f).get(f"{args.workload}/{args.precision}/{kname}")
     achieved = flops / (kernel_ms["sq_contract"] * 1e-3)
     peak = PEAK[args.precision]
+    # The same roofline against the rate the MFMA instruction reaches on this
+    # device (trj_mfma_rate: independent accumulations, no memory traffic),
+    # scaled like PEAK, and the sustained shader clock over that probe.
+    probe = None
+    if args.precision in NOMINAL_MFMA:
+        rate, mhz = T.mfma_rate(args.precision, 0)
+        ceil = rate * peak / NOMINAL_MFMA[args.precision]
+        probe = {"mfma_rate": rate / 1e12, "clock_mhz": mhz, "ceiling": ceil / 1e12,
+                 "frac_of_ceiling": achieved / ceil}
     # RDF: in-range pairs per second of the RDF stage against the device's
     # measured LDS-atomic rate (one histogram add per in-range pair).
     lds_rate = T.lds_atomic_rate(0)
@@ -328,7 +340,8 @@ def main():
         "kernel_ms": kernel_ms,
         "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved / 1e12,
                      "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
-                     "algorithmic": f"8*N*Nq = {flops:.4e} flop per launch (N={n}, Nq={info.nvec_grid})"},
+                     "algorithmic": f"8*N*Nq = {flops:.4e} flop per launch (N={n}, Nq={info.nvec_grid})",
+                     "measured_mfma": probe},
         "rdf_roofline": {"bound": "lds_atomic", "achieved": rdf_rate, "peak": lds_rate, "unit": "pairs/s",
                          "frac": rdf_rate / lds_rate, "pairs_per_frame": pairs_frame,
                          "note": "in-range pairs of the RDF stage (cell build + pairs + merge) per second "

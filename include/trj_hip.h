@@ -233,6 +233,14 @@ int trj_state_load(trj_ctx* ctx, const void* buf, int64_t bytes, int mode);
  * of a 4096-bin histogram), the ceiling the RDF's per-pair histogram update
  * is priced against.  Runs a short kernel on the default stream of `device`. */
 int trj_lds_atomic_rate(int device, double* per_s);
+/* Measured dense-MFMA rate of the device, for a roofline against what the
+ * instruction reaches rather than the nominal peak: two wavefronts per SIMD on
+ * every CU each run independent accumulations of one shape (TRJ_PREC_FP64:
+ * v_mfma_f64_16x16x4f64; TRJ_PREC_FP16X3 / TRJ_PREC_BF16X3: the 16x16x32
+ * FP16 / BF16 MFMA with FP32 accumulation).  *flops_per_s: dense flops per
+ * second; *clock_mhz (may be NULL): sustained shader clock over the loop.
+ * No counterpart in the reference. */
+int trj_mfma_rate(int device, int precision, double* flops_per_s, double* clock_mhz);
 
 /* ------------------------------------------------------------------------
  * NetCDF trajectory input (host only): LAMMPS `dump netcdf` files in the

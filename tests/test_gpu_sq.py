@@ -187,3 +187,15 @@ def test_precision_1m_halfspace():
     assert worst["fp32"] <= 1e-5
     assert worst["fp16x3"] <= 1e-5
     assert worst["bf16x3"] <= 5e-5
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp16x3", "bf16x3"])
+def test_mfma_rate_probe(prec):
+    """trj_mfma_rate: a dense rate between 30% and 105% of the nominal MI355X
+    figure for the shape, and a plausible sustained clock."""
+    nominal = {"fp64": 78.6e12, "fp16x3": 2.5e15, "bf16x3": 2.5e15}[prec]
+    print("case: mfma probe", prec, flush=True)
+    rate, mhz = T.mfma_rate(prec, 0)
+    print(prec, "rate %.4g flop/s = %.3f of nominal, clock %.0f MHz" % (rate, rate / nominal, mhz), flush=True)
+    assert 0.3 * nominal < rate < 1.05 * nominal
+    assert 500.0 < mhz < 3000.0

@@ -101,6 +101,10 @@ void launch_sq_list(const double* u, const int32_t* sp_off, int64_t n, int nsp, 
 // LDS-atomic throughput probe (random bins of a 4096-bin uint32 histogram);
 // returns the number of atomics the launch performs; *out gets their sum.
 int64_t launch_lds_atomics(int nblocks, int iters, uint32_t* out, hipStream_t s);
+// Dense-MFMA probe (one shape per precision: FP64 16x16x4, FP16/BF16 16x16x32):
+// returns the flops one launch performs; clk[0] += shader-clock cycles and
+// clk[1] += 100 MHz real-time ticks summed over wavefronts.
+double launch_mfma_rate(int precision, int nblocks, int iters, float* out, unsigned long long* clk, hipStream_t s);
 
 // ------------------------------------------------------------------ batches
 // Frames of one launch group, concatenated.  Device-side descriptors drive

@@ -1529,6 +1529,68 @@ __global__ __launch_bounds__(256) void k_lds_atomics(int iters, uint32_t* __rest
   atomicAdd(out, sum);
 }
 
+// Dense-MFMA issue-rate probe: every wavefront runs iters x 8 independent
+// accumulations of one MFMA shape (FP64 16x16x4, or 16-bit 16x16x32 with
+// FP32 accumulation); no memory traffic in the loop.  The shader-clock and
+// 100 MHz real-time counters bracket the loop (lane 0 of each wavefront adds
+// the differences to clk[0], clk[1]) to give the sustained clock.
+// Eight independent accumulation chains of one MFMA shape; a non-inlined
+// function keeps the accumulators in place across iterations.
+template <int PREC>
+__device__ __noinline__ float mfma_chains(int iters) {
+  typedef LpTypes<PREC> LT;
+  typedef typename LT::V V;
+  typedef typename LT::T T;
+  V a, b;
+  for (int k = 0; k < 8; ++k) {
+    a[k] = static_cast<T>(1e-3f * k);
+    b[k] = static_cast<T>(1e-3f * (threadIdx.x & 15));
+  }
+  f4 c0 = {0.f, 0.f, 0.f, 0.f}, c1 = c0, c2 = c0, c3 = c0, c4 = c0, c5 = c0, c6 = c0, c7 = c0;
+  for (int i = 0; i < iters; ++i) {
+    c0 = LT::mfma(a, b, c0);
+    c1 = LT::mfma(a, b, c1);
+    c2 = LT::mfma(a, b, c2);
+    c3 = LT::mfma(a, b, c3);
+    c4 = LT::mfma(a, b, c4);
+    c5 = LT::mfma(a, b, c5);
+    c6 = LT::mfma(a, b, c6);
+    c7 = LT::mfma(a, b, c7);
+  }
+  const f4 t = c0 + c1 + c2 + c3 + c4 + c5 + c6 + c7;
+  return t[0] + t[1] + t[2] + t[3];
+}
+__device__ __noinline__ float mfma_chains_f64(int iters) {
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  const double a = 1e-3 * (threadIdx.x & 7), b = 1e-3 * (threadIdx.x >> 6);
+  d4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = c0, c2 = c0, c3 = c0, c4 = c0, c5 = c0, c6 = c0, c7 = c0;
+  for (int i = 0; i < iters; ++i) {
+    c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c0, 0, 0, 0);
+    c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c1, 0, 0, 0);
+    c2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c2, 0, 0, 0);
+    c3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c3, 0, 0, 0);
+    c4 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c4, 0, 0, 0);
+    c5 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c5, 0, 0, 0);
+    c6 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c6, 0, 0, 0);
+    c7 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c7, 0, 0, 0);
+  }
+  const d4 t = c0 + c1 + c2 + c3 + c4 + c5 + c6 + c7;
+  return static_cast<float>(t[0] + t[1] + t[2] + t[3]);
+}
+
+template <int PREC>
+__global__ __launch_bounds__(512) void k_mfma_rate(int iters, float* __restrict__ out,
+                                                    unsigned long long* __restrict__ clk) {
+  const uint64_t r0 = __builtin_amdgcn_s_memrealtime(), m0 = __builtin_amdgcn_s_memtime();
+  const float s = PREC == TRJ_PREC_FP64 ? mfma_chains_f64(iters) : mfma_chains<PREC == TRJ_PREC_FP64 ? TRJ_PREC_BF16X3 : PREC>(iters);
+  const uint64_t r1 = __builtin_amdgcn_s_memrealtime(), m1 = __builtin_amdgcn_s_memtime();
+  out[blockIdx.x * 256 + threadIdx.x] = s;
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&clk[0], static_cast<unsigned long long>(m1 - m0));
+    atomicAdd(&clk[1], static_cast<unsigned long long>(r1 - r0));
+  }
+}
+
 inline unsigned grid1d(int64_t n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
 
 }  // namespace
@@ -1683,6 +1745,18 @@ void launch_sq_list(const double* u, const int32_t* sp_off, int64_t n, int nsp, 
 int64_t launch_lds_atomics(int nblocks, int iters, uint32_t* out, hipStream_t s) {
   hipLaunchKernelGGL(k_lds_atomics, dim3(nblocks), dim3(256), 0, s, iters, out);
   return static_cast<int64_t>(nblocks) * 256 * 16 * iters;
+}
+
+double launch_mfma_rate(int precision, int nblocks, int iters, float* out, unsigned long long* clk, hipStream_t s) {
+  if (precision == TRJ_PREC_FP64) {
+    hipLaunchKernelGGL(k_mfma_rate<TRJ_PREC_FP64>, dim3(nblocks), dim3(256), 0, s, iters, out, clk);
+    return static_cast<double>(nblocks) * 4 * iters * 8 * (16.0 * 16 * 4 * 2);
+  }
+  if (precision == TRJ_PREC_FP16X3)
+    hipLaunchKernelGGL(k_mfma_rate<TRJ_PREC_FP16X3>, dim3(nblocks), dim3(256), 0, s, iters, out, clk);
+  else
+    hipLaunchKernelGGL(k_mfma_rate<TRJ_PREC_BF16X3>, dim3(nblocks), dim3(256), 0, s, iters, out, clk);
+  return static_cast<double>(nblocks) * 4 * iters * 8 * (16.0 * 16 * 32 * 2);
 }
 
 void launch_batch_group(const BatchDev& d, int nblocks, size_t pair_lds, hipStream_t s) {

@@ -1033,6 +1033,48 @@ int trj_lds_atomic_rate(int device, double* per_s) {
   return rc;
 }
 
+int trj_mfma_rate(int device, int precision, double* flops_per_s, double* clock_mhz) {
+  if (!flops_per_s) return TRJ_EINVAL;
+  if (precision != TRJ_PREC_FP64 && precision != TRJ_PREC_FP16X3 && precision != TRJ_PREC_BF16X3)
+    return TRJ_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return TRJ_EDEVICE;
+  if (hipSetDevice(device) != hipSuccess) return TRJ_EDEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return TRJ_EDEVICE;
+  const int nblocks = prop.multiProcessorCount * 2;  // two wavefronts per SIMD
+  const int iters = precision == TRJ_PREC_FP64 ? 4000 : 16000;
+  float* out = nullptr;
+  unsigned long long* clk = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = TRJ_EDEVICE;
+  if (hipMalloc(&out, sizeof(float) * nblocks * 256) == hipSuccess &&
+      hipMalloc(&clk, sizeof(unsigned long long) * 2) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+      hipEventCreate(&e1) == hipSuccess) {
+    launch_mfma_rate(precision, nblocks, iters / 16, out, clk, nullptr);  // warm-up
+    hipMemset(clk, 0, sizeof(unsigned long long) * 2);
+    hipEventRecord(e0, nullptr);
+    const double flops = launch_mfma_rate(precision, nblocks, iters, out, clk, nullptr);
+    hipEventRecord(e1, nullptr);
+    unsigned long long c2[2] = {0, 0};
+    if (hipEventSynchronize(e1) == hipSuccess &&
+        hipMemcpy(c2, clk, sizeof(c2), hipMemcpyDeviceToHost) == hipSuccess) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      if (ms > 0.f) {
+        *flops_per_s = flops / (ms * 1e-3);
+        if (clock_mhz) *clock_mhz = c2[1] ? 100.0 * static_cast<double>(c2[0]) / static_cast<double>(c2[1]) : 0.0;
+        rc = TRJ_OK;
+      }
+    }
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (out) hipFree(out);
+  if (clk) hipFree(clk);
+  return rc;
+}
+
 int trj_last_timings(trj_ctx* c, double* ms4) {
   if (!c || !ms4) return TRJ_EINVAL;
   hipSetDevice(c->device);

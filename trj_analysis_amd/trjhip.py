@@ -37,7 +37,7 @@ EXPORTS = [
     "trj_rdf_counts", "trj_rdf_gmix", "trj_rdf_first_minimum", "trj_sq_shells", "trj_sq_columns",
     "trj_sq_rho", "trj_reset", "trj_last_timings", "trj_timing_totals", "trj_timing_reset",
     "trj_device_bytes", "trj_set_pipeline", "trj_state_bytes", "trj_state_save", "trj_state_load",
-    "trj_lds_atomic_rate", "trj_nc_open", "trj_nc_close", "trj_nc_last_error", "trj_nc_info", "trj_nc_read",
+    "trj_lds_atomic_rate", "trj_mfma_rate", "trj_nc_open", "trj_nc_close", "trj_nc_last_error", "trj_nc_info", "trj_nc_read",
     "trj_nc_time", "trj_nc_frame_ref",
     "trj_batch_create", "trj_batch_destroy", "trj_batch_last_error",
     "trj_batch_layout", "trj_batch_run", "trj_batch_autotune", "trj_srv_create", "trj_srv_destroy",
@@ -141,6 +141,7 @@ def _declare(L):
     L.trj_nc_last_error.restype = C.c_char_p
     L.trj_nc_info.argtypes = [vp, C.POINTER(NcFileInfo)]
     L.trj_nc_read.argtypes = [vp, C.c_int64, d, i32, d, d]
+    L.trj_mfma_rate.argtypes = [C.c_int, C.c_int, d, d]
     L.trj_nc_time.argtypes = [vp, C.c_int64, d]
     L.trj_nc_frame_ref.argtypes = [vp, C.c_int64, C.c_int32, i32, d, i32, i64, d, i32]
     L.trj_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(BatchParams)]
@@ -255,6 +256,17 @@ def lds_atomic_rate(device: int = 0) -> float:
     if rc != TRJ_OK:
         raise TrjError(rc, "trj_lds_atomic_rate")
     return out.value
+
+
+def mfma_rate(precision="bf16x3", device: int = 0) -> tuple[float, float]:
+    """(dense flops per second, sustained shader clock in MHz) of one MFMA shape
+    (trj_mfma_rate): "fp64", "fp16x3" or "bf16x3"."""
+    p = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    f, mhz = C.c_double(0.0), C.c_double(0.0)
+    rc = lib().trj_mfma_rate(int(device), p, C.byref(f), C.byref(mhz))
+    if rc != TRJ_OK:
+        raise TrjError(rc, "trj_mfma_rate")
+    return f.value, mhz.value
 
 
 def _ptr(a: np.ndarray, ct):
