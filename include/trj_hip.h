@@ -72,6 +72,13 @@ extern "C" {
                                    actually summed, instead of the reference's host
                                    count (sq.cuf:328-342), which misses the on-sphere
                                    vectors of shell nqmin */
+#define TRJ_SQ_CHARGE 2u        /* also sum |sum_j q_j e^{iq.r_j}|^2 (S_qq; sq.cuf:424-432,
+                                   563-582 with ex_qc); frames come through trj_frame_q */
+
+/* RDF flags. */
+#define TRJ_RDF_CHARGE 1u /* also accumulate the charge-weighted histogram g_qq
+                             (rdf.cuf:466-472, 285-287); frames come through
+                             trj_frame_q */
 
 typedef struct trj_ctx trj_ctx;
 
@@ -91,6 +98,7 @@ typedef struct {
   double rcrdf;        /* cutoff; <= 0 means half the shortest side (rdf.cuf:55-58) */
   int32_t periodic[3]; /* /INPUT/ periodic: min image on 2.5*L for false axes
                           (netcdf.f90:603-609) */
+  uint32_t flags;      /* TRJ_RDF_*; 0 for the plain histogram */
 } trj_rdf_params;
 
 /* RDF_init (rdf.cuf:51-122).  box: first frame's cell lengths; the cutoff is
@@ -145,6 +153,17 @@ int trj_rdf_plan(const trj_rdf_params* p, const double box[3], int32_t* lsmax, d
 int trj_frame(trj_ctx* ctx, const void* pos, int32_t dtype, int32_t layout,
               const int32_t* itype, int64_t n, const double box[3], uint32_t what);
 
+/* trj_frame with per-atom charges (the NetCDF "q" variable, netcdf.f90:
+ * 404-408; ex_qc): charge[n] on the host, in the order of pos.  Required when
+ * the context was initialised with TRJ_RDF_CHARGE or TRJ_SQ_CHARGE (trj_frame
+ * then fails with TRJ_EINVAL).  The pair kernel adds each in-range pair's
+ * q_i q_j, rounded once to a per-frame fixed point (2^(30-e), max q^2 < 2^e),
+ * as exact int64 sums; the frame's sums are then divided by avq2 = sum(q^2)/N
+ * (rdf.cuf:285-287) into FP64 accumulators.  S(Q) adds a charge-weighted
+ * channel to the contraction and the axis sums. */
+int trj_frame_q(trj_ctx* ctx, const void* pos, int32_t dtype, int32_t layout, const int32_t* itype,
+                const double* charge, int64_t n, const double box[3], uint32_t what);
+
 /* Same as trj_frame for inputs already in device memory of this context's
  * device (HBM-resident frames). */
 int trj_frame_device(trj_ctx* ctx, const void* pos_dev, int32_t dtype, int32_t layout,
@@ -165,6 +184,15 @@ int trj_rdf_counts(trj_ctx* ctx, int64_t* counts);
  * frame's.  Synchronises. */
 int trj_rdf_gmix(trj_ctx* ctx, int64_t nconf, double* out);
 
+/* Charge-weighted histogram (TRJ_RDF_CHARGE): acc[k-1], k = 1..lsmax, the
+ * reference's gqq (sum over frames of sum q_i q_j / avq2).  Synchronises. */
+int trj_rdf_qq(trj_ctx* ctx, double* acc);
+
+/* printrdf's g_qq column (log.f90:501-503): for i = 1..lsmax-2,
+ * 2 V gqq(i) / (N^2 dV Nconf) with the last frame's V and N.  nconf <= 0
+ * uses the number of RDF frames processed.  Synchronises. */
+int trj_rdf_gqq(trj_ctx* ctx, int64_t nconf, double* out);
+
 /* rdf.cuf:313-344: first minimum of the running total g(r) with parabolic
  * interpolation (find_first_minimum_interp, rdf.cuf:755-822). */
 int trj_rdf_first_minimum(trj_ctx* ctx, double* rmin);
@@ -179,6 +207,15 @@ int trj_sq_shells(trj_ctx* ctx, double* sqf, double* sqfp, int64_t* nq);
  * S_22, S_12, n); mix: nqmax rows of (Q, S_11, ..., S_nn).  nconf <= 0 uses
  * the number of S(Q) frames processed.  Synchronises. */
 int trj_sq_columns(trj_ctx* ctx, int64_t nconf, double* cols, int32_t* ncols, double* mix);
+
+/* Charge-weighted shell sums (TRJ_SQ_CHARGE): sqfq[0..nqmax], the
+ * reference's sqfq_d after the frames.  Synchronises. */
+int trj_sq_qq(trj_ctx* ctx, double* sqfq);
+
+/* printSQ's S_qq column (log.f90:371-376): for i = 1..nqmax,
+ * sqfq(i) / (N Nconf nq(i)) (0 where nq(i) = 0).  nconf <= 0 uses the number
+ * of S(Q) frames processed.  Synchronises. */
+int trj_sq_qq_column(trj_ctx* ctx, int64_t nconf, double* out);
 
 /* rho_s(q) of the last frame for nvec integer vectors kv[3*v..] that belong
  * to the dense q-set: rho[(v*nsp + s)*2 + {0,1}].  Debug/parity access to the

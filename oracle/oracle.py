@@ -91,6 +91,17 @@ def _declare(L):
     L.or_sq_columns.argtypes = [C.c_int, C.c_int, C.c_float, C.c_int64, _I64, _D, C.c_int64,
                                 _D, _D, _I64, _D, _D]
     L.or_sq_columns.restype = C.c_int
+    L.or_qq_scale.argtypes = [C.c_int64, _D]
+    L.or_qq_scale.restype = C.c_double
+    L.or_rdf_qq_brute.argtypes = [C.c_int64, _D, _D, _D, _D, C.c_int, _D, C.c_double, C.c_double,
+                                  C.c_int, C.c_double, _I64]
+    L.or_rdf_qq_brute.restype = None
+    L.or_qq_accumulate.argtypes = [C.c_int, _I64, C.c_double, C.c_int64, _D, _D]
+    L.or_qq_accumulate.restype = None
+    L.or_rdf_gqq.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int64, C.c_int64, _D, _D]
+    L.or_rdf_gqq.restype = None
+    L.or_sq_frame_q.argtypes = [C.c_int64, _D, _D, _D, _D, C.c_int, _D, C.POINTER(SqGrid), _D]
+    L.or_sq_frame_q.restype = None
 
 
 def _p(a, ctype):
@@ -254,3 +265,55 @@ def sq_columns(nsp, g: SqGrid, natoms, ntype, bsc, nconf, sqf, sqfp, nq):
                              _p(nq, C.c_int64), _p(cols, C.c_double), _p(mix, C.c_double))
     cols = cols.reshape(-1)[: g.nqmax * nc].reshape(g.nqmax, nc)
     return cols, mix
+
+
+def qq_scale(q) -> float:
+    q = _f64(q)
+    return lib().or_qq_scale(len(q), _p(q, C.c_double))
+
+
+def rdf_qq_fixed(pos, q, ndim, box, periodic, rcrdf, deltar, lsmax=None, scale=None) -> tuple:
+    """(fixed int64[lsmax], scale) of one frame: sum of llrint(q_i q_j scale) per
+    bin over the in-range pairs (raw positions, wrapped here)."""
+    g = geometry(ndim, box, periodic, rcrdf)
+    if lsmax is None:
+        lsmax = rdf_lsmax(ndim, box, rcrdf, deltar)
+    w = wrap(pos, box, periodic, ndim)
+    x, y, z = (np.ascontiguousarray(w[:, k]) for k in range(3))
+    qq = _f64(q)
+    if scale is None:
+        scale = qq_scale(qq)
+    acc = np.zeros(lsmax, dtype=np.int64)
+    lib().or_rdf_qq_brute(len(qq), _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double),
+                          _p(qq, C.c_double), ndim, _p(g.lmi, C.c_double), g.side2, deltar, lsmax,
+                          scale, _p(acc, C.c_int64))
+    return acc, scale
+
+
+def qq_accumulate(fixed, scale, q, acc):
+    """acc += (fixed/scale)/avq2 in place (avq2 = sum q^2 / N)."""
+    f = _i64(fixed)
+    qq = _f64(q)
+    lib().or_qq_accumulate(len(f), _p(f, C.c_int64), scale, len(qq), _p(qq, C.c_double),
+                           _p(acc, C.c_double))
+    return acc
+
+
+def rdf_gqq(ndim, lsmax, deltar, volume, natoms, nconf, acc) -> np.ndarray:
+    a = _f64(acc)
+    out = np.zeros(max(lsmax - 2, 0))
+    lib().or_rdf_gqq(ndim, lsmax, deltar, volume, natoms, nconf, _p(a, C.c_double), _p(out, C.c_double))
+    return out
+
+
+def sq_frame_q(pos, q, ndim, box, g: SqGrid, sqfq=None):
+    """Accumulate the charge-weighted sums of the reference sum set; pos wrapped."""
+    pos = np.asarray(pos, dtype=np.float64)
+    x, y, z = (np.ascontiguousarray(pos[:, k]) for k in range(3))
+    qq = _f64(q)
+    b = _f64(box)
+    if sqfq is None:
+        sqfq = np.zeros(g.nqmax + 1)
+    lib().or_sq_frame_q(len(qq), _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double),
+                        _p(qq, C.c_double), ndim, _p(b, C.c_double), C.byref(g), _p(sqfq, C.c_double))
+    return sqfq

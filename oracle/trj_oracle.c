@@ -196,6 +196,63 @@ void or_rdf_cells(int64_t n, const double* x, const double* y, const double* z,
   free(cell);
 }
 
+/* Charge-weighted pair histogram (rdf.cuf:466-472 g_qq, per frame).  The
+ * reference adds q_i*q_j in FP32 atomics; here every product is rounded once
+ * to the fixed point of or_qq_scale and summed exactly in int64, so the sum
+ * does not depend on the order of the pairs.  acc[k-1] += llrint(qi*qj*scale)
+ * for the in-range pairs of bin k (same pair rule as or_rdf_brute). */
+double or_qq_scale(int64_t n, const double* q) {
+  double m = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    double v = q[i] * q[i];
+    if (v > m) m = v;
+  }
+  int e = 0;
+  if (m > 0.0) frexp(m, &e); /* m < 2^e */
+  return ldexp(1.0, 30 - e);
+}
+
+void or_rdf_qq_brute(int64_t n, const double* x, const double* y, const double* z,
+                     const double* q, int ndim, const double lmi[3], double side2,
+                     double deltar, int lsmax, double scale, int64_t* acc) {
+  for (int64_t i = 0; i < n; ++i) {
+    double zi = (ndim == 3) ? z[i] : 0.0;
+    for (int64_t j = i + 1; j < n; ++j) {
+      double zj = (ndim == 3) ? z[j] : 0.0;
+      int k = pair_bin(ndim, x[i], y[i], zi, x[j], y[j], zj, lmi, side2, deltar, lsmax);
+      if (k) {
+        double p = q[i] * q[j];
+        acc[k - 1] += llrint(p * scale);
+      }
+    }
+  }
+}
+
+/* Host step of RDFcomp (rdf.cuf:285-287): gqq += gqq_frame / avq2 with
+ * avq2 = sum(q^2)/N; here the frame's fixed-point sums are converted back
+ * ((double)fixed / scale, exact power of two) and divided by avq2 (one
+ * rounding).  avq2 = 0 (no charges) adds nothing. */
+void or_qq_accumulate(int lsmax, const int64_t* fixed, double scale, int64_t n, const double* q,
+                      double* acc) {
+  double s2 = 0.0;
+  for (int64_t i = 0; i < n; ++i) s2 += q[i] * q[i];
+  double avq2 = s2 / (double)n;
+  if (!(avq2 > 0.0)) return;
+  for (int k = 0; k < lsmax; ++k) acc[k] += ((double)fixed[k] / scale) / avq2;
+}
+
+/* log.f90:501-503: g_qq(r_i) = 2 V gqq(i) / (N^2 dV Nconf), i = 1..lsmax-2,
+ * dV the spherical (3D) or circular (2D) shell, FP64. */
+void or_rdf_gqq(int ndim, int lsmax, double deltar, double volume, int64_t natoms, int64_t nconf,
+                const double* acc, double* out) {
+  const double pi = 3.14159265358979323846;
+  for (int i = 1; i <= lsmax - 2; ++i) {
+    double ri = i * deltar, a = ri + deltar / 2.0, b = ri - deltar / 2.0;
+    double dv = (ndim == 3) ? 4.0 * pi * (a * a * a - b * b * b) / 3.0 : pi * (a * a - b * b);
+    out[i - 1] = 2.0 * volume * acc[i - 1] / ((double)natoms * (double)natoms * dv * (double)nconf);
+  }
+}
+
 /* log.f90:484-518, evaluated in FP64. */
 void or_rdf_gmix(int ndim, int nsp, int lsmax, double deltar, double volume,
                  const int64_t* ntype, int64_t nconf, const int64_t* counts, double* out) {
@@ -422,12 +479,11 @@ void or_sq_rho(int64_t n, const double* x, const double* y, const double* z,
   }
 }
 
-void or_sq_frame(int64_t n, const double* x, const double* y, const double* z,
-                 const int32_t* itype, int ndim, int nsp, const double box[3],
-                 const or_sqgrid* g, const double* bsc, double* sqf, double* sqfp) {
-  /* Enumerate the sum set: sqfnD octant (sq.cuf:89, when nqmin > 1) and the
-   * axis shells of sqfact/sqfact2 (sq.cuf:94-98, 112-120; the idir = 0 form
-   * is used when nqmin <= 1). */
+/* The reference sum set: sqfnD octant (sq.cuf:89, when nqmin > 1) and the
+ * axis shells of sqfact/sqfact2 (sq.cuf:94-98, 112-120; the idir = 0 form is
+ * used when nqmin <= 1).  Returns the count; *kv_out [3 per vector] and
+ * *sh_out (shell per vector) are malloc'ed. */
+static int64_t sum_set(int ndim, const or_sqgrid* g, int32_t** kv_out, int32_t** sh_out) {
   int64_t cap = 1024, nv = 0;
   int32_t* kv = (int32_t*)malloc(sizeof(int32_t) * 3 * cap);
   int32_t* sh = (int32_t*)malloc(sizeof(int32_t) * cap);
@@ -462,6 +518,16 @@ void or_sq_frame(int64_t n, const double* x, const double* y, const double* z,
     if (ndim == 3) PUSH(0, 0, k, k);
   }
 #undef PUSH
+  *kv_out = kv;
+  *sh_out = sh;
+  return nv;
+}
+
+void or_sq_frame(int64_t n, const double* x, const double* y, const double* z,
+                 const int32_t* itype, int ndim, int nsp, const double box[3],
+                 const or_sqgrid* g, const double* bsc, double* sqf, double* sqfp) {
+  int32_t *kv, *sh;
+  int64_t nv = sum_set(ndim, g, &kv, &sh);
   double* rho = (double*)malloc(sizeof(double) * 2 * nsp * (nv > 0 ? nv : 1));
   or_sq_rho(n, x, y, ndim == 3 ? z : NULL, itype, nsp, box, nv, kv, rho);
   int ns = g->nqmax + 1;
@@ -476,6 +542,34 @@ void or_sq_frame(int64_t n, const double* x, const double* y, const double* z,
     sqf[sh[v]] += tr * tr + ti * ti;
   }
   free(rho);
+  free(kv);
+  free(sh);
+}
+
+/* Charge-weighted sums of the same set (sq.cuf:424-432 sqfact, sq.cuf:563-582
+ * sqfnD with ex_qc): sqfq[shell] += |sum_j q_j exp(i q.r_j)|^2, long double. */
+void or_sq_frame_q(int64_t n, const double* x, const double* y, const double* z, const double* q,
+                   int ndim, const double box[3], const or_sqgrid* g, double* sqfq) {
+  const long double twopi = 6.283185307179586476925286766559005768L;
+  int32_t *kv, *sh;
+  int64_t nv = sum_set(ndim, g, &kv, &sh);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t v = 0; v < nv; ++v) {
+    long double re = 0.0L, im = 0.0L;
+    int nx = kv[3 * v], ny = kv[3 * v + 1], nz = kv[3 * v + 2];
+    for (int64_t j = 0; j < n; ++j) {
+      long double ux = (long double)x[j] / (long double)box[0];
+      long double uy = (long double)y[j] / (long double)box[1];
+      long double uz = (ndim == 3) ? (long double)z[j] / (long double)box[2] : 0.0L;
+      long double ph = twopi * turns(nx, ny, ndim == 3 ? nz : 0, ux, uy, uz);
+      re += (long double)q[j] * cosl(ph);
+      im += (long double)q[j] * sinl(ph);
+    }
+    double r = (double)re, i = (double)im;
+    double p = r * r + i * i;
+#pragma omp atomic
+    sqfq[sh[v]] += p;
+  }
   free(kv);
   free(sh);
 }

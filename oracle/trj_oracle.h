@@ -32,6 +32,15 @@
  *                        ~0 elsewhere; translation invariance
  *   or_sq_columns        closed form of log.f90:327-399 on hand-made sums;
  *                        binary-mixture identity S_NN = s11+s22+2 s12
+ *   or_qq_scale          powers of two bracketing max q^2 (hand cases)
+ *   or_rdf_qq_brute      unit charges give 2^29 x the or_rdf_brute counts;
+ *                        +-1 by species gives c11 + c22 - c12; two-atom closed
+ *                        form; invariant under q -> 2q (scale drops by 4)
+ *   or_qq_accumulate     hand-made sums (exact arithmetic cases)
+ *   or_rdf_gqq           equals or_rdf_gmix's formula with N_j N_l -> N^2 on the
+ *                        same histogram (closed form of log.f90:501-503)
+ *   or_sq_frame_q        unit charges equal or_sq_frame's S with bsc = 1;
+ *                        uniform charge c scales by c^2; single atom gives q^2
  * None is "parity unpinned".
  */
 #ifndef TRJ_ORACLE_H
@@ -80,6 +89,21 @@ void or_rdf_cells(int64_t n, const double* x, const double* y, const double* z,
                   const int32_t periodic[3], const double lmi[3], double side2,
                   double deltar, int lsmax, int64_t* counts);
 
+/* Charge-weighted histogram g_qq (rdf.cuf:466-472, 285-287; log.f90:501-503).
+ * or_qq_scale: fixed-point scale 2^(30-e) with max q^2 < 2^e (2^30 for no
+ * charges).  or_rdf_qq_brute: acc[k-1] += llrint(q_i q_j scale) over the pairs
+ * of or_rdf_brute's rule.  or_qq_accumulate: acc += (fixed/scale)/avq2,
+ * avq2 = sum(q^2)/N (FP64, index order).  or_rdf_gqq: 2 V acc / (N^2 dV Nconf)
+ * for i = 1..lsmax-2. */
+double or_qq_scale(int64_t n, const double* q);
+void or_rdf_qq_brute(int64_t n, const double* x, const double* y, const double* z,
+                     const double* q, int ndim, const double lmi[3], double side2,
+                     double deltar, int lsmax, double scale, int64_t* acc);
+void or_qq_accumulate(int lsmax, const int64_t* fixed, double scale, int64_t n, const double* q,
+                      double* acc);
+void or_rdf_gqq(int ndim, int lsmax, double deltar, double volume, int64_t natoms, int64_t nconf,
+                const double* acc, double* out);
+
 /* log.f90:484-518: g_jl(r_i) for i=1..lsmax-2.  out has (lsmax-2) rows of
  * (1 + nit) doubles: r, then g_11, g_12, ..., g_1n, g_22, ... */
 void or_rdf_gmix(int ndim, int nsp, int lsmax, double deltar, double volume,
@@ -123,6 +147,12 @@ void or_sq_rho(int64_t n, const double* x, const double* y, const double* z,
 void or_sq_frame(int64_t n, const double* x, const double* y, const double* z,
                  const int32_t* itype, int ndim, int nsp, const double box[3],
                  const or_sqgrid* g, const double* bsc, double* sqf, double* sqfp);
+
+/* Charge-weighted sums over the same sum set as or_sq_frame (sq.cuf:424-432,
+ * 563-582): sqfq[shell] += |sum_j q_j exp(i 2 pi n.(r_j/L))|^2, long double
+ * phases and sums; positions already wrapped. */
+void or_sq_frame_q(int64_t n, const double* x, const double* y, const double* z, const double* q,
+                   int ndim, const double box[3], const or_sqgrid* g, double* sqfq);
 
 /* log.f90:317-401: printed columns.  Rows i=1..nqmax.  ncols_out = 3 (Q, S_NN,
  * n) or 7 (Q, S_NN, S_cc, S_11, S_22, S_12, n) for a binary mixture with all

@@ -1,0 +1,204 @@
+"""GPU parity of the charge-weighted channels: g_qq (bit-exact fixed-point
+sums, then the same FP64 finish as the oracle) and S_qq (phase sums over the
+charge channel) against the oracle, through the C-ABI."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from trj_analysis_amd import trjhip as T
+from trj_analysis_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _charges(it, seed):
+    """Species charges plus a per-atom jitter (not a multiple of a power of 2)."""
+    rng = np.random.default_rng(seed)
+    base = np.where(it == 1, 0.8473, -0.8473)
+    return base + 0.05 * rng.standard_normal(len(it))
+
+
+def _oracle_gqq_acc(frames, ndim, box, periodic, rc, dr, lsmax):
+    acc = np.zeros(lsmax)
+    for pos, it, q in frames:
+        fixed, scale = O.rdf_qq_fixed(pos, q, ndim, box, periodic, rc, dr, lsmax=lsmax)
+        O.qq_accumulate(fixed, scale, q, acc)
+    return acc
+
+
+def _run_rdf(frames, ndim, nsp, box, periodic, rc, dr):
+    with T.Context(0) as ctx:
+        st = ctx.rdf_init(T.rdf_params(ndim, nsp, dr, rc, periodic, flags=T.TRJ_RDF_CHARGE), box)
+        for pos, it, q in frames:
+            ctx.frame(pos, it, box, what=T.TRJ_WANT_RDF, charge=q)
+        return ctx.rdf_qq(), ctx.rdf_gqq(), ctx.rdf_counts(), st.lsmax
+
+
+def test_gqq_cells_lj_binary_bit_exact():
+    fr = []
+    for k, f in enumerate(W.lj_binary_4k(nframes=3)):
+        fr.append((f.pos, f.itype, _charges(f.itype, k)))
+    box = list(W.lj_binary_4k(nframes=1))[0].box
+    print("case: g_qq lj_binary_4k x3 cells", flush=True)
+    acc, gqq, counts, ls = _run_rdf(fr, 3, 2, box, (1, 1, 1), 4.0, 0.02)
+    oacc = _oracle_gqq_acc(fr, 3, box, (1, 1, 1), 4.0, 0.02, ls)
+    np.testing.assert_array_equal(acc, oacc)
+    ogqq = O.rdf_gqq(3, ls, 0.02, float(np.prod(box)), len(fr[0][1]), 3, oacc)
+    np.testing.assert_array_equal(gqq, ogqq)
+    # the species counts of the same frames are unchanged by the charge channel
+    ocounts = sum(O.rdf_counts(p, it, 3, 2, box, (1, 1, 1), 4.0, 0.02, lsmax=ls, method="cells")
+                  for p, it, _ in fr)
+    np.testing.assert_array_equal(counts, ocounts)
+
+
+def test_unit_charges_equal_total_counts():
+    rng = np.random.default_rng(11)
+    L = 15.0
+    pos = rng.random((3000, 3)) * L
+    it = rng.integers(1, 4, 3000).astype(np.int32)
+    print("case: g_qq unit charges", flush=True)
+    acc, _, counts, _ = _run_rdf([(pos, it, np.ones(3000))], 3, 3, [L] * 3, (1, 1, 1), 5.0, 0.03)
+    np.testing.assert_array_equal(acc, counts.sum(axis=0).astype(np.float64))
+
+
+@pytest.mark.parametrize("case", ["allpairs", "slab", "2d", "noshift"])
+def test_gqq_geometries(case):
+    rng = np.random.default_rng({"allpairs": 1, "slab": 2, "2d": 3, "noshift": 4}[case])
+    if case == "allpairs":           # box too small for a cell grid
+        L, n, rc, dr, ndim, per = 7.0, 500, 3.4, 0.02, 3, (1, 1, 1)
+    elif case == "slab":
+        L, n, rc, dr, ndim, per = 16.0, 3000, 5.0, 0.02, 3, (1, 1, 0)
+    elif case == "2d":
+        L, n, rc, dr, ndim, per = 40.0, 3000, 6.0, 0.05, 2, (1, 1, 1)
+    else:                            # 3-4 cells per axis: per-pair image rule
+        L, n, rc, dr, ndim, per = 14.0, 2500, 4.5, 0.03, 3, (1, 1, 1)
+    pos = rng.random((n, 3)) * L
+    if ndim == 2:
+        pos[:, 2] = 0.0
+    box = [L, L, L if ndim == 3 else 1.0]
+    it = rng.integers(1, 3, n).astype(np.int32)
+    q = rng.uniform(-1.3, 1.3, n)
+    fr = [(pos[:, :ndim] if ndim == 2 else pos, it, q), (pos[:, :ndim] if ndim == 2 else pos, it, -0.5 * q)]
+    print("case: g_qq", case, flush=True)
+    acc, gqq, _, ls = _run_rdf(fr, ndim, 2, box, per, rc, dr)
+    ofr = [(pos, it, q), (pos, it, -0.5 * q)]
+    oacc = _oracle_gqq_acc(ofr, ndim, box, per, rc, dr, ls)
+    np.testing.assert_array_equal(acc, oacc)
+    vol = L * L * (L if ndim == 3 else 1.0)
+    np.testing.assert_array_equal(gqq, O.rdf_gqq(ndim, ls, dr, vol, n, 2, oacc))
+
+
+def test_zero_charges_and_flag_checks():
+    rng = np.random.default_rng(12)
+    L = 12.0
+    pos = rng.random((1000, 3)) * L
+    it = rng.integers(1, 3, 1000).astype(np.int32)
+    print("case: g_qq zero charges", flush=True)
+    with T.Context(0) as ctx:
+        ctx.rdf_init(T.rdf_params(3, 2, 0.05, 4.0, flags=T.TRJ_RDF_CHARGE), [L] * 3)
+        with pytest.raises(T.TrjError):
+            ctx.frame(pos, it, [L] * 3, what=T.TRJ_WANT_RDF)       # charges required
+        ctx.frame(pos, it, [L] * 3, what=T.TRJ_WANT_RDF, charge=np.zeros(1000))
+        assert not ctx.rdf_qq().any()
+    with T.Context(0) as ctx:
+        ctx.rdf_init(T.rdf_params(3, 2, 0.05, 4.0), [L] * 3)
+        ctx.frame(pos, it, [L] * 3, what=T.TRJ_WANT_RDF)
+        with pytest.raises(T.TrjError):
+            ctx.rdf_qq()                                            # channel not enabled
+
+
+def _wrapped(pos, box):
+    return O.wrap(pos, box, (1, 1, 1))
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    m = np.abs(b) > 0
+    return float(np.max(np.abs(a[m] - b[m]) / np.abs(b[m]))) if m.any() else 0.0
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 2e-4), ("fp16x3", 1e-5)])
+def test_sqq_lj_binary(precision, tol):
+    frames = list(W.lj_binary_4k(nframes=2))
+    wl = W.LJ_BINARY_4K
+    qs = [_charges(f.itype, 20 + k) for k, f in enumerate(frames)]
+    print("case: S_qq lj_binary", precision, flush=True)
+    with T.Context(0) as ctx:
+        ctx.sq_init(T.sq_params(3, 2, wl.qmax, wl.qmin, wl.bsc, precision=precision,
+                                flags=T.TRJ_SQ_CHARGE), frames[0].box)
+        for f, q in zip(frames, qs):
+            ctx.frame(f.pos, f.itype, f.box, what=T.TRJ_WANT_SQ, charge=q)
+        sqfq = ctx.sq_qq()
+        col = ctx.sq_qq_column()
+        sqf, sqfp, nq = ctx.sq_shells()
+    g = O.sq_setup(3, frames[0].box, wl.qmax, wl.qmax)
+    osqfq = np.zeros(g.nqmax + 1)
+    osqf = np.zeros(g.nqmax + 1)
+    osqfp = np.zeros((2, g.nqmax + 1))
+    for f, q in zip(frames, qs):
+        O.sq_frame_q(_wrapped(f.pos, f.box), q, 3, f.box, g, osqfq)
+        O.sq_frame(_wrapped(f.pos, f.box), f.itype, 3, 2, f.box, g, wl.bsc, osqf, osqfp)
+    print("max rel sqfq", _rel(sqfq, osqfq), "sqf", _rel(sqf, osqf), flush=True)
+    assert _rel(sqfq, osqfq) <= tol
+    assert _rel(sqf, osqf) <= tol and _rel(sqfp, osqfp) <= tol
+    onq = O.sq_nq(3, g)
+    want = np.array([osqfq[i] / (4000 * 2 * onq[i]) if onq[i] else 0.0 for i in range(1, g.nqmax + 1)])
+    np.testing.assert_allclose(col, want, rtol=max(tol, 1e-12))
+
+
+def test_sqq_unit_charges_equal_total_density():
+    """q = 1 for every atom: the charge channel is rho_N, so sqfq equals the
+    bsc = 1 total sum of the same frame."""
+    f = next(W.lj_binary_4k(nframes=1))
+    print("case: S_qq unit charges", flush=True)
+    with T.Context(0) as ctx:
+        ctx.sq_init(T.sq_params(3, 2, 9.0, 9.0, [1.0, 1.0], flags=T.TRJ_SQ_CHARGE), f.box)
+        ctx.frame(f.pos, f.itype, f.box, what=T.TRJ_WANT_SQ, charge=np.ones(len(f.itype)))
+        sqfq = ctx.sq_qq()
+        sqf, _, _ = ctx.sq_shells()
+    assert _rel(sqfq, sqf) <= 1e-12
+
+
+def test_sqq_axis_list_mode():
+    f = next(W.lj_binary_4k(nframes=1))
+    q = _charges(f.itype, 30)
+    print("case: S_qq axis list", flush=True)
+    with T.Context(0) as ctx:
+        ctx.sq_init(T.sq_params(3, 2, 6.0, 0.3, [1.0, 0.5], flags=T.TRJ_SQ_CHARGE), f.box)
+        ctx.frame(f.pos, f.itype, f.box, what=T.TRJ_WANT_SQ, charge=q)
+        sqfq = ctx.sq_qq()
+    g = O.sq_setup(3, f.box, 6.0, 0.3)
+    osqfq = O.sq_frame_q(_wrapped(f.pos, f.box), q, 3, f.box, g)
+    assert _rel(sqfq, osqfq) <= 1e-12
+
+
+def test_charge_state_round_trip(tmp_path):
+    frames = list(W.lj_binary_4k(nframes=2))
+    wl = W.LJ_BINARY_4K
+    qs = [_charges(f.itype, 40 + k) for k, f in enumerate(frames)]
+    rp = T.rdf_params(3, 2, 0.02, 4.0, flags=T.TRJ_RDF_CHARGE)
+    sp = T.sq_params(3, 2, wl.qmax, wl.qmin, wl.bsc, flags=T.TRJ_SQ_CHARGE)
+    print("case: charge state round trip", flush=True)
+    with T.Context(0) as a:
+        a.rdf_init(rp, frames[0].box)
+        a.sq_init(sp, frames[0].box)
+        for f, q in zip(frames, qs):
+            a.frame(f.pos, f.itype, f.box, charge=q)
+        want_g, want_s = a.rdf_qq(), a.sq_qq()
+    with T.Context(0) as b:
+        b.rdf_init(rp, frames[0].box)
+        b.sq_init(sp, frames[0].box)
+        b.frame(frames[0].pos, frames[0].itype, frames[0].box, charge=qs[0])
+        blob = b.state_save()
+    with T.Context(0) as c:
+        c.rdf_init(rp, frames[0].box)
+        c.sq_init(sp, frames[0].box)
+        c.state_load(blob)
+        c.frame(frames[1].pos, frames[1].itype, frames[1].box, charge=qs[1])
+        np.testing.assert_array_equal(c.rdf_qq(), want_g)
+        assert _rel(c.sq_qq(), want_s) <= 1e-14
+    with T.Context(0) as d:                       # no charge channel: state refused
+        d.rdf_init(T.rdf_params(3, 2, 0.02, 4.0), frames[0].box)
+        d.sq_init(T.sq_params(3, 2, wl.qmax, wl.qmin, wl.bsc), frames[0].box)
+        with pytest.raises(T.TrjError):
+            d.state_load(blob)

@@ -27,15 +27,19 @@ void launch_ingest(const IngestArgs& a, hipStream_t s);
 
 // Stable partition of atoms by species: per-block counts (species-major), a
 // scan, then a scatter of the S(Q) turns u = x / Lq (hi + lo) and of the unit
-// steps e^{2 pi i u} into species order: u holds 12 * n doubles.
+// steps e^{2 pi i u} into species order.  u holds 12 planes of `stride`
+// doubles.  With charges q (may be NULL) the planes hold 2n entries: the n
+// species-sorted atoms, then all atoms again in input order as the charge
+// channel, and a 13th plane of weights (1 for the species entries, q_i for
+// the charge channel); sp_off then has nsp + 2 entries (extra_end = 2n).
 void launch_species_count(const int32_t* itype, int64_t n, int nsp, int32_t* blk, int nblk,
                           hipStream_t s);
 void launch_exclusive_scan(const int32_t* in, int32_t* out, int64_t count, hipStream_t s);
 void launch_species_offsets(const int32_t* blkoff, const int32_t* blk, int nblk, int nsp,
-                            int64_t n, int32_t* sp_off, hipStream_t s);
+                            int64_t extra_end, int32_t* sp_off, hipStream_t s);
 void launch_species_scatter(const int32_t* itype, int64_t n, int nsp, const int32_t* blkoff,
                             int nblk, const double* pos, const double lq[3], int ndim,
-                            double* u, hipStream_t s);
+                            const double* q, int64_t stride, double* u, hipStream_t s);
 
 // Cell list.
 struct CellGrid {
@@ -45,9 +49,10 @@ struct CellGrid {
 };
 void launch_cell_count(const double* pos, int64_t n, int ndim, const CellGrid& g,
                        int32_t* cellid, int32_t* count, hipStream_t s);
+// q (may be NULL): per-atom charges, scattered into cell order as sq.
 void launch_cell_scatter(const double* pos, const int32_t* itype, const int32_t* cellid,
                          int64_t n, const int32_t* start, int32_t* cursor, double* spos,
-                         int32_t* stype, hipStream_t s);
+                         int32_t* stype, const double* q, double* sq, hipStream_t s);
 
 struct PairArgs {
   int ndim, nsp, nit, lsmax;
@@ -59,10 +64,16 @@ struct PairArgs {
   double side2;
   float inv_dr;
   const double* thr;  // lsmax+3 edges
+  // Charge-weighted histogram (NULL q: off): q in the pair kernel's atom
+  // order, each in-range pair adds rint(q_i q_j qscale) to gq[k-1] (int64,
+  // two's complement in unsigned 64-bit atomics).
+  const double* q;
+  unsigned long long* gq;
+  double qscale;
 };
-size_t pair_lds_bytes(int nit, int lsmax);
+size_t pair_lds_bytes(int nit, int lsmax, bool qq = false);
 // LDS of the cell-list pair kernel: pair_lds_bytes plus its hit rings.
-size_t cells_lds_bytes(int nit, int lsmax);
+size_t cells_lds_bytes(int nit, int lsmax, bool qq = false);
 // Cell-list pairs (half stencil); every periodic axis needs nc >= 3.
 void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start,
                       const double* spos, const int32_t* stype, int64_t n, int waves_per_cell,
@@ -73,30 +84,37 @@ void launch_rdf_allpairs(const PairArgs& a, const double* pos, const int32_t* it
 // counts[b] += sum over slices of partial[slice][b].
 void launch_rdf_merge(const uint32_t* partial, int nslices, int nbins, int64_t* counts,
                       hipStream_t s);
+// acc[k] += ((double)gq[k] * inv_scale) / avq2, then gq[k] = 0 (k < lsmax).
+void launch_qq_finish(unsigned long long* gq, int lsmax, double inv_scale, double avq2, double* acc,
+                      hipStream_t s);
 
 // S(Q) contraction: rho_s(q) over the active tiles of the dense q-box.
 struct ContractArgs {
   const double* u;        // species-sorted turns x/L as hi[3][n], lo[3][n], then
-                          // e^{2 pi i u} as cos[3][n], sin[3][n]
-  const int32_t* sp_off;  // nsp+1
+                          // e^{2 pi i u} as cos[3][n], sin[3][n] (n = plane stride)
+  const double* w;        // per-entry weights (plane 12 of u) or NULL: all 1
+  const int32_t* sp_off;  // nch+1
   const int32_t* tiles;   // 4 ints per tile: kx0, ky0, kz0, pad
   int ntiles, ksplit, nsp, ndim;
+  int nch;                // channels: nsp species, + 1 charge channel
   int64_t n;
-  double* rho;            // [split][s][tile][tile vectors][2]
+  double* rho;            // [split][channel][tile][tile vectors][2]
 };
 void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s);
 // FP32 MFMA, or FP16 / BF16 hi+lo split with three products (precision =
 // TRJ_PREC_FP32 / FP16X3 / BF16X3); FP32 accumulation folded into FP64.
 void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s);
 // Sum the splits, square, and bin into shells (FP64 atomics).
+// With nch = nsp + 1 the last channel's |rho|^2 goes to sqfq.
 void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, int tile_vec, int ksplit,
-                      int nsp, const double* bsc, int nqmax, double* sqf, double* sqfp, hipStream_t s);
+                      int nsp, int nch, const double* bsc, int nqmax, double* sqf, double* sqfp,
+                      double* sqfq, hipStream_t s);
 // Explicit q-vectors (axis directions): direct FP64 phase sums over atom
 // blocks, combined in a fixed order; part needs sq_list_scratch_bytes().
-size_t sq_list_scratch_bytes(int nvec, int nsp);
-void launch_sq_list(const double* u, const int32_t* sp_off, int64_t n, int nsp, int ndim,
-                    const int32_t* kv, const int32_t* shell, int nvec, const double* bsc,
-                    int nqmax, double* part, double* sqf, double* sqfp, hipStream_t s);
+size_t sq_list_scratch_bytes(int nvec, int nch);
+void launch_sq_list(const double* u, const double* w, const int32_t* sp_off, int64_t stride, int nsp,
+                    int nch, int ndim, const int32_t* kv, const int32_t* shell, int nvec, const double* bsc,
+                    int nqmax, double* part, double* sqf, double* sqfp, double* sqfq, hipStream_t s);
 
 // LDS-atomic throughput probe (random bins of a 4096-bin uint32 histogram);
 // returns the number of atomics the launch performs; *out gets their sum.

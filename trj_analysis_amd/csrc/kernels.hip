@@ -117,11 +117,12 @@ __global__ __launch_bounds__(1024) void k_exclusive_scan(const int32_t* __restri
 }
 
 __global__ void k_species_offsets(const int32_t* __restrict__ blkoff, const int32_t* __restrict__ blk,
-                                  int nblk, int nsp, int32_t* __restrict__ sp_off) {
+                                  int nblk, int nsp, int64_t extra_end, int32_t* __restrict__ sp_off) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   for (int s = 0; s < nsp; ++s) sp_off[s] = blkoff[static_cast<int64_t>(s) * nblk];
   const int64_t last = static_cast<int64_t>(nsp) * nblk - 1;
   sp_off[nsp] = blkoff[last] + blk[last];
+  if (extra_end > 0) sp_off[nsp + 1] = static_cast<int32_t>(extra_end);  // the charge channel
 }
 
 // Phase in turns, frac(n*(uh + ul)), reduced exactly: the product's rounding
@@ -137,6 +138,7 @@ __global__ __launch_bounds__(256) void k_species_scatter(const int32_t* __restri
                                                          int nsp, const int32_t* __restrict__ blkoff,
                                                          int nblk, const double* __restrict__ pos,
                                                          double lq0, double lq1, double lq2, int ndim,
+                                                         const double* __restrict__ q, int64_t stride,
                                                          double* __restrict__ u) {
   __shared__ int cnt[4][kNspMax];
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
@@ -150,24 +152,42 @@ __global__ __launch_bounds__(256) void k_species_scatter(const int32_t* __restri
     if (s == sp) mine = m;
   }
   __syncthreads();
-  if (s < 0) return;
-  int off = blkoff[static_cast<int64_t>(s) * nblk + blockIdx.x];
-  for (int ww = 0; ww < w; ++ww) off += cnt[ww][s];
-  off += __popcll(mine & ((1ull << lane) - 1ull));
+  if (i >= n) return;
+  // An atom with a type outside 1..nsp (flagged by the ingest) has no species
+  // entry; its charge-channel entry gets weight 0.
+  int off = 0;
+  if (s >= 0) {
+    off = blkoff[static_cast<int64_t>(s) * nblk + blockIdx.x];
+    for (int ww = 0; ww < w; ++ww) off += cnt[ww][s];
+    off += __popcll(mine & ((1ull << lane) - 1ull));
+  }
   // Turns u = x/L as an unevaluated sum hi + lo: hi rounded, lo = (x - hi*L)/L
   // with the remainder exact through the FMA; then the unit step e^{2 pi i u}
-  // of the phase recurrences as cos, sin.
+  // of the phase recurrences as cos, sin.  With charges the same values also
+  // go to entry n + i of the charge channel, and plane 12 holds the weights.
   const double lq[3] = {lq0, lq1, lq2};
   for (int k = 0; k < 3; ++k) {
     const double x = (k < ndim) ? pos[k * n + i] : 0.0;
     const double hi = x / lq[k];
     const double lo = fma(-hi, lq[k], x) / lq[k];
-    u[k * n + off] = hi;
-    u[(3 + k) * n + off] = lo;
     double sn, cs;
     sincospi(2.0 * frac_turns(1, hi, lo), &sn, &cs);
-    u[(6 + k) * n + off] = cs;
-    u[(9 + k) * n + off] = sn;
+    if (s >= 0) {
+      u[k * stride + off] = hi;
+      u[(3 + k) * stride + off] = lo;
+      u[(6 + k) * stride + off] = cs;
+      u[(9 + k) * stride + off] = sn;
+    }
+    if (q) {  // finite values for a flagged atom, whose weight is 0
+      u[k * stride + n + i] = s >= 0 ? hi : 0.0;
+      u[(3 + k) * stride + n + i] = s >= 0 ? lo : 0.0;
+      u[(6 + k) * stride + n + i] = s >= 0 ? cs : 1.0;
+      u[(9 + k) * stride + n + i] = s >= 0 ? sn : 0.0;
+    }
+  }
+  if (q) {
+    if (s >= 0) u[12 * stride + off] = 1.0;
+    u[12 * stride + n + i] = s >= 0 ? q[i] : 0.0;
   }
 }
 
@@ -194,7 +214,9 @@ __global__ __launch_bounds__(256) void k_cell_scatter(const double* __restrict__
                                                       const int32_t* __restrict__ start,
                                                       int32_t* __restrict__ cursor,
                                                       double* __restrict__ spos,
-                                                      int32_t* __restrict__ stype) {
+                                                      int32_t* __restrict__ stype,
+                                                      const double* __restrict__ q,
+                                                      double* __restrict__ sq) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n) return;
   const int c = cellid[i];
@@ -203,6 +225,7 @@ __global__ __launch_bounds__(256) void k_cell_scatter(const double* __restrict__
   spos[n + slot] = pos[n + i];
   spos[2 * n + slot] = pos[2 * n + i];
   stype[slot] = itype[i];
+  if (q) sq[slot] = q[i];
 }
 
 // rdf.cuf:748-752 for i <= j (callers order the pair).
@@ -230,9 +253,10 @@ struct Stencil {
   int off[14][3];
 };
 
-__device__ inline void pair_accumulate(const PairArgs& a, const double* thr, uint32_t* hist,
-                                       bool ok, double xi, double yi, double zi, int ti, double xj,
-                                       double yj, double zj, int tj) {
+// Returns the bin the pair was added to (0: none).
+__device__ inline int pair_accumulate(const PairArgs& a, const double* thr, uint32_t* hist,
+                                      bool ok, double xi, double yi, double zi, int ti, double xj,
+                                      double yj, double zj, int tj) {
   const double dx = min_image(xj - xi, a.lmi[0], a.half[0]);
   const double dy = min_image(yj - yi, a.lmi[1], a.half[1]);
   double r2;
@@ -248,8 +272,16 @@ __device__ inline void pair_accumulate(const PairArgs& a, const double* thr, uin
       const int lo = ti < tj ? ti : tj, hi = ti < tj ? tj : ti;
       const int ij = pair_index(lo, hi, a.nsp);
       atomicAdd(&hist[(ij - 1) * a.lsmax + (k - 1)], 1u);
+      return k;
     }
   }
+  return 0;
+}
+
+// q_i q_j rounded once to the frame's fixed point (exact power-of-two scale),
+// as the two's complement bits of an int64 (added with unsigned atomics).
+__device__ inline unsigned long long qq_fixed(double qq, double scale) {
+  return static_cast<unsigned long long>(static_cast<long long>(rint(qq * scale)));
 }
 
 __device__ inline void hist_setup(const PairArgs& a, uint32_t* hist, double* thr) {
@@ -282,9 +314,13 @@ __device__ inline double readlane_d(double v, int l) {
 constexpr int kHitQ = 128;  // ring entries per wavefront (holds < 64 + 64)
 
 
-// Bin entries head .. head+cnt-1 (cnt <= 64) of a wavefront's ring, one per lane.
+// Bin entries head .. head+cnt-1 (cnt <= 64) of a wavefront's ring, one per
+// lane.  QQ: the ring also holds q_i q_j (qqr), added to the block's
+// fixed-point charge histogram gqs.
+template <bool QQ = false>
 __device__ inline void bin_hits(const PairArgs& a, const double* thr, uint32_t* hist, const double* qr2,
-                                const int* qtt, uint32_t head, int cnt, int lane) {
+                                const int* qtt, uint32_t head, int cnt, int lane, const double* qqr = nullptr,
+                                unsigned long long* gqs = nullptr) {
   if (lane < cnt) {
     const uint32_t e = (head + lane) & (kHitQ - 1);
     const double r2 = qr2[e];
@@ -294,6 +330,7 @@ __device__ inline void bin_hits(const PairArgs& a, const double* thr, uint32_t* 
     if (k >= 1 && k <= a.lsmax) {
       const int lo = ti < tj ? ti : tj, hi = ti < tj ? tj : ti;
       atomicAdd(&hist[(pair_index(lo, hi, a.nsp) - 1) * a.lsmax + (k - 1)], 1u);
+      if constexpr (QQ) atomicAdd(&gqs[k - 1], qq_fixed(qqr[e], a.qscale));
     }
   }
 }
@@ -301,12 +338,14 @@ __device__ inline void bin_hits(const PairArgs& a, const double* thr, uint32_t* 
 // Candidates of one neighbour cell for the 64 home atoms of a wavefront.
 // Atoms outside the cell ranges or with types outside 1..nsp carry NaN
 // coordinates, so r2 < side2 fails for them without a separate test.
-template <bool SHIFT, bool D3, bool SELF, bool IMG>
+template <bool SHIFT, bool D3, bool SELF, bool IMG, bool QQ = false>
 __device__ inline void rdf_neighbours(const PairArgs& a, const double* thr, uint32_t* hist, double* qr2, int* qtt,
                                       uint32_t& head, uint32_t& tail, int lane, int js, int je, int gi,
                                       double xi, double yi, double zi, int ti8, double shx, double shy,
                                       double shz, const double* __restrict__ sx, const double* __restrict__ sy,
-                                      const double* __restrict__ sz, const int32_t* __restrict__ stype) {
+                                      const double* __restrict__ sz, const int32_t* __restrict__ stype,
+                                      double qi = 0.0, const double* __restrict__ sq = nullptr,
+                                      double* qqr = nullptr, unsigned long long* gqs = nullptr) {
   const double nan = __builtin_nan("");
   for (int jb = js; jb < je; jb += kWave) {
     const int jl = jb + lane;
@@ -316,6 +355,8 @@ __device__ inline void rdf_neighbours(const PairArgs& a, const double* thr, uint
     const double xl = vl ? sx[jl] : nan;
     const double yl = vl ? sy[jl] : nan;
     const double zl = (D3 && vl) ? sz[jl] : nan;
+    double ql = 0.0;
+    if constexpr (QQ) ql = vl ? sq[jl] : 0.0;
     const int cnt = (je - jb) < kWave ? (je - jb) : kWave;
     for (int jj = 0; jj < cnt; ++jj) {
       double r2;
@@ -346,17 +387,20 @@ __device__ inline void rdf_neighbours(const PairArgs& a, const double* thr, uint
       bool hit = r2 < a.side2;
       if constexpr (SELF) hit = hit && (jb + jj > gi);
       const unsigned long long m = __ballot(hit);
+      double qj = 0.0;
+      if constexpr (QQ) qj = readlane_d(ql, jj);
       if (hit) {
         const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
                                                          __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
         const uint32_t e = (tail + below) & (kHitQ - 1);
         qr2[e] = r2;
         qtt[e] = ti8 | __builtin_amdgcn_readlane(tl, jj);
+        if constexpr (QQ) qqr[e] = qi * qj;
       }
       tail += static_cast<uint32_t>(__popcll(m));
       if (tail - head >= kWave) {
         __builtin_amdgcn_wave_barrier();
-        bin_hits(a, thr, hist, qr2, qtt, head, kWave, lane);
+        bin_hits<QQ>(a, thr, hist, qr2, qtt, head, kWave, lane, qqr, gqs);
         __builtin_amdgcn_wave_barrier();
         head += kWave;
       }
@@ -367,13 +411,14 @@ __device__ inline void rdf_neighbours(const PairArgs& a, const double* thr, uint
 // One chunk of up to 64 atoms of home cell cellL (cells numbered from
 // cell_off in `start`) against the half stencil; hits go to the wavefront's
 // ring (qr2, qtt, head, tail).
-template <bool SHIFT, bool D3>
+template <bool SHIFT, bool D3, bool QQ = false>
 __device__ inline void rdf_home_chunk(const PairArgs& a, const double* thr, uint32_t* hist, double* qr2, int* qtt,
                                       uint32_t& head, uint32_t& tail, int lane, const Stencil& st, const int nc[3],
                                       const int periodic[3], int cell_off, int cellL, int ch, int cs, int ce,
                                       const int32_t* __restrict__ start, const double* __restrict__ sx,
                                       const double* __restrict__ sy, const double* __restrict__ sz,
-                                      const int32_t* __restrict__ stype) {
+                                      const int32_t* __restrict__ stype, const double* __restrict__ sq = nullptr,
+                                      double* qqr = nullptr, unsigned long long* gqs = nullptr) {
   const double nan = __builtin_nan("");
   const int ncx = nc[0], ncy = nc[1], ncz = nc[2];
   const int cx = cellL % ncx, cy = (cellL / ncx) % ncy, cz = cellL / (ncx * ncy);
@@ -384,6 +429,8 @@ __device__ inline void rdf_home_chunk(const PairArgs& a, const double* thr, uint
   const double yi = oki ? sy[gi] : nan;
   const double zi = (D3 && oki) ? sz[gi] : nan;
   const int ti8 = (oki ? ti : 0) << 8;
+  double qi = 0.0;
+  if constexpr (QQ) qi = oki ? sq[gi] : 0.0;
   for (int o = 0; o < st.n; ++o) {
     int nx = cx + st.off[o][0], ny = cy + st.off[o][1], nz = cz + st.off[o][2];
     // Image shift of a wrapped neighbour, the L*nint(d/L) the FP64 rule
@@ -401,22 +448,23 @@ __device__ inline void rdf_home_chunk(const PairArgs& a, const double* thr, uint
     // 64 neighbours per pass: one coalesced load per lane, then a broadcast
     // per pair through the scalar registers.
     if (o == 0) {
-      rdf_neighbours<SHIFT, D3, true, false>(a, thr, hist, qr2, qtt, head, tail, lane, cs + ch * kWave + 1, ce,
-                                             gi, xi, yi, zi, ti8, 0.0, 0.0, 0.0, sx, sy, sz, stype);
+      rdf_neighbours<SHIFT, D3, true, false, QQ>(a, thr, hist, qr2, qtt, head, tail, lane, cs + ch * kWave + 1,
+                                                 ce, gi, xi, yi, zi, ti8, 0.0, 0.0, 0.0, sx, sy, sz, stype, qi, sq,
+                                                 qqr, gqs);
     } else {
       const int js = __builtin_amdgcn_readfirstlane(start[nb]);
       const int je = __builtin_amdgcn_readfirstlane(start[nb + 1]);
       if (shx != 0.0 || shy != 0.0 || shz != 0.0)
-        rdf_neighbours<SHIFT, D3, false, true>(a, thr, hist, qr2, qtt, head, tail, lane, js, je, gi, xi, yi, zi,
-                                               ti8, shx, shy, shz, sx, sy, sz, stype);
+        rdf_neighbours<SHIFT, D3, false, true, QQ>(a, thr, hist, qr2, qtt, head, tail, lane, js, je, gi, xi, yi,
+                                                   zi, ti8, shx, shy, shz, sx, sy, sz, stype, qi, sq, qqr, gqs);
       else
-        rdf_neighbours<SHIFT, D3, false, false>(a, thr, hist, qr2, qtt, head, tail, lane, js, je, gi, xi, yi, zi,
-                                                ti8, 0.0, 0.0, 0.0, sx, sy, sz, stype);
+        rdf_neighbours<SHIFT, D3, false, false, QQ>(a, thr, hist, qr2, qtt, head, tail, lane, js, je, gi, xi, yi,
+                                                    zi, ti8, 0.0, 0.0, 0.0, sx, sy, sz, stype, qi, sq, qqr, gqs);
     }
   }
 }
 
-template <bool SHIFT, bool D3>
+template <bool SHIFT, bool D3, bool QQ>
 __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stencil st,
                                                    const int32_t* __restrict__ start,
                                                    const double* __restrict__ spos,
@@ -427,13 +475,22 @@ __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stenc
   const int nbins = a.nit * a.lsmax;
   uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
   double* thr = reinterpret_cast<double*>(smem + ((static_cast<size_t>(nbins) * 4 + 15) & ~size_t(15)));
-  // Same layout as cells_lds_bytes: hist, thr, then the rings.
+  // Same layout as cells_lds_bytes: hist, thr, the rings, then (QQ) the
+  // charge-product rings and the block's fixed-point charge histogram.
   const size_t qoff = (((static_cast<size_t>(nbins) * 4 + 15) & ~size_t(15)) +
                        static_cast<size_t>(a.lsmax + 3) * 8 + 15) & ~size_t(15);
+  const size_t qqoff = qoff + 4 * kHitQ * (sizeof(double) + sizeof(int));
+  double* qqr = nullptr;
+  unsigned long long* gqs = nullptr;
+  if constexpr (QQ) {
+    gqs = reinterpret_cast<unsigned long long*>(smem + qqoff + 4 * kHitQ * sizeof(double));
+    for (int b = threadIdx.x; b < a.lsmax; b += blockDim.x) gqs[b] = 0ull;
+  }
   hist_setup(a, hist, thr);
   const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   double* qr2 = reinterpret_cast<double*>(smem + qoff) + wave * kHitQ;
   int* qtt = reinterpret_cast<int*>(smem + qoff + 4 * kHitQ * sizeof(double)) + wave * kHitQ;
+  if constexpr (QQ) qqr = reinterpret_cast<double*>(smem + qqoff) + wave * kHitQ;
   uint32_t head = 0, tail = 0;  // wave-uniform ring counters
   const double* __restrict__ sx = spos;
   const double* __restrict__ sy = spos + n;
@@ -445,12 +502,18 @@ __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stenc
     const int cs = __builtin_amdgcn_readfirstlane(start[cell]);
     const int ce = __builtin_amdgcn_readfirstlane(start[cell + 1]);
     for (int ch = ch0; cs + ch * kWave < ce; ch += W)
-      rdf_home_chunk<SHIFT, D3>(a, thr, hist, qr2, qtt, head, tail, lane, st, g.nc, g.periodic, 0, cell, ch, cs,
-                                ce, start, sx, sy, sz, stype);
+      rdf_home_chunk<SHIFT, D3, QQ>(a, thr, hist, qr2, qtt, head, tail, lane, st, g.nc, g.periodic, 0, cell, ch,
+                                    cs, ce, start, sx, sy, sz, stype, a.q, qqr, gqs);
   }
   __builtin_amdgcn_wave_barrier();
-  bin_hits(a, thr, hist, qr2, qtt, head, static_cast<int>(tail - head), lane);
-  hist_flush(a, hist, partial);
+  bin_hits<QQ>(a, thr, hist, qr2, qtt, head, static_cast<int>(tail - head), lane, qqr, gqs);
+  hist_flush(a, hist, partial);  // begins with a barrier: every bin_hits is done
+  if constexpr (QQ) {
+    for (int b = threadIdx.x; b < a.lsmax; b += blockDim.x) {
+      const unsigned long long v = gqs[b];
+      if (v) atomicAdd(&a.gq[b], v);
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_rdf_allpairs(PairArgs a, const double* __restrict__ pos,
@@ -460,6 +523,13 @@ __global__ __launch_bounds__(256) void k_rdf_allpairs(PairArgs a, const double* 
   const int nbins = a.nit * a.lsmax;
   uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
   double* thr = reinterpret_cast<double*>(smem + ((static_cast<size_t>(nbins) * 4 + 15) & ~size_t(15)));
+  // Same layout as pair_lds_bytes: hist, thr, then (a.q) the block's
+  // fixed-point charge histogram.
+  unsigned long long* gqs = reinterpret_cast<unsigned long long*>(
+      smem + ((((static_cast<size_t>(nbins) * 4 + 15) & ~size_t(15)) + static_cast<size_t>(a.lsmax + 3) * 8 + 15) &
+              ~size_t(15)));
+  if (a.q)
+    for (int b = threadIdx.x; b < a.lsmax; b += blockDim.x) gqs[b] = 0ull;
   hist_setup(a, hist, thr);
   const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   const double* __restrict__ px = pos;
@@ -477,16 +547,35 @@ __global__ __launch_bounds__(256) void k_rdf_allpairs(PairArgs a, const double* 
     const double zi = vi ? pz[gi] : 0.0;
     const int ti = vi ? itype[gi] : 0;
     const bool oki = vi && static_cast<unsigned>(ti - 1) < static_cast<unsigned>(a.nsp);
+    const double qi = (a.q && vi) ? a.q[gi] : 0.0;
     const int64_t js = static_cast<int64_t>(jb) * kWave;
     const int64_t je = js + kWave < n ? js + kWave : n;
     for (int64_t j = js; j < je; ++j) {
       const int tj = itype[j];
       const bool ok = oki && (jb > ib || j > gi) &&
                       static_cast<unsigned>(tj - 1) < static_cast<unsigned>(a.nsp);
-      pair_accumulate(a, thr, hist, ok, xi, yi, zi, ti, px[j], py[j], pz[j], tj);
+      const int k = pair_accumulate(a, thr, hist, ok, xi, yi, zi, ti, px[j], py[j], pz[j], tj);
+      if (a.q && k) atomicAdd(&gqs[k - 1], qq_fixed(qi * a.q[j], a.qscale));
     }
   }
-  hist_flush(a, hist, partial);
+  hist_flush(a, hist, partial);  // begins with a barrier
+  if (a.q) {
+    for (int b = threadIdx.x; b < a.lsmax; b += blockDim.x) {
+      const unsigned long long v = gqs[b];
+      if (v) atomicAdd(&a.gq[b], v);
+    }
+  }
+}
+
+// The frame's charge sums into the FP64 accumulators (rdf.cuf:285-287:
+// gqq += gqq_frame / avq2), then cleared for the next frame.
+__global__ __launch_bounds__(256) void k_qq_finish(unsigned long long* __restrict__ gq, int lsmax, double inv_scale,
+                                                   double avq2, double* __restrict__ acc) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= lsmax) return;
+  const long long f = static_cast<long long>(gq[k]);
+  if (avq2 > 0.0) acc[k] += (static_cast<double>(f) * inv_scale) / avq2;
+  gq[k] = 0ull;
 }
 
 // Slice group blockIdx.y sums its share of the per-workgroup slices, then adds
@@ -519,12 +608,12 @@ constexpr int TKX = 8, TKY = 8, TKZ = 32, TILE_VEC = TKX * TKY * TKZ;
 // conflict-free.
 constexpr int kRec = 49;
 
-// e^{2 pi i n u} for n = n0 .. n0+7 into dst: the anchor n0, then
-// angle-addition steps by e^{2 pi i u} = cs + i ss.
-__device__ inline void phase8(double uh, double ul, int n0, double cs, double ss, double2* dst) {
+// m e^{2 pi i n u} for n = n0 .. n0+7 into dst: the anchor n0 (times the
+// channel weight m), then angle-addition steps by e^{2 pi i u} = cs + i ss.
+__device__ inline void phase8(double uh, double ul, int n0, double cs, double ss, double2* dst, double m) {
   double s, c;
   sincospi(2.0 * frac_turns(n0, uh, ul), &s, &c);
-  double re = c, im = s;
+  double re = c * m, im = s * m;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     dst[k] = make_double2(re, im);
@@ -600,7 +689,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const double* u_c = a.u + 6 * a.n;
   const double* u_s = a.u + 9 * a.n;
   const uint32_t st_base = static_cast<uint32_t>(st_ax) * static_cast<uint32_t>(a.n);
-  for (int s = 0; s < a.nsp; ++s) {
+  // The x segment carries the channel weight (1, or the charge of the entry).
+  const bool st_w = a.w != nullptr && st_seg == 0;
+  for (int s = 0; s < a.nch; ++s) {
     const int a0 = a.sp_off[s], a1 = a.sp_off[s + 1];
     const int64_t len = a1 - a0;
     const int b0 = a0 + static_cast<int>(len * split / a.ksplit);
@@ -617,13 +708,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         cmpR[nb][r] = cmpI[nb][r] = 0.0f;
       }
     }
-    double uh = 0.0, ul = 0.0, uc = 1.0, us = 0.0;
+    double uh = 0.0, ul = 0.0, uc = 1.0, us = 0.0, uw = 1.0;
     if (stager && b0 + st_at < b1) {
       const uint32_t o = st_base + static_cast<uint32_t>(b0 + st_at);
       uh = u_hi[o];
       ul = u_lo[o];
       uc = u_c[o];
       us = u_s[o];
+      if (st_w) uw = a.w[b0 + st_at];
     }
     // Write chunk c0's phases into buffer buf from the prefetched turns, then
     // prefetch the turns of chunk c0 + KC.
@@ -631,7 +723,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       if (!stager) return;
       double2* dst = ph[buf][st_at] + st_slot;
       if (c0 + st_at < b1) {
-        phase8(uh, ul, st_n0, uc, us, dst);
+        phase8(uh, ul, st_n0, uc, us, dst, uw);
       } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k) dst[k] = make_double2(0.0, 0.0);
@@ -643,6 +735,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         ul = u_lo[o];
         uc = u_c[o];
         us = u_s[o];
+        if (st_w) uw = a.w[nx];
       }
     };
     __syncthreads();  // the previous species' readers are done with both buffers
@@ -679,7 +772,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     // Vector index within the tile: row mrow = w*16 + 4r + kq (= kx*8 + ky),
     // column nb*16 + i16 (kz), i.e. mrow * TKZ + nb * 16 + i16.
     double2* out = reinterpret_cast<double2*>(a.rho) +
-                   ((static_cast<size_t>(split) * a.nsp + s) * a.ntiles + tile) * TILE_VEC +
+                   ((static_cast<size_t>(split) * a.nch + s) * a.ntiles + tile) * TILE_VEC +
                    ((w * 16 + kq) * TKZ + i16);
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) {
@@ -760,7 +853,7 @@ __global__ __launch_bounds__(256) void k_sq_contract_f32(ContractArgs a) {
   const double* __restrict__ lx = a.u + 3 * a.n;
   const double* __restrict__ ly = a.u + 4 * a.n;
   const double* __restrict__ lz = a.u + 5 * a.n;
-  for (int s = 0; s < a.nsp; ++s) {
+  for (int s = 0; s < a.nch; ++s) {
     const int a0 = a.sp_off[s], a1 = a.sp_off[s + 1];
     const int64_t len = a1 - a0;
     const int b0 = a0 + static_cast<int>(len * split / a.ksplit);
@@ -782,6 +875,13 @@ __global__ __launch_bounds__(256) void k_sq_contract_f32(ContractArgs a) {
           const int ai = c0 + aq + q;
           if (ai < b1) {
             phase_f32(ux[ai], lx[ai], kx0 + 8 * seg, 8, re[q], im[q]);
+            if (a.w) {  // channel weight on ex
+              const float wq = static_cast<float>(a.w[ai]);
+              for (int k = 0; k < 8; ++k) {
+                re[q][k] *= wq;
+                im[q][k] *= wq;
+              }
+            }
           } else {
             for (int k = 0; k < 8; ++k) re[q][k] = im[q][k] = 0.f;
           }
@@ -846,7 +946,7 @@ __global__ __launch_bounds__(256) void k_sq_contract_f32(ContractArgs a) {
         }
       }
     }
-    const size_t base = ((static_cast<size_t>(split) * a.nsp + s) * a.ntiles + tile) * (LX * LN);
+    const size_t base = ((static_cast<size_t>(split) * a.nch + s) * a.ntiles + tile) * (LX * LN);
 #pragma unroll
     for (int nb = 0; nb < 8; ++nb) {
 #pragma unroll
@@ -975,14 +1075,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   // Products: column n = (ky, kz), atoms 16*ph .. 16*ph+15.
   const int pn = t & (LN - 1), ph = t >> 7;
   const int pky = pn / LZ, pkz = pn % LZ;
-  for (int s = 0; s < a.nsp; ++s) {
+  for (int s = 0; s < a.nch; ++s) {
     const int a0 = a.sp_off[s], a1 = a.sp_off[s + 1];
     const int64_t len = a1 - a0;
     const int b0 = a0 + static_cast<int>(len * split / a.ksplit);
     const int b1 = a0 + static_cast<int>(len * (split + 1) / a.ksplit);
     if (b0 >= b1) {
-      // no atoms of this species in this split: zero partials
-      const size_t base = ((static_cast<size_t>(split) * a.nsp + s) * a.ntiles + tile) * (LX * LN);
+      // no atoms of this channel in this split: zero partials
+      const size_t base = ((static_cast<size_t>(split) * a.nch + s) * a.ntiles + tile) * (LX * LN);
       for (int v = t; v < LX * LN; v += 256) reinterpret_cast<double2*>(a.rho)[base + v] = make_double2(0.0, 0.0);
       continue;
     }
@@ -1004,7 +1104,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       xl = u_lo[o];
       xc = u_c[o];
       xs = u_s[o];
-      xm = (axis != 0 || ai < b1) ? 1.f : 0.f;
+      // ex carries the channel weight (1, or the charge of the entry); zero
+      // past the end of the range.
+      xm = axis != 0 ? 1.f : (ai < b1 ? (a.w ? static_cast<float>(a.w[o]) : 1.f) : 0.f);
     };
     __syncthreads();  // the previous species' readers are done with rec and B
     fetch(b0);
@@ -1107,7 +1209,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
     // D layout of the 16x16 FP32-output MFMAs: register r holds row
     // 4*(lane/16) + r, column lane%16.
-    const size_t base = ((static_cast<size_t>(split) * a.nsp + s) * a.ntiles + tile) * (LX * LN);
+    const size_t base = ((static_cast<size_t>(split) * a.nch + s) * a.ntiles + tile) * (LX * LN);
 #pragma unroll
     for (int nb = 0; nb < 8; ++nb) {
 #pragma unroll
@@ -1129,14 +1231,14 @@ struct Bsc {
 
 __global__ __launch_bounds__(256) void k_sq_shells(const double* __restrict__ rho,
                                                    const int16_t* __restrict__ tile_shell, int ntiles,
-                                                   int tile_vec, int ksplit, int nsp, Bsc bsc, int nqmax,
+                                                   int tile_vec, int ksplit, int nsp, int nch, Bsc bsc, int nqmax,
                                                    double* __restrict__ sqf, double* __restrict__ sqfp,
-                                                   int use_lds) {
+                                                   double* __restrict__ sqfq, int use_lds) {
   extern __shared__ __align__(16) unsigned char smem[];
-  double* acc = reinterpret_cast<double*>(smem);  // [(1+nsp)][nqmax+1]
+  double* acc = reinterpret_cast<double*>(smem);  // [(1+nch)][nqmax+1]: sqf, sqfp[nsp], sqfq
   const int ns = nqmax + 1;
   if (use_lds) {
-    for (int b = threadIdx.x; b < (1 + nsp) * ns; b += 256) acc[b] = 0.0;
+    for (int b = threadIdx.x; b < (1 + nch) * ns; b += 256) acc[b] = 0.0;
     __syncthreads();
   }
   const int64_t total = static_cast<int64_t>(ntiles) * tile_vec;
@@ -1146,14 +1248,21 @@ __global__ __launch_bounds__(256) void k_sq_shells(const double* __restrict__ rh
     const int sh = tile_shell[e];
     if (sh < 0) continue;
     double tr = 0.0, ti = 0.0;
-    for (int s = 0; s < nsp; ++s) {
+    for (int s = 0; s < nch; ++s) {
       double rr = 0.0, ri = 0.0;
       for (int sp = 0; sp < ksplit; ++sp) {
-        const double2 v = r2[(static_cast<int64_t>(sp) * nsp + s) * total + e];
+        const double2 v = r2[(static_cast<int64_t>(sp) * nch + s) * total + e];
         rr += v.x;
         ri += v.y;
       }
       const double p = rr * rr + ri * ri;
+      if (s == nsp) {  // the charge channel
+        if (use_lds)
+          atomicAdd(&acc[(1 + nsp) * ns + sh], p);
+        else
+          atomicAdd(&sqfq[sh], p);
+        continue;
+      }
       if (use_lds)
         atomicAdd(&acc[(1 + s) * ns + sh], p);
       else
@@ -1169,13 +1278,15 @@ __global__ __launch_bounds__(256) void k_sq_shells(const double* __restrict__ rh
   }
   if (use_lds) {
     __syncthreads();
-    for (int b = threadIdx.x; b < (1 + nsp) * ns; b += 256) {
+    for (int b = threadIdx.x; b < (1 + nch) * ns; b += 256) {
       const double v = acc[b];
       if (v != 0.0) {
         if (b < ns)
           atomicAdd(&sqf[b], v);
-        else
+        else if (b < (1 + nsp) * ns)
           atomicAdd(&sqfp[b - ns], v);
+        else
+          atomicAdd(&sqfq[b - (1 + nsp) * ns], v);
       }
     }
   }
@@ -1186,14 +1297,14 @@ __global__ __launch_bounds__(256) void k_sq_shells(const double* __restrict__ rh
 // k_sq_list_finish (deterministic), which then bins |rho|^2 into the shells.
 constexpr int kListBlocks = 64;
 
-__global__ __launch_bounds__(256) void k_sq_list(const double* __restrict__ u,
-                                                 const int32_t* __restrict__ sp_off, int64_t n, int nsp,
+__global__ __launch_bounds__(256) void k_sq_list(const double* __restrict__ u, const double* __restrict__ wt,
+                                                 const int32_t* __restrict__ sp_off, int64_t n, int nch,
                                                  int ndim, const int32_t* __restrict__ kv,
                                                  double* __restrict__ part) {
   __shared__ double red[2][256];
   const int v = blockIdx.x, b = blockIdx.y;
   const int kx = kv[3 * v], ky = kv[3 * v + 1], kz = kv[3 * v + 2];
-  for (int s = 0; s < nsp; ++s) {
+  for (int s = 0; s < nch; ++s) {
     const int64_t a0 = sp_off[s], a1 = sp_off[s + 1], len = a1 - a0;
     const int64_t j0 = a0 + len * b / kListBlocks, j1 = a0 + len * (b + 1) / kListBlocks;
     double pr = 0.0, pi = 0.0;
@@ -1203,8 +1314,9 @@ __global__ __launch_bounds__(256) void k_sq_list(const double* __restrict__ u,
       f -= rint(f);
       double sn, cn;
       sincospi(2.0 * f, &sn, &cn);
-      pr += cn;
-      pi += sn;
+      const double m = wt ? wt[j] : 1.0;  // channel weight
+      pr += m * cn;
+      pi += m * sn;
     }
     red[0][threadIdx.x] = pr;
     red[1][threadIdx.x] = pi;
@@ -1217,7 +1329,7 @@ __global__ __launch_bounds__(256) void k_sq_list(const double* __restrict__ u,
       __syncthreads();
     }
     if (threadIdx.x == 0) {
-      double* o = part + ((static_cast<int64_t>(v) * kListBlocks + b) * nsp + s) * 2;
+      double* o = part + ((static_cast<int64_t>(v) * kListBlocks + b) * nch + s) * 2;
       o[0] = red[0][0];
       o[1] = red[1][0];
     }
@@ -1226,18 +1338,22 @@ __global__ __launch_bounds__(256) void k_sq_list(const double* __restrict__ u,
 }
 
 __global__ void k_sq_list_finish(const double* __restrict__ part, const int32_t* __restrict__ shell, int nvec,
-                                 int nsp, Bsc bsc, int nqmax, double* __restrict__ sqf,
-                                 double* __restrict__ sqfp) {
+                                 int nsp, int nch, Bsc bsc, int nqmax, double* __restrict__ sqf,
+                                 double* __restrict__ sqfp, double* __restrict__ sqfq) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nvec) return;
   const int sh = shell[v], ns = nqmax + 1;
   double tr = 0.0, ti = 0.0;
-  for (int s = 0; s < nsp; ++s) {
+  for (int s = 0; s < nch; ++s) {
     double rr = 0.0, ri = 0.0;
     for (int b = 0; b < kListBlocks; ++b) {
-      const double* o = part + ((static_cast<int64_t>(v) * kListBlocks + b) * nsp + s) * 2;
+      const double* o = part + ((static_cast<int64_t>(v) * kListBlocks + b) * nch + s) * 2;
       rr += o[0];
       ri += o[1];
+    }
+    if (s == nsp) {  // the charge channel
+      atomicAdd(&sqfq[sh], rr * rr + ri * ri);
+      continue;
     }
     atomicAdd(&sqfp[static_cast<int64_t>(s) * ns + sh], rr * rr + ri * ri);
     tr += bsc.b[s] * rr;
@@ -1609,16 +1725,16 @@ void launch_exclusive_scan(const int32_t* in, int32_t* out, int64_t count, hipSt
   hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, s, in, out, count);
 }
 
-void launch_species_offsets(const int32_t* blkoff, const int32_t* blk, int nblk, int nsp, int64_t n,
+void launch_species_offsets(const int32_t* blkoff, const int32_t* blk, int nblk, int nsp, int64_t extra_end,
                             int32_t* sp_off, hipStream_t s) {
-  (void)n;
-  hipLaunchKernelGGL(k_species_offsets, dim3(1), dim3(64), 0, s, blkoff, blk, nblk, nsp, sp_off);
+  hipLaunchKernelGGL(k_species_offsets, dim3(1), dim3(64), 0, s, blkoff, blk, nblk, nsp, extra_end, sp_off);
 }
 
 void launch_species_scatter(const int32_t* itype, int64_t n, int nsp, const int32_t* blkoff, int nblk,
-                            const double* pos, const double lq[3], int ndim, double* u, hipStream_t s) {
+                            const double* pos, const double lq[3], int ndim, const double* q, int64_t stride,
+                            double* u, hipStream_t s) {
   hipLaunchKernelGGL(k_species_scatter, dim3(nblk), dim3(256), 0, s, itype, n, nsp, blkoff, nblk, pos,
-                     lq[0], lq[1], lq[2], ndim, u);
+                     lq[0], lq[1], lq[2], ndim, q, stride, u);
 }
 
 void launch_cell_count(const double* pos, int64_t n, int ndim, const CellGrid& g, int32_t* cellid,
@@ -1628,20 +1744,22 @@ void launch_cell_count(const double* pos, int64_t n, int ndim, const CellGrid& g
 }
 
 void launch_cell_scatter(const double* pos, const int32_t* itype, const int32_t* cellid, int64_t n,
-                         const int32_t* start, int32_t* cursor, double* spos, int32_t* stype,
-                         hipStream_t s) {
+                         const int32_t* start, int32_t* cursor, double* spos, int32_t* stype, const double* q,
+                         double* sq, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_cell_scatter, dim3(grid1d(n, 256)), dim3(256), 0, s, pos, itype, cellid, n, start,
-                     cursor, spos, stype);
+                     cursor, spos, stype, q, sq);
 }
 
-size_t pair_lds_bytes(int nit, int lsmax) {
-  return ((static_cast<size_t>(nit) * lsmax * 4 + 15) & ~size_t(15)) + static_cast<size_t>(lsmax + 3) * 8;
+size_t pair_lds_bytes(int nit, int lsmax, bool qq) {
+  const size_t b = ((static_cast<size_t>(nit) * lsmax * 4 + 15) & ~size_t(15)) + static_cast<size_t>(lsmax + 3) * 8;
+  return qq ? ((b + 15) & ~size_t(15)) + static_cast<size_t>(lsmax) * 8 : b;
 }
 
-size_t cells_lds_bytes(int nit, int lsmax) {
+size_t cells_lds_bytes(int nit, int lsmax, bool qq) {
   const size_t base = (pair_lds_bytes(nit, lsmax) + 15) & ~size_t(15);
-  return base + static_cast<size_t>(4) * kHitQ * (sizeof(double) + sizeof(int));
+  const size_t rings = static_cast<size_t>(4) * kHitQ * (sizeof(double) + sizeof(int));
+  return base + rings + (qq ? static_cast<size_t>(4) * kHitQ * sizeof(double) + static_cast<size_t>(lsmax) * 8 : 0);
 }
 
 void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start, const double* spos,
@@ -1667,29 +1785,37 @@ void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start
   const int64_t ntasks = ncell * waves_per_cell;
   bool shift = true;
   for (int k = 0; k < a.ndim; ++k) shift = shift && a.cellshift[k];
-  const size_t lds = cells_lds_bytes(a.nit, a.lsmax);
-  if (a.ndim == 3) {
-    if (shift)
-      hipLaunchKernelGGL((k_rdf_cells<true, true>), dim3(nblocks), dim3(256), lds, s, a, g, st, start, spos, stype,
-                         n, waves_per_cell, ntasks, partial);
-    else
-      hipLaunchKernelGGL((k_rdf_cells<false, true>), dim3(nblocks), dim3(256), lds, s, a, g, st, start, spos,
-                         stype, n, waves_per_cell, ntasks, partial);
+  const bool qq = a.q != nullptr;
+  const size_t lds = cells_lds_bytes(a.nit, a.lsmax, qq);
+#define TRJ_CELLS(S, D, Q)                                                                                     \
+  hipLaunchKernelGGL((k_rdf_cells<S, D, Q>), dim3(nblocks), dim3(256), lds, s, a, g, st, start, spos, stype, n, \
+                     waves_per_cell, ntasks, partial)
+  if (qq) {
+    if (a.ndim == 3) {
+      if (shift) TRJ_CELLS(true, true, true); else TRJ_CELLS(false, true, true);
+    } else {
+      if (shift) TRJ_CELLS(true, false, true); else TRJ_CELLS(false, false, true);
+    }
   } else {
-    if (shift)
-      hipLaunchKernelGGL((k_rdf_cells<true, false>), dim3(nblocks), dim3(256), lds, s, a, g, st, start, spos,
-                         stype, n, waves_per_cell, ntasks, partial);
-    else
-      hipLaunchKernelGGL((k_rdf_cells<false, false>), dim3(nblocks), dim3(256), lds, s, a, g, st, start, spos,
-                         stype, n, waves_per_cell, ntasks, partial);
+    if (a.ndim == 3) {
+      if (shift) TRJ_CELLS(true, true, false); else TRJ_CELLS(false, true, false);
+    } else {
+      if (shift) TRJ_CELLS(true, false, false); else TRJ_CELLS(false, false, false);
+    }
   }
+#undef TRJ_CELLS
 }
 
 void launch_rdf_allpairs(const PairArgs& a, const double* pos, const int32_t* itype, int64_t n,
                          int nblocks, uint32_t* partial, hipStream_t s) {
   const int nb = static_cast<int>((n + kWave - 1) / kWave);
-  hipLaunchKernelGGL(k_rdf_allpairs, dim3(nblocks), dim3(256), pair_lds_bytes(a.nit, a.lsmax), s, a, pos,
-                     itype, n, nb, partial);
+  hipLaunchKernelGGL(k_rdf_allpairs, dim3(nblocks), dim3(256), pair_lds_bytes(a.nit, a.lsmax, a.q != nullptr), s,
+                     a, pos, itype, n, nb, partial);
+}
+
+void launch_qq_finish(unsigned long long* gq, int lsmax, double inv_scale, double avq2, double* acc,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_qq_finish, dim3(grid1d(lsmax, 256)), dim3(256), 0, s, gq, lsmax, inv_scale, avq2, acc);
 }
 
 void launch_rdf_merge(const uint32_t* partial, int nslices, int nbins, int64_t* counts, hipStream_t s) {
@@ -1714,32 +1840,34 @@ void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s) 
 }
 
 void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, int tile_vec, int ksplit,
-                      int nsp, const double* bsc, int nqmax, double* sqf, double* sqfp, hipStream_t s) {
+                      int nsp, int nch, const double* bsc, int nqmax, double* sqf, double* sqfp, double* sqfq,
+                      hipStream_t s) {
   if (ntiles <= 0) return;
   Bsc b{};
   for (int i = 0; i < nsp; ++i) b.b[i] = bsc[i];
-  const size_t lds = static_cast<size_t>(1 + nsp) * (nqmax + 1) * sizeof(double);
+  const size_t lds = static_cast<size_t>(1 + nch) * (nqmax + 1) * sizeof(double);
   const int use_lds = lds <= 48 * 1024;
   const int64_t total = static_cast<int64_t>(ntiles) * tile_vec;
   int64_t g = (total + 255) / 256;
   if (g > 1024) g = 1024;
   hipLaunchKernelGGL(k_sq_shells, dim3(static_cast<unsigned>(g)), dim3(256), use_lds ? lds : 0, s, rho,
-                     tile_shell, ntiles, tile_vec, ksplit, nsp, b, nqmax, sqf, sqfp, use_lds);
+                     tile_shell, ntiles, tile_vec, ksplit, nsp, nch, b, nqmax, sqf, sqfp, sqfq, use_lds);
 }
 
-size_t sq_list_scratch_bytes(int nvec, int nsp) {
-  return sizeof(double) * 2 * static_cast<size_t>(nvec) * kListBlocks * nsp;
+size_t sq_list_scratch_bytes(int nvec, int nch) {
+  return sizeof(double) * 2 * static_cast<size_t>(nvec) * kListBlocks * nch;
 }
 
-void launch_sq_list(const double* u, const int32_t* sp_off, int64_t n, int nsp, int ndim, const int32_t* kv,
-                    const int32_t* shell, int nvec, const double* bsc, int nqmax, double* part, double* sqf,
-                    double* sqfp, hipStream_t s) {
+void launch_sq_list(const double* u, const double* w, const int32_t* sp_off, int64_t stride, int nsp, int nch,
+                    int ndim, const int32_t* kv, const int32_t* shell, int nvec, const double* bsc, int nqmax,
+                    double* part, double* sqf, double* sqfp, double* sqfq, hipStream_t s) {
   if (nvec <= 0) return;
   Bsc b{};
   for (int i = 0; i < nsp; ++i) b.b[i] = bsc[i];
-  hipLaunchKernelGGL(k_sq_list, dim3(nvec, kListBlocks), dim3(256), 0, s, u, sp_off, n, nsp, ndim, kv, part);
-  hipLaunchKernelGGL(k_sq_list_finish, dim3((nvec + 63) / 64), dim3(64), 0, s, part, shell, nvec, nsp, b, nqmax,
-                     sqf, sqfp);
+  hipLaunchKernelGGL(k_sq_list, dim3(nvec, kListBlocks), dim3(256), 0, s, u, w, sp_off, stride, nch, ndim, kv,
+                     part);
+  hipLaunchKernelGGL(k_sq_list_finish, dim3((nvec + 63) / 64), dim3(64), 0, s, part, shell, nvec, nsp, nch, b,
+                     nqmax, sqf, sqfp, sqfq);
 }
 
 int64_t launch_lds_atomics(int nblocks, int iters, uint32_t* out, hipStream_t s) {

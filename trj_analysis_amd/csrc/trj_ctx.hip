@@ -48,6 +48,11 @@ struct trj_ctx {
   // device buffers
   DevBuf raw, itype, pos, spos, stype, cellid, cellcount, cellstart, cursor, partial, counts, thr, blk,
       blkoff, sp_off, u, rho, tiles, tile_shell, sqf, sqfp, axis_kv, axis_shell, listpart, flags;
+  // Charges (TRJ_RDF_CHARGE / TRJ_SQ_CHARGE): the frame's charges, their
+  // cell-sorted copy, the frame's fixed-point g_qq sums, the g_qq and S_qq
+  // accumulators.
+  DevBuf qraw, qsort, gqf, gqacc, sqfq;
+  bool rdf_charge = false, sq_charge = false;
   // pinned staging for host inputs (double-buffered)
   void* pinned[2] = {nullptr, nullptr};
   size_t pinned_cap[2] = {0, 0};
@@ -173,22 +178,29 @@ int validate_box(trj_ctx* c, int ndim, const double box[3]) {
 }
 
 // The frame pipeline, device inputs.
+// q_dev: the frame's charges on the device (NULL without); qscale: the
+// frame's fixed-point scale of q_i q_j; avq2: sum(q^2)/n.
 int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int32_t* itype_dev, int64_t n,
-              const double box_in[3], uint32_t what) {
+              const double box_in[3], uint32_t what, const double* q_dev = nullptr, double qscale = 1.0,
+              double avq2 = 0.0) {
   const bool do_rdf = (what & TRJ_WANT_RDF) != 0;
   const bool do_sq = (what & TRJ_WANT_SQ) != 0;
   if (do_rdf && !c->rdf_ready) return fail(c, TRJ_ESTATE, "frame asks for RDF before trj_rdf_init");
   if (do_sq && !c->sq_ready) return fail(c, TRJ_ESTATE, "frame asks for S(Q) before trj_sq_init");
   if (!do_rdf && !do_sq) return fail(c, TRJ_EINVAL, "frame asks for nothing");
+  const bool rdf_q = do_rdf && c->rdf_charge, sq_q = do_sq && c->sq_charge;
+  if ((rdf_q || sq_q) && !q_dev)
+    return fail(c, TRJ_EINVAL, "charges are enabled (TRJ_RDF_CHARGE / TRJ_SQ_CHARGE): frames go through trj_frame_q");
   const int ndim = do_rdf ? c->rdf.ndim : c->sq.p.ndim;
   const int nsp = do_rdf ? c->rdf.nsp : c->sq.p.nsp;
   if (do_rdf && do_sq && (c->rdf.ndim != c->sq.p.ndim || c->rdf.nsp != c->sq.p.nsp))
     return fail(c, TRJ_EINVAL, "RDF and S(Q) were initialised with different ndim/nsp");
   if (n < 1) return fail(c, TRJ_EINVAL, "n must be positive");
   if (n > (int64_t(1) << 31) - 1024) return fail(c, TRJ_ELIMIT, "n above 2^31 is not supported");
-  // The S(Q) kernels address the turn planes with 32-bit offsets up to 3 n.
-  if (do_sq && n > int64_t(0xFFFFFFFFu) / 3)
-    return fail(c, TRJ_ELIMIT, "S(Q) with n above 1431655765 is not supported");
+  // The S(Q) kernels address the turn planes with 32-bit offsets up to 3 x the
+  // plane stride (n, or 2n with the charge channel).
+  if (do_sq && n * (sq_q ? 2 : 1) > int64_t(0xFFFFFFFFu) / 3)
+    return fail(c, TRJ_ELIMIT, "S(Q) with n above 1431655765 (715827882 with charges) is not supported");
   if (dtype != TRJ_F32 && dtype != TRJ_F64) return fail(c, TRJ_EINVAL, "bad dtype");
   if (layout != TRJ_AOS && layout != TRJ_SOA) return fail(c, TRJ_EINVAL, "bad layout");
   if (int r = validate_box(c, ndim, box_in)) return r;
@@ -226,18 +238,21 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
   const int nblk = static_cast<int>((n + 255) / 256);
   if ((r = ensure(c, c->blk, sizeof(int32_t) * nsp * nblk))) return r;
   if ((r = ensure(c, c->blkoff, sizeof(int32_t) * nsp * nblk))) return r;
-  if ((r = ensure(c, c->sp_off, sizeof(int32_t) * (nsp + 1)))) return r;
+  if ((r = ensure(c, c->sp_off, sizeof(int32_t) * (nsp + 2)))) return r;
+  // Plane stride of u: n, or 2n with the charge channel (all atoms again, in
+  // input order, weighted by their charges).
+  const int64_t ustride = sq_q ? 2 * n : n;
   launch_species_count(itype_dev, n, nsp, P<int32_t>(c->blk), nblk, c->stream);
   launch_exclusive_scan(P<int32_t>(c->blk), P<int32_t>(c->blkoff), static_cast<int64_t>(nsp) * nblk, c->stream);
-  launch_species_offsets(P<int32_t>(c->blkoff), P<int32_t>(c->blk), nblk, nsp, n, P<int32_t>(c->sp_off),
-                         c->stream);
+  launch_species_offsets(P<int32_t>(c->blkoff), P<int32_t>(c->blk), nblk, nsp, sq_q ? 2 * n : 0,
+                         P<int32_t>(c->sp_off), c->stream);
   if (do_sq) {
     // Species-contiguous turns u = x / Lq for the contraction (Lq: the box the
     // q-grid was built on, sq.cuf:189).
-    if ((r = ensure(c, c->u, sizeof(double) * 12 * n))) return r;
+    if ((r = ensure(c, c->u, sizeof(double) * (sq_q ? 13 : 12) * ustride))) return r;
     const double lq[3] = {c->sq.box[0], c->sq.box[1], c->sq.box[2]};
     launch_species_scatter(itype_dev, n, nsp, P<int32_t>(c->blkoff), nblk, P<double>(c->pos), lq, ndim,
-                           P<double>(c->u), c->stream);
+                           sq_q ? q_dev : nullptr, ustride, P<double>(c->u), c->stream);
   }
   HIPC(c, hipGetLastError());
   HIPC(c, hipEventRecord(es.ev[1], c->stream));
@@ -265,6 +280,10 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     pa.side2 = g.side2;
     pa.inv_dr = static_cast<float>(1.0 / R.deltar);
     pa.thr = P<double>(c->thr);
+    if (rdf_q) {
+      pa.gq = P<unsigned long long>(c->gqf);
+      pa.qscale = qscale;
+    }
     const int nbins = R.nit * R.lsmax;
     // Cell grid: side >= cutoff on every axis.
     const double rc = std::sqrt(g.side2);
@@ -300,8 +319,11 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
       launch_cell_count(P<double>(c->pos), n, ndim, cg, P<int32_t>(c->cellid), P<int32_t>(c->cellcount),
                         rs);
       launch_exclusive_scan(P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), ncell + 1, rs);
+      if (rdf_q && (r = ensure(c, c->qsort, sizeof(double) * n))) return r;
       launch_cell_scatter(P<double>(c->pos), itype_dev, P<int32_t>(c->cellid), n, P<int32_t>(c->cellstart),
-                          P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype), rs);
+                          P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype),
+                          rdf_q ? q_dev : nullptr, rdf_q ? P<double>(c->qsort) : nullptr, rs);
+      if (rdf_q) pa.q = P<double>(c->qsort);
       const double mean = static_cast<double>(n) / static_cast<double>(ncell);
       int W = static_cast<int>(std::lround(mean / 64.0));
       W = std::max(1, std::min(W, 8));
@@ -314,9 +336,11 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
       const int64_t nb = (n + 63) / 64;
       nblocks = static_cast<int>(std::min<int64_t>((nb * nb + 3) / 4, 1024));
       if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
+      if (rdf_q) pa.q = q_dev;  // input order, like pos and itype here
       launch_rdf_allpairs(pa, P<double>(c->pos), itype_dev, n, nblocks, P<uint32_t>(c->partial), rs);
     }
     launch_rdf_merge(P<uint32_t>(c->partial), nblocks, nbins, P<int64_t>(c->counts), rs);
+    if (rdf_q) launch_qq_finish(P<unsigned long long>(c->gqf), R.lsmax, 1.0 / qscale, avq2, P<double>(c->gqacc), rs);
     HIPC(c, hipGetLastError());
     c->rdf_frames += 1;
     c->last_volume = g.volume;
@@ -333,19 +357,22 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     SqPlan& S = c->sq;
     const int ntiles = static_cast<int>(S.tiles.size());
     if (ntiles > 0) {
-      const int ks = contraction_splits(ntiles, n, S.p.precision, nsp);
+      const int nch = nsp + (sq_q ? 1 : 0);
+      const int ks = contraction_splits(ntiles, n, S.p.precision, nch);
       c->ksplit = ks;
-      const size_t rho_bytes = sizeof(double) * 2 * static_cast<size_t>(ks) * nsp * ntiles * S.tk.vec();
+      const size_t rho_bytes = sizeof(double) * 2 * static_cast<size_t>(ks) * nch * ntiles * S.tk.vec();
       if ((r = ensure(c, c->rho, rho_bytes))) return r;
       ContractArgs ca{};
       ca.u = P<double>(c->u);
+      ca.w = sq_q ? P<double>(c->u) + 12 * ustride : nullptr;
       ca.sp_off = P<int32_t>(c->sp_off);
       ca.tiles = P<int32_t>(c->tiles);
       ca.ntiles = ntiles;
       ca.ksplit = ks;
       ca.nsp = nsp;
+      ca.nch = nch;
       ca.ndim = ndim;
-      ca.n = n;
+      ca.n = ustride;
       ca.rho = P<double>(c->rho);
       if (S.p.precision == TRJ_PREC_FP64)
         launch_sq_contract_f64(ca, c->stream);
@@ -354,15 +381,15 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     }
     HIPC(c, hipGetLastError());
     HIPC(c, hipEventRecord(es.ev[3], c->stream));
+    const int nch = nsp + (sq_q ? 1 : 0);
     if (ntiles > 0)
-      launch_sq_shells(P<double>(c->rho), P<int16_t>(c->tile_shell), ntiles, S.tk.vec(), c->ksplit, nsp, S.p.bsc,
-                       S.nqmax,
-                       P<double>(c->sqf), P<double>(c->sqfp), c->stream);
+      launch_sq_shells(P<double>(c->rho), P<int16_t>(c->tile_shell), ntiles, S.tk.vec(), c->ksplit, nsp, nch,
+                       S.p.bsc, S.nqmax, P<double>(c->sqf), P<double>(c->sqfp), P<double>(c->sqfq), c->stream);
     const int nax = static_cast<int>(S.axis_shell.size());
-    if ((r = ensure(c, c->listpart, sq_list_scratch_bytes(nax, nsp)))) return r;
-    launch_sq_list(P<double>(c->u), P<int32_t>(c->sp_off), n, nsp, ndim, P<int32_t>(c->axis_kv),
-                   P<int32_t>(c->axis_shell), nax, S.p.bsc, S.nqmax, P<double>(c->listpart), P<double>(c->sqf),
-                   P<double>(c->sqfp), c->stream);
+    if ((r = ensure(c, c->listpart, sq_list_scratch_bytes(nax, nch)))) return r;
+    launch_sq_list(P<double>(c->u), sq_q ? P<double>(c->u) + 12 * ustride : nullptr, P<int32_t>(c->sp_off), ustride,
+                   nsp, nch, ndim, P<int32_t>(c->axis_kv), P<int32_t>(c->axis_shell), nax, S.p.bsc, S.nqmax,
+                   P<double>(c->listpart), P<double>(c->sqf), P<double>(c->sqfp), P<double>(c->sqfq), c->stream);
     HIPC(c, hipGetLastError());
     c->sq_frames += 1;
   } else {
@@ -462,7 +489,7 @@ void trj_destroy(trj_ctx* c) {
                     &c->cellcount, &c->cellstart, &c->cursor, &c->partial, &c->counts, &c->thr,
                     &c->blk,  &c->blkoff,    &c->sp_off,    &c->u,      &c->rho,    &c->tiles,
                     &c->tile_shell, &c->sqf, &c->sqfp,      &c->axis_kv, &c->axis_shell, &c->listpart,
-                    &c->flags};
+                    &c->flags, &c->qraw, &c->qsort, &c->gqf, &c->gqacc, &c->sqfq};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   for (int k = 0; k < 2; ++k) {
@@ -509,9 +536,11 @@ int trj_rdf_init(trj_ctx* c, const trj_rdf_params* p, const double box[3], int32
   for (int k = 0; k < 3; ++k) R.periodic[k] = (k < p->ndim) ? (p->periodic[k] != 0) : 1;
   R.lsmax = rdf_lsmax(p->ndim, box, p->rcrdf, p->deltar, &R.rc);
   if (R.lsmax < 1) return fail(c, TRJ_EINVAL, "rdf: lsmax < 1");
-  if (cells_lds_bytes(R.nit, R.lsmax) > kPairLdsLimit)
+  if (p->flags & ~TRJ_RDF_CHARGE) return fail(c, TRJ_EINVAL, "rdf: unknown flags");
+  const bool qq = (p->flags & TRJ_RDF_CHARGE) != 0;
+  if (cells_lds_bytes(R.nit, R.lsmax, qq) > kPairLdsLimit)
     return fail(c, TRJ_ELIMIT,
-                "rdf: g(r) grid needs " + std::to_string(cells_lds_bytes(R.nit, R.lsmax)) +
+                "rdf: g(r) grid needs " + std::to_string(cells_lds_bytes(R.nit, R.lsmax, qq)) +
                     " bytes of LDS, above 65536; increase deltar or reduce rcrdf (rdf.cuf:61-64)");
   std::vector<double> thr = rdf_bin_thresholds(R.deltar, R.lsmax);
   thr.push_back(INFINITY);  // thr[lsmax+2]
@@ -521,8 +550,15 @@ int trj_rdf_init(trj_ctx* c, const trj_rdf_params* p, const double box[3], int32
   if ((r = ensure(c, c->counts, sizeof(int64_t) * R.nit * R.lsmax))) return r;
   HIPC(c, hipMemcpyAsync(c->thr.p, thr.data(), sizeof(double) * thr.size(), hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipMemsetAsync(c->counts.p, 0, sizeof(int64_t) * R.nit * R.lsmax, c->stream));
+  if (qq) {
+    if ((r = ensure(c, c->gqf, sizeof(unsigned long long) * R.lsmax))) return r;
+    if ((r = ensure(c, c->gqacc, sizeof(double) * R.lsmax))) return r;
+    HIPC(c, hipMemsetAsync(c->gqf.p, 0, sizeof(unsigned long long) * R.lsmax, c->stream));
+    HIPC(c, hipMemsetAsync(c->gqacc.p, 0, sizeof(double) * R.lsmax, c->stream));
+  }
   HIPC(c, hipStreamSynchronize(c->stream));
   c->rdf = R;
+  c->rdf_charge = qq;
   c->rdf_ready = true;
   c->rdf_frames = 0;
   if (lsmax_out) *lsmax_out = R.lsmax;
@@ -548,6 +584,11 @@ int trj_sq_init(trj_ctx* c, const trj_sq_params* p, const double box[3], trj_sq_
   if ((r = ensure(c, c->axis_shell, sizeof(int32_t) * std::max<size_t>(1, S.axis_shell.size())))) return r;
   HIPC(c, hipMemsetAsync(c->sqf.p, 0, sizeof(double) * ns, c->stream));
   HIPC(c, hipMemsetAsync(c->sqfp.p, 0, sizeof(double) * ns * S.p.nsp, c->stream));
+  const bool sqq = (p->flags & TRJ_SQ_CHARGE) != 0;
+  if (sqq) {
+    if ((r = ensure(c, c->sqfq, sizeof(double) * ns))) return r;
+    HIPC(c, hipMemsetAsync(c->sqfq.p, 0, sizeof(double) * ns, c->stream));
+  }
   if (!S.tiles.empty()) {
     HIPC(c, hipMemcpyAsync(c->tiles.p, S.tiles.data(), sizeof(SqTile) * S.tiles.size(), hipMemcpyHostToDevice,
                            c->stream));
@@ -565,6 +606,7 @@ int trj_sq_init(trj_ctx* c, const trj_sq_params* p, const double box[3], trj_sq_
   for (size_t t = 0; t < S.tiles.size(); ++t)
     c->tile_index[pack_tile(S.tiles[t].kx0, S.tiles[t].ky0, S.tiles[t].kz0)] = static_cast<int>(t);
   c->sq = std::move(S);
+  c->sq_charge = sqq;
   c->sq_ready = true;
   c->sq_frames = 0;
   if (info) *info = c->sq.info;
@@ -579,8 +621,14 @@ int trj_frame_device(trj_ctx* c, const void* pos_dev, int32_t dtype, int32_t lay
   return run_frame(c, pos_dev, dtype, layout, itype_dev, n, box, what);
 }
 
-int trj_frame(trj_ctx* c, const void* pos, int32_t dtype, int32_t layout, const int32_t* itype, int64_t n,
-              const double box[3], uint32_t what) {
+}  // extern "C"
+
+namespace {
+
+// Host-input frame: stage positions, types and (optional) charges through
+// pinned memory, then run the frame from the device copies.
+int frame_host(trj_ctx* c, const void* pos, int32_t dtype, int32_t layout, const int32_t* itype,
+               const double* charge, int64_t n, const double box[3], uint32_t what) {
   if (!c) return TRJ_EINVAL;
   if (!pos || !itype) return fail(c, TRJ_EINVAL, "NULL input");
   if (n < 1) return fail(c, TRJ_EINVAL, "n must be positive");
@@ -589,20 +637,40 @@ int trj_frame(trj_ctx* c, const void* pos, int32_t dtype, int32_t layout, const 
   const int ndim = (what & TRJ_WANT_RDF) && c->rdf_ready ? c->rdf.ndim : (c->sq_ready ? c->sq.p.ndim : 3);
   const size_t pbytes = static_cast<size_t>(n) * ndim * (dtype == TRJ_F32 ? 4 : 8);
   const size_t tbytes = static_cast<size_t>(n) * sizeof(int32_t);
+  const size_t qbytes = charge ? static_cast<size_t>(n) * sizeof(double) : 0;
   const size_t pbytes_al = (pbytes + 255) & ~size_t(255);
+  const size_t tbytes_al = (tbytes + 255) & ~size_t(255);
+  const size_t total = pbytes_al + tbytes_al + qbytes;
   int r;
   if ((r = ensure(c, c->raw, pbytes))) return r;
   if ((r = ensure(c, c->itype, tbytes))) return r;
+  if (charge && (r = ensure(c, c->qraw, qbytes))) return r;
+  // Fixed-point scale of q_i q_j (2^(30-e), max q^2 < 2^e) and avq2 =
+  // sum(q^2)/n, both from the host copy in index order.
+  double qscale = 1.0, avq2 = 0.0;
+  if (charge) {
+    double m = 0.0, s2 = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      const double v = charge[i] * charge[i];
+      if (!std::isfinite(v)) return fail(c, TRJ_EINVAL, "non-finite charge");
+      if (v > m) m = v;
+      s2 += v;
+    }
+    int e = 0;
+    if (m > 0.0) std::frexp(m, &e);
+    qscale = std::ldexp(1.0, 30 - e);
+    avq2 = s2 / static_cast<double>(n);
+  }
   // Stage through pinned memory; wait until the previous copy out of this
   // staging buffer has completed.
   const int k = c->pinned_next;
   c->pinned_next ^= 1;
   HIPC(c, hipEventSynchronize(c->pinned_done[k]));
-  if (c->pinned_cap[k] < pbytes_al + tbytes) {
+  if (c->pinned_cap[k] < total) {
     if (c->pinned[k]) hipHostFree(c->pinned[k]);
     c->pinned[k] = nullptr;
     c->pinned_cap[k] = 0;
-    size_t want = (pbytes_al + tbytes) + (pbytes_al + tbytes) / 8;
+    size_t want = total + total / 8;
     if (hipHostMalloc(&c->pinned[k], want, hipHostMallocDefault) != hipSuccess)
       return fail(c, TRJ_ENOMEM, "pinned staging allocation failed");
     c->pinned_cap[k] = want;
@@ -610,10 +678,30 @@ int trj_frame(trj_ctx* c, const void* pos, int32_t dtype, int32_t layout, const 
   char* st = static_cast<char*>(c->pinned[k]);
   std::memcpy(st, pos, pbytes);
   std::memcpy(st + pbytes_al, itype, tbytes);
+  if (charge) std::memcpy(st + pbytes_al + tbytes_al, charge, qbytes);
   HIPC(c, hipMemcpyAsync(c->raw.p, st, pbytes, hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipMemcpyAsync(c->itype.p, st + pbytes_al, tbytes, hipMemcpyHostToDevice, c->stream));
+  if (charge)
+    HIPC(c, hipMemcpyAsync(c->qraw.p, st + pbytes_al + tbytes_al, qbytes, hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipEventRecord(c->pinned_done[k], c->stream));
-  return run_frame(c, c->raw.p, dtype, layout, P<int32_t>(c->itype), n, box, what);
+  return run_frame(c, c->raw.p, dtype, layout, P<int32_t>(c->itype), n, box, what,
+                   charge ? P<double>(c->qraw) : nullptr, qscale, avq2);
+}
+
+}  // namespace
+
+extern "C" {
+
+int trj_frame(trj_ctx* c, const void* pos, int32_t dtype, int32_t layout, const int32_t* itype, int64_t n,
+              const double box[3], uint32_t what) {
+  return frame_host(c, pos, dtype, layout, itype, nullptr, n, box, what);
+}
+
+int trj_frame_q(trj_ctx* c, const void* pos, int32_t dtype, int32_t layout, const int32_t* itype,
+                const double* charge, int64_t n, const double box[3], uint32_t what) {
+  if (!c) return TRJ_EINVAL;
+  if (!charge) return fail(c, TRJ_EINVAL, "NULL charges");
+  return frame_host(c, pos, dtype, layout, itype, charge, n, box, what);
 }
 
 int trj_sync(trj_ctx* c) {
@@ -661,6 +749,36 @@ int trj_rdf_gmix(trj_ctx* c, int64_t nconf, double* out) {
                      (dv * static_cast<double>(ntype[l - 1]) * static_cast<double>(ntype[j - 1]) *
                       static_cast<double>(nconf));
       }
+  }
+  return TRJ_OK;
+}
+
+int trj_rdf_qq(trj_ctx* c, double* acc) {
+  if (!c || !acc) return TRJ_EINVAL;
+  if (!c->rdf_ready || !c->rdf_charge) return fail(c, TRJ_ESTATE, "RDF not initialised with TRJ_RDF_CHARGE");
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  HIPC(c, hipMemcpy(acc, c->gqacc.p, sizeof(double) * c->rdf.lsmax, hipMemcpyDeviceToHost));
+  return TRJ_OK;
+}
+
+int trj_rdf_gqq(trj_ctx* c, int64_t nconf, double* out) {
+  if (!c || !out) return TRJ_EINVAL;
+  if (!c->rdf_ready || !c->rdf_charge) return fail(c, TRJ_ESTATE, "RDF not initialised with TRJ_RDF_CHARGE");
+  if (c->rdf_frames == 0) return fail(c, TRJ_ESTATE, "no RDF frame processed");
+  const RdfPlan& R = c->rdf;
+  std::vector<double> acc(R.lsmax);
+  int r = trj_rdf_qq(c, acc.data());
+  if (r) return r;
+  if (nconf <= 0) nconf = c->rdf_frames;
+  // log.f90:501-503: gchch = 2 V gqq(i) / (N^2 dV Nconf)
+  const double pi = 3.14159265358979323846;
+  const double N = static_cast<double>(c->last_n);
+  for (int i = 1; i <= R.lsmax - 2; ++i) {
+    const double ri = i * R.deltar, a = ri + R.deltar / 2, b = ri - R.deltar / 2;
+    const double dv = (R.ndim == 3) ? 4.0 * pi * (a * a * a - b * b * b) / 3.0 : pi * (a * a - b * b);
+    out[i - 1] = 2.0 * c->last_volume * acc[i - 1] / (N * N * dv * static_cast<double>(nconf));
   }
   return TRJ_OK;
 }
@@ -727,6 +845,34 @@ int trj_sq_shells(trj_ctx* c, double* sqf, double* sqfp, int64_t* nq) {
   if (sqfp) HIPC(c, hipMemcpy(sqfp, c->sqfp.p, sizeof(double) * ns * c->sq.p.nsp, hipMemcpyDeviceToHost));
   if (nq)
     for (int s = 0; s < ns; ++s) nq[s] = c->sq.nq[s];
+  return TRJ_OK;
+}
+
+int trj_sq_qq(trj_ctx* c, double* sqfq) {
+  if (!c || !sqfq) return TRJ_EINVAL;
+  if (!c->sq_ready || !c->sq_charge) return fail(c, TRJ_ESTATE, "S(Q) not initialised with TRJ_SQ_CHARGE");
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  HIPC(c, hipMemcpy(sqfq, c->sqfq.p, sizeof(double) * (c->sq.nqmax + 1), hipMemcpyDeviceToHost));
+  return TRJ_OK;
+}
+
+int trj_sq_qq_column(trj_ctx* c, int64_t nconf, double* out) {
+  if (!c || !out) return TRJ_EINVAL;
+  if (!c->sq_ready || !c->sq_charge) return fail(c, TRJ_ESTATE, "S(Q) not initialised with TRJ_SQ_CHARGE");
+  if (c->sq_frames == 0) return fail(c, TRJ_ESTATE, "no S(Q) frame processed");
+  const SqPlan& S = c->sq;
+  std::vector<double> q(S.nqmax + 1);
+  int r = trj_sq_qq(c, q.data());
+  if (r) return r;
+  if (nconf <= 0) nconf = c->sq_frames;
+  // log.f90:371-376: sqfq(i) / (Nsites Nconf nq(i))
+  const double N = static_cast<double>(c->last_n);
+  for (int i = 1; i <= S.nqmax; ++i) {
+    const double cn = static_cast<double>(nconf) * static_cast<double>(S.nq[i]);
+    out[i - 1] = cn > 0.0 ? q[i] / (N * cn) : 0.0;
+  }
   return TRJ_OK;
 }
 
@@ -828,10 +974,15 @@ int trj_reset(trj_ctx* c) {
   if (r) return r;
   if (c->rdf_ready)
     HIPC(c, hipMemsetAsync(c->counts.p, 0, sizeof(int64_t) * c->rdf.nit * c->rdf.lsmax, c->stream));
+  if (c->rdf_ready && c->rdf_charge) {
+    HIPC(c, hipMemsetAsync(c->gqf.p, 0, sizeof(unsigned long long) * c->rdf.lsmax, c->stream));
+    HIPC(c, hipMemsetAsync(c->gqacc.p, 0, sizeof(double) * c->rdf.lsmax, c->stream));
+  }
   if (c->sq_ready) {
     const int ns = c->sq.nqmax + 1;
     HIPC(c, hipMemsetAsync(c->sqf.p, 0, sizeof(double) * ns, c->stream));
     HIPC(c, hipMemsetAsync(c->sqfp.p, 0, sizeof(double) * ns * c->sq.p.nsp, c->stream));
+    if (c->sq_charge) HIPC(c, hipMemsetAsync(c->sqfq.p, 0, sizeof(double) * ns, c->stream));
   }
   HIPC(c, hipStreamSynchronize(c->stream));
   c->rdf_frames = 0;
@@ -850,12 +1001,12 @@ struct StateHeader {
   int32_t version, has_rdf, has_sq, nsp;
   int32_t rdf_ndim, nit, lsmax, rdf_periodic[3];
   double deltar, rc;
-  int32_t sq_ndim, nqmax, qset, pad0;
+  int32_t sq_ndim, nqmax, qset, sq_charge;
   double dq, qbox[3];
   int64_t nvec_grid, nvec_axis;
   int64_t rdf_frames, sq_frames, last_n;
   double last_volume;
-  int32_t last_nsp, pad1;
+  int32_t last_nsp, rdf_charge;
   int64_t ntype[kNspMax];
 };
 constexpr char kStateMagic[8] = {'T', 'R', 'J', 'S', 'T', 'A', 'T', 'E'};
@@ -863,7 +1014,9 @@ constexpr char kStateMagic[8] = {'T', 'R', 'J', 'S', 'T', 'A', 'T', 'E'};
 int64_t state_payload(const trj_ctx* c) {
   int64_t b = 0;
   if (c->rdf_ready) b += int64_t(8) * c->rdf.nit * c->rdf.lsmax;
+  if (c->rdf_ready && c->rdf_charge) b += int64_t(8) * c->rdf.lsmax;
   if (c->sq_ready) b += int64_t(8) * (c->sq.nqmax + 1) * (1 + c->sq.p.nsp);
+  if (c->sq_ready && c->sq_charge) b += int64_t(8) * (c->sq.nqmax + 1);
   return b;
 }
 
@@ -881,6 +1034,7 @@ StateHeader state_header(const trj_ctx* c) {
     for (int k = 0; k < 3; ++k) h.rdf_periodic[k] = c->rdf.periodic[k];
     h.deltar = c->rdf.deltar;
     h.rc = c->rdf.rc;
+    h.rdf_charge = c->rdf_charge;
   }
   if (c->sq_ready) {
     h.sq_ndim = c->sq.p.ndim;
@@ -890,6 +1044,7 @@ StateHeader state_header(const trj_ctx* c) {
     for (int k = 0; k < 3; ++k) h.qbox[k] = c->sq.box[k];
     h.nvec_grid = c->sq.info.nvec_grid;
     h.nvec_axis = c->sq.info.nvec_axis;
+    h.sq_charge = c->sq_charge;
   }
   h.rdf_frames = c->rdf_frames;
   h.sq_frames = c->sq_frames;
@@ -924,12 +1079,18 @@ int trj_state_save(trj_ctx* c, void* buf, int64_t bytes) {
     const size_t b = sizeof(int64_t) * c->rdf.nit * c->rdf.lsmax;
     HIPC(c, hipMemcpy(p, c->counts.p, b, hipMemcpyDeviceToHost));
     p += b;
+    if (c->rdf_charge) {
+      HIPC(c, hipMemcpy(p, c->gqacc.p, sizeof(double) * c->rdf.lsmax, hipMemcpyDeviceToHost));
+      p += sizeof(double) * c->rdf.lsmax;
+    }
   }
   if (c->sq_ready) {
     const size_t ns = c->sq.nqmax + 1;
     HIPC(c, hipMemcpy(p, c->sqf.p, sizeof(double) * ns, hipMemcpyDeviceToHost));
     p += sizeof(double) * ns;
     HIPC(c, hipMemcpy(p, c->sqfp.p, sizeof(double) * ns * c->sq.p.nsp, hipMemcpyDeviceToHost));
+    p += sizeof(double) * ns * c->sq.p.nsp;
+    if (c->sq_charge) HIPC(c, hipMemcpy(p, c->sqfq.p, sizeof(double) * ns, hipMemcpyDeviceToHost));
   }
   return TRJ_OK;
 }
@@ -948,7 +1109,8 @@ int trj_state_load(trj_ctx* c, const void* buf, int64_t bytes, int mode) {
                     h.rdf_periodic[0] == m.rdf_periodic[0] && h.rdf_periodic[1] == m.rdf_periodic[1] &&
                     h.rdf_periodic[2] == m.rdf_periodic[2] && h.sq_ndim == m.sq_ndim && h.nqmax == m.nqmax &&
                     h.qset == m.qset && h.dq == m.dq && h.qbox[0] == m.qbox[0] && h.qbox[1] == m.qbox[1] &&
-                    h.qbox[2] == m.qbox[2] && h.nvec_grid == m.nvec_grid && h.nvec_axis == m.nvec_axis;
+                    h.qbox[2] == m.qbox[2] && h.nvec_grid == m.nvec_grid && h.nvec_axis == m.nvec_axis &&
+                    h.rdf_charge == m.rdf_charge && h.sq_charge == m.sq_charge;
   if (!same) return fail(c, TRJ_EINVAL, "state blob belongs to other RDF / S(Q) plans than this context's");
   if (bytes < trj_state_bytes(c)) return fail(c, TRJ_EINVAL, "state blob too short");
   if (h.last_nsp < 0 || h.last_nsp > kNspMax) return fail(c, TRJ_EINVAL, "corrupt state blob");
@@ -967,6 +1129,17 @@ int trj_state_load(trj_ctx* c, const void* buf, int64_t bytes, int mode) {
       for (size_t k = 0; k < nb; ++k) v[k] += cur[k];
     }
     HIPC(c, hipMemcpy(c->counts.p, v.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice));
+    if (c->rdf_charge) {
+      const size_t nl = c->rdf.lsmax;
+      std::vector<double> g(nl), gc(add ? nl : 0);
+      std::memcpy(g.data(), p, sizeof(double) * nl);
+      p += sizeof(double) * nl;
+      if (add) {
+        HIPC(c, hipMemcpy(gc.data(), c->gqacc.p, sizeof(double) * nl, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < nl; ++k) g[k] += gc[k];
+      }
+      HIPC(c, hipMemcpy(c->gqacc.p, g.data(), sizeof(double) * nl, hipMemcpyHostToDevice));
+    }
   }
   if (c->sq_ready) {
     const size_t nv = static_cast<size_t>(c->sq.nqmax + 1) * (1 + c->sq.p.nsp);
@@ -981,6 +1154,16 @@ int trj_state_load(trj_ctx* c, const void* buf, int64_t bytes, int mode) {
     const size_t ns = c->sq.nqmax + 1;
     HIPC(c, hipMemcpy(c->sqf.p, v.data(), sizeof(double) * ns, hipMemcpyHostToDevice));
     HIPC(c, hipMemcpy(c->sqfp.p, v.data() + ns, sizeof(double) * (nv - ns), hipMemcpyHostToDevice));
+    if (c->sq_charge) {
+      p += sizeof(double) * nv;
+      std::vector<double> g(ns), gc(add ? ns : 0);
+      std::memcpy(g.data(), p, sizeof(double) * ns);
+      if (add) {
+        HIPC(c, hipMemcpy(gc.data(), c->sqfq.p, sizeof(double) * ns, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < ns; ++k) g[k] += gc[k];
+      }
+      HIPC(c, hipMemcpy(c->sqfq.p, g.data(), sizeof(double) * ns, hipMemcpyHostToDevice));
+    }
   }
   // The normalisation inputs: the loaded ones unless this context has
   // processed frames of its own and is adding.

@@ -23,7 +23,8 @@ TRJ_AOS, TRJ_SOA = 0, 1
 TRJ_WANT_RDF, TRJ_WANT_SQ = 1, 2
 TRJ_QSET_REF, TRJ_QSET_HALFSPACE = 0, 1
 TRJ_PREC_FP64, TRJ_PREC_FP32, TRJ_PREC_FP16X3, TRJ_PREC_BF16X3 = 0, 1, 2, 3
-TRJ_SQ_NQ_CONSISTENT = 1
+TRJ_SQ_NQ_CONSISTENT, TRJ_SQ_CHARGE = 1, 2
+TRJ_RDF_CHARGE = 1
 TRJ_PIPE_SERIAL, TRJ_PIPE_OVERLAP = 0, 1
 TRJ_STATE_REPLACE, TRJ_STATE_ADD = 0, 1
 
@@ -35,7 +36,7 @@ EXPORTS = [
     "trj_abi_version", "trj_create", "trj_destroy", "trj_last_error", "trj_rdf_init",
     "trj_sq_init", "trj_sq_plan", "trj_rdf_plan", "trj_frame", "trj_frame_device", "trj_sync",
     "trj_rdf_counts", "trj_rdf_gmix", "trj_rdf_first_minimum", "trj_sq_shells", "trj_sq_columns",
-    "trj_sq_rho", "trj_reset", "trj_last_timings", "trj_timing_totals", "trj_timing_reset",
+    "trj_sq_rho", "trj_frame_q", "trj_rdf_qq", "trj_rdf_gqq", "trj_sq_qq", "trj_sq_qq_column", "trj_reset", "trj_last_timings", "trj_timing_totals", "trj_timing_reset",
     "trj_device_bytes", "trj_set_pipeline", "trj_state_bytes", "trj_state_save", "trj_state_load",
     "trj_lds_atomic_rate", "trj_mfma_rate", "trj_nc_open", "trj_nc_close", "trj_nc_last_error", "trj_nc_info", "trj_nc_read",
     "trj_nc_time", "trj_nc_frame_ref",
@@ -54,7 +55,7 @@ class TrjError(RuntimeError):
 
 class RdfParams(C.Structure):
     _fields_ = [("ndim", C.c_int32), ("nsp", C.c_int32), ("deltar", C.c_double),
-                ("rcrdf", C.c_double), ("periodic", C.c_int32 * 3)]
+                ("rcrdf", C.c_double), ("periodic", C.c_int32 * 3), ("flags", C.c_uint32)]
 
 
 class SqParams(C.Structure):
@@ -114,6 +115,11 @@ def _declare(L):
     L.trj_sq_plan.argtypes = [C.POINTER(SqParams), d, C.POINTER(SqInfo), i64]
     L.trj_rdf_plan.argtypes = [C.POINTER(RdfParams), d, i32, d]
     L.trj_frame.argtypes = [vp, vp, C.c_int32, C.c_int32, i32, C.c_int64, d, u32]
+    L.trj_frame_q.argtypes = [vp, vp, C.c_int32, C.c_int32, i32, d, C.c_int64, d, u32]
+    L.trj_rdf_qq.argtypes = [vp, d]
+    L.trj_rdf_gqq.argtypes = [vp, C.c_int64, d]
+    L.trj_sq_qq.argtypes = [vp, d]
+    L.trj_sq_qq_column.argtypes = [vp, C.c_int64, d]
     L.trj_frame_device.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, C.c_int64, d, u32]
     L.trj_sync.argtypes = [vp]
     L.trj_rdf_counts.argtypes = [vp, i64]
@@ -280,11 +286,12 @@ def _box3(box) -> np.ndarray:
     return b
 
 
-def rdf_params(ndim, nsp, deltar, rcrdf, periodic=(True, True, True)) -> RdfParams:
+def rdf_params(ndim, nsp, deltar, rcrdf, periodic=(True, True, True), flags=0) -> RdfParams:
     p = RdfParams()
     p.ndim, p.nsp, p.deltar, p.rcrdf = ndim, nsp, float(deltar), float(rcrdf)
     for k in range(3):
         p.periodic[k] = int(bool(periodic[k]))
+    p.flags = flags
     return p
 
 
@@ -391,7 +398,8 @@ class Context:
 
     # -- frames
     def frame(self, pos: np.ndarray, itype: np.ndarray, box, what=TRJ_WANT_RDF | TRJ_WANT_SQ,
-              layout=TRJ_AOS):
+              layout=TRJ_AOS, charge=None):
+        """One frame (trj_frame, or trj_frame_q when per-atom charges are given)."""
         pos = np.ascontiguousarray(pos)
         if pos.dtype not in (np.float32, np.float64):
             pos = pos.astype(np.float64)
@@ -399,9 +407,17 @@ class Context:
         it = np.ascontiguousarray(itype, dtype=np.int32)
         n = len(it)
         b = _box3(box)
-        self._check(self._L.trj_frame(self._h, pos.ctypes.data_as(C.c_void_p), dtype, layout,
-                                      _ptr(it, C.c_int32), n, _ptr(b, C.c_double), what),
-                    "trj_frame")
+        if charge is None:
+            self._check(self._L.trj_frame(self._h, pos.ctypes.data_as(C.c_void_p), dtype, layout,
+                                          _ptr(it, C.c_int32), n, _ptr(b, C.c_double), what),
+                        "trj_frame")
+        else:
+            q = np.ascontiguousarray(charge, dtype=np.float64)
+            if len(q) != n:
+                raise TrjError(TRJ_EINVAL, "charge must have one value per atom")
+            self._check(self._L.trj_frame_q(self._h, pos.ctypes.data_as(C.c_void_p), dtype, layout,
+                                            _ptr(it, C.c_int32), _ptr(q, C.c_double), n,
+                                            _ptr(b, C.c_double), what), "trj_frame_q")
 
     def frame_device(self, pos_ptr: int, dtype: int, layout: int, itype_ptr: int, n: int, box,
                      what=TRJ_WANT_RDF | TRJ_WANT_SQ):
@@ -424,6 +440,18 @@ class Context:
         r = self.rdf
         out = np.zeros((max(r.lsmax - 2, 0), 1 + r.nit))
         self._check(self._L.trj_rdf_gmix(self._h, nconf, _ptr(out, C.c_double)), "trj_rdf_gmix")
+        return out
+
+    def rdf_qq(self) -> np.ndarray:
+        """Charge-weighted accumulator gqq[k-1], k = 1..lsmax (trj_rdf_qq)."""
+        out = np.zeros(self.rdf.lsmax)
+        self._check(self._L.trj_rdf_qq(self._h, _ptr(out, C.c_double)), "trj_rdf_qq")
+        return out
+
+    def rdf_gqq(self, nconf: int = 0) -> np.ndarray:
+        """printrdf's g_qq column for r_i, i = 1..lsmax-2 (trj_rdf_gqq)."""
+        out = np.zeros(max(self.rdf.lsmax - 2, 0))
+        self._check(self._L.trj_rdf_gqq(self._h, nconf, _ptr(out, C.c_double)), "trj_rdf_gqq")
         return out
 
     def rdf_first_minimum(self) -> float:
@@ -449,6 +477,18 @@ class Context:
                                            _ptr(mix, C.c_double)), "trj_sq_columns")
         cols = cols.reshape(-1)[: nq * nc.value].reshape(nq, nc.value)
         return cols, mix
+
+    def sq_qq(self) -> np.ndarray:
+        """Charge-weighted shell sums sqfq[0..nqmax] (trj_sq_qq)."""
+        out = np.zeros(self.sq_info.nqmax + 1)
+        self._check(self._L.trj_sq_qq(self._h, _ptr(out, C.c_double)), "trj_sq_qq")
+        return out
+
+    def sq_qq_column(self, nconf: int = 0) -> np.ndarray:
+        """printSQ's S_qq column for shells 1..nqmax (trj_sq_qq_column)."""
+        out = np.zeros(self.sq_info.nqmax)
+        self._check(self._L.trj_sq_qq_column(self._h, nconf, _ptr(out, C.c_double)), "trj_sq_qq_column")
+        return out
 
     def sq_rho(self, kvecs: Sequence) -> np.ndarray:
         kv = np.ascontiguousarray(np.asarray(kvecs, dtype=np.int32).reshape(-1, 3))
