@@ -225,6 +225,8 @@ def main():
     ap.add_argument("--pipeline", default="serial", choices=["serial", "overlap"],
                     help="overlap: the RDF runs on its own stream beside the S(Q) contraction")
     ap.add_argument("--latency-frames", type=int, default=10)
+    ap.add_argument("--charges", action="store_true",
+                    help="also accumulate g_qq and S_qq (per-atom charges, trj_frame_device_q)")
     ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -250,19 +252,27 @@ def main():
     dev = torch.device("cuda:0")
     dpos = [torch.from_numpy(np.ascontiguousarray(f.pos, dtype=np.float64)).to(dev) for f in frames]
     dtyp = torch.from_numpy(frames[0].itype.astype(np.int32)).to(dev)
+    dq = None
+    if args.charges:  # species charges +-0.8 with a small per-atom spread, seeded
+        rng = np.random.default_rng(7)
+        qh = np.where(frames[0].itype == 1, 0.8, -0.8) + 0.05 * rng.standard_normal(n)
+        dq = torch.from_numpy(qh).to(dev)
     torch.cuda.synchronize()
 
     ctx = T.Context(0)
     ctx.set_pipeline(T.TRJ_PIPE_OVERLAP if args.pipeline == "overlap" else T.TRJ_PIPE_SERIAL)
     box = frames[0].box
-    st = ctx.rdf_init(T.rdf_params(3, wl.nsp, wl.deltar, wl.rcrdf), box)
+    rflags = T.TRJ_RDF_CHARGE if args.charges else 0
+    sflags = T.TRJ_SQ_CHARGE if args.charges else 0
+    st = ctx.rdf_init(T.rdf_params(3, wl.nsp, wl.deltar, wl.rcrdf, flags=rflags), box)
     qset = T.TRJ_QSET_HALFSPACE if wl.qset == "halfspace" else T.TRJ_QSET_REF
-    info = ctx.sq_init(T.sq_params(3, wl.nsp, wl.qmax, wl.qmin, wl.bsc, qset, wl.grid, args.precision),
-                       box)
+    info = ctx.sq_init(T.sq_params(3, wl.nsp, wl.qmax, wl.qmin, wl.bsc, qset, wl.grid, args.precision,
+                                   flags=sflags), box)
     nvec = info.nvec_grid + info.nvec_axis
 
     def submit(k):
-        ctx.frame_device(dpos[k % pool].data_ptr(), T.TRJ_F64, T.TRJ_AOS, dtyp.data_ptr(), n, box)
+        ctx.frame_device(dpos[k % pool].data_ptr(), T.TRJ_F64, T.TRJ_AOS, dtyp.data_ptr(), n, box,
+                         charge_ptr=None if dq is None else dq.data_ptr())
 
     log(f"warmup {args.warmup} frames (N={n}, lsmax={st.lsmax}, q-vectors={nvec})")
     for k in range(args.warmup):
@@ -293,7 +303,10 @@ def main():
         lat.append((time.perf_counter() - s0) * 1e3)
 
     fps = args.steps / elapsed
-    flops = 8.0 * n * info.nvec_grid
+    # With charges every atom enters the contraction twice: in its species
+    # channel and in the charge channel.
+    n_entries = 2 * n if args.charges else n
+    flops = 8.0 * n_entries * info.nvec_grid
     kname = "k_sq_contract_f64" if args.precision == "fp64" else f"k_sq_contract_lp<{args.precision}>"
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
@@ -332,6 +345,7 @@ def main():
         "config": {"workload": args.workload, "n_atoms": n, "nsp": wl.nsp, "rcrdf": wl.rcrdf,
                    "deltar": wl.deltar, "qmax": wl.qmax, "qset": wl.qset, "q_vectors": nvec,
                    "lsmax": st.lsmax, "precision": args.precision, "pipeline": args.pipeline,
+                   "charges": bool(args.charges),
                    "inputs": "HBM-resident float64"},
         "pairs_per_s": pairs / elapsed,
         "q_atom_terms_per_s": nvec * n * args.steps / elapsed,
@@ -340,7 +354,7 @@ def main():
         "kernel_ms": kernel_ms,
         "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved / 1e12,
                      "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
-                     "algorithmic": f"8*N*Nq = {flops:.4e} flop per launch (N={n}, Nq={info.nvec_grid})",
+                     "algorithmic": f"8*N*Nq = {flops:.4e} flop per launch (N={n_entries} entries, Nq={info.nvec_grid})",
                      "measured_mfma": probe},
         "rdf_roofline": {"bound": "lds_atomic", "achieved": rdf_rate, "peak": lds_rate, "unit": "pairs/s",
                          "frac": rdf_rate / lds_rate, "pairs_per_frame": pairs_frame,

@@ -164,6 +164,17 @@ int trj_frame(trj_ctx* ctx, const void* pos, int32_t dtype, int32_t layout,
 int trj_frame_q(trj_ctx* ctx, const void* pos, int32_t dtype, int32_t layout, const int32_t* itype,
                 const double* charge, int64_t n, const double box[3], uint32_t what);
 
+/* trj_frame_device_q: trj_frame_q with device-resident positions, types and
+ * charges (charge_dev[n] FP64, in the order of pos_dev); asynchronous like
+ * trj_frame_device.  The fixed-point scale and avq2 are reduced on the device
+ * in a fixed order (1024 contiguous runs, then a pairwise tree), so avq2 can
+ * differ from trj_frame_q's index-order host sum in the last bit; the int64
+ * pair sums are the same.  A non-finite charge is reported at the next
+ * synchronising call. */
+int trj_frame_device_q(trj_ctx* ctx, const void* pos_dev, int32_t dtype, int32_t layout,
+                       const int32_t* itype_dev, const double* charge_dev, int64_t n, const double box[3],
+                       uint32_t what);
+
 /* Same as trj_frame for inputs already in device memory of this context's
  * device (HBM-resident frames). */
 int trj_frame_device(trj_ctx* ctx, const void* pos_dev, int32_t dtype, int32_t layout,
@@ -213,8 +224,9 @@ int trj_sq_columns(trj_ctx* ctx, int64_t nconf, double* cols, int32_t* ncols, do
 int trj_sq_qq(trj_ctx* ctx, double* sqfq);
 
 /* printSQ's S_qq column (log.f90:371-376): for i = 1..nqmax,
- * sqfq(i) / (N Nconf nq(i)) (0 where nq(i) = 0).  nconf <= 0 uses the number
- * of S(Q) frames processed.  Synchronises. */
+ * (sqfq(i) / avq2) / (N Nconf nq(i)), avq2 = sum(q^2)/N of the last S(Q)
+ * frame (sq.cuf:143-145); 0 where nq(i) = 0 or avq2 = 0.  nconf <= 0 uses the
+ * number of S(Q) frames processed.  Synchronises. */
 int trj_sq_qq_column(trj_ctx* ctx, int64_t nconf, double* out);
 
 /* rho_s(q) of the last frame for nvec integer vectors kv[3*v..] that belong
@@ -292,6 +304,7 @@ typedef struct {
   int32_t version;      /* 1, 2 or 5 */
   int32_t coord_dtype;  /* TRJ_F32 / TRJ_F64, as stored */
   int32_t has_type, has_cell_origin, has_cell_lengths, has_time;
+  int32_t has_charge;   /* q(frame, atom) present (the reference's ex_qc) */
 } trj_nc_file_info;
 /* On failure *out is NULL and errbuf (errlen bytes, may be NULL) holds the reason. */
 int trj_nc_open(trj_nc** out, const char* path, char* errbuf, int32_t errlen);
@@ -316,6 +329,12 @@ int trj_nc_time(trj_nc* h, int64_t frame, double* t);
  * pos_soa x[n] y[n] z[n] and itype[n] (capacity natoms each), n, box. */
 int trj_nc_frame_ref(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_types, double* pos_soa,
                      int32_t* itype, int64_t* n_out, double box[3], int32_t periodic[3]);
+/* trj_nc_frame_ref plus the charges of the same atoms, charge[n] (capacity
+ * natoms; may be NULL): q(frame, atom) as stored, converted to FP64
+ * (netcdf.f90:404-408, 657-658).  Fails with TRJ_EINVAL when charge is given
+ * and the file has no q variable. */
+int trj_nc_frame_ref_q(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_types, double* pos_soa,
+                       int32_t* itype, double* charge, int64_t* n_out, double box[3], int32_t periodic[3]);
 
 /* ------------------------------------------------------------------------
  * Batched frames: many independent frames of uneven size and species count,

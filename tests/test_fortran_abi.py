@@ -12,7 +12,7 @@ def test_fortran_abi_check():
     assert os.path.exists(EXE), "run __graft_entry__.build() first (amdflang)"
     out = subprocess.run([EXE], capture_output=True, text=True, check=True).stdout.split("\n")
     vals = {line.split()[0]: line.split()[1:] for line in out if line.strip()}
-    assert vals["sizes"] == ["40", "128", "104", "40"]
+    assert vals["sizes"] == ["40", "128", "104", "48"]
     assert vals["abi"] == ["1"]
     assert vals["plan"] == ["67", "14", "14"]
     with open(os.path.join(ROOT, "tests", "golden", "nq_golden.json")) as f:

@@ -142,7 +142,9 @@ def test_sqq_lj_binary(precision, tol):
     assert _rel(sqfq, osqfq) <= tol
     assert _rel(sqf, osqf) <= tol and _rel(sqfp, osqfp) <= tol
     onq = O.sq_nq(3, g)
-    want = np.array([osqfq[i] / (4000 * 2 * onq[i]) if onq[i] else 0.0 for i in range(1, g.nqmax + 1)])
+    avq2 = np.sum(qs[-1] ** 2) / 4000       # last frame's charges (sq.cuf:143-145)
+    want = np.array([osqfq[i] / avq2 / (4000 * 2 * onq[i]) if onq[i] else 0.0
+                     for i in range(1, g.nqmax + 1)])
     np.testing.assert_allclose(col, want, rtol=max(tol, 1e-12))
 
 
@@ -184,7 +186,7 @@ def test_charge_state_round_trip(tmp_path):
         a.sq_init(sp, frames[0].box)
         for f, q in zip(frames, qs):
             a.frame(f.pos, f.itype, f.box, charge=q)
-        want_g, want_s = a.rdf_qq(), a.sq_qq()
+        want_g, want_s, want_col = a.rdf_qq(), a.sq_qq(), a.sq_qq_column()
     with T.Context(0) as b:
         b.rdf_init(rp, frames[0].box)
         b.sq_init(sp, frames[0].box)
@@ -197,8 +199,50 @@ def test_charge_state_round_trip(tmp_path):
         c.frame(frames[1].pos, frames[1].itype, frames[1].box, charge=qs[1])
         np.testing.assert_array_equal(c.rdf_qq(), want_g)
         assert _rel(c.sq_qq(), want_s) <= 1e-14
+        assert _rel(c.sq_qq_column(), want_col) <= 1e-14
     with T.Context(0) as d:                       # no charge channel: state refused
         d.rdf_init(T.rdf_params(3, 2, 0.02, 4.0), frames[0].box)
         d.sq_init(T.sq_params(3, 2, wl.qmax, wl.qmin, wl.bsc), frames[0].box)
         with pytest.raises(T.TrjError):
             d.state_load(blob)
+
+
+def test_device_entry_matches_host_entry():
+    """trj_frame_device_q (charges reduced on the device) against trj_frame_q:
+    the int64 pair sums are the same, the S(Q) charge sums agree to the FP64
+    shell atomics' order; avq2 may differ
+    in the last bit (fixed-order device reduction against the index-order
+    host sum), which bounds the g_qq and S_qq column difference."""
+    import torch
+    frames = list(W.lj_binary_4k(nframes=2))
+    wl = W.LJ_BINARY_4K
+    qs = [_charges(f.itype, 50 + k) for k, f in enumerate(frames)]
+    rp = T.rdf_params(3, 2, 0.02, 4.0, flags=T.TRJ_RDF_CHARGE)
+    sp = T.sq_params(3, 2, wl.qmax, wl.qmin, wl.bsc, flags=T.TRJ_SQ_CHARGE)
+    print("case: device charge entry", flush=True)
+    out = {}
+    for mode in ("host", "device"):
+        with T.Context(0) as ctx:
+            ctx.rdf_init(rp, frames[0].box)
+            ctx.sq_init(sp, frames[0].box)
+            keep = []
+            for f, q in zip(frames, qs):
+                if mode == "host":
+                    ctx.frame(f.pos, f.itype, f.box, charge=q)
+                else:
+                    dp = torch.from_numpy(np.ascontiguousarray(f.pos, dtype=np.float64)).cuda()
+                    dt = torch.from_numpy(f.itype.astype(np.int32)).cuda()
+                    dq = torch.from_numpy(q).cuda()
+                    keep += [dp, dt, dq]
+                    torch.cuda.synchronize()
+                    ctx.frame_device(dp.data_ptr(), T.TRJ_F64, T.TRJ_AOS, dt.data_ptr(), len(q), f.box,
+                                     charge_ptr=dq.data_ptr())
+            ctx.sync()
+            out[mode] = (ctx.rdf_counts(), ctx.rdf_qq(), ctx.sq_qq(), ctx.sq_qq_column(), ctx.rdf_gqq())
+    h, d = out["host"], out["device"]
+    np.testing.assert_array_equal(d[0], h[0])
+    np.testing.assert_allclose(d[1], h[1], rtol=5e-13, atol=0)
+    # shell sums are FP64 atomics (order-dependent in the last bit)
+    np.testing.assert_allclose(d[2], h[2], rtol=1e-14, atol=0)
+    np.testing.assert_allclose(d[3], h[3], rtol=5e-13, atol=0)
+    np.testing.assert_allclose(d[4], h[4], rtol=5e-13, atol=0)

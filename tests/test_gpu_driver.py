@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 scipy_io = pytest.importorskip("scipy.io")
 
 
-def _write(path, coords, types, org, cell, times):
+def _write(path, coords, types, org, cell, times, q=None):
     nf, na, _ = coords.shape
     with scipy_io.netcdf_file(path, "w", version=2) as nc:
         nc.createDimension("frame", None)
@@ -29,12 +29,15 @@ def _write(path, coords, types, org, cell, times):
         cl = nc.createVariable("cell_lengths", "f", ("frame", "cell_spatial"))
         ty = nc.createVariable("type", "i", ("frame", "atom"))
         xyz = nc.createVariable("coordinates", "f", ("frame", "atom", "spatial"))
+        qv = nc.createVariable("q", "f", ("frame", "atom")) if q is not None else None
         for f in range(nf):
             t[f] = times[f]
             co[f] = org
             cl[f] = cell[f]
             ty[f] = types
             xyz[f] = coords[f]
+            if qv is not None:
+                qv[f] = q[f]
 
 
 def _nml(path, nsp, sel, ncfs, flags, bsc, periodic="T T T"):
@@ -165,3 +168,66 @@ def test_driver_subset_slab_stride(tmp_path):
     np.testing.assert_array_equal(res.rdf_counts, counts)
     assert os.path.exists(os.path.join(tmp_path, "out", "gmixsim.dat"))
     assert not os.path.exists(os.path.join(tmp_path, "out", "sq.dat"))
+
+
+def test_driver_charges_gqq_sqq(tmp_path):
+    """A q variable in the file switches the charge channels on (the
+    reference's ex_qc): gmixsim.dat gains g_qq after r (f16.6) and sq.dat S_qq
+    after S_NN; both are checked against the oracle on the same frames."""
+    rng = np.random.default_rng(23)
+    na, nf = 2400, 3
+    cell = np.tile(np.array([15.0, 15.0, 15.0], dtype=np.float32), (nf, 1))
+    org = np.zeros(3, dtype=np.float32)
+    types = rng.integers(1, 3, na).astype(np.int32)
+    types[:2] = [1, 2]
+    coords = (rng.random((nf, na, 3)) * 15.0).astype(np.float32)
+    q = (np.where(types == 1, 0.6, -0.6) + 0.1 * rng.standard_normal((nf, na))).astype(np.float32)
+    _write(str(tmp_path / "run.nc"), coords, types, org, cell, [10.0, 20.0, 30.0], q=q)
+    _nml(str(tmp_path / "input.nml"), 2, "1 2", "0 1 0", "T T", "1.0 1.0")
+    print("case: driver with charges, RDF + S(Q) + g_qq + S_qq", flush=True)
+    res = D.run(str(tmp_path / "input.nml"), outdir=str(tmp_path / "out"))
+    assert res.charges and res.frames_processed == 3 and res.nconf == 3
+
+    it = types.copy()
+    box = cell[0].astype(np.float64)
+    ls = O.rdf_lsmax(3, box, 6.0, 0.05)
+    acc = np.zeros(ls)
+    counts = None
+    g = O.sq_setup(3, box, 8.0, 4.0)
+    sqf = np.zeros(g.nqmax + 1)
+    sqfp = np.zeros((2, g.nqmax + 1))
+    sqfq = np.zeros(g.nqmax + 1)
+    for f in range(3):
+        pos = coords[f].astype(np.float64)
+        qf = q[f].astype(np.float64)
+        fixed, scale = O.rdf_qq_fixed(pos, qf, 3, box, (1, 1, 1), 6.0, 0.05, lsmax=ls)
+        O.qq_accumulate(fixed, scale, qf, acc)
+        c = O.rdf_counts(pos, it, 3, 2, box, (1, 1, 1), 6.0, 0.05, lsmax=ls)
+        counts = c if counts is None else counts + c
+        w = O.wrap(pos, box, (1, 1, 1))
+        O.sq_frame(w, it, 3, 2, box, g, [1.0, 1.0], sqf, sqfp)
+        O.sq_frame_q(w, qf, 3, box, g, sqfq)
+    np.testing.assert_array_equal(res.rdf_qq, acc)
+    np.testing.assert_array_equal(res.rdf_counts, counts)
+    m = sqfq > 0
+    assert np.max(np.abs(res.sq_qq[m] - sqfq[m]) / sqfq[m]) <= 1e-12
+
+    ntype = np.bincount(it, minlength=3)[1:]
+    vol = float(np.prod(box))
+    gm = O.rdf_gmix(3, 2, ls, 0.05, vol, ntype, 3, counts)
+    gq = O.rdf_gqq(3, ls, 0.05, vol, na, 3, acc)
+    want = np.concatenate([gm[:, :1], gq[:, None], gm[:, 1:]], axis=1)
+    fgm = np.loadtxt(os.path.join(tmp_path, "out", "gmixsim.dat"))
+    assert fgm.shape == want.shape
+    np.testing.assert_allclose(fgm, want, rtol=0, atol=1.1e-6)
+    with open(os.path.join(tmp_path, "out", "gmixsim.dat")) as fh:
+        assert "g_qq(r)" in fh.readline()
+
+    nq = O.sq_nq(3, g)
+    cols, _ = O.sq_columns(2, g, na, ntype, [1.0, 1.0], 3, sqf, sqfp, nq)
+    avq2 = np.sum(q[2].astype(np.float64) ** 2) / na          # last frame (sq.cuf:143-145)
+    sqq = np.array([sqfq[i] / avq2 / (na * 3 * nq[i]) if nq[i] else 0.0 for i in range(1, g.nqmax + 1)])
+    want = np.insert(cols, 2, sqq, axis=1)
+    fsq = np.loadtxt(os.path.join(tmp_path, "out", "sq.dat"))
+    assert fsq.shape == want.shape == (g.nqmax, 8)
+    np.testing.assert_allclose(fsq, want, rtol=0, atol=1.1e-7)

@@ -183,3 +183,46 @@ def test_time_and_periodic_flags(tmp_path):
         assert hi == pytest.approx(L[0] - lo)
         np.testing.assert_allclose(np.diff(pos[0]), np.diff(x), atol=1e-12)
         np.testing.assert_array_equal(pos[1], coords[2][:, 1].astype(np.float64) - np.float64(org[1]))
+
+
+@pytest.mark.parametrize("qtype", ["f", "d"])
+def test_charges_follow_selected_atoms(tmp_path, qtype):
+    """q(frame, atom) comes back with the selected atoms, in their order
+    (netcdf.f90:404-408, 657-658); a file without q reports has_charge 0 and
+    refuses a charge read."""
+    path = str(tmp_path / "q.nc")
+    coords, types, org, L = _frames(nframes=2, natoms=40, seed=9)
+    rng = np.random.default_rng(2)
+    q = rng.uniform(-1.0, 1.0, size=(2, 40))
+    with scipy_io.netcdf_file(path, "w", version=2) as nc:
+        nc.createDimension("frame", None)
+        nc.createDimension("atom", 40)
+        nc.createDimension("spatial", 3)
+        nc.createDimension("cell_spatial", 3)
+        co = nc.createVariable("cell_origin", "f", ("frame", "cell_spatial"))
+        cl = nc.createVariable("cell_lengths", "f", ("frame", "cell_spatial"))
+        ty = nc.createVariable("type", "i", ("frame", "atom"))
+        xyz = nc.createVariable("coordinates", "f", ("frame", "atom", "spatial"))
+        qv = nc.createVariable("q", qtype, ("frame", "atom"))
+        for f in range(2):
+            co[f] = org
+            cl[f] = L
+            ty[f] = types
+            xyz[f] = coords[f]
+            qv[f] = q[f]
+    stored = q.astype(np.float32 if qtype == "f" else np.float64).astype(np.float64)
+    with T.NcTrajectory(path) as tr:
+        assert tr.has_charge
+        for f in range(2):
+            pos, it, box, per, qq = tr.frame_ref(f, [3, 1], charges=True)
+            sel = np.isin(types, [3, 1])
+            np.testing.assert_array_equal(qq, stored[f][sel])
+            pos2, it2, _, _ = tr.frame_ref(f, [3, 1])
+            np.testing.assert_array_equal(pos, pos2)
+            np.testing.assert_array_equal(it, it2)
+    p2 = str(tmp_path / "noq.nc")
+    _write_scipy(p2, 1)
+    with T.NcTrajectory(p2) as tr:
+        assert not tr.has_charge
+        with pytest.raises(T.TrjError):
+            tr.frame_ref(0, [1, 2, 3], charges=True)

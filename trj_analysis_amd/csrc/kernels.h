@@ -65,11 +65,12 @@ struct PairArgs {
   float inv_dr;
   const double* thr;  // lsmax+3 edges
   // Charge-weighted histogram (NULL q: off): q in the pair kernel's atom
-  // order, each in-range pair adds rint(q_i q_j qscale) to gq[k-1] (int64,
-  // two's complement in unsigned 64-bit atomics).
+  // order, each in-range pair adds rint(q_i q_j qp[0]) to gq[k-1] (int64,
+  // two's complement in unsigned 64-bit atomics).  qp = {scale, avq2} of
+  // the frame, in device memory.
   const double* q;
   unsigned long long* gq;
-  double qscale;
+  const double* qp;
 };
 size_t pair_lds_bytes(int nit, int lsmax, bool qq = false);
 // LDS of the cell-list pair kernel: pair_lds_bytes plus its hit rings.
@@ -84,9 +85,11 @@ void launch_rdf_allpairs(const PairArgs& a, const double* pos, const int32_t* it
 // counts[b] += sum over slices of partial[slice][b].
 void launch_rdf_merge(const uint32_t* partial, int nslices, int nbins, int64_t* counts,
                       hipStream_t s);
-// acc[k] += ((double)gq[k] * inv_scale) / avq2, then gq[k] = 0 (k < lsmax).
-void launch_qq_finish(unsigned long long* gq, int lsmax, double inv_scale, double avq2, double* acc,
-                      hipStream_t s);
+// acc[k] += ((double)gq[k] / qp[0]) / qp[1], then gq[k] = 0 (k < lsmax).
+void launch_qq_finish(unsigned long long* gq, int lsmax, const double* qp, double* acc, hipStream_t s);
+// qp = {2^(30-e) with max q^2 < 2^e, sum(q^2)/n} from device charges q[n], in
+// a fixed order (one workgroup); a non-finite charge sets kFlagRange.
+void launch_qq_params(const double* q, int64_t n, double* qp, uint32_t* flags, hipStream_t s);
 
 // S(Q) contraction: rho_s(q) over the active tiles of the dense q-box.
 struct ContractArgs {

@@ -320,7 +320,7 @@ constexpr int kHitQ = 128;  // ring entries per wavefront (holds < 64 + 64)
 template <bool QQ = false>
 __device__ inline void bin_hits(const PairArgs& a, const double* thr, uint32_t* hist, const double* qr2,
                                 const int* qtt, uint32_t head, int cnt, int lane, const double* qqr = nullptr,
-                                unsigned long long* gqs = nullptr) {
+                                unsigned long long* gqs = nullptr, double qsc = 0.0) {
   if (lane < cnt) {
     const uint32_t e = (head + lane) & (kHitQ - 1);
     const double r2 = qr2[e];
@@ -330,7 +330,7 @@ __device__ inline void bin_hits(const PairArgs& a, const double* thr, uint32_t* 
     if (k >= 1 && k <= a.lsmax) {
       const int lo = ti < tj ? ti : tj, hi = ti < tj ? tj : ti;
       atomicAdd(&hist[(pair_index(lo, hi, a.nsp) - 1) * a.lsmax + (k - 1)], 1u);
-      if constexpr (QQ) atomicAdd(&gqs[k - 1], qq_fixed(qqr[e], a.qscale));
+      if constexpr (QQ) atomicAdd(&gqs[k - 1], qq_fixed(qqr[e], qsc));
     }
   }
 }
@@ -345,7 +345,8 @@ __device__ inline void rdf_neighbours(const PairArgs& a, const double* thr, uint
                                       double shz, const double* __restrict__ sx, const double* __restrict__ sy,
                                       const double* __restrict__ sz, const int32_t* __restrict__ stype,
                                       double qi = 0.0, const double* __restrict__ sq = nullptr,
-                                      double* qqr = nullptr, unsigned long long* gqs = nullptr) {
+                                      double* qqr = nullptr, unsigned long long* gqs = nullptr,
+                                      double qsc = 0.0) {
   const double nan = __builtin_nan("");
   for (int jb = js; jb < je; jb += kWave) {
     const int jl = jb + lane;
@@ -400,7 +401,7 @@ __device__ inline void rdf_neighbours(const PairArgs& a, const double* thr, uint
       tail += static_cast<uint32_t>(__popcll(m));
       if (tail - head >= kWave) {
         __builtin_amdgcn_wave_barrier();
-        bin_hits<QQ>(a, thr, hist, qr2, qtt, head, kWave, lane, qqr, gqs);
+        bin_hits<QQ>(a, thr, hist, qr2, qtt, head, kWave, lane, qqr, gqs, qsc);
         __builtin_amdgcn_wave_barrier();
         head += kWave;
       }
@@ -418,7 +419,8 @@ __device__ inline void rdf_home_chunk(const PairArgs& a, const double* thr, uint
                                       const int32_t* __restrict__ start, const double* __restrict__ sx,
                                       const double* __restrict__ sy, const double* __restrict__ sz,
                                       const int32_t* __restrict__ stype, const double* __restrict__ sq = nullptr,
-                                      double* qqr = nullptr, unsigned long long* gqs = nullptr) {
+                                      double* qqr = nullptr, unsigned long long* gqs = nullptr,
+                                      double qsc = 0.0) {
   const double nan = __builtin_nan("");
   const int ncx = nc[0], ncy = nc[1], ncz = nc[2];
   const int cx = cellL % ncx, cy = (cellL / ncx) % ncy, cz = cellL / (ncx * ncy);
@@ -450,16 +452,18 @@ __device__ inline void rdf_home_chunk(const PairArgs& a, const double* thr, uint
     if (o == 0) {
       rdf_neighbours<SHIFT, D3, true, false, QQ>(a, thr, hist, qr2, qtt, head, tail, lane, cs + ch * kWave + 1,
                                                  ce, gi, xi, yi, zi, ti8, 0.0, 0.0, 0.0, sx, sy, sz, stype, qi, sq,
-                                                 qqr, gqs);
+                                                 qqr, gqs, qsc);
     } else {
       const int js = __builtin_amdgcn_readfirstlane(start[nb]);
       const int je = __builtin_amdgcn_readfirstlane(start[nb + 1]);
       if (shx != 0.0 || shy != 0.0 || shz != 0.0)
         rdf_neighbours<SHIFT, D3, false, true, QQ>(a, thr, hist, qr2, qtt, head, tail, lane, js, je, gi, xi, yi,
-                                                   zi, ti8, shx, shy, shz, sx, sy, sz, stype, qi, sq, qqr, gqs);
+                                                   zi, ti8, shx, shy, shz, sx, sy, sz, stype, qi, sq, qqr, gqs,
+                                                   qsc);
       else
         rdf_neighbours<SHIFT, D3, false, false, QQ>(a, thr, hist, qr2, qtt, head, tail, lane, js, je, gi, xi, yi,
-                                                    zi, ti8, 0.0, 0.0, 0.0, sx, sy, sz, stype, qi, sq, qqr, gqs);
+                                                    zi, ti8, 0.0, 0.0, 0.0, sx, sy, sz, stype, qi, sq, qqr, gqs,
+                                                    qsc);
     }
   }
 }
@@ -482,7 +486,9 @@ __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stenc
   const size_t qqoff = qoff + 4 * kHitQ * (sizeof(double) + sizeof(int));
   double* qqr = nullptr;
   unsigned long long* gqs = nullptr;
+  double qsc = 0.0;  // the frame's fixed-point scale (qp[0])
   if constexpr (QQ) {
+    qsc = a.qp[0];
     gqs = reinterpret_cast<unsigned long long*>(smem + qqoff + 4 * kHitQ * sizeof(double));
     for (int b = threadIdx.x; b < a.lsmax; b += blockDim.x) gqs[b] = 0ull;
   }
@@ -503,10 +509,10 @@ __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stenc
     const int ce = __builtin_amdgcn_readfirstlane(start[cell + 1]);
     for (int ch = ch0; cs + ch * kWave < ce; ch += W)
       rdf_home_chunk<SHIFT, D3, QQ>(a, thr, hist, qr2, qtt, head, tail, lane, st, g.nc, g.periodic, 0, cell, ch,
-                                    cs, ce, start, sx, sy, sz, stype, a.q, qqr, gqs);
+                                    cs, ce, start, sx, sy, sz, stype, a.q, qqr, gqs, qsc);
   }
   __builtin_amdgcn_wave_barrier();
-  bin_hits<QQ>(a, thr, hist, qr2, qtt, head, static_cast<int>(tail - head), lane, qqr, gqs);
+  bin_hits<QQ>(a, thr, hist, qr2, qtt, head, static_cast<int>(tail - head), lane, qqr, gqs, qsc);
   hist_flush(a, hist, partial);  // begins with a barrier: every bin_hits is done
   if constexpr (QQ) {
     for (int b = threadIdx.x; b < a.lsmax; b += blockDim.x) {
@@ -530,6 +536,7 @@ __global__ __launch_bounds__(256) void k_rdf_allpairs(PairArgs a, const double* 
               ~size_t(15)));
   if (a.q)
     for (int b = threadIdx.x; b < a.lsmax; b += blockDim.x) gqs[b] = 0ull;
+  const double qsc = a.q ? a.qp[0] : 0.0;  // the frame's fixed-point scale
   hist_setup(a, hist, thr);
   const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   const double* __restrict__ px = pos;
@@ -555,7 +562,7 @@ __global__ __launch_bounds__(256) void k_rdf_allpairs(PairArgs a, const double* 
       const bool ok = oki && (jb > ib || j > gi) &&
                       static_cast<unsigned>(tj - 1) < static_cast<unsigned>(a.nsp);
       const int k = pair_accumulate(a, thr, hist, ok, xi, yi, zi, ti, px[j], py[j], pz[j], tj);
-      if (a.q && k) atomicAdd(&gqs[k - 1], qq_fixed(qi * a.q[j], a.qscale));
+      if (a.q && k) atomicAdd(&gqs[k - 1], qq_fixed(qi * a.q[j], qsc));
     }
   }
   hist_flush(a, hist, partial);  // begins with a barrier
@@ -567,15 +574,56 @@ __global__ __launch_bounds__(256) void k_rdf_allpairs(PairArgs a, const double* 
   }
 }
 
-// The frame's charge sums into the FP64 accumulators (rdf.cuf:285-287:
-// gqq += gqq_frame / avq2), then cleared for the next frame.
-__global__ __launch_bounds__(256) void k_qq_finish(unsigned long long* __restrict__ gq, int lsmax, double inv_scale,
-                                                   double avq2, double* __restrict__ acc) {
+// The frame's charge sums into the FP64 accumulators (rdf.cuf:270-274:
+// gqq += gqq_frame / avq2), then cleared for the next frame.  qp = {scale,
+// avq2}; the scale is a power of two, so f / scale = f * (1 / scale) exactly.
+__global__ __launch_bounds__(256) void k_qq_finish(unsigned long long* __restrict__ gq, int lsmax,
+                                                   const double* __restrict__ qp, double* __restrict__ acc) {
   const int k = blockIdx.x * 256 + threadIdx.x;
   if (k >= lsmax) return;
+  const double inv_scale = 1.0 / qp[0], avq2 = qp[1];
   const long long f = static_cast<long long>(gq[k]);
   if (avq2 > 0.0) acc[k] += (static_cast<double>(f) * inv_scale) / avq2;
   gq[k] = 0ull;
+}
+
+// The frame's charge parameters from device-resident charges, in one
+// workgroup with a fixed reduction order: qp[0] = 2^(30-e) with max q^2 < 2^e
+// (2^30 when every charge is 0), qp[1] = sum(q^2)/n.  Thread t sums a
+// contiguous run of q^2 in index order, then the 1024 partial sums meet in a
+// pairwise tree.  A non-finite charge raises kFlagRange.
+__global__ __launch_bounds__(1024) void k_qq_params(const double* __restrict__ q, int64_t n, double* __restrict__ qp,
+                                                    uint32_t* __restrict__ flags) {
+  __shared__ double ssum[1024];
+  __shared__ double smax[1024];
+  const int t = threadIdx.x;
+  const int64_t per = (n + 1023) / 1024;
+  const int64_t b = t * per, e = b + per < n ? b + per : n;
+  double s = 0.0, m = 0.0;
+  bool bad = false;
+  for (int64_t i = b; i < e; ++i) {
+    const double v = q[i] * q[i];
+    bad |= !isfinite(v);
+    s += v;
+    m = v > m ? v : m;
+  }
+  ssum[t] = s;
+  smax[t] = m;
+  if (bad) atomicOr(flags, kFlagRange);
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (t < w) {
+      ssum[t] = ssum[t] + ssum[t + w];
+      smax[t] = smax[t + w] > smax[t] ? smax[t + w] : smax[t];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    int ex = 0;
+    if (smax[0] > 0.0) frexp(smax[0], &ex);
+    qp[0] = ldexp(1.0, 30 - ex);
+    qp[1] = ssum[0] / static_cast<double>(n);
+  }
 }
 
 // Slice group blockIdx.y sums its share of the per-workgroup slices, then adds
@@ -1813,9 +1861,13 @@ void launch_rdf_allpairs(const PairArgs& a, const double* pos, const int32_t* it
                      a, pos, itype, n, nb, partial);
 }
 
-void launch_qq_finish(unsigned long long* gq, int lsmax, double inv_scale, double avq2, double* acc,
+void launch_qq_params(const double* q, int64_t n, double* qp, uint32_t* flags, hipStream_t s) {
+  hipLaunchKernelGGL(k_qq_params, dim3(1), dim3(1024), 0, s, q, n, qp, flags);
+}
+
+void launch_qq_finish(unsigned long long* gq, int lsmax, const double* qp, double* acc,
                       hipStream_t s) {
-  hipLaunchKernelGGL(k_qq_finish, dim3(grid1d(lsmax, 256)), dim3(256), 0, s, gq, lsmax, inv_scale, avq2, acc);
+  hipLaunchKernelGGL(k_qq_finish, dim3(grid1d(lsmax, 256)), dim3(256), 0, s, gq, lsmax, qp, acc);
 }
 
 void launch_rdf_merge(const uint32_t* partial, int nslices, int nbins, int64_t* counts, hipStream_t s) {

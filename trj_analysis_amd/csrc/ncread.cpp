@@ -4,8 +4,8 @@
 //
 // It replaces the NetCDF-Fortran calls of read_nc_cfg (src/core/netcdf.f90:
 // 173-550) for the variables the structural path reads: coordinates
-// (frame, atom, spatial), type (frame, atom), cell_origin and cell_lengths
-// (frame, cell_spatial).  netCDF-4 files (HDF5 containers) are refused with a
+// (frame, atom, spatial), type (frame, atom), q (frame, atom), cell_origin and
+// cell_lengths (frame, cell_spatial).  netCDF-4 files (HDF5 containers) are refused with a
 // message; `nccopy -k cdf5 in.nc out.nc` converts them.
 #include <algorithm>
 #include <cmath>
@@ -259,11 +259,12 @@ struct trj_nc {
   const NcVar* origin = nullptr;
   const NcVar* lengths = nullptr;
   const NcVar* time = nullptr;
+  const NcVar* charge = nullptr;
   int64_t natoms = 0;
   // LAMMPS type id -> species 1..nsp, fixed by the first trj_nc_frame_ref call
   std::vector<int32_t> species_of;  // indexed by type id; 0 = not selected
   std::vector<int32_t> selected;    // the sp_types that fixed the map
-  std::vector<double> xyz, tyd;      // frame_ref scratch, kept between frames
+  std::vector<double> xyz, tyd, qd;  // frame_ref scratch, kept between frames
   std::vector<int32_t> ty;
   std::string err;
 };
@@ -297,6 +298,8 @@ int trj_nc_open(trj_nc** out, const char* path, char* errbuf, int32_t errlen) {
   if (h->lengths && (!h->lengths->record || h->lengths->inner != 3)) h->lengths = nullptr;
   h->time = h->r.var("time");
   if (h->time && (!h->time->record || h->time->inner != 1)) h->time = nullptr;
+  h->charge = h->r.var("q");
+  if (h->charge && (!h->charge->record || h->charge->inner != h->natoms)) h->charge = nullptr;
   *out = h;
   return TRJ_OK;
 }
@@ -319,6 +322,7 @@ int trj_nc_info(const trj_nc* h, trj_nc_file_info* info) {
   info->has_cell_origin = h->origin != nullptr;
   info->has_cell_lengths = h->lengths != nullptr;
   info->has_time = h->time != nullptr;
+  info->has_charge = h->charge != nullptr;
   return TRJ_OK;
 }
 
@@ -357,6 +361,11 @@ int trj_nc_time(trj_nc* h, int64_t frame, double* t) {
 
 int trj_nc_frame_ref(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_types, double* pos_soa,
                      int32_t* itype, int64_t* n_out, double box[3], int32_t periodic[3]) {
+  return trj_nc_frame_ref_q(h, frame, nsp, sp_types, pos_soa, itype, nullptr, n_out, box, periodic);
+}
+
+int trj_nc_frame_ref_q(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_types, double* pos_soa,
+                       int32_t* itype, double* charge, int64_t* n_out, double box[3], int32_t periodic[3]) {
   if (!h || !sp_types || !pos_soa || !itype || !n_out || !box || !periodic) return TRJ_EINVAL;
   if (nsp < 1 || nsp > 10) return (h->err = "nsp must be in 1..10", TRJ_EINVAL);
   if (!h->type) return (h->err = "the file has no type variable", TRJ_EINVAL);
@@ -368,6 +377,13 @@ int trj_nc_frame_ref(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_ty
   double org[3], cell[3];
   int rc = trj_nc_read(h, frame, r.data(), ty.data(), org, cell);
   if (rc) return rc;
+  // Charges (netcdf.f90:404-408, 657-658): q(frame, atom) as stored, carried
+  // with the selected atoms.
+  if (charge) {
+    if (!h->charge) return (h->err = "the file has no q(frame, atom) variable", TRJ_EINVAL);
+    h->qd.resize(static_cast<size_t>(n));
+    if (!h->r.read(*h->charge, frame, 0, n, h->qd.data())) return (h->err = h->r.err, TRJ_EINVAL);
+  }
   // Species map, fixed at the first frame read: the selected LAMMPS types in
   // order of first appearance (netcdf.f90:441-506).
   const std::vector<int32_t> sel(sp_types, sp_types + nsp);
@@ -445,6 +461,7 @@ int trj_nc_frame_ref(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_ty
     const int32_t s = (t >= 0 && t < ntab) ? h->species_of[t] : 0;
     if (s == 0) continue;
     itype[j] = s;
+    if (charge) charge[j] = h->qd[i];
     for (int k = 0; k < 3; ++k) {
       double x = r[3 * i + k] - sub[k];
       if (shift[k] != 0.0) x = x + shift[k];

@@ -52,6 +52,9 @@ struct trj_ctx {
   // cell-sorted copy, the frame's fixed-point g_qq sums, the g_qq and S_qq
   // accumulators.
   DevBuf qraw, qsort, gqf, gqacc, sqfq;
+  // qparm: the frame's {fixed-point scale, avq2}, then avq2 of the last S(Q)
+  // frame with charges (the S_qq column divides by it).
+  DevBuf qparm;
   bool rdf_charge = false, sq_charge = false;
   // pinned staging for host inputs (double-buffered)
   void* pinned[2] = {nullptr, nullptr};
@@ -178,11 +181,10 @@ int validate_box(trj_ctx* c, int ndim, const double box[3]) {
 }
 
 // The frame pipeline, device inputs.
-// q_dev: the frame's charges on the device (NULL without); qscale: the
-// frame's fixed-point scale of q_i q_j; avq2: sum(q^2)/n.
+// q_dev: the frame's charges on the device (NULL without); the caller has
+// queued the frame's {scale, avq2} into qparm[0..1] on the context stream.
 int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int32_t* itype_dev, int64_t n,
-              const double box_in[3], uint32_t what, const double* q_dev = nullptr, double qscale = 1.0,
-              double avq2 = 0.0) {
+              const double box_in[3], uint32_t what, const double* q_dev = nullptr) {
   const bool do_rdf = (what & TRJ_WANT_RDF) != 0;
   const bool do_sq = (what & TRJ_WANT_SQ) != 0;
   if (do_rdf && !c->rdf_ready) return fail(c, TRJ_ESTATE, "frame asks for RDF before trj_rdf_init");
@@ -282,7 +284,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     pa.thr = P<double>(c->thr);
     if (rdf_q) {
       pa.gq = P<unsigned long long>(c->gqf);
-      pa.qscale = qscale;
+      pa.qp = P<double>(c->qparm);
     }
     const int nbins = R.nit * R.lsmax;
     // Cell grid: side >= cutoff on every axis.
@@ -340,7 +342,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
       launch_rdf_allpairs(pa, P<double>(c->pos), itype_dev, n, nblocks, P<uint32_t>(c->partial), rs);
     }
     launch_rdf_merge(P<uint32_t>(c->partial), nblocks, nbins, P<int64_t>(c->counts), rs);
-    if (rdf_q) launch_qq_finish(P<unsigned long long>(c->gqf), R.lsmax, 1.0 / qscale, avq2, P<double>(c->gqacc), rs);
+    if (rdf_q) launch_qq_finish(P<unsigned long long>(c->gqf), R.lsmax, P<double>(c->qparm), P<double>(c->gqacc), rs);
     HIPC(c, hipGetLastError());
     c->rdf_frames += 1;
     c->last_volume = g.volume;
@@ -390,6 +392,10 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     launch_sq_list(P<double>(c->u), sq_q ? P<double>(c->u) + 12 * ustride : nullptr, P<int32_t>(c->sp_off), ustride,
                    nsp, nch, ndim, P<int32_t>(c->axis_kv), P<int32_t>(c->axis_shell), nax, S.p.bsc, S.nqmax,
                    P<double>(c->listpart), P<double>(c->sqf), P<double>(c->sqfp), P<double>(c->sqfq), c->stream);
+    // sq.cuf:143-145 divides by avq2 of the last frame: keep it in qparm[2].
+    if (sq_q)
+      HIPC(c, hipMemcpyAsync(P<double>(c->qparm) + 2, P<double>(c->qparm) + 1, sizeof(double),
+                             hipMemcpyDeviceToDevice, c->stream));
     HIPC(c, hipGetLastError());
     c->sq_frames += 1;
   } else {
@@ -489,7 +495,7 @@ void trj_destroy(trj_ctx* c) {
                     &c->cellcount, &c->cellstart, &c->cursor, &c->partial, &c->counts, &c->thr,
                     &c->blk,  &c->blkoff,    &c->sp_off,    &c->u,      &c->rho,    &c->tiles,
                     &c->tile_shell, &c->sqf, &c->sqfp,      &c->axis_kv, &c->axis_shell, &c->listpart,
-                    &c->flags, &c->qraw, &c->qsort, &c->gqf, &c->gqacc, &c->sqfq};
+                    &c->flags, &c->qraw, &c->qsort, &c->gqf, &c->gqacc, &c->sqfq, &c->qparm};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   for (int k = 0; k < 2; ++k) {
@@ -553,6 +559,7 @@ int trj_rdf_init(trj_ctx* c, const trj_rdf_params* p, const double box[3], int32
   if (qq) {
     if ((r = ensure(c, c->gqf, sizeof(unsigned long long) * R.lsmax))) return r;
     if ((r = ensure(c, c->gqacc, sizeof(double) * R.lsmax))) return r;
+    if ((r = ensure(c, c->qparm, 4 * sizeof(double)))) return r;
     HIPC(c, hipMemsetAsync(c->gqf.p, 0, sizeof(unsigned long long) * R.lsmax, c->stream));
     HIPC(c, hipMemsetAsync(c->gqacc.p, 0, sizeof(double) * R.lsmax, c->stream));
   }
@@ -587,7 +594,9 @@ int trj_sq_init(trj_ctx* c, const trj_sq_params* p, const double box[3], trj_sq_
   const bool sqq = (p->flags & TRJ_SQ_CHARGE) != 0;
   if (sqq) {
     if ((r = ensure(c, c->sqfq, sizeof(double) * ns))) return r;
+    if ((r = ensure(c, c->qparm, 4 * sizeof(double)))) return r;
     HIPC(c, hipMemsetAsync(c->sqfq.p, 0, sizeof(double) * ns, c->stream));
+    HIPC(c, hipMemsetAsync(c->qparm.p, 0, 4 * sizeof(double), c->stream));
   }
   if (!S.tiles.empty()) {
     HIPC(c, hipMemcpyAsync(c->tiles.p, S.tiles.data(), sizeof(SqTile) * S.tiles.size(), hipMemcpyHostToDevice,
@@ -640,11 +649,13 @@ int frame_host(trj_ctx* c, const void* pos, int32_t dtype, int32_t layout, const
   const size_t qbytes = charge ? static_cast<size_t>(n) * sizeof(double) : 0;
   const size_t pbytes_al = (pbytes + 255) & ~size_t(255);
   const size_t tbytes_al = (tbytes + 255) & ~size_t(255);
-  const size_t total = pbytes_al + tbytes_al + qbytes;
+  const size_t qbytes_al = (qbytes + 255) & ~size_t(255);
+  const size_t total = pbytes_al + tbytes_al + qbytes_al + (charge ? 2 * sizeof(double) : 0);
   int r;
   if ((r = ensure(c, c->raw, pbytes))) return r;
   if ((r = ensure(c, c->itype, tbytes))) return r;
   if (charge && (r = ensure(c, c->qraw, qbytes))) return r;
+  if (charge && (r = ensure(c, c->qparm, 4 * sizeof(double)))) return r;
   // Fixed-point scale of q_i q_j (2^(30-e), max q^2 < 2^e) and avq2 =
   // sum(q^2)/n, both from the host copy in index order.
   double qscale = 1.0, avq2 = 0.0;
@@ -678,14 +689,21 @@ int frame_host(trj_ctx* c, const void* pos, int32_t dtype, int32_t layout, const
   char* st = static_cast<char*>(c->pinned[k]);
   std::memcpy(st, pos, pbytes);
   std::memcpy(st + pbytes_al, itype, tbytes);
-  if (charge) std::memcpy(st + pbytes_al + tbytes_al, charge, qbytes);
+  if (charge) {
+    std::memcpy(st + pbytes_al + tbytes_al, charge, qbytes);
+    const double qp[2] = {qscale, avq2};
+    std::memcpy(st + pbytes_al + tbytes_al + qbytes_al, qp, sizeof(qp));
+  }
   HIPC(c, hipMemcpyAsync(c->raw.p, st, pbytes, hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipMemcpyAsync(c->itype.p, st + pbytes_al, tbytes, hipMemcpyHostToDevice, c->stream));
-  if (charge)
+  if (charge) {
     HIPC(c, hipMemcpyAsync(c->qraw.p, st + pbytes_al + tbytes_al, qbytes, hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(c->qparm.p, st + pbytes_al + tbytes_al + qbytes_al, 2 * sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
+  }
   HIPC(c, hipEventRecord(c->pinned_done[k], c->stream));
   return run_frame(c, c->raw.p, dtype, layout, P<int32_t>(c->itype), n, box, what,
-                   charge ? P<double>(c->qraw) : nullptr, qscale, avq2);
+                   charge ? P<double>(c->qraw) : nullptr);
 }
 
 }  // namespace
@@ -702,6 +720,20 @@ int trj_frame_q(trj_ctx* c, const void* pos, int32_t dtype, int32_t layout, cons
   if (!c) return TRJ_EINVAL;
   if (!charge) return fail(c, TRJ_EINVAL, "NULL charges");
   return frame_host(c, pos, dtype, layout, itype, charge, n, box, what);
+}
+
+int trj_frame_device_q(trj_ctx* c, const void* pos_dev, int32_t dtype, int32_t layout, const int32_t* itype_dev,
+                       const double* charge_dev, int64_t n, const double box[3], uint32_t what) {
+  if (!c) return TRJ_EINVAL;
+  if (!pos_dev || !itype_dev || !charge_dev) return fail(c, TRJ_EINVAL, "NULL input");
+  if (n < 1) return fail(c, TRJ_EINVAL, "n must be positive");
+  hipSetDevice(c->device);
+  int r;
+  if ((r = ensure(c, c->qparm, 4 * sizeof(double)))) return r;
+  if ((r = ensure(c, c->flags, sizeof(uint32_t)))) return r;
+  launch_qq_params(charge_dev, n, P<double>(c->qparm), P<uint32_t>(c->flags), c->stream);
+  HIPC(c, hipGetLastError());
+  return run_frame(c, pos_dev, dtype, layout, itype_dev, n, box, what, charge_dev);
 }
 
 int trj_sync(trj_ctx* c) {
@@ -867,11 +899,14 @@ int trj_sq_qq_column(trj_ctx* c, int64_t nconf, double* out) {
   int r = trj_sq_qq(c, q.data());
   if (r) return r;
   if (nconf <= 0) nconf = c->sq_frames;
-  // log.f90:371-376: sqfq(i) / (Nsites Nconf nq(i))
+  // sq.cuf:143-145 divides the accumulated sums by avq2 of the last frame
+  // read; log.f90:371-376 then writes sqfq(i) / (Nsites Nconf nq(i)).
+  double av = 0.0;
+  HIPC(c, hipMemcpy(&av, P<double>(c->qparm) + 2, sizeof(double), hipMemcpyDeviceToHost));
   const double N = static_cast<double>(c->last_n);
   for (int i = 1; i <= S.nqmax; ++i) {
     const double cn = static_cast<double>(nconf) * static_cast<double>(S.nq[i]);
-    out[i - 1] = cn > 0.0 ? q[i] / (N * cn) : 0.0;
+    out[i - 1] = (cn > 0.0 && av > 0.0) ? (q[i] / av) / (N * cn) : 0.0;
   }
   return TRJ_OK;
 }
@@ -1007,6 +1042,7 @@ struct StateHeader {
   int64_t rdf_frames, sq_frames, last_n;
   double last_volume;
   int32_t last_nsp, rdf_charge;
+  double sq_avq2;
   int64_t ntype[kNspMax];
 };
 constexpr char kStateMagic[8] = {'T', 'R', 'J', 'S', 'T', 'A', 'T', 'E'};
@@ -1067,6 +1103,8 @@ int trj_state_save(trj_ctx* c, void* buf, int64_t bytes) {
   int r = sync_all(c);
   if (r) return r;
   StateHeader h = state_header(c);
+  if (c->sq_ready && c->sq_charge)
+    HIPC(c, hipMemcpy(&h.sq_avq2, P<double>(c->qparm) + 2, sizeof(double), hipMemcpyDeviceToHost));
   if (c->last_nsp > 0 && (c->rdf_frames > 0 || c->sq_frames > 0)) {
     std::vector<int64_t> nt;
     if ((r = read_ntype(c, nt))) return r;
@@ -1174,6 +1212,8 @@ int trj_state_load(trj_ctx* c, const void* buf, int64_t bytes, int mode) {
     c->last_n = h.last_n;
     c->last_volume = h.last_volume;
     c->last_nsp = h.last_nsp;
+    if (c->sq_ready && c->sq_charge)
+      HIPC(c, hipMemcpy(P<double>(c->qparm) + 2, &h.sq_avq2, sizeof(double), hipMemcpyHostToDevice));
     std::vector<int32_t> off(h.last_nsp + 1, 0);
     for (int k = 0; k < h.last_nsp; ++k) off[k + 1] = off[k] + static_cast<int32_t>(h.ntype[k]);
     if ((r = ensure(c, c->sp_off, sizeof(int32_t) * off.size()))) return r;

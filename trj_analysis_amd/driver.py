@@ -58,6 +58,9 @@ class RunResult:
     sq_info: Optional[object] = None
     rdf_counts: Optional[np.ndarray] = None   # (nit, lsmax) int64, as trj_rdf_counts
     sq_shells: Optional[tuple] = None         # (sqf, sqfp, nq), as trj_sq_shells
+    charges: bool = False                     # the file holds q (ex_qc): g_qq and S_qq
+    rdf_qq: Optional[np.ndarray] = None       # charge-weighted accumulator, as trj_rdf_qq
+    sq_qq: Optional[np.ndarray] = None        # charge-weighted shell sums, as trj_sq_qq
     files: List[str] = field(default_factory=list)
     t_read: float = 0.0             # seconds in trj_nc_frame_ref (host read + conversion)
     t_total: float = 0.0            # seconds from the first read to the written files
@@ -182,7 +185,9 @@ def run(inp: Union[str, StructuralInput], *, outdir: str = ".", trj_file: Option
         sel, species_types, bsc_index = resolve_species(nsp, inp.sp_types_selected, types0)
         bsc = [inp.bsc[k] if k < len(inp.bsc) else 1.0 for k in bsc_index]
         ctx = T.Context(int(os.environ.get("TRJ_DEVICE", "0")) if device is None else device)
-        res = RunResult(nconf=nconf, frames_processed=0, species_types=species_types)
+        res = RunResult(nconf=nconf, frames_processed=0, species_types=species_types,
+                        charges=tr.has_charge)
+        qch = None
         per_in = tuple(bool(p) for p in inp.periodic)
         first = True
         what = (T.TRJ_WANT_RDF if inp.run_rdf else 0) | (T.TRJ_WANT_SQ if inp.run_sq else 0)
@@ -192,7 +197,10 @@ def run(inp: Union[str, StructuralInput], *, outdir: str = ".", trj_file: Option
             t_start = time.perf_counter()
             for i, (ncstart, skip) in enumerate(schedule, start=1):
                 t0 = time.perf_counter()
-                pos, itype, box, per = tr.frame_ref(ncstart - 1, sel, per_in)
+                if res.charges:  # netcdf.f90:404-408: a q variable switches ex_qc on
+                    pos, itype, box, per, qch = tr.frame_ref(ncstart - 1, sel, per_in, charges=True)
+                else:
+                    pos, itype, box, per = tr.frame_ref(ncstart - 1, sel, per_in)
                 res.t_read += time.perf_counter() - t0
                 res.frames_read.append(ncstart)
                 if first:  # init_modules (util.f90:180-260) on the first frame read
@@ -200,10 +208,12 @@ def run(inp: Union[str, StructuralInput], *, outdir: str = ".", trj_file: Option
                     res.nsites = len(itype)
                     if inp.run_rdf:
                         res.lsmax = mod_rdf.RDF_init(nsp, ndim=ndim, deltar=inp.deltar, rcrdf=inp.rcrdf,
-                                                     sidel=box, periodic=tuple(bool(p) for p in per), ctx=ctx)
+                                                     sidel=box, periodic=tuple(bool(p) for p in per),
+                                                     charges=res.charges, ctx=ctx)
                     if inp.run_sq:
                         res.sq_info = mod_sq.sq_init(len(itype), nsp, ndim=ndim, qmax=inp.qmax, qmin=inp.qmin,
-                                                     bsc=bsc, sidel=box, precision=precision, ctx=ctx)
+                                                     bsc=bsc, sidel=box, precision=precision,
+                                                     flags=T.TRJ_SQ_CHARGE if res.charges else 0, ctx=ctx)
                     first = False
                     res.t_init += time.perf_counter() - t0
                 if skip:
@@ -212,7 +222,7 @@ def run(inp: Union[str, StructuralInput], *, outdir: str = ".", trj_file: Option
                 if len(itype) != res.nsites:
                     raise T.TrjError(T.TRJ_EINVAL, f"frame {ncstart} holds {len(itype)} selected atoms, "
                                                    f"the first frame read {res.nsites}")
-                ctx.frame(pos, itype, box, what=what, layout=T.TRJ_SOA)
+                ctx.frame(pos, itype, box, what=what, layout=T.TRJ_SOA, charge=qch)
                 res.frames_processed += 1
                 if inp.run_rdf and i == 1:  # rdf.cuf:344-352 (Iconf == 1)
                     t0 = time.perf_counter()
@@ -225,9 +235,13 @@ def run(inp: Union[str, StructuralInput], *, outdir: str = ".", trj_file: Option
             # print_results (util.f90:280-300)
             if inp.run_sq:
                 res.sq_shells = ctx.sq_shells()
+                if res.charges:
+                    res.sq_qq = ctx.sq_qq()
                 res.files += mod_sq.printSQ(res.nsites, nconf=nconf, outdir=outdir)
             if inp.run_rdf:
                 res.rdf_counts = ctx.rdf_counts()
+                if res.charges:
+                    res.rdf_qq = ctx.rdf_qq()
                 res.files += mod_rdf.printrdf(0.0, res.lsmax, nconf=nconf, outdir=outdir)
             res.t_total = time.perf_counter() - t_start
             res.t_files = time.perf_counter() - t_files

@@ -13,7 +13,8 @@ module trj_hip
   integer(c_int32_t), parameter :: TRJ_QSET_REF = 0, TRJ_QSET_HALFSPACE = 1
   integer(c_int32_t), parameter :: TRJ_PREC_FP64 = 0, TRJ_PREC_FP32 = 1, &
                                    TRJ_PREC_FP16X3 = 2, TRJ_PREC_BF16X3 = 3
-  integer(c_int32_t), parameter :: TRJ_SQ_NQ_CONSISTENT = 1
+  integer(c_int32_t), parameter :: TRJ_SQ_NQ_CONSISTENT = 1, TRJ_SQ_CHARGE = 2
+  integer(c_int32_t), parameter :: TRJ_RDF_CHARGE = 1
   integer(c_int), parameter :: TRJ_PIPE_SERIAL = 0, TRJ_PIPE_OVERLAP = 1
   integer(c_int), parameter :: TRJ_STATE_REPLACE = 0, TRJ_STATE_ADD = 1
 
@@ -21,6 +22,7 @@ module trj_hip
     integer(c_int32_t) :: ndim, nsp
     real(c_double) :: deltar, rcrdf
     integer(c_int32_t) :: periodic(3)
+    integer(c_int32_t) :: flags = 0
   end type
 
   type, bind(C) :: trj_sq_params
@@ -44,6 +46,7 @@ module trj_hip
     integer(c_int64_t) :: nframes, natoms
     integer(c_int32_t) :: version, coord_dtype
     integer(c_int32_t) :: has_type, has_cell_origin, has_cell_lengths, has_time
+    integer(c_int32_t) :: has_charge
   end type
 
   interface
@@ -86,6 +89,41 @@ module trj_hip
       integer(c_int64_t), value :: n
       real(c_double), intent(in) :: box(3)
       integer(c_int32_t), value :: what
+    end function
+    ! trj_frame with per-atom charges charge(n) (TRJ_RDF_CHARGE / TRJ_SQ_CHARGE)
+    integer(c_int) function trj_frame_q(ctx, pos, dtype, layout, itype, charge, n, box, what) &
+        bind(C, name="trj_frame_q")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: pos
+      integer(c_int32_t), value :: dtype, layout
+      integer(c_int32_t), intent(in) :: itype(*)
+      real(c_double), intent(in) :: charge(*)
+      integer(c_int64_t), value :: n
+      real(c_double), intent(in) :: box(3)
+      integer(c_int32_t), value :: what
+    end function
+    integer(c_int) function trj_rdf_qq(ctx, acc) bind(C, name="trj_rdf_qq")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: acc(*)
+    end function
+    integer(c_int) function trj_rdf_gqq(ctx, nconf, out) bind(C, name="trj_rdf_gqq")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nconf
+      real(c_double), intent(out) :: out(*)
+    end function
+    integer(c_int) function trj_sq_qq(ctx, sqfq) bind(C, name="trj_sq_qq")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: sqfq(*)
+    end function
+    integer(c_int) function trj_sq_qq_column(ctx, nconf, out) bind(C, name="trj_sq_qq_column")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nconf
+      real(c_double), intent(out) :: out(*)
     end function
     integer(c_int) function trj_sync(ctx) bind(C, name="trj_sync")
       import :: c_int, c_ptr
@@ -207,6 +245,21 @@ module trj_hip
       integer(c_int32_t), intent(in) :: sp_types(*)
       real(c_double), intent(out) :: pos_soa(*)
       integer(c_int32_t), intent(out) :: itype(*)
+      integer(c_int64_t), intent(out) :: n
+      real(c_double), intent(out) :: box(3)
+      integer(c_int32_t), intent(inout) :: periodic(3)
+    end function
+    ! trj_nc_frame_ref plus charge(natoms): the q variable of the same atoms
+    integer(c_int) function trj_nc_frame_ref_q(h, frame, nsp, sp_types, pos_soa, itype, charge, n, box, &
+        periodic) bind(C, name="trj_nc_frame_ref_q")
+      import :: c_int, c_ptr, c_int64_t, c_double, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int64_t), value :: frame
+      integer(c_int32_t), value :: nsp
+      integer(c_int32_t), intent(in) :: sp_types(*)
+      real(c_double), intent(out) :: pos_soa(*)
+      integer(c_int32_t), intent(out) :: itype(*)
+      real(c_double), intent(out) :: charge(*)
       integer(c_int64_t), intent(out) :: n
       real(c_double), intent(out) :: box(3)
       integer(c_int32_t), intent(inout) :: periodic(3)

@@ -9,6 +9,10 @@ replaced by explicit arguments and one module-level context:
   rdf_clear()                        rdf.cuf:124-182
   printrdf(rcl, lsmax, ...)          log.f90:414-596 (gmixsim.dat / gmixsim<k>.dat)
 
+With charges (the reference's ex_qc: the trajectory holds a q variable),
+RDF_init(..., charges=True) adds the charge-weighted histogram and printrdf
+its g_qq column (rdf.cuf:466-472, 270-274; log.f90:466-468, 501-503, 565-575).
+
 Errors raise trjhip.TrjError (the reference prints and stops, rdf.cuf:241-242).
 Histograms are exact int64 counts (the reference accumulates them in FP32,
 rdf.cuf:306, common.cuf:90).
@@ -34,24 +38,28 @@ def context() -> T.Context:
 
 
 def RDF_init(nsp: int, *, ndim: int, deltar: float, rcrdf: float, sidel, periodic=(True, True, True),
-             ctx: Optional[T.Context] = None) -> int:
-    """RDF_init (rdf.cuf:51-122): returns lsmax; sidel is the first frame's box."""
+             charges: bool = False, ctx: Optional[T.Context] = None) -> int:
+    """RDF_init (rdf.cuf:51-122): returns lsmax; sidel is the first frame's box.
+    charges=True also accumulates g_qq (frames then need qcharge)."""
     c = ctx or context()
-    st = c.rdf_init(T.rdf_params(ndim, nsp, deltar, rcrdf, periodic), sidel)
+    flags = T.TRJ_RDF_CHARGE if charges else 0
+    st = c.rdf_init(T.rdf_params(ndim, nsp, deltar, rcrdf, periodic, flags=flags), sidel)
     _state.update(ctx=c, lsmax=st.lsmax, nit=st.nit, nsp=nsp, ndim=ndim, deltar=deltar,
-                  rmingr=None, nframes=0)
+                  rmingr=None, nframes=0, charges=charges)
     return st.lsmax
 
 
-def RDFcomp(Nsites: int, Iconf: int, r: np.ndarray, itype: np.ndarray, sidel, units: str = "lj"):
+def RDFcomp(Nsites: int, Iconf: int, r: np.ndarray, itype: np.ndarray, sidel, units: str = "lj",
+            qcharge: Optional[np.ndarray] = None):
     """RDFcomp (rdf.cuf:184-353) for one frame: r is (Nsites, ndim) (or the
-    reference's r(ndim,N) transposed), itype 1..nsp.  On the first frame the
-    suggested rcl of rdf.cuf:344-352 is computed and kept in rdf_rmingr()."""
+    reference's r(ndim,N) transposed), itype 1..nsp, qcharge the atoms'
+    charges (required after RDF_init(..., charges=True)).  On the first frame
+    the suggested rcl of rdf.cuf:344-352 is computed and kept in rdf_rmingr()."""
     c = _state["ctx"]
     r = np.asarray(r)
     if r.shape[0] != Nsites:
         r = r.T
-    c.frame(np.ascontiguousarray(r), itype, sidel, what=T.TRJ_WANT_RDF)
+    c.frame(np.ascontiguousarray(r), itype, sidel, what=T.TRJ_WANT_RDF, charge=qcharge)
     _state["nframes"] += 1
     if Iconf == 1:
         _state["rmingr"] = c.rdf_first_minimum()
@@ -98,15 +106,23 @@ def printrdf(rcl: float = 0.0, lsmax: Optional[int] = None, nconf: int = 0, outd
     c = _state["ctx"]
     nsp, nit = _state["nsp"], _state["nit"]
     g = c.rdf_gmix(nconf)
+    qq = _state.get("charges", False)
+    gqq = c.rdf_gqq(nconf) if qq else None
+    if qq and len(gqq) != len(g):
+        raise T.TrjError(T.TRJ_ESTATE, "g_qq and g_ab row counts differ")
+    qhead = "g_qq(r)" + " " * 9 if qq else ""
     paths = []
     if nsp <= 6:
         pairs = [(j, k) for j in range(1, nsp + 1) for k in range(j, nsp + 1)]
-        head = "#       r        " + " " * 9 + "".join(f"g_{j}{k}(r)" + " " * 9 for j, k in pairs)
+        head = "#       r        " + " " * 9 + qhead + "".join(f"g_{j}{k}(r)" + " " * 9 for j, k in pairs)
         path = os.path.join(outdir, "gmixsim.dat")
         with open(path, "w") as f:
             f.write(head + "\n")
-            for row in g:
-                f.write(_fmt_records(list(row), 16, 5) + "\n")
+            for i, row in enumerate(g):
+                if qq:   # log.f90:565-566: '(16f16.6)' ri, gchch, gmix
+                    f.write(_fmt_records([row[0], gqq[i]] + list(row[1:]), 16, 6) + "\n")
+                else:
+                    f.write(_fmt_records(list(row), 16, 5) + "\n")
         paths.append(path)
     else:
         full = np.zeros((len(g), nsp, nsp))
@@ -117,11 +133,13 @@ def printrdf(rcl: float = 0.0, lsmax: Optional[int] = None, nconf: int = 0, outd
                 full[:, k, j] = g[:, col]
                 col += 1
         for k in range(1, nsp + 1):
-            head = "#       r        " + " " * 9 + "".join(f"g_{k}{j}(r)" + " " * 9 for j in range(1, nsp + 1))
+            lead = qhead if k == 1 else ""   # log.f90:474-476, 572-573: g_qq in the first file
+            head = "#       r        " + " " * 9 + lead + "".join(f"g_{k}{j}(r)" + " " * 9 for j in range(1, nsp + 1))
             path = os.path.join(outdir, f"gmixsim{k}.dat")
             with open(path, "w") as f:
                 f.write(head + "\n")
                 for i, row in enumerate(g):
-                    f.write(_fmt_records([row[0]] + list(full[i, k - 1]), 16, 5) + "\n")
+                    extra = [gqq[i]] if (qq and k == 1) else []
+                    f.write(_fmt_records([row[0]] + extra + list(full[i, k - 1]), 16, 5) + "\n")
             paths.append(path)
     return paths

@@ -9,6 +9,10 @@ writer of src/core/log.f90:
   printSQ(Nsites, ...)               log.f90:317-412 (sq.dat, sqmix.dat)
   sq_clear()                         sq.cuf:774
 
+With flags=TRJ_SQ_CHARGE (the reference's ex_qc) the charge channel S_qq is
+summed too (sq.cuf:424-432, 563-582, 143-145) and printSQ writes its column
+after S_NN (log.f90:351-360, 385-393).
+
 Values of the reference's FP32 bookkeeping (dq, nqmax, nqmin, lmax, qmin after
 the lattice adjust) are returned by sq_init as the reference computes them.
 """
@@ -32,17 +36,19 @@ def sq_init(Nsites: int, nsp: int, *, ndim: int, qmax: float, qmin: Optional[flo
     (dq, nqmax, nqmin, lmax, qmin_eff)."""
     c = ctx or context()
     info = c.sq_init(T.sq_params(ndim, nsp, qmax, qmin, bsc, qset, grid, precision, flags), sidel)
-    _state.update(ctx=c, info=info, nsp=nsp, Nsites=Nsites, nframes=0)
+    _state.update(ctx=c, info=info, nsp=nsp, Nsites=Nsites, nframes=0,
+                  charges=bool(flags & T.TRJ_SQ_CHARGE))
     return info
 
 
-def SQcalc(r: np.ndarray, itype: np.ndarray, sidel):
-    """SQcalc (sq.cuf:58-126) for one frame: r (Nsites, ndim), itype 1..nsp."""
+def SQcalc(r: np.ndarray, itype: np.ndarray, sidel, qcharge: Optional[np.ndarray] = None):
+    """SQcalc (sq.cuf:58-126) for one frame: r (Nsites, ndim), itype 1..nsp,
+    qcharge the atoms' charges (required with TRJ_SQ_CHARGE)."""
     c = _state["ctx"]
     r = np.asarray(r)
     if r.shape[0] != len(itype):
         r = r.T
-    c.frame(np.ascontiguousarray(r), itype, sidel, what=T.TRJ_WANT_SQ)
+    c.frame(np.ascontiguousarray(r), itype, sidel, what=T.TRJ_WANT_SQ, charge=qcharge)
     _state["nframes"] += 1
 
 
@@ -64,18 +70,22 @@ def printSQ(Nsites: int, nconf: int = 0, outdir: str = ".") -> list:
     c = _state["ctx"]
     nsp = _state["nsp"]
     cols, mix = c.sq_columns(nconf)
+    if _state.get("charges", False):   # S_qq after S_NN (log.f90:351-360, 385-393)
+        cols = np.insert(cols, 2, c.sq_qq_column(nconf), axis=1)
+        qh = "          S_qq(Q)"
+    else:
+        qh = ""
     p_sq = os.path.join(outdir, "sq.dat")
     p_mix = os.path.join(outdir, "sqmix.dat")
+    nv = cols.shape[1] - 1
     with open(p_sq, "w") as f:
-        if cols.shape[1] == 7:
-            f.write("#           Q        S_NN(Q)          S_cc(Q)       S_11(Q)        S_22(Q)"
+        if nv >= 6:
+            f.write("#           Q        S_NN(Q)" + qh + "          S_cc(Q)       S_11(Q)        S_22(Q)"
                     "           S_12(Q)         n(Q)\n")
-            for row in cols:
-                f.write("".join(f"{v:15.7f}" for v in row[:6]) + f"{int(row[6]):12d}\n")
         else:
-            f.write("#           Q       S_NN(Q)          n(Q)\n")
-            for row in cols:
-                f.write("".join(f"{v:15.7f}" for v in row[:2]) + f"{int(row[2]):12d}\n")
+            f.write("#           Q       S_NN(Q)" + qh + "          n(Q)\n")
+        for row in cols:
+            f.write("".join(f"{v:15.7f}" for v in row[:nv]) + f"{int(row[nv]):12d}\n")
     with open(p_mix, "w") as f:
         f.write("#       Q" + " " * 14 + "".join(f"S_{j}{j}(Q)" + " " * 9 for j in range(1, nsp + 1)) + "\n")
         for row in mix:

@@ -34,12 +34,12 @@ PRECISIONS = {"fp64": TRJ_PREC_FP64, "fp32": TRJ_PREC_FP32, "fp16x3": TRJ_PREC_F
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTS = [
     "trj_abi_version", "trj_create", "trj_destroy", "trj_last_error", "trj_rdf_init",
-    "trj_sq_init", "trj_sq_plan", "trj_rdf_plan", "trj_frame", "trj_frame_device", "trj_sync",
+    "trj_sq_init", "trj_sq_plan", "trj_rdf_plan", "trj_frame", "trj_frame_device", "trj_frame_device_q", "trj_sync",
     "trj_rdf_counts", "trj_rdf_gmix", "trj_rdf_first_minimum", "trj_sq_shells", "trj_sq_columns",
     "trj_sq_rho", "trj_frame_q", "trj_rdf_qq", "trj_rdf_gqq", "trj_sq_qq", "trj_sq_qq_column", "trj_reset", "trj_last_timings", "trj_timing_totals", "trj_timing_reset",
     "trj_device_bytes", "trj_set_pipeline", "trj_state_bytes", "trj_state_save", "trj_state_load",
     "trj_lds_atomic_rate", "trj_mfma_rate", "trj_nc_open", "trj_nc_close", "trj_nc_last_error", "trj_nc_info", "trj_nc_read",
-    "trj_nc_time", "trj_nc_frame_ref",
+    "trj_nc_time", "trj_nc_frame_ref", "trj_nc_frame_ref_q",
     "trj_batch_create", "trj_batch_destroy", "trj_batch_last_error",
     "trj_batch_layout", "trj_batch_run", "trj_batch_autotune", "trj_srv_create", "trj_srv_destroy",
     "trj_srv_last_error", "trj_request_footprint", "trj_srv_submit", "trj_srv_poll", "trj_srv_result",
@@ -121,6 +121,7 @@ def _declare(L):
     L.trj_sq_qq.argtypes = [vp, d]
     L.trj_sq_qq_column.argtypes = [vp, C.c_int64, d]
     L.trj_frame_device.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, C.c_int64, d, u32]
+    L.trj_frame_device_q.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, vp, C.c_int64, d, u32]
     L.trj_sync.argtypes = [vp]
     L.trj_rdf_counts.argtypes = [vp, i64]
     L.trj_rdf_gmix.argtypes = [vp, C.c_int64, d]
@@ -150,6 +151,7 @@ def _declare(L):
     L.trj_mfma_rate.argtypes = [C.c_int, C.c_int, d, d]
     L.trj_nc_time.argtypes = [vp, C.c_int64, d]
     L.trj_nc_frame_ref.argtypes = [vp, C.c_int64, C.c_int32, i32, d, i32, i64, d, i32]
+    L.trj_nc_frame_ref_q.argtypes = [vp, C.c_int64, C.c_int32, i32, d, i32, d, i64, d, i32]
     L.trj_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(BatchParams)]
     L.trj_batch_destroy.argtypes = [vp]
     L.trj_batch_destroy.restype = None
@@ -182,7 +184,7 @@ def _declare(L):
 class NcFileInfo(C.Structure):
     _fields_ = [("nframes", C.c_int64), ("natoms", C.c_int64), ("version", C.c_int32),
                 ("coord_dtype", C.c_int32), ("has_type", C.c_int32), ("has_cell_origin", C.c_int32),
-                ("has_cell_lengths", C.c_int32), ("has_time", C.c_int32)]
+                ("has_cell_lengths", C.c_int32), ("has_time", C.c_int32), ("has_charge", C.c_int32)]
 
 
 class NcTrajectory:
@@ -205,6 +207,7 @@ class NcTrajectory:
         self.has_cell_origin = bool(info.has_cell_origin)
         self.has_cell_lengths = bool(info.has_cell_lengths)
         self.has_time = bool(info.has_time)
+        self.has_charge = bool(info.has_charge)
 
     def _check(self, rc, what):
         if rc != TRJ_OK:
@@ -225,19 +228,30 @@ class NcTrajectory:
         self._check(self._L.trj_nc_time(self._h, int(frame), C.byref(t)), "trj_nc_time")
         return t.value
 
-    def frame_ref(self, frame: int, sp_types, periodic=(True, True, True)):
+    def frame_ref(self, frame: int, sp_types, periodic=(True, True, True), charges: bool = False):
         """(pos SoA (3, n), itype (n,), box (3,), periodic (3,)) as read_nc_cfg hands
-        them on; periodic in is the /INPUT/ flags, out has zero-length axes cleared."""
+        them on; periodic in is the /INPUT/ flags, out has zero-length axes cleared.
+        With charges=True a fifth item, the atoms' charges q (n,), is returned
+        (trj_nc_frame_ref_q)."""
         st = np.ascontiguousarray(np.asarray(sp_types, dtype=np.int32))
         pos = np.empty(3 * self.natoms)
         it = np.empty(self.natoms, dtype=np.int32)
         n = C.c_int64()
         box = np.zeros(3)
         per = np.array([1 if p else 0 for p in periodic][:3], dtype=np.int32)
-        self._check(self._L.trj_nc_frame_ref(self._h, int(frame), len(st), _ptr(st, C.c_int32),
-                                             _ptr(pos, C.c_double), _ptr(it, C.c_int32), C.byref(n),
-                                             _ptr(box, C.c_double), _ptr(per, C.c_int32)), "trj_nc_frame_ref")
+        if charges:
+            q = np.empty(self.natoms)
+            self._check(self._L.trj_nc_frame_ref_q(self._h, int(frame), len(st), _ptr(st, C.c_int32),
+                                                   _ptr(pos, C.c_double), _ptr(it, C.c_int32),
+                                                   _ptr(q, C.c_double), C.byref(n), _ptr(box, C.c_double),
+                                                   _ptr(per, C.c_int32)), "trj_nc_frame_ref_q")
+        else:
+            self._check(self._L.trj_nc_frame_ref(self._h, int(frame), len(st), _ptr(st, C.c_int32),
+                                                 _ptr(pos, C.c_double), _ptr(it, C.c_int32), C.byref(n),
+                                                 _ptr(box, C.c_double), _ptr(per, C.c_int32)), "trj_nc_frame_ref")
         m = n.value
+        if charges:
+            return pos[:3 * m].reshape(3, m), it[:m], box, per, q[:m]
         return pos[:3 * m].reshape(3, m), it[:m], box, per
 
     def close(self):
@@ -420,11 +434,18 @@ class Context:
                                             _ptr(b, C.c_double), what), "trj_frame_q")
 
     def frame_device(self, pos_ptr: int, dtype: int, layout: int, itype_ptr: int, n: int, box,
-                     what=TRJ_WANT_RDF | TRJ_WANT_SQ):
+                     what=TRJ_WANT_RDF | TRJ_WANT_SQ, charge_ptr: Optional[int] = None):
+        """Device-resident frame (trj_frame_device, or trj_frame_device_q with
+        charge_ptr: n FP64 charges in device memory)."""
         b = _box3(box)
-        self._check(self._L.trj_frame_device(self._h, C.c_void_p(pos_ptr), dtype, layout,
-                                             C.c_void_p(itype_ptr), n, _ptr(b, C.c_double), what),
-                    "trj_frame_device")
+        if charge_ptr is None:
+            self._check(self._L.trj_frame_device(self._h, C.c_void_p(pos_ptr), dtype, layout,
+                                                 C.c_void_p(itype_ptr), n, _ptr(b, C.c_double), what),
+                        "trj_frame_device")
+        else:
+            self._check(self._L.trj_frame_device_q(self._h, C.c_void_p(pos_ptr), dtype, layout,
+                                                   C.c_void_p(itype_ptr), C.c_void_p(charge_ptr), n,
+                                                   _ptr(b, C.c_double), what), "trj_frame_device_q")
 
     def sync(self):
         self._check(self._L.trj_sync(self._h), "trj_sync")
