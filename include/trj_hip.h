@@ -337,6 +337,48 @@ int trj_nc_frame_ref_q(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_
                        int32_t* itype, double* charge, int64_t* n_out, double box[3], int32_t periodic[3]);
 
 /* ------------------------------------------------------------------------
+ * Confined slices: the reference's twoDstruc_3D analysis (input.f90:166-191:
+ * rdf_sq_cl_dyn_sqw_conf_ord(6) with RDF or S(Q); /INPUT_CONF/ zslice, zgrid).
+ * Atoms are assigned to the first nearest zslice when |zslice - z| <= zgrid/2
+ * (FP32, as the reference; rdf.cuf:692-696); z is the coordinate as given,
+ * never wrapped or range-checked: a caller with a periodic z axis wraps it
+ * first, as select_ncdfinput does (r - L int(r/L), netcdf.f90:679-682).  Call trj_set_slices before
+ * trj_rdf_init / trj_sq_init; nslice = 0 turns slices off.  With slices:
+ *  - the RDF is rdfxy (rdf.cuf:672-736): pairs of one slice, xy distance with
+ *    the image rule on x and y (both must be periodic), bin int(r/dr) + 1;
+ *    each frame's slice histogram is divided by countp_j * countp_l, the
+ *    slice's species counts without the last atom of the species-sorted order
+ *    (rdf.cuf:284-296, 688); charges add g_qq per slice, divided by
+ *    count^2 times the slice's mean q^2;
+ *  - S(Q) is sqf3xy / sqf2xy (sq.cuf:596-771): the 2D q-set in the xy plane
+ *    (the plan of ndim = 2 on the frame's Lx, Ly), per slice, each vector's
+ *    |rho|^2 divided by the slice's (species') atom count; S_qq per slice is
+ *    divided at output by the slice's mean q^2 of the last frame
+ *    (sq.cuf:135-138);
+ *  - the unsliced result calls and the state blob return TRJ_ESTATE.
+ * Arrays are slice-major: hxy/counts [nslice][nit][lsmax]. */
+#define TRJ_MAX_SLICES 16
+int trj_set_slices(trj_ctx* ctx, int32_t nslice, const float* zslice, float zgrid);
+/* The last frame's slice counts (rdfxy rule): count[nslice], countp[nslice][nsp]. */
+int trj_slice_counts(trj_ctx* ctx, int64_t* count, int64_t* countp);
+/* histomix_xy (normalised per frame) and the raw pair counts; either may be NULL. */
+int trj_rdf_xy(trj_ctx* ctx, double* hxy, int64_t* counts);
+/* gxy_j-l.dat tables (log.f90:494-551): out[pair][i-1][slice] for r_i = i dr,
+ * i = 1..lsmax-2, pairs in ind_ij order; nconf <= 0: frames processed. */
+int trj_rdf_xy_gmix(trj_ctx* ctx, int64_t nconf, double* out);
+/* Charge-weighted slice accumulators gqqxy [nslice][lsmax] (TRJ_RDF_CHARGE). */
+int trj_rdf_xy_qq(trj_ctx* ctx, double* acc);
+/* gxy_cc.dat table: out[i-1][slice] = V (zgrid/Lz) gqqxy / (dV Nconf). */
+int trj_rdf_xy_gcc(trj_ctx* ctx, int64_t nconf, double* out);
+/* Slice shell sums: sqf [nslice][nqmax+1], sqfp [nslice][nsp][nqmax+1],
+ * sqfq [nslice][nqmax+1] (TRJ_SQ_CHARGE); any may be NULL. */
+int trj_sq_xy(trj_ctx* ctx, double* sqf, double* sqfp, double* sqfq);
+/* sqxy.dat / sqpxy_j-j.dat / sqxy_qq.dat tables (log.f90:333-378): rows
+ * i = 1..nqmax of (Q, one value per slice): sqxy [nqmax][1+nslice], sqpxy
+ * [nsp][nqmax][1+nslice], sqqxy [nqmax][1+nslice] (may be NULL). */
+int trj_sq_xy_columns(trj_ctx* ctx, int64_t nconf, double* sqxy, double* sqpxy, double* sqqxy);
+
+/* ------------------------------------------------------------------------
  * Batched frames: many independent frames of uneven size and species count,
  * each with its own pair histogram (the frame loop of trj_analysis.f90:173-242
  * with RDFcomp per frame, rdf.cuf:184-353), run as groups of frames in

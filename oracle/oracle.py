@@ -102,6 +102,19 @@ def _declare(L):
     L.or_rdf_gqq.restype = None
     L.or_sq_frame_q.argtypes = [C.c_int64, _D, _D, _D, _D, C.c_int, _D, C.POINTER(SqGrid), _D]
     L.or_sq_frame_q.restype = None
+    _F = C.POINTER(C.c_float)
+    L.or_slice_of.argtypes = [C.c_float, C.c_int, _F, C.c_float]
+    L.or_slice_of.restype = C.c_int
+    L.or_rdf_xy_brute.argtypes = [C.c_int64, _D, _D, _D, _I32, C.c_int, _D, C.c_double, C.c_double, C.c_int,
+                                  C.c_int, _F, C.c_float, _I64, _I64, _I64, _D, C.c_double, _I64]
+    L.or_rdf_xy_brute.restype = None
+    L.or_rdf_xy_accumulate.argtypes = [C.c_int, C.c_int, C.c_int, _I64, _I64, _D]
+    L.or_rdf_xy_accumulate.restype = None
+    L.or_rdf_xy_gmix.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _D, C.c_double, C.c_int64, _D, _D]
+    L.or_rdf_xy_gmix.restype = None
+    L.or_sq_frame_xy.argtypes = [C.c_int64, _D, _D, _D, _I32, _D, C.c_int, _D, C.POINTER(SqGrid), _D,
+                                 C.c_int, _F, C.c_float, _D, _D, _D]
+    L.or_sq_frame_xy.restype = None
 
 
 def _p(a, ctype):
@@ -317,3 +330,91 @@ def sq_frame_q(pos, q, ndim, box, g: SqGrid, sqfq=None):
     lib().or_sq_frame_q(len(qq), _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double),
                         _p(qq, C.c_double), ndim, _p(b, C.c_double), C.byref(g), _p(sqfq, C.c_double))
     return sqfq
+
+
+# ------------------------------------------------------------------ slices
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def slice_of(z, zslice, zgrid) -> int:
+    zs = _f32(zslice)
+    return lib().or_slice_of(float(np.float32(z)), len(zs), _p(zs, C.c_float), float(np.float32(zgrid)))
+
+
+def rdf_xy_counts(pos, itype, nsp, box, rcrdf, deltar, zslice, zgrid, lsmax=None, q=None, scale=None):
+    """One frame of rdfxy: (counts (nslice, nit, lsmax) int64, count (nslice,),
+    countp (nslice, nsp), fixed (nslice, lsmax) or None, scale).  pos (N,3) as
+    ingested (x, y wrapped; z as given); box is the frame's box (Lz enters the
+    cutoff cap as in 3D)."""
+    pos = np.asarray(pos, dtype=np.float64)
+    g = geometry(3, box, (1, 1, 0), rcrdf)
+    if lsmax is None:
+        lsmax = rdf_lsmax(3, box, rcrdf, deltar)
+    zs = _f32(zslice)
+    ns = len(zs)
+    nit = nsp * (nsp + 1) // 2
+    x, y, z = (np.ascontiguousarray(pos[:, k]) for k in range(3))
+    it = _i32(itype)
+    counts = np.zeros((ns, nit, lsmax), dtype=np.int64)
+    count = np.zeros(ns, dtype=np.int64)
+    countp = np.zeros((ns, nsp), dtype=np.int64)
+    lxy = _f64([box[0], box[1]])
+    fixed = None
+    qq = None
+    if q is not None:
+        qq = _f64(q)
+        if scale is None:
+            scale = qq_scale(qq)
+        fixed = np.zeros((ns, lsmax), dtype=np.int64)
+    lib().or_rdf_xy_brute(len(it), _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double), _p(it, C.c_int32),
+                          nsp, _p(lxy, C.c_double), g.side2, deltar, lsmax, ns, _p(zs, C.c_float),
+                          float(np.float32(zgrid)), _p(counts, C.c_int64), _p(count, C.c_int64),
+                          _p(countp, C.c_int64), _p(qq, C.c_double) if qq is not None else None,
+                          float(scale) if scale is not None else 0.0,
+                          _p(fixed, C.c_int64) if fixed is not None else None)
+    return counts, count, countp, fixed, scale
+
+
+def rdf_xy_accumulate(nsp, counts, countp, hxy=None):
+    counts = _i64(counts)
+    countp = _i64(countp)
+    ns, nit, ls = counts.shape
+    if hxy is None:
+        hxy = np.zeros((ns, nsp, nsp, ls))
+    lib().or_rdf_xy_accumulate(nsp, ls, ns, _p(counts, C.c_int64), _p(countp, C.c_int64), _p(hxy, C.c_double))
+    return hxy
+
+
+def rdf_xy_gmix(nsp, lsmax, deltar, box, zgrid, nconf, hxy):
+    """(npairs, lsmax-2, nslice) g_xy tables, pairs in ind_ij order."""
+    h = _f64(hxy)
+    ns = h.shape[0]
+    nit = nsp * (nsp + 1) // 2
+    out = np.zeros((nit, max(lsmax - 2, 0), ns))
+    b = _f64(box)
+    lib().or_rdf_xy_gmix(nsp, lsmax, ns, deltar, _p(b, C.c_double), float(np.float32(zgrid)), nconf,
+                         _p(h, C.c_double), _p(out, C.c_double))
+    return out
+
+
+def sq_frame_xy(pos, itype, nsp, box, g: SqGrid, bsc, zslice, zgrid, q=None, acc=None):
+    """Accumulate one frame of sqf3xy/sqf2xy; returns (sqf, sqfp, sqfq) with
+    shapes (nslice, nqmax+1), (nslice, nsp, nqmax+1), (nslice, nqmax+1)."""
+    pos = np.asarray(pos, dtype=np.float64)
+    x, y, z = (np.ascontiguousarray(pos[:, k]) for k in range(3))
+    it = _i32(itype)
+    zs = _f32(zslice)
+    ns = len(zs)
+    if acc is None:
+        acc = (np.zeros((ns, g.nqmax + 1)), np.zeros((ns, nsp, g.nqmax + 1)), np.zeros((ns, g.nqmax + 1)))
+    sqf, sqfp, sqfq = acc
+    b = _f64(box)
+    bs = _f64(bsc)
+    qq = _f64(q) if q is not None else None
+    lib().or_sq_frame_xy(len(it), _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double), _p(it, C.c_int32),
+                         _p(qq, C.c_double) if qq is not None else None, nsp, _p(b, C.c_double), C.byref(g),
+                         _p(bs, C.c_double), ns, _p(zs, C.c_float), float(np.float32(zgrid)),
+                         _p(sqf, C.c_double), _p(sqfp, C.c_double), _p(sqfq, C.c_double))
+    return acc
+

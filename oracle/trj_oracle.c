@@ -618,3 +618,229 @@ int or_sq_columns(int nsp, int nqmax, float dq, int64_t natoms, const int64_t* n
   }
   return ncols;
 }
+
+/* ------------------------------------------------------------------ slices */
+
+/* Slice of a z coordinate (rdf.cuf:692-696, sq.cuf:640-643, common.cuf:225-226),
+ * in FP32 as the reference: the first zslice nearest to z (minloc), and -1
+ * unless |zslice - z| <= zgrid/2. */
+int or_slice_of(float z, int nslice, const float* zslice, float zgrid) {
+  int best = 0;
+  float bd = fabsf(zslice[0] - z);
+  for (int s = 1; s < nslice; ++s) {
+    float d = fabsf(zslice[s] - z);
+    if (d < bd) {
+      bd = d;
+      best = s;
+    }
+  }
+  return (bd <= zgrid / 2.0f) ? best : -1;
+}
+
+/* Index of the last atom of the reference's species-sorted order (species
+ * 1..nsp, file order within a species): the last atom of the highest species
+ * present.  rdfxy's thread loop (rdf.cuf:688) stops before it. */
+static int64_t last_in_species_order(int64_t n, const int32_t* itype) {
+  int64_t last = -1;
+  int32_t ts = 0;
+  for (int64_t i = 0; i < n; ++i)
+    if (itype[i] >= ts) {
+      ts = itype[i];
+      last = i;
+    }
+  return last;
+}
+
+/* rdfxy (rdf.cuf:672-736): pairs i<j of one slice, xy distance with the
+ * nint image on x and y (lxy), r^2 < side2, bin k = int(r/deltar) + 1 in
+ * FP64 here, 1 <= k <= lsmax.  counts[(sl*nit + ij-1)*lsmax + k-1]; count[sl]
+ * and countp[sl*nsp + s-1] are the slice members without the last atom of
+ * the species-sorted order (rdf.cuf:688, 699-700).  With q (may be NULL)
+ * fixed[sl*lsmax + k-1] += llrint(q_i q_j scale). */
+void or_rdf_xy_brute(int64_t n, const double* x, const double* y, const double* z,
+                     const int32_t* itype, int nsp, const double lxy[2], double side2, double deltar,
+                     int lsmax, int nslice, const float* zslice, float zgrid, int64_t* counts,
+                     int64_t* count, int64_t* countp, const double* q, double scale, int64_t* fixed) {
+  int nit = nsp * (nsp + 1) / 2;
+  int* sl = (int*)malloc(sizeof(int) * (n > 0 ? n : 1));
+  for (int64_t i = 0; i < n; ++i) sl[i] = or_slice_of((float)z[i], nslice, zslice, zgrid);
+  int64_t last = last_in_species_order(n, itype);
+  for (int64_t i = 0; i < n; ++i) {
+    if (sl[i] < 0 || i == last) continue;
+    count[sl[i]] += 1;
+    countp[(int64_t)sl[i] * nsp + itype[i] - 1] += 1;
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    if (sl[i] < 0) continue;
+    for (int64_t j = i + 1; j < n; ++j) {
+      if (sl[j] != sl[i]) continue;
+      double xd = x[j] - x[i], yd = y[j] - y[i];
+      xd = xd - lxy[0] * round(xd / lxy[0]);
+      yd = yd - lxy[1] * round(yd / lxy[1]);
+      double r2 = xd * xd + yd * yd;
+      if (!(r2 < side2)) continue;
+      long k = (long)(sqrt(r2) / deltar) + 1;
+      if (k < 1 || k > lsmax) continue;
+      int ij = or_ind_ij(itype[i], itype[j], nsp);
+      counts[((int64_t)sl[i] * nit + ij - 1) * lsmax + k - 1] += 1;
+      if (q) fixed[(int64_t)sl[i] * lsmax + k - 1] += llrint(q[i] * q[j] * scale);
+    }
+  }
+  free(sl);
+}
+
+/* rdf.cuf:284-296 per frame, in FP64: hxy[sl][j][l][k] += counts / (countp_j countp_l)
+ * for j <= l with both counts positive (hxy: [nslice][nsp][nsp][lsmax], j <= l
+ * filled). */
+void or_rdf_xy_accumulate(int nsp, int lsmax, int nslice, const int64_t* counts, const int64_t* countp,
+                          double* hxy) {
+  int nit = nsp * (nsp + 1) / 2;
+  for (int sl = 0; sl < nslice; ++sl) {
+    int ij = 0;
+    for (int j = 0; j < nsp; ++j)
+      for (int l = j; l < nsp; ++l, ++ij) {
+        double cj = (double)countp[(int64_t)sl * nsp + j], cl = (double)countp[(int64_t)sl * nsp + l];
+        if (!(cj > 0.0 && cl > 0.0)) continue;
+        for (int k = 0; k < lsmax; ++k)
+          hxy[(((int64_t)sl * nsp + j) * nsp + l) * lsmax + k] +=
+              (double)counts[((int64_t)sl * nit + ij) * lsmax + k] / (cj * cl);
+      }
+  }
+}
+
+/* log.f90:494-496, 511-514, 545-551: g_xy for pair (j, l), j <= l, and
+ * r_i = i deltar, i = 1..lsmax-2: (j/l + 1) V (zgrid/Lz) hxy / (dV Nconf),
+ * dV = pi ((r+dr/2)^2 - (r-dr/2)^2) zgrid; the Fortran j/l + 1 is 2 for
+ * j = l and 1 otherwise.  out[(pair*(lsmax-2) + i-1)*nslice + sl], pairs in
+ * ind_ij order. */
+void or_rdf_xy_gmix(int nsp, int lsmax, int nslice, double deltar, const double box[3], double zgrid,
+                    int64_t nconf, const double* hxy, double* out) {
+  const double pi = 3.14159265358979323846;
+  double v = box[0] * box[1] * box[2];
+  int pair = 0;
+  for (int j = 0; j < nsp; ++j)
+    for (int l = j; l < nsp; ++l, ++pair)
+      for (int i = 1; i <= lsmax - 2; ++i) {
+        double ri = i * deltar, a = ri + deltar / 2.0, b = ri - deltar / 2.0;
+        double dv = pi * (a * a - b * b) * zgrid;
+        double f = (j == l) ? 2.0 : 1.0;
+        for (int sl = 0; sl < nslice; ++sl)
+          out[((int64_t)pair * (lsmax - 2) + i - 1) * nslice + sl] =
+              f * v * (zgrid / box[2]) * hxy[(((int64_t)sl * nsp + j) * nsp + l) * lsmax + i - 1] /
+              (dv * (double)nconf);
+      }
+}
+
+/* The slice sum set: the 2D q-set (sq.cuf:89 sqf3xy when nqmin > 1, kx, ky
+ * >= 0) and the x and y axis vectors k = nqmin+1 .. nqmax of sqf2xy
+ * (sq.cuf:91-92, 114-116: also from nqmin+1 when nqmin <= 1, so shell 1
+ * gets no vector when nqmin = 1). */
+static int64_t sum_set_xy(const or_sqgrid* g, int32_t** kv_out, int32_t** sh_out) {
+  int64_t cap = 1024, nv = 0;
+  int32_t* kv = (int32_t*)malloc(sizeof(int32_t) * 2 * cap);
+  int32_t* sh = (int32_t*)malloc(sizeof(int32_t) * cap);
+#define PUSH2(a, b, s)                                                \
+  do {                                                                \
+    if (nv == cap) {                                                  \
+      cap *= 2;                                                       \
+      kv = (int32_t*)realloc(kv, sizeof(int32_t) * 2 * cap);          \
+      sh = (int32_t*)realloc(sh, sizeof(int32_t) * cap);              \
+    }                                                                 \
+    kv[2 * nv] = (a);                                                 \
+    kv[2 * nv + 1] = (b);                                             \
+    sh[nv++] = (s);                                                   \
+  } while (0)
+  if (g->nqmin > 1)
+    for (int a = 0; a < g->kext[0]; ++a)
+      for (int b = 0; b < g->kext[1]; ++b) {
+        int s = or_sq_member(2, g, a, b, 0);
+        if (s >= 0) PUSH2(a, b, s);
+      }
+  /* nqmin can be negative after the 16-adjust; the reference's axis loop
+   * then starts below 1, and only k >= 1 lands in a shell. */
+  int kmin = g->nqmin + 1 > 1 ? g->nqmin + 1 : 1;
+  for (int k = kmin; k <= g->nqmax; ++k) {
+    PUSH2(k, 0, k);
+    PUSH2(0, k, k);
+  }
+#undef PUSH2
+  *kv_out = kv;
+  *sh_out = sh;
+  return nv;
+}
+
+/* sqf3xy / sqf2xy (sq.cuf:596-771) for one frame, long-double sums: per
+ * slice sl and vector, sqf[sl][shell] += |sum_s bsc_s rho_s|^2 / count_sl,
+ * sqfp[sl][s][shell] += |rho_s|^2 / countsp_sl,s and (q not NULL)
+ * sqfq[sl][shell] += |sum q_j e^{iq.r_j}|^2 / count_sl, over the slice's
+ * members (all atoms, unlike rdfxy).  Phases use x and y only; g is the 2D
+ * grid (or_sq_setup with ndim = 2).  Arrays are [nslice][...][nqmax+1]. */
+void or_sq_frame_xy(int64_t n, const double* x, const double* y, const double* z, const int32_t* itype,
+                    const double* q, int nsp, const double box[3], const or_sqgrid* g, const double* bsc,
+                    int nslice, const float* zslice, float zgrid, double* sqf, double* sqfp, double* sqfq) {
+  const long double twopi = 6.283185307179586476925286766559005768L;
+  int32_t *kv, *sh;
+  int64_t nv = sum_set_xy(g, &kv, &sh);
+  int* sl = (int*)malloc(sizeof(int) * (n > 0 ? n : 1));
+  int64_t cnt[64], cntp[64 * 16];
+  for (int s = 0; s < nslice; ++s) {
+    cnt[s] = 0;
+    for (int t = 0; t < nsp; ++t) cntp[s * nsp + t] = 0;
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    sl[i] = or_slice_of((float)z[i], nslice, zslice, zgrid);
+    if (sl[i] >= 0) {
+      cnt[sl[i]] += 1;
+      cntp[sl[i] * nsp + itype[i] - 1] += 1;
+    }
+  }
+  int ns = g->nqmax + 1;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t v = 0; v < nv; ++v) {
+    long double re[64 * 16], im[64 * 16], qre[64], qim[64];
+    for (int s = 0; s < nslice * nsp; ++s) re[s] = im[s] = 0.0L;
+    for (int s = 0; s < nslice; ++s) qre[s] = qim[s] = 0.0L;
+    int nx = kv[2 * v], ny = kv[2 * v + 1];
+    for (int64_t j = 0; j < n; ++j) {
+      if (sl[j] < 0) continue;
+      long double ux = (long double)x[j] / (long double)box[0];
+      long double uy = (long double)y[j] / (long double)box[1];
+      long double ph = twopi * turns(nx, ny, 0, ux, uy, 0.0L);
+      long double c = cosl(ph), s = sinl(ph);
+      int ch = sl[j] * nsp + itype[j] - 1;
+      re[ch] += c;
+      im[ch] += s;
+      if (q) {
+        qre[sl[j]] += (long double)q[j] * c;
+        qim[sl[j]] += (long double)q[j] * s;
+      }
+    }
+    for (int s = 0; s < nslice; ++s) {
+      double tr = 0.0, ti = 0.0;
+      for (int t = 0; t < nsp; ++t) {
+        double r = (double)re[s * nsp + t], i = (double)im[s * nsp + t];
+        if (cntp[s * nsp + t] > 0) {
+          double p = (r * r + i * i) / (double)cntp[s * nsp + t];
+#pragma omp atomic
+          sqfp[((int64_t)s * nsp + t) * ns + sh[v]] += p;
+        }
+        tr += bsc[t] * r;
+        ti += bsc[t] * i;
+      }
+      if (cnt[s] > 0) {
+        double p = (tr * tr + ti * ti) / (double)cnt[s];
+#pragma omp atomic
+        sqf[(int64_t)s * ns + sh[v]] += p;
+        if (q) {
+          double qr = (double)qre[s], qi = (double)qim[s];
+          double pq = (qr * qr + qi * qi) / (double)cnt[s];
+#pragma omp atomic
+          sqfq[(int64_t)s * ns + sh[v]] += pq;
+        }
+      }
+    }
+  }
+  free(sl);
+  free(kv);
+  free(sh);
+}

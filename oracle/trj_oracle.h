@@ -39,6 +39,17 @@
  *   or_qq_accumulate     hand-made sums (exact arithmetic cases)
  *   or_rdf_gqq           equals or_rdf_gmix's formula with N_j N_l -> N^2 on the
  *                        same histogram (closed form of log.f90:501-503)
+ *   or_slice_of          hand cases: nearest centre, first of two equidistant
+ *                        centres (minloc), the zgrid/2 boundary in FP32
+ *   or_rdf_xy_brute      one slice spanning every atom equals the 2D rule on the
+ *                        xy projection with floor binning (a two-atom closed form,
+ *                        pair totals); slices partition pairs (sum over slices of
+ *                        a split equals the pairs with both atoms in one slice);
+ *                        the last-atom count rule on hand cases
+ *   or_rdf_xy_accumulate hand-made counts
+ *   or_sq_frame_xy       one slice with every atom equals or_sq_frame (ndim 2)
+ *                        divided by N; a single atom per slice gives 1 per vector;
+ *                        unit charges give the bsc = 1 sum
  *   or_sq_frame_q        unit charges equal or_sq_frame's S with bsc = 1;
  *                        uniform charge c scales by c^2; single atom gives q^2
  * None is "parity unpinned".
@@ -160,6 +171,26 @@ void or_sq_frame_q(int64_t n, const double* x, const double* y, const double* z,
 int or_sq_columns(int nsp, int nqmax, float dq, int64_t natoms, const int64_t* ntype,
                   const double* bsc, int64_t nconf, const double* sqf, const double* sqfp,
                   const int64_t* nq, double* cols, double* mix);
+
+/* Confined slices (twoDstruc_3D).  or_slice_of: FP32 nearest slice, -1 when
+ * outside zgrid/2 (rdf.cuf:692-696).  or_rdf_xy_brute: per-slice xy pair
+ * histogram, bin int(r/dr)+1, counts without the species-sorted last atom
+ * (rdf.cuf:672-736).  or_rdf_xy_accumulate: per-frame normalisation by the
+ * slice's species counts (rdf.cuf:284-296).  or_rdf_xy_gmix: gxy tables
+ * (log.f90:494-496, 511-514, 545-551).  or_sq_frame_xy: sqf3xy/sqf2xy
+ * per-slice sums divided per vector by the slice counts (sq.cuf:596-771). */
+int or_slice_of(float z, int nslice, const float* zslice, float zgrid);
+void or_rdf_xy_brute(int64_t n, const double* x, const double* y, const double* z,
+                     const int32_t* itype, int nsp, const double lxy[2], double side2, double deltar,
+                     int lsmax, int nslice, const float* zslice, float zgrid, int64_t* counts,
+                     int64_t* count, int64_t* countp, const double* q, double scale, int64_t* fixed);
+void or_rdf_xy_accumulate(int nsp, int lsmax, int nslice, const int64_t* counts, const int64_t* countp,
+                          double* hxy);
+void or_rdf_xy_gmix(int nsp, int lsmax, int nslice, double deltar, const double box[3], double zgrid,
+                    int64_t nconf, const double* hxy, double* out);
+void or_sq_frame_xy(int64_t n, const double* x, const double* y, const double* z, const int32_t* itype,
+                    const double* q, int nsp, const double box[3], const or_sqgrid* g, const double* bsc,
+                    int nslice, const float* zslice, float zgrid, double* sqf, double* sqfp, double* sqfq);
 
 #ifdef __cplusplus
 }

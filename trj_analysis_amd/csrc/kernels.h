@@ -5,11 +5,20 @@
 
 #include <cstdint>
 
+#include "trj_internal.h"
+
 namespace trj {
 
 // Error bits raised by device kernels, read back at synchronisation points.
 constexpr uint32_t kFlagRange = 1u;  // non-periodic coordinate out of range / non-finite
 constexpr uint32_t kFlagType = 2u;   // species type outside 1..nsp
+
+// Confined slices (twoDstruc_3D): centres and width in FP32, as the reference.
+struct SliceSet {
+  int n;
+  float z[kMaxSlices];
+  float zgrid;
+};
 
 struct IngestArgs {
   const void* raw;       // positions as given
@@ -22,24 +31,34 @@ struct IngestArgs {
   double box[3];
   double* pos;           // out: wrapped FP64 SoA, 3*n
   uint32_t* flags;
+  int free_axis;         // non-periodic axis without the range check (-1: none)
 };
 void launch_ingest(const IngestArgs& a, hipStream_t s);
 
-// Stable partition of atoms by species: per-block counts (species-major), a
-// scan, then a scatter of the S(Q) turns u = x / Lq (hi + lo) and of the unit
-// steps e^{2 pi i u} into species order.  u holds 12 planes of `stride`
-// doubles.  With charges q (may be NULL) the planes hold 2n entries: the n
-// species-sorted atoms, then all atoms again in input order as the charge
-// channel, and a 13th plane of weights (1 for the species entries, q_i for
-// the charge channel); sp_off then has nsp + 2 entries (extra_end = 2n).
-void launch_species_count(const int32_t* itype, int64_t n, int nsp, int32_t* blk, int nblk,
+// Stable partition of atoms by key: per-block counts (key-major), a scan,
+// then a scatter of the S(Q) turns u = x / Lq (hi + lo) and of the unit steps
+// e^{2 pi i u} into key order.  The key is the species (key NULL; nkeys =
+// nsp) or slice * nsp + species (nkeys = nslice * nsp); atoms without a key
+// have no entry.  u holds 12 planes of `stride` doubles.  With charges q (may
+// be NULL) the M sorted entries are written again at M + offset, weighted by
+// q_i in a 13th plane (1 for the first copy): the charge channel of each
+// group of gsize keys; sp_off then has nkeys + 1 + ngroups entries.
+void launch_species_count(const int32_t* itype, const int32_t* key, int64_t n, int nkeys, int32_t* blk, int nblk,
                           hipStream_t s);
 void launch_exclusive_scan(const int32_t* in, int32_t* out, int64_t count, hipStream_t s);
-void launch_species_offsets(const int32_t* blkoff, const int32_t* blk, int nblk, int nsp,
-                            int64_t extra_end, int32_t* sp_off, hipStream_t s);
-void launch_species_scatter(const int32_t* itype, int64_t n, int nsp, const int32_t* blkoff,
-                            int nblk, const double* pos, const double lq[3], int ndim,
-                            const double* q, int64_t stride, double* u, hipStream_t s);
+void launch_species_offsets(const int32_t* blkoff, const int32_t* blk, int nblk, int nkeys, int gsize, int ngroups,
+                            int32_t* sp_off, hipStream_t s);
+void launch_species_scatter(const int32_t* itype, const int32_t* key, int64_t n, int nkeys, const int32_t* blkoff,
+                            int nblk, const double* pos, const double lq[3], int ndim, const double* q,
+                            const int32_t* sp_off, int64_t stride, double* u, hipStream_t s);
+
+// Slices: keys from z (FP32 rule of the reference) and the species-sorted
+// last atom; per-slice counts (without that atom; q: sum of q^2 and count
+// over all members); 2D cells per slice; the per-frame normalisation.
+void launch_slice_keys(const double* pos, int64_t n, const int32_t* itype, int nsp, const SliceSet& ss,
+                       int32_t* key, unsigned long long* last, hipStream_t s);
+void launch_slice_counts(const int32_t* key, int64_t n, const unsigned long long* last, int nsp, int32_t* count,
+                         int32_t* countp, const double* q, double* qsum, int32_t* qcnt, hipStream_t s);
 
 // Cell list.
 struct CellGrid {
@@ -49,6 +68,8 @@ struct CellGrid {
 };
 void launch_cell_count(const double* pos, int64_t n, int ndim, const CellGrid& g,
                        int32_t* cellid, int32_t* count, hipStream_t s);
+void launch_cell_count_slices(const double* pos, const int32_t* key, int64_t n, int nsp, const CellGrid& g,
+                              int nslice, int32_t* cellid, int32_t* count, hipStream_t s);
 // q (may be NULL): per-atom charges, scattered into cell order as sq.
 void launch_cell_scatter(const double* pos, const int32_t* itype, const int32_t* cellid,
                          int64_t n, const int32_t* start, int32_t* cursor, double* spos,
@@ -79,9 +100,17 @@ size_t cells_lds_bytes(int nit, int lsmax, bool qq = false);
 void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start,
                       const double* spos, const int32_t* stype, int64_t n, int waves_per_cell,
                       int nblocks, uint32_t* partial, hipStream_t s);
-// All pairs i<j (small boxes).
+// All pairs i<j (small boxes) of n atoms in SoA planes of `stride` (0: n);
+// range (device, may be NULL): the atoms [range[0], range[1]) instead.
 void launch_rdf_allpairs(const PairArgs& a, const double* pos, const int32_t* itype, int64_t n,
-                         int nblocks, uint32_t* partial, hipStream_t s);
+                         int nblocks, uint32_t* partial, hipStream_t s, int64_t stride = 0,
+                         const int32_t* range = nullptr);
+// Slice histograms of the frame into the accumulators, then cleared:
+// counts += frame; hxy += frame / (countp_j countp_l); gq (may be NULL):
+// gqacc += (gq / qp[0]) / (count^2 qsum/qcnt).
+void launch_xy_finish(int nslice, int nsp, int lsmax, int64_t* frame, const int32_t* count, const int32_t* countp,
+                      int64_t* counts, double* hxy, unsigned long long* gq, const double* qp, const double* qsum,
+                      const int32_t* qcnt, double* gqacc, hipStream_t s);
 // counts[b] += sum over slices of partial[slice][b].
 void launch_rdf_merge(const uint32_t* partial, int nslices, int nbins, int64_t* counts,
                       hipStream_t s);
@@ -107,17 +136,20 @@ void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s);
 // FP32 MFMA, or FP16 / BF16 hi+lo split with three products (precision =
 // TRJ_PREC_FP32 / FP16X3 / BF16X3); FP32 accumulation folded into FP64.
 void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s);
-// Sum the splits, square, and bin into shells (FP64 atomics).
-// With nch = nsp + 1 the last channel's |rho|^2 goes to sqfq.
+// Sum the splits, square, and bin into shells (FP64 atomics).  Channels are
+// nslice groups of nsp species, then (nch > nslice * nsp) one charge channel
+// per slice; outputs sqf [nslice][nqmax+1], sqfp [nslice][nsp][nqmax+1],
+// sqfq [nslice][nqmax+1].  norm (the channel offsets, may be NULL) divides
+// each vector's values by the atom counts (slices).
 void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, int tile_vec, int ksplit,
-                      int nsp, int nch, const double* bsc, int nqmax, double* sqf, double* sqfp,
-                      double* sqfq, hipStream_t s);
+                      int nsp, int nslice, int nch, const double* bsc, int nqmax, const int32_t* norm, double* sqf,
+                      double* sqfp, double* sqfq, hipStream_t s);
 // Explicit q-vectors (axis directions): direct FP64 phase sums over atom
 // blocks, combined in a fixed order; part needs sq_list_scratch_bytes().
 size_t sq_list_scratch_bytes(int nvec, int nch);
-void launch_sq_list(const double* u, const double* w, const int32_t* sp_off, int64_t stride, int nsp,
+void launch_sq_list(const double* u, const double* w, const int32_t* sp_off, int64_t stride, int nsp, int nslice,
                     int nch, int ndim, const int32_t* kv, const int32_t* shell, int nvec, const double* bsc,
-                    int nqmax, double* part, double* sqf, double* sqfp, double* sqfq, hipStream_t s);
+                    int nqmax, bool norm, double* part, double* sqf, double* sqfp, double* sqfq, hipStream_t s);
 
 // LDS-atomic throughput probe (random bins of a 4096-bin uint32 histogram);
 // returns the number of atomics the launch performs; *out gets their sum.

@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void k_ingest(IngestArgs a) {
       bad |= kFlagRange;
     } else if (a.periodic[k]) {
       c[k] = wrap_periodic(c[k], a.box[k]);
-    } else if (c[k] < -0.25 * a.box[k] || c[k] >= 1.25 * a.box[k]) {
+    } else if (k != a.free_axis && (c[k] < -0.25 * a.box[k] || c[k] >= 1.25 * a.box[k])) {
       bad |= kFlagRange;
     }
   }
@@ -72,20 +72,27 @@ __global__ __launch_bounds__(256) void k_ingest(IngestArgs a) {
   a.pos[2 * a.n + i] = c[2];
 }
 
-__global__ __launch_bounds__(256) void k_species_count(const int32_t* __restrict__ itype, int64_t n,
+// The sort key of atom i: key[i] when given (slice * nsp + species, -1 for
+// atoms outside every slice), else its species 0..nsp-1 (-1 when invalid).
+__device__ inline int sort_key(const int32_t* itype, const int32_t* key, int64_t i, int64_t n, int nkeys) {
+  if (i >= n) return -1;
+  int s = key ? key[i] : itype[i] - 1;
+  return (s < 0 || s >= nkeys) ? -1 : s;
+}
+
+__global__ __launch_bounds__(256) void k_species_count(const int32_t* __restrict__ itype,
+                                                       const int32_t* __restrict__ key, int64_t n,
                                                        int nsp, int32_t* __restrict__ blk, int nblk) {
-  __shared__ int cnt[4][kNspMax];
+  __shared__ int cnt[4][kKeyMax];
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  int s = (i < n) ? itype[i] - 1 : -1;
-  if (s >= nsp) s = -1;
+  const int s = sort_key(itype, key, i, n, nsp);
   for (int sp = 0; sp < nsp; ++sp) {
     const unsigned long long m = __ballot(s == sp);
     if (lane == 0) cnt[w][sp] = __popcll(m);
   }
   __syncthreads();
-  if (static_cast<int>(threadIdx.x) < nsp) {
-    const int sp = threadIdx.x;
+  for (int sp = threadIdx.x; sp < nsp; sp += 256) {
     blk[static_cast<int64_t>(sp) * nblk + blockIdx.x] = cnt[0][sp] + cnt[1][sp] + cnt[2][sp] + cnt[3][sp];
   }
 }
@@ -116,13 +123,18 @@ __global__ __launch_bounds__(1024) void k_exclusive_scan(const int32_t* __restri
   }
 }
 
+// sp_off[0..nkeys]: key ranges of the sorted entries (M = sp_off[nkeys]
+// members).  With ngroups > 0 (charges) the charge channels follow: group g
+// is keys [g*gsize, (g+1)*gsize) again at M + their offsets, so
+// sp_off[nkeys + 1 + g] = M + sp_off[(g+1)*gsize].
 __global__ void k_species_offsets(const int32_t* __restrict__ blkoff, const int32_t* __restrict__ blk,
-                                  int nblk, int nsp, int64_t extra_end, int32_t* __restrict__ sp_off) {
+                                  int nblk, int nkeys, int gsize, int ngroups, int32_t* __restrict__ sp_off) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  for (int s = 0; s < nsp; ++s) sp_off[s] = blkoff[static_cast<int64_t>(s) * nblk];
-  const int64_t last = static_cast<int64_t>(nsp) * nblk - 1;
-  sp_off[nsp] = blkoff[last] + blk[last];
-  if (extra_end > 0) sp_off[nsp + 1] = static_cast<int32_t>(extra_end);  // the charge channel
+  for (int s = 0; s < nkeys; ++s) sp_off[s] = blkoff[static_cast<int64_t>(s) * nblk];
+  const int64_t last = static_cast<int64_t>(nkeys) * nblk - 1;
+  const int32_t m = blkoff[last] + blk[last];
+  sp_off[nkeys] = m;
+  for (int g = 0; g < ngroups; ++g) sp_off[nkeys + 1 + g] = m + ((g + 1) * gsize < nkeys ? sp_off[(g + 1) * gsize] : m);
 }
 
 // Phase in turns, frac(n*(uh + ul)), reduced exactly: the product's rounding
@@ -134,17 +146,18 @@ __device__ inline double frac_turns(int nk, double uh, double ul) {
   return (hi - rint(hi)) + lo;
 }
 
-__global__ __launch_bounds__(256) void k_species_scatter(const int32_t* __restrict__ itype, int64_t n,
+__global__ __launch_bounds__(256) void k_species_scatter(const int32_t* __restrict__ itype,
+                                                         const int32_t* __restrict__ key, int64_t n,
                                                          int nsp, const int32_t* __restrict__ blkoff,
                                                          int nblk, const double* __restrict__ pos,
                                                          double lq0, double lq1, double lq2, int ndim,
-                                                         const double* __restrict__ q, int64_t stride,
+                                                         const double* __restrict__ q,
+                                                         const int32_t* __restrict__ sp_off, int64_t stride,
                                                          double* __restrict__ u) {
-  __shared__ int cnt[4][kNspMax];
+  __shared__ int cnt[4][kKeyMax];
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  int s = (i < n) ? itype[i] - 1 : -1;
-  if (s >= nsp) s = -1;
+  const int s = sort_key(itype, key, i, n, nsp);
   unsigned long long mine = 0;
   for (int sp = 0; sp < nsp; ++sp) {
     const unsigned long long m = __ballot(s == sp);
@@ -152,19 +165,18 @@ __global__ __launch_bounds__(256) void k_species_scatter(const int32_t* __restri
     if (s == sp) mine = m;
   }
   __syncthreads();
-  if (i >= n) return;
-  // An atom with a type outside 1..nsp (flagged by the ingest) has no species
-  // entry; its charge-channel entry gets weight 0.
-  int off = 0;
-  if (s >= 0) {
-    off = blkoff[static_cast<int64_t>(s) * nblk + blockIdx.x];
-    for (int ww = 0; ww < w; ++ww) off += cnt[ww][s];
-    off += __popcll(mine & ((1ull << lane) - 1ull));
-  }
+  // An atom without a key (type outside 1..nsp, flagged by the ingest, or
+  // outside every slice) has no entry.
+  if (s < 0) return;
+  int off = blkoff[static_cast<int64_t>(s) * nblk + blockIdx.x];
+  for (int ww = 0; ww < w; ++ww) off += cnt[ww][s];
+  off += __popcll(mine & ((1ull << lane) - 1ull));
   // Turns u = x/L as an unevaluated sum hi + lo: hi rounded, lo = (x - hi*L)/L
   // with the remainder exact through the FMA; then the unit step e^{2 pi i u}
   // of the phase recurrences as cos, sin.  With charges the same values also
-  // go to entry n + i of the charge channel, and plane 12 holds the weights.
+  // go to entry M + off of the charge channels (M = members), and plane 12
+  // holds the weights (1, and q_i for the charge entry).
+  const int64_t qo = q ? static_cast<int64_t>(sp_off[nsp]) + off : 0;
   const double lq[3] = {lq0, lq1, lq2};
   for (int k = 0; k < 3; ++k) {
     const double x = (k < ndim) ? pos[k * n + i] : 0.0;
@@ -172,22 +184,136 @@ __global__ __launch_bounds__(256) void k_species_scatter(const int32_t* __restri
     const double lo = fma(-hi, lq[k], x) / lq[k];
     double sn, cs;
     sincospi(2.0 * frac_turns(1, hi, lo), &sn, &cs);
-    if (s >= 0) {
-      u[k * stride + off] = hi;
-      u[(3 + k) * stride + off] = lo;
-      u[(6 + k) * stride + off] = cs;
-      u[(9 + k) * stride + off] = sn;
-    }
-    if (q) {  // finite values for a flagged atom, whose weight is 0
-      u[k * stride + n + i] = s >= 0 ? hi : 0.0;
-      u[(3 + k) * stride + n + i] = s >= 0 ? lo : 0.0;
-      u[(6 + k) * stride + n + i] = s >= 0 ? cs : 1.0;
-      u[(9 + k) * stride + n + i] = s >= 0 ? sn : 0.0;
+    u[k * stride + off] = hi;
+    u[(3 + k) * stride + off] = lo;
+    u[(6 + k) * stride + off] = cs;
+    u[(9 + k) * stride + off] = sn;
+    if (q) {
+      u[k * stride + qo] = hi;
+      u[(3 + k) * stride + qo] = lo;
+      u[(6 + k) * stride + qo] = cs;
+      u[(9 + k) * stride + qo] = sn;
     }
   }
   if (q) {
-    if (s >= 0) u[12 * stride + off] = 1.0;
-    u[12 * stride + n + i] = s >= 0 ? q[i] : 0.0;
+    u[12 * stride + off] = 1.0;
+    u[12 * stride + qo] = q[i];
+  }
+}
+
+// ------------------------------------------------------------------ slices
+// Slice of each atom (rdf.cuf:692-696, sq.cuf:640-643) from its ingested z in
+// FP32, as the reference: the first nearest zslice (minloc), member when
+// |zslice - z| <= zgrid/2.  key = slice * nsp + species - 1, or -1.  last
+// gets max(species << 32 | i) over valid atoms: the last atom of the
+// species-sorted order, which rdfxy's thread loop leaves out (rdf.cuf:688).
+__global__ __launch_bounds__(256) void k_slice_keys(const double* __restrict__ pos, int64_t n,
+                                                    const int32_t* __restrict__ itype, int nsp, SliceSet ss,
+                                                    int32_t* __restrict__ key,
+                                                    unsigned long long* __restrict__ last) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int t = itype[i];
+  const float z = static_cast<float>(pos[2 * n + i]);
+  int best = 0;
+  float bd = fabsf(ss.z[0] - z);
+  for (int s = 1; s < ss.n; ++s) {
+    const float d = fabsf(ss.z[s] - z);
+    if (d < bd) {
+      bd = d;
+      best = s;
+    }
+  }
+  const bool ok = t >= 1 && t <= nsp;
+  key[i] = (ok && bd <= ss.zgrid / 2.0f) ? best * nsp + (t - 1) : -1;
+  if (ok) atomicMax(last, (static_cast<unsigned long long>(t) << 32) | static_cast<unsigned long long>(i));
+}
+
+// Per-slice counts of the pair normalisation: members per slice and species
+// without the species-sorted last atom (rdf.cuf:699-700); qsum/qcnt (q not
+// NULL): sum of q^2 and the count over all members (common.cuf:218-232).
+__global__ __launch_bounds__(256) void k_slice_counts(const int32_t* __restrict__ key, int64_t n,
+                                                      const unsigned long long* __restrict__ last, int nsp,
+                                                      int32_t* __restrict__ count, int32_t* __restrict__ countp,
+                                                      const double* __restrict__ q, double* __restrict__ qsum,
+                                                      int32_t* __restrict__ qcnt) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int k = key[i];
+  if (k < 0) return;
+  const int sl = k / nsp;
+  if (static_cast<int64_t>(*last & 0xffffffffull) != i) {
+    atomicAdd(&count[sl], 1);
+    atomicAdd(&countp[k], 1);
+  }
+  if (q) {
+    atomicAdd(&qsum[sl], q[i] * q[i]);
+    atomicAdd(&qcnt[sl], 1);
+  }
+}
+
+// Cell of each atom for the slice pairs: 2D cells of slice sl numbered
+// sl * ncell2 + cy * ncx + cx; atoms outside every slice go to cell
+// nslice * ncell2 (never visited).
+__global__ __launch_bounds__(256) void k_cell_count_slices(const double* __restrict__ pos,
+                                                           const int32_t* __restrict__ key, int64_t n, int nsp,
+                                                           CellGrid g, int nslice,
+                                                           int32_t* __restrict__ cellid,
+                                                           int32_t* __restrict__ count) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int ncell2 = g.nc[0] * g.nc[1];
+  const int k = key[i];
+  int id = nslice * ncell2;
+  if (k >= 0) {
+    int c[2];
+    for (int d = 0; d < 2; ++d) {
+      int ck = static_cast<int>(floor(pos[d * n + i] * g.inv_h[d]));
+      c[d] = ck < 0 ? 0 : (ck >= g.nc[d] ? g.nc[d] - 1 : ck);
+    }
+    id = (k / nsp) * ncell2 + c[1] * g.nc[0] + c[0];
+  }
+  cellid[i] = id;
+  atomicAdd(&count[id], 1);
+}
+
+// The frame's slice histograms into the accumulators (rdf.cuf:284-296):
+// counts += frame, hxy += frame / (countp_j countp_l) when both are positive;
+// with charges (gq not NULL) gqq += (fixed / scale) / (count^2 q2) with q2
+// the slice's mean q^2 (rdf.cuf:262-266 divides by countslice^2 * avq2).  The
+// frame buffers are cleared for the next frame.
+__global__ __launch_bounds__(256) void k_xy_finish(int nslice, int nsp, int lsmax,
+                                                   int64_t* __restrict__ frame, const int32_t* __restrict__ count,
+                                                   const int32_t* __restrict__ countp,
+                                                   int64_t* __restrict__ counts, double* __restrict__ hxy,
+                                                   unsigned long long* __restrict__ gq, const double* __restrict__ qp,
+                                                   const double* __restrict__ qsum, const int32_t* __restrict__ qcnt,
+                                                   double* __restrict__ gqacc) {
+  const int nit = nsp * (nsp + 1) / 2;
+  const int64_t total = static_cast<int64_t>(nslice) * nit * lsmax;
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (e < total) {
+    const int sl = static_cast<int>(e / (static_cast<int64_t>(nit) * lsmax));
+    const int ij = static_cast<int>((e / lsmax) % nit);
+    int j = 0, rem = ij;
+    while (rem >= nsp - j) {
+      rem -= nsp - j;
+      ++j;
+    }
+    const int l = j + rem;
+    const int64_t f = frame[e];
+    counts[e] += f;
+    const double cj = countp[sl * nsp + j], cl = countp[sl * nsp + l];
+    if (cj > 0.0 && cl > 0.0) hxy[e] += static_cast<double>(f) / (cj * cl);
+    frame[e] = 0;
+  }
+  if (gq && e < static_cast<int64_t>(nslice) * lsmax) {
+    const int sl = static_cast<int>(e / lsmax);
+    const long long f = static_cast<long long>(gq[e]);
+    const double c = count[sl];
+    const double q2 = qcnt[sl] > 0 ? qsum[sl] / qcnt[sl] : 0.0;
+    if (c > 0.0 && q2 > 0.0) gqacc[e] += (static_cast<double>(f) / qp[0]) / (c * c * q2);
+    gq[e] = 0ull;
   }
 }
 
@@ -522,9 +648,12 @@ __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stenc
   }
 }
 
+// All pairs i < j of atoms [0, n) of SoA arrays with plane stride `stride`;
+// with range (device, may be NULL) the atoms [range[0], range[1]) instead.
 __global__ __launch_bounds__(256) void k_rdf_allpairs(PairArgs a, const double* __restrict__ pos,
                                                       const int32_t* __restrict__ itype, int64_t n,
-                                                      int nb, uint32_t* __restrict__ partial) {
+                                                      int64_t stride, const int32_t* __restrict__ range,
+                                                      uint32_t* __restrict__ partial) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int nbins = a.nit * a.lsmax;
   uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
@@ -539,9 +668,14 @@ __global__ __launch_bounds__(256) void k_rdf_allpairs(PairArgs a, const double* 
   const double qsc = a.q ? a.qp[0] : 0.0;  // the frame's fixed-point scale
   hist_setup(a, hist, thr);
   const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  const double* __restrict__ px = pos;
-  const double* __restrict__ py = pos + n;
-  const double* __restrict__ pz = pos + 2 * n;
+  const int64_t base = range ? range[0] : 0;
+  if (range) n = range[1] - range[0];
+  const int nb = static_cast<int>((n + kWave - 1) / kWave);
+  const double* __restrict__ px = pos + base;
+  const double* __restrict__ py = pos + stride + base;
+  const double* __restrict__ pz = pos + 2 * stride + base;
+  itype += base;
+  const double* __restrict__ qv = a.q ? a.q + base : nullptr;
   const int64_t ntasks = static_cast<int64_t>(nb) * nb;
   for (int64_t task = static_cast<int64_t>(blockIdx.x) * 4 + wave; task < ntasks;
        task += static_cast<int64_t>(gridDim.x) * 4) {
@@ -554,7 +688,7 @@ __global__ __launch_bounds__(256) void k_rdf_allpairs(PairArgs a, const double* 
     const double zi = vi ? pz[gi] : 0.0;
     const int ti = vi ? itype[gi] : 0;
     const bool oki = vi && static_cast<unsigned>(ti - 1) < static_cast<unsigned>(a.nsp);
-    const double qi = (a.q && vi) ? a.q[gi] : 0.0;
+    const double qi = (qv && vi) ? qv[gi] : 0.0;
     const int64_t js = static_cast<int64_t>(jb) * kWave;
     const int64_t je = js + kWave < n ? js + kWave : n;
     for (int64_t j = js; j < je; ++j) {
@@ -562,7 +696,7 @@ __global__ __launch_bounds__(256) void k_rdf_allpairs(PairArgs a, const double* 
       const bool ok = oki && (jb > ib || j > gi) &&
                       static_cast<unsigned>(tj - 1) < static_cast<unsigned>(a.nsp);
       const int k = pair_accumulate(a, thr, hist, ok, xi, yi, zi, ti, px[j], py[j], pz[j], tj);
-      if (a.q && k) atomicAdd(&gqs[k - 1], qq_fixed(qi * a.q[j], qsc));
+      if (qv && k) atomicAdd(&gqs[k - 1], qq_fixed(qi * qv[j], qsc));
     }
   }
   hist_flush(a, hist, partial);  // begins with a barrier
@@ -1277,16 +1411,60 @@ struct Bsc {
   double b[kNspMax];
 };
 
+// |rho|^2 of one q-vector into the shell sums.  Channels: slice sl, species
+// s at sl * nsp + s; the charge channel of slice sl (nch > nslice * nsp) at
+// nslice * nsp + sl.  Outputs (add(kind, index, value)): kind 0 the bsc-
+// weighted total of slice sl, kind 1 the partial of channel sl * nsp + s,
+// kind 2 the charge channel of slice sl.  With norm (the channel offsets,
+// sqf3xy/sqf2xy, sq.cuf:653-670) each value is divided by the slice's (or
+// channel's) atom count and skipped when that count is 0; without, nslice = 1
+// and values are added as they are.
+template <class Fetch, class Add>
+__device__ inline void shell_vector(int nsp, int nslice, int nch, const Bsc& bsc, const int32_t* __restrict__ norm,
+                                    Fetch fetch, Add add) {
+  for (int sl = 0; sl < nslice; ++sl) {
+    double cnt = 1.0;
+    if (norm) cnt = static_cast<double>(norm[(sl + 1) * nsp] - norm[sl * nsp]);
+    double tr = 0.0, ti = 0.0;
+    for (int s = 0; s < nsp; ++s) {
+      const int c = sl * nsp + s;
+      double rr, ri;
+      fetch(c, rr, ri);
+      const double p = rr * rr + ri * ri;
+      if (norm) {
+        const double cs = static_cast<double>(norm[c + 1] - norm[c]);
+        if (cs > 0.0) add(1, c, p / cs);
+      } else {
+        add(1, c, p);
+      }
+      tr += bsc.b[s] * rr;
+      ti += bsc.b[s] * ri;
+    }
+    if (cnt > 0.0) {
+      add(0, sl, norm ? (tr * tr + ti * ti) / cnt : tr * tr + ti * ti);
+      if (nch > nslice * nsp) {
+        double rr, ri;
+        fetch(nslice * nsp + sl, rr, ri);
+        add(2, sl, norm ? (rr * rr + ri * ri) / cnt : rr * rr + ri * ri);
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_sq_shells(const double* __restrict__ rho,
                                                    const int16_t* __restrict__ tile_shell, int ntiles,
-                                                   int tile_vec, int ksplit, int nsp, int nch, Bsc bsc, int nqmax,
+                                                   int tile_vec, int ksplit, int nsp, int nslice, int nch, Bsc bsc,
+                                                   int nqmax, const int32_t* __restrict__ norm,
                                                    double* __restrict__ sqf, double* __restrict__ sqfp,
                                                    double* __restrict__ sqfq, int use_lds) {
   extern __shared__ __align__(16) unsigned char smem[];
-  double* acc = reinterpret_cast<double*>(smem);  // [(1+nch)][nqmax+1]: sqf, sqfp[nsp], sqfq
+  // LDS accumulators: sqf [nslice][ns], sqfp [nslice*nsp][ns], sqfq [nslice][ns]
+  double* acc = reinterpret_cast<double*>(smem);
   const int ns = nqmax + 1;
+  const int o1 = nslice * ns, o2 = nslice * (1 + nsp) * ns;
+  const int nacc = o2 + (nch > nslice * nsp ? nslice * ns : 0);
   if (use_lds) {
-    for (int b = threadIdx.x; b < (1 + nch) * ns; b += 256) acc[b] = 0.0;
+    for (int b = threadIdx.x; b < nacc; b += 256) acc[b] = 0.0;
     __syncthreads();
   }
   const int64_t total = static_cast<int64_t>(ntiles) * tile_vec;
@@ -1295,46 +1473,39 @@ __global__ __launch_bounds__(256) void k_sq_shells(const double* __restrict__ rh
        e += static_cast<int64_t>(gridDim.x) * 256) {
     const int sh = tile_shell[e];
     if (sh < 0) continue;
-    double tr = 0.0, ti = 0.0;
-    for (int s = 0; s < nch; ++s) {
-      double rr = 0.0, ri = 0.0;
+    auto fetch = [&](int c, double& rr, double& ri) {
+      rr = 0.0;
+      ri = 0.0;
       for (int sp = 0; sp < ksplit; ++sp) {
-        const double2 v = r2[(static_cast<int64_t>(sp) * nch + s) * total + e];
+        const double2 v = r2[(static_cast<int64_t>(sp) * nch + c) * total + e];
         rr += v.x;
         ri += v.y;
       }
-      const double p = rr * rr + ri * ri;
-      if (s == nsp) {  // the charge channel
-        if (use_lds)
-          atomicAdd(&acc[(1 + nsp) * ns + sh], p);
-        else
-          atomicAdd(&sqfq[sh], p);
-        continue;
-      }
+    };
+    auto add = [&](int kind, int idx, double v) {
+      const int off = (kind == 0 ? 0 : (kind == 1 ? o1 : o2)) + idx * ns + sh;
       if (use_lds)
-        atomicAdd(&acc[(1 + s) * ns + sh], p);
+        atomicAdd(&acc[off], v);
+      else if (kind == 0)
+        atomicAdd(&sqf[idx * ns + sh], v);
+      else if (kind == 1)
+        atomicAdd(&sqfp[static_cast<int64_t>(idx) * ns + sh], v);
       else
-        atomicAdd(&sqfp[static_cast<int64_t>(s) * ns + sh], p);
-      tr += bsc.b[s] * rr;
-      ti += bsc.b[s] * ri;
-    }
-    const double pt = tr * tr + ti * ti;
-    if (use_lds)
-      atomicAdd(&acc[sh], pt);
-    else
-      atomicAdd(&sqf[sh], pt);
+        atomicAdd(&sqfq[idx * ns + sh], v);
+    };
+    shell_vector(nsp, nslice, nch, bsc, norm, fetch, add);
   }
   if (use_lds) {
     __syncthreads();
-    for (int b = threadIdx.x; b < (1 + nch) * ns; b += 256) {
+    for (int b = threadIdx.x; b < nacc; b += 256) {
       const double v = acc[b];
       if (v != 0.0) {
-        if (b < ns)
+        if (b < o1)
           atomicAdd(&sqf[b], v);
-        else if (b < (1 + nsp) * ns)
-          atomicAdd(&sqfp[b - ns], v);
+        else if (b < o2)
+          atomicAdd(&sqfp[b - o1], v);
         else
-          atomicAdd(&sqfq[b - (1 + nsp) * ns], v);
+          atomicAdd(&sqfq[b - o2], v);
       }
     }
   }
@@ -1386,28 +1557,29 @@ __global__ __launch_bounds__(256) void k_sq_list(const double* __restrict__ u, c
 }
 
 __global__ void k_sq_list_finish(const double* __restrict__ part, const int32_t* __restrict__ shell, int nvec,
-                                 int nsp, int nch, Bsc bsc, int nqmax, double* __restrict__ sqf,
-                                 double* __restrict__ sqfp, double* __restrict__ sqfq) {
+                                 int nsp, int nslice, int nch, Bsc bsc, int nqmax, const int32_t* __restrict__ norm,
+                                 double* __restrict__ sqf, double* __restrict__ sqfp, double* __restrict__ sqfq) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nvec) return;
   const int sh = shell[v], ns = nqmax + 1;
-  double tr = 0.0, ti = 0.0;
-  for (int s = 0; s < nch; ++s) {
-    double rr = 0.0, ri = 0.0;
+  auto fetch = [&](int c, double& rr, double& ri) {
+    rr = 0.0;
+    ri = 0.0;
     for (int b = 0; b < kListBlocks; ++b) {
-      const double* o = part + ((static_cast<int64_t>(v) * kListBlocks + b) * nch + s) * 2;
+      const double* o = part + ((static_cast<int64_t>(v) * kListBlocks + b) * nch + c) * 2;
       rr += o[0];
       ri += o[1];
     }
-    if (s == nsp) {  // the charge channel
-      atomicAdd(&sqfq[sh], rr * rr + ri * ri);
-      continue;
-    }
-    atomicAdd(&sqfp[static_cast<int64_t>(s) * ns + sh], rr * rr + ri * ri);
-    tr += bsc.b[s] * rr;
-    ti += bsc.b[s] * ri;
-  }
-  atomicAdd(&sqf[sh], tr * tr + ti * ti);
+  };
+  auto add = [&](int kind, int idx, double val) {
+    if (kind == 0)
+      atomicAdd(&sqf[idx * ns + sh], val);
+    else if (kind == 1)
+      atomicAdd(&sqfp[static_cast<int64_t>(idx) * ns + sh], val);
+    else
+      atomicAdd(&sqfq[idx * ns + sh], val);
+  };
+  shell_vector(nsp, nslice, nch, bsc, norm, fetch, add);
 }
 
 // ------------------------------------------------------------------ batches
@@ -1764,25 +1936,25 @@ void launch_ingest(const IngestArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_ingest, dim3(grid1d(a.n, 256)), dim3(256), 0, s, a);
 }
 
-void launch_species_count(const int32_t* itype, int64_t n, int nsp, int32_t* blk, int nblk,
+void launch_species_count(const int32_t* itype, const int32_t* key, int64_t n, int nkeys, int32_t* blk, int nblk,
                           hipStream_t s) {
-  hipLaunchKernelGGL(k_species_count, dim3(nblk), dim3(256), 0, s, itype, n, nsp, blk, nblk);
+  hipLaunchKernelGGL(k_species_count, dim3(nblk), dim3(256), 0, s, itype, key, n, nkeys, blk, nblk);
 }
 
 void launch_exclusive_scan(const int32_t* in, int32_t* out, int64_t count, hipStream_t s) {
   hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, s, in, out, count);
 }
 
-void launch_species_offsets(const int32_t* blkoff, const int32_t* blk, int nblk, int nsp, int64_t extra_end,
+void launch_species_offsets(const int32_t* blkoff, const int32_t* blk, int nblk, int nkeys, int gsize, int ngroups,
                             int32_t* sp_off, hipStream_t s) {
-  hipLaunchKernelGGL(k_species_offsets, dim3(1), dim3(64), 0, s, blkoff, blk, nblk, nsp, extra_end, sp_off);
+  hipLaunchKernelGGL(k_species_offsets, dim3(1), dim3(64), 0, s, blkoff, blk, nblk, nkeys, gsize, ngroups, sp_off);
 }
 
-void launch_species_scatter(const int32_t* itype, int64_t n, int nsp, const int32_t* blkoff, int nblk,
-                            const double* pos, const double lq[3], int ndim, const double* q, int64_t stride,
-                            double* u, hipStream_t s) {
-  hipLaunchKernelGGL(k_species_scatter, dim3(nblk), dim3(256), 0, s, itype, n, nsp, blkoff, nblk, pos,
-                     lq[0], lq[1], lq[2], ndim, q, stride, u);
+void launch_species_scatter(const int32_t* itype, const int32_t* key, int64_t n, int nkeys, const int32_t* blkoff,
+                            int nblk, const double* pos, const double lq[3], int ndim, const double* q,
+                            const int32_t* sp_off, int64_t stride, double* u, hipStream_t s) {
+  hipLaunchKernelGGL(k_species_scatter, dim3(nblk), dim3(256), 0, s, itype, key, n, nkeys, blkoff, nblk, pos,
+                     lq[0], lq[1], lq[2], ndim, q, sp_off, stride, u);
 }
 
 void launch_cell_count(const double* pos, int64_t n, int ndim, const CellGrid& g, int32_t* cellid,
@@ -1855,10 +2027,34 @@ void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start
 }
 
 void launch_rdf_allpairs(const PairArgs& a, const double* pos, const int32_t* itype, int64_t n,
-                         int nblocks, uint32_t* partial, hipStream_t s) {
-  const int nb = static_cast<int>((n + kWave - 1) / kWave);
+                         int nblocks, uint32_t* partial, hipStream_t s, int64_t stride, const int32_t* range) {
   hipLaunchKernelGGL(k_rdf_allpairs, dim3(nblocks), dim3(256), pair_lds_bytes(a.nit, a.lsmax, a.q != nullptr), s,
-                     a, pos, itype, n, nb, partial);
+                     a, pos, itype, n, stride > 0 ? stride : n, range, partial);
+}
+
+void launch_slice_keys(const double* pos, int64_t n, const int32_t* itype, int nsp, const SliceSet& ss,
+                       int32_t* key, unsigned long long* last, hipStream_t s) {
+  hipLaunchKernelGGL(k_slice_keys, dim3(grid1d(n, 256)), dim3(256), 0, s, pos, n, itype, nsp, ss, key, last);
+}
+
+void launch_slice_counts(const int32_t* key, int64_t n, const unsigned long long* last, int nsp, int32_t* count,
+                         int32_t* countp, const double* q, double* qsum, int32_t* qcnt, hipStream_t s) {
+  hipLaunchKernelGGL(k_slice_counts, dim3(grid1d(n, 256)), dim3(256), 0, s, key, n, last, nsp, count, countp, q,
+                     qsum, qcnt);
+}
+
+void launch_cell_count_slices(const double* pos, const int32_t* key, int64_t n, int nsp, const CellGrid& g,
+                              int nslice, int32_t* cellid, int32_t* count, hipStream_t s) {
+  hipLaunchKernelGGL(k_cell_count_slices, dim3(grid1d(n, 256)), dim3(256), 0, s, pos, key, n, nsp, g, nslice,
+                     cellid, count);
+}
+
+void launch_xy_finish(int nslice, int nsp, int lsmax, int64_t* frame, const int32_t* count, const int32_t* countp,
+                      int64_t* counts, double* hxy, unsigned long long* gq, const double* qp, const double* qsum,
+                      const int32_t* qcnt, double* gqacc, hipStream_t s) {
+  const int64_t total = static_cast<int64_t>(nslice) * (nsp * (nsp + 1) / 2) * lsmax;
+  hipLaunchKernelGGL(k_xy_finish, dim3(grid1d(total, 256)), dim3(256), 0, s, nslice, nsp, lsmax, frame, count,
+                     countp, counts, hxy, gq, qp, qsum, qcnt, gqacc);
 }
 
 void launch_qq_params(const double* q, int64_t n, double* qp, uint32_t* flags, hipStream_t s) {
@@ -1892,34 +2088,36 @@ void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s) 
 }
 
 void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, int tile_vec, int ksplit,
-                      int nsp, int nch, const double* bsc, int nqmax, double* sqf, double* sqfp, double* sqfq,
-                      hipStream_t s) {
+                      int nsp, int nslice, int nch, const double* bsc, int nqmax, const int32_t* norm, double* sqf,
+                      double* sqfp, double* sqfq, hipStream_t s) {
   if (ntiles <= 0) return;
   Bsc b{};
   for (int i = 0; i < nsp; ++i) b.b[i] = bsc[i];
-  const size_t lds = static_cast<size_t>(1 + nch) * (nqmax + 1) * sizeof(double);
+  const int nacc = nslice * (1 + nsp) + (nch > nslice * nsp ? nslice : 0);
+  const size_t lds = static_cast<size_t>(nacc) * (nqmax + 1) * sizeof(double);
   const int use_lds = lds <= 48 * 1024;
   const int64_t total = static_cast<int64_t>(ntiles) * tile_vec;
   int64_t g = (total + 255) / 256;
   if (g > 1024) g = 1024;
   hipLaunchKernelGGL(k_sq_shells, dim3(static_cast<unsigned>(g)), dim3(256), use_lds ? lds : 0, s, rho,
-                     tile_shell, ntiles, tile_vec, ksplit, nsp, nch, b, nqmax, sqf, sqfp, sqfq, use_lds);
+                     tile_shell, ntiles, tile_vec, ksplit, nsp, nslice, nch, b, nqmax, norm, sqf, sqfp, sqfq,
+                     use_lds);
 }
 
 size_t sq_list_scratch_bytes(int nvec, int nch) {
   return sizeof(double) * 2 * static_cast<size_t>(nvec) * kListBlocks * nch;
 }
 
-void launch_sq_list(const double* u, const double* w, const int32_t* sp_off, int64_t stride, int nsp, int nch,
-                    int ndim, const int32_t* kv, const int32_t* shell, int nvec, const double* bsc, int nqmax,
-                    double* part, double* sqf, double* sqfp, double* sqfq, hipStream_t s) {
+void launch_sq_list(const double* u, const double* w, const int32_t* sp_off, int64_t stride, int nsp, int nslice,
+                    int nch, int ndim, const int32_t* kv, const int32_t* shell, int nvec, const double* bsc,
+                    int nqmax, bool norm, double* part, double* sqf, double* sqfp, double* sqfq, hipStream_t s) {
   if (nvec <= 0) return;
   Bsc b{};
   for (int i = 0; i < nsp; ++i) b.b[i] = bsc[i];
   hipLaunchKernelGGL(k_sq_list, dim3(nvec, kListBlocks), dim3(256), 0, s, u, w, sp_off, stride, nch, ndim, kv,
                      part);
-  hipLaunchKernelGGL(k_sq_list_finish, dim3((nvec + 63) / 64), dim3(64), 0, s, part, shell, nvec, nsp, nch, b,
-                     nqmax, sqf, sqfp, sqfq);
+  hipLaunchKernelGGL(k_sq_list_finish, dim3((nvec + 63) / 64), dim3(64), 0, s, part, shell, nvec, nsp, nslice,
+                     nch, b, nqmax, norm ? sp_off : nullptr, sqf, sqfp, sqfq);
 }
 
 int64_t launch_lds_atomics(int nblocks, int iters, uint32_t* out, hipStream_t s) {

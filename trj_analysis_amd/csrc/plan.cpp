@@ -73,6 +73,18 @@ std::vector<double> rdf_bin_thresholds(double deltar, int lsmax) {
   return thr;
 }
 
+// Edges of rdfxy's rule k = int(r/deltar) + 1 (rdf.cuf:718), FP64 here:
+// thr[m] = smallest r^2 with int(sqrt(r^2)/deltar) + 1 >= m.
+std::vector<double> rdf_bin_thresholds_floor(double deltar, int lsmax) {
+  std::vector<double> thr(static_cast<size_t>(lsmax) + 2, 0.0);
+  for (int m = 2; m <= lsmax + 1; ++m) {
+    double hi = (m + 1.0) * deltar;
+    hi = hi * hi * 4.0 + 1.0;
+    thr[m] = first_true([&](double r2) { return static_cast<int64_t>(std::sqrt(r2) / deltar) + 1 >= m; }, hi);
+  }
+  return thr;
+}
+
 // d - L*nint(d/L) (rdf.cuf:455-457) switches image where nint(d/L) reaches 1;
 // by symmetry of division and of nint, at -t on the negative side.
 double minimage_threshold(double L) {

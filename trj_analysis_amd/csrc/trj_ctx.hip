@@ -56,6 +56,14 @@ struct trj_ctx {
   // frame with charges (the S_qq column divides by it).
   DevBuf qparm;
   bool rdf_charge = false, sq_charge = false;
+  // Confined slices (twoDstruc_3D; off when slices.n == 0): per-atom keys,
+  // the species-sorted last atom, per-slice counts and q^2 sums of the frame
+  // (and of the last S(Q) frame), the slice pair histograms (frame, raw
+  // totals, normalised accumulators; charges: fixed-point frame sums and
+  // their accumulator), and the floor-rule bin edges.
+  SliceSet slices{};
+  DevBuf slkey, sllast, slcount, slcountp, slqsum, slqcnt, sqqsum, sqqcnt, xyframe, xycounts, xyacc, xygqf,
+      xygqacc, thr_xy;
   // pinned staging for host inputs (double-buffered)
   void* pinned[2] = {nullptr, nullptr};
   size_t pinned_cap[2] = {0, 0};
@@ -74,6 +82,7 @@ struct trj_ctx {
   // last frame
   int64_t last_n = 0;
   double last_volume = 0.0;
+  double last_box[3] = {0.0, 0.0, 0.0};
   int last_nsp = 0;
   // timing
   EventSet evs[kEventRing];
@@ -180,6 +189,88 @@ int validate_box(trj_ctx* c, int ndim, const double box[3]) {
   return TRJ_OK;
 }
 
+// RDF of a frame with slices (rdfxy, rdf.cuf:672-736): the slice members in
+// 2D cells of each slice (or, when a slice's box has fewer than 3 cells per
+// axis, all pairs of the slice), one pair launch per slice into that slice's
+// frame histogram, then the per-frame normalisation (k_xy_finish).
+int rdf_slices(trj_ctx* c, PairArgs pa, const FrameGeom& g, const double box[3], int64_t n,
+               const int32_t* itype_dev, const double* q_dev, hipStream_t rs) {
+  const RdfPlan& R = c->rdf;
+  const int nsl = c->slices.n, nsp = R.nsp;
+  const int nbins = R.nit * R.lsmax;
+  int r;
+  pa.ndim = 2;  // x and y: the reference's image rule on both (rdf.cuf:712-715)
+  pa.thr = P<double>(c->thr_xy);
+  pa.cellshift[2] = 0;
+  const double rc = std::sqrt(g.side2);
+  CellGrid cg{};
+  bool cells_ok = true;
+  for (int k = 0; k < 2; ++k) {
+    int nc = static_cast<int>(std::floor(box[k] / (rc * (1.0 + 1e-6))));
+    nc = std::max(1, std::min(nc, 2048));
+    cg.nc[k] = nc;
+    cg.inv_h[k] = nc / box[k];
+    cg.periodic[k] = 1;
+    if (nc < 3) cells_ok = false;
+    pa.cellshift[k] = nc >= 5 ? 1 : 0;
+  }
+  if (!cells_ok)
+    for (int k = 0; k < 2; ++k) {
+      cg.nc[k] = 1;
+      cg.inv_h[k] = 0.0;
+    }
+  cg.nc[2] = 1;
+  cg.inv_h[2] = 0.0;
+  cg.periodic[2] = 0;
+  const int64_t ncell2 = static_cast<int64_t>(cg.nc[0]) * cg.nc[1];
+  const int64_t ncell = nsl * ncell2 + 1;  // + the cell of atoms outside every slice
+  if ((r = ensure(c, c->cellid, sizeof(int32_t) * n))) return r;
+  if ((r = ensure(c, c->cellcount, sizeof(int32_t) * (ncell + 1)))) return r;
+  if ((r = ensure(c, c->cellstart, sizeof(int32_t) * (ncell + 1)))) return r;
+  if ((r = ensure(c, c->cursor, sizeof(int32_t) * ncell))) return r;
+  if ((r = ensure(c, c->spos, sizeof(double) * 3 * n))) return r;
+  if ((r = ensure(c, c->stype, sizeof(int32_t) * n))) return r;
+  HIPC(c, hipMemsetAsync(c->cellcount.p, 0, sizeof(int32_t) * (ncell + 1), rs));
+  HIPC(c, hipMemsetAsync(c->cursor.p, 0, sizeof(int32_t) * ncell, rs));
+  launch_cell_count_slices(P<double>(c->pos), P<int32_t>(c->slkey), n, nsp, cg, nsl, P<int32_t>(c->cellid),
+                           P<int32_t>(c->cellcount), rs);
+  launch_exclusive_scan(P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), ncell + 1, rs);
+  if (q_dev && (r = ensure(c, c->qsort, sizeof(double) * n))) return r;
+  launch_cell_scatter(P<double>(c->pos), itype_dev, P<int32_t>(c->cellid), n, P<int32_t>(c->cellstart),
+                      P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype), q_dev,
+                      q_dev ? P<double>(c->qsort) : nullptr, rs);
+  if (q_dev) {
+    pa.q = P<double>(c->qsort);
+    pa.qp = P<double>(c->qparm);
+  }
+  int nblocks, W = 1;
+  if (cells_ok) {
+    const double mean = static_cast<double>(n) / static_cast<double>(nsl * ncell2);
+    W = static_cast<int>(std::lround(mean / 64.0));
+    W = std::max(1, std::min(W, 8));
+    nblocks = static_cast<int>(std::min<int64_t>((ncell2 * W + 3) / 4, 2048));
+  } else {
+    const int64_t nb = (n + 63) / 64;
+    nblocks = static_cast<int>(std::min<int64_t>((nb * nb + 3) / 4, 1024));
+  }
+  if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
+  for (int sl = 0; sl < nsl; ++sl) {
+    if (q_dev) pa.gq = P<unsigned long long>(c->xygqf) + static_cast<int64_t>(sl) * R.lsmax;
+    if (cells_ok)
+      launch_rdf_cells(pa, cg, P<int32_t>(c->cellstart) + sl * ncell2, P<double>(c->spos), P<int32_t>(c->stype), n,
+                       W, nblocks, P<uint32_t>(c->partial), rs);
+    else
+      launch_rdf_allpairs(pa, P<double>(c->spos), P<int32_t>(c->stype), n, nblocks, P<uint32_t>(c->partial), rs, n,
+                          P<int32_t>(c->cellstart) + sl);
+    launch_rdf_merge(P<uint32_t>(c->partial), nblocks, nbins, P<int64_t>(c->xyframe) + static_cast<int64_t>(sl) * nbins,
+                     rs);
+  }
+  launch_xy_finish(nsl, nsp, R.lsmax, P<int64_t>(c->xyframe), P<int32_t>(c->slcount), P<int32_t>(c->slcountp),
+                   P<int64_t>(c->xycounts), P<double>(c->xyacc), q_dev ? P<unsigned long long>(c->xygqf) : nullptr,
+                   P<double>(c->qparm), P<double>(c->slqsum), P<int32_t>(c->slqcnt), P<double>(c->xygqacc), rs);
+  return TRJ_OK;
+}
+
 // The frame pipeline, device inputs.
 // q_dev: the frame's charges on the device (NULL without); the caller has
 // queued the frame's {scale, avq2} into qparm[0..1] on the context stream.
@@ -236,25 +327,51 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
   }
   ia.pos = P<double>(c->pos);
   ia.flags = P<uint32_t>(c->flags);
+  // With slices z only assigns atoms to slices: it is taken as given (no
+  // wrap, no range check), as select_ncdfinput leaves it (netcdf.f90:317-331,
+  // 679-682); S(Q)-only contexts have no periodic flags to wrap it by.
+  const int nsl = c->slices.n;
+  ia.free_axis = nsl > 0 ? 2 : -1;
+  if (nsl > 0) ia.periodic[2] = 0;
   launch_ingest(ia, c->stream);
+  if (nsl > 0) {
+    // Slice keys, the species-sorted last atom, and the slice counts.
+    if ((r = ensure(c, c->slkey, sizeof(int32_t) * n))) return r;
+    HIPC(c, hipMemsetAsync(c->sllast.p, 0, sizeof(unsigned long long), c->stream));
+    launch_slice_keys(P<double>(c->pos), n, itype_dev, nsp, c->slices, P<int32_t>(c->slkey),
+                      P<unsigned long long>(c->sllast), c->stream);
+    HIPC(c, hipMemsetAsync(c->slcount.p, 0, sizeof(int32_t) * nsl, c->stream));
+    HIPC(c, hipMemsetAsync(c->slcountp.p, 0, sizeof(int32_t) * nsl * nsp, c->stream));
+    HIPC(c, hipMemsetAsync(c->slqsum.p, 0, sizeof(double) * nsl, c->stream));
+    HIPC(c, hipMemsetAsync(c->slqcnt.p, 0, sizeof(int32_t) * nsl, c->stream));
+    launch_slice_counts(P<int32_t>(c->slkey), n, P<unsigned long long>(c->sllast), nsp, P<int32_t>(c->slcount),
+                        P<int32_t>(c->slcountp), (rdf_q || sq_q) ? q_dev : nullptr, P<double>(c->slqsum),
+                        P<int32_t>(c->slqcnt), c->stream);
+  }
+  // Sort keys: the species, or slice * nsp + species with slices; the charge
+  // channels repeat each slice's (or all) sorted entries.
+  const int nkeys = nsl > 0 ? nsl * nsp : nsp;
+  const int ngroups = sq_q ? (nsl > 0 ? nsl : 1) : 0;
+  const int32_t* keys = nsl > 0 ? P<int32_t>(c->slkey) : nullptr;
   const int nblk = static_cast<int>((n + 255) / 256);
-  if ((r = ensure(c, c->blk, sizeof(int32_t) * nsp * nblk))) return r;
-  if ((r = ensure(c, c->blkoff, sizeof(int32_t) * nsp * nblk))) return r;
-  if ((r = ensure(c, c->sp_off, sizeof(int32_t) * (nsp + 2)))) return r;
-  // Plane stride of u: n, or 2n with the charge channel (all atoms again, in
-  // input order, weighted by their charges).
+  if ((r = ensure(c, c->blk, sizeof(int32_t) * nkeys * nblk))) return r;
+  if ((r = ensure(c, c->blkoff, sizeof(int32_t) * nkeys * nblk))) return r;
+  if ((r = ensure(c, c->sp_off, sizeof(int32_t) * (nkeys + 2 + nsl)))) return r;
+  // Plane stride of u: n, or 2n with charge channels (the sorted entries
+  // again, weighted by their charges).
   const int64_t ustride = sq_q ? 2 * n : n;
-  launch_species_count(itype_dev, n, nsp, P<int32_t>(c->blk), nblk, c->stream);
-  launch_exclusive_scan(P<int32_t>(c->blk), P<int32_t>(c->blkoff), static_cast<int64_t>(nsp) * nblk, c->stream);
-  launch_species_offsets(P<int32_t>(c->blkoff), P<int32_t>(c->blk), nblk, nsp, sq_q ? 2 * n : 0,
+  launch_species_count(itype_dev, keys, n, nkeys, P<int32_t>(c->blk), nblk, c->stream);
+  launch_exclusive_scan(P<int32_t>(c->blk), P<int32_t>(c->blkoff), static_cast<int64_t>(nkeys) * nblk, c->stream);
+  launch_species_offsets(P<int32_t>(c->blkoff), P<int32_t>(c->blk), nblk, nkeys, nsp, ngroups,
                          P<int32_t>(c->sp_off), c->stream);
   if (do_sq) {
-    // Species-contiguous turns u = x / Lq for the contraction (Lq: the box the
-    // q-grid was built on, sq.cuf:189).
+    // Key-contiguous turns u = x / Lq for the contraction (Lq: the box the
+    // q-grid was built on, sq.cuf:189); slices use x and y only.
     if ((r = ensure(c, c->u, sizeof(double) * (sq_q ? 13 : 12) * ustride))) return r;
     const double lq[3] = {c->sq.box[0], c->sq.box[1], c->sq.box[2]};
-    launch_species_scatter(itype_dev, n, nsp, P<int32_t>(c->blkoff), nblk, P<double>(c->pos), lq, ndim,
-                           sq_q ? q_dev : nullptr, ustride, P<double>(c->u), c->stream);
+    launch_species_scatter(itype_dev, keys, n, nkeys, P<int32_t>(c->blkoff), nblk, P<double>(c->pos), lq,
+                           nsl > 0 ? 2 : ndim, sq_q ? q_dev : nullptr, P<int32_t>(c->sp_off), ustride,
+                           P<double>(c->u), c->stream);
   }
   HIPC(c, hipGetLastError());
   HIPC(c, hipEventRecord(es.ev[1], c->stream));
@@ -287,62 +404,66 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
       pa.qp = P<double>(c->qparm);
     }
     const int nbins = R.nit * R.lsmax;
-    // Cell grid: side >= cutoff on every axis.
-    const double rc = std::sqrt(g.side2);
-    CellGrid cg{};
-    bool cells_ok = true;
-    int64_t ncell = 1;
-    for (int k = 0; k < 3; ++k) {
-      if (k < ndim) {
-        int nc = static_cast<int>(std::floor(box[k] / (rc * (1.0 + 1e-6))));
-        nc = std::max(1, std::min(nc, ndim == 3 ? 160 : 2048));
-        cg.nc[k] = nc;
-        cg.inv_h[k] = nc / box[k];
-        cg.periodic[k] = periodic[k];
-        if (periodic[k] && nc < 3) cells_ok = false;
-        pa.cellshift[k] = (periodic[k] && nc >= 5) ? 1 : 0;
-      } else {
-        cg.nc[k] = 1;
-        cg.inv_h[k] = 0.0;
-        cg.periodic[k] = 0;
-      }
-      ncell *= cg.nc[k];
-    }
-    int nblocks;
-    if (cells_ok) {
-      if ((r = ensure(c, c->cellid, sizeof(int32_t) * n))) return r;
-      if ((r = ensure(c, c->cellcount, sizeof(int32_t) * (ncell + 1)))) return r;
-      if ((r = ensure(c, c->cellstart, sizeof(int32_t) * (ncell + 1)))) return r;
-      if ((r = ensure(c, c->cursor, sizeof(int32_t) * ncell))) return r;
-      if ((r = ensure(c, c->spos, sizeof(double) * 3 * n))) return r;
-      if ((r = ensure(c, c->stype, sizeof(int32_t) * n))) return r;
-      HIPC(c, hipMemsetAsync(c->cellcount.p, 0, sizeof(int32_t) * (ncell + 1), rs));
-      HIPC(c, hipMemsetAsync(c->cursor.p, 0, sizeof(int32_t) * ncell, rs));
-      launch_cell_count(P<double>(c->pos), n, ndim, cg, P<int32_t>(c->cellid), P<int32_t>(c->cellcount),
-                        rs);
-      launch_exclusive_scan(P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), ncell + 1, rs);
-      if (rdf_q && (r = ensure(c, c->qsort, sizeof(double) * n))) return r;
-      launch_cell_scatter(P<double>(c->pos), itype_dev, P<int32_t>(c->cellid), n, P<int32_t>(c->cellstart),
-                          P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype),
-                          rdf_q ? q_dev : nullptr, rdf_q ? P<double>(c->qsort) : nullptr, rs);
-      if (rdf_q) pa.q = P<double>(c->qsort);
-      const double mean = static_cast<double>(n) / static_cast<double>(ncell);
-      int W = static_cast<int>(std::lround(mean / 64.0));
-      W = std::max(1, std::min(W, 8));
-      const int64_t ntasks = ncell * W;
-      nblocks = static_cast<int>(std::min<int64_t>((ntasks + 3) / 4, 2048));
-      if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
-      launch_rdf_cells(pa, cg, P<int32_t>(c->cellstart), P<double>(c->spos), P<int32_t>(c->stype), n, W,
-                       nblocks, P<uint32_t>(c->partial), rs);
+    if (nsl > 0) {
+      if ((r = rdf_slices(c, pa, g, box, n, itype_dev, rdf_q ? q_dev : nullptr, rs))) return r;
     } else {
-      const int64_t nb = (n + 63) / 64;
-      nblocks = static_cast<int>(std::min<int64_t>((nb * nb + 3) / 4, 1024));
-      if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
-      if (rdf_q) pa.q = q_dev;  // input order, like pos and itype here
-      launch_rdf_allpairs(pa, P<double>(c->pos), itype_dev, n, nblocks, P<uint32_t>(c->partial), rs);
+      // Cell grid: side >= cutoff on every axis.
+      const double rc = std::sqrt(g.side2);
+      CellGrid cg{};
+      bool cells_ok = true;
+      int64_t ncell = 1;
+      for (int k = 0; k < 3; ++k) {
+        if (k < ndim) {
+          int nc = static_cast<int>(std::floor(box[k] / (rc * (1.0 + 1e-6))));
+          nc = std::max(1, std::min(nc, ndim == 3 ? 160 : 2048));
+          cg.nc[k] = nc;
+          cg.inv_h[k] = nc / box[k];
+          cg.periodic[k] = periodic[k];
+          if (periodic[k] && nc < 3) cells_ok = false;
+          pa.cellshift[k] = (periodic[k] && nc >= 5) ? 1 : 0;
+        } else {
+          cg.nc[k] = 1;
+          cg.inv_h[k] = 0.0;
+          cg.periodic[k] = 0;
+        }
+        ncell *= cg.nc[k];
+      }
+      int nblocks;
+      if (cells_ok) {
+        if ((r = ensure(c, c->cellid, sizeof(int32_t) * n))) return r;
+        if ((r = ensure(c, c->cellcount, sizeof(int32_t) * (ncell + 1)))) return r;
+        if ((r = ensure(c, c->cellstart, sizeof(int32_t) * (ncell + 1)))) return r;
+        if ((r = ensure(c, c->cursor, sizeof(int32_t) * ncell))) return r;
+        if ((r = ensure(c, c->spos, sizeof(double) * 3 * n))) return r;
+        if ((r = ensure(c, c->stype, sizeof(int32_t) * n))) return r;
+        HIPC(c, hipMemsetAsync(c->cellcount.p, 0, sizeof(int32_t) * (ncell + 1), rs));
+        HIPC(c, hipMemsetAsync(c->cursor.p, 0, sizeof(int32_t) * ncell, rs));
+        launch_cell_count(P<double>(c->pos), n, ndim, cg, P<int32_t>(c->cellid), P<int32_t>(c->cellcount),
+                          rs);
+        launch_exclusive_scan(P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), ncell + 1, rs);
+        if (rdf_q && (r = ensure(c, c->qsort, sizeof(double) * n))) return r;
+        launch_cell_scatter(P<double>(c->pos), itype_dev, P<int32_t>(c->cellid), n, P<int32_t>(c->cellstart),
+                            P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype),
+                            rdf_q ? q_dev : nullptr, rdf_q ? P<double>(c->qsort) : nullptr, rs);
+        if (rdf_q) pa.q = P<double>(c->qsort);
+        const double mean = static_cast<double>(n) / static_cast<double>(ncell);
+        int W = static_cast<int>(std::lround(mean / 64.0));
+        W = std::max(1, std::min(W, 8));
+        const int64_t ntasks = ncell * W;
+        nblocks = static_cast<int>(std::min<int64_t>((ntasks + 3) / 4, 2048));
+        if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
+        launch_rdf_cells(pa, cg, P<int32_t>(c->cellstart), P<double>(c->spos), P<int32_t>(c->stype), n, W,
+                         nblocks, P<uint32_t>(c->partial), rs);
+      } else {
+        const int64_t nb = (n + 63) / 64;
+        nblocks = static_cast<int>(std::min<int64_t>((nb * nb + 3) / 4, 1024));
+        if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
+        if (rdf_q) pa.q = q_dev;  // input order, like pos and itype here
+        launch_rdf_allpairs(pa, P<double>(c->pos), itype_dev, n, nblocks, P<uint32_t>(c->partial), rs);
+      }
+      launch_rdf_merge(P<uint32_t>(c->partial), nblocks, nbins, P<int64_t>(c->counts), rs);
+      if (rdf_q) launch_qq_finish(P<unsigned long long>(c->gqf), R.lsmax, P<double>(c->qparm), P<double>(c->gqacc), rs);
     }
-    launch_rdf_merge(P<uint32_t>(c->partial), nblocks, nbins, P<int64_t>(c->counts), rs);
-    if (rdf_q) launch_qq_finish(P<unsigned long long>(c->gqf), R.lsmax, P<double>(c->qparm), P<double>(c->gqacc), rs);
     HIPC(c, hipGetLastError());
     c->rdf_frames += 1;
     c->last_volume = g.volume;
@@ -358,8 +479,9 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
   if (do_sq) {
     SqPlan& S = c->sq;
     const int ntiles = static_cast<int>(S.tiles.size());
+    const int nch = nkeys + ngroups;  // (slice,) species channels, then charge channels
+    const int sdim = nsl > 0 ? 2 : ndim;
     if (ntiles > 0) {
-      const int nch = nsp + (sq_q ? 1 : 0);
       const int ks = contraction_splits(ntiles, n, S.p.precision, nch);
       c->ksplit = ks;
       const size_t rho_bytes = sizeof(double) * 2 * static_cast<size_t>(ks) * nch * ntiles * S.tk.vec();
@@ -373,7 +495,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
       ca.ksplit = ks;
       ca.nsp = nsp;
       ca.nch = nch;
-      ca.ndim = ndim;
+      ca.ndim = sdim;
       ca.n = ustride;
       ca.rho = P<double>(c->rho);
       if (S.p.precision == TRJ_PREC_FP64)
@@ -383,19 +505,28 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     }
     HIPC(c, hipGetLastError());
     HIPC(c, hipEventRecord(es.ev[3], c->stream));
-    const int nch = nsp + (sq_q ? 1 : 0);
+    // With slices every vector's values are divided by the slice's (species')
+    // atom count (sqf3xy / sqf2xy, sq.cuf:653-670, 741-765).
+    const int32_t* norm = nsl > 0 ? P<int32_t>(c->sp_off) : nullptr;
+    const int ngs = nsl > 0 ? nsl : 1;
     if (ntiles > 0)
-      launch_sq_shells(P<double>(c->rho), P<int16_t>(c->tile_shell), ntiles, S.tk.vec(), c->ksplit, nsp, nch,
-                       S.p.bsc, S.nqmax, P<double>(c->sqf), P<double>(c->sqfp), P<double>(c->sqfq), c->stream);
+      launch_sq_shells(P<double>(c->rho), P<int16_t>(c->tile_shell), ntiles, S.tk.vec(), c->ksplit, nsp, ngs, nch,
+                       S.p.bsc, S.nqmax, norm, P<double>(c->sqf), P<double>(c->sqfp), P<double>(c->sqfq), c->stream);
     const int nax = static_cast<int>(S.axis_shell.size());
     if ((r = ensure(c, c->listpart, sq_list_scratch_bytes(nax, nch)))) return r;
     launch_sq_list(P<double>(c->u), sq_q ? P<double>(c->u) + 12 * ustride : nullptr, P<int32_t>(c->sp_off), ustride,
-                   nsp, nch, ndim, P<int32_t>(c->axis_kv), P<int32_t>(c->axis_shell), nax, S.p.bsc, S.nqmax,
-                   P<double>(c->listpart), P<double>(c->sqf), P<double>(c->sqfp), P<double>(c->sqfq), c->stream);
-    // sq.cuf:143-145 divides by avq2 of the last frame: keep it in qparm[2].
-    if (sq_q)
+                   nsp, ngs, nch, sdim, P<int32_t>(c->axis_kv), P<int32_t>(c->axis_shell), nax, S.p.bsc, S.nqmax,
+                   nsl > 0, P<double>(c->listpart), P<double>(c->sqf), P<double>(c->sqfp), P<double>(c->sqfq),
+                   c->stream);
+    // sq.cuf:143-145 divides by avq2 of the last frame: keep it in qparm[2];
+    // with slices, by each slice's mean q^2 of the last frame (sq.cuf:135-138).
+    if (sq_q && nsl == 0)
       HIPC(c, hipMemcpyAsync(P<double>(c->qparm) + 2, P<double>(c->qparm) + 1, sizeof(double),
                              hipMemcpyDeviceToDevice, c->stream));
+    if (sq_q && nsl > 0) {
+      HIPC(c, hipMemcpyAsync(c->sqqsum.p, c->slqsum.p, sizeof(double) * nsl, hipMemcpyDeviceToDevice, c->stream));
+      HIPC(c, hipMemcpyAsync(c->sqqcnt.p, c->slqcnt.p, sizeof(int32_t) * nsl, hipMemcpyDeviceToDevice, c->stream));
+    }
     HIPC(c, hipGetLastError());
     c->sq_frames += 1;
   } else {
@@ -409,6 +540,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
   es.overlapped = overlap;
   c->last_n = n;
   c->last_nsp = nsp;
+  for (int k = 0; k < 3; ++k) c->last_box[k] = box[k];
   return TRJ_OK;
 }
 
@@ -495,7 +627,9 @@ void trj_destroy(trj_ctx* c) {
                     &c->cellcount, &c->cellstart, &c->cursor, &c->partial, &c->counts, &c->thr,
                     &c->blk,  &c->blkoff,    &c->sp_off,    &c->u,      &c->rho,    &c->tiles,
                     &c->tile_shell, &c->sqf, &c->sqfp,      &c->axis_kv, &c->axis_shell, &c->listpart,
-                    &c->flags, &c->qraw, &c->qsort, &c->gqf, &c->gqacc, &c->sqfq, &c->qparm};
+                    &c->flags, &c->qraw, &c->qsort, &c->gqf, &c->gqacc, &c->sqfq, &c->qparm, &c->slkey,
+                    &c->sllast, &c->slcount, &c->slcountp, &c->slqsum, &c->slqcnt, &c->sqqsum, &c->sqqcnt,
+                    &c->xyframe, &c->xycounts, &c->xyacc, &c->xygqf, &c->xygqacc, &c->thr_xy};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   for (int k = 0; k < 2; ++k) {
@@ -524,6 +658,34 @@ int trj_set_pipeline(trj_ctx* c, int mode) {
   return TRJ_OK;
 }
 
+int trj_set_slices(trj_ctx* c, int32_t nslice, const float* zslice, float zgrid) {
+  if (!c) return TRJ_EINVAL;
+  if (c->rdf_ready || c->sq_ready) return fail(c, TRJ_ESTATE, "trj_set_slices comes before trj_rdf_init / trj_sq_init");
+  if (nslice < 0 || nslice > kMaxSlices) return fail(c, TRJ_EINVAL, "nslice must be in 0..16");
+  if (nslice > 0 && !zslice) return fail(c, TRJ_EINVAL, "NULL zslice");
+  if (nslice > 0 && !(zgrid > 0.0f && std::isfinite(zgrid))) return fail(c, TRJ_EINVAL, "zgrid must be positive");
+  for (int k = 0; k < nslice; ++k)
+    if (!std::isfinite(zslice[k])) return fail(c, TRJ_EINVAL, "zslice values must be finite");
+  hipSetDevice(c->device);
+  int r;
+  c->slices = SliceSet{};
+  if (nslice == 0) return TRJ_OK;
+  if ((r = ensure(c, c->sllast, sizeof(unsigned long long)))) return r;
+  if ((r = ensure(c, c->slcount, sizeof(int32_t) * nslice))) return r;
+  if ((r = ensure(c, c->slcountp, sizeof(int32_t) * nslice * kNspMax))) return r;
+  if ((r = ensure(c, c->slqsum, sizeof(double) * nslice))) return r;
+  if ((r = ensure(c, c->slqcnt, sizeof(int32_t) * nslice))) return r;
+  if ((r = ensure(c, c->sqqsum, sizeof(double) * nslice))) return r;
+  if ((r = ensure(c, c->sqqcnt, sizeof(int32_t) * nslice))) return r;
+  HIPC(c, hipMemsetAsync(c->sqqsum.p, 0, sizeof(double) * nslice, c->stream));
+  HIPC(c, hipMemsetAsync(c->sqqcnt.p, 0, sizeof(int32_t) * nslice, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  c->slices.n = nslice;
+  for (int k = 0; k < nslice; ++k) c->slices.z[k] = zslice[k];
+  c->slices.zgrid = zgrid;
+  return TRJ_OK;
+}
+
 int trj_rdf_init(trj_ctx* c, const trj_rdf_params* p, const double box[3], int32_t* lsmax_out,
                  int32_t* nit_out) {
   if (!c || !p) return TRJ_EINVAL;
@@ -548,6 +710,32 @@ int trj_rdf_init(trj_ctx* c, const trj_rdf_params* p, const double box[3], int32
     return fail(c, TRJ_ELIMIT,
                 "rdf: g(r) grid needs " + std::to_string(cells_lds_bytes(R.nit, R.lsmax, qq)) +
                     " bytes of LDS, above 65536; increase deltar or reduce rcrdf (rdf.cuf:61-64)");
+  const int nsl = c->slices.n;
+  if (nsl > 0) {
+    if (R.ndim != 3) return fail(c, TRJ_EINVAL, "rdf: slices need ndim = 3");
+    if (!R.periodic[0] || !R.periodic[1]) return fail(c, TRJ_EINVAL, "rdf: slices need periodic x and y");
+    std::vector<double> tx = rdf_bin_thresholds_floor(R.deltar, R.lsmax);
+    tx.push_back(INFINITY);
+    tx[0] = 0.0;
+    const size_t nb = static_cast<size_t>(nsl) * R.nit * R.lsmax;
+    if ((r = ensure(c, c->thr_xy, sizeof(double) * tx.size()))) return r;
+    if ((r = ensure(c, c->xyframe, sizeof(int64_t) * nb))) return r;
+    if ((r = ensure(c, c->xycounts, sizeof(int64_t) * nb))) return r;
+    if ((r = ensure(c, c->xyacc, sizeof(double) * nb))) return r;
+    HIPC(c, hipMemcpyAsync(c->thr_xy.p, tx.data(), sizeof(double) * tx.size(), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemsetAsync(c->xyframe.p, 0, sizeof(int64_t) * nb, c->stream));
+    HIPC(c, hipMemsetAsync(c->xycounts.p, 0, sizeof(int64_t) * nb, c->stream));
+    HIPC(c, hipMemsetAsync(c->xyacc.p, 0, sizeof(double) * nb, c->stream));
+    if (qq) {
+      const size_t nq = static_cast<size_t>(nsl) * R.lsmax;
+      if ((r = ensure(c, c->xygqf, sizeof(unsigned long long) * nq))) return r;
+      if ((r = ensure(c, c->xygqacc, sizeof(double) * nq))) return r;
+      HIPC(c, hipMemsetAsync(c->xygqf.p, 0, sizeof(unsigned long long) * nq, c->stream));
+      HIPC(c, hipMemsetAsync(c->xygqacc.p, 0, sizeof(double) * nq, c->stream));
+    }
+    // Keep the host copy alive until the copy above has completed.
+    HIPC(c, hipStreamSynchronize(c->stream));
+  }
   std::vector<double> thr = rdf_bin_thresholds(R.deltar, R.lsmax);
   thr.push_back(INFINITY);  // thr[lsmax+2]
   thr[0] = 0.0;
@@ -580,22 +768,52 @@ int trj_sq_init(trj_ctx* c, const trj_sq_params* p, const double box[3], trj_sq_
   if (r) return r;
   SqPlan S;
   std::string err;
-  r = sq_plan(*p, box, &S, &err);
-  if (r) return fail(c, r, err);
+  const int nsl = c->slices.n;
+  if (nsl > 0) {
+    // Slices: the 2D q-set in the xy plane of each slice (sq.cuf:156-200 with
+    // twoDstruc_3D), axis vectors from nqmin + 1 (sqf2xy, sq.cuf:91-92,
+    // 114-116; from 1 when the 16-adjust leaves nqmin below 1).
+    if (p->ndim != 3) return fail(c, TRJ_EINVAL, "sq: slices need ndim = 3");
+    if (p->qset != TRJ_QSET_REF) return fail(c, TRJ_EINVAL, "sq: slices use the reference q-set");
+    trj_sq_params p2 = *p;
+    p2.ndim = 2;
+    r = sq_plan(p2, box, &S, &err);
+    if (r) return fail(c, r, err);
+    S.p.ndim = 3;
+    S.axis_kv.clear();
+    S.axis_shell.clear();
+    std::fill(S.nq_summed.begin(), S.nq_summed.end(), 0);
+    for (size_t e = 0; e < S.tile_shell.size(); ++e)
+      if (S.tile_shell[e] >= 0) S.nq_summed[S.tile_shell[e]] += 1;
+    for (int k = std::max(1, S.nqmin + 1); k <= S.nqmax; ++k)
+      for (int d = 0; d < 2; ++d) {
+        int v[3] = {0, 0, 0};
+        v[d] = k;
+        S.axis_kv.insert(S.axis_kv.end(), v, v + 3);
+        S.axis_shell.push_back(k);
+        S.nq_summed[k] += 1;
+      }
+    S.info.nvec_axis = static_cast<int64_t>(S.axis_shell.size());
+    if (p->flags & TRJ_SQ_NQ_CONSISTENT) S.nq = S.nq_summed;
+  } else {
+    r = sq_plan(*p, box, &S, &err);
+    if (r) return fail(c, r, err);
+  }
   const int ns = S.nqmax + 1;
-  if ((r = ensure(c, c->sqf, sizeof(double) * ns))) return r;
-  if ((r = ensure(c, c->sqfp, sizeof(double) * ns * S.p.nsp))) return r;
+  const int ngs = nsl > 0 ? nsl : 1;
+  if ((r = ensure(c, c->sqf, sizeof(double) * ns * ngs))) return r;
+  if ((r = ensure(c, c->sqfp, sizeof(double) * ns * S.p.nsp * ngs))) return r;
   if ((r = ensure(c, c->tiles, sizeof(SqTile) * std::max<size_t>(1, S.tiles.size())))) return r;
   if ((r = ensure(c, c->tile_shell, sizeof(int16_t) * std::max<size_t>(1, S.tile_shell.size())))) return r;
   if ((r = ensure(c, c->axis_kv, sizeof(int32_t) * std::max<size_t>(1, S.axis_kv.size())))) return r;
   if ((r = ensure(c, c->axis_shell, sizeof(int32_t) * std::max<size_t>(1, S.axis_shell.size())))) return r;
-  HIPC(c, hipMemsetAsync(c->sqf.p, 0, sizeof(double) * ns, c->stream));
-  HIPC(c, hipMemsetAsync(c->sqfp.p, 0, sizeof(double) * ns * S.p.nsp, c->stream));
+  HIPC(c, hipMemsetAsync(c->sqf.p, 0, sizeof(double) * ns * ngs, c->stream));
+  HIPC(c, hipMemsetAsync(c->sqfp.p, 0, sizeof(double) * ns * S.p.nsp * ngs, c->stream));
   const bool sqq = (p->flags & TRJ_SQ_CHARGE) != 0;
   if (sqq) {
-    if ((r = ensure(c, c->sqfq, sizeof(double) * ns))) return r;
+    if ((r = ensure(c, c->sqfq, sizeof(double) * ns * ngs))) return r;
     if ((r = ensure(c, c->qparm, 4 * sizeof(double)))) return r;
-    HIPC(c, hipMemsetAsync(c->sqfq.p, 0, sizeof(double) * ns, c->stream));
+    HIPC(c, hipMemsetAsync(c->sqfq.p, 0, sizeof(double) * ns * ngs, c->stream));
     HIPC(c, hipMemsetAsync(c->qparm.p, 0, 4 * sizeof(double), c->stream));
   }
   if (!S.tiles.empty()) {
@@ -744,6 +962,7 @@ int trj_sync(trj_ctx* c) {
 
 int trj_rdf_counts(trj_ctx* c, int64_t* counts) {
   if (!c || !counts) return TRJ_EINVAL;
+  if (c->slices.n > 0) return fail(c, TRJ_ESTATE, "trj_rdf_counts: the context analyses z-slices (trj_set_slices)");
   if (!c->rdf_ready) return fail(c, TRJ_ESTATE, "RDF not initialised");
   hipSetDevice(c->device);
   int r = sync_all(c);
@@ -754,6 +973,7 @@ int trj_rdf_counts(trj_ctx* c, int64_t* counts) {
 
 int trj_rdf_gmix(trj_ctx* c, int64_t nconf, double* out) {
   if (!c || !out) return TRJ_EINVAL;
+  if (c->slices.n > 0) return fail(c, TRJ_ESTATE, "trj_rdf_gmix: the context analyses z-slices (trj_set_slices)");
   if (!c->rdf_ready) return fail(c, TRJ_ESTATE, "RDF not initialised");
   if (c->rdf_frames == 0) return fail(c, TRJ_ESTATE, "no RDF frame processed");
   const RdfPlan& R = c->rdf;
@@ -787,6 +1007,7 @@ int trj_rdf_gmix(trj_ctx* c, int64_t nconf, double* out) {
 
 int trj_rdf_qq(trj_ctx* c, double* acc) {
   if (!c || !acc) return TRJ_EINVAL;
+  if (c->slices.n > 0) return fail(c, TRJ_ESTATE, "trj_rdf_qq: the context analyses z-slices (trj_set_slices)");
   if (!c->rdf_ready || !c->rdf_charge) return fail(c, TRJ_ESTATE, "RDF not initialised with TRJ_RDF_CHARGE");
   hipSetDevice(c->device);
   int r = sync_all(c);
@@ -797,6 +1018,7 @@ int trj_rdf_qq(trj_ctx* c, double* acc) {
 
 int trj_rdf_gqq(trj_ctx* c, int64_t nconf, double* out) {
   if (!c || !out) return TRJ_EINVAL;
+  if (c->slices.n > 0) return fail(c, TRJ_ESTATE, "trj_rdf_gqq: the context analyses z-slices (trj_set_slices)");
   if (!c->rdf_ready || !c->rdf_charge) return fail(c, TRJ_ESTATE, "RDF not initialised with TRJ_RDF_CHARGE");
   if (c->rdf_frames == 0) return fail(c, TRJ_ESTATE, "no RDF frame processed");
   const RdfPlan& R = c->rdf;
@@ -817,6 +1039,7 @@ int trj_rdf_gqq(trj_ctx* c, int64_t nconf, double* out) {
 
 int trj_rdf_first_minimum(trj_ctx* c, double* rmin) {
   if (!c || !rmin) return TRJ_EINVAL;
+  if (c->slices.n > 0) return fail(c, TRJ_ESTATE, "trj_rdf_first_minimum: the context analyses z-slices (trj_set_slices)");
   if (!c->rdf_ready || c->rdf_frames == 0) return fail(c, TRJ_ESTATE, "no RDF frame processed");
   const RdfPlan& R = c->rdf;
   std::vector<int64_t> counts(static_cast<size_t>(R.nit) * R.lsmax);
@@ -868,6 +1091,7 @@ int trj_rdf_first_minimum(trj_ctx* c, double* rmin) {
 
 int trj_sq_shells(trj_ctx* c, double* sqf, double* sqfp, int64_t* nq) {
   if (!c) return TRJ_EINVAL;
+  if (c->slices.n > 0) return fail(c, TRJ_ESTATE, "trj_sq_shells: the context analyses z-slices (trj_set_slices)");
   if (!c->sq_ready) return fail(c, TRJ_ESTATE, "S(Q) not initialised");
   hipSetDevice(c->device);
   int r = sync_all(c);
@@ -882,6 +1106,7 @@ int trj_sq_shells(trj_ctx* c, double* sqf, double* sqfp, int64_t* nq) {
 
 int trj_sq_qq(trj_ctx* c, double* sqfq) {
   if (!c || !sqfq) return TRJ_EINVAL;
+  if (c->slices.n > 0) return fail(c, TRJ_ESTATE, "trj_sq_qq: the context analyses z-slices (trj_set_slices)");
   if (!c->sq_ready || !c->sq_charge) return fail(c, TRJ_ESTATE, "S(Q) not initialised with TRJ_SQ_CHARGE");
   hipSetDevice(c->device);
   int r = sync_all(c);
@@ -892,6 +1117,7 @@ int trj_sq_qq(trj_ctx* c, double* sqfq) {
 
 int trj_sq_qq_column(trj_ctx* c, int64_t nconf, double* out) {
   if (!c || !out) return TRJ_EINVAL;
+  if (c->slices.n > 0) return fail(c, TRJ_ESTATE, "trj_sq_qq_column: the context analyses z-slices (trj_set_slices)");
   if (!c->sq_ready || !c->sq_charge) return fail(c, TRJ_ESTATE, "S(Q) not initialised with TRJ_SQ_CHARGE");
   if (c->sq_frames == 0) return fail(c, TRJ_ESTATE, "no S(Q) frame processed");
   const SqPlan& S = c->sq;
@@ -913,6 +1139,7 @@ int trj_sq_qq_column(trj_ctx* c, int64_t nconf, double* out) {
 
 int trj_sq_columns(trj_ctx* c, int64_t nconf, double* cols, int32_t* ncols_out, double* mix) {
   if (!c || !cols) return TRJ_EINVAL;
+  if (c->slices.n > 0) return fail(c, TRJ_ESTATE, "trj_sq_columns: the context analyses z-slices (trj_set_slices)");
   if (!c->sq_ready || c->sq_frames == 0) return fail(c, TRJ_ESTATE, "no S(Q) frame processed");
   const SqPlan& S = c->sq;
   const int nsp = S.p.nsp, ns = S.nqmax + 1;
@@ -965,6 +1192,7 @@ int trj_sq_columns(trj_ctx* c, int64_t nconf, double* cols, int32_t* ncols_out, 
 
 int trj_sq_rho(trj_ctx* c, int64_t nvec, const int32_t* kv, double* rho) {
   if (!c || (nvec > 0 && (!kv || !rho))) return TRJ_EINVAL;
+  if (c->slices.n > 0) return fail(c, TRJ_ESTATE, "trj_sq_rho: the context analyses z-slices (trj_set_slices)");
   if (!c->sq_ready || c->sq_frames == 0) return fail(c, TRJ_ESTATE, "no S(Q) frame processed");
   hipSetDevice(c->device);
   int r = sync_all(c);
@@ -1002,6 +1230,168 @@ int trj_sq_rho(trj_ctx* c, int64_t nvec, const int32_t* kv, double* rho) {
   return TRJ_OK;
 }
 
+// ---------------------------------------------------------------- slices
+
+int trj_slice_counts(trj_ctx* c, int64_t* count, int64_t* countp) {
+  if (!c) return TRJ_EINVAL;
+  if (c->slices.n == 0) return fail(c, TRJ_ESTATE, "no slices (trj_set_slices)");
+  if (c->rdf_frames == 0 && c->sq_frames == 0) return fail(c, TRJ_ESTATE, "no frame processed");
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  const int nsl = c->slices.n, nsp = c->last_nsp;
+  std::vector<int32_t> a(nsl), b(static_cast<size_t>(nsl) * nsp);
+  HIPC(c, hipMemcpy(a.data(), c->slcount.p, sizeof(int32_t) * nsl, hipMemcpyDeviceToHost));
+  HIPC(c, hipMemcpy(b.data(), c->slcountp.p, sizeof(int32_t) * b.size(), hipMemcpyDeviceToHost));
+  if (count)
+    for (int k = 0; k < nsl; ++k) count[k] = a[k];
+  if (countp)
+    for (size_t k = 0; k < b.size(); ++k) countp[k] = b[k];
+  return TRJ_OK;
+}
+
+int trj_rdf_xy(trj_ctx* c, double* hxy, int64_t* counts) {
+  if (!c) return TRJ_EINVAL;
+  if (!c->rdf_ready || c->slices.n == 0) return fail(c, TRJ_ESTATE, "RDF not initialised with slices");
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  const size_t nb = static_cast<size_t>(c->slices.n) * c->rdf.nit * c->rdf.lsmax;
+  if (hxy) HIPC(c, hipMemcpy(hxy, c->xyacc.p, sizeof(double) * nb, hipMemcpyDeviceToHost));
+  if (counts) HIPC(c, hipMemcpy(counts, c->xycounts.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost));
+  return TRJ_OK;
+}
+
+int trj_rdf_xy_qq(trj_ctx* c, double* acc) {
+  if (!c || !acc) return TRJ_EINVAL;
+  if (!c->rdf_ready || c->slices.n == 0 || !c->rdf_charge)
+    return fail(c, TRJ_ESTATE, "RDF not initialised with slices and TRJ_RDF_CHARGE");
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  HIPC(c, hipMemcpy(acc, c->xygqacc.p, sizeof(double) * c->slices.n * c->rdf.lsmax, hipMemcpyDeviceToHost));
+  return TRJ_OK;
+}
+
+namespace {
+// log.f90:494-496: cylindrical shell of r_i = i deltar times zgrid.
+double xy_shell(double deltar, int i, double zgrid) {
+  const double pi = 3.14159265358979323846;
+  const double ri = i * deltar, a = ri + deltar / 2, b = ri - deltar / 2;
+  return pi * (a * a - b * b) * zgrid;
+}
+}  // namespace
+
+int trj_rdf_xy_gmix(trj_ctx* c, int64_t nconf, double* out) {
+  if (!c || !out) return TRJ_EINVAL;
+  if (!c->rdf_ready || c->slices.n == 0) return fail(c, TRJ_ESTATE, "RDF not initialised with slices");
+  if (c->rdf_frames == 0) return fail(c, TRJ_ESTATE, "no RDF frame processed");
+  const RdfPlan& R = c->rdf;
+  const int nsl = c->slices.n, nsp = R.nsp;
+  std::vector<double> h(static_cast<size_t>(nsl) * R.nit * R.lsmax);
+  int r = trj_rdf_xy(c, h.data(), nullptr);
+  if (r) return r;
+  if (nconf <= 0) nconf = c->rdf_frames;
+  // log.f90:511-514: (j/l + 1) V (zgrid/Lz) histomix_xy / (dV Nconf); the
+  // integer j/l + 1 is 2 for j = l, 1 otherwise.
+  const double zg = c->slices.zgrid, V = c->last_volume, fz = zg / c->last_box[2];
+  int pair = 0;
+  for (int j = 0; j < nsp; ++j)
+    for (int l = j; l < nsp; ++l, ++pair)
+      for (int i = 1; i <= R.lsmax - 2; ++i) {
+        const double dv = xy_shell(R.deltar, i, zg);
+        const double f = (j == l) ? 2.0 : 1.0;
+        for (int sl = 0; sl < nsl; ++sl)
+          out[(static_cast<int64_t>(pair) * (R.lsmax - 2) + i - 1) * nsl + sl] =
+              f * V * fz * h[(static_cast<size_t>(sl) * R.nit + pair) * R.lsmax + i - 1] /
+              (dv * static_cast<double>(nconf));
+      }
+  return TRJ_OK;
+}
+
+int trj_rdf_xy_gcc(trj_ctx* c, int64_t nconf, double* out) {
+  if (!c || !out) return TRJ_EINVAL;
+  const RdfPlan& R = c->rdf;
+  const int nsl = c->slices.n;
+  std::vector<double> g(static_cast<size_t>(std::max(nsl, 1)) * std::max(R.lsmax, 1));
+  int r = trj_rdf_xy_qq(c, g.data());
+  if (r) return r;
+  if (c->rdf_frames == 0) return fail(c, TRJ_ESTATE, "no RDF frame processed");
+  if (nconf <= 0) nconf = c->rdf_frames;
+  // log.f90:520-524: V (zgrid/Lz) gqqxy / (dV Nconf)
+  const double zg = c->slices.zgrid, V = c->last_volume, fz = zg / c->last_box[2];
+  for (int i = 1; i <= R.lsmax - 2; ++i) {
+    const double dv = xy_shell(R.deltar, i, zg);
+    for (int sl = 0; sl < nsl; ++sl)
+      out[static_cast<int64_t>(i - 1) * nsl + sl] =
+          V * fz * g[static_cast<size_t>(sl) * R.lsmax + i - 1] / (dv * static_cast<double>(nconf));
+  }
+  return TRJ_OK;
+}
+
+int trj_sq_xy(trj_ctx* c, double* sqf, double* sqfp, double* sqfq) {
+  if (!c) return TRJ_EINVAL;
+  if (!c->sq_ready || c->slices.n == 0) return fail(c, TRJ_ESTATE, "S(Q) not initialised with slices");
+  if (sqfq && !c->sq_charge) return fail(c, TRJ_ESTATE, "S(Q) not initialised with TRJ_SQ_CHARGE");
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  const size_t ns = static_cast<size_t>(c->sq.nqmax + 1) * c->slices.n;
+  if (sqf) HIPC(c, hipMemcpy(sqf, c->sqf.p, sizeof(double) * ns, hipMemcpyDeviceToHost));
+  if (sqfp) HIPC(c, hipMemcpy(sqfp, c->sqfp.p, sizeof(double) * ns * c->sq.p.nsp, hipMemcpyDeviceToHost));
+  if (sqfq) HIPC(c, hipMemcpy(sqfq, c->sqfq.p, sizeof(double) * ns, hipMemcpyDeviceToHost));
+  return TRJ_OK;
+}
+
+int trj_sq_xy_columns(trj_ctx* c, int64_t nconf, double* sqxy, double* sqpxy, double* sqqxy) {
+  if (!c) return TRJ_EINVAL;
+  if (!c->sq_ready || c->slices.n == 0) return fail(c, TRJ_ESTATE, "S(Q) not initialised with slices");
+  if (c->sq_frames == 0) return fail(c, TRJ_ESTATE, "no S(Q) frame processed");
+  if (sqqxy && !c->sq_charge) return fail(c, TRJ_ESTATE, "S(Q) not initialised with TRJ_SQ_CHARGE");
+  const SqPlan& S = c->sq;
+  const int nsl = c->slices.n, nsp = S.p.nsp, ns = S.nqmax + 1;
+  std::vector<double> f(static_cast<size_t>(ns) * nsl), fp(static_cast<size_t>(ns) * nsl * nsp),
+      fq(sqqxy ? static_cast<size_t>(ns) * nsl : 0);
+  int r = trj_sq_xy(c, f.data(), fp.data(), sqqxy ? fq.data() : nullptr);
+  if (r) return r;
+  if (nconf <= 0) nconf = c->sq_frames;
+  // sq.cuf:135-138: the charge sums divided by each slice's mean q^2 of the
+  // last frame (q2slice, common.cuf:218-232).
+  std::vector<double> q2(nsl, 0.0);
+  if (sqqxy) {
+    std::vector<double> qs(nsl);
+    std::vector<int32_t> qc(nsl);
+    HIPC(c, hipMemcpy(qs.data(), c->sqqsum.p, sizeof(double) * nsl, hipMemcpyDeviceToHost));
+    HIPC(c, hipMemcpy(qc.data(), c->sqqcnt.p, sizeof(int32_t) * nsl, hipMemcpyDeviceToHost));
+    for (int k = 0; k < nsl; ++k) q2[k] = qc[k] > 0 ? qs[k] / qc[k] : 0.0;
+  }
+  // log.f90:366-378: Q = i dq (FP32), values / (Nconf nq(i)); no 1/N (the
+  // kernels divided each vector by the slice counts).
+  for (int i = 1; i <= S.nqmax; ++i) {
+    const double cn = static_cast<double>(nconf) * static_cast<double>(S.nq[i]);
+    const double qv = static_cast<double>(static_cast<float>(i) * S.dq);
+    if (sqxy) {
+      double* row = sqxy + static_cast<int64_t>(i - 1) * (1 + nsl);
+      row[0] = qv;
+      for (int sl = 0; sl < nsl; ++sl) row[1 + sl] = cn > 0.0 ? f[static_cast<size_t>(sl) * ns + i] / cn : 0.0;
+    }
+    if (sqpxy)
+      for (int s = 0; s < nsp; ++s) {
+        double* row = sqpxy + (static_cast<int64_t>(s) * S.nqmax + i - 1) * (1 + nsl);
+        row[0] = qv;
+        for (int sl = 0; sl < nsl; ++sl)
+          row[1 + sl] = cn > 0.0 ? fp[(static_cast<size_t>(sl) * nsp + s) * ns + i] / cn : 0.0;
+      }
+    if (sqqxy) {
+      double* row = sqqxy + static_cast<int64_t>(i - 1) * (1 + nsl);
+      row[0] = qv;
+      for (int sl = 0; sl < nsl; ++sl)
+        row[1 + sl] = (cn > 0.0 && q2[sl] > 0.0) ? (fq[static_cast<size_t>(sl) * ns + i] / q2[sl]) / cn : 0.0;
+    }
+  }
+  return TRJ_OK;
+}
+
 int trj_reset(trj_ctx* c) {
   if (!c) return TRJ_EINVAL;
   hipSetDevice(c->device);
@@ -1013,11 +1403,18 @@ int trj_reset(trj_ctx* c) {
     HIPC(c, hipMemsetAsync(c->gqf.p, 0, sizeof(unsigned long long) * c->rdf.lsmax, c->stream));
     HIPC(c, hipMemsetAsync(c->gqacc.p, 0, sizeof(double) * c->rdf.lsmax, c->stream));
   }
+  const int ngs = c->slices.n > 0 ? c->slices.n : 1;
   if (c->sq_ready) {
-    const int ns = c->sq.nqmax + 1;
+    const int ns = (c->sq.nqmax + 1) * ngs;
     HIPC(c, hipMemsetAsync(c->sqf.p, 0, sizeof(double) * ns, c->stream));
     HIPC(c, hipMemsetAsync(c->sqfp.p, 0, sizeof(double) * ns * c->sq.p.nsp, c->stream));
     if (c->sq_charge) HIPC(c, hipMemsetAsync(c->sqfq.p, 0, sizeof(double) * ns, c->stream));
+  }
+  if (c->rdf_ready && c->slices.n > 0) {
+    const size_t nb = static_cast<size_t>(ngs) * c->rdf.nit * c->rdf.lsmax;
+    HIPC(c, hipMemsetAsync(c->xycounts.p, 0, sizeof(int64_t) * nb, c->stream));
+    HIPC(c, hipMemsetAsync(c->xyacc.p, 0, sizeof(double) * nb, c->stream));
+    if (c->rdf_charge) HIPC(c, hipMemsetAsync(c->xygqacc.p, 0, sizeof(double) * ngs * c->rdf.lsmax, c->stream));
   }
   HIPC(c, hipStreamSynchronize(c->stream));
   c->rdf_frames = 0;
@@ -1093,11 +1490,13 @@ StateHeader state_header(const trj_ctx* c) {
 
 int64_t trj_state_bytes(const trj_ctx* c) {
   if (!c) return TRJ_EINVAL;
+  if (c->slices.n > 0) return TRJ_ESTATE;
   return static_cast<int64_t>(sizeof(StateHeader)) + state_payload(c);
 }
 
 int trj_state_save(trj_ctx* c, void* buf, int64_t bytes) {
   if (!c || !buf) return TRJ_EINVAL;
+  if (c->slices.n > 0) return fail(c, TRJ_ESTATE, "trj_state_save: the context analyses z-slices (trj_set_slices)");
   if (bytes < trj_state_bytes(c)) return fail(c, TRJ_EINVAL, "state buffer too small");
   hipSetDevice(c->device);
   int r = sync_all(c);
@@ -1135,6 +1534,7 @@ int trj_state_save(trj_ctx* c, void* buf, int64_t bytes) {
 
 int trj_state_load(trj_ctx* c, const void* buf, int64_t bytes, int mode) {
   if (!c || !buf) return TRJ_EINVAL;
+  if (c->slices.n > 0) return fail(c, TRJ_ESTATE, "trj_state_load: the context analyses z-slices (trj_set_slices)");
   if (mode != TRJ_STATE_REPLACE && mode != TRJ_STATE_ADD) return fail(c, TRJ_EINVAL, "unknown state load mode");
   if (bytes < static_cast<int64_t>(sizeof(StateHeader))) return fail(c, TRJ_EINVAL, "state blob too short");
   StateHeader h;
@@ -1333,7 +1733,9 @@ int64_t trj_device_bytes(const trj_ctx* c) {
                           &c->cellcount, &c->cellstart, &c->cursor, &c->partial, &c->counts, &c->thr,
                           &c->blk,  &c->blkoff,    &c->sp_off,    &c->u,      &c->rho,    &c->tiles,
                           &c->tile_shell, &c->sqf, &c->sqfp,      &c->axis_kv, &c->axis_shell, &c->listpart,
-                          &c->flags};
+                          &c->flags, &c->qraw, &c->qsort, &c->gqf, &c->gqacc, &c->sqfq, &c->qparm, &c->slkey,
+                          &c->sllast, &c->slcount, &c->slcountp, &c->slqsum, &c->slqcnt, &c->sqqsum, &c->sqqcnt,
+                          &c->xyframe, &c->xycounts, &c->xyacc, &c->xygqf, &c->xygqacc, &c->thr_xy};
   int64_t s = 0;
   for (const DevBuf* b : bufs) s += static_cast<int64_t>(b->cap);
   return s;

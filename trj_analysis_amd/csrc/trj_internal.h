@@ -11,6 +11,8 @@
 namespace trj {
 
 constexpr int kNspMax = 10;  // nspmax, src/core/common.cuf:41
+constexpr int kMaxSlices = 16;              // z-slices of the confined analysis
+constexpr int kKeyMax = kMaxSlices * kNspMax;  // sort keys: slice * nsp + species
 
 // ---------------------------------------------------------------- RDF plan
 struct RdfPlan {
@@ -27,6 +29,8 @@ struct RdfPlan {
 int rdf_lsmax(int ndim, const double box[3], double rcrdf, double deltar, double* rc_eff);
 // Thresholds so that bin(r2) = max{m : thr[m] <= r2} equals nint(sqrt(r2)/deltar).
 std::vector<double> rdf_bin_thresholds(double deltar, int lsmax);
+// Edges of the slice rule k = int(r/deltar) + 1, same layout.
+std::vector<double> rdf_bin_thresholds_floor(double deltar, int lsmax);
 // Smallest d >= 0 with nint(d/L) >= 1 in FP64 (minimum-image switch point).
 double minimage_threshold(double L);
 

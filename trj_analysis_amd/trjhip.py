@@ -36,7 +36,8 @@ EXPORTS = [
     "trj_abi_version", "trj_create", "trj_destroy", "trj_last_error", "trj_rdf_init",
     "trj_sq_init", "trj_sq_plan", "trj_rdf_plan", "trj_frame", "trj_frame_device", "trj_frame_device_q", "trj_sync",
     "trj_rdf_counts", "trj_rdf_gmix", "trj_rdf_first_minimum", "trj_sq_shells", "trj_sq_columns",
-    "trj_sq_rho", "trj_frame_q", "trj_rdf_qq", "trj_rdf_gqq", "trj_sq_qq", "trj_sq_qq_column", "trj_reset", "trj_last_timings", "trj_timing_totals", "trj_timing_reset",
+    "trj_sq_rho", "trj_set_slices", "trj_slice_counts", "trj_rdf_xy", "trj_rdf_xy_gmix", "trj_rdf_xy_qq",
+    "trj_rdf_xy_gcc", "trj_sq_xy", "trj_sq_xy_columns", "trj_frame_q", "trj_rdf_qq", "trj_rdf_gqq", "trj_sq_qq", "trj_sq_qq_column", "trj_reset", "trj_last_timings", "trj_timing_totals", "trj_timing_reset",
     "trj_device_bytes", "trj_set_pipeline", "trj_state_bytes", "trj_state_save", "trj_state_load",
     "trj_lds_atomic_rate", "trj_mfma_rate", "trj_nc_open", "trj_nc_close", "trj_nc_last_error", "trj_nc_info", "trj_nc_read",
     "trj_nc_time", "trj_nc_frame_ref", "trj_nc_frame_ref_q",
@@ -117,6 +118,14 @@ def _declare(L):
     L.trj_frame.argtypes = [vp, vp, C.c_int32, C.c_int32, i32, C.c_int64, d, u32]
     L.trj_frame_q.argtypes = [vp, vp, C.c_int32, C.c_int32, i32, d, C.c_int64, d, u32]
     L.trj_rdf_qq.argtypes = [vp, d]
+    L.trj_set_slices.argtypes = [vp, C.c_int32, C.POINTER(C.c_float), C.c_float]
+    L.trj_slice_counts.argtypes = [vp, i64, i64]
+    L.trj_rdf_xy.argtypes = [vp, d, i64]
+    L.trj_rdf_xy_gmix.argtypes = [vp, C.c_int64, d]
+    L.trj_rdf_xy_qq.argtypes = [vp, d]
+    L.trj_rdf_xy_gcc.argtypes = [vp, C.c_int64, d]
+    L.trj_sq_xy.argtypes = [vp, d, d, d]
+    L.trj_sq_xy_columns.argtypes = [vp, C.c_int64, d, d, d]
     L.trj_rdf_gqq.argtypes = [vp, C.c_int64, d]
     L.trj_sq_qq.argtypes = [vp, d]
     L.trj_sq_qq_column.argtypes = [vp, C.c_int64, d]
@@ -368,6 +377,7 @@ class Context:
         self.rdf: Optional[RdfState] = None
         self.sq_info: Optional[SqInfo] = None
         self.sq_nsp = 0
+        self.nslice = 0
 
     # -- lifetime
     def close(self):
@@ -462,6 +472,67 @@ class Context:
         out = np.zeros((max(r.lsmax - 2, 0), 1 + r.nit))
         self._check(self._L.trj_rdf_gmix(self._h, nconf, _ptr(out, C.c_double)), "trj_rdf_gmix")
         return out
+
+    # ------------------------------------------------------------ slices
+    def set_slices(self, zslice, zgrid):
+        """Confined slices (trj_set_slices); before rdf_init / sq_init.  An
+        empty zslice turns them off."""
+        z = np.ascontiguousarray(np.asarray(zslice, dtype=np.float32).ravel())
+        self._check(self._L.trj_set_slices(self._h, len(z), z.ctypes.data_as(C.POINTER(C.c_float)),
+                                           float(np.float32(zgrid))), "trj_set_slices")
+        self.nslice = len(z)
+
+    def slice_counts(self):
+        """(count (nslice,), countp (nslice, nsp)) of the last frame (rdfxy rule)."""
+        nsp = self.rdf.nsp if self.rdf else self.sq_nsp
+        a = np.zeros(self.nslice, dtype=np.int64)
+        b = np.zeros((self.nslice, nsp), dtype=np.int64)
+        self._check(self._L.trj_slice_counts(self._h, _ptr(a, C.c_int64), _ptr(b, C.c_int64)), "trj_slice_counts")
+        return a, b
+
+    def rdf_xy(self):
+        """(hxy (nslice, nit, lsmax) normalised per frame, raw counts (nslice, nit, lsmax))."""
+        h = np.zeros((self.nslice, self.rdf.nit, self.rdf.lsmax))
+        c = np.zeros((self.nslice, self.rdf.nit, self.rdf.lsmax), dtype=np.int64)
+        self._check(self._L.trj_rdf_xy(self._h, _ptr(h, C.c_double), _ptr(c, C.c_int64)), "trj_rdf_xy")
+        return h, c
+
+    def rdf_xy_gmix(self, nconf: int = 0) -> np.ndarray:
+        """gxy tables (nit, lsmax-2, nslice)."""
+        out = np.zeros((self.rdf.nit, max(self.rdf.lsmax - 2, 0), self.nslice))
+        self._check(self._L.trj_rdf_xy_gmix(self._h, nconf, _ptr(out, C.c_double)), "trj_rdf_xy_gmix")
+        return out
+
+    def rdf_xy_qq(self) -> np.ndarray:
+        out = np.zeros((self.nslice, self.rdf.lsmax))
+        self._check(self._L.trj_rdf_xy_qq(self._h, _ptr(out, C.c_double)), "trj_rdf_xy_qq")
+        return out
+
+    def rdf_xy_gcc(self, nconf: int = 0) -> np.ndarray:
+        """gxy_cc table (lsmax-2, nslice)."""
+        out = np.zeros((max(self.rdf.lsmax - 2, 0), self.nslice))
+        self._check(self._L.trj_rdf_xy_gcc(self._h, nconf, _ptr(out, C.c_double)), "trj_rdf_xy_gcc")
+        return out
+
+    def sq_xy(self, charges: bool = False):
+        """(sqf (nslice, nqmax+1), sqfp (nslice, nsp, nqmax+1), sqfq or None)."""
+        ns = self.sq_info.nqmax + 1
+        f = np.zeros((self.nslice, ns))
+        fp = np.zeros((self.nslice, self.sq_nsp, ns))
+        fq = np.zeros((self.nslice, ns)) if charges else None
+        self._check(self._L.trj_sq_xy(self._h, _ptr(f, C.c_double), _ptr(fp, C.c_double),
+                                      _ptr(fq, C.c_double) if charges else None), "trj_sq_xy")
+        return f, fp, fq
+
+    def sq_xy_columns(self, nconf: int = 0, charges: bool = False):
+        """(sqxy (nqmax, 1+nslice), sqpxy (nsp, nqmax, 1+nslice), sqqxy or None)."""
+        nq = self.sq_info.nqmax
+        a = np.zeros((nq, 1 + self.nslice))
+        b = np.zeros((self.sq_nsp, nq, 1 + self.nslice))
+        q = np.zeros((nq, 1 + self.nslice)) if charges else None
+        self._check(self._L.trj_sq_xy_columns(self._h, nconf, _ptr(a, C.c_double), _ptr(b, C.c_double),
+                                              _ptr(q, C.c_double) if charges else None), "trj_sq_xy_columns")
+        return a, b, q
 
     def rdf_qq(self) -> np.ndarray:
         """Charge-weighted accumulator gqq[k-1], k = 1..lsmax (trj_rdf_qq)."""
