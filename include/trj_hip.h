@@ -329,6 +329,10 @@ int trj_nc_time(trj_nc* h, int64_t frame, double* t);
  * pos_soa x[n] y[n] z[n] and itype[n] (capacity natoms each), n, box. */
 int trj_nc_frame_ref(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_types, double* pos_soa,
                      int32_t* itype, int64_t* n_out, double box[3], int32_t periodic[3]);
+/* keep != 0: trj_nc_frame_ref hands coordinates of non-periodic axes on as
+ * read, without the centring (the reference's behaviour; needed by the
+ * z-slices, whose centres are absolute z values).  Default 0. */
+int trj_nc_keep_nonperiodic(trj_nc* h, int32_t keep);
 /* trj_nc_frame_ref plus the charges of the same atoms, charge[n] (capacity
  * natoms; may be NULL): q(frame, atom) as stored, converted to FP64
  * (netcdf.f90:404-408, 657-658).  Fails with TRJ_EINVAL when charge is given

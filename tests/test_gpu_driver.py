@@ -231,3 +231,81 @@ def test_driver_charges_gqq_sqq(tmp_path):
     fsq = np.loadtxt(os.path.join(tmp_path, "out", "sq.dat"))
     assert fsq.shape == want.shape == (g.nqmax, 8)
     np.testing.assert_allclose(fsq, want, rtol=0, atol=1.1e-7)
+
+
+def test_driver_confined_slices(tmp_path):
+    """rdf_sq_cl_dyn_sqw_conf_ord(6) with RDF and S(Q) (twoDstruc_3D): a slab
+    with zero z cell length and two /INPUT_CONF/ slices.  gxy_j-l.dat and
+    sqxy.dat / sqpxy_j-j.dat are compared with the oracle's slice functions on
+    the stored coordinates (z as read: the slice centres are absolute)."""
+    rng = np.random.default_rng(31)
+    na, nf = 3000, 3
+    cell = np.tile(np.array([20.0, 20.0, 0.0], dtype=np.float32), (nf, 1))
+    org = np.zeros(3, dtype=np.float32)
+    types = rng.integers(1, 3, na).astype(np.int32)
+    types[:2] = [1, 2]
+    coords = np.empty((nf, na, 3), dtype=np.float32)
+    coords[:, :, :2] = rng.random((nf, na, 2)) * 20.0
+    coords[:, :, 2] = -8.0 + rng.random((nf, na)) * 20.0
+    _write(str(tmp_path / "run.nc"), coords, types, org, cell, [5.0, 10.0, 15.0])
+    with open(tmp_path / "input.nml", "w") as fh:
+        fh.write("""&INPUT
+  trj_input_file='run.nc'
+  ndim=3
+  nsp=2
+  nslice=2
+  ncfs_from_to=0 1 0
+  rdf_sq_cl_dyn_sqw_conf_ord=T T F F F T F
+/
+&INPUT_SP
+  sp_types_selected=1 2
+/
+&INPUT_RDF
+  deltar=0.05
+  rcrdf=6.0
+/
+&INPUT_SQ
+  qmax=8.0
+  qmin=5.0
+  bsc=1.0 0.5
+/
+&INPUT_CONF
+  zslice=-3.0 6.0
+  zgrid=5.0
+/
+""")
+    print("case: driver confined slices", flush=True)
+    res = D.run(str(tmp_path / "input.nml"), outdir=str(tmp_path / "out"))
+    assert res.frames_processed == 3
+    out = tmp_path / "out"
+    names = sorted(os.listdir(out))
+    assert names == ["gxy_1-1.dat", "gxy_1-2.dat", "gxy_2-2.dat", "sqpxy_1-1.dat", "sqpxy_2-2.dat", "sqxy.dat"]
+    zs, zg = [-3.0, 6.0], 5.0
+    ls = res.lsmax
+    oh = None
+    g = None
+    acc = None
+    for f in range(nf):
+        pos = coords[f].astype(np.float64)
+        z = coords[f, :, 2]
+        box = np.array([20.0, 20.0, float(abs(z.max() - z.min())) + 8.0])
+        oc, _, ocntp, _, _ = O.rdf_xy_counts(pos, types, 2, box, 6.0, 0.05, zs, zg, lsmax=ls)
+        oh = O.rdf_xy_accumulate(2, oc, ocntp, oh)
+        if g is None:
+            g = O.sq_setup(2, box, 8.0, 5.0)
+        acc = O.sq_frame_xy(pos, types, 2, box, g, [1.0, 0.5], zs, zg, acc=acc)
+    gm = O.rdf_xy_gmix(2, ls, 0.05, box, zg, 3, oh)
+    with open(out / "gxy_1-2.dat") as fh:
+        head = fh.readline().rstrip("\n")
+    assert head == "#     r           g_xy(r,  -3.00)   g_xy(r,   6.00)"
+    for pair, name in enumerate(["gxy_1-1.dat", "gxy_1-2.dat", "gxy_2-2.dat"]):
+        tab = np.loadtxt(out / name)
+        assert tab.shape == (ls - 2, 3)
+        np.testing.assert_allclose(tab[:, 1:], gm[pair], rtol=0, atol=1.1e-7)
+    nq = O.sq_nq(2, g)
+    of, ofp, _ = acc
+    den = 3.0 * nq[1:]
+    sq = np.loadtxt(out / "sqxy.dat")
+    np.testing.assert_allclose(sq[:, 1:], (of[:, 1:] / den).T, rtol=0, atol=1.1e-7)
+    sp2 = np.loadtxt(out / "sqpxy_2-2.dat")
+    np.testing.assert_allclose(sp2[:, 1:], (ofp[:, 1, 1:] / den).T, rtol=0, atol=1.1e-7)

@@ -60,3 +60,40 @@ def test_repeat_and_logicals():
     assert g["a"]["x"] == [0, 0, 0]
     assert g["a"]["y"] == [True, False, True]
     assert g["a"]["s"] == ["run.nc"]
+
+
+def test_confinement_groups_of_the_conf_example():
+    """examples/3D/Conf/input.nml: conf flag with RDF and S(Q) -> z-slices
+    (input.f90:166-191); nslice < 1 switches to one automatic slice."""
+    from trj_analysis_amd.namelist import read_input
+    text = """&INPUT
+  trj_input_file='run.nc'
+  ndim=3
+  nsp=7
+  nslice=1
+  ncfs_from_to=1000 1 1000
+  rdf_sq_cl_dyn_sqw_conf_ord=T T F F F T F
+/
+&INPUT_SP
+  sp_types_selected= 1 2 3 4 5 6 7
+/
+&INPUT_RDF
+  deltar=0.25
+  rcrdf=150.0
+/
+&INPUT_SQ
+  qmax=8.0
+  qmin=1.4
+/
+&INPUT_CONF
+zslice=0.0
+zgrid=4.0
+/
+"""
+    inp = read_input(text)
+    assert inp.slices and inp.nslice == 1 and inp.zslice == [0.0] and inp.zgrid == 4.0
+    assert not inp.auto_zslice
+    auto = read_input(text.replace("  nslice=1\n", ""))
+    assert auto.auto_zslice and auto.nslice == 1
+    plain = read_input(text.replace("T T F F F T F", "T T F F F F F"))
+    assert not plain.slices

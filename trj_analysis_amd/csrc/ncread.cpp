@@ -266,6 +266,7 @@ struct trj_nc {
   std::vector<int32_t> selected;    // the sp_types that fixed the map
   std::vector<double> xyz, tyd, qd;  // frame_ref scratch, kept between frames
   std::vector<int32_t> ty;
+  bool keep_nonperiodic = false;  // hand non-periodic coordinates on as read
   std::string err;
 };
 
@@ -311,6 +312,12 @@ void trj_nc_close(trj_nc* h) {
 }
 
 const char* trj_nc_last_error(const trj_nc* h) { return h ? h->err.c_str() : "null handle"; }
+
+int trj_nc_keep_nonperiodic(trj_nc* h, int32_t keep) {
+  if (!h) return TRJ_EINVAL;
+  h->keep_nonperiodic = keep != 0;
+  return TRJ_OK;
+}
 
 int trj_nc_info(const trj_nc* h, trj_nc_file_info* info) {
   if (!h || !info) return TRJ_EINVAL;
@@ -445,6 +452,7 @@ int trj_nc_frame_ref_q(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_
     } else if (!periodic[k]) {
       shift[k] = 0.5 * (cell[k] - (u - l)) - l;
     }
+    if (h->keep_nonperiodic && !periodic[k]) shift[k] = 0.0;
     periodic[k] = periodic[k] != 0;
   }
   // Selected atoms only, in file order, as SoA: (x - org) + shift, each step

@@ -56,7 +56,7 @@ class RunResult:
     lsmax: Optional[int] = None
     rmingr: Optional[float] = None
     sq_info: Optional[object] = None
-    rdf_counts: Optional[np.ndarray] = None   # (nit, lsmax) int64, as trj_rdf_counts
+    rdf_counts: Optional[np.ndarray] = None   # (nit, lsmax) int64, as trj_rdf_counts (slices: per slice)
     sq_shells: Optional[tuple] = None         # (sqf, sqfp, nq), as trj_sq_shells
     charges: bool = False                     # the file holds q (ex_qc): g_qq and S_qq
     rdf_qq: Optional[np.ndarray] = None       # charge-weighted accumulator, as trj_rdf_qq
@@ -151,6 +151,64 @@ def frame_schedule(ncfs_from_to, nmconf: int, nstep_of: Callable[[int], int],
     return nconf, out
 
 
+def _file_z(tr: T.NcTrajectory, frame: int, per_in):
+    """z of all atoms as read_nc_cfg has it (netcdf.f90:317-325), FP32: minus
+    cell_origin when the /INPUT/ z axis is periodic; and the z cell length."""
+    raw, _, org, cell = tr.read(frame)
+    z = raw[:, 2].astype(np.float32)
+    if per_in[2]:
+        z = z - np.float32(org[2])
+    return z, cell[2]
+
+
+def _setup_slices(inp: StructuralInput, tr: T.NcTrajectory, frame: int, per_in, ctx: T.Context, say):
+    """The z-slices on the first frame read (netcdf.f90:326-346, input.f90:
+    176-191): auto slices (nslice < 1) from the z range of all atoms, in FP32;
+    with a zero z cell length the walls pwall = min z - 6, pwallp = pwall +
+    span + 8 bound the slice centres (and, per frame, the atoms).  Returns the
+    walls or None."""
+    z, lz = _file_z(tr, frame, per_in)
+    zmin, zmax = z.min(), z.max()
+    walls = None
+    if lz == 0.0:
+        pwall = np.float32(zmin - np.float32(6.0))
+        pwallp = np.float32(pwall + np.float32(abs(zmax - zmin) + np.float32(8.0)))
+        walls = (pwall, pwallp)
+    if inp.auto_zslice:
+        if lz != 0.0:
+            raise T.TrjError(T.TRJ_EINVAL, "automatic z-slices (nslice < 1) need a zero z cell length "
+                                           "(netcdf.f90:333-339)")
+        zs = [np.float32((zmax + zmin) / np.float32(2.0))]
+        zg = np.float32((zmax - zmin) / np.float32(10.0))
+        say(f" ** Auto-defined z-slices: zslice(1) = {zs[0]:8.2f}, zgrid = {zg:8.2f}")
+    else:
+        zs = np.asarray(inp.zslice, dtype=np.float32)
+        if inp.zgrid is None:
+            raise T.TrjError(T.TRJ_EINVAL, "zgrid is not set in /INPUT_CONF/")
+        zg = np.float32(inp.zgrid)
+        if walls is not None and (zs.min() < walls[0] or zs.max() > walls[1]):
+            raise T.TrjError(T.TRJ_EINVAL, "z-slices out of box limits, check your input parameters "
+                                           "(netcdf.f90:341-344)")
+    ctx.set_slices(zs, zg)
+    return walls
+
+
+def _slice_z(tr: T.NcTrajectory, frame: int, pos: np.ndarray, box, per, per_in, walls) -> np.ndarray:
+    """z as select_ncdfinput leaves it (netcdf.f90:679-682): wrapped by
+    r - L*int(r/L) in FP32 on a periodic z axis, as read otherwise (checked
+    against the walls, netcdf.f90:348-355)."""
+    if per[2]:
+        z = pos[2].astype(np.float32)
+        L = np.float32(box[2])
+        pos[2] = (z - L * np.trunc(z / L).astype(np.float32)).astype(np.float64)
+    elif walls is not None:
+        z, _ = _file_z(tr, frame, per_in)
+        if z.min() < walls[0] or z.max() > walls[1]:
+            raise T.TrjError(T.TRJ_EINVAL, f"frame {frame + 1}: atoms outside the walls "
+                                           f"[{walls[0]:.2f}, {walls[1]:.2f}] (netcdf.f90:348-355)")
+    return pos
+
+
 def run(inp: Union[str, StructuralInput], *, outdir: str = ".", trj_file: Optional[str] = None,
         precision: str = "fp64", pipeline: Optional[int] = None, device: Optional[int] = None,
         log: Optional[Callable[[str], None]] = None) -> RunResult:
@@ -176,8 +234,12 @@ def run(inp: Union[str, StructuralInput], *, outdir: str = ".", trj_file: Option
         raise T.TrjError(T.TRJ_EINVAL, "ndim must be 2 or 3")
     os.makedirs(outdir, exist_ok=True)
 
+    if inp.slices and ndim != 3:
+        raise T.TrjError(T.TRJ_EINVAL, "system must be 3D to compute confinement properties (input.f90:167-170)")
     with T.NcTrajectory(path) as tr:
         nmconf = tr.nframes
+        if inp.slices:  # slice centres are absolute z: no centring of non-periodic axes
+            tr.keep_nonperiodic(True)
         if not tr.has_type:
             raise T.TrjError(T.TRJ_EINVAL, "the trajectory has no type variable (netcdf.f90:546-548)")
         nconf, schedule = frame_schedule(inp.ncfs_from_to, nmconf, lambda f: _nstep(tr, f - 1), say)
@@ -191,6 +253,7 @@ def run(inp: Union[str, StructuralInput], *, outdir: str = ".", trj_file: Option
         per_in = tuple(bool(p) for p in inp.periodic)
         first = True
         what = (T.TRJ_WANT_RDF if inp.run_rdf else 0) | (T.TRJ_WANT_SQ if inp.run_sq else 0)
+        walls = None
         try:
             if pipeline is not None:
                 ctx.set_pipeline(pipeline)
@@ -203,6 +266,10 @@ def run(inp: Union[str, StructuralInput], *, outdir: str = ".", trj_file: Option
                     pos, itype, box, per = tr.frame_ref(ncstart - 1, sel, per_in)
                 res.t_read += time.perf_counter() - t0
                 res.frames_read.append(ncstart)
+                if inp.slices:
+                    if first:
+                        walls = _setup_slices(inp, tr, ncstart - 1, per_in, ctx, say)
+                    pos = _slice_z(tr, ncstart - 1, pos, box, per, per_in, walls)
                 if first:  # init_modules (util.f90:180-260) on the first frame read
                     t0 = time.perf_counter()
                     res.nsites = len(itype)
@@ -224,7 +291,7 @@ def run(inp: Union[str, StructuralInput], *, outdir: str = ".", trj_file: Option
                                                    f"the first frame read {res.nsites}")
                 ctx.frame(pos, itype, box, what=what, layout=T.TRJ_SOA, charge=qch)
                 res.frames_processed += 1
-                if inp.run_rdf and i == 1:  # rdf.cuf:344-352 (Iconf == 1)
+                if inp.run_rdf and i == 1 and not inp.slices:  # rdf.cuf:343-352 (Iconf == 1)
                     t0 = time.perf_counter()
                     res.rmingr = ctx.rdf_first_minimum()
                     res.t_init += time.perf_counter() - t0
@@ -234,14 +301,22 @@ def run(inp: Union[str, StructuralInput], *, outdir: str = ".", trj_file: Option
             t_files = time.perf_counter()
             # print_results (util.f90:280-300)
             if inp.run_sq:
-                res.sq_shells = ctx.sq_shells()
-                if res.charges:
-                    res.sq_qq = ctx.sq_qq()
+                if inp.slices:  # (sqf, sqfp, sqfq) per slice
+                    res.sq_shells = ctx.sq_xy(charges=res.charges)
+                else:
+                    res.sq_shells = ctx.sq_shells()
+                    if res.charges:
+                        res.sq_qq = ctx.sq_qq()
                 res.files += mod_sq.printSQ(res.nsites, nconf=nconf, outdir=outdir)
             if inp.run_rdf:
-                res.rdf_counts = ctx.rdf_counts()
-                if res.charges:
-                    res.rdf_qq = ctx.rdf_qq()
+                if inp.slices:  # raw slice pair counts (nslice, nit, lsmax)
+                    res.rdf_counts = ctx.rdf_xy()[1]
+                    if res.charges:
+                        res.rdf_qq = ctx.rdf_xy_qq()
+                else:
+                    res.rdf_counts = ctx.rdf_counts()
+                    if res.charges:
+                        res.rdf_qq = ctx.rdf_qq()
                 res.files += mod_rdf.printrdf(0.0, res.lsmax, nconf=nconf, outdir=outdir)
             res.t_total = time.perf_counter() - t_start
             res.t_files = time.perf_counter() - t_files

@@ -125,6 +125,53 @@ module trj_hip
       integer(c_int64_t), value :: nconf
       real(c_double), intent(out) :: out(*)
     end function
+    ! z-slices (twoDstruc_3D), before trj_rdf_init / trj_sq_init; zslice(nslice) FP32
+    integer(c_int) function trj_set_slices(ctx, nslice, zslice, zgrid) bind(C, name="trj_set_slices")
+      import :: c_int, c_ptr, c_int32_t, c_float
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nslice
+      real(c_float), intent(in) :: zslice(*)
+      real(c_float), value :: zgrid
+    end function
+    integer(c_int) function trj_slice_counts(ctx, count, countp) bind(C, name="trj_slice_counts")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: count(*), countp(*)
+    end function
+    integer(c_int) function trj_rdf_xy(ctx, hxy, counts) bind(C, name="trj_rdf_xy")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: hxy(*)
+      integer(c_int64_t), intent(out) :: counts(*)
+    end function
+    integer(c_int) function trj_rdf_xy_gmix(ctx, nconf, out) bind(C, name="trj_rdf_xy_gmix")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nconf
+      real(c_double), intent(out) :: out(*)
+    end function
+    integer(c_int) function trj_rdf_xy_qq(ctx, acc) bind(C, name="trj_rdf_xy_qq")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: acc(*)
+    end function
+    integer(c_int) function trj_rdf_xy_gcc(ctx, nconf, out) bind(C, name="trj_rdf_xy_gcc")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nconf
+      real(c_double), intent(out) :: out(*)
+    end function
+    integer(c_int) function trj_sq_xy(ctx, sqf, sqfp, sqfq) bind(C, name="trj_sq_xy")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: sqf(*), sqfp(*), sqfq(*)
+    end function
+    integer(c_int) function trj_sq_xy_columns(ctx, nconf, sqxy, sqpxy, sqqxy) bind(C, name="trj_sq_xy_columns")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nconf
+      real(c_double), intent(out) :: sqxy(*), sqpxy(*), sqqxy(*)
+    end function
     integer(c_int) function trj_sync(ctx) bind(C, name="trj_sync")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
@@ -248,6 +295,12 @@ module trj_hip
       integer(c_int64_t), intent(out) :: n
       real(c_double), intent(out) :: box(3)
       integer(c_int32_t), intent(inout) :: periodic(3)
+    end function
+    ! keep /= 0: non-periodic coordinates handed on as read (needed by the z-slices)
+    integer(c_int) function trj_nc_keep_nonperiodic(h, keep) bind(C, name="trj_nc_keep_nonperiodic")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: keep
     end function
     ! trj_nc_frame_ref plus charge(natoms): the q variable of the same atoms
     integer(c_int) function trj_nc_frame_ref_q(h, frame, nsp, sp_types, pos_soa, itype, charge, n, box, &

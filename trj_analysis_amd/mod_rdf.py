@@ -13,6 +13,11 @@ With charges (the reference's ex_qc: the trajectory holds a q variable),
 RDF_init(..., charges=True) adds the charge-weighted histogram and printrdf
 its g_qq column (rdf.cuf:466-472, 270-274; log.f90:466-468, 501-503, 565-575).
 
+With z-slices set on the context (the reference's twoDstruc_3D,
+Context.set_slices before RDF_init), the histograms are rdfxy's and printrdf
+writes gxy_j-l.dat (and gxy_cc.dat with charges) instead (log.f90:424-437,
+494-561).
+
 Errors raise trjhip.TrjError (the reference prints and stops, rdf.cuf:241-242).
 Histograms are exact int64 counts (the reference accumulates them in FP32,
 rdf.cuf:306, common.cuf:90).
@@ -61,7 +66,7 @@ def RDFcomp(Nsites: int, Iconf: int, r: np.ndarray, itype: np.ndarray, sidel, un
         r = r.T
     c.frame(np.ascontiguousarray(r), itype, sidel, what=T.TRJ_WANT_RDF, charge=qcharge)
     _state["nframes"] += 1
-    if Iconf == 1:
+    if Iconf == 1 and c.nslice == 0:  # rdf.cuf:343 (not with slices)
         _state["rmingr"] = c.rdf_first_minimum()
 
 
@@ -100,10 +105,50 @@ def _fmt_records(values, width, prec, per_record=16):
     return "\n".join(out)
 
 
+def _ri(i: int, dr: float) -> float:
+    """ri = i*deltar in FP32, as log.f90:485."""
+    return float(np.float32(i) * np.float32(dr))
+
+
+def _slice_head(first: str, label: str, zslice, gap: int) -> str:
+    """'(a,3x,16(label,f7.2,')',gap x:))' without trailing blanks."""
+    return (first + "   " + "".join(f"{label}{z:7.2f})" + " " * gap for z in zslice)).rstrip()
+
+
+def _printrdf_slices(c, nconf: int, outdir: str) -> list:
+    """log.f90:424-437, 494-561: gxy_j-l.dat per pair j <= l and gxy_cc.dat."""
+    nsp = _state["nsp"]
+    g = c.rdf_xy_gmix(nconf)                        # (pair, lsmax-2, nslice)
+    dr = _state["deltar"]
+    head = _slice_head("#     r        ", "g_xy(r,", c.zslice, 3)
+    paths = []
+    pair = 0
+    for j in range(1, nsp + 1):
+        for l in range(j, nsp + 1):
+            path = os.path.join(outdir, f"gxy_{j}-{l}.dat")
+            with open(path, "w") as f:
+                f.write(head + "\n")
+                for i in range(g.shape[1]):
+                    f.write(_fmt_records([_ri(i + 1, dr)] + list(g[pair, i]), 16, 7, 15) + "\n")
+            paths.append(path)
+            pair += 1
+    if _state.get("charges", False):
+        gc = c.rdf_xy_gcc(nconf)
+        path = os.path.join(outdir, "gxy_cc.dat")
+        with open(path, "w") as f:
+            f.write(_slice_head("#     r        ", "g_xy_cc(r,", c.zslice, 4) + "\n")
+            for i in range(gc.shape[0]):
+                f.write(_fmt_records([_ri(i + 1, dr)] + list(gc[i]), 16, 7, 15) + "\n")
+        paths.append(path)
+    return paths
+
+
 def printrdf(rcl: float = 0.0, lsmax: Optional[int] = None, nconf: int = 0, outdir: str = ".") -> list:
     """printrdf (log.f90:414-596): writes gmixsim.dat (nsp <= 6) or gmixsim<k>.dat
     and returns the written paths.  nconf <= 0 uses the number of frames processed."""
     c = _state["ctx"]
+    if c.nslice > 0:
+        return _printrdf_slices(c, nconf, outdir)
     nsp, nit = _state["nsp"], _state["nit"]
     g = c.rdf_gmix(nconf)
     qq = _state.get("charges", False)
@@ -114,7 +159,7 @@ def printrdf(rcl: float = 0.0, lsmax: Optional[int] = None, nconf: int = 0, outd
     paths = []
     if nsp <= 6:
         pairs = [(j, k) for j in range(1, nsp + 1) for k in range(j, nsp + 1)]
-        head = "#       r        " + " " * 9 + qhead + "".join(f"g_{j}{k}(r)" + " " * 9 for j, k in pairs)
+        head = ("#       r        " + " " * 9 + qhead + "".join(f"g_{j}{k}(r)" + " " * 9 for j, k in pairs)).rstrip()
         path = os.path.join(outdir, "gmixsim.dat")
         with open(path, "w") as f:
             f.write(head + "\n")
@@ -134,7 +179,8 @@ def printrdf(rcl: float = 0.0, lsmax: Optional[int] = None, nconf: int = 0, outd
                 col += 1
         for k in range(1, nsp + 1):
             lead = qhead if k == 1 else ""   # log.f90:474-476, 572-573: g_qq in the first file
-            head = "#       r        " + " " * 9 + lead + "".join(f"g_{k}{j}(r)" + " " * 9 for j in range(1, nsp + 1))
+            head = ("#       r        " + " " * 9 + lead
+                    + "".join(f"g_{k}{j}(r)" + " " * 9 for j in range(1, nsp + 1))).rstrip()
             path = os.path.join(outdir, f"gmixsim{k}.dat")
             with open(path, "w") as f:
                 f.write(head + "\n")

@@ -65,10 +65,52 @@ def sq_clear():
     _state.clear()
 
 
+def _fmt15(values) -> str:
+    """Fortran '(15f16.7)' with format reversion."""
+    out = []
+    for k in range(0, len(values), 15):
+        out.append("".join(f"{v:16.7f}" for v in values[k:k + 15]))
+    return "\n".join(out)
+
+
+def _printSQ_slices(c, nconf: int, outdir: str) -> list:
+    """log.f90:333-345, 366-378: sqxy.dat, sqpxy_j-j.dat and sqxy_qq.dat."""
+    from .mod_rdf import _slice_head
+    nsp = _state["nsp"]
+    qq = _state.get("charges", False)
+    cols, pcols, qcols = c.sq_xy_columns(nconf, charges=qq)
+    head = _slice_head("#     Q        ", "S_xy(Q,", c.zslice, 3)
+    paths = []
+    path = os.path.join(outdir, "sqxy.dat")
+    with open(path, "w") as f:
+        f.write(head + "\n")
+        for row in cols:
+            f.write(_fmt15(row) + "\n")
+    paths.append(path)
+    if qq:
+        path = os.path.join(outdir, "sqxy_qq.dat")
+        with open(path, "w") as f:
+            f.write(_slice_head("#     Q        ", "S_xy_qq(Q,", c.zslice, 3) + "\n")
+            for row in qcols:
+                f.write(_fmt15(row) + "\n")
+        paths.append(path)
+    for j in range(nsp):
+        path = os.path.join(outdir, f"sqpxy_{j + 1}-{j + 1}.dat")
+        with open(path, "w") as f:
+            f.write(head + "\n")
+            for row in pcols[j]:
+                f.write(_fmt15(row) + "\n")
+        paths.append(path)
+    return paths
+
+
 def printSQ(Nsites: int, nconf: int = 0, outdir: str = ".") -> list:
-    """printSQ (log.f90:317-412): writes sq.dat and sqmix.dat; returns their paths."""
+    """printSQ (log.f90:317-412): writes sq.dat and sqmix.dat (sqxy*.dat with
+    z-slices); returns their paths."""
     c = _state["ctx"]
     nsp = _state["nsp"]
+    if c.nslice > 0:
+        return _printSQ_slices(c, nconf, outdir)
     cols, mix = c.sq_columns(nconf)
     if _state.get("charges", False):   # S_qq after S_NN (log.f90:351-360, 385-393)
         cols = np.insert(cols, 2, c.sq_qq_column(nconf), axis=1)
@@ -87,7 +129,7 @@ def printSQ(Nsites: int, nconf: int = 0, outdir: str = ".") -> list:
         for row in cols:
             f.write("".join(f"{v:15.7f}" for v in row[:nv]) + f"{int(row[nv]):12d}\n")
     with open(p_mix, "w") as f:
-        f.write("#       Q" + " " * 14 + "".join(f"S_{j}{j}(Q)" + " " * 9 for j in range(1, nsp + 1)) + "\n")
+        f.write(("#       Q" + " " * 14 + "".join(f"S_{j}{j}(Q)" + " " * 9 for j in range(1, nsp + 1))).rstrip() + "\n")
         for row in mix:
             f.write("".join(f"{v:16.7f}" for v in row) + "\n")
     return [p_sq, p_mix]

@@ -86,6 +86,19 @@ class StructuralInput:
     qmin: Optional[float] = None
     bsc: List[float] = field(default_factory=list)
     sp_types_selected: List[int] = field(default_factory=list)
+    nslice: int = -1                 # /INPUT/ (common.cuf:55: default -1)
+    zslice: List[float] = field(default_factory=list)   # /INPUT_CONF/
+    zgrid: Optional[float] = None
+    auto_zslice: bool = False        # nslice < 1: one slice set from the first frame
+
+    @property
+    def run_conf(self) -> bool:
+        return bool(self.flags[5])
+
+    @property
+    def slices(self) -> bool:
+        """twoDstruc_3D (input.f90:186-189): confinement with RDF or S(Q)."""
+        return self.run_conf and (self.run_rdf or self.run_sq)
 
     @property
     def run_rdf(self) -> bool:
@@ -120,6 +133,8 @@ def read_input(text: str) -> StructuralInput:
     if "rdf_sq_cl_dyn_sqw_conf_ord" in i:
         f = [bool(v) for v in i["rdf_sq_cl_dyn_sqw_conf_ord"]] + [False] * 7
         inp.flags = tuple(f[:7])
+    if "nslice" in i:
+        inp.nslice = int(i["nslice"][0])
     inp.bsc = [1.0] * inp.nsp
     sp = g.get("input_sp", {})
     sel = [int(x) for x in sp.get("sp_types_selected", [])]
@@ -144,6 +159,15 @@ def read_input(text: str) -> StructuralInput:
             inp.bsc = (b + [1.0] * inp.nsp)[: inp.nsp]
     if inp.qmin is None:
         inp.qmin = inp.qmax
+    if fl[5]:  # input.f90:166-191
+        if inp.nslice < 1:
+            inp.nslice = 1
+            inp.auto_zslice = True
+        c = g.get("input_conf", {})
+        z = [float(x) for x in c.get("zslice", [])]
+        inp.zslice = (z + [0.0] * inp.nslice)[: inp.nslice]
+        if "zgrid" in c:
+            inp.zgrid = float(c["zgrid"][0])
     return inp
 
 

@@ -40,7 +40,7 @@ EXPORTS = [
     "trj_rdf_xy_gcc", "trj_sq_xy", "trj_sq_xy_columns", "trj_frame_q", "trj_rdf_qq", "trj_rdf_gqq", "trj_sq_qq", "trj_sq_qq_column", "trj_reset", "trj_last_timings", "trj_timing_totals", "trj_timing_reset",
     "trj_device_bytes", "trj_set_pipeline", "trj_state_bytes", "trj_state_save", "trj_state_load",
     "trj_lds_atomic_rate", "trj_mfma_rate", "trj_nc_open", "trj_nc_close", "trj_nc_last_error", "trj_nc_info", "trj_nc_read",
-    "trj_nc_time", "trj_nc_frame_ref", "trj_nc_frame_ref_q",
+    "trj_nc_time", "trj_nc_frame_ref", "trj_nc_frame_ref_q", "trj_nc_keep_nonperiodic",
     "trj_batch_create", "trj_batch_destroy", "trj_batch_last_error",
     "trj_batch_layout", "trj_batch_run", "trj_batch_autotune", "trj_srv_create", "trj_srv_destroy",
     "trj_srv_last_error", "trj_request_footprint", "trj_srv_submit", "trj_srv_poll", "trj_srv_result",
@@ -161,6 +161,7 @@ def _declare(L):
     L.trj_nc_time.argtypes = [vp, C.c_int64, d]
     L.trj_nc_frame_ref.argtypes = [vp, C.c_int64, C.c_int32, i32, d, i32, i64, d, i32]
     L.trj_nc_frame_ref_q.argtypes = [vp, C.c_int64, C.c_int32, i32, d, i32, d, i64, d, i32]
+    L.trj_nc_keep_nonperiodic.argtypes = [vp, C.c_int32]
     L.trj_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(BatchParams)]
     L.trj_batch_destroy.argtypes = [vp]
     L.trj_batch_destroy.restype = None
@@ -230,6 +231,10 @@ class NcTrajectory:
                                         _ptr(typ, C.c_int32) if self.has_type else None,
                                         _ptr(org, C.c_double), _ptr(cell, C.c_double)), "trj_nc_read")
         return pos, (typ if self.has_type else None), org, cell
+
+    def keep_nonperiodic(self, keep: bool = True):
+        """Hand non-periodic coordinates on as read (trj_nc_keep_nonperiodic)."""
+        self._check(self._L.trj_nc_keep_nonperiodic(self._h, int(bool(keep))), "trj_nc_keep_nonperiodic")
 
     def time(self, frame: int) -> float:
         """time(frame) as stored (the reference's nstep is its integer part)."""
@@ -481,6 +486,8 @@ class Context:
         self._check(self._L.trj_set_slices(self._h, len(z), z.ctypes.data_as(C.POINTER(C.c_float)),
                                            float(np.float32(zgrid))), "trj_set_slices")
         self.nslice = len(z)
+        self.zslice = z.copy()
+        self.zgrid = np.float32(zgrid)
 
     def slice_counts(self):
         """(count (nslice,), countp (nslice, nsp)) of the last frame (rdfxy rule)."""
