@@ -341,6 +341,25 @@ int trj_nc_frame_ref_q(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_
                        int32_t* itype, double* charge, int64_t* n_out, double box[3], int32_t periodic[3]);
 
 /* ------------------------------------------------------------------------
+ * Number fluctuations (nrandom > 0 of /INPUT_RDF/; s2D3_sh / s2D2_sh,
+ * rdf.cuf:489-615, s2n.dat of log.f90:449, 482, 526-533): around nrandom
+ * points fixed for the run, each frame adds N_k (atoms with nint(r/dr) <= k
+ * and r^2 < side2, the RDF's rule) and N_k^2 to int64 sums.  The reference
+ * draws its points with CUDA's device generator (seed blockidx + 12345); this
+ * library draws them with SplitMix64 (Steele, Lea and Flood 2014) from `seed`
+ * on the host, so the points differ from the reference's but are returned by
+ * trj_rdf_sampling_points.  Call after trj_rdf_init (which turns sampling
+ * off); nrandom = 0 turns it off; the sums restart at 0.  Not with slices;
+ * the state blob refuses a context that samples. */
+int trj_rdf_sampling(trj_ctx* ctx, int32_t nrandom, uint64_t seed);
+/* The unit points, pts[nrandom][3] in (0, 1]; the frame's point is box * pts. */
+int trj_rdf_sampling_points(trj_ctx* ctx, double* pts);
+/* gi[k], g2i[k], k = 0..lsmax: sums of N_k and N_k^2 over points and frames. */
+int trj_rdf_s2n_sums(trj_ctx* ctx, int64_t* gi, int64_t* g2i);
+/* s2n.dat column for r_i, i = 1..lsmax-2: g2i/(Nconf nrandom) - (gi/(Nconf nrandom))^2. */
+int trj_rdf_s2n(trj_ctx* ctx, int64_t nconf, double* out);
+
+/* ------------------------------------------------------------------------
  * Confined slices: the reference's twoDstruc_3D analysis (input.f90:166-191:
  * rdf_sq_cl_dyn_sqw_conf_ord(6) with RDF or S(Q); /INPUT_CONF/ zslice, zgrid).
  * Atoms are assigned to the first nearest zslice when |zslice - z| <= zgrid/2

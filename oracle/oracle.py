@@ -102,6 +102,9 @@ def _declare(L):
     L.or_rdf_gqq.restype = None
     L.or_sq_frame_q.argtypes = [C.c_int64, _D, _D, _D, _D, C.c_int, _D, C.POINTER(SqGrid), _D]
     L.or_sq_frame_q.restype = None
+    L.or_s2n.argtypes = [C.c_int64, _D, _D, C.c_int64, _D, _D, _D, C.c_int, _D, C.c_double, C.c_double, C.c_int,
+                         _I64, _I64]
+    L.or_s2n.restype = None
     _F = C.POINTER(C.c_float)
     L.or_slice_of.argtypes = [C.c_float, C.c_int, _F, C.c_float]
     L.or_slice_of.restype = C.c_int
@@ -417,4 +420,23 @@ def sq_frame_xy(pos, itype, nsp, box, g: SqGrid, bsc, zslice, zgrid, q=None, acc
                          _p(bs, C.c_double), ns, _p(zs, C.c_float), float(np.float32(zgrid)),
                          _p(sqf, C.c_double), _p(sqfp, C.c_double), _p(sqfq, C.c_double))
     return acc
+
+
+def s2n_sums(pts, pos, ndim, box, periodic, rcrdf, deltar, lsmax=None, gi=None, g2i=None):
+    """Add one frame of the number-fluctuation sums for unit points pts (np, 3);
+    pos raw (N, 3), wrapped here.  Returns (gi, g2i) int64 (lsmax+1,)."""
+    g = geometry(ndim, box, periodic, rcrdf)
+    if lsmax is None:
+        lsmax = rdf_lsmax(ndim, box, rcrdf, deltar)
+    w = wrap(pos, box, periodic, ndim)
+    x, y, z = (np.ascontiguousarray(w[:, k]) for k in range(3))
+    pt = _f64(pts)
+    b = _f64(box)
+    if gi is None:
+        gi = np.zeros(lsmax + 1, dtype=np.int64)
+        g2i = np.zeros(lsmax + 1, dtype=np.int64)
+    lib().or_s2n(len(pt), _p(pt, C.c_double), _p(b, C.c_double), len(x), _p(x, C.c_double), _p(y, C.c_double),
+                 _p(z, C.c_double), ndim, _p(g.lmi, C.c_double), g.side2, deltar, lsmax, _p(gi, C.c_int64),
+                 _p(g2i, C.c_int64))
+    return gi, g2i
 

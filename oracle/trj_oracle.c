@@ -844,3 +844,41 @@ void or_sq_frame_xy(int64_t n, const double* x, const double* y, const double* z
   free(kv);
   free(sh);
 }
+
+/* ------------------------------------------------------ number fluctuations */
+
+/* s2D3_sh / s2D2_sh (rdf.cuf:489-615) for given unit points (pts[p*3 + k] in
+ * (0,1], point = box * pts): per point the atoms with r^2 < side2 binned by
+ * k = nint(r/deltar) in FP64 (the image rule of or_rdf_brute), then the
+ * cumulative N_k = sum_{b<=k} h_b, k = 0..lsmax, added as gi[k] += N_k and
+ * g2i[k] += N_k^2 (int64). */
+void or_s2n(int64_t np, const double* pts, const double box[3], int64_t n, const double* x, const double* y,
+            const double* z, int ndim, const double lmi[3], double side2, double deltar, int lsmax, int64_t* gi,
+            int64_t* g2i) {
+  int64_t* h = (int64_t*)malloc(sizeof(int64_t) * (lsmax + 1));
+  for (int64_t p = 0; p < np; ++p) {
+    for (int k = 0; k <= lsmax; ++k) h[k] = 0;
+    double xp = box[0] * pts[3 * p], yp = box[1] * pts[3 * p + 1], zp = box[2] * pts[3 * p + 2];
+    for (int64_t j = 0; j < n; ++j) {
+      double dx = xp - x[j], dy = yp - y[j];
+      dx = dx - lmi[0] * round(dx / lmi[0]);
+      dy = dy - lmi[1] * round(dy / lmi[1]);
+      double r2 = dx * dx + dy * dy;
+      if (ndim == 3) {
+        double dz = zp - z[j];
+        dz = dz - lmi[2] * round(dz / lmi[2]);
+        r2 += dz * dz;
+      }
+      if (!(r2 < side2)) continue;
+      long k = or_nint_d(sqrt(r2) / deltar);
+      if (k >= 0 && k <= lsmax) h[k] += 1;
+    }
+    int64_t nk = 0;
+    for (int k = 0; k <= lsmax; ++k) {
+      nk += h[k];
+      gi[k] += nk;
+      g2i[k] += nk * nk;
+    }
+  }
+  free(h);
+}

@@ -50,6 +50,9 @@
  *   or_sq_frame_xy       one slice with every atom equals or_sq_frame (ndim 2)
  *                        divided by N; a single atom per slice gives 1 per vector;
  *                        unit charges give the bsc = 1 sum
+ *   or_s2n               a single atom at a known distance; a point on a site of a
+ *                        simple-cubic lattice, whose N_k is an integer-vector count;
+ *                        N_k at lsmax equals the atoms within the cutoff
  *   or_sq_frame_q        unit charges equal or_sq_frame's S with bsc = 1;
  *                        uniform charge c scales by c^2; single atom gives q^2
  * None is "parity unpinned".
@@ -191,6 +194,12 @@ void or_rdf_xy_gmix(int nsp, int lsmax, int nslice, double deltar, const double 
 void or_sq_frame_xy(int64_t n, const double* x, const double* y, const double* z, const int32_t* itype,
                     const double* q, int nsp, const double box[3], const or_sqgrid* g, const double* bsc,
                     int nslice, const float* zslice, float zgrid, double* sqf, double* sqfp, double* sqfq);
+
+/* s2D3_sh / s2D2_sh for given unit points: int64 sums of the cumulative
+ * counts N_k and N_k^2 over points (rdf.cuf:489-615). */
+void or_s2n(int64_t np, const double* pts, const double box[3], int64_t n, const double* x, const double* y,
+            const double* z, int ndim, const double lmi[3], double side2, double deltar, int lsmax, int64_t* gi,
+            int64_t* g2i);
 
 #ifdef __cplusplus
 }

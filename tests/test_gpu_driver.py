@@ -309,3 +309,29 @@ def test_driver_confined_slices(tmp_path):
     np.testing.assert_allclose(sq[:, 1:], (of[:, 1:] / den).T, rtol=0, atol=1.1e-7)
     sp2 = np.loadtxt(out / "sqpxy_2-2.dat")
     np.testing.assert_allclose(sp2[:, 1:], (ofp[:, 1, 1:] / den).T, rtol=0, atol=1.1e-7)
+
+
+def test_driver_number_fluctuations(tmp_path):
+    """nrandom in /INPUT_RDF/: s2n.dat holds g2i/(Nconf nrandom) -
+    (gi/(Nconf nrandom))^2 for r_i, i = 1..lsmax-2 (log.f90:526-533)."""
+    rng = np.random.default_rng(41)
+    na, nf = 2000, 2
+    cell = np.tile(np.array([14.0, 14.0, 14.0], dtype=np.float32), (nf, 1))
+    types = rng.integers(1, 3, na).astype(np.int32)
+    types[:2] = [1, 2]
+    coords = (rng.random((nf, na, 3)) * 14.0).astype(np.float32)
+    _write(str(tmp_path / "run.nc"), coords, types, np.zeros(3, np.float32), cell, [1.0, 2.0])
+    _nml(str(tmp_path / "input.nml"), 2, "1 2", "0 1 0", "T F", "1.0 1.0")
+    with open(tmp_path / "input.nml") as fh:
+        text = fh.read().replace("  rcrdf=6.0\n", "  rcrdf=6.0\n  nrandom=64\n")
+    with open(tmp_path / "input.nml", "w") as fh:
+        fh.write(text)
+    print("case: driver number fluctuations", flush=True)
+    res = D.run(str(tmp_path / "input.nml"), outdir=str(tmp_path / "out"))
+    gi, g2i = res.s2n_sums
+    assert gi[-1] > 0
+    tab = np.loadtxt(os.path.join(tmp_path, "out", "s2n.dat"))
+    i = np.arange(1, res.lsmax - 1)
+    m = 2.0 * 64
+    assert tab.shape == (res.lsmax - 2, 2)
+    np.testing.assert_allclose(tab[:, 1], g2i[i] / m - (gi[i] / m) ** 2, rtol=0, atol=1.1e-5)

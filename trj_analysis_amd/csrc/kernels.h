@@ -105,6 +105,23 @@ void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start
 void launch_rdf_allpairs(const PairArgs& a, const double* pos, const int32_t* itype, int64_t n,
                          int nblocks, uint32_t* partial, hipStream_t s, int64_t stride = 0,
                          const int32_t* range = nullptr);
+// Number fluctuations around fixed random points (s2D3_sh / s2D2_sh).
+struct S2nArgs {
+  const double* pts;    // unit points, [3][np], in (0, 1]
+  int np, ndim, lsmax;
+  double box[3], lmi[3], half[3], side2;
+  float inv_dr;
+  const double* thr;    // the RDF's nint-rule edges
+  const double* pos;    // atoms, SoA of stride n (cell-sorted when start is given)
+  int64_t n;
+  const int32_t* start; // cell starts of the RDF's cell list, or NULL: all atoms
+  CellGrid g;
+  unsigned long long* gi;   // [lsmax+1] sums of N_k over points
+  unsigned long long* g2i;  // [lsmax+1] sums of N_k^2
+};
+size_t s2n_lds_bytes(int lsmax);
+void launch_s2n(const S2nArgs& a, hipStream_t s);
+
 // Slice histograms of the frame into the accumulators, then cleared:
 // counts += frame; hxy += frame / (countp_j countp_l); gq (may be NULL):
 // gqacc += (gq / qp[0]) / (count^2 qsum/qcnt).

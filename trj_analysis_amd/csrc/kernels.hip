@@ -379,6 +379,87 @@ struct Stencil {
   int off[14][3];
 };
 
+// Number fluctuations around fixed random points (s2D3_sh / s2D2_sh,
+// rdf.cuf:489-615): block p counts the atoms in bins k = nint(r/dr) (r^2 <
+// side2, the RDF's rule and edges) around point p, then adds the cumulative
+// counts N_k = sum_{b <= k} h_b and N_k^2 to gi / g2i (int64, exact).  With
+// start (cell-sorted atoms) only the 3^ndim cells around the point are read.
+__global__ __launch_bounds__(256) void k_s2n(S2nArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
+  uint32_t* part = hist + ((a.lsmax + 1 + 3) & ~3);
+  const int nb = a.lsmax + 1;
+  for (int b = threadIdx.x; b < nb; b += 256) hist[b] = 0u;
+  __syncthreads();
+  const int p = blockIdx.x;
+  double xp[3];
+  for (int k = 0; k < 3; ++k) xp[k] = k < a.ndim ? a.box[k] * a.pts[static_cast<int64_t>(k) * a.np + p] : 0.0;
+  const double* __restrict__ px = a.pos;
+  const double* __restrict__ py = a.pos + a.n;
+  const double* __restrict__ pz = a.pos + 2 * a.n;
+  auto visit = [&](int64_t j0, int64_t j1) {
+    for (int64_t j = j0 + threadIdx.x; j < j1; j += 256) {
+      const double dx = min_image(xp[0] - px[j], a.lmi[0], a.half[0]);
+      const double dy = min_image(xp[1] - py[j], a.lmi[1], a.half[1]);
+      double r2 = dx * dx + dy * dy;
+      if (a.ndim == 3) {
+        const double dz = min_image(xp[2] - pz[j], a.lmi[2], a.half[2]);
+        r2 += dz * dz;
+      }
+      if (r2 < a.side2) {
+        const int k = bin_of(r2, a.thr, a.inv_dr, a.lsmax);
+        if (k <= a.lsmax) atomicAdd(&hist[k], 1u);
+      }
+    }
+  };
+  if (a.start) {
+    int c[3] = {0, 0, 0};
+    for (int k = 0; k < a.ndim; ++k) {
+      int ck = static_cast<int>(floor(xp[k] * a.g.inv_h[k]));
+      c[k] = ck < 0 ? 0 : (ck >= a.g.nc[k] ? a.g.nc[k] - 1 : ck);
+    }
+    const int zr = a.ndim == 3 ? 1 : 0;
+    for (int dz = -zr; dz <= zr; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          int nx = c[0] + dx, ny = c[1] + dy, nz = c[2] + dz;
+          if (a.g.periodic[0]) nx = (nx + a.g.nc[0]) % a.g.nc[0]; else if (nx < 0 || nx >= a.g.nc[0]) continue;
+          if (a.g.periodic[1]) ny = (ny + a.g.nc[1]) % a.g.nc[1]; else if (ny < 0 || ny >= a.g.nc[1]) continue;
+          if (a.ndim == 3) {
+            if (a.g.periodic[2]) nz = (nz + a.g.nc[2]) % a.g.nc[2]; else if (nz < 0 || nz >= a.g.nc[2]) continue;
+          }
+          const int cell = (nz * a.g.nc[1] + ny) * a.g.nc[0] + nx;
+          visit(a.start[cell], a.start[cell + 1]);
+        }
+  } else {
+    visit(0, a.n);
+  }
+  __syncthreads();
+  // Cumulative counts: each thread sums a contiguous run of bins, thread 0
+  // scans the 256 run totals, then each thread finishes its run.
+  const int per = (nb + 255) / 256;
+  const int b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
+  uint32_t run = 0;
+  for (int b = b0; b < b1; ++b) run += hist[b];
+  part[threadIdx.x] = run;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (int t = 0; t < 256; ++t) {
+      const uint32_t v = part[t];
+      part[t] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  unsigned long long nk = part[threadIdx.x];
+  for (int b = b0; b < b1; ++b) {
+    nk += hist[b];
+    atomicAdd(&a.gi[b], nk);
+    atomicAdd(&a.g2i[b], nk * nk);
+  }
+}
+
 // Returns the bin the pair was added to (0: none).
 __device__ inline int pair_accumulate(const PairArgs& a, const double* thr, uint32_t* hist,
                                       bool ok, double xi, double yi, double zi, int ti, double xj,
@@ -2047,6 +2128,13 @@ void launch_cell_count_slices(const double* pos, const int32_t* key, int64_t n, 
                               int nslice, int32_t* cellid, int32_t* count, hipStream_t s) {
   hipLaunchKernelGGL(k_cell_count_slices, dim3(grid1d(n, 256)), dim3(256), 0, s, pos, key, n, nsp, g, nslice,
                      cellid, count);
+}
+
+size_t s2n_lds_bytes(int lsmax) { return sizeof(uint32_t) * (((lsmax + 1 + 3) & ~3) + 256); }
+
+void launch_s2n(const S2nArgs& a, hipStream_t s) {
+  if (a.np <= 0) return;
+  hipLaunchKernelGGL(k_s2n, dim3(a.np), dim3(256), s2n_lds_bytes(a.lsmax), s, a);
 }
 
 void launch_xy_finish(int nslice, int nsp, int lsmax, int64_t* frame, const int32_t* count, const int32_t* countp,

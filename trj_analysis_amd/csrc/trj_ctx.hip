@@ -62,6 +62,11 @@ struct trj_ctx {
   // totals, normalised accumulators; charges: fixed-point frame sums and
   // their accumulator), and the floor-rule bin edges.
   SliceSet slices{};
+  // Number fluctuations (nrandom > 0): fixed unit points and the int64 sums
+  // of N_k and N_k^2 over points and frames.
+  int nrand = 0;
+  std::vector<double> rpts_host;  // [nrand][3]
+  DevBuf rpts, s2gi, s2g2i;
   DevBuf slkey, sllast, slcount, slcountp, slqsum, slqcnt, sqqsum, sqqcnt, xyframe, xycounts, xyacc, xygqf,
       xygqacc, thr_xy;
   // pinned staging for host inputs (double-buffered)
@@ -463,6 +468,30 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
       }
       launch_rdf_merge(P<uint32_t>(c->partial), nblocks, nbins, P<int64_t>(c->counts), rs);
       if (rdf_q) launch_qq_finish(P<unsigned long long>(c->gqf), R.lsmax, P<double>(c->qparm), P<double>(c->gqacc), rs);
+      if (c->nrand > 0) {
+        // Number fluctuations around the fixed random points (rdf.cuf:243-245,
+        // 251-253), on this frame's cell list when there is one.
+        S2nArgs sa{};
+        sa.pts = P<double>(c->rpts);
+        sa.np = c->nrand;
+        sa.ndim = ndim;
+        sa.lsmax = R.lsmax;
+        for (int k = 0; k < 3; ++k) {
+          sa.box[k] = box[k];
+          sa.lmi[k] = g.lmi[k];
+          sa.half[k] = g.half[k];
+        }
+        sa.side2 = g.side2;
+        sa.inv_dr = pa.inv_dr;
+        sa.thr = P<double>(c->thr);
+        sa.pos = cells_ok ? P<double>(c->spos) : P<double>(c->pos);
+        sa.n = n;
+        sa.start = cells_ok ? P<int32_t>(c->cellstart) : nullptr;
+        sa.g = cg;
+        sa.gi = P<unsigned long long>(c->s2gi);
+        sa.g2i = P<unsigned long long>(c->s2g2i);
+        launch_s2n(sa, rs);
+      }
     }
     HIPC(c, hipGetLastError());
     c->rdf_frames += 1;
@@ -629,7 +658,8 @@ void trj_destroy(trj_ctx* c) {
                     &c->tile_shell, &c->sqf, &c->sqfp,      &c->axis_kv, &c->axis_shell, &c->listpart,
                     &c->flags, &c->qraw, &c->qsort, &c->gqf, &c->gqacc, &c->sqfq, &c->qparm, &c->slkey,
                     &c->sllast, &c->slcount, &c->slcountp, &c->slqsum, &c->slqcnt, &c->sqqsum, &c->sqqcnt,
-                    &c->xyframe, &c->xycounts, &c->xyacc, &c->xygqf, &c->xygqacc, &c->thr_xy};
+                    &c->xyframe, &c->xycounts, &c->xyacc, &c->xygqf, &c->xygqacc, &c->thr_xy, &c->rpts,
+                    &c->s2gi, &c->s2g2i};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   for (int k = 0; k < 2; ++k) {
@@ -755,6 +785,8 @@ int trj_rdf_init(trj_ctx* c, const trj_rdf_params* p, const double box[3], int32
   c->rdf = R;
   c->rdf_charge = qq;
   c->rdf_ready = true;
+  c->nrand = 0;  // sampling points follow the plan: trj_rdf_sampling again
+  c->rpts_host.clear();
   c->rdf_frames = 0;
   if (lsmax_out) *lsmax_out = R.lsmax;
   if (nit_out) *nit_out = R.nit;
@@ -1230,6 +1262,89 @@ int trj_sq_rho(trj_ctx* c, int64_t nvec, const int32_t* kv, double* rho) {
   return TRJ_OK;
 }
 
+// ------------------------------------------------------ number fluctuations
+
+namespace {
+// SplitMix64 (Steele, Lea and Flood, OOPSLA 2014): the state advances by the
+// golden-ratio increment and each output is its variant-13 mix.
+uint64_t splitmix64(uint64_t& x) {
+  uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+}  // namespace
+
+int trj_rdf_sampling(trj_ctx* c, int32_t nrandom, uint64_t seed) {
+  if (!c) return TRJ_EINVAL;
+  if (!c->rdf_ready) return fail(c, TRJ_ESTATE, "trj_rdf_sampling comes after trj_rdf_init");
+  if (c->slices.n > 0) return fail(c, TRJ_EINVAL, "number fluctuations are not sampled with slices (rdf.cuf:226-245)");
+  if (nrandom < 0 || nrandom > (1 << 20)) return fail(c, TRJ_EINVAL, "nrandom must be in 0..1048576");
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  c->nrand = 0;
+  c->rpts_host.clear();
+  if (nrandom == 0) return TRJ_OK;
+  // Unit points in (0, 1], fixed for the run as the reference's per-block
+  // seeds are (rdf.cuf:507-518); [3][nrandom] on the device.
+  std::vector<double> dev(static_cast<size_t>(3) * nrandom);
+  c->rpts_host.resize(static_cast<size_t>(3) * nrandom);
+  uint64_t x = seed;
+  for (int p = 0; p < nrandom; ++p)
+    for (int k = 0; k < 3; ++k) {
+      const double u = static_cast<double>((splitmix64(x) >> 11) + 1) * 0x1.0p-53;
+      c->rpts_host[static_cast<size_t>(p) * 3 + k] = u;
+      dev[static_cast<size_t>(k) * nrandom + p] = u;
+    }
+  const size_t nb = static_cast<size_t>(c->rdf.lsmax) + 1;
+  if ((r = ensure(c, c->rpts, sizeof(double) * dev.size()))) return r;
+  if ((r = ensure(c, c->s2gi, sizeof(unsigned long long) * nb))) return r;
+  if ((r = ensure(c, c->s2g2i, sizeof(unsigned long long) * nb))) return r;
+  if (s2n_lds_bytes(c->rdf.lsmax) > kPairLdsLimit) return fail(c, TRJ_ELIMIT, "rdf: lsmax too large for the sampling");
+  HIPC(c, hipMemcpy(c->rpts.p, dev.data(), sizeof(double) * dev.size(), hipMemcpyHostToDevice));
+  HIPC(c, hipMemset(c->s2gi.p, 0, sizeof(unsigned long long) * nb));
+  HIPC(c, hipMemset(c->s2g2i.p, 0, sizeof(unsigned long long) * nb));
+  c->nrand = nrandom;
+  return TRJ_OK;
+}
+
+int trj_rdf_sampling_points(trj_ctx* c, double* pts) {
+  if (!c || !pts) return TRJ_EINVAL;
+  if (c->nrand == 0) return fail(c, TRJ_ESTATE, "no sampling points (trj_rdf_sampling)");
+  std::memcpy(pts, c->rpts_host.data(), sizeof(double) * c->rpts_host.size());
+  return TRJ_OK;
+}
+
+int trj_rdf_s2n_sums(trj_ctx* c, int64_t* gi, int64_t* g2i) {
+  if (!c) return TRJ_EINVAL;
+  if (c->nrand == 0) return fail(c, TRJ_ESTATE, "no sampling points (trj_rdf_sampling)");
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  const size_t nb = static_cast<size_t>(c->rdf.lsmax) + 1;
+  if (gi) HIPC(c, hipMemcpy(gi, c->s2gi.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost));
+  if (g2i) HIPC(c, hipMemcpy(g2i, c->s2g2i.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost));
+  return TRJ_OK;
+}
+
+int trj_rdf_s2n(trj_ctx* c, int64_t nconf, double* out) {
+  if (!c || !out) return TRJ_EINVAL;
+  const int ls = c->rdf.lsmax;
+  std::vector<int64_t> gi(static_cast<size_t>(ls) + 1), g2i(static_cast<size_t>(ls) + 1);
+  int r = trj_rdf_s2n_sums(c, gi.data(), g2i.data());
+  if (r) return r;
+  if (c->rdf_frames == 0) return fail(c, TRJ_ESTATE, "no RDF frame processed");
+  if (nconf <= 0) nconf = c->rdf_frames;
+  // log.f90:526-533: g2i/(Nconf nrandom) - (gi/(Nconf nrandom))^2, i = 1..lsmax-2
+  const double m = static_cast<double>(nconf) * static_cast<double>(c->nrand);
+  for (int i = 1; i <= ls - 2; ++i) {
+    const double a = static_cast<double>(gi[i]) / m;
+    out[i - 1] = static_cast<double>(g2i[i]) / m - a * a;
+  }
+  return TRJ_OK;
+}
+
 // ---------------------------------------------------------------- slices
 
 int trj_slice_counts(trj_ctx* c, int64_t* count, int64_t* countp) {
@@ -1410,6 +1525,10 @@ int trj_reset(trj_ctx* c) {
     HIPC(c, hipMemsetAsync(c->sqfp.p, 0, sizeof(double) * ns * c->sq.p.nsp, c->stream));
     if (c->sq_charge) HIPC(c, hipMemsetAsync(c->sqfq.p, 0, sizeof(double) * ns, c->stream));
   }
+  if (c->nrand > 0) {
+    HIPC(c, hipMemsetAsync(c->s2gi.p, 0, sizeof(unsigned long long) * (c->rdf.lsmax + 1), c->stream));
+    HIPC(c, hipMemsetAsync(c->s2g2i.p, 0, sizeof(unsigned long long) * (c->rdf.lsmax + 1), c->stream));
+  }
   if (c->rdf_ready && c->slices.n > 0) {
     const size_t nb = static_cast<size_t>(ngs) * c->rdf.nit * c->rdf.lsmax;
     HIPC(c, hipMemsetAsync(c->xycounts.p, 0, sizeof(int64_t) * nb, c->stream));
@@ -1490,13 +1609,14 @@ StateHeader state_header(const trj_ctx* c) {
 
 int64_t trj_state_bytes(const trj_ctx* c) {
   if (!c) return TRJ_EINVAL;
-  if (c->slices.n > 0) return TRJ_ESTATE;
+  if (c->slices.n > 0 || c->nrand > 0) return TRJ_ESTATE;
   return static_cast<int64_t>(sizeof(StateHeader)) + state_payload(c);
 }
 
 int trj_state_save(trj_ctx* c, void* buf, int64_t bytes) {
   if (!c || !buf) return TRJ_EINVAL;
   if (c->slices.n > 0) return fail(c, TRJ_ESTATE, "trj_state_save: the context analyses z-slices (trj_set_slices)");
+  if (c->nrand > 0) return fail(c, TRJ_ESTATE, "trj_state_save: the state blob does not carry the number-fluctuation sums");
   if (bytes < trj_state_bytes(c)) return fail(c, TRJ_EINVAL, "state buffer too small");
   hipSetDevice(c->device);
   int r = sync_all(c);
@@ -1535,6 +1655,7 @@ int trj_state_save(trj_ctx* c, void* buf, int64_t bytes) {
 int trj_state_load(trj_ctx* c, const void* buf, int64_t bytes, int mode) {
   if (!c || !buf) return TRJ_EINVAL;
   if (c->slices.n > 0) return fail(c, TRJ_ESTATE, "trj_state_load: the context analyses z-slices (trj_set_slices)");
+  if (c->nrand > 0) return fail(c, TRJ_ESTATE, "trj_state_load: the state blob does not carry the number-fluctuation sums");
   if (mode != TRJ_STATE_REPLACE && mode != TRJ_STATE_ADD) return fail(c, TRJ_EINVAL, "unknown state load mode");
   if (bytes < static_cast<int64_t>(sizeof(StateHeader))) return fail(c, TRJ_EINVAL, "state blob too short");
   StateHeader h;
@@ -1735,7 +1856,8 @@ int64_t trj_device_bytes(const trj_ctx* c) {
                           &c->tile_shell, &c->sqf, &c->sqfp,      &c->axis_kv, &c->axis_shell, &c->listpart,
                           &c->flags, &c->qraw, &c->qsort, &c->gqf, &c->gqacc, &c->sqfq, &c->qparm, &c->slkey,
                           &c->sllast, &c->slcount, &c->slcountp, &c->slqsum, &c->slqcnt, &c->sqqsum, &c->sqqcnt,
-                          &c->xyframe, &c->xycounts, &c->xyacc, &c->xygqf, &c->xygqacc, &c->thr_xy};
+                          &c->xyframe, &c->xycounts, &c->xyacc, &c->xygqf, &c->xygqacc, &c->thr_xy, &c->rpts,
+                          &c->s2gi, &c->s2g2i};
   int64_t s = 0;
   for (const DevBuf* b : bufs) s += static_cast<int64_t>(b->cap);
   return s;

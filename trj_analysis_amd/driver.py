@@ -61,6 +61,7 @@ class RunResult:
     charges: bool = False                     # the file holds q (ex_qc): g_qq and S_qq
     rdf_qq: Optional[np.ndarray] = None       # charge-weighted accumulator, as trj_rdf_qq
     sq_qq: Optional[np.ndarray] = None        # charge-weighted shell sums, as trj_sq_qq
+    s2n_sums: Optional[tuple] = None          # (gi, g2i) of the number fluctuations (nrandom > 0)
     files: List[str] = field(default_factory=list)
     t_read: float = 0.0             # seconds in trj_nc_frame_ref (host read + conversion)
     t_total: float = 0.0            # seconds from the first read to the written files
@@ -276,7 +277,8 @@ def run(inp: Union[str, StructuralInput], *, outdir: str = ".", trj_file: Option
                     if inp.run_rdf:
                         res.lsmax = mod_rdf.RDF_init(nsp, ndim=ndim, deltar=inp.deltar, rcrdf=inp.rcrdf,
                                                      sidel=box, periodic=tuple(bool(p) for p in per),
-                                                     charges=res.charges, ctx=ctx)
+                                                     charges=res.charges,
+                                                     nrandom=0 if inp.slices else inp.nrandom, ctx=ctx)
                     if inp.run_sq:
                         res.sq_info = mod_sq.sq_init(len(itype), nsp, ndim=ndim, qmax=inp.qmax, qmin=inp.qmin,
                                                      bsc=bsc, sidel=box, precision=precision,
@@ -317,6 +319,8 @@ def run(inp: Union[str, StructuralInput], *, outdir: str = ".", trj_file: Option
                     res.rdf_counts = ctx.rdf_counts()
                     if res.charges:
                         res.rdf_qq = ctx.rdf_qq()
+                    if inp.nrandom > 0:
+                        res.s2n_sums = ctx.rdf_s2n_sums()
                 res.files += mod_rdf.printrdf(0.0, res.lsmax, nconf=nconf, outdir=outdir)
             res.t_total = time.perf_counter() - t_start
             res.t_files = time.perf_counter() - t_files

@@ -125,6 +125,29 @@ module trj_hip
       integer(c_int64_t), value :: nconf
       real(c_double), intent(out) :: out(*)
     end function
+    ! number fluctuations (nrandom of /INPUT_RDF/), after trj_rdf_init
+    integer(c_int) function trj_rdf_sampling(ctx, nrandom, seed) bind(C, name="trj_rdf_sampling")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nrandom
+      integer(c_int64_t), value :: seed
+    end function
+    integer(c_int) function trj_rdf_sampling_points(ctx, pts) bind(C, name="trj_rdf_sampling_points")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: pts(*)
+    end function
+    integer(c_int) function trj_rdf_s2n_sums(ctx, gi, g2i) bind(C, name="trj_rdf_s2n_sums")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: gi(*), g2i(*)
+    end function
+    integer(c_int) function trj_rdf_s2n(ctx, nconf, out) bind(C, name="trj_rdf_s2n")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nconf
+      real(c_double), intent(out) :: out(*)
+    end function
     ! z-slices (twoDstruc_3D), before trj_rdf_init / trj_sq_init; zslice(nslice) FP32
     integer(c_int) function trj_set_slices(ctx, nslice, zslice, zgrid) bind(C, name="trj_set_slices")
       import :: c_int, c_ptr, c_int32_t, c_float

@@ -43,14 +43,17 @@ def context() -> T.Context:
 
 
 def RDF_init(nsp: int, *, ndim: int, deltar: float, rcrdf: float, sidel, periodic=(True, True, True),
-             charges: bool = False, ctx: Optional[T.Context] = None) -> int:
+             charges: bool = False, nrandom: int = 0, ctx: Optional[T.Context] = None) -> int:
     """RDF_init (rdf.cuf:51-122): returns lsmax; sidel is the first frame's box.
-    charges=True also accumulates g_qq (frames then need qcharge)."""
+    charges=True also accumulates g_qq (frames then need qcharge); nrandom > 0
+    samples the number fluctuations around that many fixed points (s2n.dat)."""
     c = ctx or context()
     flags = T.TRJ_RDF_CHARGE if charges else 0
     st = c.rdf_init(T.rdf_params(ndim, nsp, deltar, rcrdf, periodic, flags=flags), sidel)
+    if nrandom > 0:
+        c.rdf_sampling(nrandom)
     _state.update(ctx=c, lsmax=st.lsmax, nit=st.nit, nsp=nsp, ndim=ndim, deltar=deltar,
-                  rmingr=None, nframes=0, charges=charges)
+                  rmingr=None, nframes=0, charges=charges, nrandom=nrandom)
     return st.lsmax
 
 
@@ -149,6 +152,15 @@ def printrdf(rcl: float = 0.0, lsmax: Optional[int] = None, nconf: int = 0, outd
     c = _state["ctx"]
     if c.nslice > 0:
         return _printrdf_slices(c, nconf, outdir)
+    paths = []
+    if _state.get("nrandom", 0) > 0:  # log.f90:449, 482, 526-533
+        s2 = c.rdf_s2n(nconf)
+        path = os.path.join(outdir, "s2n.dat")
+        with open(path, "w") as f:
+            f.write("#       r" + " " * 16 + "s2n\n")
+            for i, v in enumerate(s2):
+                f.write(_fmt_records([_ri(i + 1, _state["deltar"]), v], 16, 5, 18) + "\n")
+        paths.append(path)
     nsp, nit = _state["nsp"], _state["nit"]
     g = c.rdf_gmix(nconf)
     qq = _state.get("charges", False)
@@ -156,7 +168,6 @@ def printrdf(rcl: float = 0.0, lsmax: Optional[int] = None, nconf: int = 0, outd
     if qq and len(gqq) != len(g):
         raise T.TrjError(T.TRJ_ESTATE, "g_qq and g_ab row counts differ")
     qhead = "g_qq(r)" + " " * 9 if qq else ""
-    paths = []
     if nsp <= 6:
         pairs = [(j, k) for j in range(1, nsp + 1) for k in range(j, nsp + 1)]
         head = ("#       r        " + " " * 9 + qhead + "".join(f"g_{j}{k}(r)" + " " * 9 for j, k in pairs)).rstrip()

@@ -36,7 +36,8 @@ EXPORTS = [
     "trj_abi_version", "trj_create", "trj_destroy", "trj_last_error", "trj_rdf_init",
     "trj_sq_init", "trj_sq_plan", "trj_rdf_plan", "trj_frame", "trj_frame_device", "trj_frame_device_q", "trj_sync",
     "trj_rdf_counts", "trj_rdf_gmix", "trj_rdf_first_minimum", "trj_sq_shells", "trj_sq_columns",
-    "trj_sq_rho", "trj_set_slices", "trj_slice_counts", "trj_rdf_xy", "trj_rdf_xy_gmix", "trj_rdf_xy_qq",
+    "trj_sq_rho", "trj_rdf_sampling", "trj_rdf_sampling_points", "trj_rdf_s2n_sums", "trj_rdf_s2n",
+    "trj_set_slices", "trj_slice_counts", "trj_rdf_xy", "trj_rdf_xy_gmix", "trj_rdf_xy_qq",
     "trj_rdf_xy_gcc", "trj_sq_xy", "trj_sq_xy_columns", "trj_frame_q", "trj_rdf_qq", "trj_rdf_gqq", "trj_sq_qq", "trj_sq_qq_column", "trj_reset", "trj_last_timings", "trj_timing_totals", "trj_timing_reset",
     "trj_device_bytes", "trj_set_pipeline", "trj_state_bytes", "trj_state_save", "trj_state_load",
     "trj_lds_atomic_rate", "trj_mfma_rate", "trj_nc_open", "trj_nc_close", "trj_nc_last_error", "trj_nc_info", "trj_nc_read",
@@ -119,6 +120,10 @@ def _declare(L):
     L.trj_frame_q.argtypes = [vp, vp, C.c_int32, C.c_int32, i32, d, C.c_int64, d, u32]
     L.trj_rdf_qq.argtypes = [vp, d]
     L.trj_set_slices.argtypes = [vp, C.c_int32, C.POINTER(C.c_float), C.c_float]
+    L.trj_rdf_sampling.argtypes = [vp, C.c_int32, C.c_uint64]
+    L.trj_rdf_sampling_points.argtypes = [vp, d]
+    L.trj_rdf_s2n_sums.argtypes = [vp, i64, i64]
+    L.trj_rdf_s2n.argtypes = [vp, C.c_int64, d]
     L.trj_slice_counts.argtypes = [vp, i64, i64]
     L.trj_rdf_xy.argtypes = [vp, d, i64]
     L.trj_rdf_xy_gmix.argtypes = [vp, C.c_int64, d]
@@ -383,6 +388,7 @@ class Context:
         self.sq_info: Optional[SqInfo] = None
         self.sq_nsp = 0
         self.nslice = 0
+        self.nrandom = 0
 
     # -- lifetime
     def close(self):
@@ -476,6 +482,28 @@ class Context:
         r = self.rdf
         out = np.zeros((max(r.lsmax - 2, 0), 1 + r.nit))
         self._check(self._L.trj_rdf_gmix(self._h, nconf, _ptr(out, C.c_double)), "trj_rdf_gmix")
+        return out
+
+    # ------------------------------------------------ number fluctuations
+    def rdf_sampling(self, nrandom: int, seed: int = 12345):
+        """Fixed random points for the number fluctuations (trj_rdf_sampling)."""
+        self._check(self._L.trj_rdf_sampling(self._h, int(nrandom), int(seed)), "trj_rdf_sampling")
+        self.nrandom = int(nrandom)
+
+    def rdf_sampling_points(self) -> np.ndarray:
+        out = np.zeros((self.nrandom, 3))
+        self._check(self._L.trj_rdf_sampling_points(self._h, _ptr(out, C.c_double)), "trj_rdf_sampling_points")
+        return out
+
+    def rdf_s2n_sums(self):
+        gi = np.zeros(self.rdf.lsmax + 1, dtype=np.int64)
+        g2i = np.zeros(self.rdf.lsmax + 1, dtype=np.int64)
+        self._check(self._L.trj_rdf_s2n_sums(self._h, _ptr(gi, C.c_int64), _ptr(g2i, C.c_int64)), "trj_rdf_s2n_sums")
+        return gi, g2i
+
+    def rdf_s2n(self, nconf: int = 0) -> np.ndarray:
+        out = np.zeros(max(self.rdf.lsmax - 2, 0))
+        self._check(self._L.trj_rdf_s2n(self._h, nconf, _ptr(out, C.c_double)), "trj_rdf_s2n")
         return out
 
     # ------------------------------------------------------------ slices
