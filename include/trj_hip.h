@@ -79,6 +79,11 @@ extern "C" {
 #define TRJ_RDF_CHARGE 1u /* also accumulate the charge-weighted histogram g_qq
                              (rdf.cuf:466-472, 285-287); frames come through
                              trj_frame_q */
+#define TRJ_RDF_FLOOR_BINS 2u /* bin k = int(r/deltar) + 1, i.e. [(k-1)dr, k dr),
+                                 as histogram tools bin (freud's RegularAxis);
+                                 the reference's rule is nint(r/deltar).  The
+                                 printrdf normalisations assume the reference's
+                                 rule. */
 
 typedef struct trj_ctx trj_ctx;
 

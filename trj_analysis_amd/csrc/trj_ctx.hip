@@ -734,7 +734,7 @@ int trj_rdf_init(trj_ctx* c, const trj_rdf_params* p, const double box[3], int32
   for (int k = 0; k < 3; ++k) R.periodic[k] = (k < p->ndim) ? (p->periodic[k] != 0) : 1;
   R.lsmax = rdf_lsmax(p->ndim, box, p->rcrdf, p->deltar, &R.rc);
   if (R.lsmax < 1) return fail(c, TRJ_EINVAL, "rdf: lsmax < 1");
-  if (p->flags & ~TRJ_RDF_CHARGE) return fail(c, TRJ_EINVAL, "rdf: unknown flags");
+  if (p->flags & ~(TRJ_RDF_CHARGE | TRJ_RDF_FLOOR_BINS)) return fail(c, TRJ_EINVAL, "rdf: unknown flags");
   const bool qq = (p->flags & TRJ_RDF_CHARGE) != 0;
   if (cells_lds_bytes(R.nit, R.lsmax, qq) > kPairLdsLimit)
     return fail(c, TRJ_ELIMIT,
@@ -766,7 +766,8 @@ int trj_rdf_init(trj_ctx* c, const trj_rdf_params* p, const double box[3], int32
     // Keep the host copy alive until the copy above has completed.
     HIPC(c, hipStreamSynchronize(c->stream));
   }
-  std::vector<double> thr = rdf_bin_thresholds(R.deltar, R.lsmax);
+  std::vector<double> thr = (p->flags & TRJ_RDF_FLOOR_BINS) ? rdf_bin_thresholds_floor(R.deltar, R.lsmax)
+                                                            : rdf_bin_thresholds(R.deltar, R.lsmax);
   thr.push_back(INFINITY);  // thr[lsmax+2]
   thr[0] = 0.0;
   R.thr = thr;
@@ -1231,6 +1232,13 @@ int trj_sq_rho(trj_ctx* c, int64_t nvec, const int32_t* kv, double* rho) {
   if (r) return r;
   const SqPlan& S = c->sq;
   const int nsp = S.p.nsp, ntiles = static_cast<int>(S.tiles.size());
+  const int nch = nsp + (c->sq_charge ? 1 : 0);  // channels of the phase sums
+  // Many vectors: one copy of the frame's phase sums instead of one per value.
+  std::vector<double> all;
+  if (nvec > 16) {
+    all.resize(static_cast<size_t>(2) * c->ksplit * nch * ntiles * S.tk.vec());
+    HIPC(c, hipMemcpy(all.data(), c->rho.p, sizeof(double) * all.size(), hipMemcpyDeviceToHost));
+  }
   for (int64_t v = 0; v < nvec; ++v) {
     const int x = kv[3 * v], y = kv[3 * v + 1], z = kv[3 * v + 2];
     if (x < S.kmin[0] || y < S.kmin[1] || z < S.kmin[2])
@@ -1249,9 +1257,13 @@ int trj_sq_rho(trj_ctx* c, int64_t nvec, const int32_t* kv, double* rho) {
       double acc[2] = {0.0, 0.0};
       for (int sp = 0; sp < c->ksplit; ++sp) {
         double val[2];
-        const size_t off =
-            (((static_cast<size_t>(sp) * nsp + s) * ntiles + tile) * TILE_VEC + vi) * 2 * sizeof(double);
-        HIPC(c, hipMemcpy(val, static_cast<char*>(c->rho.p) + off, sizeof(val), hipMemcpyDeviceToHost));
+        const size_t idx = (((static_cast<size_t>(sp) * nch + s) * ntiles + tile) * TILE_VEC + vi) * 2;
+        if (!all.empty()) {
+          val[0] = all[idx];
+          val[1] = all[idx + 1];
+        } else {
+          HIPC(c, hipMemcpy(val, static_cast<double*>(c->rho.p) + idx, sizeof(val), hipMemcpyDeviceToHost));
+        }
         acc[0] += val[0];
         acc[1] += val[1];
       }
