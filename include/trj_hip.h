@@ -346,6 +346,36 @@ int trj_nc_frame_ref_q(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_
                        int32_t* itype, double* charge, int64_t* n_out, double box[3], int32_t periodic[3]);
 
 /* ------------------------------------------------------------------------
+ * F(Q,t) and Fs(Q,t) (FqtnD, dynamics.cuf:421-508, with the time origins of
+ * rtcorr, dynamics.cuf:157-245; fkt.dat / fskt.dat of print_rtcor, :246-330).
+ * Frames ask for it with TRJ_WANT_FQT.  The q-set is the reference's S(Q,w)
+ * set (sq.cuf:234-282): vectors (i, j, k) >= 0 of the box given here with
+ * qw(l) - dq/2 < |q| < qw(l) + dq/2, FP32.  Buffer b (0-based) starts at frame
+ * b*jump + 1, restarts itlimit frames later (its sums then go to the totals
+ * and count_org grows); lag indx = frame - origin + 1 <= ndyn is accumulated:
+ *   F(q, indx)  += Re(rho_b(q, t0) conj(rho_b(q, t))), rho_b = sum bsc_s e^{iq.r}
+ *   Fs(q, indx) += sum_j cos(q.(r_j(t) - r_j(t0)))
+ * summed over the vectors of each qw(l), FP64 phases, fixed-order sums. */
+#define TRJ_WANT_FQT 4u
+typedef struct {
+  int32_t ndim, nsp;
+  int32_t nqw;          /* target |q| values (/INPUT_SQW/ qw), 1..64 */
+  int32_t nbuffer;      /* time origins (/INPUT_DYN/ nbuffer), 1..64 */
+  int32_t jump;         /* frames between origins (/INPUT_DYN/ jump) */
+  int32_t ndyn;         /* lags kept: Nconf - (nbuffer-1) jump (dyn_init) */
+  int32_t itlimit;      /* frames after which an origin restarts (rtcorr: ndyn when tlimit < 0) */
+  int32_t pad;
+  const double* qw;     /* [nqw] */
+  const double* bsc;    /* [nsp] scattering lengths */
+} trj_fqt_params;
+/* nvec: vectors of the q-set; nw[nqw]: vectors per qw (may be NULL). */
+int trj_fqt_init(trj_ctx* ctx, const trj_fqt_params* p, const double box[3], int32_t* nvec, int64_t* nw);
+/* fkt, fskt [nqw][ndyn]: the totals over completed origins divided by
+ * (N count_org nw(l)) (print_rtcor), raw when no origin has completed; either
+ * may be NULL. */
+int trj_fqt_results(trj_ctx* ctx, double* fkt, double* fskt, int64_t* count_org);
+
+/* ------------------------------------------------------------------------
  * Number fluctuations (nrandom > 0 of /INPUT_RDF/; s2D3_sh / s2D2_sh,
  * rdf.cuf:489-615, s2n.dat of log.f90:449, 482, 526-533): around nrandom
  * points fixed for the run, each frame adds N_k (atoms with nint(r/dr) <= k

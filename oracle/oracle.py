@@ -105,6 +105,11 @@ def _declare(L):
     L.or_s2n.argtypes = [C.c_int64, _D, _D, C.c_int64, _D, _D, _D, C.c_int, _D, C.c_double, C.c_double, C.c_int,
                          _I64, _I64]
     L.or_s2n.restype = None
+    L.or_fqt_vectors.argtypes = [C.c_int, _D, C.c_int, _D, C.c_int64, _I32, _I32]
+    L.or_fqt_vectors.restype = C.c_int64
+    L.or_fqt.argtypes = [C.c_int, C.c_int, _D, _D, C.c_int, _D, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64,
+                         C.c_int64, _D, _I32, _D, _D, _I64]
+    L.or_fqt.restype = None
     _F = C.POINTER(C.c_float)
     L.or_slice_of.argtypes = [C.c_float, C.c_int, _F, C.c_float]
     L.or_slice_of.restype = C.c_int
@@ -439,4 +444,31 @@ def s2n_sums(pts, pos, ndim, box, periodic, rcrdf, deltar, lsmax=None, gi=None, 
                  _p(z, C.c_double), ndim, _p(g.lmi, C.c_double), g.side2, deltar, lsmax, _p(gi, C.c_int64),
                  _p(g2i, C.c_int64))
     return gi, g2i
+
+
+def fqt_vectors(ndim, box, qw):
+    """(kv (m,3) int32, bin (m,)) of the S(Q,w) q-set, grouped by qw."""
+    b = _f64(box)
+    q = _f64(qw)
+    m = lib().or_fqt_vectors(ndim, _p(b, C.c_double), len(q), _p(q, C.c_double), 0, None, None)
+    kv = np.zeros((max(m, 1), 3), dtype=np.int32)
+    bn = np.zeros(max(m, 1), dtype=np.int32)
+    lib().or_fqt_vectors(ndim, _p(b, C.c_double), len(q), _p(q, C.c_double), m, _p(kv, C.c_int32), _p(bn, C.c_int32))
+    return kv[:m], bn[:m]
+
+
+def fqt(frames, itype, ndim, nsp, bsc, box, qw, nbuffer, jump, ndyn, itlimit):
+    """(tot (nqw, ndyn), tots (nqw, ndyn), count_org) over frames (list of (N,3) positions)."""
+    pos = np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.float64) for f in frames]))
+    it = _i32(itype)
+    b = _f64(box)
+    q = _f64(qw)
+    bs = _f64(bsc)
+    tot = np.zeros((len(q), ndyn))
+    tots = np.zeros((len(q), ndyn))
+    co = C.c_int64(0)
+    lib().or_fqt(ndim, nsp, _p(bs, C.c_double), _p(b, C.c_double), len(q), _p(q, C.c_double), nbuffer, jump,
+                 ndyn, itlimit, pos.shape[0], pos.shape[1], _p(pos, C.c_double), _p(it, C.c_int32),
+                 _p(tot, C.c_double), _p(tots, C.c_double), C.byref(co))
+    return tot, tots, co.value
 

@@ -882,3 +882,119 @@ void or_s2n(int64_t np, const double* pts, const double box[3], int64_t n, const
   }
   free(h);
 }
+
+/* ------------------------------------------------------------------ F(Q,t) */
+
+/* The S(Q,w) q-set (sq.cuf:234-282, FP32; the storing loop's k range):
+ * vectors (i,j,k) >= 0, not all 0, i <= nint(qmm/fk1), ..., in the first l with
+ * qw(l) - dq/2 < |q| < qw(l) + dq/2.  Writes up to cap vectors grouped by l
+ * (kv[3*v], bin[v]); returns the count. */
+int64_t or_fqt_vectors(int ndim, const double box[3], int nqw, const double* qw, int64_t cap, int32_t* kv,
+                       int32_t* bin) {
+  float fk[3];
+  for (int k = 0; k < 3; ++k) fk[k] = (2.0f * OR_PI_F) / (float)(k < ndim ? box[k] : 1.0);
+  float dq = fk[0] < fk[1] ? fk[0] : fk[1];
+  if (ndim == 3 && fk[2] < dq) dq = fk[2];
+  float qmm = 0.0f;
+  for (int l = 0; l < nqw; ++l)
+    if ((float)qw[l] > qmm) qmm = (float)qw[l];
+  int imax = or_nint_f(qmm / fk[0]), jmax = or_nint_f(qmm / fk[1]), kmax = ndim == 3 ? or_nint_f(qmm / fk[2]) : 0;
+  int64_t nv = 0;
+  for (int l = 0; l < nqw; ++l)
+    for (int i = 0; i <= imax; ++i)
+      for (int j = 0; j <= jmax; ++j)
+        for (int k = 0; k <= kmax; ++k) {
+          if (i + j + k == 0) continue;
+          float a = i * fk[0], b = j * fk[1], c = k * fk[2];
+          float rr = sqrtf(a * a + b * b + c * c);
+          int first = -1;
+          for (int m = 0; m < nqw; ++m)
+            if ((float)qw[m] - dq / 2 < rr && (float)qw[m] + dq / 2 > rr) {
+              first = m;
+              break;
+            }
+          if (first != l) continue;
+          if (nv < cap) {
+            kv[3 * nv] = i;
+            kv[3 * nv + 1] = j;
+            kv[3 * nv + 2] = k;
+            bin[nv] = l;
+          }
+          ++nv;
+        }
+  return nv;
+}
+
+/* FqtnD over a trajectory with rtcorr's origins (dynamics.cuf:157-245,
+ * 421-508), long-double phases: pos[f][j][3] for f = 0..nframes-1 (frame
+ * f+1).  tot/tots [nqw][ndyn] get the sums of the origins that completed
+ * (itlimit frames), *count_org their number. */
+void or_fqt(int ndim, int nsp, const double* bsc, const double box[3], int nqw, const double* qw, int nbuffer,
+            int jump, int ndyn, int itlimit, int64_t nframes, int64_t n, const double* pos, const int32_t* itype,
+            double* tot, double* tots, int64_t* count_org) {
+  const long double twopi = 6.283185307179586476925286766559005768L;
+  int64_t nv = or_fqt_vectors(ndim, box, nqw, qw, 0, NULL, NULL);
+  int32_t* kv = (int32_t*)malloc(sizeof(int32_t) * 3 * (nv > 0 ? nv : 1));
+  int32_t* bin = (int32_t*)malloc(sizeof(int32_t) * (nv > 0 ? nv : 1));
+  or_fqt_vectors(ndim, box, nqw, qw, nv, kv, bin);
+  int* origin = (int*)malloc(sizeof(int) * nbuffer);
+  double* buf = (double*)calloc((size_t)nbuffer * nqw * ndyn, sizeof(double));
+  double* bufs = (double*)calloc((size_t)nbuffer * nqw * ndyn, sizeof(double));
+  int64_t* orgframe = (int64_t*)malloc(sizeof(int64_t) * nbuffer);
+  for (int b = 0; b < nbuffer; ++b) {
+    origin[b] = b * jump + 1;
+    orgframe[b] = -1;
+  }
+  *count_org = 0;
+  for (int64_t f = 0; f < nframes; ++f) {
+    const int iconf = (int)f + 1;
+    for (int b = 0; b < nbuffer; ++b)
+      if (iconf == origin[b]) orgframe[b] = f;
+    for (int b = 0; b < nbuffer; ++b) {
+      int ix = iconf - origin[b] + 1;
+      if (iconf < origin[b] || ix > ndyn) continue;
+      const double* r0 = pos + orgframe[b] * n * 3;
+      const double* rt = pos + f * n * 3;
+#pragma omp parallel for schedule(dynamic, 4)
+      for (int64_t v = 0; v < nv; ++v) {
+        long double c0 = 0, s0 = 0, ct = 0, st = 0, self = 0;
+        for (int64_t j = 0; j < n; ++j) {
+          long double t0 = 0, tt = 0;
+          for (int k = 0; k < ndim; ++k) {
+            t0 += (long double)kv[3 * v + k] * (long double)r0[3 * j + k] / (long double)box[k];
+            tt += (long double)kv[3 * v + k] * (long double)rt[3 * j + k] / (long double)box[k];
+          }
+          long double w = bsc[itype[j] - 1];
+          c0 += w * cosl(twopi * t0);
+          s0 += w * sinl(twopi * t0);
+          ct += w * cosl(twopi * tt);
+          st += w * sinl(twopi * tt);
+          self += cosl(twopi * (tt - t0));
+        }
+        double fv = (double)(c0 * ct + s0 * st), fs = (double)self;
+        size_t idx = ((size_t)b * nqw + bin[v]) * ndyn + (ix - 1);
+#pragma omp atomic
+        buf[idx] += fv;
+#pragma omp atomic
+        bufs[idx] += fs;
+      }
+    }
+    for (int b = 0; b < nbuffer; ++b)
+      if (iconf - origin[b] == itlimit) {
+        origin[b] = iconf + 1;
+        for (int i = 0; i < nqw * ndyn; ++i) {
+          tot[i] += buf[(size_t)b * nqw * ndyn + i];
+          tots[i] += bufs[(size_t)b * nqw * ndyn + i];
+          buf[(size_t)b * nqw * ndyn + i] = 0.0;
+          bufs[(size_t)b * nqw * ndyn + i] = 0.0;
+        }
+        *count_org += 1;
+      }
+  }
+  free(kv);
+  free(bin);
+  free(origin);
+  free(buf);
+  free(bufs);
+  free(orgframe);
+}

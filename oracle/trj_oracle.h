@@ -53,6 +53,10 @@
  *   or_s2n               a single atom at a known distance; a point on a site of a
  *                        simple-cubic lattice, whose N_k is an integer-vector count;
  *                        N_k at lsmax equals the atoms within the cutoff
+ *   or_fqt_vectors       brute-force shell membership in numpy (FP32) on hand grids
+ *   or_fqt               identical frames: F = sum |rho|^2 per lag and Fs = N per
+ *                        vector; a uniform drift d per frame: Fs = N cos(q.d lag),
+ *                        F = |rho(0)|^2 cos(q.d lag); origin counts worked by hand
  *   or_sq_frame_q        unit charges equal or_sq_frame's S with bsc = 1;
  *                        uniform charge c scales by c^2; single atom gives q^2
  * None is "parity unpinned".
@@ -200,6 +204,14 @@ void or_sq_frame_xy(int64_t n, const double* x, const double* y, const double* z
 void or_s2n(int64_t np, const double* pts, const double box[3], int64_t n, const double* x, const double* y,
             const double* z, int ndim, const double lmi[3], double side2, double deltar, int lsmax, int64_t* gi,
             int64_t* g2i);
+
+/* F(Q,t): the S(Q,w) q-set and FqtnD over rtcorr's origins (sq.cuf:234-282,
+ * dynamics.cuf:157-245, 421-508). */
+int64_t or_fqt_vectors(int ndim, const double box[3], int nqw, const double* qw, int64_t cap, int32_t* kv,
+                       int32_t* bin);
+void or_fqt(int ndim, int nsp, const double* bsc, const double box[3], int nqw, const double* qw, int nbuffer,
+            int jump, int ndyn, int itlimit, int64_t nframes, int64_t n, const double* pos, const int32_t* itype,
+            double* tot, double* tots, int64_t* count_org);
 
 #ifdef __cplusplus
 }

@@ -105,6 +105,35 @@ void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start
 void launch_rdf_allpairs(const PairArgs& a, const double* pos, const int32_t* itype, int64_t n,
                          int nblocks, uint32_t* partial, hipStream_t s, int64_t stride = 0,
                          const int32_t* range = nullptr);
+// F(Q,t) and Fs(Q,t) over time origins (FqtnD).
+constexpr int kFqtChunks = 32;   // atom chunks per q-vector (fixed-order partials)
+constexpr int kFqtMaxBuffers = 64;
+struct FqtBsc {
+  double b[kNspMax];
+};
+struct FqtArgs {
+  const int32_t* kv;      // [nvec][3], sorted by |q| bin
+  const int32_t* bin_off; // [nqw+1] vector ranges of the bins
+  int nvec, nqw, ndyn, nbuffer, ndim, nsp;
+  double invl[3];         // 1/L of the q-grid box
+  const double* pos;      // current atoms, SoA (stride n)
+  const int32_t* itype;
+  int64_t n;
+  FqtBsc bsc;
+  const double* r0;       // [nbuffer][3][n] origin positions
+  double* rho0;           // [nbuffer][nvec][2] rho(q, t0)
+  int nact;                         // active origins this frame
+  int32_t act[kFqtMaxBuffers];      // their buffers
+  int32_t act_indx[kFqtMaxBuffers]; // their lag index 1..ndyn
+  int32_t start_now[kFqtMaxBuffers];// 1 where the origin is this frame
+  double* part;           // [nvec][nact+1][kFqtChunks][2]
+  double* vsum;           // [nvec][nact+1][2]
+  double* fkt;            // [nbuffer][nqw][ndyn]
+  double* fskt;
+};
+void launch_fqt(const FqtArgs& a, hipStream_t s);
+void launch_fqt_collect(int b, int nqw, int ndyn, double* fkt, double* fskt, double* tot, double* tots, hipStream_t s);
+
 // Number fluctuations around fixed random points (s2D3_sh / s2D2_sh).
 struct S2nArgs {
   const double* pts;    // unit points, [3][np], in (0, 1]

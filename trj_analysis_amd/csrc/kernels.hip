@@ -460,6 +460,115 @@ __global__ __launch_bounds__(256) void k_s2n(S2nArgs a) {
   }
 }
 
+// ------------------------------------------------------------------ F(Q,t)
+// FqtnD (dynamics.cuf:421-508): per q-vector v and chunk of atoms, block
+// z = 0 sums the current bsc-weighted phases (rho(q,t)), block z = 1 + a sums
+// cos(q.(r(t) - r(t0))) against active origin a's stored positions.  Phases
+// in FP64: turns t = k.r/L reduced by rint before sincospi.
+__device__ inline double fqt_turns(const int* k, const double* x, int64_t j, int64_t stride, const double* invl,
+                                   int ndim) {
+  double t = k[0] * (x[j] * invl[0]) + k[1] * (x[stride + j] * invl[1]);
+  if (ndim == 3) t += k[2] * (x[2 * stride + j] * invl[2]);
+  return t - rint(t);
+}
+
+__global__ __launch_bounds__(256) void k_fqt_partial(FqtArgs a) {
+  __shared__ double red[2][256];
+  const int v = blockIdx.x, chunk = blockIdx.y, z = blockIdx.z;
+  const int k[3] = {a.kv[3 * v], a.kv[3 * v + 1], a.kv[3 * v + 2]};
+  const int64_t j0 = a.n * chunk / kFqtChunks, j1 = a.n * (chunk + 1) / kFqtChunks;
+  double pr = 0.0, pi = 0.0;
+  if (z == 0) {
+    for (int64_t j = j0 + threadIdx.x; j < j1; j += 256) {
+      const int t = a.itype[j];
+      const double w = (t >= 1 && t <= a.nsp) ? a.bsc.b[t - 1] : 0.0;
+      double sn, cs;
+      sincospi(2.0 * fqt_turns(k, a.pos, j, a.n, a.invl, a.ndim), &sn, &cs);
+      pr += w * cs;
+      pi += w * sn;
+    }
+  } else {
+    const double* r0 = a.r0 + static_cast<int64_t>(a.act[z - 1]) * 3 * a.n;
+    for (int64_t j = j0 + threadIdx.x; j < j1; j += 256) {
+      const double f = fqt_turns(k, a.pos, j, a.n, a.invl, a.ndim) - fqt_turns(k, r0, j, a.n, a.invl, a.ndim);
+      pr += cospi(2.0 * f);
+    }
+  }
+  red[0][threadIdx.x] = pr;
+  red[1][threadIdx.x] = pi;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (static_cast<int>(threadIdx.x) < off) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + off];
+      red[1][threadIdx.x] += red[1][threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double* o = a.part + ((static_cast<int64_t>(v) * gridDim.z + z) * kFqtChunks + chunk) * 2;
+    o[0] = red[0][0];
+    o[1] = red[1][0];
+  }
+}
+
+// Per vector: rho(q,t) and the self sums from the chunks in a fixed order;
+// rho0 of origins starting now is set to rho(q,t) first (the reference stores
+// r0 = r at the origin frame, dynamics.cuf:202-206).
+__global__ void k_fqt_vectors(FqtArgs a) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= a.nvec) return;
+  const int nz = a.nact + 1;
+  double* o = a.vsum + static_cast<int64_t>(v) * nz * 2;
+  for (int z = 0; z < nz; ++z) {
+    double sr = 0.0, si = 0.0;
+    for (int c = 0; c < kFqtChunks; ++c) {
+      const double* p = a.part + ((static_cast<int64_t>(v) * nz + z) * kFqtChunks + c) * 2;
+      sr += p[0];
+      si += p[1];
+    }
+    o[2 * z] = sr;
+    o[2 * z + 1] = si;
+  }
+  for (int b = 0; b < a.nbuffer; ++b)
+    if (a.start_now[b]) {
+      a.rho0[(static_cast<int64_t>(b) * a.nvec + v) * 2] = o[0];
+      a.rho0[(static_cast<int64_t>(b) * a.nvec + v) * 2 + 1] = o[1];
+    }
+}
+
+// Per (|q| bin l, active origin a): fkt[b][l][indx] += sum over the bin's
+// vectors of Re(rho0 conj(rho)) and fskt[b][l][indx] += their self sums
+// (dynamics.cuf:500-501), vectors in a fixed order.
+__global__ void k_fqt_bins(FqtArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.nqw * a.nact) return;
+  const int l = e % a.nqw, ai = e / a.nqw;
+  const int b = a.act[ai], indx = a.act_indx[ai];
+  const int nz = a.nact + 1;
+  double f = 0.0, fs = 0.0;
+  for (int v = a.bin_off[l]; v < a.bin_off[l + 1]; ++v) {
+    const double* o = a.vsum + static_cast<int64_t>(v) * nz * 2;
+    const double* r0 = a.rho0 + (static_cast<int64_t>(b) * a.nvec + v) * 2;
+    f += r0[0] * o[0] + r0[1] * o[1];
+    fs += o[2 * (ai + 1)];
+  }
+  const int64_t idx = (static_cast<int64_t>(b) * a.nqw + l) * a.ndyn + (indx - 1);
+  a.fkt[idx] += f;
+  a.fskt[idx] += fs;
+}
+
+// A restarting origin's sums into the totals, then cleared (dynamics.cuf:
+// 225-238).
+__global__ void k_fqt_collect(int b, int nqw, int ndyn, double* fkt, double* fskt, double* tot, double* tots) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nqw * ndyn) return;
+  const int64_t i = static_cast<int64_t>(b) * nqw * ndyn + e;
+  tot[e] += fkt[i];
+  tots[e] += fskt[i];
+  fkt[i] = 0.0;
+  fskt[i] = 0.0;
+}
+
 // Returns the bin the pair was added to (0: none).
 __device__ inline int pair_accumulate(const PairArgs& a, const double* thr, uint32_t* hist,
                                       bool ok, double xi, double yi, double zi, int ti, double xj,
@@ -2128,6 +2237,17 @@ void launch_cell_count_slices(const double* pos, const int32_t* key, int64_t n, 
                               int nslice, int32_t* cellid, int32_t* count, hipStream_t s) {
   hipLaunchKernelGGL(k_cell_count_slices, dim3(grid1d(n, 256)), dim3(256), 0, s, pos, key, n, nsp, g, nslice,
                      cellid, count);
+}
+
+void launch_fqt(const FqtArgs& a, hipStream_t s) {
+  if (a.nvec <= 0) return;
+  hipLaunchKernelGGL(k_fqt_partial, dim3(a.nvec, kFqtChunks, a.nact + 1), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_fqt_vectors, dim3((a.nvec + 63) / 64), dim3(64), 0, s, a);
+  if (a.nact > 0) hipLaunchKernelGGL(k_fqt_bins, dim3((a.nqw * a.nact + 63) / 64), dim3(64), 0, s, a);
+}
+
+void launch_fqt_collect(int b, int nqw, int ndyn, double* fkt, double* fskt, double* tot, double* tots, hipStream_t s) {
+  hipLaunchKernelGGL(k_fqt_collect, dim3((nqw * ndyn + 255) / 256), dim3(256), 0, s, b, nqw, ndyn, fkt, fskt, tot, tots);
 }
 
 size_t s2n_lds_bytes(int lsmax) { return sizeof(uint32_t) * (((lsmax + 1 + 3) & ~3) + 256); }

@@ -62,6 +62,18 @@ struct trj_ctx {
   // totals, normalised accumulators; charges: fixed-point frame sums and
   // their accumulator), and the floor-rule bin edges.
   SliceSet slices{};
+  // F(Q,t) (TRJ_WANT_FQT): the S(Q,w) q-set sorted by target |q|, origin
+  // bookkeeping of rtcorr, origin positions and rho(q,t0), per-origin sums
+  // and their totals over completed origins.
+  bool fqt_ready = false;
+  int fqt_nqw = 0, fqt_nbuf = 0, fqt_jump = 1, fqt_ndyn = 0, fqt_itlimit = 0, fqt_nvec = 0, fqt_ndim = 3,
+      fqt_nsp = 1;
+  double fqt_invl[3] = {1.0, 1.0, 1.0};
+  FqtBsc fqt_bsc{};
+  std::vector<int64_t> fqt_nw;
+  std::vector<int> fqt_origin;
+  int64_t fqt_frames = 0, fqt_count_org = 0, fqt_n = 0;
+  DevBuf fqkv, fqbin, fqr0, fqrho0, fqpart, fqvsum, fqfkt, fqfskt, fqtot, fqtots;
   // Number fluctuations (nrandom > 0): fixed unit points and the int64 sums
   // of N_k and N_k^2 over points and frames.
   int nrand = 0;
@@ -276,6 +288,69 @@ int rdf_slices(trj_ctx* c, PairArgs pa, const FrameGeom& g, const double box[3],
   return TRJ_OK;
 }
 
+// One frame of F(Q,t) (rtcorr's origin bookkeeping, dynamics.cuf:157-245, and
+// FqtnD): origins of buffer b at frames (b-1) jump + 1, restarted after
+// itlimit frames; lags beyond ndyn are not accumulated.  Everything the host
+// decides comes from its own counters: no synchronisation.
+int fqt_step(trj_ctx* c, int64_t n, const int32_t* itype_dev) {
+  int r;
+  const int iconf = static_cast<int>(++c->fqt_frames);
+  const int nb = c->fqt_nbuf;
+  if (iconf == 1) {
+    c->fqt_n = n;
+    for (int b = 0; b < nb; ++b) c->fqt_origin[b] = b * c->fqt_jump + 1;
+    if ((r = ensure(c, c->fqr0, sizeof(double) * 3 * n * nb))) return r;
+  }
+  FqtArgs a{};
+  a.kv = P<int32_t>(c->fqkv);
+  a.bin_off = P<int32_t>(c->fqbin);
+  a.nvec = c->fqt_nvec;
+  a.nqw = c->fqt_nqw;
+  a.ndyn = c->fqt_ndyn;
+  a.nbuffer = nb;
+  a.ndim = c->fqt_ndim;
+  a.nsp = c->fqt_nsp;
+  for (int k = 0; k < 3; ++k) a.invl[k] = c->fqt_invl[k];
+  a.pos = P<double>(c->pos);
+  a.itype = itype_dev;
+  a.n = n;
+  a.bsc = c->fqt_bsc;
+  a.r0 = P<double>(c->fqr0);
+  a.rho0 = P<double>(c->fqrho0);
+  for (int b = 0; b < nb; ++b) {
+    if (iconf == c->fqt_origin[b]) {  // r0 = r at the origin frame (dynamics.cuf:202-206)
+      HIPC(c, hipMemcpyAsync(P<double>(c->fqr0) + static_cast<int64_t>(b) * 3 * n, c->pos.p, sizeof(double) * 3 * n,
+                             hipMemcpyDeviceToDevice, c->stream));
+      a.start_now[b] = 1;
+    }
+    const int ix = iconf - c->fqt_origin[b] + 1;
+    if (iconf >= c->fqt_origin[b] && ix <= c->fqt_ndyn) {
+      a.act[a.nact] = b;
+      a.act_indx[a.nact] = ix;
+      ++a.nact;
+    }
+  }
+  const size_t np = static_cast<size_t>(c->fqt_nvec) * (a.nact + 1);
+  if ((r = ensure(c, c->fqpart, sizeof(double) * 2 * kFqtChunks * std::max<size_t>(np, 1)))) return r;
+  if ((r = ensure(c, c->fqvsum, sizeof(double) * 2 * std::max<size_t>(np, 1)))) return r;
+  a.part = P<double>(c->fqpart);
+  a.vsum = P<double>(c->fqvsum);
+  a.fkt = P<double>(c->fqfkt);
+  a.fskt = P<double>(c->fqfskt);
+  launch_fqt(a, c->stream);
+  // Origins that reached itlimit restart at the next frame; their sums go to
+  // the totals (dynamics.cuf:220-238).
+  for (int b = 0; b < nb; ++b)
+    if (iconf - c->fqt_origin[b] == c->fqt_itlimit) {
+      c->fqt_origin[b] = iconf + 1;
+      launch_fqt_collect(b, c->fqt_nqw, c->fqt_ndyn, P<double>(c->fqfkt), P<double>(c->fqfskt), P<double>(c->fqtot),
+                         P<double>(c->fqtots), c->stream);
+      ++c->fqt_count_org;
+    }
+  HIPC(c, hipGetLastError());
+  return TRJ_OK;
+}
+
 // The frame pipeline, device inputs.
 // q_dev: the frame's charges on the device (NULL without); the caller has
 // queued the frame's {scale, avq2} into qparm[0..1] on the context stream.
@@ -283,16 +358,22 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
               const double box_in[3], uint32_t what, const double* q_dev = nullptr) {
   const bool do_rdf = (what & TRJ_WANT_RDF) != 0;
   const bool do_sq = (what & TRJ_WANT_SQ) != 0;
+  const bool do_fqt = (what & TRJ_WANT_FQT) != 0;
   if (do_rdf && !c->rdf_ready) return fail(c, TRJ_ESTATE, "frame asks for RDF before trj_rdf_init");
   if (do_sq && !c->sq_ready) return fail(c, TRJ_ESTATE, "frame asks for S(Q) before trj_sq_init");
-  if (!do_rdf && !do_sq) return fail(c, TRJ_EINVAL, "frame asks for nothing");
+  if (do_fqt && !c->fqt_ready) return fail(c, TRJ_ESTATE, "frame asks for F(Q,t) before trj_fqt_init");
+  if (!do_rdf && !do_sq && !do_fqt) return fail(c, TRJ_EINVAL, "frame asks for nothing");
   const bool rdf_q = do_rdf && c->rdf_charge, sq_q = do_sq && c->sq_charge;
   if ((rdf_q || sq_q) && !q_dev)
     return fail(c, TRJ_EINVAL, "charges are enabled (TRJ_RDF_CHARGE / TRJ_SQ_CHARGE): frames go through trj_frame_q");
-  const int ndim = do_rdf ? c->rdf.ndim : c->sq.p.ndim;
-  const int nsp = do_rdf ? c->rdf.nsp : c->sq.p.nsp;
+  const int ndim = do_rdf ? c->rdf.ndim : (do_sq ? c->sq.p.ndim : c->fqt_ndim);
+  const int nsp = do_rdf ? c->rdf.nsp : (do_sq ? c->sq.p.nsp : c->fqt_nsp);
   if (do_rdf && do_sq && (c->rdf.ndim != c->sq.p.ndim || c->rdf.nsp != c->sq.p.nsp))
     return fail(c, TRJ_EINVAL, "RDF and S(Q) were initialised with different ndim/nsp");
+  if (do_fqt && (c->fqt_ndim != ndim || c->fqt_nsp != nsp))
+    return fail(c, TRJ_EINVAL, "F(Q,t) was initialised with a different ndim/nsp");
+  if (do_fqt && c->fqt_frames > 0 && n != c->fqt_n)
+    return fail(c, TRJ_EINVAL, "F(Q,t): the number of atoms changed between frames");
   if (n < 1) return fail(c, TRJ_EINVAL, "n must be positive");
   if (n > (int64_t(1) << 31) - 1024) return fail(c, TRJ_ELIMIT, "n above 2^31 is not supported");
   // The S(Q) kernels address the turn planes with 32-bit offsets up to 3 x the
@@ -567,6 +648,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
   es.has_rdf = do_rdf;
   es.has_sq = do_sq;
   es.overlapped = overlap;
+  if (do_fqt && (r = fqt_step(c, n, itype_dev))) return r;
   c->last_n = n;
   c->last_nsp = nsp;
   for (int k = 0; k < 3; ++k) c->last_box[k] = box[k];
@@ -659,7 +741,8 @@ void trj_destroy(trj_ctx* c) {
                     &c->flags, &c->qraw, &c->qsort, &c->gqf, &c->gqacc, &c->sqfq, &c->qparm, &c->slkey,
                     &c->sllast, &c->slcount, &c->slcountp, &c->slqsum, &c->slqcnt, &c->sqqsum, &c->sqqcnt,
                     &c->xyframe, &c->xycounts, &c->xyacc, &c->xygqf, &c->xygqacc, &c->thr_xy, &c->rpts,
-                    &c->s2gi, &c->s2g2i};
+                    &c->s2gi, &c->s2g2i, &c->fqkv, &c->fqbin, &c->fqr0, &c->fqrho0,
+                    &c->fqpart, &c->fqvsum, &c->fqfkt, &c->fqfskt, &c->fqtot, &c->fqtots};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   for (int k = 0; k < 2; ++k) {
@@ -1274,6 +1357,122 @@ int trj_sq_rho(trj_ctx* c, int64_t nvec, const int32_t* kv, double* rho) {
   return TRJ_OK;
 }
 
+// ------------------------------------------------------------------ F(Q,t)
+
+int trj_fqt_init(trj_ctx* c, const trj_fqt_params* p, const double box[3], int32_t* nvec_out, int64_t* nw_out) {
+  if (!c || !p || !box) return TRJ_EINVAL;
+  if (p->ndim != 2 && p->ndim != 3) return fail(c, TRJ_EINVAL, "fqt: ndim must be 2 or 3");
+  if (p->nsp < 1 || p->nsp > kNspMax) return fail(c, TRJ_EINVAL, "fqt: nsp must be in 1..10");
+  if (p->nqw < 1 || p->nqw > 64 || !p->qw) return fail(c, TRJ_EINVAL, "fqt: nqw must be in 1..64");
+  if (p->nbuffer < 1 || p->nbuffer > kFqtMaxBuffers) return fail(c, TRJ_EINVAL, "fqt: nbuffer must be in 1..64");
+  if (p->jump < 1 || p->ndyn < 1 || p->itlimit < 1) return fail(c, TRJ_EINVAL, "fqt: jump, ndyn, itlimit must be >= 1");
+  if (int r = validate_box(c, p->ndim, box)) return r;
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  // The S(Q,w) q-set (sq.cuf:234-282) in FP32: (i, j, k) >= 0, not all 0, up
+  // to nint(qmm/fk) per axis, assigned to the first qw(l) with
+  // qw(l) - dq/2 < |q| < qw(l) + dq/2.  The reference's first (counting)
+  // loop keeps k = 0 unless confined while its second (storing) loop runs k
+  // over 0..nint(qmm/fk3); the storing loop's set is used here.
+  const int ndim = p->ndim;
+  float fk[3], side[3];
+  for (int k = 0; k < 3; ++k) {
+    side[k] = static_cast<float>(k < ndim ? box[k] : 1.0);
+    fk[k] = (2.0f * 3.1415926535898f) / side[k];
+  }
+  float dq = std::fmin(fk[0], fk[1]);
+  if (ndim == 3) dq = std::fmin(dq, fk[2]);
+  std::vector<float> qw(p->nqw);
+  float qmm = 0.0f;
+  for (int l = 0; l < p->nqw; ++l) {
+    qw[l] = static_cast<float>(p->qw[l]);
+    qmm = std::fmax(qmm, qw[l]);
+  }
+  const int imax = static_cast<int>(std::lround(qmm / fk[0])), jmax = static_cast<int>(std::lround(qmm / fk[1]));
+  const int kmax = ndim == 3 ? static_cast<int>(std::lround(qmm / fk[2])) : 0;
+  std::vector<std::vector<int32_t>> bins(p->nqw);
+  for (int i = 0; i <= imax; ++i)
+    for (int j = 0; j <= jmax; ++j)
+      for (int k = 0; k <= kmax; ++k) {
+        if (i + j + k == 0) continue;
+        const float a = i * fk[0], b = j * fk[1], cc = k * fk[2];
+        const float rr = std::sqrt(a * a + b * b + cc * cc);
+        for (int l = 0; l < p->nqw; ++l)
+          if (qw[l] - dq / 2 < rr && qw[l] + dq / 2 > rr) {
+            bins[l].insert(bins[l].end(), {i, j, k});
+            break;
+          }
+      }
+  std::vector<int32_t> kv, off(p->nqw + 1, 0);
+  c->fqt_nw.assign(p->nqw, 0);
+  for (int l = 0; l < p->nqw; ++l) {
+    kv.insert(kv.end(), bins[l].begin(), bins[l].end());
+    c->fqt_nw[l] = static_cast<int64_t>(bins[l].size() / 3);
+    off[l + 1] = off[l] + static_cast<int32_t>(bins[l].size() / 3);
+  }
+  const int nvec = off[p->nqw];
+  const size_t nacc = static_cast<size_t>(p->nbuffer) * p->nqw * p->ndyn;
+  if ((r = ensure(c, c->fqkv, sizeof(int32_t) * std::max<size_t>(kv.size(), 3)))) return r;
+  if ((r = ensure(c, c->fqbin, sizeof(int32_t) * off.size()))) return r;
+  if ((r = ensure(c, c->fqrho0, sizeof(double) * 2 * std::max<size_t>(1, static_cast<size_t>(nvec) * p->nbuffer)))) return r;
+  if ((r = ensure(c, c->fqfkt, sizeof(double) * nacc))) return r;
+  if ((r = ensure(c, c->fqfskt, sizeof(double) * nacc))) return r;
+  if ((r = ensure(c, c->fqtot, sizeof(double) * p->nqw * p->ndyn))) return r;
+  if ((r = ensure(c, c->fqtots, sizeof(double) * p->nqw * p->ndyn))) return r;
+  if (!kv.empty()) HIPC(c, hipMemcpy(c->fqkv.p, kv.data(), sizeof(int32_t) * kv.size(), hipMemcpyHostToDevice));
+  HIPC(c, hipMemcpy(c->fqbin.p, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice));
+  HIPC(c, hipMemset(c->fqfkt.p, 0, sizeof(double) * nacc));
+  HIPC(c, hipMemset(c->fqfskt.p, 0, sizeof(double) * nacc));
+  HIPC(c, hipMemset(c->fqtot.p, 0, sizeof(double) * p->nqw * p->ndyn));
+  HIPC(c, hipMemset(c->fqtots.p, 0, sizeof(double) * p->nqw * p->ndyn));
+  c->fqt_nqw = p->nqw;
+  c->fqt_nbuf = p->nbuffer;
+  c->fqt_jump = p->jump;
+  c->fqt_ndyn = p->ndyn;
+  c->fqt_itlimit = p->itlimit;
+  c->fqt_nvec = nvec;
+  c->fqt_ndim = ndim;
+  c->fqt_nsp = p->nsp;
+  for (int k = 0; k < 3; ++k) c->fqt_invl[k] = 1.0 / (k < ndim ? box[k] : 1.0);
+  for (int s = 0; s < kNspMax; ++s) c->fqt_bsc.b[s] = s < p->nsp ? p->bsc[s] : 0.0;
+  c->fqt_origin.assign(p->nbuffer, 0);
+  c->fqt_frames = 0;
+  c->fqt_count_org = 0;
+  c->fqt_ready = true;
+  if (nvec_out) *nvec_out = nvec;
+  if (nw_out)
+    for (int l = 0; l < p->nqw; ++l) nw_out[l] = c->fqt_nw[l];
+  return TRJ_OK;
+}
+
+int trj_fqt_results(trj_ctx* c, double* fkt, double* fskt, int64_t* count_org) {
+  if (!c) return TRJ_EINVAL;
+  if (!c->fqt_ready) return fail(c, TRJ_ESTATE, "F(Q,t) not initialised");
+  hipSetDevice(c->device);
+  int r = sync_all(c);
+  if (r) return r;
+  const size_t m = static_cast<size_t>(c->fqt_nqw) * c->fqt_ndyn;
+  std::vector<double> a(m), b(m);
+  HIPC(c, hipMemcpy(a.data(), c->fqtot.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+  HIPC(c, hipMemcpy(b.data(), c->fqtots.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+  if (count_org) *count_org = c->fqt_count_org;
+  // print_rtcor (dynamics.cuf:268-272): / (Nsites count_org nw(i)); raw sums
+  // when no origin has completed.
+  const double N = static_cast<double>(c->fqt_n);
+  for (int l = 0; l < c->fqt_nqw; ++l) {
+    const double d = c->fqt_count_org > 0 && c->fqt_nw[l] > 0
+                         ? N * static_cast<double>(c->fqt_count_org) * static_cast<double>(c->fqt_nw[l])
+                         : 1.0;
+    for (int t = 0; t < c->fqt_ndyn; ++t) {
+      const size_t i = static_cast<size_t>(l) * c->fqt_ndyn + t;
+      if (fkt) fkt[i] = a[i] / d;
+      if (fskt) fskt[i] = b[i] / d;
+    }
+  }
+  return TRJ_OK;
+}
+
 // ------------------------------------------------------ number fluctuations
 
 namespace {
@@ -1536,6 +1735,15 @@ int trj_reset(trj_ctx* c) {
     HIPC(c, hipMemsetAsync(c->sqf.p, 0, sizeof(double) * ns, c->stream));
     HIPC(c, hipMemsetAsync(c->sqfp.p, 0, sizeof(double) * ns * c->sq.p.nsp, c->stream));
     if (c->sq_charge) HIPC(c, hipMemsetAsync(c->sqfq.p, 0, sizeof(double) * ns, c->stream));
+  }
+  if (c->fqt_ready) {
+    const size_t m = static_cast<size_t>(c->fqt_nqw) * c->fqt_ndyn;
+    HIPC(c, hipMemsetAsync(c->fqfkt.p, 0, sizeof(double) * m * c->fqt_nbuf, c->stream));
+    HIPC(c, hipMemsetAsync(c->fqfskt.p, 0, sizeof(double) * m * c->fqt_nbuf, c->stream));
+    HIPC(c, hipMemsetAsync(c->fqtot.p, 0, sizeof(double) * m, c->stream));
+    HIPC(c, hipMemsetAsync(c->fqtots.p, 0, sizeof(double) * m, c->stream));
+    c->fqt_frames = 0;
+    c->fqt_count_org = 0;
   }
   if (c->nrand > 0) {
     HIPC(c, hipMemsetAsync(c->s2gi.p, 0, sizeof(unsigned long long) * (c->rdf.lsmax + 1), c->stream));
@@ -1869,7 +2077,8 @@ int64_t trj_device_bytes(const trj_ctx* c) {
                           &c->flags, &c->qraw, &c->qsort, &c->gqf, &c->gqacc, &c->sqfq, &c->qparm, &c->slkey,
                           &c->sllast, &c->slcount, &c->slcountp, &c->slqsum, &c->slqcnt, &c->sqqsum, &c->sqqcnt,
                           &c->xyframe, &c->xycounts, &c->xyacc, &c->xygqf, &c->xygqacc, &c->thr_xy, &c->rpts,
-                          &c->s2gi, &c->s2g2i};
+                          &c->s2gi, &c->s2g2i, &c->fqkv, &c->fqbin, &c->fqr0, &c->fqrho0,
+                    &c->fqpart, &c->fqvsum, &c->fqfkt, &c->fqfskt, &c->fqtot, &c->fqtots};
   int64_t s = 0;
   for (const DevBuf* b : bufs) s += static_cast<int64_t>(b->cap);
   return s;

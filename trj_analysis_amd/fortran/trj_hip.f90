@@ -9,12 +9,12 @@ module trj_hip
   integer(c_int), parameter :: TRJ_OK = 0
   integer(c_int32_t), parameter :: TRJ_F32 = 0, TRJ_F64 = 1
   integer(c_int32_t), parameter :: TRJ_AOS = 0, TRJ_SOA = 1
-  integer(c_int32_t), parameter :: TRJ_WANT_RDF = 1, TRJ_WANT_SQ = 2
+  integer(c_int32_t), parameter :: TRJ_WANT_RDF = 1, TRJ_WANT_SQ = 2, TRJ_WANT_FQT = 4
   integer(c_int32_t), parameter :: TRJ_QSET_REF = 0, TRJ_QSET_HALFSPACE = 1
   integer(c_int32_t), parameter :: TRJ_PREC_FP64 = 0, TRJ_PREC_FP32 = 1, &
                                    TRJ_PREC_FP16X3 = 2, TRJ_PREC_BF16X3 = 3
   integer(c_int32_t), parameter :: TRJ_SQ_NQ_CONSISTENT = 1, TRJ_SQ_CHARGE = 2
-  integer(c_int32_t), parameter :: TRJ_RDF_CHARGE = 1
+  integer(c_int32_t), parameter :: TRJ_RDF_CHARGE = 1, TRJ_RDF_FLOOR_BINS = 2
   integer(c_int), parameter :: TRJ_PIPE_SERIAL = 0, TRJ_PIPE_OVERLAP = 1
   integer(c_int), parameter :: TRJ_STATE_REPLACE = 0, TRJ_STATE_ADD = 1
 
@@ -39,6 +39,12 @@ module trj_hip
     real(c_double) :: dq, fk(3), qmin_eff
     integer(c_int32_t) :: nqmax, nqmin, lmax(3)
     integer(c_int64_t) :: nvec_grid, nvec_axis, box_ext(3)
+  end type
+
+  ! F(Q,t) over time origins (qw and bsc: c_loc of real(c_double) arrays)
+  type, bind(C) :: trj_fqt_params
+    integer(c_int32_t) :: ndim, nsp, nqw, nbuffer, jump, ndyn, itlimit, pad = 0
+    type(c_ptr) :: qw, bsc
   end type
 
   ! NetCDF trajectory (classic formats); the handle is an opaque type(c_ptr)
@@ -124,6 +130,20 @@ module trj_hip
       type(c_ptr), value :: ctx
       integer(c_int64_t), value :: nconf
       real(c_double), intent(out) :: out(*)
+    end function
+    integer(c_int) function trj_fqt_init(ctx, p, box, nvec, nw) bind(C, name="trj_fqt_init")
+      import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t, trj_fqt_params
+      type(c_ptr), value :: ctx
+      type(trj_fqt_params), intent(in) :: p
+      real(c_double), intent(in) :: box(3)
+      integer(c_int32_t), intent(out) :: nvec
+      integer(c_int64_t), intent(out) :: nw(*)
+    end function
+    integer(c_int) function trj_fqt_results(ctx, fkt, fskt, count_org) bind(C, name="trj_fqt_results")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: fkt(*), fskt(*)
+      integer(c_int64_t), intent(out) :: count_org
     end function
     ! number fluctuations (nrandom of /INPUT_RDF/), after trj_rdf_init
     integer(c_int) function trj_rdf_sampling(ctx, nrandom, seed) bind(C, name="trj_rdf_sampling")

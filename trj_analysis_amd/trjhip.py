@@ -20,7 +20,7 @@ TRJ_OK = 0
 TRJ_EINVAL, TRJ_EDEVICE, TRJ_ENOMEM, TRJ_ESTATE, TRJ_ERANGE, TRJ_ELIMIT = -1, -2, -3, -4, -5, -6
 TRJ_F32, TRJ_F64 = 0, 1
 TRJ_AOS, TRJ_SOA = 0, 1
-TRJ_WANT_RDF, TRJ_WANT_SQ = 1, 2
+TRJ_WANT_RDF, TRJ_WANT_SQ, TRJ_WANT_FQT = 1, 2, 4
 TRJ_QSET_REF, TRJ_QSET_HALFSPACE = 0, 1
 TRJ_PREC_FP64, TRJ_PREC_FP32, TRJ_PREC_FP16X3, TRJ_PREC_BF16X3 = 0, 1, 2, 3
 TRJ_SQ_NQ_CONSISTENT, TRJ_SQ_CHARGE = 1, 2
@@ -36,7 +36,7 @@ EXPORTS = [
     "trj_abi_version", "trj_create", "trj_destroy", "trj_last_error", "trj_rdf_init",
     "trj_sq_init", "trj_sq_plan", "trj_rdf_plan", "trj_frame", "trj_frame_device", "trj_frame_device_q", "trj_sync",
     "trj_rdf_counts", "trj_rdf_gmix", "trj_rdf_first_minimum", "trj_sq_shells", "trj_sq_columns",
-    "trj_sq_rho", "trj_rdf_sampling", "trj_rdf_sampling_points", "trj_rdf_s2n_sums", "trj_rdf_s2n",
+    "trj_sq_rho", "trj_fqt_init", "trj_fqt_results", "trj_rdf_sampling", "trj_rdf_sampling_points", "trj_rdf_s2n_sums", "trj_rdf_s2n",
     "trj_set_slices", "trj_slice_counts", "trj_rdf_xy", "trj_rdf_xy_gmix", "trj_rdf_xy_qq",
     "trj_rdf_xy_gcc", "trj_sq_xy", "trj_sq_xy_columns", "trj_frame_q", "trj_rdf_qq", "trj_rdf_gqq", "trj_sq_qq", "trj_sq_qq_column", "trj_reset", "trj_last_timings", "trj_timing_totals", "trj_timing_reset",
     "trj_device_bytes", "trj_set_pipeline", "trj_state_bytes", "trj_state_save", "trj_state_load",
@@ -64,6 +64,12 @@ class SqParams(C.Structure):
     _fields_ = [("ndim", C.c_int32), ("nsp", C.c_int32), ("qmax", C.c_double),
                 ("qmin", C.c_double), ("bsc", C.c_double * 10), ("qset", C.c_int32),
                 ("grid", C.c_int32 * 3), ("precision", C.c_int32), ("flags", C.c_uint32)]
+
+
+class FqtParams(C.Structure):
+    _fields_ = [("ndim", C.c_int32), ("nsp", C.c_int32), ("nqw", C.c_int32), ("nbuffer", C.c_int32),
+                ("jump", C.c_int32), ("ndyn", C.c_int32), ("itlimit", C.c_int32), ("pad", C.c_int32),
+                ("qw", C.POINTER(C.c_double)), ("bsc", C.POINTER(C.c_double))]
 
 
 class SqInfo(C.Structure):
@@ -121,6 +127,8 @@ def _declare(L):
     L.trj_rdf_qq.argtypes = [vp, d]
     L.trj_set_slices.argtypes = [vp, C.c_int32, C.POINTER(C.c_float), C.c_float]
     L.trj_rdf_sampling.argtypes = [vp, C.c_int32, C.c_uint64]
+    L.trj_fqt_init.argtypes = [vp, C.POINTER(FqtParams), d, C.POINTER(C.c_int32), i64]
+    L.trj_fqt_results.argtypes = [vp, d, d, i64]
     L.trj_rdf_sampling_points.argtypes = [vp, d]
     L.trj_rdf_s2n_sums.argtypes = [vp, i64, i64]
     L.trj_rdf_s2n.argtypes = [vp, C.c_int64, d]
@@ -483,6 +491,30 @@ class Context:
         out = np.zeros((max(r.lsmax - 2, 0), 1 + r.nit))
         self._check(self._L.trj_rdf_gmix(self._h, nconf, _ptr(out, C.c_double)), "trj_rdf_gmix")
         return out
+
+    # ------------------------------------------------------------ F(Q,t)
+    def fqt_init(self, qw, nbuffer: int, jump: int, ndyn: int, itlimit: int, box, ndim: int = 3, nsp: int = 1,
+                 bsc=None):
+        """F(Q,t) over time origins (trj_fqt_init): returns (nvec, nw)."""
+        q = np.ascontiguousarray(np.asarray(qw, dtype=np.float64).ravel())
+        b = np.ascontiguousarray(np.asarray([1.0] * nsp if bsc is None else bsc, dtype=np.float64))
+        p = FqtParams(ndim, nsp, len(q), nbuffer, jump, ndyn, itlimit, 0, _ptr(q, C.c_double), _ptr(b, C.c_double))
+        nv = C.c_int32(0)
+        nw = np.zeros(len(q), dtype=np.int64)
+        bx = _box3(box)
+        self._check(self._L.trj_fqt_init(self._h, C.byref(p), _ptr(bx, C.c_double), C.byref(nv), _ptr(nw, C.c_int64)),
+                    "trj_fqt_init")
+        self.fqt_shape = (len(q), ndyn)
+        return nv.value, nw
+
+    def fqt_results(self):
+        """(fkt (nqw, ndyn), fskt (nqw, ndyn), count_org), normalised as print_rtcor."""
+        f = np.zeros(self.fqt_shape)
+        fs = np.zeros(self.fqt_shape)
+        co = C.c_int64(0)
+        self._check(self._L.trj_fqt_results(self._h, _ptr(f, C.c_double), _ptr(fs, C.c_double), C.byref(co)),
+                    "trj_fqt_results")
+        return f, fs, co.value
 
     # ------------------------------------------------ number fluctuations
     def rdf_sampling(self, nrandom: int, seed: int = 12345):
