@@ -74,6 +74,11 @@ extern "C" {
                                    vectors of shell nqmin */
 #define TRJ_SQ_CHARGE 2u        /* also sum |sum_j q_j e^{iq.r_j}|^2 (S_qq; sq.cuf:424-432,
                                    563-582 with ex_qc); frames come through trj_frame_q */
+#define TRJ_SQ_CENTRES 4u       /* sqfnDcl (sq.cuf:798-839, the S(Q) of cluster centres
+                                   passed as the frame's points): each vector's |rho|^2
+                                   divided by the frame's point count, octant set only
+                                   (no axis vectors); sqcl.dat is sqf(i)/(Nconf nq(i)),
+                                   i = 1..nqmin (log.f90:637-642) */
 
 /* RDF flags. */
 #define TRJ_RDF_CHARGE 1u /* also accumulate the charge-weighted histogram g_qq

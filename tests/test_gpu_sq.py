@@ -199,3 +199,28 @@ def test_mfma_rate_probe(prec):
     print(prec, "rate %.4g flop/s = %.3f of nominal, clock %.0f MHz" % (rate, rate / nominal, mhz), flush=True)
     assert 0.3 * nominal < rate < 1.05 * nominal
     assert 500.0 < mhz < 3000.0
+
+
+def test_cluster_centres_sqfndcl():
+    """TRJ_SQ_CENTRES (sqfnDcl): per frame, each octant vector's |rho|^2 / Nbcl,
+    no axis vectors; frames with different point counts."""
+    rng = np.random.default_rng(17)
+    L = 20.0
+    box = [L] * 3
+    frames = [rng.random((m, 3)) * L for m in (40, 57, 33)]
+    print("case: cluster centres S(Q)", flush=True)
+    with T.Context(0) as ctx:
+        info = ctx.sq_init(T.sq_params(3, 1, 6.0, 4.0, [1.0], flags=T.TRJ_SQ_CENTRES), box)
+        for x in frames:
+            ctx.frame(x, np.ones(len(x), dtype=np.int32), box, what=T.TRJ_WANT_SQ)
+        sqf, _, nq = ctx.sq_shells()
+    assert info.nvec_axis == 0
+    g = O.sq_setup(3, box, 6.0, 4.0)
+    want = np.zeros(g.nqmax + 1)
+    for x in frames:
+        f, _ = O.sq_frame(x, np.ones(len(x), dtype=np.int32), 3, 1, box, g, [1.0])
+        # the oracle's set adds the axis shells above nqmin; keep the octant shells
+        want[: g.nqmin + 1] += f[: g.nqmin + 1] / len(x)
+    m = want > 0
+    assert np.max(np.abs(sqf[m] - want[m]) / want[m]) < 1e-12
+    assert not sqf[g.nqmin + 1:].any()

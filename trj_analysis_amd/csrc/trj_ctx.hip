@@ -56,6 +56,7 @@ struct trj_ctx {
   // frame with charges (the S_qq column divides by it).
   DevBuf qparm;
   bool rdf_charge = false, sq_charge = false;
+  bool sq_centres = false;  // TRJ_SQ_CENTRES: per-frame 1/N, octant set only (sqfnDcl)
   // Confined slices (twoDstruc_3D; off when slices.n == 0): per-atom keys,
   // the species-sorted last atom, per-slice counts and q^2 sums of the frame
   // (and of the last S(Q) frame), the slice pair histograms (frame, raw
@@ -617,7 +618,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     HIPC(c, hipEventRecord(es.ev[3], c->stream));
     // With slices every vector's values are divided by the slice's (species')
     // atom count (sqf3xy / sqf2xy, sq.cuf:653-670, 741-765).
-    const int32_t* norm = nsl > 0 ? P<int32_t>(c->sp_off) : nullptr;
+    const int32_t* norm = (nsl > 0 || c->sq_centres) ? P<int32_t>(c->sp_off) : nullptr;
     const int ngs = nsl > 0 ? nsl : 1;
     if (ntiles > 0)
       launch_sq_shells(P<double>(c->rho), P<int16_t>(c->tile_shell), ntiles, S.tk.vec(), c->ksplit, nsp, ngs, nch,
@@ -626,7 +627,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     if ((r = ensure(c, c->listpart, sq_list_scratch_bytes(nax, nch)))) return r;
     launch_sq_list(P<double>(c->u), sq_q ? P<double>(c->u) + 12 * ustride : nullptr, P<int32_t>(c->sp_off), ustride,
                    nsp, ngs, nch, sdim, P<int32_t>(c->axis_kv), P<int32_t>(c->axis_shell), nax, S.p.bsc, S.nqmax,
-                   nsl > 0, P<double>(c->listpart), P<double>(c->sqf), P<double>(c->sqfp), P<double>(c->sqfq),
+                   nsl > 0 || c->sq_centres, P<double>(c->listpart), P<double>(c->sqf), P<double>(c->sqfp), P<double>(c->sqfq),
                    c->stream);
     // sq.cuf:143-145 divides by avq2 of the last frame: keep it in qparm[2];
     // with slices, by each slice's mean q^2 of the last frame (sq.cuf:135-138).
@@ -914,6 +915,14 @@ int trj_sq_init(trj_ctx* c, const trj_sq_params* p, const double box[3], trj_sq_
   } else {
     r = sq_plan(*p, box, &S, &err);
     if (r) return fail(c, r, err);
+    if (p->flags & TRJ_SQ_CENTRES) {
+      // sqfnDcl (sq.cuf:798-839) sums the octant set only.
+      if (p->qset != TRJ_QSET_REF) return fail(c, TRJ_EINVAL, "sq: TRJ_SQ_CENTRES uses the reference q-set");
+      for (int k : S.axis_shell) S.nq_summed[k] -= 1;
+      S.axis_kv.clear();
+      S.axis_shell.clear();
+      S.info.nvec_axis = 0;
+    }
   }
   const int ns = S.nqmax + 1;
   const int ngs = nsl > 0 ? nsl : 1;
@@ -950,6 +959,7 @@ int trj_sq_init(trj_ctx* c, const trj_sq_params* p, const double box[3], trj_sq_
     c->tile_index[pack_tile(S.tiles[t].kx0, S.tiles[t].ky0, S.tiles[t].kz0)] = static_cast<int>(t);
   c->sq = std::move(S);
   c->sq_charge = sqq;
+  c->sq_centres = (p->flags & TRJ_SQ_CENTRES) != 0;
   c->sq_ready = true;
   c->sq_frames = 0;
   if (info) *info = c->sq.info;

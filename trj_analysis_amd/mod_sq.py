@@ -133,3 +133,24 @@ def printSQ(Nsites: int, nconf: int = 0, outdir: str = ".") -> list:
         for row in mix:
             f.write("".join(f"{v:16.7f}" for v in row) + "\n")
     return [p_sq, p_mix]
+
+
+def printSQcl(nconf: int, outdir: str = ".", ctx: Optional[T.Context] = None) -> str:
+    """sqcl.dat (log.f90:637-642) of a context initialised with TRJ_SQ_CENTRES
+    and fed the cluster centres (sqfnDcl): i dq, sqfcl(i)/(Nconf nq(i)) for
+    i = 1..nqmin."""
+    c = ctx or _state["ctx"]
+    if nconf <= 0:
+        raise ValueError("printSQcl needs Nconf (the number of frames)")
+    sqf, _, nq = c.sq_shells()
+    nc = nconf
+    info = c.sq_info
+    path = os.path.join(outdir, "sqcl.dat")
+    with open(path, "w") as f:
+        f.write("#      Q            S_cl-cl(Q)\n")
+        for i in range(1, info.nqmin + 1):
+            q = float(np.float32(i) * np.float32(info.dq))
+            v = sqf[i] / (nc * nq[i]) if nq[i] else 0.0
+            f.write(f"{q:15.7f}{v:15.7f}\n")
+    return path
+

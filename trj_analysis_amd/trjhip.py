@@ -23,7 +23,7 @@ TRJ_AOS, TRJ_SOA = 0, 1
 TRJ_WANT_RDF, TRJ_WANT_SQ, TRJ_WANT_FQT = 1, 2, 4
 TRJ_QSET_REF, TRJ_QSET_HALFSPACE = 0, 1
 TRJ_PREC_FP64, TRJ_PREC_FP32, TRJ_PREC_FP16X3, TRJ_PREC_BF16X3 = 0, 1, 2, 3
-TRJ_SQ_NQ_CONSISTENT, TRJ_SQ_CHARGE = 1, 2
+TRJ_SQ_NQ_CONSISTENT, TRJ_SQ_CHARGE, TRJ_SQ_CENTRES = 1, 2, 4
 TRJ_RDF_CHARGE, TRJ_RDF_FLOOR_BINS = 1, 2
 TRJ_PIPE_SERIAL, TRJ_PIPE_OVERLAP = 0, 1
 TRJ_STATE_REPLACE, TRJ_STATE_ADD = 0, 1
