@@ -224,3 +224,28 @@ def test_cluster_centres_sqfndcl():
     m = want > 0
     assert np.max(np.abs(sqf[m] - want[m]) / want[m]) < 1e-12
     assert not sqf[g.nqmin + 1:].any()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp16x3", 1e-5)])
+def test_2d_shells(precision, tol):
+    """ndim = 2 (sqfnD's 2D set and sqfact2's x, y axes) against the oracle."""
+    rng = np.random.default_rng(23)
+    L = 40.0
+    n = 3000
+    pos = np.zeros((n, 3))
+    pos[:, :2] = rng.random((n, 2)) * L
+    it = rng.integers(1, 3, n).astype(np.int32)
+    box = [L, L, 1.0]
+    print("case: 2d S(Q)", precision, flush=True)
+    with T.Context(0) as ctx:
+        info = ctx.sq_init(T.sq_params(2, 2, 6.0, 4.5, [1.0, 0.5], precision=precision), box)
+        ctx.frame(pos[:, :2].copy(), it, box, what=T.TRJ_WANT_SQ)
+        sqf, sqfp, nq = ctx.sq_shells()
+    g = O.sq_setup(2, box, 6.0, 4.5)
+    assert info.nqmax == g.nqmax and info.nqmin == g.nqmin
+    osqf, osqfp = O.sq_frame(pos, it, 2, 2, box, g, [1.0, 0.5])
+    m = osqf > 0
+    assert np.max(np.abs(sqf[m] - osqf[m]) / osqf[m]) <= tol
+    mp = osqfp > 0
+    assert np.max(np.abs(sqfp[mp] - osqfp[mp]) / osqfp[mp]) <= tol
+    np.testing.assert_array_equal(nq, O.sq_nq(2, g))
