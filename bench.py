@@ -225,8 +225,10 @@ def main():
     ap.add_argument("--pipeline", default="serial", choices=["serial", "overlap"],
                     help="overlap: the RDF runs on its own stream beside the S(Q) contraction")
     ap.add_argument("--latency-frames", type=int, default=10)
-    ap.add_argument("--charges", action="store_true",
-                    help="also accumulate g_qq and S_qq (per-atom charges, trj_frame_device_q)")
+    ap.add_argument("--charges", nargs="?", const="atoms", default=None, choices=["atoms", "species"],
+                    help="also accumulate g_qq and S_qq: 'atoms' = per-atom charges from HBM "
+                         "(trj_frame_device_q); 'species' = per-type charges through the host entry "
+                         "trj_frame_q, which forms both from the species sums (inputs host-staged)")
     ap.add_argument("--cpu-baseline-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -253,10 +255,13 @@ def main():
     dpos = [torch.from_numpy(np.ascontiguousarray(f.pos, dtype=np.float64)).to(dev) for f in frames]
     dtyp = torch.from_numpy(frames[0].itype.astype(np.int32)).to(dev)
     dq = None
-    if args.charges:  # species charges +-0.8 with a small per-atom spread, seeded
+    qh = None
+    if args.charges == "atoms":  # species charges +-0.8 with a small per-atom spread, seeded
         rng = np.random.default_rng(7)
         qh = np.where(frames[0].itype == 1, 0.8, -0.8) + 0.05 * rng.standard_normal(n)
         dq = torch.from_numpy(qh).to(dev)
+    elif args.charges == "species":  # per-type charges
+        qh = np.where(frames[0].itype == 1, 0.8, -0.8)
     torch.cuda.synchronize()
 
     ctx = T.Context(0)
@@ -270,7 +275,13 @@ def main():
                                    flags=sflags), box)
     nvec = info.nvec_grid + info.nvec_axis
 
+    hpos = [np.ascontiguousarray(f.pos, dtype=np.float64) for f in frames] if args.charges == "species" else None
+    hit = frames[0].itype.astype(np.int32)
+
     def submit(k):
+        if args.charges == "species":
+            ctx.frame(hpos[k % pool], hit, box, charge=qh)
+            return
         ctx.frame_device(dpos[k % pool].data_ptr(), T.TRJ_F64, T.TRJ_AOS, dtyp.data_ptr(), n, box,
                          charge_ptr=None if dq is None else dq.data_ptr())
 
@@ -305,7 +316,7 @@ def main():
     fps = args.steps / elapsed
     # With charges every atom enters the contraction twice: in its species
     # channel and in the charge channel.
-    n_entries = 2 * n if args.charges else n
+    n_entries = 2 * n if args.charges == "atoms" else n
     flops = 8.0 * n_entries * info.nvec_grid
     kname = "k_sq_contract_f64" if args.precision == "fp64" else f"k_sq_contract_lp<{args.precision}>"
     traffic = None
@@ -345,8 +356,8 @@ def main():
         "config": {"workload": args.workload, "n_atoms": n, "nsp": wl.nsp, "rcrdf": wl.rcrdf,
                    "deltar": wl.deltar, "qmax": wl.qmax, "qset": wl.qset, "q_vectors": nvec,
                    "lsmax": st.lsmax, "precision": args.precision, "pipeline": args.pipeline,
-                   "charges": bool(args.charges),
-                   "inputs": "HBM-resident float64"},
+                   "charges": args.charges or "none",
+                   "inputs": "host-staged float64" if args.charges == "species" else "HBM-resident float64"},
         "pairs_per_s": pairs / elapsed,
         "q_atom_terms_per_s": nvec * n * args.steps / elapsed,
         "latency_ms_p50": float(np.percentile(lat, 50)) if lat else None,

@@ -246,3 +246,56 @@ def test_device_entry_matches_host_entry():
     np.testing.assert_allclose(d[2], h[2], rtol=1e-14, atol=0)
     np.testing.assert_allclose(d[3], h[3], rtol=5e-13, atol=0)
     np.testing.assert_allclose(d[4], h[4], rtol=5e-13, atol=0)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp16x3", 1e-5)])
+def test_sqq_species_charges_without_channel(precision, tol):
+    """Charges constant per species (per-type charges): the host path forms
+    rho_q = sum_s q_s rho_s from the species channels; S_qq matches the oracle
+    and the direct channel (jittered charges) path stays as it was."""
+    frames = list(W.lj_binary_4k(nframes=2))
+    wl = W.LJ_BINARY_4K
+    qs = [np.where(f.itype == 1, 0.8473, -0.8473) for f in frames]
+    print("case: S_qq species charges", precision, flush=True)
+    with T.Context(0) as ctx:
+        ctx.sq_init(T.sq_params(3, 2, wl.qmax, wl.qmin, wl.bsc, precision=precision,
+                                flags=T.TRJ_SQ_CHARGE), frames[0].box)
+        for f, q in zip(frames, qs):
+            ctx.frame(f.pos, f.itype, f.box, what=T.TRJ_WANT_SQ, charge=q)
+        sqfq = ctx.sq_qq()
+        sqf, _, _ = ctx.sq_shells()
+        kv = np.array([[1, 2, 3], [4, 0, 1]], dtype=np.int32)
+        rho = ctx.sq_rho(kv)                          # the rho layout without a charge channel
+    g = O.sq_setup(3, frames[0].box, wl.qmax, wl.qmax)
+    osqfq = np.zeros(g.nqmax + 1)
+    osqf = np.zeros(g.nqmax + 1)
+    osqfp = np.zeros((2, g.nqmax + 1))
+    for f, q in zip(frames, qs):
+        O.sq_frame_q(_wrapped(f.pos, f.box), q, 3, f.box, g, osqfq)
+        O.sq_frame(_wrapped(f.pos, f.box), f.itype, 3, 2, f.box, g, wl.bsc, osqf, osqfp)
+    assert _rel(sqfq, osqfq) <= tol and _rel(sqf, osqf) <= tol
+    orho = O.sq_rho(_wrapped(frames[-1].pos, frames[-1].box), frames[-1].itype, 2, frames[-1].box, kv)
+    assert np.abs(rho - orho).max() < (1e-9 if precision == "fp64" else 1e-2)
+
+
+@pytest.mark.parametrize("case", ["cells", "allpairs"])
+def test_gqq_species_charges_from_histogram(case):
+    """Charges constant per species: the frame's fixed-point sums come from the
+    species histogram (merge kernel), bit-exact against the oracle's pair-by-
+    pair sums."""
+    if case == "cells":
+        fr0 = list(W.lj_binary_4k(nframes=2))
+        box = fr0[0].box
+        fr = [(f.pos, f.itype, np.where(f.itype == 1, 0.8473, -0.8473)) for f in fr0]
+        rc, dr = 4.0, 0.02
+    else:
+        rng = np.random.default_rng(9)
+        L = 7.0
+        box = [L] * 3
+        it = rng.integers(1, 3, 500).astype(np.int32)
+        fr = [(rng.random((500, 3)) * L, it, np.where(it == 1, 1.1, -0.55)) for _ in range(2)]
+        rc, dr = 3.4, 0.02
+    print("case: g_qq species charges", case, flush=True)
+    acc, gqq, counts, ls = _run_rdf(fr, 3, 2, box, (1, 1, 1), rc, dr)
+    oacc = _oracle_gqq_acc(fr, 3, box, (1, 1, 1), rc, dr, ls)
+    np.testing.assert_array_equal(acc, oacc)

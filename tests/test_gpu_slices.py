@@ -148,3 +148,27 @@ def test_slice_argument_checks():
         ctx.rdf_init(T.rdf_params(3, 1, 0.1, 3.0, (1, 1, 0)), [10.0, 10.0, 10.0])
         with pytest.raises(T.TrjError):                # after init
             ctx.set_slices([1.0], 2.0)
+
+
+def test_sq_xy_species_charges():
+    """Charges constant per species: the slice charge sums are formed from the
+    (slice, species) channels, no charge channel; S_xy_qq matches the oracle."""
+    frames, it = _frames(2, 4000, 26.0, -6.0, 14.0, seed=13)
+    box = [26.0, 26.0, 28.0]
+    zs, zg = [-2.0, 9.0], 7.0
+    bsc = [1.0, 0.7]
+    q = np.where(it == 1, 0.6, -0.6)
+    print("case: sq_xy species charges", flush=True)
+    with T.Context(0) as ctx:
+        ctx.set_slices(zs, zg)
+        ctx.sq_init(T.sq_params(3, 2, 8.0, 5.0, bsc, flags=T.TRJ_SQ_CHARGE), box)
+        for pos in frames:
+            ctx.frame(pos, it, box, what=T.TRJ_WANT_SQ, charge=q)
+        f, fp, fq = ctx.sq_xy(charges=True)
+    g = O.sq_setup(2, box, 8.0, 5.0)
+    acc = None
+    for pos in frames:
+        acc = O.sq_frame_xy(pos, it, 2, box, g, bsc, zs, zg, q=q, acc=acc)
+    of, ofp, ofq = acc
+    assert _rel(f, of) <= 1e-12 and _rel(fp, ofp) <= 1e-12 and _rel(fq, ofq) <= 1e-12
+    assert np.abs(ofq).max() > 0

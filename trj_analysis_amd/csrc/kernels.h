@@ -157,9 +157,14 @@ void launch_s2n(const S2nArgs& a, hipStream_t s);
 void launch_xy_finish(int nslice, int nsp, int lsmax, int64_t* frame, const int32_t* count, const int32_t* countp,
                       int64_t* counts, double* hxy, unsigned long long* gq, const double* qp, const double* qsum,
                       const int32_t* qcnt, double* gqacc, hipStream_t s);
-// counts[b] += sum over slices of partial[slice][b].
+// counts[b] += sum over slices of partial[slice][b]; with qw (species
+// charges) also gq[k] += qw->w[ij] * that sum, b = ij * lsmax + k.
+struct QqWeights {
+  long long w[kNspMax * (kNspMax + 1) / 2];  // rint(q_i q_j scale) per species pair (ind_ij order)
+};
 void launch_rdf_merge(const uint32_t* partial, int nslices, int nbins, int64_t* counts,
-                      hipStream_t s);
+                      hipStream_t s, const QqWeights* qw = nullptr, int lsmax = 0,
+                      unsigned long long* gq = nullptr);
 // acc[k] += ((double)gq[k] / qp[0]) / qp[1], then gq[k] = 0 (k < lsmax).
 void launch_qq_finish(unsigned long long* gq, int lsmax, const double* qp, double* acc, hipStream_t s);
 // qp = {2^(30-e) with max q^2 < 2^e, sum(q^2)/n} from device charges q[n], in
@@ -187,15 +192,18 @@ void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s);
 // per slice; outputs sqf [nslice][nqmax+1], sqfp [nslice][nsp][nqmax+1],
 // sqfq [nslice][nqmax+1].  norm (the channel offsets, may be NULL) divides
 // each vector's values by the atom counts (slices).
+// qsp (may be NULL): charges constant within each species; the charge sums
+// are then sum_s q_s rho_s of the species channels (no charge channel).
 void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, int tile_vec, int ksplit,
-                      int nsp, int nslice, int nch, const double* bsc, int nqmax, const int32_t* norm, double* sqf,
-                      double* sqfp, double* sqfq, hipStream_t s);
+                      int nsp, int nslice, int nch, const double* bsc, int nqmax, const int32_t* norm,
+                      const double* qsp, double* sqf, double* sqfp, double* sqfq, hipStream_t s);
 // Explicit q-vectors (axis directions): direct FP64 phase sums over atom
 // blocks, combined in a fixed order; part needs sq_list_scratch_bytes().
 size_t sq_list_scratch_bytes(int nvec, int nch);
 void launch_sq_list(const double* u, const double* w, const int32_t* sp_off, int64_t stride, int nsp, int nslice,
                     int nch, int ndim, const int32_t* kv, const int32_t* shell, int nvec, const double* bsc,
-                    int nqmax, bool norm, double* part, double* sqf, double* sqfp, double* sqfq, hipStream_t s);
+                    int nqmax, bool norm, const double* qsp, double* part, double* sqf, double* sqfp, double* sqfq,
+                    hipStream_t s);
 
 // LDS-atomic throughput probe (random bins of a 4096-bin uint32 histogram);
 // returns the number of atomics the launch performs; *out gets their sum.

@@ -954,8 +954,13 @@ __global__ __launch_bounds__(1024) void k_qq_params(const double* __restrict__ q
 // into the int64 totals (integer atomics: the result is order independent).
 constexpr int kMergeGroups = 32;
 
+// With species charges (qw not NULL): every pair of species pair ij adds
+// the same rint(q_i q_j scale) = qw->w[ij] to the fixed-point g_qq sums, so
+// the frame's sums are sum_ij w_ij count_ij[k] (gq[k], int64 two's
+// complement), exactly what the pair kernel's charge path adds pair by pair.
 __global__ __launch_bounds__(256) void k_rdf_merge(const uint32_t* __restrict__ partial, int nslices,
-                                                   int nbins, int64_t* __restrict__ counts) {
+                                                   int nbins, int64_t* __restrict__ counts, QqWeights qw,
+                                                   int lsmax, unsigned long long* __restrict__ gq) {
   const int b = blockIdx.x * 256 + threadIdx.x;
   if (b >= nbins) return;
   const int g = blockIdx.y;
@@ -963,7 +968,12 @@ __global__ __launch_bounds__(256) void k_rdf_merge(const uint32_t* __restrict__ 
   const int s1 = static_cast<int>(static_cast<int64_t>(nslices) * (g + 1) / kMergeGroups);
   uint64_t s = 0;
   for (int sl = s0; sl < s1; ++sl) s += partial[static_cast<int64_t>(sl) * nbins + b];
-  if (s) atomicAdd(reinterpret_cast<unsigned long long*>(&counts[b]), static_cast<unsigned long long>(s));
+  if (s) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(&counts[b]), static_cast<unsigned long long>(s));
+    if (gq)
+      atomicAdd(&gq[b % lsmax],
+                static_cast<unsigned long long>(qw.w[b / lsmax] * static_cast<long long>(s)));
+  }
 }
 
 // ------------------------------------------------------------------ S(Q)
@@ -1609,13 +1619,15 @@ struct Bsc {
 // sqf3xy/sqf2xy, sq.cuf:653-670) each value is divided by the slice's (or
 // channel's) atom count and skipped when that count is 0; without, nslice = 1
 // and values are added as they are.
+// With qsp (charges constant within each species) the charge channel is not
+// contracted: rho_q = sum_s q_s rho_s of the slice's species channels.
 template <class Fetch, class Add>
 __device__ inline void shell_vector(int nsp, int nslice, int nch, const Bsc& bsc, const int32_t* __restrict__ norm,
-                                    Fetch fetch, Add add) {
+                                    const Bsc* qsp, Fetch fetch, Add add) {
   for (int sl = 0; sl < nslice; ++sl) {
     double cnt = 1.0;
     if (norm) cnt = static_cast<double>(norm[(sl + 1) * nsp] - norm[sl * nsp]);
-    double tr = 0.0, ti = 0.0;
+    double tr = 0.0, ti = 0.0, qr = 0.0, qi = 0.0;
     for (int s = 0; s < nsp; ++s) {
       const int c = sl * nsp + s;
       double rr, ri;
@@ -1629,10 +1641,16 @@ __device__ inline void shell_vector(int nsp, int nslice, int nch, const Bsc& bsc
       }
       tr += bsc.b[s] * rr;
       ti += bsc.b[s] * ri;
+      if (qsp) {
+        qr += qsp->b[s] * rr;
+        qi += qsp->b[s] * ri;
+      }
     }
     if (cnt > 0.0) {
       add(0, sl, norm ? (tr * tr + ti * ti) / cnt : tr * tr + ti * ti);
-      if (nch > nslice * nsp) {
+      if (qsp) {
+        add(2, sl, norm ? (qr * qr + qi * qi) / cnt : qr * qr + qi * qi);
+      } else if (nch > nslice * nsp) {
         double rr, ri;
         fetch(nslice * nsp + sl, rr, ri);
         add(2, sl, norm ? (rr * rr + ri * ri) / cnt : rr * rr + ri * ri);
@@ -1644,7 +1662,7 @@ __device__ inline void shell_vector(int nsp, int nslice, int nch, const Bsc& bsc
 __global__ __launch_bounds__(256) void k_sq_shells(const double* __restrict__ rho,
                                                    const int16_t* __restrict__ tile_shell, int ntiles,
                                                    int tile_vec, int ksplit, int nsp, int nslice, int nch, Bsc bsc,
-                                                   int nqmax, const int32_t* __restrict__ norm,
+                                                   int nqmax, const int32_t* __restrict__ norm, Bsc qsp, int qsp_on,
                                                    double* __restrict__ sqf, double* __restrict__ sqfp,
                                                    double* __restrict__ sqfq, int use_lds) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1652,7 +1670,7 @@ __global__ __launch_bounds__(256) void k_sq_shells(const double* __restrict__ rh
   double* acc = reinterpret_cast<double*>(smem);
   const int ns = nqmax + 1;
   const int o1 = nslice * ns, o2 = nslice * (1 + nsp) * ns;
-  const int nacc = o2 + (nch > nslice * nsp ? nslice * ns : 0);
+  const int nacc = o2 + ((nch > nslice * nsp || qsp_on) ? nslice * ns : 0);
   if (use_lds) {
     for (int b = threadIdx.x; b < nacc; b += 256) acc[b] = 0.0;
     __syncthreads();
@@ -1683,7 +1701,7 @@ __global__ __launch_bounds__(256) void k_sq_shells(const double* __restrict__ rh
       else
         atomicAdd(&sqfq[idx * ns + sh], v);
     };
-    shell_vector(nsp, nslice, nch, bsc, norm, fetch, add);
+    shell_vector(nsp, nslice, nch, bsc, norm, qsp_on ? &qsp : nullptr, fetch, add);
   }
   if (use_lds) {
     __syncthreads();
@@ -1748,7 +1766,8 @@ __global__ __launch_bounds__(256) void k_sq_list(const double* __restrict__ u, c
 
 __global__ void k_sq_list_finish(const double* __restrict__ part, const int32_t* __restrict__ shell, int nvec,
                                  int nsp, int nslice, int nch, Bsc bsc, int nqmax, const int32_t* __restrict__ norm,
-                                 double* __restrict__ sqf, double* __restrict__ sqfp, double* __restrict__ sqfq) {
+                                 Bsc qsp, int qsp_on, double* __restrict__ sqf, double* __restrict__ sqfp,
+                                 double* __restrict__ sqfq) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nvec) return;
   const int sh = shell[v], ns = nqmax + 1;
@@ -1769,7 +1788,7 @@ __global__ void k_sq_list_finish(const double* __restrict__ part, const int32_t*
     else
       atomicAdd(&sqfq[idx * ns + sh], val);
   };
-  shell_vector(nsp, nslice, nch, bsc, norm, fetch, add);
+  shell_vector(nsp, nslice, nch, bsc, norm, qsp_on ? &qsp : nullptr, fetch, add);
 }
 
 // ------------------------------------------------------------------ batches
@@ -2274,9 +2293,11 @@ void launch_qq_finish(unsigned long long* gq, int lsmax, const double* qp, doubl
   hipLaunchKernelGGL(k_qq_finish, dim3(grid1d(lsmax, 256)), dim3(256), 0, s, gq, lsmax, qp, acc);
 }
 
-void launch_rdf_merge(const uint32_t* partial, int nslices, int nbins, int64_t* counts, hipStream_t s) {
+void launch_rdf_merge(const uint32_t* partial, int nslices, int nbins, int64_t* counts, hipStream_t s,
+                      const QqWeights* qw, int lsmax, unsigned long long* gq) {
+  const QqWeights w = qw ? *qw : QqWeights{};
   hipLaunchKernelGGL(k_rdf_merge, dim3(grid1d(nbins, 256), kMergeGroups), dim3(256), 0, s, partial, nslices,
-                     nbins, counts);
+                     nbins, counts, w, lsmax > 0 ? lsmax : 1, qw ? gq : nullptr);
 }
 
 void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s) {
@@ -2296,20 +2317,22 @@ void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s) 
 }
 
 void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, int tile_vec, int ksplit,
-                      int nsp, int nslice, int nch, const double* bsc, int nqmax, const int32_t* norm, double* sqf,
-                      double* sqfp, double* sqfq, hipStream_t s) {
+                      int nsp, int nslice, int nch, const double* bsc, int nqmax, const int32_t* norm,
+                      const double* qsp, double* sqf, double* sqfp, double* sqfq, hipStream_t s) {
   if (ntiles <= 0) return;
-  Bsc b{};
+  Bsc b{}, q{};
   for (int i = 0; i < nsp; ++i) b.b[i] = bsc[i];
-  const int nacc = nslice * (1 + nsp) + (nch > nslice * nsp ? nslice : 0);
+  if (qsp)
+    for (int i = 0; i < nsp; ++i) q.b[i] = qsp[i];
+  const int nacc = nslice * (1 + nsp) + ((nch > nslice * nsp || qsp) ? nslice : 0);
   const size_t lds = static_cast<size_t>(nacc) * (nqmax + 1) * sizeof(double);
   const int use_lds = lds <= 48 * 1024;
   const int64_t total = static_cast<int64_t>(ntiles) * tile_vec;
   int64_t g = (total + 255) / 256;
   if (g > 1024) g = 1024;
   hipLaunchKernelGGL(k_sq_shells, dim3(static_cast<unsigned>(g)), dim3(256), use_lds ? lds : 0, s, rho,
-                     tile_shell, ntiles, tile_vec, ksplit, nsp, nslice, nch, b, nqmax, norm, sqf, sqfp, sqfq,
-                     use_lds);
+                     tile_shell, ntiles, tile_vec, ksplit, nsp, nslice, nch, b, nqmax, norm, q, qsp ? 1 : 0, sqf,
+                     sqfp, sqfq, use_lds);
 }
 
 size_t sq_list_scratch_bytes(int nvec, int nch) {
@@ -2318,14 +2341,17 @@ size_t sq_list_scratch_bytes(int nvec, int nch) {
 
 void launch_sq_list(const double* u, const double* w, const int32_t* sp_off, int64_t stride, int nsp, int nslice,
                     int nch, int ndim, const int32_t* kv, const int32_t* shell, int nvec, const double* bsc,
-                    int nqmax, bool norm, double* part, double* sqf, double* sqfp, double* sqfq, hipStream_t s) {
+                    int nqmax, bool norm, const double* qsp, double* part, double* sqf, double* sqfp, double* sqfq,
+                    hipStream_t s) {
   if (nvec <= 0) return;
-  Bsc b{};
+  Bsc b{}, q{};
   for (int i = 0; i < nsp; ++i) b.b[i] = bsc[i];
+  if (qsp)
+    for (int i = 0; i < nsp; ++i) q.b[i] = qsp[i];
   hipLaunchKernelGGL(k_sq_list, dim3(nvec, kListBlocks), dim3(256), 0, s, u, w, sp_off, stride, nch, ndim, kv,
                      part);
   hipLaunchKernelGGL(k_sq_list_finish, dim3((nvec + 63) / 64), dim3(64), 0, s, part, shell, nvec, nsp, nslice,
-                     nch, b, nqmax, norm ? sp_off : nullptr, sqf, sqfp, sqfq);
+                     nch, b, nqmax, norm ? sp_off : nullptr, q, qsp ? 1 : 0, sqf, sqfp, sqfq);
 }
 
 int64_t launch_lds_atomics(int nblocks, int iters, uint32_t* out, hipStream_t s) {

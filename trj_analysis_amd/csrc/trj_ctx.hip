@@ -57,6 +57,7 @@ struct trj_ctx {
   DevBuf qparm;
   bool rdf_charge = false, sq_charge = false;
   bool sq_centres = false;  // TRJ_SQ_CENTRES: per-frame 1/N, octant set only (sqfnDcl)
+  int sq_nch = 1;           // channels of the last frame's phase sums (rho layout)
   // Confined slices (twoDstruc_3D; off when slices.n == 0): per-atom keys,
   // the species-sorted last atom, per-slice counts and q^2 sums of the frame
   // (and of the last S(Q) frame), the slice pair histograms (frame, raw
@@ -355,8 +356,14 @@ int fqt_step(trj_ctx* c, int64_t n, const int32_t* itype_dev) {
 // The frame pipeline, device inputs.
 // q_dev: the frame's charges on the device (NULL without); the caller has
 // queued the frame's {scale, avq2} into qparm[0..1] on the context stream.
+// qspecies (host, nsp values, may be NULL): the charges when they are constant
+// within each species; S(Q) then forms the charge sums from the species
+// channels instead of contracting a charge channel, and the RDF takes its
+// fixed-point charge sums from the species histogram (qscale_host: the
+// frame's scale).
 int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int32_t* itype_dev, int64_t n,
-              const double box_in[3], uint32_t what, const double* q_dev = nullptr) {
+              const double box_in[3], uint32_t what, const double* q_dev = nullptr,
+              const double* qspecies = nullptr, double qscale_host = 0.0) {
   const bool do_rdf = (what & TRJ_WANT_RDF) != 0;
   const bool do_sq = (what & TRJ_WANT_SQ) != 0;
   const bool do_fqt = (what & TRJ_WANT_FQT) != 0;
@@ -365,6 +372,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
   if (do_fqt && !c->fqt_ready) return fail(c, TRJ_ESTATE, "frame asks for F(Q,t) before trj_fqt_init");
   if (!do_rdf && !do_sq && !do_fqt) return fail(c, TRJ_EINVAL, "frame asks for nothing");
   const bool rdf_q = do_rdf && c->rdf_charge, sq_q = do_sq && c->sq_charge;
+  const bool sq_ch = sq_q && !qspecies;  // a charge channel in the contraction
   if ((rdf_q || sq_q) && !q_dev)
     return fail(c, TRJ_EINVAL, "charges are enabled (TRJ_RDF_CHARGE / TRJ_SQ_CHARGE): frames go through trj_frame_q");
   const int ndim = do_rdf ? c->rdf.ndim : (do_sq ? c->sq.p.ndim : c->fqt_ndim);
@@ -379,7 +387,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
   if (n > (int64_t(1) << 31) - 1024) return fail(c, TRJ_ELIMIT, "n above 2^31 is not supported");
   // The S(Q) kernels address the turn planes with 32-bit offsets up to 3 x the
   // plane stride (n, or 2n with the charge channel).
-  if (do_sq && n * (sq_q ? 2 : 1) > int64_t(0xFFFFFFFFu) / 3)
+  if (do_sq && n * (sq_ch ? 2 : 1) > int64_t(0xFFFFFFFFu) / 3)
     return fail(c, TRJ_ELIMIT, "S(Q) with n above 1431655765 (715827882 with charges) is not supported");
   if (dtype != TRJ_F32 && dtype != TRJ_F64) return fail(c, TRJ_EINVAL, "bad dtype");
   if (layout != TRJ_AOS && layout != TRJ_SOA) return fail(c, TRJ_EINVAL, "bad layout");
@@ -438,7 +446,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
   // Sort keys: the species, or slice * nsp + species with slices; the charge
   // channels repeat each slice's (or all) sorted entries.
   const int nkeys = nsl > 0 ? nsl * nsp : nsp;
-  const int ngroups = sq_q ? (nsl > 0 ? nsl : 1) : 0;
+  const int ngroups = sq_ch ? (nsl > 0 ? nsl : 1) : 0;
   const int32_t* keys = nsl > 0 ? P<int32_t>(c->slkey) : nullptr;
   const int nblk = static_cast<int>((n + 255) / 256);
   if ((r = ensure(c, c->blk, sizeof(int32_t) * nkeys * nblk))) return r;
@@ -446,7 +454,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
   if ((r = ensure(c, c->sp_off, sizeof(int32_t) * (nkeys + 2 + nsl)))) return r;
   // Plane stride of u: n, or 2n with charge channels (the sorted entries
   // again, weighted by their charges).
-  const int64_t ustride = sq_q ? 2 * n : n;
+  const int64_t ustride = sq_ch ? 2 * n : n;
   launch_species_count(itype_dev, keys, n, nkeys, P<int32_t>(c->blk), nblk, c->stream);
   launch_exclusive_scan(P<int32_t>(c->blk), P<int32_t>(c->blkoff), static_cast<int64_t>(nkeys) * nblk, c->stream);
   launch_species_offsets(P<int32_t>(c->blkoff), P<int32_t>(c->blk), nblk, nkeys, nsp, ngroups,
@@ -454,10 +462,10 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
   if (do_sq) {
     // Key-contiguous turns u = x / Lq for the contraction (Lq: the box the
     // q-grid was built on, sq.cuf:189); slices use x and y only.
-    if ((r = ensure(c, c->u, sizeof(double) * (sq_q ? 13 : 12) * ustride))) return r;
+    if ((r = ensure(c, c->u, sizeof(double) * (sq_ch ? 13 : 12) * ustride))) return r;
     const double lq[3] = {c->sq.box[0], c->sq.box[1], c->sq.box[2]};
     launch_species_scatter(itype_dev, keys, n, nkeys, P<int32_t>(c->blkoff), nblk, P<double>(c->pos), lq,
-                           nsl > 0 ? 2 : ndim, sq_q ? q_dev : nullptr, P<int32_t>(c->sp_off), ustride,
+                           nsl > 0 ? 2 : ndim, sq_ch ? q_dev : nullptr, P<int32_t>(c->sp_off), ustride,
                            P<double>(c->u), c->stream);
   }
   HIPC(c, hipGetLastError());
@@ -486,6 +494,16 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     pa.side2 = g.side2;
     pa.inv_dr = static_cast<float>(1.0 / R.deltar);
     pa.thr = P<double>(c->thr);
+    // Species charges: the merge adds the fixed-point sums from the species
+    // histogram; the pair kernels then run without their charge path.
+    const bool qq_species = rdf_q && qspecies && nsl == 0;
+    QqWeights qw{};
+    if (qq_species) {
+      int ij = 0;
+      for (int i = 0; i < nsp; ++i)
+        for (int j = i; j < nsp; ++j, ++ij)
+          qw.w[ij] = static_cast<long long>(std::rint(qspecies[i] * qspecies[j] * qscale_host));
+    }
     if (rdf_q) {
       pa.gq = P<unsigned long long>(c->gqf);
       pa.qp = P<double>(c->qparm);
@@ -528,11 +546,12 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
         launch_cell_count(P<double>(c->pos), n, ndim, cg, P<int32_t>(c->cellid), P<int32_t>(c->cellcount),
                           rs);
         launch_exclusive_scan(P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), ncell + 1, rs);
-        if (rdf_q && (r = ensure(c, c->qsort, sizeof(double) * n))) return r;
+        const bool qq_pairs = rdf_q && !qq_species;
+        if (qq_pairs && (r = ensure(c, c->qsort, sizeof(double) * n))) return r;
         launch_cell_scatter(P<double>(c->pos), itype_dev, P<int32_t>(c->cellid), n, P<int32_t>(c->cellstart),
                             P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype),
-                            rdf_q ? q_dev : nullptr, rdf_q ? P<double>(c->qsort) : nullptr, rs);
-        if (rdf_q) pa.q = P<double>(c->qsort);
+                            qq_pairs ? q_dev : nullptr, qq_pairs ? P<double>(c->qsort) : nullptr, rs);
+        if (qq_pairs) pa.q = P<double>(c->qsort);
         const double mean = static_cast<double>(n) / static_cast<double>(ncell);
         int W = static_cast<int>(std::lround(mean / 64.0));
         W = std::max(1, std::min(W, 8));
@@ -545,10 +564,11 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
         const int64_t nb = (n + 63) / 64;
         nblocks = static_cast<int>(std::min<int64_t>((nb * nb + 3) / 4, 1024));
         if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
-        if (rdf_q) pa.q = q_dev;  // input order, like pos and itype here
+        if (rdf_q && !qq_species) pa.q = q_dev;  // input order, like pos and itype here
         launch_rdf_allpairs(pa, P<double>(c->pos), itype_dev, n, nblocks, P<uint32_t>(c->partial), rs);
       }
-      launch_rdf_merge(P<uint32_t>(c->partial), nblocks, nbins, P<int64_t>(c->counts), rs);
+      launch_rdf_merge(P<uint32_t>(c->partial), nblocks, nbins, P<int64_t>(c->counts), rs, qq_species ? &qw : nullptr,
+                       R.lsmax, P<unsigned long long>(c->gqf));
       if (rdf_q) launch_qq_finish(P<unsigned long long>(c->gqf), R.lsmax, P<double>(c->qparm), P<double>(c->gqacc), rs);
       if (c->nrand > 0) {
         // Number fluctuations around the fixed random points (rdf.cuf:243-245,
@@ -591,6 +611,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     SqPlan& S = c->sq;
     const int ntiles = static_cast<int>(S.tiles.size());
     const int nch = nkeys + ngroups;  // (slice,) species channels, then charge channels
+    c->sq_nch = nch;
     const int sdim = nsl > 0 ? 2 : ndim;
     if (ntiles > 0) {
       const int ks = contraction_splits(ntiles, n, S.p.precision, nch);
@@ -599,7 +620,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
       if ((r = ensure(c, c->rho, rho_bytes))) return r;
       ContractArgs ca{};
       ca.u = P<double>(c->u);
-      ca.w = sq_q ? P<double>(c->u) + 12 * ustride : nullptr;
+      ca.w = sq_ch ? P<double>(c->u) + 12 * ustride : nullptr;
       ca.sp_off = P<int32_t>(c->sp_off);
       ca.tiles = P<int32_t>(c->tiles);
       ca.ntiles = ntiles;
@@ -622,13 +643,14 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     const int ngs = nsl > 0 ? nsl : 1;
     if (ntiles > 0)
       launch_sq_shells(P<double>(c->rho), P<int16_t>(c->tile_shell), ntiles, S.tk.vec(), c->ksplit, nsp, ngs, nch,
-                       S.p.bsc, S.nqmax, norm, P<double>(c->sqf), P<double>(c->sqfp), P<double>(c->sqfq), c->stream);
+                       S.p.bsc, S.nqmax, norm, sq_q ? qspecies : nullptr, P<double>(c->sqf), P<double>(c->sqfp),
+                       P<double>(c->sqfq), c->stream);
     const int nax = static_cast<int>(S.axis_shell.size());
     if ((r = ensure(c, c->listpart, sq_list_scratch_bytes(nax, nch)))) return r;
-    launch_sq_list(P<double>(c->u), sq_q ? P<double>(c->u) + 12 * ustride : nullptr, P<int32_t>(c->sp_off), ustride,
+    launch_sq_list(P<double>(c->u), sq_ch ? P<double>(c->u) + 12 * ustride : nullptr, P<int32_t>(c->sp_off), ustride,
                    nsp, ngs, nch, sdim, P<int32_t>(c->axis_kv), P<int32_t>(c->axis_shell), nax, S.p.bsc, S.nqmax,
-                   nsl > 0 || c->sq_centres, P<double>(c->listpart), P<double>(c->sqf), P<double>(c->sqfp), P<double>(c->sqfq),
-                   c->stream);
+                   nsl > 0 || c->sq_centres, sq_q ? qspecies : nullptr, P<double>(c->listpart), P<double>(c->sqf),
+                   P<double>(c->sqfp), P<double>(c->sqfq), c->stream);
     // sq.cuf:143-145 divides by avq2 of the last frame: keep it in qparm[2];
     // with slices, by each slice's mean q^2 of the last frame (sq.cuf:135-138).
     if (sq_q && nsl == 0)
@@ -1003,6 +1025,12 @@ int frame_host(trj_ctx* c, const void* pos, int32_t dtype, int32_t layout, const
   // Fixed-point scale of q_i q_j (2^(30-e), max q^2 < 2^e) and avq2 =
   // sum(q^2)/n, both from the host copy in index order.
   double qscale = 1.0, avq2 = 0.0;
+  // Charges constant within each species (per-type charges): S(Q) then needs
+  // no charge channel, rho_q = sum_s q_s rho_s.
+  const int nspf = (what & TRJ_WANT_RDF) && c->rdf_ready ? c->rdf.nsp : (c->sq_ready ? c->sq.p.nsp : 0);
+  double qsp[kNspMax];
+  bool by_species = charge != nullptr && nspf > 0;
+  bool seen[kNspMax] = {false};
   if (charge) {
     double m = 0.0, s2 = 0.0;
     for (int64_t i = 0; i < n; ++i) {
@@ -1010,7 +1038,21 @@ int frame_host(trj_ctx* c, const void* pos, int32_t dtype, int32_t layout, const
       if (!std::isfinite(v)) return fail(c, TRJ_EINVAL, "non-finite charge");
       if (v > m) m = v;
       s2 += v;
+      if (by_species) {
+        const int t = itype[i] - 1;
+        if (t < 0 || t >= nspf) {
+          by_species = false;
+        } else if (!seen[t]) {
+          seen[t] = true;
+          qsp[t] = charge[i];
+        } else if (charge[i] != qsp[t]) {
+          by_species = false;
+        }
+      }
     }
+    if (by_species)
+      for (int t = 0; t < nspf; ++t)
+        if (!seen[t]) qsp[t] = 0.0;
     int e = 0;
     if (m > 0.0) std::frexp(m, &e);
     qscale = std::ldexp(1.0, 30 - e);
@@ -1047,7 +1089,7 @@ int frame_host(trj_ctx* c, const void* pos, int32_t dtype, int32_t layout, const
   }
   HIPC(c, hipEventRecord(c->pinned_done[k], c->stream));
   return run_frame(c, c->raw.p, dtype, layout, P<int32_t>(c->itype), n, box, what,
-                   charge ? P<double>(c->qraw) : nullptr);
+                   charge ? P<double>(c->qraw) : nullptr, by_species ? qsp : nullptr, qscale);
 }
 
 }  // namespace
@@ -1325,7 +1367,7 @@ int trj_sq_rho(trj_ctx* c, int64_t nvec, const int32_t* kv, double* rho) {
   if (r) return r;
   const SqPlan& S = c->sq;
   const int nsp = S.p.nsp, ntiles = static_cast<int>(S.tiles.size());
-  const int nch = nsp + (c->sq_charge ? 1 : 0);  // channels of the phase sums
+  const int nch = c->sq_nch;  // channels of the last frame's phase sums
   // Many vectors: one copy of the frame's phase sums instead of one per value.
   std::vector<double> all;
   if (nvec > 16) {
