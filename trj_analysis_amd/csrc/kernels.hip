@@ -1366,6 +1366,26 @@ __device__ inline void split8(const f8& x, V& hi, V& lo) {
   const f8 back = __builtin_convertvector(hi, f8);
   lo = __builtin_convertvector(x - back, V);
 }
+// FP16: lo = round(x - hi) in one v_fma_mix{lo,hi}_f16 per value (x - hi is
+// exact in FP32 and rounded once to FP16, as the generic form does with a
+// widening convert, a subtraction and a narrowing convert).
+__device__ inline uint32_t lo_pair_f16(float x0, float x1, uint32_t h) {
+  uint32_t d;
+  asm("v_fma_mixlo_f16 %0, %1, 1.0, -%2 op_sel_hi:[0,0,1]\n\t"
+      "v_fma_mixhi_f16 %0, %3, 1.0, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]"
+      : "=&v"(d)
+      : "v"(x0), "v"(h), "v"(x1));
+  return d;
+}
+template <>
+__device__ inline void split8<h8>(const f8& x, h8& hi, h8& lo) {
+  hi = __builtin_convertvector(x, h8);
+  uint32_t ph[4], pl[4];
+  __builtin_memcpy(ph, &hi, sizeof(ph));
+#pragma unroll
+  for (int k = 0; k < 4; ++k) pl[k] = lo_pair_f16(x[2 * k], x[2 * k + 1], ph[k]);
+  __builtin_memcpy(&lo, pl, sizeof(pl));
+}
 // BF16: the FP32 value of a packed pair's halves is a bit shift or mask, so hi
 // is widened by bit operations instead of a second rounding per element.
 template <>
@@ -1410,200 +1430,255 @@ __device__ inline float2 cis_turns_f32(double f) {
   return make_float2(((qi + 1) & 2) ? -re : re, (qi & 2) ? -im : im);
 }
 
-// e^{2 pi i n u} for n = n0 .. n0+15, times m (1, or 0 for an atom past the
-// end), into dst[k * kRS] as FP32 complex: the anchor from the FP64-reduced
-// turns, then steps by the FP64 unit step rounded to FP32 (16 steps add at
-// most ~2e-6 relative error, below the 16-bit operands' own).
-constexpr int kRS = 34;  // row stride of the value-major records (float2): 32 atoms + 2
-__device__ inline void phase_rec16(double uh, double ul, double c, double s, int n0, float m, float2* dst) {
-  const float2 an = cis_turns_f32(frac_turns(n0, uh, ul));
-  const float sc = static_cast<float>(c), ss = static_cast<float>(s);
-  float re = an.x * m, im = an.y * m;
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    dst[k * kRS] = make_float2(re, im);
-    const float nr = fmaf(re, sc, -(im * ss));
-    const float ni = fmaf(re, ss, im * sc);
-    re = nr;
-    im = ni;
-  }
-}
+// Phase records of the 16-bit kernel: e^{2 pi i n u} in FP32 complex, value-
+// major (row = one kx, kz or ky value, column = atom), row stride kRS float2
+// (32 atoms + 2: lane-per-atom writes and the operand reads are free of bank
+// conflicts).  Each segment is an anchor from the FP64-reduced turns and then
+// steps by the FP64 unit step rounded to FP32 (8 steps add at most ~1e-6
+// relative error, below the 16-bit operands' own).
+constexpr int kRS = 34;
 
+// 16-bit contraction (FP16 / BF16 hi+lo splits, three products per real
+// product) on tiles of 64 kx x 8 ky x 32 kz with eight wavefronts, two per
+// SIMD.  Wavefront w owns the 32 columns of ky = ky0 + w against all 64 rows:
+// it forms and splits its B operand ey * ez in registers, and only the A
+// operand (ex, split once per workgroup) goes through LDS, 16 KB per
+// wavefront and chunk (a shared B would be 32 KB).  Per 32-atom chunk i:
+//   S1(i): the phases of the chunk into rec[i % 3]: 13 segments of 8 values
+//          per atom (x 8, z 4, y 1; lanes past 416 repeat segments 0-2,
+//          writing the same values to the same places);
+//   S2(i): A split into As[i % 2] (4 atoms per lane);
+//   S3(i): B of the wavefront's columns into registers, then 96 MFMAs
+//          (v_mfma_f32_16x16x32_{f16,bf16}; the -Im*Im products in their own
+//          accumulators, no operand negation).
+// One loop step runs S3(i), S2(i+1) and S1(i+2); one barrier per chunk orders
+// the LDS buffers (three phase buffers, two A buffers).  The turns of a chunk
+// are loaded two chunks before they are staged.  FP32 accumulation runs over
+// at most 8192 atoms per workgroup and channel (the K-splits are summed in
+// FP64).
+constexpr int WX = 64, WY = 8, WZ = 32, WN = WY * WZ, WROWS = 104;
 template <int PREC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sq_contract_lp16(
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k_sq_contract_lp16w(
     ContractArgs a) {
   typedef LpTypes<PREC> LT;
   typedef typename LT::V V;
-  __shared__ __attribute__((aligned(16))) float2 rec[112][kRS];  // value-major: x 0..63, z 64..95, y 96..111
-  __shared__ __attribute__((aligned(16))) V B[4][LKC / 8][LN];  // re_hi, re_lo, im_hi, im_lo of ey*ez; 8 atoms per entry
+  __shared__ __attribute__((aligned(16))) float2 rec[3][WROWS][kRS];  // x 0..63, z 64..95, y 96..103
+  __shared__ __attribute__((aligned(16))) V As[2][4][LKC / 8][WX];     // re_hi, re_lo, im_hi, im_lo of ex
   const int tile = blockIdx.x % a.ntiles;
   const int split = blockIdx.x / a.ntiles;
   const int kx0 = a.tiles[4 * tile], ky0 = a.tiles[4 * tile + 1], kz0 = a.tiles[4 * tile + 2];
   const int t = threadIdx.x, w = t / kWave, lane = t & (kWave - 1);
   const int i16 = lane & 15, kq = lane >> 4;
-  // Staging: one 16-value segment per lane, same code for every lane.
-  // t < 128: ex segment t/32 of atom t%32; t < 192: ez segment (t-128)/32;
-  // t >= 192: ey (16 values from ky0, 4 used; lanes 224-255 repeat 192-223).
+  // Staging: segment sg of atom sa.
   const int sa = t % LKC;
-  const int axis = t < 128 ? 0 : (t < 192 ? 2 : 1);
-  const int seg = t < 128 ? t / 32 : (t < 192 ? (t - 128) / 32 : 0);
-  const int sn0 = (axis == 0 ? kx0 : (axis == 2 ? kz0 : ky0)) + 16 * seg;
-  float2* const dst = &rec[axis == 0 ? 16 * seg : (axis == 2 ? 64 + 16 * seg : 96)][sa];
+  const int sg = (t / LKC) % 13;
+  const int axis = sg < 8 ? 0 : (sg < 12 ? 2 : 1);
+  const int sn0 = sg < 8 ? kx0 + 8 * sg : (sg < 12 ? kz0 + 8 * (sg - 8) : ky0);
+  const int drow = sg < 8 ? 8 * sg : (sg < 12 ? 64 + 8 * (sg - 8) : 96);
   const uint32_t n32 = static_cast<uint32_t>(a.n);
   const double* u_hi = a.u + axis * n32;
   const double* u_lo = a.u + (3 + axis) * n32;
   const double* u_c = a.u + (6 + axis) * n32;
   const double* u_s = a.u + (9 + axis) * n32;
-  // Products: column n = (ky, kz), atoms 16*ph .. 16*ph+15.
-  const int pn = t & (LN - 1), ph = t >> 7;
-  const int pky = pn / LZ, pkz = pn % LZ;
+  const double* wsrc = a.w ? a.w : u_hi;
+  // A split: row ar, atoms 4 aq .. 4 aq + 3 (half of one 8-atom entry).
+  const int ar = t & 63, aq = t >> 6;
   for (int s = 0; s < a.nch; ++s) {
     const int a0 = a.sp_off[s], a1 = a.sp_off[s + 1];
     const int64_t len = a1 - a0;
     const int b0 = a0 + static_cast<int>(len * split / a.ksplit);
     const int b1 = a0 + static_cast<int>(len * (split + 1) / a.ksplit);
     if (b0 >= b1) {
-      // no atoms of this channel in this split: zero partials
-      const size_t base = ((static_cast<size_t>(split) * a.nch + s) * a.ntiles + tile) * (LX * LN);
-      for (int v = t; v < LX * LN; v += 256) reinterpret_cast<double2*>(a.rho)[base + v] = make_double2(0.0, 0.0);
+      const size_t base = ((static_cast<size_t>(split) * a.nch + s) * a.ntiles + tile) * (WX * WN);
+      for (int v = t; v < WX * WN; v += 512) reinterpret_cast<double2*>(a.rho)[base + v] = make_double2(0.0, 0.0);
       continue;
     }
-    // FP32 accumulation over this workgroup's atoms (at most 8192 per species,
-    // set by the host through ksplit); the K-splits are summed in FP64.  The
-    // -Im*Im terms go to their own accumulators (no operand negation).
-    f4 accR[8], accI[8], accM[8];
-    for (int nb = 0; nb < 8; ++nb) {
-      accR[nb] = f4{0.f, 0.f, 0.f, 0.f};
-      accI[nb] = accR[nb];
-      accM[nb] = accR[nb];
-    }
-    double xh, xl, xc, xs;
-    float xm;
-    auto fetch = [&](int c0) {
+    f4 accR[4][2], accI[4][2], accM[4][2];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        accR[mb][e] = f4{0.f, 0.f, 0.f, 0.f};
+        accI[mb][e] = accR[mb][e];
+        accM[mb][e] = accR[mb][e];
+      }
+    // Turns of one chunk, loaded two chunks before they are staged.
+    struct Turns {
+      double h, l, c, s, w;
+      float fw, f1;
+    } F0, F1;
+    auto fetch = [&](Turns& f, int c0) {
       const int ai = c0 + sa;
       const uint32_t o = static_cast<uint32_t>(ai < b1 ? ai : b1 - 1);
-      xh = u_hi[o];
-      xl = u_lo[o];
-      xc = u_c[o];
-      xs = u_s[o];
-      // ex carries the channel weight (1, or the charge of the entry); zero
-      // past the end of the range.
-      xm = axis != 0 ? 1.f : (ai < b1 ? (a.w ? static_cast<float>(a.w[o]) : 1.f) : 0.f);
+      f.h = u_hi[o];
+      f.l = u_lo[o];
+      f.c = u_c[o];
+      f.s = u_s[o];
+      // The weight is loaded and used unconditionally (from the turns when
+      // there are none; times 0 then) so that no branch splits the loop body:
+      // the factor is w (ex, weights), 1 (ex without weights, ey, ez), 0 (past b1).
+      f.w = wsrc[o];
+      const bool xin = axis == 0 && ai < b1;
+      f.fw = (xin && a.w) ? 1.f : 0.f;
+      f.f1 = (axis != 0 || (xin && !a.w)) ? 1.f : 0.f;
     };
-    __syncthreads();  // the previous species' readers are done with rec and B
-    fetch(b0);
-    phase_rec16(xh, xl, xc, xs, sn0, xm, dst);
-    fetch(b0 + LKC);
-    __syncthreads();
-    for (int c0 = b0; c0 < b1; c0 += LKC) {
-      // B = ey*ez for column pn and 16 atoms, split, 8 atoms per 16-byte store.
+    // Phases of one segment in pieces: the anchor, then steps k0 .. k1-1
+    // (the rotation state carries over between pieces).
+    float pre, pim, psc, pss;
+    auto stage_begin = [&](const Turns& f) {
+      const float2 an = cis_turns_f32(frac_turns(sn0, f.h, f.l));
+      const float m = fmaf(static_cast<float>(f.w), f.fw, f.f1);
+      psc = static_cast<float>(f.c);
+      pss = static_cast<float>(f.s);
+      pre = an.x * m;
+      pim = an.y * m;
+    };
+    auto stage_steps = [&](int buf, int k0, int k1) {
+      float2* dst = &rec[buf][drow][sa];
 #pragma unroll
-      for (int g = 0; g < 2; ++g) {
-        f8 re, im;
-        const float4* yr = reinterpret_cast<const float4*>(&rec[96 + pky][16 * ph + 8 * g]);
-        const float4* zr = reinterpret_cast<const float4*>(&rec[64 + pkz][16 * ph + 8 * g]);
-        float4 yv[4], zv[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          yv[k] = yr[k];
-          zv[k] = zr[k];
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          re[2 * k] = fmaf(yv[k].x, zv[k].x, -(yv[k].y * zv[k].y));
-          im[2 * k] = fmaf(yv[k].x, zv[k].y, yv[k].y * zv[k].x);
-          re[2 * k + 1] = fmaf(yv[k].z, zv[k].z, -(yv[k].w * zv[k].w));
-          im[2 * k + 1] = fmaf(yv[k].z, zv[k].w, yv[k].w * zv[k].z);
-        }
-        V rh, rl, ih, il;
-        split8(re, rh, rl);
-        split8(im, ih, il);
-        const int oc = 2 * ph + g;
-        B[0][oc][pn] = rh;
-        B[1][oc][pn] = rl;
-        B[2][oc][pn] = ih;
-        B[3][oc][pn] = il;
+      for (int k = k0; k < k1; ++k) {
+        dst[k * kRS] = make_float2(pre, pim);
+        const float nr = fmaf(pre, psc, -(pim * pss));
+        const float ni = fmaf(pre, pss, pim * psc);
+        pre = nr;
+        pim = ni;
       }
-      // A = ex for this wavefront's row 16w + i16, atoms 8kq .. 8kq+7.
-      V arh, arl, aih, ail;
+    };
+    auto split_a = [&](int rb, int buf) {
+      const float4* er = reinterpret_cast<const float4*>(&rec[rb][ar][4 * aq]);
+      const float4 e0 = er[0], e1 = er[1];
+      // re of the 4 atoms in halves 0-3, im in halves 4-7
+      const f8 x = {e0.x, e0.z, e1.x, e1.z, e0.y, e0.w, e1.y, e1.w};
+      V hi, lo;
+      split8(x, hi, lo);
+      typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+      const uint4 ph = __builtin_bit_cast(uint4, hi), pl = __builtin_bit_cast(uint4, lo);
+      u2* dst = reinterpret_cast<u2*>(&As[buf][0][aq >> 1][ar]) + (aq & 1);
+      constexpr int kPart = (LKC / 8) * WX * 2;  // one part, in 8-byte units
+      dst[0] = u2{ph.x, ph.y};
+      dst[kPart] = u2{pl.x, pl.y};
+      dst[2 * kPart] = u2{ph.z, ph.w};
+      dst[3 * kPart] = u2{pl.z, pl.w};
+    };
+    auto gen_b = [&](int buf, V (&B)[2][4]) {
+      const float4* yr = reinterpret_cast<const float4*>(&rec[buf][96 + w][8 * kq]);
+      float4 yv[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) yv[k] = yr[k];
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const float4* zr = reinterpret_cast<const float4*>(&rec[buf][64 + 16 * e + i16][8 * kq]);
+        f8 re, im;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const float4 zv = zr[k];
+          re[2 * k] = fmaf(yv[k].x, zv.x, -(yv[k].y * zv.y));
+          im[2 * k] = fmaf(yv[k].x, zv.y, yv[k].y * zv.x);
+          re[2 * k + 1] = fmaf(yv[k].z, zv.z, -(yv[k].w * zv.w));
+          im[2 * k + 1] = fmaf(yv[k].z, zv.w, yv[k].w * zv.z);
+        }
+        split8(re, B[e][0], B[e][1]);
+        split8(im, B[e][2], B[e][3]);
+      }
+    };
+    auto mma = [&](int buf, const V (&B)[2][4], int mb) {
       {
-        f8 re, im;
-        const float4* er = reinterpret_cast<const float4*>(&rec[16 * w + i16][8 * kq]);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const float4 e = er[k];
-          re[2 * k] = e.x;
-          im[2 * k] = e.y;
-          re[2 * k + 1] = e.z;
-          im[2 * k + 1] = e.w;
-        }
-        split8(re, arh, arl);
-        split8(im, aih, ail);
+        const V arh = As[buf][0][kq][16 * mb + i16];
+        const V arl = As[buf][1][kq][16 * mb + i16];
+        const V aih = As[buf][2][kq][16 * mb + i16];
+        const V ail = As[buf][3][kq][16 * mb + i16];
+        f4 r0 = accR[mb][0], i0 = accI[mb][0], r1 = accR[mb][1], i1 = accI[mb][1];
+        f4 m0 = accM[mb][0], m1 = accM[mb][1];
+        r0 = LT::mfma(arh, B[0][0], r0);
+        i0 = LT::mfma(arh, B[0][2], i0);
+        r1 = LT::mfma(arh, B[1][0], r1);
+        i1 = LT::mfma(arh, B[1][2], i1);
+        r0 = LT::mfma(arh, B[0][1], r0);
+        i0 = LT::mfma(arh, B[0][3], i0);
+        r1 = LT::mfma(arh, B[1][1], r1);
+        i1 = LT::mfma(arh, B[1][3], i1);
+        r0 = LT::mfma(arl, B[0][0], r0);
+        i0 = LT::mfma(arl, B[0][2], i0);
+        r1 = LT::mfma(arl, B[1][0], r1);
+        i1 = LT::mfma(arl, B[1][2], i1);
+        m0 = LT::mfma(aih, B[0][2], m0);
+        i0 = LT::mfma(aih, B[0][0], i0);
+        m1 = LT::mfma(aih, B[1][2], m1);
+        i1 = LT::mfma(aih, B[1][0], i1);
+        m0 = LT::mfma(aih, B[0][3], m0);
+        i0 = LT::mfma(aih, B[0][1], i0);
+        m1 = LT::mfma(aih, B[1][3], m1);
+        i1 = LT::mfma(aih, B[1][1], i1);
+        m0 = LT::mfma(ail, B[0][2], m0);
+        i0 = LT::mfma(ail, B[0][0], i0);
+        m1 = LT::mfma(ail, B[1][2], m1);
+        i1 = LT::mfma(ail, B[1][0], i1);
+        accR[mb][0] = r0;
+        accI[mb][0] = i0;
+        accR[mb][1] = r1;
+        accI[mb][1] = i1;
+        accM[mb][0] = m0;
+        accM[mb][1] = m1;
       }
-      __syncthreads();  // B complete; rec free for the next chunk
-      // Records of the next chunk (zeros past the end), prefetch of the one after.
-      phase_rec16(xh, xl, xc, xs, sn0, xm, dst);
-      fetch(c0 + 2 * LKC);
+    };
+    V B0[2][4];
+    __syncthreads();  // the previous channel's readers are done with rec and As
+    fetch(F0, b0);
+    fetch(F1, b0 + LKC);
+    stage_begin(F0);
+    stage_steps(0, 0, 8);
+    fetch(F0, b0 + 2 * LKC);
+    __syncthreads();
+    split_a(0, 0);
+    stage_begin(F1);
+    stage_steps(1, 0, 8);
+    fetch(F1, b0 + 3 * LKC);
+    __syncthreads();
+    // Chunk c0: phases in rec[rb], A in As[0] on the first half of a trip and
+    // As[1] on the second.  A step reads rec[rb] (B) and rec[rb + 1] (A of the
+    // next chunk) and writes rec[rb + 2] (mod 3), which no wavefront reads
+    // until after the barrier.  Past b1 the stages run on zero-weight atoms.
+    // One step: B and the MFMAs of this chunk, the A split of the next one,
+    // the phases of the one after.
+    auto step = [&](int rbuf, int abuf, int rnext, int anext, int rstage, const Turns& f) {
+      gen_b(rbuf, B0);
 #pragma unroll
-      for (int nb = 0; nb < 8; nb += 2) {
-        V brh[2], brl[2], bih[2], bil[2];
-        for (int e = 0; e < 2; ++e) {
-          const int col = 16 * (nb + e) + i16;
-          brh[e] = B[0][kq][col];
-          brl[e] = B[1][kq][col];
-          bih[e] = B[2][kq][col];
-          bil[e] = B[3][kq][col];
-        }
-        f4 r0 = accR[nb], i0 = accI[nb], r1 = accR[nb + 1], i1 = accI[nb + 1];
-        f4 m0 = accM[nb], m1 = accM[nb + 1];
-        r0 = LT::mfma(arh, brh[0], r0);
-        i0 = LT::mfma(arh, bih[0], i0);
-        r1 = LT::mfma(arh, brh[1], r1);
-        i1 = LT::mfma(arh, bih[1], i1);
-        r0 = LT::mfma(arh, brl[0], r0);
-        i0 = LT::mfma(arh, bil[0], i0);
-        r1 = LT::mfma(arh, brl[1], r1);
-        i1 = LT::mfma(arh, bil[1], i1);
-        r0 = LT::mfma(arl, brh[0], r0);
-        i0 = LT::mfma(arl, bih[0], i0);
-        r1 = LT::mfma(arl, brh[1], r1);
-        i1 = LT::mfma(arl, bih[1], i1);
-        m0 = LT::mfma(aih, bih[0], m0);
-        i0 = LT::mfma(aih, brh[0], i0);
-        m1 = LT::mfma(aih, bih[1], m1);
-        i1 = LT::mfma(aih, brh[1], i1);
-        m0 = LT::mfma(aih, bil[0], m0);
-        i0 = LT::mfma(aih, brl[0], i0);
-        m1 = LT::mfma(aih, bil[1], m1);
-        i1 = LT::mfma(aih, brl[1], i1);
-        m0 = LT::mfma(ail, bih[0], m0);
-        i0 = LT::mfma(ail, brh[0], i0);
-        m1 = LT::mfma(ail, bih[1], m1);
-        i1 = LT::mfma(ail, brh[1], i1);
-        accR[nb] = r0;
-        accI[nb] = i0;
-        accR[nb + 1] = r1;
-        accI[nb + 1] = i1;
-        accM[nb] = m0;
-        accM[nb + 1] = m1;
-      }
-      __syncthreads();  // rec of the next chunk complete; B free
+      for (int mb = 0; mb < 4; ++mb) mma(abuf, B0, mb);
+      split_a(rnext, anext);
+      stage_begin(f);
+      stage_steps(rstage, 0, 8);
+    };
+    int rb = 0;
+    for (int c0 = b0;;) {
+      const int r1 = rb == 2 ? 0 : rb + 1, r2 = rb == 0 ? 2 : rb - 1;
+      step(rb, 0, r1, 1, r2, F0);
+      fetch(F0, c0 + 4 * LKC);
+      __syncthreads();
+      c0 += LKC;
+      rb = r1;
+      if (c0 >= b1) break;
+      const int q1 = rb == 2 ? 0 : rb + 1, q2 = rb == 0 ? 2 : rb - 1;
+      step(rb, 1, q1, 0, q2, F1);
+      fetch(F1, c0 + 4 * LKC);
+      __syncthreads();
+      c0 += LKC;
+      rb = q1;
+      if (c0 >= b1) break;
     }
-    // D layout of the 16x16 FP32-output MFMAs: register r holds row
-    // 4*(lane/16) + r, column lane%16.
-    const size_t base = ((static_cast<size_t>(split) * a.nch + s) * a.ntiles + tile) * (LX * LN);
+    // D layout: register r holds row 4*(lane/16) + r, column lane%16.
+    const size_t base = ((static_cast<size_t>(split) * a.nch + s) * a.ntiles + tile) * (WX * WN);
 #pragma unroll
-    for (int nb = 0; nb < 8; ++nb) {
+    for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int kxl = 16 * w + 4 * kq + r;
-        const int n = 16 * nb + i16;
-        const int v = (kxl * LY + n / LZ) * LZ + (n % LZ);
-        reinterpret_cast<double2*>(a.rho)[base + v] =
-            make_double2(static_cast<double>(accR[nb][r]) - static_cast<double>(accM[nb][r]),
-                         static_cast<double>(accI[nb][r]));
-      }
-    }
+      for (int e = 0; e < 2; ++e)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int kxl = 16 * mb + 4 * kq + r;
+          const int v = (kxl * WY + w) * WZ + 16 * e + i16;
+          reinterpret_cast<double2*>(a.rho)[base + v] =
+              make_double2(static_cast<double>(accR[mb][e][r]) - static_cast<double>(accM[mb][e][r]),
+                           static_cast<double>(accI[mb][e][r]));
+        }
   }
 }
 
@@ -2311,9 +2386,9 @@ void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s) 
   if (precision == TRJ_PREC_FP32)
     hipLaunchKernelGGL(k_sq_contract_f32, grid, block, 0, s, a);
   else if (precision == TRJ_PREC_FP16X3)
-    hipLaunchKernelGGL(k_sq_contract_lp16<TRJ_PREC_FP16X3>, grid, block, 0, s, a);
+    hipLaunchKernelGGL(k_sq_contract_lp16w<TRJ_PREC_FP16X3>, grid, dim3(512), 0, s, a);
   else
-    hipLaunchKernelGGL(k_sq_contract_lp16<TRJ_PREC_BF16X3>, grid, block, 0, s, a);
+    hipLaunchKernelGGL(k_sq_contract_lp16w<TRJ_PREC_BF16X3>, grid, dim3(512), 0, s, a);
 }
 
 void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, int tile_vec, int ksplit,

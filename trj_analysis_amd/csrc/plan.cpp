@@ -337,7 +337,8 @@ int sq_plan(const trj_sq_params& p_in, const double box[3], SqPlan* out, std::st
                 if (s >= 0) {
                   ++count;
                   P.nq_summed[s] += 1;
-                  blocks |= 1u << (((i * TKY + j) / 16) * (TKZ / 16) + k / 16);
+                  const int bit = ((i * TKY + j) / 16) * (TKZ / 16) + k / 16;
+                  if (bit < 32) blocks |= 1u << bit;  // used by the FP64 tiles (16 blocks)
                 }
               }
           if (count > 0) {

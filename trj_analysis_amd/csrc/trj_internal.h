@@ -56,7 +56,8 @@ struct TileShape {
 // M = 64 kx x N = 128 (ky,kz) pairs, so the phase products ey*ez form the
 // operand shared by all four wavefronts.
 inline TileShape tile_shape(int precision) {
-  return precision == TRJ_PREC_FP64 ? TileShape{8, 8, 32} : TileShape{64, 4, 32};
+  if (precision == TRJ_PREC_FP64) return TileShape{8, 8, 32};
+  return precision == TRJ_PREC_FP32 ? TileShape{64, 4, 32} : TileShape{64, 8, 32};
 }
 
 struct SqTile {
