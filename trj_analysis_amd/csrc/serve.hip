@@ -133,7 +133,11 @@ int64_t trj_request_footprint(const trj_request* r) {
     if (trj_sq_plan(&r->sq, r->box, &info, nullptr) == TRJ_OK) {
       const trj::TileShape tk = trj::tile_shape(r->sq.precision);
       const int64_t tiles = (info.nvec_grid + tk.vec() - 1) / tk.vec() + 1;
-      const int ks = trj::contraction_splits(static_cast<int>(tiles), n, r->sq.precision, r->sq.nsp);
+      int dev = 0, slots = 0;  // the splits depend on the CU count (16-bit)
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&slots, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        slots = 0;
+      const int ks = trj::contraction_splits(static_cast<int>(tiles), n, r->sq.precision, r->sq.nsp, slots);
       b += int64_t(ks) * r->sq.nsp * tiles * tk.vec() * 16 + n * 96;
     }
   }
