@@ -318,7 +318,8 @@ def main():
     # channel and in the charge channel.
     n_entries = 2 * n if args.charges == "atoms" else n
     flops = 8.0 * n_entries * info.nvec_grid
-    kname = "k_sq_contract_f64" if args.precision == "fp64" else f"k_sq_contract_lp<{args.precision}>"
+    kname = {"fp64": "k_sq_contract_f64", "fp32": "k_sq_contract_f32"}.get(args.precision,
+                                                                          f"k_sq_contract_lp16w<{args.precision}>")
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
