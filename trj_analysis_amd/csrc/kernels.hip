@@ -1168,10 +1168,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 }
 
 // ---------------------------------------------------------- reduced precision
-// Tile 64 kx x 4 ky x 32 kz: M = 64 kx (one 16-row block per wavefront),
+// FP32 tile 64 kx x 4 ky x 32 kz: M = 64 kx (one 16-row block per wavefront),
 // N = 128 (ky,kz) pairs (eight 16-column blocks), K = 32 atoms per chunk.
 // The phase products ey*ez are formed once per workgroup (the B operand all
-// four wavefronts share); ex needs no product.
+// four wavefronts share); ex needs no product.  (The 16-bit kernel further
+// down uses 64 x 8 x 32 tiles and forms B per wavefront.)
 //
 // FP32 kernel (v_mfma_f32_16x16x4_f32, MFMA-bound): phases by exact FP64
 // reduction of the anchor n0*u and of the step u, FP32 sincospi, FP32 angle
