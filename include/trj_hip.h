@@ -492,10 +492,15 @@ int trj_batch_autotune(trj_batch* b, int32_t nframes, const trj_batch_frame* fr,
  * Request server: continuous batching of independent analysis requests
  * (RDF and/or S(Q) of one frame each).  Each slot is a context with its own
  * stream and its own worker thread.  A request waits in a FIFO until a slot
- * is free and its estimated device footprint fits in the budget next to the
- * requests already running (a request alone is always admitted); its host
- * preparation and device work then overlap with the other slots'.  Host
- * arrays of a request must stay valid until trj_srv_poll reports it done. */
+ * is free and the device memory it may need fits the HBM budget next to what
+ * every slot holds: a running slot its context's retained buffers plus its
+ * request's footprint, an idle slot its retained buffers.  When the next
+ * request does not fit, idle slots free their buffers.  A request whose
+ * footprint alone exceeds the budget is refused (TRJ_ELIMIT).  Its host
+ * preparation and device work overlap with the other slots'.  Host arrays of
+ * a request must stay valid until trj_srv_poll reports it done.
+ * There is no single counterpart in the reference, which processes one frame
+ * at a time from its driver loop (src/main/trj_analysis.f90:173-242). */
 typedef struct trj_srv trj_srv;
 
 typedef struct {
@@ -512,7 +517,8 @@ typedef struct {
 int trj_srv_create(trj_srv** out, int device, int32_t slots, int64_t hbm_budget_bytes);
 void trj_srv_destroy(trj_srv* s);
 const char* trj_srv_last_error(const trj_srv* s);
-/* Host-only: device bytes a request needs while it runs (admission estimate). */
+/* Host-only: an upper bound on the device bytes a new context allocates for
+ * the request (admission). */
 int64_t trj_request_footprint(const trj_request* r);
 /* Queue a request; *ticket identifies it.  Admits queued requests whose turn has come. */
 int trj_srv_submit(trj_srv* s, const trj_request* r, int64_t* ticket);
@@ -521,8 +527,13 @@ int trj_srv_poll(trj_srv* s, int64_t ticket, int32_t* done);
 /* Results of a completed request (any pointer may be NULL); releases its slot.
  * counts: nit x lsmax (RDF); sqf, sqfp, nq as trj_sq_shells (S(Q)). */
 int trj_srv_result(trj_srv* s, int64_t ticket, int64_t* counts, double* sqf, double* sqfp, int64_t* nq);
-/* Requests queued, running, and device bytes reserved by running requests. */
+/* Requests queued, running, and device bytes the slots hold now (as trj_srv_memory). */
 int trj_srv_stats(const trj_srv* s, int32_t* queued, int32_t* running, int64_t* reserved_bytes);
+/* Device memory accounting: bytes the slots hold now (running: retained +
+ * footprint; idle: retained), the largest such total so far (never above the
+ * budget), and the number of requests whose context grew past what was held
+ * for it (0 while trj_request_footprint is an upper bound). */
+int trj_srv_memory(const trj_srv* s, int64_t* reserved, int64_t* peak, int64_t* misses);
 
 #ifdef __cplusplus
 }

@@ -59,3 +59,48 @@ def test_fqt_with_rdf_in_one_frame_call():
     assert co == oco
     np.testing.assert_allclose(fs, tots / (900.0 * co * len(kv)), rtol=1e-9)
     assert counts.sum() > 0
+
+
+def test_fqt_only_2d_host_frame():
+    """An F(Q,t)-only frame on a 2D context reads r(2,N) from the host: the
+    staging copy is sized by the F(Q,t) module's ndim (regression: it used to
+    take 3 coordinates per atom from a 2-coordinate array)."""
+    L = 20.0
+    rng = np.random.default_rng(21)
+    n = 1200
+    x = rng.random((n, 2)) * L
+    it = rng.integers(1, 3, n).astype(np.int32)
+    frames = []
+    for _ in range(7):
+        x = (x + 0.2 * rng.standard_normal((n, 2))) % L
+        frames.append(x.copy())
+    qw = [0.9, 1.7]
+    print("case: fqt-only 2D host frame", flush=True)
+    with T.Context(0) as ctx:
+        nv, nw = ctx.fqt_init(qw, 2, 2, 4, 4, [L, L, 1.0], ndim=2, nsp=2, bsc=[1.0, 0.5])
+        for f in frames:
+            ctx.frame(f, it, [L, L, 1.0], what=T.TRJ_WANT_FQT)
+        f2, fs2, co = ctx.fqt_results()
+    padded = [np.c_[f, np.zeros(n)] for f in frames]       # the oracle reads 3 values per atom
+    tot, tots, oco = O.fqt(padded, it, 2, 2, [1.0, 0.5], [L, L, 1.0], qw, 2, 2, 4, 4)
+    assert co == oco > 0
+    den = np.array([n * co * w for w in nw], dtype=np.float64)[:, None]
+    np.testing.assert_allclose(f2, tot / den, rtol=1e-9, atol=1e-11)
+    np.testing.assert_allclose(fs2, tots / den, rtol=1e-9, atol=1e-11)
+
+
+def test_fqt_ndim_mismatch_rejected():
+    """S(Q) set up in 2D and F(Q,t) in 3D: a frame asking for both is refused
+    with TRJ_EINVAL before any input is read."""
+    L = 15.0
+    rng = np.random.default_rng(3)
+    x = rng.random((500, 2)) * L
+    it = np.ones(500, dtype=np.int32)
+    print("case: fqt ndim mismatch", flush=True)
+    with T.Context(0) as ctx:
+        ctx.sq_init(T.sq_params(2, 1, 3.0), [L, L, 1.0])
+        ctx.fqt_init([1.0], 1, 1, 2, 2, [L] * 3, ndim=3, nsp=1)
+        with pytest.raises(T.TrjError) as e:
+            ctx.frame(x, it, [L, L, 1.0], what=T.TRJ_WANT_SQ | T.TRJ_WANT_FQT)
+        assert e.value.code == T.TRJ_EINVAL
+        assert "different ndim" in str(e.value)

@@ -226,3 +226,41 @@ def test_charges_follow_selected_atoms(tmp_path, qtype):
         assert not tr.has_charge
         with pytest.raises(T.TrjError):
             tr.frame_ref(0, [1, 2, 3], charges=True)
+
+
+@pytest.mark.parametrize("bad", [np.nan, 1e12, -3e9])
+def test_corrupt_type_values_refused(tmp_path, bad):
+    """A type variable stored as float with a NaN or a huge value (a corrupt
+    file) is refused with TRJ_EINVAL before it can size the species map."""
+    path = str(tmp_path / "bad.nc")
+    coords, types, org, L = _frames(nframes=1, natoms=20)
+    with scipy_io.netcdf_file(path, "w", version=1) as nc:
+        nc.createDimension("frame", None)
+        nc.createDimension("atom", 20)
+        nc.createDimension("spatial", 3)
+        nc.createDimension("cell_spatial", 3)
+        cl = nc.createVariable("cell_lengths", "f", ("frame", "cell_spatial"))
+        ty = nc.createVariable("type", "d", ("frame", "atom"))
+        xyz = nc.createVariable("coordinates", "f", ("frame", "atom", "spatial"))
+        t = types.astype(np.float64)
+        t[7] = bad
+        cl[0] = np.array([10.0, 12.0, 9.0], dtype=np.float32)
+        ty[0] = t
+        xyz[0] = coords[0]
+    with T.NcTrajectory(path) as tr:
+        with pytest.raises(T.TrjError) as e:
+            tr.frame_ref(0, [1, 2])
+        assert e.value.code == T.TRJ_EINVAL
+        assert "type value of atom 7" in str(e.value)
+        with pytest.raises(T.TrjError) as e:
+            tr.read(0)
+        assert e.value.code == T.TRJ_EINVAL
+
+
+def test_selected_type_out_of_range_refused(tmp_path):
+    path = str(tmp_path / "t.nc")
+    _write_scipy(path, 1)
+    with T.NcTrajectory(path) as tr:
+        with pytest.raises(T.TrjError) as e:
+            tr.frame_ref(0, [1, 2_000_000_000])
+        assert e.value.code == T.TRJ_EINVAL

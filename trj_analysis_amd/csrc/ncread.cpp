@@ -12,12 +12,15 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
 #include "../../include/trj_hip.h"
 
 namespace {
+
+constexpr int32_t kTypeMax = 1 << 20;  // largest |type| accepted from a file
 
 enum NcType { kByte = 1, kChar = 2, kShort = 3, kInt = 4, kFloat = 5, kDouble = 6, kUByte = 7, kUShort = 8,
               kUInt = 9, kInt64 = 10, kUInt64 = 11 };
@@ -272,7 +275,7 @@ struct trj_nc {
 
 extern "C" {
 
-int trj_nc_open(trj_nc** out, const char* path, char* errbuf, int32_t errlen) {
+static int trj_nc_open_impl(trj_nc** out, const char* path, char* errbuf, int32_t errlen) {
   if (!out || !path) return TRJ_EINVAL;
   *out = nullptr;
   trj_nc* h = new trj_nc();
@@ -333,7 +336,7 @@ int trj_nc_info(const trj_nc* h, trj_nc_file_info* info) {
   return TRJ_OK;
 }
 
-int trj_nc_read(trj_nc* h, int64_t frame, double* pos, int32_t* type, double origin[3], double lengths[3]) {
+static int trj_nc_read_impl(trj_nc* h, int64_t frame, double* pos, int32_t* type, double origin[3], double lengths[3]) {
   if (!h) return TRJ_EINVAL;
   if (frame < 0 || frame >= h->r.numrecs) {
     h->err = "frame " + std::to_string(frame) + " outside 0.." + std::to_string(h->r.numrecs - 1);
@@ -345,7 +348,16 @@ int trj_nc_read(trj_nc* h, int64_t frame, double* pos, int32_t* type, double ori
     std::vector<double>& t = h->tyd;
     t.resize(static_cast<size_t>(h->natoms));
     if (!h->r.read(*h->type, frame, 0, h->natoms, t.data())) return (h->err = h->r.err, TRJ_EINVAL);
-    for (int64_t i = 0; i < h->natoms; ++i) type[i] = static_cast<int32_t>(t[i]);
+    // LAMMPS types are small positive integers; a value outside this range
+    // (or NaN) would be undefined to convert and would size the species map.
+    for (int64_t i = 0; i < h->natoms; ++i) {
+      if (!(std::fabs(t[i]) <= kTypeMax)) {
+        h->err = "type value of atom " + std::to_string(i) + " in frame " + std::to_string(frame) +
+                 " is not a number in -2^20..2^20";
+        return TRJ_EINVAL;
+      }
+      type[i] = static_cast<int32_t>(t[i]);
+    }
   }
   double o[3] = {0, 0, 0}, l[3] = {0, 0, 0};
   if (h->origin && !h->r.read(*h->origin, frame, 0, 3, o)) return (h->err = h->r.err, TRJ_EINVAL);
@@ -371,7 +383,7 @@ int trj_nc_frame_ref(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_ty
   return trj_nc_frame_ref_q(h, frame, nsp, sp_types, pos_soa, itype, nullptr, n_out, box, periodic);
 }
 
-int trj_nc_frame_ref_q(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_types, double* pos_soa,
+static int trj_nc_frame_ref_q_impl(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_types, double* pos_soa,
                        int32_t* itype, double* charge, int64_t* n_out, double box[3], int32_t periodic[3]) {
   if (!h || !sp_types || !pos_soa || !itype || !n_out || !box || !periodic) return TRJ_EINVAL;
   if (nsp < 1 || nsp > 10) return (h->err = "nsp must be in 1..10", TRJ_EINVAL);
@@ -394,6 +406,8 @@ int trj_nc_frame_ref_q(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_
   // Species map, fixed at the first frame read: the selected LAMMPS types in
   // order of first appearance (netcdf.f90:441-506).
   const std::vector<int32_t> sel(sp_types, sp_types + nsp);
+  for (int32_t t : sel)
+    if (t < 0 || t > kTypeMax) return (h->err = "sp_types must lie in 0..2^20", TRJ_EINVAL);
   if (h->species_of.empty()) {
     int32_t maxt = 0;
     for (int32_t t : ty) maxt = std::max(maxt, t);
@@ -479,6 +493,36 @@ int trj_nc_frame_ref_q(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_
   }
   *n_out = m;
   return TRJ_OK;
+}
+
+// The reader's containers are sized from the file; an allocation failure is
+// reported as TRJ_ENOMEM instead of crossing the C boundary as an exception.
+int trj_nc_open(trj_nc** out, const char* path, char* errbuf, int32_t errlen) {
+  try {
+    return trj_nc_open_impl(out, path, errbuf, errlen);
+  } catch (const std::bad_alloc&) {
+    if (errbuf && errlen > 0) std::snprintf(errbuf, static_cast<size_t>(errlen), "host allocation failed");
+    return TRJ_ENOMEM;
+  }
+}
+
+int trj_nc_read(trj_nc* h, int64_t frame, double* pos, int32_t* type, double origin[3], double lengths[3]) {
+  try {
+    return trj_nc_read_impl(h, frame, pos, type, origin, lengths);
+  } catch (const std::bad_alloc&) {
+    if (h) h->err = "host allocation failed";
+    return TRJ_ENOMEM;
+  }
+}
+
+int trj_nc_frame_ref_q(trj_nc* h, int64_t frame, int32_t nsp, const int32_t* sp_types, double* pos_soa,
+                       int32_t* itype, double* charge, int64_t* n_out, double box[3], int32_t periodic[3]) {
+  try {
+    return trj_nc_frame_ref_q_impl(h, frame, nsp, sp_types, pos_soa, itype, charge, n_out, box, periodic);
+  } catch (const std::bad_alloc&) {
+    if (h) h->err = "host allocation failed";
+    return TRJ_ENOMEM;
+  }
 }
 
 }  // extern "C"

@@ -2,8 +2,11 @@
 // batching of independent RDF / S(Q) requests.  Each slot is a context (own
 // HIP stream) driven by its own worker thread, so the host preparation of one
 // request (q-set plan, staging) overlaps the device work of the others.  A
-// request is admitted in FIFO order when a worker is free and its estimated
-// device footprint fits next to the running requests.
+// request is admitted in FIFO order when a worker is free and the device
+// memory it may need fits the HBM budget next to what every slot holds: a
+// running slot holds its context's retained buffers plus the request's
+// footprint, an idle slot its retained buffers (contexts keep what they grew).
+// When the next request does not fit, the idle slots give their buffers back.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,13 +39,19 @@ struct Req {
 
 struct trj_srv {
   int device = 0;
-  int64_t budget = 0, reserved = 0;
+  int64_t budget = 0;
+  int64_t reserved = 0;       // sum of held[]
+  int64_t peak = 0;           // largest `reserved` so far
+  int64_t misses = 0;         // requests whose context grew past what was held for them
   int running = 0;
   std::mutex mu;
   std::condition_variable cv;
   std::deque<int64_t> queue;
   std::unordered_map<int64_t, Req> reqs;
   std::vector<trj_ctx*> ctxs;
+  std::vector<int64_t> retained;  // device bytes each slot's context holds between requests
+  std::vector<int64_t> held;      // what each slot counts against the budget now
+  std::vector<char> busy;
   std::vector<std::thread> workers;
   bool stop = false;
   int64_t next = 1;
@@ -80,6 +89,29 @@ void execute(trj_ctx* ctx, Req& q) {
   if (rc != TRJ_OK) q.err = trj_last_error(ctx);
 }
 
+void set_held(trj_srv* s, int k, int64_t v) {
+  s->reserved += v - s->held[k];
+  s->held[k] = v;
+  s->peak = std::max(s->peak, s->reserved);
+}
+
+// Slot k may take a request of footprint fp when its context's retained
+// buffers plus fp fit next to what the other slots hold.
+bool fits(const trj_srv* s, int k, int64_t fp) {
+  return s->reserved - s->held[k] + s->retained[k] + fp <= s->budget;
+}
+
+// Gives the buffers of every idle slot back to the device (called with the
+// lock held: an idle slot's worker is waiting on the lock, so its context is
+// not in use).
+void release_idle(trj_srv* s) {
+  for (size_t j = 0; j < s->ctxs.size(); ++j)
+    if (!s->busy[j] && s->retained[j] > 0 && trj::ctx_release(s->ctxs[j]) == TRJ_OK) {
+      s->retained[j] = 0;
+      set_held(s, static_cast<int>(j), 0);
+    }
+}
+
 void worker(trj_srv* s, int k) {
   (void)hipSetDevice(s->device);
   for (;;) {
@@ -87,25 +119,35 @@ void worker(trj_srv* s, int k) {
     Req* q;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      s->cv.wait(lk, [&] {
-        if (s->stop) return true;
-        if (s->queue.empty()) return false;
-        const Req& h = s->reqs[s->queue.front()];
-        return s->running == 0 || s->reserved + h.footprint <= s->budget;
-      });
-      if (s->stop) return;
+      for (;;) {
+        if (s->stop) return;
+        if (!s->queue.empty()) {
+          const int64_t fp = s->reqs[s->queue.front()].footprint;
+          if (!fits(s, k, fp)) release_idle(s);
+          // Every request's footprint is at most the budget (trj_srv_submit),
+          // so with all idle slots released the request fits once the
+          // running ones are done.
+          if (fits(s, k, fp)) break;
+        }
+        s->cv.wait(lk);
+      }
       t = s->queue.front();
       s->queue.pop_front();
       q = &s->reqs[t];  // element references are stable in unordered_map
       q->state = 1;
-      s->reserved += q->footprint;
+      s->busy[k] = 1;
+      set_held(s, k, s->retained[k] + q->footprint);
       s->running += 1;
     }
     execute(s->ctxs[k], *q);
+    const int64_t after = trj_device_bytes(s->ctxs[k]);
     {
       std::lock_guard<std::mutex> lk(s->mu);
       q->state = 2;
-      s->reserved -= q->footprint;
+      if (after > s->held[k]) s->misses += 1;
+      s->retained[k] = after;
+      s->busy[k] = 0;
+      set_held(s, k, after);
       s->running -= 1;
     }
     s->cv.notify_all();
@@ -117,31 +159,50 @@ void worker(trj_srv* s, int k) {
 extern "C" {
 
 int64_t trj_request_footprint(const trj_request* r) {
+  // An upper bound on the device bytes a new context allocates for the
+  // request (every buffer run_frame and the inits size for it, each with the
+  // 1/8 headroom of the context's allocator).
   if (!r || r->n < 1) return 0;
   const int64_t n = r->n;
-  int64_t b = n * 3 * 8 + n * 4;  // staged input
-  b += n * 24 + n * 12;           // wrapped positions, species sort
+  const int nsp = std::max(1, std::max(r->rdf.nsp, r->sq.nsp));
+  const int64_t nblk = (n + 255) / 256;
+  int64_t b = n * 3 * 8 + n * 4;             // staged input, types
+  b += n * 24 + 64;                          // wrapped FP64 positions, flags
+  b += 2 * int64_t(4) * nsp * nblk + 4 * (nsp + 3);  // species sort
   if (r->what & TRJ_WANT_RDF) {
     const int nit = r->rdf.nsp * (r->rdf.nsp + 1) / 2;
     int32_t ls = 0;
     double rc = 0.0;
-    if (trj_rdf_plan(&r->rdf, r->box, &ls, &rc) == TRJ_OK) b += int64_t(2048) * nit * ls * 4 + nit * ls * 8;
-    b += n * 40;  // cell list
+    if (trj_rdf_plan(&r->rdf, r->box, &ls, &rc) == TRJ_OK && rc > 0.0) {
+      int64_t ncell = 1;
+      for (int k = 0; k < r->rdf.ndim; ++k)
+        ncell *= std::max<int64_t>(1, std::min<int64_t>(static_cast<int64_t>(r->box[k] / rc), 2048));
+      b += int64_t(2048) * nit * ls * 4;     // per-workgroup histogram slices
+      b += int64_t(nit) * ls * 8 + int64_t(ls + 2) * 8;  // totals, bin edges
+      b += 12 * (ncell + 1);                 // cell counts, starts, cursors
+    }
+    b += n * (4 + 24 + 4);                   // cell ids, cell-sorted positions and types
   }
   if (r->what & TRJ_WANT_SQ) {
-    trj_sq_info info;
-    if (trj_sq_plan(&r->sq, r->box, &info, nullptr) == TRJ_OK) {
-      const trj::TileShape tk = trj::tile_shape(r->sq.precision);
-      const int64_t tiles = (info.nvec_grid + tk.vec() - 1) / tk.vec() + 1;
+    trj::SqPlan S;
+    std::string err;
+    if (trj::sq_plan(r->sq, r->box, &S, &err) == TRJ_OK) {
+      const int64_t tiles = static_cast<int64_t>(S.tiles.size());
+      const int64_t nax = static_cast<int64_t>(S.axis_shell.size());
       int dev = 0, slots = 0;  // the splits depend on the CU count (16-bit)
       if (hipGetDevice(&dev) != hipSuccess ||
           hipDeviceGetAttribute(&slots, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         slots = 0;
-      const int ks = trj::contraction_splits(static_cast<int>(tiles), n, r->sq.precision, r->sq.nsp, slots);
-      b += int64_t(ks) * r->sq.nsp * tiles * tk.vec() * 16 + n * 96;
+      const int ks = tiles > 0 ? trj::contraction_splits(static_cast<int>(tiles), n, r->sq.precision, r->sq.nsp, slots)
+                               : 0;
+      b += int64_t(ks) * r->sq.nsp * tiles * S.tk.vec() * 16;  // rho partials
+      b += n * 96;                                             // turn planes
+      b += tiles * (16 + 2 * S.tk.vec()) + 32;                 // tiles, shells of tile vectors
+      b += int64_t(S.nqmax + 1) * 8 * (1 + r->sq.nsp);         // shell sums
+      b += nax * 16 + static_cast<int64_t>(trj::sq_list_scratch_bytes(static_cast<int>(nax), r->sq.nsp));
     }
   }
-  return b;
+  return b + b / 8 + (int64_t(1) << 20);
 }
 
 int trj_srv_create(trj_srv** out, int device, int32_t nslots, int64_t budget) {
@@ -151,6 +212,9 @@ int trj_srv_create(trj_srv** out, int device, int32_t nslots, int64_t budget) {
   s->device = device;
   s->budget = budget > 0 ? budget : (int64_t(64) << 30);
   s->ctxs.assign(nslots, nullptr);
+  s->retained.assign(nslots, 0);
+  s->held.assign(nslots, 0);
+  s->busy.assign(nslots, 0);
   for (int k = 0; k < nslots; ++k) {
     const int rc = trj_create(&s->ctxs[k], device);
     if (rc != TRJ_OK) {
@@ -185,6 +249,9 @@ int trj_srv_submit(trj_srv* s, const trj_request* r, int64_t* ticket) {
   Req q;
   q.r = *r;
   q.footprint = trj_request_footprint(r);
+  if (q.footprint > s->budget)
+    return sfail(s, TRJ_ELIMIT, "request needs up to " + std::to_string(q.footprint) +
+                                    " device bytes, above the server's HBM budget of " + std::to_string(s->budget));
   {
     std::lock_guard<std::mutex> lk(s->mu);
     const int64_t t = s->next++;
@@ -232,6 +299,16 @@ int trj_srv_stats(const trj_srv* s_in, int32_t* queued, int32_t* running, int64_
   if (queued) *queued = static_cast<int32_t>(s->queue.size());
   if (running) *running = s->running;
   if (reserved) *reserved = s->reserved;
+  return TRJ_OK;
+}
+
+int trj_srv_memory(const trj_srv* s_in, int64_t* reserved, int64_t* peak, int64_t* misses) {
+  if (!s_in) return TRJ_EINVAL;
+  trj_srv* s = const_cast<trj_srv*>(s_in);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (reserved) *reserved = s->reserved;
+  if (peak) *peak = s->peak;
+  if (misses) *misses = s->misses;
   return TRJ_OK;
 }
 

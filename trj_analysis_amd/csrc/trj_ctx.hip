@@ -114,6 +114,21 @@ struct trj_ctx {
 
 namespace {
 
+// Every device buffer a context owns (trj_destroy frees them, trj_device_bytes
+// sums their capacities, the server's release path frees them).
+std::vector<DevBuf*> dev_bufs(trj_ctx* c) {
+  return {&c->raw,       &c->itype,     &c->pos,      &c->spos,       &c->stype,    &c->cellid,   &c->cellcount,
+          &c->cellstart, &c->cursor,    &c->partial,  &c->counts,     &c->thr,      &c->blk,      &c->blkoff,
+          &c->sp_off,    &c->u,         &c->rho,      &c->tiles,      &c->tile_shell, &c->sqf,    &c->sqfp,
+          &c->axis_kv,   &c->axis_shell, &c->listpart, &c->flags,     &c->qraw,     &c->qsort,    &c->gqf,
+          &c->gqacc,     &c->sqfq,      &c->qparm,    &c->slkey,      &c->sllast,   &c->slcount,  &c->slcountp,
+          &c->slqsum,    &c->slqcnt,    &c->sqqsum,   &c->sqqcnt,     &c->xyframe,  &c->xycounts, &c->xyacc,
+          &c->xygqf,     &c->xygqacc,   &c->thr_xy,   &c->rpts,       &c->s2gi,     &c->s2g2i,    &c->fqkv,
+          &c->fqbin,     &c->fqr0,      &c->fqrho0,   &c->fqpart,     &c->fqvsum,   &c->fqfkt,    &c->fqfskt,
+          &c->fqtot,     &c->fqtots};
+}
+
+
 int fail(trj_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   return code;
@@ -354,6 +369,31 @@ int fqt_step(trj_ctx* c, int64_t n, const int32_t* itype_dev) {
   return TRJ_OK;
 }
 
+// The dimensionality and species count a frame with these `what` bits is read
+// with: the RDF's when it is asked for, else the S(Q)'s, else F(Q,t)'s.  Every
+// module the frame asks for must be initialised with the same ndim and nsp.
+// Host and device entry points both call this before touching any input, so
+// the bytes copied out of the caller's r(ndim,N) array and the bytes the
+// ingest kernel reads are sized by the same rule.
+int frame_shape(trj_ctx* c, uint32_t what, int* ndim_out, int* nsp_out) {
+  const bool do_rdf = (what & TRJ_WANT_RDF) != 0;
+  const bool do_sq = (what & TRJ_WANT_SQ) != 0;
+  const bool do_fqt = (what & TRJ_WANT_FQT) != 0;
+  if (do_rdf && !c->rdf_ready) return fail(c, TRJ_ESTATE, "frame asks for RDF before trj_rdf_init");
+  if (do_sq && !c->sq_ready) return fail(c, TRJ_ESTATE, "frame asks for S(Q) before trj_sq_init");
+  if (do_fqt && !c->fqt_ready) return fail(c, TRJ_ESTATE, "frame asks for F(Q,t) before trj_fqt_init");
+  if (!do_rdf && !do_sq && !do_fqt) return fail(c, TRJ_EINVAL, "frame asks for nothing");
+  const int ndim = do_rdf ? c->rdf.ndim : (do_sq ? c->sq.p.ndim : c->fqt_ndim);
+  const int nsp = do_rdf ? c->rdf.nsp : (do_sq ? c->sq.p.nsp : c->fqt_nsp);
+  if (do_rdf && do_sq && (c->rdf.ndim != c->sq.p.ndim || c->rdf.nsp != c->sq.p.nsp))
+    return fail(c, TRJ_EINVAL, "RDF and S(Q) were initialised with different ndim/nsp");
+  if (do_fqt && (c->fqt_ndim != ndim || c->fqt_nsp != nsp))
+    return fail(c, TRJ_EINVAL, "F(Q,t) was initialised with a different ndim/nsp");
+  *ndim_out = ndim;
+  *nsp_out = nsp;
+  return TRJ_OK;
+}
+
 // The frame pipeline, device inputs.
 // q_dev: the frame's charges on the device (NULL without); the caller has
 // queued the frame's {scale, avq2} into qparm[0..1] on the context stream.
@@ -368,20 +408,12 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
   const bool do_rdf = (what & TRJ_WANT_RDF) != 0;
   const bool do_sq = (what & TRJ_WANT_SQ) != 0;
   const bool do_fqt = (what & TRJ_WANT_FQT) != 0;
-  if (do_rdf && !c->rdf_ready) return fail(c, TRJ_ESTATE, "frame asks for RDF before trj_rdf_init");
-  if (do_sq && !c->sq_ready) return fail(c, TRJ_ESTATE, "frame asks for S(Q) before trj_sq_init");
-  if (do_fqt && !c->fqt_ready) return fail(c, TRJ_ESTATE, "frame asks for F(Q,t) before trj_fqt_init");
-  if (!do_rdf && !do_sq && !do_fqt) return fail(c, TRJ_EINVAL, "frame asks for nothing");
+  int ndim = 0, nsp = 0;
+  if (int r0 = frame_shape(c, what, &ndim, &nsp)) return r0;
   const bool rdf_q = do_rdf && c->rdf_charge, sq_q = do_sq && c->sq_charge;
   const bool sq_ch = sq_q && !qspecies;  // a charge channel in the contraction
   if ((rdf_q || sq_q) && !q_dev)
     return fail(c, TRJ_EINVAL, "charges are enabled (TRJ_RDF_CHARGE / TRJ_SQ_CHARGE): frames go through trj_frame_q");
-  const int ndim = do_rdf ? c->rdf.ndim : (do_sq ? c->sq.p.ndim : c->fqt_ndim);
-  const int nsp = do_rdf ? c->rdf.nsp : (do_sq ? c->sq.p.nsp : c->fqt_nsp);
-  if (do_rdf && do_sq && (c->rdf.ndim != c->sq.p.ndim || c->rdf.nsp != c->sq.p.nsp))
-    return fail(c, TRJ_EINVAL, "RDF and S(Q) were initialised with different ndim/nsp");
-  if (do_fqt && (c->fqt_ndim != ndim || c->fqt_nsp != nsp))
-    return fail(c, TRJ_EINVAL, "F(Q,t) was initialised with a different ndim/nsp");
   if (do_fqt && c->fqt_frames > 0 && n != c->fqt_n)
     return fail(c, TRJ_EINVAL, "F(Q,t): the number of atoms changed between frames");
   if (n < 1) return fail(c, TRJ_EINVAL, "n must be positive");
@@ -710,6 +742,33 @@ int contraction_splits(int ntiles, int64_t n, int precision, int nsp, int slots)
   const int64_t cap = std::max<int64_t>(1, (int64_t(4) << 30) / per_split);
   return static_cast<int>(std::min<int64_t>(ks, cap));
 }
+
+// Frees every device buffer of an idle context and leaves it as a new one:
+// no module initialised, no slices, no sampling points.  The streams, events
+// and pinned staging stay.
+int ctx_release(trj_ctx* c) {
+  hipSetDevice(c->device);
+  if (int r = sync_all(c)) return r;
+  if (c->rdf_stream) HIPC(c, hipStreamSynchronize(c->rdf_stream));
+  for (DevBuf* b : dev_bufs(c)) {
+    if (b->p) hipFree(b->p);
+    b->p = nullptr;
+    b->cap = 0;
+  }
+  c->rdf_ready = c->sq_ready = c->fqt_ready = false;
+  c->rdf_charge = c->sq_charge = c->sq_centres = false;
+  c->slices = SliceSet{};
+  c->nrand = 0;
+  c->rpts_host.clear();
+  c->rdf_frames = c->sq_frames = 0;
+  c->fqt_frames = c->fqt_count_org = 0;
+  // The error flags word is read after every frame and must start at zero,
+  // as trj_create leaves it.
+  if (int r = ensure(c, c->flags, sizeof(uint32_t))) return r;
+  HIPC(c, hipMemsetAsync(c->flags.p, 0, sizeof(uint32_t), c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return TRJ_OK;
+}
 }  // namespace trj
 
 namespace {
@@ -770,16 +829,7 @@ void trj_destroy(trj_ctx* c) {
   if (!c) return;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
-  DevBuf* bufs[] = {&c->raw,  &c->itype,     &c->pos,       &c->spos,   &c->stype,  &c->cellid,
-                    &c->cellcount, &c->cellstart, &c->cursor, &c->partial, &c->counts, &c->thr,
-                    &c->blk,  &c->blkoff,    &c->sp_off,    &c->u,      &c->rho,    &c->tiles,
-                    &c->tile_shell, &c->sqf, &c->sqfp,      &c->axis_kv, &c->axis_shell, &c->listpart,
-                    &c->flags, &c->qraw, &c->qsort, &c->gqf, &c->gqacc, &c->sqfq, &c->qparm, &c->slkey,
-                    &c->sllast, &c->slcount, &c->slcountp, &c->slqsum, &c->slqcnt, &c->sqqsum, &c->sqqcnt,
-                    &c->xyframe, &c->xycounts, &c->xyacc, &c->xygqf, &c->xygqacc, &c->thr_xy, &c->rpts,
-                    &c->s2gi, &c->s2g2i, &c->fqkv, &c->fqbin, &c->fqr0, &c->fqrho0,
-                    &c->fqpart, &c->fqvsum, &c->fqfkt, &c->fqfskt, &c->fqtot, &c->fqtots};
-  for (DevBuf* b : bufs)
+  for (DevBuf* b : dev_bufs(c))
     if (b->p) hipFree(b->p);
   for (int k = 0; k < 2; ++k) {
     if (c->pinned[k]) hipHostFree(c->pinned[k]);
@@ -1022,7 +1072,8 @@ int frame_host(trj_ctx* c, const void* pos, int32_t dtype, int32_t layout, const
   if (n < 1) return fail(c, TRJ_EINVAL, "n must be positive");
   if (dtype != TRJ_F32 && dtype != TRJ_F64) return fail(c, TRJ_EINVAL, "bad dtype");
   hipSetDevice(c->device);
-  const int ndim = (what & TRJ_WANT_RDF) && c->rdf_ready ? c->rdf.ndim : (c->sq_ready ? c->sq.p.ndim : 3);
+  int ndim = 0, nspf = 0;
+  if (int r0 = frame_shape(c, what, &ndim, &nspf)) return r0;
   const size_t pbytes = static_cast<size_t>(n) * ndim * (dtype == TRJ_F32 ? 4 : 8);
   const size_t tbytes = static_cast<size_t>(n) * sizeof(int32_t);
   const size_t qbytes = charge ? static_cast<size_t>(n) * sizeof(double) : 0;
@@ -1040,7 +1091,6 @@ int frame_host(trj_ctx* c, const void* pos, int32_t dtype, int32_t layout, const
   double qscale = 1.0, avq2 = 0.0;
   // Charges constant within each species (per-type charges): S(Q) then needs
   // no charge channel, rho_q = sum_s q_s rho_s.
-  const int nspf = (what & TRJ_WANT_RDF) && c->rdf_ready ? c->rdf.nsp : (c->sq_ready ? c->sq.p.nsp : 0);
   double qsp[kNspMax];
   bool by_species = charge != nullptr && nspf > 0;
   bool seen[kNspMax] = {false};
@@ -2135,17 +2185,8 @@ int trj_timing_reset(trj_ctx* c) {
 
 int64_t trj_device_bytes(const trj_ctx* c) {
   if (!c) return 0;
-  const DevBuf* bufs[] = {&c->raw,  &c->itype,     &c->pos,       &c->spos,   &c->stype,  &c->cellid,
-                          &c->cellcount, &c->cellstart, &c->cursor, &c->partial, &c->counts, &c->thr,
-                          &c->blk,  &c->blkoff,    &c->sp_off,    &c->u,      &c->rho,    &c->tiles,
-                          &c->tile_shell, &c->sqf, &c->sqfp,      &c->axis_kv, &c->axis_shell, &c->listpart,
-                          &c->flags, &c->qraw, &c->qsort, &c->gqf, &c->gqacc, &c->sqfq, &c->qparm, &c->slkey,
-                          &c->sllast, &c->slcount, &c->slcountp, &c->slqsum, &c->slqcnt, &c->sqqsum, &c->sqqcnt,
-                          &c->xyframe, &c->xycounts, &c->xyacc, &c->xygqf, &c->xygqacc, &c->thr_xy, &c->rpts,
-                          &c->s2gi, &c->s2g2i, &c->fqkv, &c->fqbin, &c->fqr0, &c->fqrho0,
-                    &c->fqpart, &c->fqvsum, &c->fqfkt, &c->fqfskt, &c->fqtot, &c->fqtots};
   int64_t s = 0;
-  for (const DevBuf* b : bufs) s += static_cast<int64_t>(b->cap);
+  for (const DevBuf* b : dev_bufs(const_cast<trj_ctx*>(c))) s += static_cast<int64_t>(b->cap);
   return s;
 }
 

@@ -90,6 +90,10 @@ int sq_plan(const trj_sq_params& p, const double box[3], SqPlan* out, std::strin
 // K-splits of the S(Q) contraction for ntiles tiles and n atoms.
 int contraction_splits(int ntiles, int64_t n, int precision, int nsp, int slots = 0);
 
+// Frees an idle context's device buffers and leaves it uninitialised
+// (the request server's way of returning memory to its HBM budget).
+int ctx_release(trj_ctx* c);
+
 // ---------------------------------------------------------------- kernels
 struct DevFrame;  // defined in trj_ctx.hip
 

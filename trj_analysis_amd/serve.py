@@ -38,19 +38,24 @@ class Job:
     nqmax: int
 
 
+def make_job(f: W.Frame, arrival: float = 0.0, precision: str = "fp16x3") -> Job:
+    """One RDF + S(Q) request on frame f: rc = 4, deltar = 0.01, the 64^3
+    half-space q-grid, unit scattering lengths."""
+    rdf = T.rdf_params(3, f.nsp, DELTAR, RCRDF, f.periodic)
+    sq = T.sq_params(3, f.nsp, 1e9, bsc=[1.0] * f.nsp, qset=T.TRJ_QSET_HALFSPACE, grid=GRID,
+                     precision=precision)
+    ls, _ = T.rdf_plan(rdf, f.box)
+    info, _ = T.sq_plan(sq, f.box)
+    return Job(f, arrival, rdf, sq, ls, f.nsp * (f.nsp + 1) // 2, info.nqmax)
+
+
 def make_jobs(nreq: int, rate: float, seed: int = 7, nmax: float = 2e6, precision: str = "fp16x3") -> List[Job]:
     rng = np.random.default_rng(seed)
     t = 0.0
     jobs = []
     for k in range(nreq):
         t += rng.exponential(1.0 / rate)
-        f = W.serving_frame(seed * 1000 + k, 1e4, nmax)
-        rdf = T.rdf_params(3, f.nsp, DELTAR, RCRDF, f.periodic)
-        sq = T.sq_params(3, f.nsp, 1e9, bsc=[1.0] * f.nsp, qset=T.TRJ_QSET_HALFSPACE, grid=GRID,
-                         precision=precision)
-        ls, _ = T.rdf_plan(rdf, f.box)
-        info, _ = T.sq_plan(sq, f.box)
-        jobs.append(Job(f, t, rdf, sq, ls, f.nsp * (f.nsp + 1) // 2, info.nqmax))
+        jobs.append(make_job(W.serving_frame(seed * 1000 + k, 1e4, nmax), t, precision))
     return jobs
 
 

@@ -45,7 +45,7 @@ EXPORTS = [
     "trj_batch_create", "trj_batch_destroy", "trj_batch_last_error",
     "trj_batch_layout", "trj_batch_run", "trj_batch_autotune", "trj_srv_create", "trj_srv_destroy",
     "trj_srv_last_error", "trj_request_footprint", "trj_srv_submit", "trj_srv_poll", "trj_srv_result",
-    "trj_srv_stats",
+    "trj_srv_stats", "trj_srv_memory",
 ]
 
 
@@ -196,6 +196,7 @@ def _declare(L):
     L.trj_srv_poll.argtypes = [vp, C.c_int64, i32]
     L.trj_srv_result.argtypes = [vp, C.c_int64, i64, d, d, i64]
     L.trj_srv_stats.argtypes = [vp, i32, i32, i64]
+    L.trj_srv_memory.argtypes = [vp, i64, i64, i64]
     no_int = ("trj_destroy", "trj_last_error", "trj_device_bytes", "trj_batch_destroy",
               "trj_batch_last_error", "trj_srv_destroy", "trj_srv_last_error", "trj_request_footprint",
               "trj_state_bytes", "trj_nc_close", "trj_nc_last_error")
@@ -870,3 +871,9 @@ class Server:
         q, r, b = C.c_int32(), C.c_int32(), C.c_int64()
         self._check(self._L.trj_srv_stats(self._h, C.byref(q), C.byref(r), C.byref(b)), "trj_srv_stats")
         return q.value, r.value, b.value
+
+    def memory(self):
+        """(bytes the slots hold now, the peak of that, requests that outgrew their reservation)."""
+        a, b, m = C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(self._L.trj_srv_memory(self._h, C.byref(a), C.byref(b), C.byref(m)), "trj_srv_memory")
+        return a.value, b.value, m.value
