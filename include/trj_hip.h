@@ -190,6 +190,24 @@ int trj_frame_device_q(trj_ctx* ctx, const void* pos_dev, int32_t dtype, int32_t
 int trj_frame_device(trj_ctx* ctx, const void* pos_dev, int32_t dtype, int32_t layout,
                      const int32_t* itype_dev, int64_t n, const double box[3], uint32_t what);
 
+/* trj_frame_device for an RDF-only frame (what = TRJ_WANT_RDF) that allocates
+ * nothing: no hipMalloc, hipFree or allocating library call.  It has no
+ * counterpart in the reference; it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid it).
+ * The frame's per-frame buffers (wrapped positions, species blocks, cell
+ * list, per-workgroup histograms) come from a static device array of 64 MiB
+ * that each GPU the library runs on holds until the process ends; the
+ * context's own buffers (bin edges, totals) come from trj_rdf_init, which
+ * does allocate.  Frames whose buffers do not fit (about 0.75M atoms with 3
+ * species and lsmax 501) are refused with TRJ_ELIMIT, as are contexts with
+ * charges, slices or sampling points (TRJ_EINVAL).  Counts accumulate into
+ * the context as with trj_frame_device.  All calls on one GPU share the one
+ * array, and it is in use until the frame's work on the context stream has
+ * finished: calls that overlap on one GPU (other contexts or threads before
+ * the previous frame completed) race. */
+int trj_frame_device_noalloc(trj_ctx* ctx, const void* pos_dev, int32_t dtype, int32_t layout,
+                             const int32_t* itype_dev, int64_t n, const double box[3], uint32_t what);
+
 /* Wait for the context's queued work; reports deferred device errors
  * (e.g. TRJ_ERANGE from the coordinate check). */
 int trj_sync(trj_ctx* ctx);

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -110,6 +111,9 @@ struct trj_ctx {
   double last_ms[4] = {0, 0, 0, 0};
   double tot_ms[4] = {0, 0, 0, 0};
   int64_t tot_frames = 0;
+  // Set while trj_frame_device_noalloc runs: the context may then use only
+  // buffers that already have their size (ensure() refuses to allocate).
+  bool noalloc = false;
 };
 
 namespace {
@@ -128,6 +132,47 @@ std::vector<DevBuf*> dev_bufs(trj_ctx* c) {
           &c->fqtot,     &c->fqtots};
 }
 
+}  // namespace
+
+// Scratch of the allocation-free entry point (trj_frame_device_noalloc): the
+// per-frame buffers of one RDF frame live here instead of in buffers the
+// context allocates.  Every device the library runs on holds its own copy
+// until the process ends.  In host code the name `scratch` is not a device
+// address: the address is taken once per device with hipGetSymbolAddress
+// (resolve_scratch, from trj_create with the device current) and only that
+// address is passed to kernels and copies.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+void resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  if (e == hipSuccess) g_scratch_ptr[device] = p;
+}
+
+// This device's scratch address, or NULL with the reason in *why.
+void* scratch_address(int device, std::string* why) {
+  if (device < 0 || device >= kMaxDevices) {
+    *why = "device index above the allocation-free entry point's limit of " + std::to_string(kMaxDevices);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device])
+    *why = std::string("hipGetSymbolAddress of the scratch array failed: ") + hipGetErrorString(g_scratch_err[device]);
+  return g_scratch_ptr[device];
+}
+
 
 int fail(trj_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -144,6 +189,9 @@ int fail(trj_ctx* c, int code, const std::string& msg) {
 int ensure(trj_ctx* c, DevBuf& b, size_t bytes) {
   if (bytes == 0) bytes = 16;
   if (b.cap >= bytes) return TRJ_OK;
+  if (c->noalloc)
+    return fail(c, TRJ_ELIMIT, "allocation-free frame: a buffer of " + std::to_string(bytes) +
+                                   " bytes is not provided (the library refuses to allocate here)");
   if (b.p) {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail(c, TRJ_EDEVICE, hipGetErrorString(e));
@@ -369,6 +417,50 @@ int fqt_step(trj_ctx* c, int64_t n, const int32_t* itype_dev) {
   return TRJ_OK;
 }
 
+// Cell grid and launch shape of the RDF pair kernels for one frame: cells of
+// side >= the cutoff on every axis (at most 160 per axis in 3D, 2048 in 2D);
+// a periodic axis with fewer than 3 cells sends the frame to the all-pairs
+// kernel.  The image shift replaces the per-pair minimum image on periodic
+// axes with at least 5 cells.  W wavefronts share a home cell.
+struct RdfCells {
+  CellGrid cg{};
+  bool cells_ok = true;
+  int64_t ncell = 1;
+  int cellshift[3] = {0, 0, 0};
+  int W = 1;
+  int nblocks = 1;
+};
+
+RdfCells rdf_cell_plan(int ndim, const double box[3], const int periodic[3], double side2, int64_t n) {
+  RdfCells p;
+  const double rc = std::sqrt(side2);
+  for (int k = 0; k < 3; ++k) {
+    if (k < ndim) {
+      int nc = static_cast<int>(std::floor(box[k] / (rc * (1.0 + 1e-6))));
+      nc = std::max(1, std::min(nc, ndim == 3 ? 160 : 2048));
+      p.cg.nc[k] = nc;
+      p.cg.inv_h[k] = nc / box[k];
+      p.cg.periodic[k] = periodic[k];
+      if (periodic[k] && nc < 3) p.cells_ok = false;
+      p.cellshift[k] = (periodic[k] && nc >= 5) ? 1 : 0;
+    } else {
+      p.cg.nc[k] = 1;
+      p.cg.inv_h[k] = 0.0;
+      p.cg.periodic[k] = 0;
+    }
+    p.ncell *= p.cg.nc[k];
+  }
+  if (p.cells_ok) {
+    const double mean = static_cast<double>(n) / static_cast<double>(p.ncell);
+    p.W = std::max(1, std::min(static_cast<int>(std::lround(mean / 64.0)), 8));
+    p.nblocks = static_cast<int>(std::min<int64_t>((p.ncell * p.W + 3) / 4, 2048));
+  } else {
+    const int64_t nb = (n + 63) / 64;
+    p.nblocks = static_cast<int>(std::min<int64_t>((nb * nb + 3) / 4, 1024));
+  }
+  return p;
+}
+
 // The dimensionality and species count a frame with these `what` bits is read
 // with: the RDF's when it is asked for, else the S(Q)'s, else F(Q,t)'s.  Every
 // module the frame asks for must be initialised with the same ndim and nsp.
@@ -546,27 +638,12 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
       if ((r = rdf_slices(c, pa, g, box, n, itype_dev, rdf_q ? q_dev : nullptr, rs))) return r;
     } else {
       // Cell grid: side >= cutoff on every axis.
-      const double rc = std::sqrt(g.side2);
-      CellGrid cg{};
-      bool cells_ok = true;
-      int64_t ncell = 1;
-      for (int k = 0; k < 3; ++k) {
-        if (k < ndim) {
-          int nc = static_cast<int>(std::floor(box[k] / (rc * (1.0 + 1e-6))));
-          nc = std::max(1, std::min(nc, ndim == 3 ? 160 : 2048));
-          cg.nc[k] = nc;
-          cg.inv_h[k] = nc / box[k];
-          cg.periodic[k] = periodic[k];
-          if (periodic[k] && nc < 3) cells_ok = false;
-          pa.cellshift[k] = (periodic[k] && nc >= 5) ? 1 : 0;
-        } else {
-          cg.nc[k] = 1;
-          cg.inv_h[k] = 0.0;
-          cg.periodic[k] = 0;
-        }
-        ncell *= cg.nc[k];
-      }
-      int nblocks;
+      const RdfCells rcl = rdf_cell_plan(ndim, box, periodic, g.side2, n);
+      const CellGrid& cg = rcl.cg;
+      const bool cells_ok = rcl.cells_ok;
+      const int64_t ncell = rcl.ncell;
+      for (int k = 0; k < 3; ++k) pa.cellshift[k] = rcl.cellshift[k];
+      const int nblocks = rcl.nblocks;
       if (cells_ok) {
         if ((r = ensure(c, c->cellid, sizeof(int32_t) * n))) return r;
         if ((r = ensure(c, c->cellcount, sizeof(int32_t) * (ncell + 1)))) return r;
@@ -585,17 +662,11 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
                             P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype),
                             qq_pairs ? q_dev : nullptr, qq_pairs ? P<double>(c->qsort) : nullptr, rs);
         if (qq_pairs) pa.q = P<double>(c->qsort);
-        const double mean = static_cast<double>(n) / static_cast<double>(ncell);
-        int W = static_cast<int>(std::lround(mean / 64.0));
-        W = std::max(1, std::min(W, 8));
-        const int64_t ntasks = ncell * W;
-        nblocks = static_cast<int>(std::min<int64_t>((ntasks + 3) / 4, 2048));
+        const int W = rcl.W;
         if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
         launch_rdf_cells(pa, cg, P<int32_t>(c->cellstart), P<double>(c->spos), P<int32_t>(c->stype), n, W,
                          nblocks, P<uint32_t>(c->partial), rs);
       } else {
-        const int64_t nb = (n + 63) / 64;
-        nblocks = static_cast<int>(std::min<int64_t>((nb * nb + 3) / 4, 1024));
         if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
         if (rdf_q && !qq_species) pa.q = q_dev;  // input order, like pos and itype here
         launch_rdf_allpairs(pa, P<double>(c->pos), itype_dev, n, nblocks, P<uint32_t>(c->partial), rs);
@@ -816,6 +887,9 @@ int trj_create(trj_ctx** out, int device) {
       trj_destroy(c);
       return TRJ_EDEVICE;
     }
+  // The device address of this device's allocation-free scratch array (a
+  // failure is reported by trj_frame_device_noalloc, the only user).
+  resolve_scratch(device);
   if (ensure(c, c->flags, sizeof(uint32_t)) != TRJ_OK ||
       hipMemsetAsync(c->flags.p, 0, sizeof(uint32_t), c->stream) != hipSuccess) {
     trj_destroy(c);
@@ -942,6 +1016,10 @@ int trj_rdf_init(trj_ctx* c, const trj_rdf_params* p, const double box[3], int32
   R.thr = thr;
   if ((r = ensure(c, c->thr, sizeof(double) * thr.size()))) return r;
   if ((r = ensure(c, c->counts, sizeof(int64_t) * R.nit * R.lsmax))) return r;
+  // Species offsets of the last frame, at their largest size: the result
+  // calls read them, so they stay in the context (also after an
+  // allocation-free frame).
+  if ((r = ensure(c, c->sp_off, sizeof(int32_t) * (kKeyMax + 2 + kMaxSlices)))) return r;
   HIPC(c, hipMemcpyAsync(c->thr.p, thr.data(), sizeof(double) * thr.size(), hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipMemsetAsync(c->counts.p, 0, sizeof(int64_t) * R.nit * R.lsmax, c->stream));
   if (qq) {
@@ -1057,6 +1135,60 @@ int trj_frame_device(trj_ctx* c, const void* pos_dev, int32_t dtype, int32_t lay
   if (!pos_dev || !itype_dev) return fail(c, TRJ_EINVAL, "NULL input");
   hipSetDevice(c->device);
   return run_frame(c, pos_dev, dtype, layout, itype_dev, n, box, what);
+}
+
+int trj_frame_device_noalloc(trj_ctx* c, const void* pos_dev, int32_t dtype, int32_t layout,
+                             const int32_t* itype_dev, int64_t n, const double box_in[3], uint32_t what) {
+  if (!c) return TRJ_EINVAL;
+  if (!pos_dev || !itype_dev) return fail(c, TRJ_EINVAL, "NULL input");
+  if (what != TRJ_WANT_RDF) return fail(c, TRJ_EINVAL, "allocation-free frames compute the RDF only (what = TRJ_WANT_RDF)");
+  if (!c->rdf_ready) return fail(c, TRJ_ESTATE, "frame asks for RDF before trj_rdf_init");
+  if (c->rdf_charge || c->slices.n > 0 || c->nrand > 0)
+    return fail(c, TRJ_EINVAL, "allocation-free frames take no charges, slices or sampling points");
+  if (n < 1) return fail(c, TRJ_EINVAL, "n must be positive");
+  const RdfPlan& R = c->rdf;
+  const int ndim = R.ndim;
+  if (int r = validate_box(c, ndim, box_in)) return r;
+  hipSetDevice(c->device);
+  std::string why;
+  char* base = static_cast<char*>(scratch_address(c->device, &why));
+  if (!base) return fail(c, TRJ_EDEVICE, why);
+  // The buffers run_frame asks for on this frame, at the sizes it asks
+  // (same geometry, same cell plan), each at a 256-byte offset.
+  const double box[3] = {box_in[0], box_in[1], ndim == 3 ? box_in[2] : 1.0};
+  double rc = R.rc;
+  const FrameGeom g = frame_geometry(ndim, box, R.periodic, &rc);
+  const RdfCells cp = rdf_cell_plan(ndim, box, R.periodic, g.side2, n);
+  const size_t nblk = static_cast<size_t>((n + 255) / 256);
+  const size_t un = static_cast<size_t>(n), nc = static_cast<size_t>(cp.ncell);
+  const size_t cells = cp.cells_ok ? 1 : 0;
+  DevBuf* sw[] = {&c->pos, &c->blk, &c->blkoff, &c->cellid, &c->cellcount, &c->cellstart, &c->cursor,
+                  &c->spos, &c->stype, &c->partial};
+  const size_t len[] = {24 * un, 4 * R.nsp * nblk, 4 * R.nsp * nblk, cells * 4 * un, cells * 4 * (nc + 1),
+                        cells * 4 * (nc + 1), cells * 4 * nc, cells * 24 * un, cells * 4 * un,
+                        4 * static_cast<size_t>(R.nit) * R.lsmax * static_cast<size_t>(cp.nblocks)};
+  constexpr int nb = sizeof(len) / sizeof(len[0]);
+  size_t off[nb], total = 0;
+  for (int k = 0; k < nb; ++k) {
+    off[k] = total;
+    total += (len[k] + 255) & ~size_t(255);
+  }
+  // 64 KiB stay free at the end of the array as a guard.
+  if (total + (size_t(64) << 10) > kScratchBytes)
+    return fail(c, TRJ_ELIMIT, "allocation-free frame needs " + std::to_string(total) +
+                                   " bytes of scratch; the library holds " + std::to_string(kScratchBytes) +
+                                   " (64 KiB of it kept free)");
+  DevBuf saved[nb];
+  for (int k = 0; k < nb; ++k) {
+    saved[k] = *sw[k];
+    sw[k]->p = base + off[k];
+    sw[k]->cap = len[k];
+  }
+  c->noalloc = true;
+  const int r = run_frame(c, pos_dev, dtype, layout, itype_dev, n, box_in, TRJ_WANT_RDF);
+  c->noalloc = false;
+  for (int k = 0; k < nb; ++k) *sw[k] = saved[k];
+  return r;
 }
 
 }  // extern "C"

@@ -34,7 +34,7 @@ PRECISIONS = {"fp64": TRJ_PREC_FP64, "fp32": TRJ_PREC_FP32, "fp16x3": TRJ_PREC_F
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTS = [
     "trj_abi_version", "trj_create", "trj_destroy", "trj_last_error", "trj_rdf_init",
-    "trj_sq_init", "trj_sq_plan", "trj_rdf_plan", "trj_frame", "trj_frame_device", "trj_frame_device_q", "trj_sync",
+    "trj_sq_init", "trj_sq_plan", "trj_rdf_plan", "trj_frame", "trj_frame_device", "trj_frame_device_q", "trj_frame_device_noalloc", "trj_sync",
     "trj_rdf_counts", "trj_rdf_gmix", "trj_rdf_first_minimum", "trj_sq_shells", "trj_sq_columns",
     "trj_sq_rho", "trj_fqt_init", "trj_fqt_results", "trj_rdf_sampling", "trj_rdf_sampling_points", "trj_rdf_s2n_sums", "trj_rdf_s2n",
     "trj_set_slices", "trj_slice_counts", "trj_rdf_xy", "trj_rdf_xy_gmix", "trj_rdf_xy_qq",
@@ -144,6 +144,7 @@ def _declare(L):
     L.trj_sq_qq_column.argtypes = [vp, C.c_int64, d]
     L.trj_frame_device.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, C.c_int64, d, u32]
     L.trj_frame_device_q.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, vp, C.c_int64, d, u32]
+    L.trj_frame_device_noalloc.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, C.c_int64, d, u32]
     L.trj_sync.argtypes = [vp]
     L.trj_rdf_counts.argtypes = [vp, i64]
     L.trj_rdf_gmix.argtypes = [vp, C.c_int64, d]
@@ -476,6 +477,15 @@ class Context:
             self._check(self._L.trj_frame_device_q(self._h, C.c_void_p(pos_ptr), dtype, layout,
                                                    C.c_void_p(itype_ptr), C.c_void_p(charge_ptr), n,
                                                    _ptr(b, C.c_double), what), "trj_frame_device_q")
+
+    def frame_device_noalloc(self, pos_ptr: int, dtype: int, layout: int, itype_ptr: int, n: int, box):
+        """RDF of a device-resident frame without any allocation
+        (trj_frame_device_noalloc; per-frame buffers from the library's
+        static scratch)."""
+        b = _box3(box)
+        self._check(self._L.trj_frame_device_noalloc(self._h, C.c_void_p(pos_ptr), dtype, layout,
+                                                     C.c_void_p(itype_ptr), n, _ptr(b, C.c_double), TRJ_WANT_RDF),
+                    "trj_frame_device_noalloc")
 
     def sync(self):
         self._check(self._L.trj_sync(self._h), "trj_sync")
