@@ -88,6 +88,10 @@ def _declare(L):
     L.or_sq_frame.argtypes = [C.c_int64, _D, _D, _D, _I32, C.c_int, C.c_int, _D,
                               C.POINTER(SqGrid), _D, _D, _D]
     L.or_sq_frame.restype = None
+    L.or_sq_frame_f64.argtypes = L.or_sq_frame.argtypes
+    L.or_sq_frame_f64.restype = None
+    L.or_sq_rho_f64.argtypes = L.or_sq_rho.argtypes
+    L.or_sq_rho_f64.restype = None
     L.or_sq_columns.argtypes = [C.c_int, C.c_int, C.c_float, C.c_int64, _I64, _D, C.c_int64,
                                 _D, _D, _I64, _D, _D]
     L.or_sq_columns.restype = C.c_int
@@ -256,6 +260,21 @@ def sq_rho(pos, itype, nsp, box, kvecs) -> np.ndarray:
     return out[..., 0] + 1j * out[..., 1]
 
 
+def sq_rho_f64(pos, itype, nsp, box, kvecs) -> np.ndarray:
+    """sq_rho with FP64 phases and OpenMP over vectors: the CPU baseline's
+    timing port."""
+    pos = np.asarray(pos, dtype=np.float64)
+    x, y, z = (np.ascontiguousarray(pos[:, k]) for k in range(3))
+    it = _i32(itype)
+    kv = _i32(kvecs).reshape(-1, 3)
+    b = _f64(box)
+    out = np.zeros((len(kv), nsp, 2))
+    lib().or_sq_rho_f64(len(it), _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double),
+                        _p(it, C.c_int32), nsp, _p(b, C.c_double), len(kv), _p(kv, C.c_int32),
+                        _p(out, C.c_double))
+    return out[..., 0] + 1j * out[..., 1]
+
+
 def sq_frame(pos, itype, ndim, nsp, box, g: SqGrid, bsc, sqf=None, sqfp=None):
     """Accumulate one frame of the reference sum set; pos already wrapped."""
     pos = np.asarray(pos, dtype=np.float64)
@@ -270,6 +289,24 @@ def sq_frame(pos, itype, ndim, nsp, box, g: SqGrid, bsc, sqf=None, sqfp=None):
     lib().or_sq_frame(len(it), _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double),
                       _p(it, C.c_int32), ndim, nsp, _p(b, C.c_double), C.byref(g),
                       _p(bs, C.c_double), _p(sqf, C.c_double), _p(sqfp, C.c_double))
+    return sqf, sqfp
+
+
+def sq_frame_f64(pos, itype, ndim, nsp, box, g: SqGrid, bsc, sqf=None, sqfp=None):
+    """sq_frame with FP64 phases and OpenMP over vectors: the CPU baseline's
+    timing port (same sum set and shells)."""
+    pos = np.asarray(pos, dtype=np.float64)
+    x, y, z = (np.ascontiguousarray(pos[:, k]) for k in range(3))
+    it = _i32(itype)
+    b = _f64(box)
+    bs = _f64(bsc)
+    if sqf is None:
+        sqf = np.zeros(g.nqmax + 1)
+    if sqfp is None:
+        sqfp = np.zeros((nsp, g.nqmax + 1))
+    lib().or_sq_frame_f64(len(it), _p(x, C.c_double), _p(y, C.c_double), _p(z, C.c_double),
+                          _p(it, C.c_int32), ndim, nsp, _p(b, C.c_double), C.byref(g),
+                          _p(bs, C.c_double), _p(sqf, C.c_double), _p(sqfp, C.c_double))
     return sqf, sqfp
 
 

@@ -5,6 +5,7 @@
  * so that every FP64/FP32 operation below rounds exactly once, in the order
  * written.  Reference citations are file:line in the reference repository.
  */
+#define _GNU_SOURCE /* sincos */
 #include "trj_oracle.h"
 
 #include <math.h>
@@ -530,6 +531,60 @@ void or_sq_frame(int64_t n, const double* x, const double* y, const double* z,
   int64_t nv = sum_set(ndim, g, &kv, &sh);
   double* rho = (double*)malloc(sizeof(double) * 2 * nsp * (nv > 0 ? nv : 1));
   or_sq_rho(n, x, y, ndim == 3 ? z : NULL, itype, nsp, box, nv, kv, rho);
+  int ns = g->nqmax + 1;
+  for (int64_t v = 0; v < nv; ++v) {
+    double tr = 0.0, ti = 0.0;
+    for (int s = 0; s < nsp; ++s) {
+      double r = rho[(v * nsp + s) * 2], i = rho[(v * nsp + s) * 2 + 1];
+      sqfp[(int64_t)s * ns + sh[v]] += r * r + i * i;
+      tr += bsc[s] * r;
+      ti += bsc[s] * i;
+    }
+    sqf[sh[v]] += tr * tr + ti * ti;
+  }
+  free(rho);
+  free(kv);
+  free(sh);
+}
+
+/* rho_s(q) of or_sq_rho with FP64 phases (the CPU baseline's timing port):
+ * per vector, the turns n.(r/L) in double, reduced to [0,1), one sincos per
+ * atom, species sums in double; OpenMP over the vectors. */
+void or_sq_rho_f64(int64_t n, const double* x, const double* y, const double* z,
+                   const int32_t* itype, int nsp, const double box[3], int64_t nvec,
+                   const int32_t* kv, double* rho) {
+  const double twopi = 6.283185307179586476925286766559;
+  const double ix = 1.0 / box[0], iy = 1.0 / box[1], iz = z ? 1.0 / box[2] : 0.0;
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int64_t v = 0; v < nvec; ++v) {
+    double re[16], im[16];
+    for (int s = 0; s < nsp; ++s) re[s] = im[s] = 0.0;
+    const double nx = kv[3 * v], ny = kv[3 * v + 1], nz = kv[3 * v + 2];
+    for (int64_t j = 0; j < n; ++j) {
+      double t = nx * (x[j] * ix) + ny * (y[j] * iy) + (z ? nz * (z[j] * iz) : 0.0);
+      t -= floor(t);
+      double sn, cs;
+      sincos(twopi * t, &sn, &cs);
+      const int s = itype[j] - 1;
+      re[s] += cs;
+      im[s] += sn;
+    }
+    for (int s = 0; s < nsp; ++s) {
+      rho[(v * nsp + s) * 2] = re[s];
+      rho[(v * nsp + s) * 2 + 1] = im[s];
+    }
+  }
+}
+
+/* sq.cuf:523-591 and 372-454 as or_sq_frame, with the FP64 phases of
+ * or_sq_rho_f64 (the CPU baseline's timing port). */
+void or_sq_frame_f64(int64_t n, const double* x, const double* y, const double* z,
+                     const int32_t* itype, int ndim, int nsp, const double box[3],
+                     const or_sqgrid* g, const double* bsc, double* sqf, double* sqfp) {
+  int32_t *kv, *sh;
+  int64_t nv = sum_set(ndim, g, &kv, &sh);
+  double* rho = (double*)malloc(sizeof(double) * 2 * nsp * (nv > 0 ? nv : 1));
+  or_sq_rho_f64(n, x, y, ndim == 3 ? z : NULL, itype, nsp, box, nv, kv, rho);
   int ns = g->nqmax + 1;
   for (int64_t v = 0; v < nv; ++v) {
     double tr = 0.0, ti = 0.0;

@@ -26,12 +26,19 @@
  *   or_first_minimum     hand-made curves with a known parabola vertex
  *   or_sq_setup/or_sq_nq golden n(Q) vectors in the reference's shipped outputs
  *                        (examples/3D/LJmix, 3D/salr, 3D/meam-dyn, 2D/salr)
- *   or_sq_member         brute force on tiny grids; shells partition the set
+ *   or_sq_member         exact integer lattice-point counts per shell on a cubic
+ *                        and a 1:2:1 box, away from the FP32 edges
+ *                        (tests/test_oracle_pins.py); brute force on tiny grids
  *   or_sq_rho            single atom |rho|^2 = 1; two atoms 2+2cos(q.dr);
  *                        perfect lattice gives N^2 on reciprocal-lattice q and
  *                        ~0 elsewhere; translation invariance
- *   or_sq_columns        closed form of log.f90:327-399 on hand-made sums;
- *                        binary-mixture identity S_NN = s11+s22+2 s12
+ *   or_sq_columns        perfect lattices: exact Bragg values of S_NN, S_jj and the
+ *                        ordered-alloy S_cc = (N/4) nB3/nq; ideal gas: S_jj -> 1,
+ *                        S_NN -> <b^2>/<b>^2, s11 -> x1, s22 -> x2, s12 -> 0,
+ *                        S_cc -> x1 x2; identical frames with Nconf = k; species
+ *                        relabelling (tests/test_oracle_pins.py)
+ *   or_sq_rho_f64,       the CPU-baseline timing port: equal to or_sq_rho and
+ *   or_sq_frame_f64      or_sq_frame to 1e-12
  *   or_qq_scale          powers of two bracketing max q^2 (hand cases)
  *   or_rdf_qq_brute      unit charges give 2^29 x the or_rdf_brute counts;
  *                        +-1 by species gives c11 + c22 - c12; two-atom closed
@@ -47,6 +54,9 @@
  *                        a split equals the pairs with both atoms in one slice);
  *                        the last-atom count rule on hand cases
  *   or_rdf_xy_accumulate hand-made counts
+ *   or_rdf_xy_gmix       an ideal slab: g_xy -> (2i-1)/(2i) per floor bin i, the
+ *                        value the centred cylindrical shell fixes
+ *                        (tests/test_oracle_pins.py)
  *   or_sq_frame_xy       one slice with every atom equals or_sq_frame (ndim 2)
  *                        divided by N; a single atom per slice gives 1 per vector;
  *                        unit charges give the bsc = 1 sum
@@ -165,6 +175,19 @@ void or_sq_rho(int64_t n, const double* x, const double* y, const double* z,
 void or_sq_frame(int64_t n, const double* x, const double* y, const double* z,
                  const int32_t* itype, int ndim, int nsp, const double box[3],
                  const or_sqgrid* g, const double* bsc, double* sqf, double* sqfp);
+
+/* or_sq_rho with FP64 phases (sincos in double), OpenMP over the vectors: the
+ * CPU-baseline timing port, not a checker. */
+void or_sq_rho_f64(int64_t n, const double* x, const double* y, const double* z,
+                   const int32_t* itype, int nsp, const double box[3], int64_t nvec,
+                   const int32_t* kv, double* rho);
+
+/* or_sq_frame with FP64 phases: sincos of 2 pi frac(n.(r/L)) in double, OpenMP
+ * over the vectors.  The CPU-baseline timing port of the reference's sqfnD /
+ * sqfact sums (bench.py), not a checker. */
+void or_sq_frame_f64(int64_t n, const double* x, const double* y, const double* z,
+                     const int32_t* itype, int ndim, int nsp, const double box[3],
+                     const or_sqgrid* g, const double* bsc, double* sqf, double* sqfp);
 
 /* Charge-weighted sums over the same sum set as or_sq_frame (sq.cuf:424-432,
  * 563-582): sqfq[shell] += |sum_j q_j exp(i 2 pi n.(r_j/L))|^2, long double

@@ -141,3 +141,30 @@ def test_columns_closed_form():
     assert cols.shape == (nqmax, 3)
     bav = (30 * 2.0 + 70 * 0.5) / 100
     assert cols[0, 1] == pytest.approx(sqf[1] / bav ** 2 / (n * 5 * nq[1]))
+
+
+def test_f64_timing_port_equals_long_double_oracle():
+    """or_sq_frame_f64 (FP64 sincos, the CPU baseline's timing port) sums the
+    same set into the same shells as or_sq_frame: equal to 1e-12 relative."""
+    rng = np.random.default_rng(9)
+    L = 12.0
+    pos = rng.random((900, 3)) * L
+    it = rng.integers(1, 3, 900).astype(np.int32)
+    g = O.sq_setup(3, [L] * 3, 7.0, 7.0)
+    a, ap = O.sq_frame(pos, it, 3, 2, [L] * 3, g, [1.0, 0.5])
+    b, bp = O.sq_frame_f64(pos, it, 3, 2, [L] * 3, g, [1.0, 0.5])
+    m = a > 0
+    assert np.max(np.abs(b[m] - a[m]) / a[m]) < 1e-12
+    mp = ap > 0
+    assert np.max(np.abs(bp[mp] - ap[mp]) / ap[mp]) < 1e-12
+
+
+def test_f64_rho_port_equals_long_double_oracle():
+    rng = np.random.default_rng(10)
+    box = np.array([9.0, 10.0, 11.0])
+    pos = rng.random((500, 3)) * box
+    it = rng.integers(1, 4, 500).astype(np.int32)
+    kv = _kv(rng, 60, -9, 10)
+    a = O.sq_rho(pos, it, 3, box, kv)
+    b = O.sq_rho_f64(pos, it, 3, box, kv)
+    assert np.max(np.abs(a - b)) < 1e-11
