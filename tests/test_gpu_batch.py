@@ -73,3 +73,22 @@ def test_batch_rejects_tiny_box():
     with pytest.raises(T.TrjError):
         b.run(fa, pos.data_ptr(), T.TRJ_F32, it.data_ptr())
     b.close()
+
+
+def test_batch_with_million_atom_frame():
+    """A batch holding a ~1M-atom frame between small ones: every frame bit-exact
+    against the per-frame path, the large one also against the oracle's cells."""
+    big = W.uneven_frame(4242, 9.5e5, 1.0e6)
+    frames = [W.uneven_frame(500, 1e4, 3e4), big, W.uneven_frame(501, 1e4, 3e4)]
+    print("case: batch with", [len(f.itype) for f in frames], flush=True)
+    pos, it = _upload(frames)
+    fa = T.batch_frames((len(f.itype), f.nsp, f.box, f.periodic) for f in frames)
+    b = T.Batch(T.batch_params(3, 0.01, 4.0, frames_per_launch=3, use_graph=True))
+    got, _ = b.run(fa, pos.data_ptr(), T.TRJ_F32, it.data_ptr())
+    b.close()
+    want = _per_frame(frames, 0.01, 4.0)
+    for k in range(3):
+        np.testing.assert_array_equal(got[k], want[k])
+    o = O.rdf_counts(big.pos, big.itype, 3, big.nsp, big.box, (1, 1, 1), 4.0, 0.01, method="cells")
+    np.testing.assert_array_equal(got[1], o)
+    print("big frame pairs", int(o.sum()), flush=True)

@@ -249,3 +249,68 @@ def test_2d_shells(precision, tol):
     mp = osqfp > 0
     assert np.max(np.abs(sqfp[mp] - osqfp[mp]) / osqfp[mp]) <= tol
     np.testing.assert_array_equal(nq, O.sq_nq(2, g))
+
+
+def test_lattice_262k_members_and_low_shells():
+    """configs[1] at its stated size: 1200 random member vectors of the
+    reference q-set (plus the axis vectors' shell) against the oracle per
+    vector, and the shell sums of every shell up to 6 (all their vectors
+    recomputed by the oracle) within 1e-12 relative."""
+    f = next(W.lattice_262k(nframes=1))
+    wl = W.LATTICE_262K
+    print("case: lattice_262k 1200 members + low shells", flush=True)
+    g = O.sq_setup(3, f.box, wl.qmax, wl.qmax)
+    rng = np.random.default_rng(2)
+    kv = []
+    while len(kv) < 1200:
+        v = rng.integers(0, g.kext[0], 3)
+        if O.sq_member(3, g, *v) >= 0:
+            kv.append(v)
+    low = [(a, b, c) for a in range(8) for b in range(8) for c in range(8)
+           if 1 <= O.sq_member(3, g, a, b, c) <= 6]
+    kv = np.array(kv + low, dtype=np.int32)
+    with T.Context(0) as ctx:
+        ctx.sq_init(T.sq_params(3, 3, wl.qmax, wl.qmin, wl.bsc), f.box)
+        ctx.frame(f.pos, f.itype, f.box, what=T.TRJ_WANT_SQ)
+        rho = ctx.sq_rho(kv)
+        sqf, sqfp, _ = ctx.sq_shells()
+    orho = O.sq_rho(_wrapped(f), f.itype, 3, f.box, kv)
+    err = np.abs(rho - orho)
+    ns = np.bincount(f.itype, minlength=4)[1:].astype(float)
+    print("members: max |drho| %.3e, 99th pct %.3e, max |rho| %.3e" %
+          (err.max(), np.percentile(err, 99), np.abs(orho).max()), flush=True)
+    # FP64 sums of N_s unit phasors: rounding error below 4 u N_s (u = 2^-53)
+    assert np.all(err < 4 * 2.0 ** -53 * ns[None, :])
+    ol = orho[1200:]
+    sh = np.array([O.sq_member(3, g, *v) for v in low])
+    b = np.asarray(wl.bsc)
+    for s in range(1, 7):
+        m = sh == s
+        want_p = (np.abs(ol[m]) ** 2).sum(axis=0)
+        want = (np.abs(ol[m] @ b) ** 2).sum()
+        rel = abs(sqf[s] - want) / want
+        relp = np.max(np.abs(sqfp[:, s] - want_p) / want_p)
+        print("shell", s, "vectors", int(m.sum()), "rel %.2e, partial rel %.2e" % (rel, relp), flush=True)
+        assert rel <= 1e-12 and relp <= 1e-12
+
+
+def test_uniform_1m_fp64_rho_against_oracle():
+    """The 1M-atom FP64 path (the reference of the precision study) against the
+    long-double oracle on 100 vectors of the 64^3 half-space grid."""
+    f = W.uniform_1m(seed=2)
+    wl = W.UNIFORM_1M
+    rng = np.random.default_rng(4)
+    kv = np.stack([rng.integers(-32, 32, 110), rng.integers(-32, 32, 110), rng.integers(0, 32, 110)], 1)
+    kv = kv[np.any(kv != 0, axis=1)][:100].astype(np.int32)
+    print("case: uniform_1m fp64 rho", flush=True)
+    with T.Context(0) as ctx:
+        ctx.sq_init(T.sq_params(3, 3, wl.qmax, wl.qmin, wl.bsc, qset=T.TRJ_QSET_HALFSPACE, grid=wl.grid,
+                                precision="fp64"), f.box)
+        ctx.frame(f.pos, f.itype, f.box, what=T.TRJ_WANT_SQ)
+        rho = ctx.sq_rho(kv)
+    orho = O.sq_rho(f.pos, f.itype, 3, f.box, kv)
+    err = np.abs(rho - orho)
+    ns = np.bincount(f.itype, minlength=4)[1:].astype(float)
+    print("1M fp64: max |drho| %.3e, max |rho| %.3e" % (err.max(), np.abs(orho).max()), flush=True)
+    # FP64 sums of N_s unit phasors: rounding error below 4 u N_s (u = 2^-53)
+    assert np.all(err < 4 * 2.0 ** -53 * ns[None, :])
