@@ -133,7 +133,8 @@ def bench_batch(nf: int) -> dict:
 
     def per_frame(collect):
         for k, f in enumerate(frames):
-            ctx.rdf_init(T.rdf_params(3, f.nsp, wl.deltar, wl.rcrdf, f.periodic), f.box)
+            ctx.rdf_init(T.rdf_params(3, f.nsp, wl.deltar, wl.rcrdf, f.periodic, flags=T.TRJ_RDF_NO_AUTOTUNE),
+                         f.box)
             ctx.frame_device(pos.data_ptr() + 12 * int(offs[k]), T.TRJ_F32, T.TRJ_AOS,
                              it.data_ptr() + 4 * int(offs[k]), len(f.itype), f.box, T.TRJ_WANT_RDF)
             c = ctx.rdf_counts()

@@ -89,6 +89,9 @@ extern "C" {
                                  the reference's rule is nint(r/deltar).  The
                                  printrdf normalisations assume the reference's
                                  rule. */
+#define TRJ_RDF_NO_AUTOTUNE 4u /* keep the default launch shape of the pair
+                                  kernels instead of timing the candidates on
+                                  the first frame (trj_rdf_tuning) */
 
 typedef struct trj_ctx trj_ctx;
 
@@ -113,9 +116,22 @@ typedef struct {
 
 /* RDF_init (rdf.cuf:51-122).  box: first frame's cell lengths; the cutoff is
  * first capped to half the shortest side (netcdf.f90:610-612).  Returns lsmax
- * and nit = nsp(nsp+1)/2.  Zeroes the accumulated counts. */
+ * and nit = nsp(nsp+1)/2.  Zeroes the accumulated counts.  Where the reference
+ * refuses a histogram larger than the shared memory (rdf.cuf:61-64), the pair
+ * kernels keep it in LDS in tiles of whole pair-type rows, one pass per tile,
+ * as many rows per tile as the device's LDS holds (TRJ_ELIMIT only when one
+ * row does not fit).  The first frame after init times the candidate launch
+ * shapes (rows per tile, wavefronts per home cell) and keeps the fastest. */
 int trj_rdf_init(trj_ctx* ctx, const trj_rdf_params* p, const double box[3],
                  int32_t* lsmax_out, int32_t* nit_out);
+
+/* The RDF autotuner's choice (no counterpart in the reference, whose launch
+ * shape is fixed, rdf.cuf:208-214): histogram rows per LDS tile, passes per
+ * frame, wavefronts per home cell, the chosen shape's time in ms on the
+ * tuning frame and the number of shapes timed (0 before the first frame, or
+ * with TRJ_RDF_NO_AUTOTUNE).  Any argument may be NULL. */
+int trj_rdf_tuning(trj_ctx* ctx, int32_t* rows_per_pass, int32_t* passes, int32_t* waves_per_cell, double* ms,
+                   int32_t* tried);
 
 /* /INPUT_SQ/ (input.f90:70) plus the build's options. */
 typedef struct {

@@ -77,6 +77,11 @@ void launch_cell_scatter(const double* pos, const int32_t* itype, const int32_t*
 
 struct PairArgs {
   int ndim, nsp, nit, lsmax;
+  // Histogram tile of one pass: pair-type rows row0 .. row0+nrows-1 of the
+  // nit x lsmax histogram sit in LDS; pairs of other rows are counted by the
+  // other passes (the launchers loop over them).  To a launcher, nrows is the
+  // rows per pass (0: all nit rows in one pass).
+  int row0, nrows;
   // Axis k takes the image shift of the neighbour cell (no per-pair test):
   // periodic with >= 5 cells, where every pair between two cells has the same
   // nearest image and the FP64 rule d - L*nint(d/L) picks exactly that shift.
@@ -93,9 +98,10 @@ struct PairArgs {
   unsigned long long* gq;
   const double* qp;
 };
-size_t pair_lds_bytes(int nit, int lsmax, bool qq = false);
+// LDS of the all-pairs kernel for a histogram tile of `rows` pair-type rows.
+size_t pair_lds_bytes(int rows, int lsmax, bool qq = false);
 // LDS of the cell-list pair kernel: pair_lds_bytes plus its hit rings.
-size_t cells_lds_bytes(int nit, int lsmax, bool qq = false);
+size_t cells_lds_bytes(int rows, int lsmax, bool qq = false);
 // Cell-list pairs (half stencil); every periodic axis needs nc >= 3.
 void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start,
                       const double* spos, const int32_t* stype, int64_t n, int waves_per_cell,

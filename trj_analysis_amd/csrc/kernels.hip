@@ -23,6 +23,7 @@
 // rounds once, in the order written, like the oracle.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -586,8 +587,8 @@ __device__ inline int pair_accumulate(const PairArgs& a, const double* thr, uint
     const int k = bin_of(r2, thr, a.inv_dr, a.lsmax);
     if (k >= 1 && k <= a.lsmax) {
       const int lo = ti < tj ? ti : tj, hi = ti < tj ? tj : ti;
-      const int ij = pair_index(lo, hi, a.nsp);
-      atomicAdd(&hist[(ij - 1) * a.lsmax + (k - 1)], 1u);
+      const int row = pair_index(lo, hi, a.nsp) - 1 - a.row0;
+      if (static_cast<unsigned>(row) < static_cast<unsigned>(a.nrows)) atomicAdd(&hist[row * a.lsmax + (k - 1)], 1u);
       return k;
     }
   }
@@ -601,16 +602,17 @@ __device__ inline unsigned long long qq_fixed(double qq, double scale) {
 }
 
 __device__ inline void hist_setup(const PairArgs& a, uint32_t* hist, double* thr) {
-  const int nbins = a.nit * a.lsmax;
+  const int nbins = a.nrows * a.lsmax;
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) hist[b] = 0u;
   for (int b = threadIdx.x; b < a.lsmax + 3; b += blockDim.x) thr[b] = a.thr[b];
   __syncthreads();
 }
 
+// The block's tile (rows row0 .. row0+nrows-1) into its nit x lsmax slice.
 __device__ inline void hist_flush(const PairArgs& a, const uint32_t* hist, uint32_t* partial) {
   __syncthreads();
-  const int nbins = a.nit * a.lsmax;
-  uint32_t* out = partial + static_cast<int64_t>(blockIdx.x) * nbins;
+  const int nbins = a.nrows * a.lsmax;
+  uint32_t* out = partial + static_cast<int64_t>(blockIdx.x) * a.nit * a.lsmax + static_cast<int64_t>(a.row0) * a.lsmax;
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) out[b] = hist[b];
 }
 
@@ -645,8 +647,11 @@ __device__ inline void bin_hits(const PairArgs& a, const double* thr, uint32_t* 
     const int k = bin_of(r2, thr, a.inv_dr, a.lsmax);
     if (k >= 1 && k <= a.lsmax) {
       const int lo = ti < tj ? ti : tj, hi = ti < tj ? tj : ti;
-      atomicAdd(&hist[(pair_index(lo, hi, a.nsp) - 1) * a.lsmax + (k - 1)], 1u);
-      if constexpr (QQ) atomicAdd(&gqs[k - 1], qq_fixed(qqr[e], qsc));
+      const int row = pair_index(lo, hi, a.nsp) - 1 - a.row0;
+      if (static_cast<unsigned>(row) < static_cast<unsigned>(a.nrows)) atomicAdd(&hist[row * a.lsmax + (k - 1)], 1u);
+      // the charge sums are per bin, for every pair type: the first pass adds them
+      if constexpr (QQ)
+        if (a.row0 == 0) atomicAdd(&gqs[k - 1], qq_fixed(qqr[e], qsc));
     }
   }
 }
@@ -792,7 +797,7 @@ __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stenc
                                                    int W, int64_t ntasks,
                                                    uint32_t* __restrict__ partial) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int nbins = a.nit * a.lsmax;
+  const int nbins = a.nrows * a.lsmax;  // this pass's tile
   uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
   double* thr = reinterpret_cast<double*>(smem + ((static_cast<size_t>(nbins) * 4 + 15) & ~size_t(15)));
   // Same layout as cells_lds_bytes: hist, thr, the rings, then (QQ) the
@@ -845,7 +850,7 @@ __global__ __launch_bounds__(256) void k_rdf_allpairs(PairArgs a, const double* 
                                                       int64_t stride, const int32_t* __restrict__ range,
                                                       uint32_t* __restrict__ partial) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int nbins = a.nit * a.lsmax;
+  const int nbins = a.nrows * a.lsmax;  // this pass's tile
   uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
   double* thr = reinterpret_cast<double*>(smem + ((static_cast<size_t>(nbins) * 4 + 15) & ~size_t(15)));
   // Same layout as pair_lds_bytes: hist, thr, then (a.q) the block's
@@ -886,7 +891,7 @@ __global__ __launch_bounds__(256) void k_rdf_allpairs(PairArgs a, const double* 
       const bool ok = oki && (jb > ib || j > gi) &&
                       static_cast<unsigned>(tj - 1) < static_cast<unsigned>(a.nsp);
       const int k = pair_accumulate(a, thr, hist, ok, xi, yi, zi, ti, px[j], py[j], pz[j], tj);
-      if (qv && k) atomicAdd(&gqs[k - 1], qq_fixed(qi * qv[j], qsc));
+      if (qv && k && a.row0 == 0) atomicAdd(&gqs[k - 1], qq_fixed(qi * qv[j], qsc));
     }
   }
   hist_flush(a, hist, partial);  // begins with a barrier
@@ -2076,6 +2081,8 @@ __global__ __launch_bounds__(256) void bk_pairs(BatchDev d) {
     }
     a.side2 = F.side2;
     a.inv_dr = F.inv_dr;
+    a.row0 = 0;
+    a.nrows = a.nit;
     const int nbins = a.nit * a.lsmax;
     uint32_t* hist = reinterpret_cast<uint32_t*>(hsm);
     double* thr = reinterpret_cast<double*>(hsm + ((static_cast<size_t>(nbins) * 4 + 15) & ~size_t(15)));
@@ -2256,13 +2263,13 @@ void launch_cell_scatter(const double* pos, const int32_t* itype, const int32_t*
                      cursor, spos, stype, q, sq);
 }
 
-size_t pair_lds_bytes(int nit, int lsmax, bool qq) {
-  const size_t b = ((static_cast<size_t>(nit) * lsmax * 4 + 15) & ~size_t(15)) + static_cast<size_t>(lsmax + 3) * 8;
+size_t pair_lds_bytes(int rows, int lsmax, bool qq) {
+  const size_t b = ((static_cast<size_t>(rows) * lsmax * 4 + 15) & ~size_t(15)) + static_cast<size_t>(lsmax + 3) * 8;
   return qq ? ((b + 15) & ~size_t(15)) + static_cast<size_t>(lsmax) * 8 : b;
 }
 
-size_t cells_lds_bytes(int nit, int lsmax, bool qq) {
-  const size_t base = (pair_lds_bytes(nit, lsmax) + 15) & ~size_t(15);
+size_t cells_lds_bytes(int rows, int lsmax, bool qq) {
+  const size_t base = (pair_lds_bytes(rows, lsmax) + 15) & ~size_t(15);
   const size_t rings = static_cast<size_t>(4) * kHitQ * (sizeof(double) + sizeof(int));
   return base + rings + (qq ? static_cast<size_t>(4) * kHitQ * sizeof(double) + static_cast<size_t>(lsmax) * 8 : 0);
 }
@@ -2291,30 +2298,52 @@ void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start
   bool shift = true;
   for (int k = 0; k < a.ndim; ++k) shift = shift && a.cellshift[k];
   const bool qq = a.q != nullptr;
-  const size_t lds = cells_lds_bytes(a.nit, a.lsmax, qq);
-#define TRJ_CELLS(S, D, Q)                                                                                     \
-  hipLaunchKernelGGL((k_rdf_cells<S, D, Q>), dim3(nblocks), dim3(256), lds, s, a, g, st, start, spos, stype, n, \
-                     waves_per_cell, ntasks, partial)
-  if (qq) {
-    if (a.ndim == 3) {
-      if (shift) TRJ_CELLS(true, true, true); else TRJ_CELLS(false, true, true);
+  // One launch per histogram tile of rows_per_pass pair-type rows.
+  const int rows = a.nrows > 0 ? std::min(a.nrows, a.nit) : a.nit;
+  for (int row0 = 0; row0 < a.nit; row0 += rows) {
+    PairArgs t = a;
+    t.row0 = row0;
+    t.nrows = std::min(rows, a.nit - row0);
+    const size_t lds = cells_lds_bytes(t.nrows, t.lsmax, qq);
+#define TRJ_CELLS(S, D, Q)                                                                                        \
+  do {                                                                                                            \
+    if (lds > 65536)                                                                                              \
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rdf_cells<S, D, Q>),                            \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));              \
+    hipLaunchKernelGGL((k_rdf_cells<S, D, Q>), dim3(nblocks), dim3(256), lds, s, t, g, st, start, spos, stype, n, \
+                       waves_per_cell, ntasks, partial);                                                          \
+  } while (0)
+    if (qq) {
+      if (a.ndim == 3) {
+        if (shift) TRJ_CELLS(true, true, true); else TRJ_CELLS(false, true, true);
+      } else {
+        if (shift) TRJ_CELLS(true, false, true); else TRJ_CELLS(false, false, true);
+      }
     } else {
-      if (shift) TRJ_CELLS(true, false, true); else TRJ_CELLS(false, false, true);
+      if (a.ndim == 3) {
+        if (shift) TRJ_CELLS(true, true, false); else TRJ_CELLS(false, true, false);
+      } else {
+        if (shift) TRJ_CELLS(true, false, false); else TRJ_CELLS(false, false, false);
+      }
     }
-  } else {
-    if (a.ndim == 3) {
-      if (shift) TRJ_CELLS(true, true, false); else TRJ_CELLS(false, true, false);
-    } else {
-      if (shift) TRJ_CELLS(true, false, false); else TRJ_CELLS(false, false, false);
-    }
-  }
 #undef TRJ_CELLS
+  }
 }
 
 void launch_rdf_allpairs(const PairArgs& a, const double* pos, const int32_t* itype, int64_t n,
                          int nblocks, uint32_t* partial, hipStream_t s, int64_t stride, const int32_t* range) {
-  hipLaunchKernelGGL(k_rdf_allpairs, dim3(nblocks), dim3(256), pair_lds_bytes(a.nit, a.lsmax, a.q != nullptr), s,
-                     a, pos, itype, n, stride > 0 ? stride : n, range, partial);
+  const int rows = a.nrows > 0 ? std::min(a.nrows, a.nit) : a.nit;
+  for (int row0 = 0; row0 < a.nit; row0 += rows) {
+    PairArgs t = a;
+    t.row0 = row0;
+    t.nrows = std::min(rows, a.nit - row0);
+    const size_t lds = pair_lds_bytes(t.nrows, t.lsmax, t.q != nullptr);
+    if (lds > 65536)
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rdf_allpairs),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    hipLaunchKernelGGL(k_rdf_allpairs, dim3(nblocks), dim3(256), lds, s, t, pos, itype, n, stride > 0 ? stride : n,
+                       range, partial);
+  }
 }
 
 void launch_slice_keys(const double* pos, int64_t n, const int32_t* itype, int nsp, const SliceSet& ss,

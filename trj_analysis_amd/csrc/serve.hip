@@ -71,7 +71,11 @@ void execute(trj_ctx* ctx, Req& q) {
   int rc = TRJ_OK;
   int32_t ls = 0, nit = 0;
   trj_sq_info info{};
-  if (r.what & TRJ_WANT_RDF) rc = trj_rdf_init(ctx, &r.rdf, r.box, &ls, &nit);
+  // Every request initialises its slot's context afresh: no first-frame
+  // autotuning of the pair kernels (it would time candidates per request).
+  trj_rdf_params rp = r.rdf;
+  rp.flags |= TRJ_RDF_NO_AUTOTUNE;
+  if (r.what & TRJ_WANT_RDF) rc = trj_rdf_init(ctx, &rp, r.box, &ls, &nit);
   if (rc == TRJ_OK && (r.what & TRJ_WANT_SQ)) rc = trj_sq_init(ctx, &r.sq, r.box, &info);
   if (rc == TRJ_OK) rc = trj_frame(ctx, r.pos, r.dtype, TRJ_AOS, r.itype, r.n, r.box, r.what);
   if (rc == TRJ_OK && (r.what & TRJ_WANT_RDF)) {

@@ -114,6 +114,18 @@ struct trj_ctx {
   // Set while trj_frame_device_noalloc runs: the context may then use only
   // buffers that already have their size (ensure() refuses to allocate).
   bool noalloc = false;
+  // Largest LDS allocation of one workgroup on this device.
+  size_t lds_max = 65536;
+  // Launch shape of the RDF pair kernels picked on the first frame after
+  // trj_rdf_init by timing the candidates (histogram rows per LDS tile,
+  // wavefronts per home cell); kept until the next trj_rdf_init.
+  struct {
+    bool enabled = true, done = false;
+    int rows = 0, W = 0;
+    double ms = 0.0;
+    int tried = 0;
+  } rdf_tune;
+  hipEvent_t tune_ev[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -283,6 +295,8 @@ int rdf_slices(trj_ctx* c, PairArgs pa, const FrameGeom& g, const double box[3],
   const int nbins = R.nit * R.lsmax;
   int r;
   pa.ndim = 2;  // x and y: the reference's image rule on both (rdf.cuf:712-715)
+  pa.row0 = 0;
+  pa.nrows = R.rows;
   pa.thr = P<double>(c->thr_xy);
   pa.cellshift[2] = 0;
   const double rc = std::sqrt(g.side2);
@@ -431,7 +445,8 @@ struct RdfCells {
   int nblocks = 1;
 };
 
-RdfCells rdf_cell_plan(int ndim, const double box[3], const int periodic[3], double side2, int64_t n) {
+RdfCells rdf_cell_plan(int ndim, const double box[3], const int periodic[3], double side2, int64_t n,
+                       int W_override = 0) {
   RdfCells p;
   const double rc = std::sqrt(side2);
   for (int k = 0; k < 3; ++k) {
@@ -453,12 +468,83 @@ RdfCells rdf_cell_plan(int ndim, const double box[3], const int periodic[3], dou
   if (p.cells_ok) {
     const double mean = static_cast<double>(n) / static_cast<double>(p.ncell);
     p.W = std::max(1, std::min(static_cast<int>(std::lround(mean / 64.0)), 8));
+    if (W_override > 0) p.W = W_override;
     p.nblocks = static_cast<int>(std::min<int64_t>((p.ncell * p.W + 3) / 4, 2048));
   } else {
     const int64_t nb = (n + 63) / 64;
     p.nblocks = static_cast<int>(std::min<int64_t>((nb * nb + 3) / 4, 1024));
   }
   return p;
+}
+
+// Largest number of pair-type rows (<= nit) whose histogram tile fits
+// `budget` bytes of LDS in the cell-list kernel (0: not even one row).
+int rows_fitting(int nit, int lsmax, bool qq, size_t budget) {
+  int r = nit;
+  while (r > 0 && cells_lds_bytes(r, lsmax, qq) > budget) --r;
+  return r;
+}
+
+// The RDF autotuner: on the first frame after trj_rdf_init, times the pair
+// kernel (all its passes) for each candidate launch shape on this frame's
+// cell list and keeps the fastest in c->rdf_tune.  Candidates: histogram
+// tiles that fit the device's LDS at 1, 2, 3 and 4 workgroups per CU
+// (fewer rows per tile means more passes but more resident workgroups), and
+// the default waves per home cell with one fewer and one more.  Every
+// candidate counts the same pairs, so the choice changes speed only; the
+// trial runs leave nothing behind (the frame's own run follows).
+int tune_pairs(trj_ctx* c, const PairArgs& pa0, const CellGrid& cg, const RdfCells& base, int64_t n,
+               hipStream_t rs) {
+  auto& tu = c->rdf_tune;
+  const RdfPlan& R = c->rdf;
+  std::vector<int> rows;
+  for (int per_cu = 1; per_cu <= 4; ++per_cu) {
+    const int r = rows_fitting(R.nit, R.lsmax, false, c->lds_max / per_cu);
+    if (r >= 1 && std::find(rows.begin(), rows.end(), r) == rows.end()) rows.push_back(r);
+  }
+  std::vector<int> ws = {base.W};
+  if (base.W > 1) ws.push_back(base.W - 1);
+  if (base.W < 8) ws.push_back(base.W + 1);
+  tu.rows = R.rows;
+  tu.W = base.W;
+  tu.tried = 0;
+  if (rows.size() * ws.size() <= 1) {
+    tu.done = true;
+    return TRJ_OK;
+  }
+  const size_t nbins = static_cast<size_t>(R.nit) * R.lsmax;
+  int r;
+  if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * 2048))) return r;
+  const int32_t* start = P<int32_t>(c->cellstart);
+  const double* spos = P<double>(c->spos);
+  const int32_t* stype = P<int32_t>(c->stype);
+  uint32_t* part = P<uint32_t>(c->partial);
+  // one untimed run (code object and caches warm)
+  PairArgs pa = pa0;
+  pa.nrows = rows[0];
+  launch_rdf_cells(pa, cg, start, spos, stype, n, base.W, base.nblocks, part, rs);
+  double best = 1e300;
+  for (int rw : rows)
+    for (int W : ws) {
+      const int nb = static_cast<int>(std::min<int64_t>((base.ncell * W + 3) / 4, 2048));
+      pa.nrows = rw;
+      HIPC(c, hipEventRecord(c->tune_ev[0], rs));
+      launch_rdf_cells(pa, cg, start, spos, stype, n, W, nb, part, rs);
+      HIPC(c, hipEventRecord(c->tune_ev[1], rs));
+      HIPC(c, hipEventSynchronize(c->tune_ev[1]));
+      float ms = 0.f;
+      HIPC(c, hipEventElapsedTime(&ms, c->tune_ev[0], c->tune_ev[1]));
+      ++tu.tried;
+      if (ms < best) {
+        best = ms;
+        tu.rows = rw;
+        tu.W = W;
+      }
+    }
+  HIPC(c, hipGetLastError());
+  tu.ms = best;
+  tu.done = true;
+  return TRJ_OK;
 }
 
 // The dimensionality and species count a frame with these `what` bits is read
@@ -638,12 +724,15 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
       if ((r = rdf_slices(c, pa, g, box, n, itype_dev, rdf_q ? q_dev : nullptr, rs))) return r;
     } else {
       // Cell grid: side >= cutoff on every axis.
-      const RdfCells rcl = rdf_cell_plan(ndim, box, periodic, g.side2, n);
+      auto& tu = c->rdf_tune;
+      RdfCells rcl = rdf_cell_plan(ndim, box, periodic, g.side2, n, tu.done ? tu.W : 0);
       const CellGrid& cg = rcl.cg;
       const bool cells_ok = rcl.cells_ok;
       const int64_t ncell = rcl.ncell;
       for (int k = 0; k < 3; ++k) pa.cellshift[k] = rcl.cellshift[k];
-      const int nblocks = rcl.nblocks;
+      pa.row0 = 0;
+      pa.nrows = tu.done ? tu.rows : R.rows;
+      int nblocks = rcl.nblocks;
       if (cells_ok) {
         if ((r = ensure(c, c->cellid, sizeof(int32_t) * n))) return r;
         if ((r = ensure(c, c->cellcount, sizeof(int32_t) * (ncell + 1)))) return r;
@@ -662,6 +751,12 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
                             P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype),
                             qq_pairs ? q_dev : nullptr, qq_pairs ? P<double>(c->qsort) : nullptr, rs);
         if (qq_pairs) pa.q = P<double>(c->qsort);
+        if (!tu.done && tu.enabled && !c->noalloc && !qq_pairs) {
+          if ((r = tune_pairs(c, pa, cg, rcl, n, rs))) return r;
+          rcl = rdf_cell_plan(ndim, box, periodic, g.side2, n, tu.W);
+          pa.nrows = tu.rows;
+          nblocks = rcl.nblocks;
+        }
         const int W = rcl.W;
         if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
         launch_rdf_cells(pa, cg, P<int32_t>(c->cellstart), P<double>(c->spos), P<int32_t>(c->stype), n, W,
@@ -887,6 +982,14 @@ int trj_create(trj_ctx** out, int device) {
       trj_destroy(c);
       return TRJ_EDEVICE;
     }
+  for (auto& e : c->tune_ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      trj_destroy(c);
+      return TRJ_EDEVICE;
+    }
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+    c->lds_max = static_cast<size_t>(lds);
   // The device address of this device's allocation-free scratch array (a
   // failure is reported by trj_frame_device_noalloc, the only user).
   resolve_scratch(device);
@@ -912,6 +1015,8 @@ void trj_destroy(trj_ctx* c) {
   for (auto& s : c->evs)
     for (auto& e : s.ev)
       if (e) hipEventDestroy(e);
+  for (auto& e : c->tune_ev)
+    if (e) hipEventDestroy(e);
   if (c->rdf_stream) hipStreamDestroy(c->rdf_stream);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -977,12 +1082,20 @@ int trj_rdf_init(trj_ctx* c, const trj_rdf_params* p, const double box[3], int32
   for (int k = 0; k < 3; ++k) R.periodic[k] = (k < p->ndim) ? (p->periodic[k] != 0) : 1;
   R.lsmax = rdf_lsmax(p->ndim, box, p->rcrdf, p->deltar, &R.rc);
   if (R.lsmax < 1) return fail(c, TRJ_EINVAL, "rdf: lsmax < 1");
-  if (p->flags & ~(TRJ_RDF_CHARGE | TRJ_RDF_FLOOR_BINS)) return fail(c, TRJ_EINVAL, "rdf: unknown flags");
+  if (p->flags & ~(TRJ_RDF_CHARGE | TRJ_RDF_FLOOR_BINS | TRJ_RDF_NO_AUTOTUNE))
+    return fail(c, TRJ_EINVAL, "rdf: unknown flags");
   const bool qq = (p->flags & TRJ_RDF_CHARGE) != 0;
-  if (cells_lds_bytes(R.nit, R.lsmax, qq) > kPairLdsLimit)
+  // The histogram sits in LDS in tiles of whole pair-type rows (rdf.cuf:61-64
+  // refuses a grid larger than the shared memory instead): as many rows as
+  // the device's LDS holds per pass, at least one.
+  R.rows = rows_fitting(R.nit, R.lsmax, qq, c->lds_max);
+  if (R.rows < 1)
     return fail(c, TRJ_ELIMIT,
-                "rdf: g(r) grid needs " + std::to_string(cells_lds_bytes(R.nit, R.lsmax, qq)) +
-                    " bytes of LDS, above 65536; increase deltar or reduce rcrdf (rdf.cuf:61-64)");
+                "rdf: one row of the g(r) grid needs " + std::to_string(cells_lds_bytes(1, R.lsmax, qq)) +
+                    " bytes of LDS, above the device's " + std::to_string(c->lds_max) +
+                    "; increase deltar or reduce rcrdf (rdf.cuf:61-64)");
+  c->rdf_tune = {};
+  c->rdf_tune.enabled = (p->flags & TRJ_RDF_NO_AUTOTUNE) == 0;
   const int nsl = c->slices.n;
   if (nsl > 0) {
     if (R.ndim != 3) return fail(c, TRJ_EINVAL, "rdf: slices need ndim = 3");
@@ -1137,6 +1250,20 @@ int trj_frame_device(trj_ctx* c, const void* pos_dev, int32_t dtype, int32_t lay
   return run_frame(c, pos_dev, dtype, layout, itype_dev, n, box, what);
 }
 
+int trj_rdf_tuning(trj_ctx* c, int32_t* rows_per_pass, int32_t* passes, int32_t* waves_per_cell, double* ms,
+                   int32_t* tried) {
+  if (!c) return TRJ_EINVAL;
+  if (!c->rdf_ready) return fail(c, TRJ_ESTATE, "trj_rdf_tuning before trj_rdf_init");
+  const auto& tu = c->rdf_tune;
+  const int rows = tu.done ? tu.rows : c->rdf.rows;
+  if (rows_per_pass) *rows_per_pass = rows;
+  if (passes) *passes = (c->rdf.nit + rows - 1) / rows;
+  if (waves_per_cell) *waves_per_cell = tu.done ? tu.W : 0;
+  if (ms) *ms = tu.ms;
+  if (tried) *tried = tu.tried;
+  return TRJ_OK;
+}
+
 int trj_frame_device_noalloc(trj_ctx* c, const void* pos_dev, int32_t dtype, int32_t layout,
                              const int32_t* itype_dev, int64_t n, const double box_in[3], uint32_t what) {
   if (!c) return TRJ_EINVAL;
@@ -1158,7 +1285,7 @@ int trj_frame_device_noalloc(trj_ctx* c, const void* pos_dev, int32_t dtype, int
   const double box[3] = {box_in[0], box_in[1], ndim == 3 ? box_in[2] : 1.0};
   double rc = R.rc;
   const FrameGeom g = frame_geometry(ndim, box, R.periodic, &rc);
-  const RdfCells cp = rdf_cell_plan(ndim, box, R.periodic, g.side2, n);
+  const RdfCells cp = rdf_cell_plan(ndim, box, R.periodic, g.side2, n, c->rdf_tune.done ? c->rdf_tune.W : 0);
   const size_t nblk = static_cast<size_t>((n + 255) / 256);
   const size_t un = static_cast<size_t>(n), nc = static_cast<size_t>(cp.ncell);
   const size_t cells = cp.cells_ok ? 1 : 0;

@@ -17,6 +17,7 @@ constexpr int kKeyMax = kMaxSlices * kNspMax;  // sort keys: slice * nsp + speci
 // ---------------------------------------------------------------- RDF plan
 struct RdfPlan {
   int ndim = 3, nsp = 1, nit = 1, lsmax = 0;
+  int rows = 0;  // pair-type rows of the histogram per LDS tile (pass) of the pair kernels
   double deltar = 0.0;
   double rc = 0.0;       // sticky capped cutoff (netcdf.f90:610-612)
   int periodic[3] = {1, 1, 1};

@@ -41,7 +41,8 @@ class Job:
 def make_job(f: W.Frame, arrival: float = 0.0, precision: str = "fp16x3") -> Job:
     """One RDF + S(Q) request on frame f: rc = 4, deltar = 0.01, the 64^3
     half-space q-grid, unit scattering lengths."""
-    rdf = T.rdf_params(3, f.nsp, DELTAR, RCRDF, f.periodic)
+    # one frame per request: no first-frame autotuning of the pair kernels
+    rdf = T.rdf_params(3, f.nsp, DELTAR, RCRDF, f.periodic, flags=T.TRJ_RDF_NO_AUTOTUNE)
     sq = T.sq_params(3, f.nsp, 1e9, bsc=[1.0] * f.nsp, qset=T.TRJ_QSET_HALFSPACE, grid=GRID,
                      precision=precision)
     ls, _ = T.rdf_plan(rdf, f.box)
