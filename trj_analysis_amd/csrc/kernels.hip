@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <utility>
 
 #include "../../include/trj_hip.h"
 #include "kernels.h"
@@ -343,16 +344,27 @@ __global__ __launch_bounds__(256) void k_cell_scatter(const double* __restrict__
                                                       double* __restrict__ spos,
                                                       int32_t* __restrict__ stype,
                                                       const double* __restrict__ q,
-                                                      double* __restrict__ sq) {
+                                                      double* __restrict__ sq, float4* __restrict__ sp32,
+                                                      int nsp) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n) return;
   const int c = cellid[i];
   const int slot = start[c] + atomicAdd(&cursor[c], 1);
-  spos[slot] = pos[i];
-  spos[n + slot] = pos[n + i];
-  spos[2 * n + slot] = pos[2 * n + i];
-  stype[slot] = itype[i];
+  const double x = pos[i], y = pos[n + i], z = pos[2 * n + i];
+  const int t = itype[i];
+  spos[slot] = x;
+  spos[n + slot] = y;
+  spos[2 * n + slot] = z;
+  stype[slot] = t;
   if (q) sq[slot] = q[i];
+  // FP32 copy for the pair kernel's prefilter; atoms outside 1..nsp are NaN
+  // there, so no FP32 distance to them passes.
+  if (sp32) {
+    const bool ok = static_cast<unsigned>(t - 1) < static_cast<unsigned>(nsp);
+    const float nanf = __builtin_nanf("");
+    sp32[slot] = ok ? make_float4(static_cast<float>(x), static_cast<float>(y), static_cast<float>(z), 0.f)
+                    : make_float4(nanf, nanf, nanf, 0.f);
+  }
 }
 
 // rdf.cuf:748-752 for i <= j (callers order the pair).
@@ -789,13 +801,177 @@ __device__ inline void rdf_home_chunk(const PairArgs& a, const double* thr, uint
   }
 }
 
+// ------------------------------------------------------------------------
+// Prefiltered pair loop (every axis takes its neighbour cell's image shift).
+// A wavefront takes 16 home atoms, the same 16 in each of its four 16-lane
+// rows, and a batch of 64 neighbours, one per lane (neighbour 16 r + k of
+// the batch in lane k of row r).  Step k broadcasts lane k of every row to
+// its row (DPP row_newbcast, a modifier of the subtraction), so each step
+// tests 64 (home, neighbour) pairs with FP32 arithmetic and no register or
+// memory traffic for the broadcast.  Home coordinates carry the neighbour
+// cell's image shift (added once per cell).  Pairs whose FP32 r2 is below
+// side2_pf (side2 plus a bound on the FP32 rounding: no pair in range is
+// missed) go to the wavefront's ring as (home index, neighbour index); the
+// ring is binned 64 entries at a time through the exact FP64 rule, every
+// lane busy: d = x_j - x_i, minimum image (on these axes the neighbour
+// cell's image shift), r2 < side2, the bin edges.
+constexpr int kRow = 16;
+
+template <int K>
+__device__ inline float row_bcast(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x150 + K, 0xF, 0xF,
+                                                               false));
+}
+
+template <bool D3, bool QQ>
+__device__ inline void bin_hits_pf(const PairArgs& a, const double* thr, uint32_t* hist, const int2* ring,
+                                   uint32_t head, int cnt, int lane, const double* __restrict__ sx,
+                                   const double* __restrict__ sy, const double* __restrict__ sz,
+                                   const int32_t* __restrict__ stype, const double* __restrict__ sq,
+                                   unsigned long long* gqs, double qsc) {
+  if (lane < cnt) {
+    const int2 en = ring[(head + lane) & (kHitQ - 1)];
+    const int gi = en.x, gj = en.y;
+    const double dx = min_image(sx[gj] - sx[gi], a.lmi[0], a.half[0]);
+    const double dy = min_image(sy[gj] - sy[gi], a.lmi[1], a.half[1]);
+    double r2 = dx * dx + dy * dy;
+    if constexpr (D3) {
+      const double dz = min_image(sz[gj] - sz[gi], a.lmi[2], a.half[2]);
+      r2 = r2 + dz * dz;
+    }
+    if (r2 < a.side2) {
+      const int k = bin_of(r2, thr, a.inv_dr, a.lsmax);
+      if (k >= 1 && k <= a.lsmax) {
+        const int ti = stype[gi], tj = stype[gj];
+        const int lo = ti < tj ? ti : tj, hi = ti < tj ? tj : ti;
+        const int row = pair_index(lo, hi, a.nsp) - 1 - a.row0;
+        if (static_cast<unsigned>(row) < static_cast<unsigned>(a.nrows))
+          atomicAdd(&hist[row * a.lsmax + (k - 1)], 1u);
+        if constexpr (QQ)
+          if (a.row0 == 0) atomicAdd(&gqs[k - 1], qq_fixed(sq[gi] * sq[gj], qsc));
+      }
+    }
+  }
+}
+
+struct PfWave {
+  int2* ring;
+  uint32_t tail, fill;  // wave-uniform: entries written so far, entries not yet binned
+};
+
+// Step K of a batch: home lanes against neighbour 16 r + K of the batch.
+template <int K, bool D3, bool SELF, bool QQ>
+__device__ inline void pf_step(const PairArgs& a, const double* thr, uint32_t* hist, PfWave& w, int lane, int jb,
+                               int je, int gi, int jrow, float hx, float hy, float hz, const float4& q,
+                               const double* __restrict__ sx, const double* __restrict__ sy,
+                               const double* __restrict__ sz, const int32_t* __restrict__ stype,
+                               const double* __restrict__ sq, unsigned long long* gqs, double qsc) {
+  if (jb + K >= je) return;  // no row holds a neighbour at K (row 0 fills first)
+  const float dx = row_bcast<K>(q.x) - hx, dy = row_bcast<K>(q.y) - hy;
+  float r2 = fmaf(dy, dy, dx * dx);
+  if constexpr (D3) {
+    const float dz = row_bcast<K>(q.z) - hz;
+    r2 = fmaf(dz, dz, r2);
+  }
+  const int gj = jrow + K;
+  bool hit = r2 < a.side2_pf;
+  if constexpr (SELF) hit = hit && (gj > gi);
+  const unsigned long long m = __ballot(hit);
+  if (hit) {
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+    w.ring[(w.tail + below) & (kHitQ - 1)] = make_int2(gi, gj);
+  }
+  const uint32_t c = static_cast<uint32_t>(__popcll(m));
+  w.tail += c;
+  w.fill += c;
+  if (w.fill >= static_cast<uint32_t>(kWave)) {
+    __builtin_amdgcn_wave_barrier();
+    bin_hits_pf<D3, QQ>(a, thr, hist, w.ring, w.tail - w.fill, kWave, lane, sx, sy, sz, stype, sq, gqs, qsc);
+    __builtin_amdgcn_wave_barrier();
+    w.fill -= kWave;
+  }
+}
+
+template <bool D3, bool SELF, bool QQ, int... K>
+__device__ inline void pf_batch(const PairArgs& a, const double* thr, uint32_t* hist, PfWave& w, int lane, int jb,
+                                int je, int gi, int jrow, float hx, float hy, float hz, const float4& q,
+                                const double* __restrict__ sx, const double* __restrict__ sy,
+                                const double* __restrict__ sz, const int32_t* __restrict__ stype,
+                                const double* __restrict__ sq, unsigned long long* gqs, double qsc,
+                                std::integer_sequence<int, K...>) {
+  (pf_step<K, D3, SELF, QQ>(a, thr, hist, w, lane, jb, je, gi, jrow, hx, hy, hz, q, sx, sy, sz, stype, sq, gqs, qsc),
+   ...);
+}
+
+template <bool D3, bool SELF, bool QQ>
+__device__ inline void rdf_neighbours_pf(const PairArgs& a, const double* thr, uint32_t* hist, PfWave& w, int lane,
+                                         int js, int je, int gi, float hx, float hy, float hz,
+                                         const float4* __restrict__ sp32, const double* __restrict__ sx,
+                                         const double* __restrict__ sy, const double* __restrict__ sz,
+                                         const int32_t* __restrict__ stype, const double* __restrict__ sq,
+                                         unsigned long long* gqs, double qsc) {
+  const float fnan = __builtin_nanf("");
+  const int r16 = (lane >> 4) * kRow;
+  for (int jb = js; jb < je; jb += kWave) {
+    // lane r*16 + k holds neighbour jb + r*16 + k (NaN past the cell)
+    const int jl = jb + r16 + (lane & (kRow - 1));
+    const float4 q = jl < je ? sp32[jl] : make_float4(fnan, fnan, fnan, 0.f);
+    pf_batch<D3, SELF, QQ>(a, thr, hist, w, lane, jb, je, gi, jb + r16, hx, hy, hz, q, sx, sy, sz, stype, sq, gqs,
+                           qsc, std::make_integer_sequence<int, kRow>{});
+  }
+}
+
+template <bool D3, bool QQ>
+__device__ inline void rdf_home_chunk_pf(const PairArgs& a, const double* thr, uint32_t* hist, PfWave& w, int lane,
+                                         const Stencil& st, const CellGrid& g, int cellL, int ch, int cs, int ce,
+                                         const int32_t* __restrict__ start, const float4* __restrict__ sp32,
+                                         const double* __restrict__ sx, const double* __restrict__ sy,
+                                         const double* __restrict__ sz, const int32_t* __restrict__ stype,
+                                         const double* __restrict__ sq, unsigned long long* gqs, double qsc) {
+  const float fnan = __builtin_nanf("");
+  const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
+  const int cx = cellL % ncx, cy = (cellL / ncx) % ncy, cz = cellL / (ncx * ncy);
+  const int h0 = cs + ch * kRow;
+  const int gi = h0 + (lane & (kRow - 1));  // the same 16 home atoms in every row
+  const float4 pi = gi < ce ? sp32[gi] : make_float4(fnan, fnan, fnan, 0.f);
+  for (int o = 0; o < st.n; ++o) {
+    int nx = cx + st.off[o][0], ny = cy + st.off[o][1], nz = cz + st.off[o][2];
+    // s = +1 below cell 0 (d ~ +L: subtract +L), -1 above the top cell
+    const int sxi = nx < 0 ? 1 : (nx >= ncx ? -1 : 0);
+    const int syi = ny < 0 ? 1 : (ny >= ncy ? -1 : 0);
+    const int szi = D3 ? (nz < 0 ? 1 : (nz >= ncz ? -1 : 0)) : 0;
+    if (g.periodic[0]) nx = (nx + ncx) % ncx; else if (nx < 0 || nx >= ncx) continue;
+    if (g.periodic[1]) ny = (ny + ncy) % ncy; else if (ny < 0 || ny >= ncy) continue;
+    if (D3) {
+      if (g.periodic[2]) nz = (nz + ncz) % ncz; else if (nz < 0 || nz >= ncz) continue;
+    }
+    // the home coordinates moved by the image shift (d = x_j - (x_i + s))
+    const float hx = pi.x + static_cast<float>(static_cast<double>(sxi) * a.lmi[0]);
+    const float hy = pi.y + static_cast<float>(static_cast<double>(syi) * a.lmi[1]);
+    const float hz = D3 ? pi.z + static_cast<float>(static_cast<double>(szi) * a.lmi[2]) : 0.f;
+    if (o == 0) {
+      // own cell: j > i; neighbours below the chunk are all below its atoms
+      rdf_neighbours_pf<D3, true, QQ>(a, thr, hist, w, lane, h0, ce, gi, hx, hy, hz, sp32, sx, sy, sz, stype, sq,
+                                      gqs, qsc);
+    } else {
+      const int nb = (nz * ncy + ny) * ncx + nx;
+      const int js = __builtin_amdgcn_readfirstlane(start[nb]);
+      const int je = __builtin_amdgcn_readfirstlane(start[nb + 1]);
+      rdf_neighbours_pf<D3, false, QQ>(a, thr, hist, w, lane, js, je, gi, hx, hy, hz, sp32, sx, sy, sz, stype, sq,
+                                       gqs, qsc);
+    }
+  }
+}
+
 template <bool SHIFT, bool D3, bool QQ>
 __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stencil st,
                                                    const int32_t* __restrict__ start,
                                                    const double* __restrict__ spos,
                                                    const int32_t* __restrict__ stype, int64_t n,
                                                    int W, int64_t ntasks,
-                                                   uint32_t* __restrict__ partial) {
+                                                   uint32_t* __restrict__ partial,
+                                                   const float4* __restrict__ sp32) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int nbins = a.nrows * a.lsmax;  // this pass's tile
   uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
@@ -814,7 +990,8 @@ __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stenc
     for (int b = threadIdx.x; b < a.lsmax; b += blockDim.x) gqs[b] = 0ull;
   }
   hist_setup(a, hist, thr);
-  const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  // wave-uniform (so are the task loop and the ring counters it carries)
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
   double* qr2 = reinterpret_cast<double*>(smem + qoff) + wave * kHitQ;
   int* qtt = reinterpret_cast<int*>(smem + qoff + 4 * kHitQ * sizeof(double)) + wave * kHitQ;
   if constexpr (QQ) qqr = reinterpret_cast<double*>(smem + qqoff) + wave * kHitQ;
@@ -822,18 +999,31 @@ __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stenc
   const double* __restrict__ sx = spos;
   const double* __restrict__ sy = spos + n;
   const double* __restrict__ sz = spos + 2 * n;
+  // Prefilter path: (home, neighbour) entries in the space of the (r2, type)
+  // ring; home chunks of 16 atoms.
+  PfWave pw{reinterpret_cast<int2*>(qr2), 0u, 0u};
+  const int chunk = SHIFT ? kRow : kWave;
   for (int64_t task = static_cast<int64_t>(blockIdx.x) * 4 + wave; task < ntasks;
        task += static_cast<int64_t>(gridDim.x) * 4) {
     const int cell = static_cast<int>(task / W);
     const int ch0 = static_cast<int>(task % W);
     const int cs = __builtin_amdgcn_readfirstlane(start[cell]);
     const int ce = __builtin_amdgcn_readfirstlane(start[cell + 1]);
-    for (int ch = ch0; cs + ch * kWave < ce; ch += W)
-      rdf_home_chunk<SHIFT, D3, QQ>(a, thr, hist, qr2, qtt, head, tail, lane, st, g.nc, g.periodic, 0, cell, ch,
-                                    cs, ce, start, sx, sy, sz, stype, a.q, qqr, gqs, qsc);
+    for (int ch = ch0; cs + ch * chunk < ce; ch += W) {
+      if constexpr (SHIFT)
+        rdf_home_chunk_pf<D3, QQ>(a, thr, hist, pw, lane, st, g, cell, ch, cs, ce, start, sp32, sx, sy, sz, stype,
+                                  a.q, gqs, qsc);
+      else
+        rdf_home_chunk<SHIFT, D3, QQ>(a, thr, hist, qr2, qtt, head, tail, lane, st, g.nc, g.periodic, 0, cell, ch,
+                                      cs, ce, start, sx, sy, sz, stype, a.q, qqr, gqs, qsc);
+    }
   }
   __builtin_amdgcn_wave_barrier();
-  bin_hits<QQ>(a, thr, hist, qr2, qtt, head, static_cast<int>(tail - head), lane, qqr, gqs, qsc);
+  if constexpr (SHIFT)
+    bin_hits_pf<D3, QQ>(a, thr, hist, pw.ring, pw.tail - pw.fill, static_cast<int>(pw.fill), lane, sx, sy, sz,
+                        stype, a.q, gqs, qsc);
+  else
+    bin_hits<QQ>(a, thr, hist, qr2, qtt, head, static_cast<int>(tail - head), lane, qqr, gqs, qsc);
   hist_flush(a, hist, partial);  // begins with a barrier: every bin_hits is done
   if constexpr (QQ) {
     for (int b = threadIdx.x; b < a.lsmax; b += blockDim.x) {
@@ -862,7 +1052,7 @@ __global__ __launch_bounds__(256) void k_rdf_allpairs(PairArgs a, const double* 
     for (int b = threadIdx.x; b < a.lsmax; b += blockDim.x) gqs[b] = 0ull;
   const double qsc = a.q ? a.qp[0] : 0.0;  // the frame's fixed-point scale
   hist_setup(a, hist, thr);
-  const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
   const int64_t base = range ? range[0] : 0;
   if (range) n = range[1] - range[0];
   const int nb = static_cast<int>((n + kWave - 1) / kWave);
@@ -2043,7 +2233,7 @@ __global__ __launch_bounds__(256) void bk_pairs(BatchDev d) {
   int64_t t0, t1;
   block_chunk(T, blockIdx.x, gridDim.x, &t0, &t1);
   if (t0 >= t1) return;
-  const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
   double* qr2 = reinterpret_cast<double*>(smem) + wave * kHitQ;
   int* qtt = reinterpret_cast<int*>(smem + 4 * kHitQ * sizeof(double)) + wave * kHitQ;
   unsigned char* hsm = smem + kRingBytes;
@@ -2257,10 +2447,10 @@ void launch_cell_count(const double* pos, int64_t n, int ndim, const CellGrid& g
 
 void launch_cell_scatter(const double* pos, const int32_t* itype, const int32_t* cellid, int64_t n,
                          const int32_t* start, int32_t* cursor, double* spos, int32_t* stype, const double* q,
-                         double* sq, hipStream_t s) {
+                         double* sq, hipStream_t s, float4* sp32, int nsp) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_cell_scatter, dim3(grid1d(n, 256)), dim3(256), 0, s, pos, itype, cellid, n, start,
-                     cursor, spos, stype, q, sq);
+                     cursor, spos, stype, q, sq, sp32, nsp);
 }
 
 size_t pair_lds_bytes(int rows, int lsmax, bool qq) {
@@ -2271,12 +2461,14 @@ size_t pair_lds_bytes(int rows, int lsmax, bool qq) {
 size_t cells_lds_bytes(int rows, int lsmax, bool qq) {
   const size_t base = (pair_lds_bytes(rows, lsmax) + 15) & ~size_t(15);
   const size_t rings = static_cast<size_t>(4) * kHitQ * (sizeof(double) + sizeof(int));
-  return base + rings + (qq ? static_cast<size_t>(4) * kHitQ * sizeof(double) + static_cast<size_t>(lsmax) * 8 : 0);
+  return base + rings +
+         (qq ? static_cast<size_t>(4) * kHitQ * sizeof(double) + ((static_cast<size_t>(lsmax) * 8 + 15) & ~size_t(15))
+             : 0);
 }
 
 void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start, const double* spos,
                       const int32_t* stype, int64_t n, int waves_per_cell, int nblocks,
-                      uint32_t* partial, hipStream_t s) {
+                      uint32_t* partial, hipStream_t s, const float4* sp32) {
   Stencil st{};
   int k = 0;
   st.off[k][0] = st.off[k][1] = st.off[k][2] = 0;
@@ -2298,10 +2490,33 @@ void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start
   bool shift = true;
   for (int k = 0; k < a.ndim; ++k) shift = shift && a.cellshift[k];
   const bool qq = a.q != nullptr;
+  // FP32 prefilter limit (image-shift axes): side2 plus a bound on how far the
+  // FP32 r2 of an in-range pair can exceed its FP64 r2.  Per axis, with X the
+  // largest |coordinate| (L, or 5L/4 on a non-periodic axis, whose atoms may
+  // lie in [-L/4, 5L/4)) and S the image shift (L on a periodic axis, else
+  // 0): x_j and x_i rounded to FP32 (< 2^-24 X each), the shift (< 2^-24 S),
+  // x_i + s (< 2^-24 (X + S)), the difference (< 2^-24 (rc + 2e)); so
+  // e < 2^-23 (3X + 2S + rc), and sum_k (2 rc e_k + e_k^2) bounds the change
+  // of r2, with 4 ulp for the FP32 squares and sums of r2 itself.
+  float lim_pf = 0.f;
+  if (shift) {
+    const double rc = std::sqrt(a.side2);
+    double err = 0.0;
+    for (int k = 0; k < a.ndim; ++k) {
+      const double L = g.periodic[k] ? a.lmi[k] : a.lmi[k] / 2.5;
+      const double X = g.periodic[k] ? L : 1.25 * L;
+      const double S = g.periodic[k] ? L : 0.0;
+      const double e = std::ldexp(3.0 * X + 2.0 * S + rc, -23);
+      err += 2.0 * rc * e + e * e;
+    }
+    const double lim = (a.side2 + err + std::ldexp(4.0 * a.side2, -23)) * (1.0 + std::ldexp(1.0, -20));
+    lim_pf = std::nextafter(static_cast<float>(lim), INFINITY);
+  }
   // One launch per histogram tile of rows_per_pass pair-type rows.
   const int rows = a.nrows > 0 ? std::min(a.nrows, a.nit) : a.nit;
   for (int row0 = 0; row0 < a.nit; row0 += rows) {
     PairArgs t = a;
+    t.side2_pf = lim_pf;
     t.row0 = row0;
     t.nrows = std::min(rows, a.nit - row0);
     const size_t lds = cells_lds_bytes(t.nrows, t.lsmax, qq);
@@ -2311,7 +2526,7 @@ void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start
       (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rdf_cells<S, D, Q>),                            \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));              \
     hipLaunchKernelGGL((k_rdf_cells<S, D, Q>), dim3(nblocks), dim3(256), lds, s, t, g, st, start, spos, stype, n, \
-                       waves_per_cell, ntasks, partial);                                                          \
+                       waves_per_cell, ntasks, partial, sp32);                                                    \
   } while (0)
     if (qq) {
       if (a.ndim == 3) {
