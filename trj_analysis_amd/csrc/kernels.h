@@ -71,12 +71,9 @@ void launch_cell_count(const double* pos, int64_t n, int ndim, const CellGrid& g
 void launch_cell_count_slices(const double* pos, const int32_t* key, int64_t n, int nsp, const CellGrid& g,
                               int nslice, int32_t* cellid, int32_t* count, hipStream_t s);
 // q (may be NULL): per-atom charges, scattered into cell order as sq.
-// sp32 (may be NULL): FP32 copy of the cell-sorted positions (x, y, z, 0) for
-// the pair kernel's prefilter, NaN for atoms whose type is outside 1..nsp.
 void launch_cell_scatter(const double* pos, const int32_t* itype, const int32_t* cellid,
                          int64_t n, const int32_t* start, int32_t* cursor, double* spos,
-                         int32_t* stype, const double* q, double* sq, hipStream_t s,
-                         float4* sp32 = nullptr, int nsp = 0);
+                         int32_t* stype, const double* q, double* sq, hipStream_t s);
 
 struct PairArgs {
   int ndim, nsp, nit, lsmax;
@@ -93,10 +90,11 @@ struct PairArgs {
   double lmi[3], half[3];
   double side2;
   // FP32 prefilter of the cell-list kernel when every axis takes the image
-  // shift: a pair whose FP64 r2 is below side2 has an FP32 r2 (from the FP32
-  // copy of the positions) below side2_pf, set by launch_rdf_cells from a
-  // bound on the FP32 rounding.
+  // shift, set by launch_rdf_cells from a bound on the FP32 rounding: the
+  // FP32 r2 of a pair (from coordinates relative to the home cell's corner)
+  // is within pf_delta of its FP64 r2, and side2_pf >= side2 + pf_delta.
   float side2_pf;
+  double pf_delta;
   float inv_dr;
   const double* thr;  // lsmax+3 edges
   // Charge-weighted histogram (NULL q: off): q in the pair kernel's atom
@@ -111,12 +109,13 @@ struct PairArgs {
 size_t pair_lds_bytes(int rows, int lsmax, bool qq = false);
 // LDS of the cell-list pair kernel: pair_lds_bytes plus its hit rings.
 size_t cells_lds_bytes(int rows, int lsmax, bool qq = false);
-// Cell-list pairs (half stencil); every periodic axis needs nc >= 3.  With
-// the image shift on every axis (cellshift) the FP32 copy sp32 of the
-// cell-sorted positions (launch_cell_scatter) is required.
+// Wavefronts per workgroup of the cell-list pair kernel: eight share one
+// histogram tile (LDS per wavefront, and so residency, as for its VGPRs).
+constexpr int kCellWaves = 8;
+// Cell-list pairs (half stencil); every periodic axis needs nc >= 3.
 void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start,
                       const double* spos, const int32_t* stype, int64_t n, int waves_per_cell,
-                      int nblocks, uint32_t* partial, hipStream_t s, const float4* sp32 = nullptr);
+                      int nblocks, uint32_t* partial, hipStream_t s);
 // All pairs i<j (small boxes) of n atoms in SoA planes of `stride` (0: n);
 // range (device, may be NULL): the atoms [range[0], range[1]) instead.
 void launch_rdf_allpairs(const PairArgs& a, const double* pos, const int32_t* itype, int64_t n,

@@ -344,8 +344,7 @@ __global__ __launch_bounds__(256) void k_cell_scatter(const double* __restrict__
                                                       double* __restrict__ spos,
                                                       int32_t* __restrict__ stype,
                                                       const double* __restrict__ q,
-                                                      double* __restrict__ sq, float4* __restrict__ sp32,
-                                                      int nsp) {
+                                                      double* __restrict__ sq) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n) return;
   const int c = cellid[i];
@@ -357,14 +356,6 @@ __global__ __launch_bounds__(256) void k_cell_scatter(const double* __restrict__
   spos[2 * n + slot] = z;
   stype[slot] = t;
   if (q) sq[slot] = q[i];
-  // FP32 copy for the pair kernel's prefilter; atoms outside 1..nsp are NaN
-  // there, so no FP32 distance to them passes.
-  if (sp32) {
-    const bool ok = static_cast<unsigned>(t - 1) < static_cast<unsigned>(nsp);
-    const float nanf = __builtin_nanf("");
-    sp32[slot] = ok ? make_float4(static_cast<float>(x), static_cast<float>(y), static_cast<float>(z), 0.f)
-                    : make_float4(nanf, nanf, nanf, 0.f);
-  }
 }
 
 // rdf.cuf:748-752 for i <= j (callers order the pair).
@@ -805,16 +796,22 @@ __device__ inline void rdf_home_chunk(const PairArgs& a, const double* thr, uint
 // Prefiltered pair loop (every axis takes its neighbour cell's image shift).
 // A wavefront takes 16 home atoms, the same 16 in each of its four 16-lane
 // rows, and a batch of 64 neighbours, one per lane (neighbour 16 r + k of
-// the batch in lane k of row r).  Step k broadcasts lane k of every row to
-// its row (DPP row_newbcast, a modifier of the subtraction), so each step
-// tests 64 (home, neighbour) pairs with FP32 arithmetic and no register or
-// memory traffic for the broadcast.  Home coordinates carry the neighbour
-// cell's image shift (added once per cell).  Pairs whose FP32 r2 is below
-// side2_pf (side2 plus a bound on the FP32 rounding: no pair in range is
-// missed) go to the wavefront's ring as (home index, neighbour index); the
-// ring is binned 64 entries at a time through the exact FP64 rule, every
-// lane busy: d = x_j - x_i, minimum image (on these axes the neighbour
-// cell's image shift), r2 < side2, the bin edges.
+// the batch in lane k of row r).  Coordinates are FP32 relative to the home
+// cell's corner: x - corner for the home atoms, x - (corner + s) for the
+// neighbours (s the neighbour cell's image shift), each difference in FP64
+// and rounded once.  Step k broadcasts lane k of every row to its row (DPP
+// row_newbcast, a modifier of the subtraction), so each step tests 64
+// (home, neighbour) pairs in FP32 with no register or memory traffic for the
+// broadcast.  |FP32 r2 - FP64 r2| < pf_delta (a bound on the rounding, from
+// the host), so pairs with FP32 r2 below side2_pf >= side2 + pf_delta cover
+// every pair in range.  They go to the wavefront's ring as (home index,
+// neighbour index, FP32 r2); the ring is binned 64 entries at a time, every
+// lane busy.  An entry whose FP32 r2 +- pf_delta lies inside one bin and
+// below side2 has that bin under the exact FP64 rule (edges moved by the
+// bound, in LDS as FP32: lo[k] >= thr[k] + pf_delta, hi[k] <= thr[k] -
+// pf_delta); the others (near an edge) go through the FP64 rule itself:
+// d = x_j - x_i, minimum image (on these axes the neighbour cell's image
+// shift), r2 < side2, the bin edges.
 constexpr int kRow = 16;
 
 template <int K>
@@ -823,47 +820,58 @@ __device__ inline float row_bcast(float v) {
                                                                false));
 }
 
+struct PfTables {
+  const float* lo;  // lo[k] >= thr[k] + pf_delta
+  const float* hi;  // hi[k] <= thr[k] - pf_delta
+  float cut;        // cut <= side2 - pf_delta
+};
+
 template <bool D3, bool QQ>
-__device__ inline void bin_hits_pf(const PairArgs& a, const double* thr, uint32_t* hist, const int2* ring,
-                                   uint32_t head, int cnt, int lane, const double* __restrict__ sx,
+__device__ inline void bin_hits_pf(const PairArgs& a, const double* thr, const PfTables& tb, uint32_t* hist,
+                                   const int4* ring, uint32_t head, int cnt, int lane, const double* __restrict__ sx,
                                    const double* __restrict__ sy, const double* __restrict__ sz,
                                    const int32_t* __restrict__ stype, const double* __restrict__ sq,
                                    unsigned long long* gqs, double qsc) {
   if (lane < cnt) {
-    const int2 en = ring[(head + lane) & (kHitQ - 1)];
+    const int4 en = ring[(head + lane) & (kHitQ - 1)];
     const int gi = en.x, gj = en.y;
-    const double dx = min_image(sx[gj] - sx[gi], a.lmi[0], a.half[0]);
-    const double dy = min_image(sy[gj] - sy[gi], a.lmi[1], a.half[1]);
-    double r2 = dx * dx + dy * dy;
-    if constexpr (D3) {
-      const double dz = min_image(sz[gj] - sz[gi], a.lmi[2], a.half[2]);
-      r2 = r2 + dz * dz;
-    }
-    if (r2 < a.side2) {
-      const int k = bin_of(r2, thr, a.inv_dr, a.lsmax);
-      if (k >= 1 && k <= a.lsmax) {
-        const int ti = stype[gi], tj = stype[gj];
-        const int lo = ti < tj ? ti : tj, hi = ti < tj ? tj : ti;
-        const int row = pair_index(lo, hi, a.nsp) - 1 - a.row0;
-        if (static_cast<unsigned>(row) < static_cast<unsigned>(a.nrows))
-          atomicAdd(&hist[row * a.lsmax + (k - 1)], 1u);
-        if constexpr (QQ)
-          if (a.row0 == 0) atomicAdd(&gqs[k - 1], qq_fixed(sq[gi] * sq[gj], qsc));
+    const float r2f = __int_as_float(en.z);
+    // an estimate (the hardware square root): a wrong bin fails the test below
+    int k = static_cast<int>(__builtin_amdgcn_sqrtf(r2f) * a.inv_dr + 0.5f);
+    k = k > a.lsmax + 1 ? a.lsmax + 1 : k;
+    const bool sure = r2f < tb.cut && r2f >= tb.lo[k] && r2f < tb.hi[k + 1];
+    if (!sure) {
+      // near an edge: the FP64 rule
+      const double dx = min_image(sx[gj] - sx[gi], a.lmi[0], a.half[0]);
+      const double dy = min_image(sy[gj] - sy[gi], a.lmi[1], a.half[1]);
+      double r2 = dx * dx + dy * dy;
+      if constexpr (D3) {
+        const double dz = min_image(sz[gj] - sz[gi], a.lmi[2], a.half[2]);
+        r2 = r2 + dz * dz;
       }
+      k = r2 < a.side2 ? bin_of(r2, thr, a.inv_dr, a.lsmax) : 0;
+    }
+    if (k >= 1 && k <= a.lsmax) {
+      const int ti = stype[gi], tj = stype[gj];
+      const int lo = ti < tj ? ti : tj, hi = ti < tj ? tj : ti;
+      const int row = pair_index(lo, hi, a.nsp) - 1 - a.row0;
+      if (static_cast<unsigned>(row) < static_cast<unsigned>(a.nrows)) atomicAdd(&hist[row * a.lsmax + (k - 1)], 1u);
+      if constexpr (QQ)
+        if (a.row0 == 0) atomicAdd(&gqs[k - 1], qq_fixed(sq[gi] * sq[gj], qsc));
     }
   }
 }
 
 struct PfWave {
-  int2* ring;
+  int4* ring;
   uint32_t tail, fill;  // wave-uniform: entries written so far, entries not yet binned
 };
 
 // Step K of a batch: home lanes against neighbour 16 r + K of the batch.
 template <int K, bool D3, bool SELF, bool QQ>
-__device__ inline void pf_step(const PairArgs& a, const double* thr, uint32_t* hist, PfWave& w, int lane, int jb,
-                               int je, int gi, int jrow, float hx, float hy, float hz, const float4& q,
-                               const double* __restrict__ sx, const double* __restrict__ sy,
+__device__ inline void pf_step(const PairArgs& a, const double* thr, const PfTables& tb, uint32_t* hist, PfWave& w,
+                               int lane, int jb, int je, int gi, int jrow, float hx, float hy, float hz,
+                               const float4& q, const double* __restrict__ sx, const double* __restrict__ sy,
                                const double* __restrict__ sz, const int32_t* __restrict__ stype,
                                const double* __restrict__ sq, unsigned long long* gqs, double qsc) {
   if (jb + K >= je) return;  // no row holds a neighbour at K (row 0 fills first)
@@ -880,61 +888,78 @@ __device__ inline void pf_step(const PairArgs& a, const double* thr, uint32_t* h
   if (hit) {
     const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
-    w.ring[(w.tail + below) & (kHitQ - 1)] = make_int2(gi, gj);
+    w.ring[(w.tail + below) & (kHitQ - 1)] = make_int4(gi, gj, __float_as_int(r2), 0);
   }
   const uint32_t c = static_cast<uint32_t>(__popcll(m));
   w.tail += c;
   w.fill += c;
   if (w.fill >= static_cast<uint32_t>(kWave)) {
     __builtin_amdgcn_wave_barrier();
-    bin_hits_pf<D3, QQ>(a, thr, hist, w.ring, w.tail - w.fill, kWave, lane, sx, sy, sz, stype, sq, gqs, qsc);
+    bin_hits_pf<D3, QQ>(a, thr, tb, hist, w.ring, w.tail - w.fill, kWave, lane, sx, sy, sz, stype, sq, gqs, qsc);
     __builtin_amdgcn_wave_barrier();
     w.fill -= kWave;
   }
 }
 
 template <bool D3, bool SELF, bool QQ, int... K>
-__device__ inline void pf_batch(const PairArgs& a, const double* thr, uint32_t* hist, PfWave& w, int lane, int jb,
-                                int je, int gi, int jrow, float hx, float hy, float hz, const float4& q,
-                                const double* __restrict__ sx, const double* __restrict__ sy,
+__device__ inline void pf_batch(const PairArgs& a, const double* thr, const PfTables& tb, uint32_t* hist, PfWave& w,
+                                int lane, int jb, int je, int gi, int jrow, float hx, float hy, float hz,
+                                const float4& q, const double* __restrict__ sx, const double* __restrict__ sy,
                                 const double* __restrict__ sz, const int32_t* __restrict__ stype,
                                 const double* __restrict__ sq, unsigned long long* gqs, double qsc,
                                 std::integer_sequence<int, K...>) {
-  (pf_step<K, D3, SELF, QQ>(a, thr, hist, w, lane, jb, je, gi, jrow, hx, hy, hz, q, sx, sy, sz, stype, sq, gqs, qsc),
+  (pf_step<K, D3, SELF, QQ>(a, thr, tb, hist, w, lane, jb, je, gi, jrow, hx, hy, hz, q, sx, sy, sz, stype, sq, gqs,
+                            qsc),
    ...);
 }
 
+// Neighbours [js, je) of one cell; (ox, oy, oz) = corner + image shift.
 template <bool D3, bool SELF, bool QQ>
-__device__ inline void rdf_neighbours_pf(const PairArgs& a, const double* thr, uint32_t* hist, PfWave& w, int lane,
-                                         int js, int je, int gi, float hx, float hy, float hz,
-                                         const float4* __restrict__ sp32, const double* __restrict__ sx,
+__device__ inline void rdf_neighbours_pf(const PairArgs& a, const double* thr, const PfTables& tb, uint32_t* hist,
+                                         PfWave& w, int lane, int js, int je, int gi, float hx, float hy, float hz,
+                                         double ox, double oy, double oz, const double* __restrict__ sx,
                                          const double* __restrict__ sy, const double* __restrict__ sz,
                                          const int32_t* __restrict__ stype, const double* __restrict__ sq,
                                          unsigned long long* gqs, double qsc) {
   const float fnan = __builtin_nanf("");
   const int r16 = (lane >> 4) * kRow;
   for (int jb = js; jb < je; jb += kWave) {
-    // lane r*16 + k holds neighbour jb + r*16 + k (NaN past the cell)
+    // lane r*16 + k holds neighbour jb + r*16 + k (NaN past the cell or for a
+    // type outside 1..nsp: no FP32 distance to it passes)
     const int jl = jb + r16 + (lane & (kRow - 1));
-    const float4 q = jl < je ? sp32[jl] : make_float4(fnan, fnan, fnan, 0.f);
-    pf_batch<D3, SELF, QQ>(a, thr, hist, w, lane, jb, je, gi, jb + r16, hx, hy, hz, q, sx, sy, sz, stype, sq, gqs,
-                           qsc, std::make_integer_sequence<int, kRow>{});
+    const int tl = jl < je ? stype[jl] : 0;
+    const bool vl = static_cast<unsigned>(tl - 1) < static_cast<unsigned>(a.nsp);
+    float4 q;
+    q.x = vl ? static_cast<float>(sx[jl] - ox) : fnan;
+    q.y = vl ? static_cast<float>(sy[jl] - oy) : fnan;
+    q.z = (D3 && vl) ? static_cast<float>(sz[jl] - oz) : 0.f;
+    q.w = 0.f;
+    pf_batch<D3, SELF, QQ>(a, thr, tb, hist, w, lane, jb, je, gi, jb + r16, hx, hy, hz, q, sx, sy, sz, stype, sq,
+                           gqs, qsc, std::make_integer_sequence<int, kRow>{});
   }
 }
 
 template <bool D3, bool QQ>
-__device__ inline void rdf_home_chunk_pf(const PairArgs& a, const double* thr, uint32_t* hist, PfWave& w, int lane,
-                                         const Stencil& st, const CellGrid& g, int cellL, int ch, int cs, int ce,
-                                         const int32_t* __restrict__ start, const float4* __restrict__ sp32,
+__device__ inline void rdf_home_chunk_pf(const PairArgs& a, const double* thr, const PfTables& tb, uint32_t* hist,
+                                         PfWave& w, int lane, const Stencil& st, const CellGrid& g, int cellL, int ch,
+                                         int cs, int ce, const int32_t* __restrict__ start,
                                          const double* __restrict__ sx, const double* __restrict__ sy,
                                          const double* __restrict__ sz, const int32_t* __restrict__ stype,
                                          const double* __restrict__ sq, unsigned long long* gqs, double qsc) {
-  const float fnan = __builtin_nanf("");
+  const double nan = __builtin_nan("");
   const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
   const int cx = cellL % ncx, cy = (cellL / ncx) % ncy, cz = cellL / (ncx * ncy);
+  // the home cell's corner: the reference point of the FP32 coordinates
+  const double ox = static_cast<double>(cx) / g.inv_h[0];
+  const double oy = static_cast<double>(cy) / g.inv_h[1];
+  const double oz = D3 ? static_cast<double>(cz) / g.inv_h[2] : 0.0;
   const int h0 = cs + ch * kRow;
   const int gi = h0 + (lane & (kRow - 1));  // the same 16 home atoms in every row
-  const float4 pi = gi < ce ? sp32[gi] : make_float4(fnan, fnan, fnan, 0.f);
+  const int ti = gi < ce ? stype[gi] : 0;
+  const bool oki = static_cast<unsigned>(ti - 1) < static_cast<unsigned>(a.nsp);
+  const float hx = static_cast<float>(oki ? sx[gi] - ox : nan);
+  const float hy = static_cast<float>(oki ? sy[gi] - oy : nan);
+  const float hz = D3 ? static_cast<float>(oki ? sz[gi] - oz : nan) : 0.f;
   for (int o = 0; o < st.n; ++o) {
     int nx = cx + st.off[o][0], ny = cy + st.off[o][1], nz = cz + st.off[o][2];
     // s = +1 below cell 0 (d ~ +L: subtract +L), -1 above the top cell
@@ -946,32 +971,31 @@ __device__ inline void rdf_home_chunk_pf(const PairArgs& a, const double* thr, u
     if (D3) {
       if (g.periodic[2]) nz = (nz + ncz) % ncz; else if (nz < 0 || nz >= ncz) continue;
     }
-    // the home coordinates moved by the image shift (d = x_j - (x_i + s))
-    const float hx = pi.x + static_cast<float>(static_cast<double>(sxi) * a.lmi[0]);
-    const float hy = pi.y + static_cast<float>(static_cast<double>(syi) * a.lmi[1]);
-    const float hz = D3 ? pi.z + static_cast<float>(static_cast<double>(szi) * a.lmi[2]) : 0.f;
+    // neighbour x_j - s, relative to the corner: x_j - (corner + s)
+    const double px = ox + static_cast<double>(sxi) * a.lmi[0];
+    const double py = oy + static_cast<double>(syi) * a.lmi[1];
+    const double pz = D3 ? oz + static_cast<double>(szi) * a.lmi[2] : 0.0;
     if (o == 0) {
       // own cell: j > i; neighbours below the chunk are all below its atoms
-      rdf_neighbours_pf<D3, true, QQ>(a, thr, hist, w, lane, h0, ce, gi, hx, hy, hz, sp32, sx, sy, sz, stype, sq,
-                                      gqs, qsc);
+      rdf_neighbours_pf<D3, true, QQ>(a, thr, tb, hist, w, lane, h0, ce, gi, hx, hy, hz, px, py, pz, sx, sy, sz,
+                                      stype, sq, gqs, qsc);
     } else {
       const int nb = (nz * ncy + ny) * ncx + nx;
       const int js = __builtin_amdgcn_readfirstlane(start[nb]);
       const int je = __builtin_amdgcn_readfirstlane(start[nb + 1]);
-      rdf_neighbours_pf<D3, false, QQ>(a, thr, hist, w, lane, js, je, gi, hx, hy, hz, sp32, sx, sy, sz, stype, sq,
-                                       gqs, qsc);
+      rdf_neighbours_pf<D3, false, QQ>(a, thr, tb, hist, w, lane, js, je, gi, hx, hy, hz, px, py, pz, sx, sy, sz,
+                                       stype, sq, gqs, qsc);
     }
   }
 }
 
 template <bool SHIFT, bool D3, bool QQ>
-__global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stencil st,
+__global__ __launch_bounds__(512) void k_rdf_cells(PairArgs a, CellGrid g, Stencil st,
                                                    const int32_t* __restrict__ start,
                                                    const double* __restrict__ spos,
                                                    const int32_t* __restrict__ stype, int64_t n,
                                                    int W, int64_t ntasks,
-                                                   uint32_t* __restrict__ partial,
-                                                   const float4* __restrict__ sp32) {
+                                                   uint32_t* __restrict__ partial) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int nbins = a.nrows * a.lsmax;  // this pass's tile
   uint32_t* hist = reinterpret_cast<uint32_t*>(smem);
@@ -980,39 +1004,60 @@ __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stenc
   // charge-product rings and the block's fixed-point charge histogram.
   const size_t qoff = (((static_cast<size_t>(nbins) * 4 + 15) & ~size_t(15)) +
                        static_cast<size_t>(a.lsmax + 3) * 8 + 15) & ~size_t(15);
-  const size_t qqoff = qoff + 4 * kHitQ * (sizeof(double) + sizeof(int));
+  const size_t qqoff = qoff + kCellWaves * kHitQ * (sizeof(double) + sizeof(int));
   double* qqr = nullptr;
   unsigned long long* gqs = nullptr;
   double qsc = 0.0;  // the frame's fixed-point scale (qp[0])
   if constexpr (QQ) {
     qsc = a.qp[0];
-    gqs = reinterpret_cast<unsigned long long*>(smem + qqoff + 4 * kHitQ * sizeof(double));
+    gqs = reinterpret_cast<unsigned long long*>(smem + qqoff + kCellWaves * kHitQ * sizeof(double));
     for (int b = threadIdx.x; b < a.lsmax; b += blockDim.x) gqs[b] = 0ull;
   }
   hist_setup(a, hist, thr);
   // wave-uniform (so are the task loop and the ring counters it carries)
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
   double* qr2 = reinterpret_cast<double*>(smem + qoff) + wave * kHitQ;
-  int* qtt = reinterpret_cast<int*>(smem + qoff + 4 * kHitQ * sizeof(double)) + wave * kHitQ;
+  int* qtt = reinterpret_cast<int*>(smem + qoff + kCellWaves * kHitQ * sizeof(double)) + wave * kHitQ;
   if constexpr (QQ) qqr = reinterpret_cast<double*>(smem + qqoff) + wave * kHitQ;
   uint32_t head = 0, tail = 0;  // wave-uniform ring counters
   const double* __restrict__ sx = spos;
   const double* __restrict__ sy = spos + n;
   const double* __restrict__ sz = spos + 2 * n;
-  // Prefilter path: (home, neighbour) entries in the space of the (r2, type)
-  // ring; home chunks of 16 atoms.
-  PfWave pw{reinterpret_cast<int2*>(qr2), 0u, 0u};
+  // Prefilter path: (home, neighbour, FP32 r2) entries, 16 B each, in the
+  // space of the (r2, type) rings and the (QQ) charge rings; home chunks of
+  // 16 atoms; the moved edges after them.
+  PfWave pw{reinterpret_cast<int4*>(smem + qoff) + wave * kHitQ, 0u, 0u};
+  PfTables tb{};
+  if constexpr (SHIFT) {
+    // after the rings (16 B per entry, 20 B with QQ) and the (QQ) charge
+    // histogram, as pf_tables_offset lays them out
+    const size_t toff = qoff + kCellWaves * kHitQ * (QQ ? 20 : 16) +
+                        (QQ ? (static_cast<size_t>(a.lsmax) * 8 + 15) & ~size_t(15) : 0);
+    float* lo = reinterpret_cast<float*>(smem + toff);
+    float* hi = lo + (a.lsmax + 3);
+    const double dl = a.pf_delta;
+    for (int b = threadIdx.x; b < a.lsmax + 3; b += blockDim.x) {
+      // the FP32 rounding of thr +- 2 delta stays beyond thr +- delta
+      // (delta >= 2^-21 side2 exceeds that rounding below the cutoff)
+      lo[b] = static_cast<float>(a.thr[b] + 2.0 * dl);
+      hi[b] = static_cast<float>(a.thr[b] - 2.0 * dl);
+    }
+    tb.lo = lo;
+    tb.hi = hi;
+    tb.cut = static_cast<float>(a.side2 - 2.0 * dl);
+    __syncthreads();
+  }
   const int chunk = SHIFT ? kRow : kWave;
-  for (int64_t task = static_cast<int64_t>(blockIdx.x) * 4 + wave; task < ntasks;
-       task += static_cast<int64_t>(gridDim.x) * 4) {
+  for (int64_t task = static_cast<int64_t>(blockIdx.x) * kCellWaves + wave; task < ntasks;
+       task += static_cast<int64_t>(gridDim.x) * kCellWaves) {
     const int cell = static_cast<int>(task / W);
     const int ch0 = static_cast<int>(task % W);
     const int cs = __builtin_amdgcn_readfirstlane(start[cell]);
     const int ce = __builtin_amdgcn_readfirstlane(start[cell + 1]);
     for (int ch = ch0; cs + ch * chunk < ce; ch += W) {
       if constexpr (SHIFT)
-        rdf_home_chunk_pf<D3, QQ>(a, thr, hist, pw, lane, st, g, cell, ch, cs, ce, start, sp32, sx, sy, sz, stype,
-                                  a.q, gqs, qsc);
+        rdf_home_chunk_pf<D3, QQ>(a, thr, tb, hist, pw, lane, st, g, cell, ch, cs, ce, start, sx, sy, sz, stype, a.q,
+                                  gqs, qsc);
       else
         rdf_home_chunk<SHIFT, D3, QQ>(a, thr, hist, qr2, qtt, head, tail, lane, st, g.nc, g.periodic, 0, cell, ch,
                                       cs, ce, start, sx, sy, sz, stype, a.q, qqr, gqs, qsc);
@@ -1020,7 +1065,7 @@ __global__ __launch_bounds__(256) void k_rdf_cells(PairArgs a, CellGrid g, Stenc
   }
   __builtin_amdgcn_wave_barrier();
   if constexpr (SHIFT)
-    bin_hits_pf<D3, QQ>(a, thr, hist, pw.ring, pw.tail - pw.fill, static_cast<int>(pw.fill), lane, sx, sy, sz,
+    bin_hits_pf<D3, QQ>(a, thr, tb, hist, pw.ring, pw.tail - pw.fill, static_cast<int>(pw.fill), lane, sx, sy, sz,
                         stype, a.q, gqs, qsc);
   else
     bin_hits<QQ>(a, thr, hist, qr2, qtt, head, static_cast<int>(tail - head), lane, qqr, gqs, qsc);
@@ -2447,10 +2492,10 @@ void launch_cell_count(const double* pos, int64_t n, int ndim, const CellGrid& g
 
 void launch_cell_scatter(const double* pos, const int32_t* itype, const int32_t* cellid, int64_t n,
                          const int32_t* start, int32_t* cursor, double* spos, int32_t* stype, const double* q,
-                         double* sq, hipStream_t s, float4* sp32, int nsp) {
+                         double* sq, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_cell_scatter, dim3(grid1d(n, 256)), dim3(256), 0, s, pos, itype, cellid, n, start,
-                     cursor, spos, stype, q, sq, sp32, nsp);
+                     cursor, spos, stype, q, sq);
 }
 
 size_t pair_lds_bytes(int rows, int lsmax, bool qq) {
@@ -2458,17 +2503,25 @@ size_t pair_lds_bytes(int rows, int lsmax, bool qq) {
   return qq ? ((b + 15) & ~size_t(15)) + static_cast<size_t>(lsmax) * 8 : b;
 }
 
-size_t cells_lds_bytes(int rows, int lsmax, bool qq) {
+// Layout of the cell-list kernel's LDS: the histogram tile and the edges
+// (pair_lds_bytes), the per-wavefront rings (16 B per entry: (r2, types) and
+// (QQ) charge products, or the prefilter's (home, neighbour, r2)), the (QQ)
+// block's charge histogram, then the prefilter's two FP32 edge tables.
+size_t pf_tables_offset(int rows, int lsmax, bool qq) {
   const size_t base = (pair_lds_bytes(rows, lsmax) + 15) & ~size_t(15);
-  const size_t rings = static_cast<size_t>(4) * kHitQ * (sizeof(double) + sizeof(int));
-  return base + rings +
-         (qq ? static_cast<size_t>(4) * kHitQ * sizeof(double) + ((static_cast<size_t>(lsmax) * 8 + 15) & ~size_t(15))
-             : 0);
+  // 16 B per entry: (r2, types) or the prefilter's (home, neighbour, r2);
+  // 20 B with the charge products of the per-pair path
+  const size_t rings = static_cast<size_t>(kCellWaves) * kHitQ * (qq ? 20 : 16);
+  return base + rings + (qq ? (static_cast<size_t>(lsmax) * 8 + 15) & ~size_t(15) : 0);
+}
+
+size_t cells_lds_bytes(int rows, int lsmax, bool qq) {
+  return pf_tables_offset(rows, lsmax, qq) + static_cast<size_t>(2) * (lsmax + 3) * sizeof(float);
 }
 
 void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start, const double* spos,
                       const int32_t* stype, int64_t n, int waves_per_cell, int nblocks,
-                      uint32_t* partial, hipStream_t s, const float4* sp32) {
+                      uint32_t* partial, hipStream_t s) {
   Stencil st{};
   int k = 0;
   st.off[k][0] = st.off[k][1] = st.off[k][2] = 0;
@@ -2490,33 +2543,38 @@ void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start
   bool shift = true;
   for (int k = 0; k < a.ndim; ++k) shift = shift && a.cellshift[k];
   const bool qq = a.q != nullptr;
-  // FP32 prefilter limit (image-shift axes): side2 plus a bound on how far the
-  // FP32 r2 of an in-range pair can exceed its FP64 r2.  Per axis, with X the
-  // largest |coordinate| (L, or 5L/4 on a non-periodic axis, whose atoms may
-  // lie in [-L/4, 5L/4)) and S the image shift (L on a periodic axis, else
-  // 0): x_j and x_i rounded to FP32 (< 2^-24 X each), the shift (< 2^-24 S),
-  // x_i + s (< 2^-24 (X + S)), the difference (< 2^-24 (rc + 2e)); so
-  // e < 2^-23 (3X + 2S + rc), and sum_k (2 rc e_k + e_k^2) bounds the change
-  // of r2, with 4 ulp for the FP32 squares and sums of r2 itself.
+  // FP32 prefilter bound (image-shift axes): |FP32 r2 - FP64 r2| < delta for
+  // every pair with r2 <= side2 + delta.  Per axis, with U the largest
+  // |coordinate relative to the home cell's corner| (2 cells, plus L/4 + 1
+  // cell on a non-periodic axis, whose atoms may lie in [-L/4, 5L/4)) and
+  // L the axis length: corner + shift and x - (corner + shift) in FP64
+  // (< 2^-51 (2L + U) together), each coordinate rounded to FP32 (< 2^-24 U),
+  // the FP32 difference (< 2^-24 (rc + 2e)); so e < 2^-23 (2U + rc) +
+  // 2^-51 (2L + U), and sum_k (2 rc e_k + e_k^2) bounds the change of r2;
+  // 2^-21 side2 covers the FP32 squares and sums of r2 itself (and the
+  // FP32 rounding of the limit and of the moved edges).
+  double delta = 0.0;
   float lim_pf = 0.f;
   if (shift) {
-    const double rc = std::sqrt(a.side2);
+    const double rc = std::sqrt(a.side2) * (1.0 + std::ldexp(1.0, -20));
     double err = 0.0;
     for (int k = 0; k < a.ndim; ++k) {
       const double L = g.periodic[k] ? a.lmi[k] : a.lmi[k] / 2.5;
-      const double X = g.periodic[k] ? L : 1.25 * L;
-      const double S = g.periodic[k] ? L : 0.0;
-      const double e = std::ldexp(3.0 * X + 2.0 * S + rc, -23);
+      const double cs = 1.0 / g.inv_h[k];
+      const double U = 2.0 * cs + (g.periodic[k] ? 0.0 : 0.25 * L + cs);
+      const double e = std::ldexp(2.0 * U + rc, -23) + std::ldexp(2.0 * L + U, -51);
       err += 2.0 * rc * e + e * e;
     }
-    const double lim = (a.side2 + err + std::ldexp(4.0 * a.side2, -23)) * (1.0 + std::ldexp(1.0, -20));
-    lim_pf = std::nextafter(static_cast<float>(lim), INFINITY);
+    delta = err + std::ldexp(a.side2, -21);
+    // the FP32 rounding of side2 + 2 delta stays above side2 + delta
+    lim_pf = static_cast<float>(a.side2 + 2.0 * delta);
   }
   // One launch per histogram tile of rows_per_pass pair-type rows.
   const int rows = a.nrows > 0 ? std::min(a.nrows, a.nit) : a.nit;
   for (int row0 = 0; row0 < a.nit; row0 += rows) {
     PairArgs t = a;
     t.side2_pf = lim_pf;
+    t.pf_delta = delta;
     t.row0 = row0;
     t.nrows = std::min(rows, a.nit - row0);
     const size_t lds = cells_lds_bytes(t.nrows, t.lsmax, qq);
@@ -2525,8 +2583,8 @@ void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start
     if (lds > 65536)                                                                                              \
       (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rdf_cells<S, D, Q>),                            \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));              \
-    hipLaunchKernelGGL((k_rdf_cells<S, D, Q>), dim3(nblocks), dim3(256), lds, s, t, g, st, start, spos, stype, n, \
-                       waves_per_cell, ntasks, partial, sp32);                                                    \
+    hipLaunchKernelGGL((k_rdf_cells<S, D, Q>), dim3(nblocks), dim3(64 * kCellWaves), lds, s, t, g, st, start, spos, \
+                       stype, n, waves_per_cell, ntasks, partial);                                            \
   } while (0)
     if (qq) {
       if (a.ndim == 3) {

@@ -185,7 +185,7 @@ int64_t trj_request_footprint(const trj_request* r) {
       b += int64_t(nit) * ls * 8 + int64_t(ls + 2) * 8;  // totals, bin edges
       b += 12 * (ncell + 1);                 // cell counts, starts, cursors
     }
-    b += n * (4 + 24 + 4 + 16);              // cell ids, cell-sorted positions (FP64, FP32) and types
+    b += n * (4 + 24 + 4);                   // cell ids, cell-sorted positions and types
   }
   if (r->what & TRJ_WANT_SQ) {
     trj::SqPlan S;

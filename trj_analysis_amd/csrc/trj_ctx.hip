@@ -50,7 +50,6 @@ struct trj_ctx {
   // device buffers
   DevBuf raw, itype, pos, spos, stype, cellid, cellcount, cellstart, cursor, partial, counts, thr, blk,
       blkoff, sp_off, u, rho, tiles, tile_shell, sqf, sqfp, axis_kv, axis_shell, listpart, flags;
-  DevBuf spos32;  // FP32 copy of the cell-sorted positions (the pair kernel's prefilter)
   // Charges (TRJ_RDF_CHARGE / TRJ_SQ_CHARGE): the frame's charges, their
   // cell-sorted copy, the frame's fixed-point g_qq sums, the g_qq and S_qq
   // accumulators.
@@ -142,7 +141,7 @@ std::vector<DevBuf*> dev_bufs(trj_ctx* c) {
           &c->slqsum,    &c->slqcnt,    &c->sqqsum,   &c->sqqcnt,     &c->xyframe,  &c->xycounts, &c->xyacc,
           &c->xygqf,     &c->xygqacc,   &c->thr_xy,   &c->rpts,       &c->s2gi,     &c->s2g2i,    &c->fqkv,
           &c->fqbin,     &c->fqr0,      &c->fqrho0,   &c->fqpart,     &c->fqvsum,   &c->fqfkt,    &c->fqfskt,
-          &c->fqtot,     &c->fqtots,    &c->spos32};
+          &c->fqtot,     &c->fqtots};
 }
 
 }  // namespace
@@ -328,7 +327,6 @@ int rdf_slices(trj_ctx* c, PairArgs pa, const FrameGeom& g, const double box[3],
   if ((r = ensure(c, c->cursor, sizeof(int32_t) * ncell))) return r;
   if ((r = ensure(c, c->spos, sizeof(double) * 3 * n))) return r;
   if ((r = ensure(c, c->stype, sizeof(int32_t) * n))) return r;
-  if ((r = ensure(c, c->spos32, 16 * static_cast<size_t>(n)))) return r;
   HIPC(c, hipMemsetAsync(c->cellcount.p, 0, sizeof(int32_t) * (ncell + 1), rs));
   HIPC(c, hipMemsetAsync(c->cursor.p, 0, sizeof(int32_t) * ncell, rs));
   launch_cell_count_slices(P<double>(c->pos), P<int32_t>(c->slkey), n, nsp, cg, nsl, P<int32_t>(c->cellid),
@@ -337,7 +335,7 @@ int rdf_slices(trj_ctx* c, PairArgs pa, const FrameGeom& g, const double box[3],
   if (q_dev && (r = ensure(c, c->qsort, sizeof(double) * n))) return r;
   launch_cell_scatter(P<double>(c->pos), itype_dev, P<int32_t>(c->cellid), n, P<int32_t>(c->cellstart),
                       P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype), q_dev,
-                      q_dev ? P<double>(c->qsort) : nullptr, rs, P<float4>(c->spos32), nsp);
+                      q_dev ? P<double>(c->qsort) : nullptr, rs);
   if (q_dev) {
     pa.q = P<double>(c->qsort);
     pa.qp = P<double>(c->qparm);
@@ -347,7 +345,7 @@ int rdf_slices(trj_ctx* c, PairArgs pa, const FrameGeom& g, const double box[3],
     const double mean = static_cast<double>(n) / static_cast<double>(nsl * ncell2);
     W = static_cast<int>(std::lround(mean / 64.0));
     W = std::max(1, std::min(W, 8));
-    nblocks = static_cast<int>(std::min<int64_t>((ncell2 * W + 3) / 4, 2048));
+    nblocks = static_cast<int>(std::min<int64_t>((ncell2 * W + kCellWaves - 1) / kCellWaves, 2048));
   } else {
     const int64_t nb = (n + 63) / 64;
     nblocks = static_cast<int>(std::min<int64_t>((nb * nb + 3) / 4, 1024));
@@ -357,7 +355,7 @@ int rdf_slices(trj_ctx* c, PairArgs pa, const FrameGeom& g, const double box[3],
     if (q_dev) pa.gq = P<unsigned long long>(c->xygqf) + static_cast<int64_t>(sl) * R.lsmax;
     if (cells_ok)
       launch_rdf_cells(pa, cg, P<int32_t>(c->cellstart) + sl * ncell2, P<double>(c->spos), P<int32_t>(c->stype), n,
-                       W, nblocks, P<uint32_t>(c->partial), rs, P<float4>(c->spos32));
+                       W, nblocks, P<uint32_t>(c->partial), rs);
     else
       launch_rdf_allpairs(pa, P<double>(c->spos), P<int32_t>(c->stype), n, nblocks, P<uint32_t>(c->partial), rs, n,
                           P<int32_t>(c->cellstart) + sl);
@@ -472,10 +470,15 @@ RdfCells rdf_cell_plan(int ndim, const double box[3], const int periodic[3], dou
     p.ncell *= p.cg.nc[k];
   }
   if (p.cells_ok) {
+    // home chunks: 16 atoms with the image shift on every axis (the
+    // prefiltered kernel), else 64; W wavefronts share a cell's chunks
+    bool all_shift = true;
+    for (int k = 0; k < ndim; ++k) all_shift = all_shift && p.cellshift[k];
+    const double chunk = all_shift ? 16.0 : 64.0;
     const double mean = static_cast<double>(n) / static_cast<double>(p.ncell);
-    p.W = std::max(1, std::min(static_cast<int>(std::lround(mean / 64.0)), 8));
+    p.W = std::max(1, std::min(static_cast<int>(std::lround(mean / chunk)), 16));
     if (W_override > 0) p.W = W_override;
-    p.nblocks = static_cast<int>(std::min<int64_t>((p.ncell * p.W + 3) / 4, 2048));
+    p.nblocks = static_cast<int>(std::min<int64_t>((p.ncell * p.W + kCellWaves - 1) / kCellWaves, 2048));
   } else {
     const int64_t nb = (n + 63) / 64;
     p.nblocks = static_cast<int>(std::min<int64_t>((nb * nb + 3) / 4, 1024));
@@ -496,7 +499,7 @@ int rows_fitting(int nit, int lsmax, bool qq, size_t budget) {
 // cell list and keeps the fastest in c->rdf_tune.  Candidates: histogram
 // tiles that fit the device's LDS at 1, 2, 3 and 4 workgroups per CU
 // (fewer rows per tile means more passes but more resident workgroups), and
-// the default waves per home cell with one fewer and one more.  Every
+// the default waves per home cell, one fewer and one more, half and twice.  Every
 // candidate counts the same pairs, so the choice changes speed only; the
 // trial runs leave nothing behind (the frame's own run follows).
 int tune_pairs(trj_ctx* c, const PairArgs& pa0, const CellGrid& cg, const RdfCells& base, int64_t n,
@@ -509,8 +512,8 @@ int tune_pairs(trj_ctx* c, const PairArgs& pa0, const CellGrid& cg, const RdfCel
     if (r >= 1 && std::find(rows.begin(), rows.end(), r) == rows.end()) rows.push_back(r);
   }
   std::vector<int> ws = {base.W};
-  if (base.W > 1) ws.push_back(base.W - 1);
-  if (base.W < 8) ws.push_back(base.W + 1);
+  for (int w : {base.W / 2, base.W - 1, base.W + 1, 2 * base.W})
+    if (w >= 1 && w <= 16 && std::find(ws.begin(), ws.end(), w) == ws.end()) ws.push_back(w);
   tu.rows = R.rows;
   tu.W = base.W;
   tu.tried = 0;
@@ -528,15 +531,14 @@ int tune_pairs(trj_ctx* c, const PairArgs& pa0, const CellGrid& cg, const RdfCel
   // one untimed run (code object and caches warm)
   PairArgs pa = pa0;
   pa.nrows = rows[0];
-  const float4* sp32 = P<float4>(c->spos32);
-  launch_rdf_cells(pa, cg, start, spos, stype, n, base.W, base.nblocks, part, rs, sp32);
+  launch_rdf_cells(pa, cg, start, spos, stype, n, base.W, base.nblocks, part, rs);
   double best = 1e300;
   for (int rw : rows)
     for (int W : ws) {
-      const int nb = static_cast<int>(std::min<int64_t>((base.ncell * W + 3) / 4, 2048));
+      const int nb = static_cast<int>(std::min<int64_t>((base.ncell * W + kCellWaves - 1) / kCellWaves, 2048));
       pa.nrows = rw;
       HIPC(c, hipEventRecord(c->tune_ev[0], rs));
-      launch_rdf_cells(pa, cg, start, spos, stype, n, W, nb, part, rs, sp32);
+      launch_rdf_cells(pa, cg, start, spos, stype, n, W, nb, part, rs);
       HIPC(c, hipEventRecord(c->tune_ev[1], rs));
       HIPC(c, hipEventSynchronize(c->tune_ev[1]));
       float ms = 0.f;
@@ -747,7 +749,6 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
         if ((r = ensure(c, c->cursor, sizeof(int32_t) * ncell))) return r;
         if ((r = ensure(c, c->spos, sizeof(double) * 3 * n))) return r;
         if ((r = ensure(c, c->stype, sizeof(int32_t) * n))) return r;
-        if ((r = ensure(c, c->spos32, 16 * static_cast<size_t>(n)))) return r;
         HIPC(c, hipMemsetAsync(c->cellcount.p, 0, sizeof(int32_t) * (ncell + 1), rs));
         HIPC(c, hipMemsetAsync(c->cursor.p, 0, sizeof(int32_t) * ncell, rs));
         launch_cell_count(P<double>(c->pos), n, ndim, cg, P<int32_t>(c->cellid), P<int32_t>(c->cellcount),
@@ -757,8 +758,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
         if (qq_pairs && (r = ensure(c, c->qsort, sizeof(double) * n))) return r;
         launch_cell_scatter(P<double>(c->pos), itype_dev, P<int32_t>(c->cellid), n, P<int32_t>(c->cellstart),
                             P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype),
-                            qq_pairs ? q_dev : nullptr, qq_pairs ? P<double>(c->qsort) : nullptr, rs,
-                            P<float4>(c->spos32), nsp);
+                            qq_pairs ? q_dev : nullptr, qq_pairs ? P<double>(c->qsort) : nullptr, rs);
         if (qq_pairs) pa.q = P<double>(c->qsort);
         if (!tu.done && tu.enabled && !c->noalloc && !qq_pairs) {
           if ((r = tune_pairs(c, pa, cg, rcl, n, rs))) return r;
@@ -769,7 +769,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
         const int W = rcl.W;
         if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
         launch_rdf_cells(pa, cg, P<int32_t>(c->cellstart), P<double>(c->spos), P<int32_t>(c->stype), n, W,
-                         nblocks, P<uint32_t>(c->partial), rs, P<float4>(c->spos32));
+                         nblocks, P<uint32_t>(c->partial), rs);
       } else {
         if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
         if (rdf_q && !qq_species) pa.q = q_dev;  // input order, like pos and itype here
@@ -1299,9 +1299,9 @@ int trj_frame_device_noalloc(trj_ctx* c, const void* pos_dev, int32_t dtype, int
   const size_t un = static_cast<size_t>(n), nc = static_cast<size_t>(cp.ncell);
   const size_t cells = cp.cells_ok ? 1 : 0;
   DevBuf* sw[] = {&c->pos, &c->blk, &c->blkoff, &c->cellid, &c->cellcount, &c->cellstart, &c->cursor,
-                  &c->spos, &c->stype, &c->spos32, &c->partial};
+                  &c->spos, &c->stype, &c->partial};
   const size_t len[] = {24 * un, 4 * R.nsp * nblk, 4 * R.nsp * nblk, cells * 4 * un, cells * 4 * (nc + 1),
-                        cells * 4 * (nc + 1), cells * 4 * nc, cells * 24 * un, cells * 4 * un, cells * 16 * un,
+                        cells * 4 * (nc + 1), cells * 4 * nc, cells * 24 * un, cells * 4 * un,
                         4 * static_cast<size_t>(R.nit) * R.lsmax * static_cast<size_t>(cp.nblocks)};
   constexpr int nb = sizeof(len) / sizeof(len[0]);
   size_t off[nb], total = 0;
