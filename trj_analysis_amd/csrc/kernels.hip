@@ -990,7 +990,7 @@ __device__ inline void rdf_home_chunk_pf(const PairArgs& a, const double* thr, c
 }
 
 template <bool SHIFT, bool D3, bool QQ>
-__global__ __launch_bounds__(512) void k_rdf_cells(PairArgs a, CellGrid g, Stencil st,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_rdf_cells(PairArgs a, CellGrid g, Stencil st,
                                                    const int32_t* __restrict__ start,
                                                    const double* __restrict__ spos,
                                                    const int32_t* __restrict__ stype, int64_t n,
