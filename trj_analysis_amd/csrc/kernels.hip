@@ -1692,11 +1692,12 @@ constexpr int kRS = 34;
 //   S3(i): B of the wavefront's columns into registers, then 96 MFMAs
 //          (v_mfma_f32_16x16x32_{f16,bf16}; the -Im*Im products in their own
 //          accumulators, no operand negation).
-// One loop step runs S3(i), S2(i+1) and S1(i+2); one barrier per chunk orders
-// the LDS buffers (three phase buffers, two A buffers).  The turns of a chunk
-// are loaded two chunks before they are staged.  FP32 accumulation runs over
-// at most 8192 atoms per workgroup and channel (the K-splits are summed in
-// FP64).
+// The two wavefronts of a SIMD alternate between S3's MFMAs and the VALU work
+// (B of S3, S2, S1), two barriers per chunk; the LDS buffers (three phase
+// buffers, two A buffers) cover the half-step offset between them.  The turns
+// of a chunk are loaded one chunk step before they are staged.  FP32
+// accumulation runs over at most 8192 atoms per workgroup and channel (the
+// K-splits are summed in FP64).
 constexpr int WX = 64, WY = 8, WZ = 32, WN = WY * WZ, WROWS = 104;
 template <int PREC>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k_sq_contract_lp16w(
@@ -1743,11 +1744,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
         accI[mb][e] = accR[mb][e];
         accM[mb][e] = accR[mb][e];
       }
-    // Turns of one chunk, loaded two chunks before they are staged.
+    // Turns of one chunk, loaded one chunk step before they are staged.
     struct Turns {
       double h, l, c, s, w;
       float fw, f1;
-    } F0, F1;
+    } F;
     auto fetch = [&](Turns& f, int c0) {
       const int ai = c0 + sa;
       const uint32_t o = static_cast<uint32_t>(ai < b1 ? ai : b1 - 1);
@@ -1822,12 +1823,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
         split8(im, B[e][2], B[e][3]);
       }
     };
-    auto mma = [&](int buf, const V (&B)[2][4], int mb) {
+    auto mma = [&](const V (&Af)[4], const V (&B)[2][4], int mb) {
       {
-        const V arh = As[buf][0][kq][16 * mb + i16];
-        const V arl = As[buf][1][kq][16 * mb + i16];
-        const V aih = As[buf][2][kq][16 * mb + i16];
-        const V ail = As[buf][3][kq][16 * mb + i16];
+        const V arh = Af[0], arl = Af[1], aih = Af[2], ail = Af[3];
         f4 r0 = accR[mb][0], i0 = accI[mb][0], r1 = accR[mb][1], i1 = accI[mb][1];
         f4 m0 = accM[mb][0], m1 = accM[mb][1];
         r0 = LT::mfma(arh, B[0][0], r0);
@@ -1864,47 +1862,94 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
     };
     V B0[2][4];
     __syncthreads();  // the previous channel's readers are done with rec and As
-    fetch(F0, b0);
-    fetch(F1, b0 + LKC);
-    stage_begin(F0);
+    fetch(F, b0);
+    stage_begin(F);
     stage_steps(0, 0, 8);
-    fetch(F0, b0 + 2 * LKC);
+    fetch(F, b0 + LKC);
     __syncthreads();
     split_a(0, 0);
-    stage_begin(F1);
+    stage_begin(F);
     stage_steps(1, 0, 8);
-    fetch(F1, b0 + 3 * LKC);
+    fetch(F, b0 + 2 * LKC);
     __syncthreads();
-    // Chunk c0: phases in rec[rb], A in As[0] on the first half of a trip and
-    // As[1] on the second.  A step reads rec[rb] (B) and rec[rb + 1] (A of the
-    // next chunk) and writes rec[rb + 2] (mod 3), which no wavefront reads
-    // until after the barrier.  Past b1 the stages run on zero-weight atoms.
-    // One step: B and the MFMAs of this chunk, the A split of the next one,
-    // the phases of the one after.
-    auto step = [&](int rbuf, int abuf, int rnext, int anext, int rstage, const Turns& f) {
-      gen_b(rbuf, B0);
-#pragma unroll
-      for (int mb = 0; mb < 4; ++mb) mma(abuf, B0, mb);
+    // Two groups of wavefronts, one of each per SIMD (wavefront w runs on
+    // SIMD w % 4), take turns on the matrix cores: while one group issues
+    // its 96 MFMAs of chunk c, the other does its VALU work, and they swap at
+    // a barrier.  Per chunk c (phases of chunk c in rec[rb], A in As[ab]):
+    //   H1: group 0 MFMAs of c; group 1 B of c, its half of the A split of
+    //       c+1 (rec[rb+1] -> As[1-ab]) and its segments of the phases of c+2
+    //       (-> rec[rb+2], mod 3);
+    //   H2: group 1 MFMAs of c; group 0 B of c+1, its half of the A split of
+    //       c+1 and its segments of the phases of c+2.
+    // As[ab] is read by both groups' MFMAs and rewritten (for c+2) only after
+    // H2; rec[rb+2] held chunk c-1, whose last reader ran in H1 of c-1.  The
+    // turns are refilled (for c+3) right after they are staged.  Past b1 the
+    // stages run on zero-weight atoms.
+    const bool g0 = __builtin_amdgcn_readfirstlane(w) < 4;
+    auto valu = [&](int rgen, int rnext, int anext, int rstage, int cnext) {
+      gen_b(rgen, B0);
       split_a(rnext, anext);
-      stage_begin(f);
+      stage_begin(F);
       stage_steps(rstage, 0, 8);
+      fetch(F, cnext);
+    };
+    auto mfma_all = [&](int abuf) {
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) {
+        V Af[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) Af[p] = As[abuf][p][kq][16 * mb + i16];
+        mma(Af, B0, mb);
+      }
+    };
+    // Each group runs its own loop (the same number of barriers in both), so
+    // that no register lives across the two schedules.  The scheduler may not
+    // move MFMAs (register-only work) across a barrier.
+    auto half_barrier = [] {
+      __builtin_amdgcn_sched_barrier(0);
+      __syncthreads();
+      __builtin_amdgcn_sched_barrier(0);
     };
     int rb = 0;
-    for (int c0 = b0;;) {
-      const int r1 = rb == 2 ? 0 : rb + 1, r2 = rb == 0 ? 2 : rb - 1;
-      step(rb, 0, r1, 1, r2, F0);
-      fetch(F0, c0 + 4 * LKC);
-      __syncthreads();
-      c0 += LKC;
-      rb = r1;
-      if (c0 >= b1) break;
-      const int q1 = rb == 2 ? 0 : rb + 1, q2 = rb == 0 ? 2 : rb - 1;
-      step(rb, 1, q1, 0, q2, F1);
-      fetch(F1, c0 + 4 * LKC);
-      __syncthreads();
-      c0 += LKC;
-      rb = q1;
-      if (c0 >= b1) break;
+    if (g0) {
+      gen_b(0, B0);
+      for (int c0 = b0;;) {
+        const int r1 = rb == 2 ? 0 : rb + 1, r2 = rb == 0 ? 2 : rb - 1;
+        mfma_all(0);
+        half_barrier();
+        valu(r1, r1, 1, r2, c0 + 3 * LKC);
+        half_barrier();
+        c0 += LKC;
+        rb = r1;
+        if (c0 >= b1) break;
+        const int q1 = rb == 2 ? 0 : rb + 1, q2 = rb == 0 ? 2 : rb - 1;
+        mfma_all(1);
+        half_barrier();
+        valu(q1, q1, 0, q2, c0 + 3 * LKC);
+        half_barrier();
+        c0 += LKC;
+        rb = q1;
+        if (c0 >= b1) break;
+      }
+    } else {
+      for (int c0 = b0;;) {
+        const int r1 = rb == 2 ? 0 : rb + 1, r2 = rb == 0 ? 2 : rb - 1;
+        valu(rb, r1, 1, r2, c0 + 3 * LKC);
+        half_barrier();
+        mfma_all(0);
+        half_barrier();
+        c0 += LKC;
+        rb = r1;
+        if (c0 >= b1) break;
+        const int q1 = rb == 2 ? 0 : rb + 1, q2 = rb == 0 ? 2 : rb - 1;
+        valu(rb, q1, 0, q2, c0 + 3 * LKC);
+        half_barrier();
+        mfma_all(1);
+        half_barrier();
+        c0 += LKC;
+        rb = q1;
+        if (c0 >= b1) break;
+      }
     }
     // D layout: register r holds row 4*(lane/16) + r, column lane%16.
     const size_t base = ((static_cast<size_t>(split) * a.nch + s) * a.ntiles + tile) * (WX * WN);

@@ -193,12 +193,7 @@ int64_t trj_request_footprint(const trj_request* r) {
     if (trj::sq_plan(r->sq, r->box, &S, &err) == TRJ_OK) {
       const int64_t tiles = static_cast<int64_t>(S.tiles.size());
       const int64_t nax = static_cast<int64_t>(S.axis_shell.size());
-      int dev = 0, slots = 0;  // the splits depend on the CU count (16-bit)
-      if (hipGetDevice(&dev) != hipSuccess ||
-          hipDeviceGetAttribute(&slots, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        slots = 0;
-      const int ks = tiles > 0 ? trj::contraction_splits(static_cast<int>(tiles), n, r->sq.precision, r->sq.nsp, slots)
-                               : 0;
+      const int ks = tiles > 0 ? trj::contraction_splits(static_cast<int>(tiles), n, r->sq.precision, r->sq.nsp) : 0;
       b += int64_t(ks) * r->sq.nsp * tiles * S.tk.vec() * 16;  // rho partials
       b += n * 96;                                             // turn planes
       b += tiles * (16 + 2 * S.tk.vec()) + 32;                 // tiles, shells of tile vectors

@@ -822,7 +822,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     c->sq_nch = nch;
     const int sdim = nsl > 0 ? 2 : ndim;
     if (ntiles > 0) {
-      const int ks = contraction_splits(ntiles, n, S.p.precision, nch, c->ncu);
+      const int ks = contraction_splits(ntiles, n, S.p.precision, nch);
       c->ksplit = ks;
       const size_t rho_bytes = sizeof(double) * 2 * static_cast<size_t>(ks) * nch * ntiles * S.tk.vec();
       if ((r = ensure(c, c->rho, rho_bytes))) return r;
@@ -897,20 +897,24 @@ hipStream_t context_stream(trj_ctx* c) { return c->stream; }
 // their turns (96 B per atom) in L2, with the partial rho buffer (one per
 // split) kept within 4 GiB; for the FP32-accumulating kernels at most 8192
 // atoms per workgroup (their accuracy bound, not capped).  The 16-bit kernel
-// holds a whole CU per workgroup and its workgroups take equal time, so with
-// slots > 0 (the device's CU count) the split count is raised until the
-// workgroups fill whole rounds of slots (more splits, shorter ranges).
-int contraction_splits(int ntiles, int64_t n, int precision, int nsp, int slots) {
+// holds a whole CU per workgroup and its workgroups take equal time, so the
+// split count is raised (to at most twice) until the workgroups fill whole
+// rounds of kWgRound = 256, the CU count of the MI355X.  The count, and with it
+// the FP32 accumulation length that sets the 16-bit error, depends on the
+// frame only, never on the CU count the device reports.
+int contraction_splits(int ntiles, int64_t n, int precision, int nsp) {
+  constexpr int64_t kWgRound = 256;
   int ks = static_cast<int>((2048 + ntiles - 1) / ntiles);
   ks = std::max(1, std::min<int>(ks, static_cast<int>(std::max<int64_t>(1, n / 8192))));
   const int64_t span = precision == TRJ_PREC_FP64 ? 32768 : 8192;
   ks = std::max<int>(ks, static_cast<int>((n + span - 1) / span));
-  if ((precision == TRJ_PREC_FP16X3 || precision == TRJ_PREC_BF16X3) && slots > 0 && ntiles > 0) {
+  if ((precision == TRJ_PREC_FP16X3 || precision == TRJ_PREC_BF16X3) && ntiles > 0) {
     const int64_t wgs = static_cast<int64_t>(ntiles) * ks;
-    const int64_t rounds = (wgs + slots - 1) / slots;
+    const int64_t rounds = (wgs + kWgRound - 1) / kWgRound;
     // the smallest split count whose workgroups fill `rounds` rounds, if any
-    const int64_t want = (rounds * slots) / ntiles;
-    if (want * ntiles == rounds * slots && want >= ks && want <= 2 * static_cast<int64_t>(ks)) ks = static_cast<int>(want);
+    const int64_t want = (rounds * kWgRound) / ntiles;
+    if (want * ntiles == rounds * kWgRound && want >= ks && want <= 2 * static_cast<int64_t>(ks))
+      ks = static_cast<int>(want);
   }
   if (precision != TRJ_PREC_FP64) return ks;
   const int64_t per_split = int64_t(16) * std::max(1, nsp) * std::max(1, ntiles) * tile_shape(precision).vec();
