@@ -342,6 +342,14 @@ def bench_north_star(steps: int, warmup: int) -> dict:
     c16 = runs["fp16x3_serial"]["contraction"]
     c16["measured_mfma"] = {"mfma_rate": rate / 1e12, "clock_mhz": mhz,
                             "frac_of_ceiling": c16["achieved"] * 1e12 / (rate / 3)}
+    # Shader cycles per 16-bit MFMA on one SIMD with 2 or 4 FP32 FMAs per MFMA
+    # (alone, in the other wavefront of the SIMD, interleaved in the same one):
+    # what the contraction's VALU work (B operand, splits, phases) costs beside
+    # its MFMAs.
+    co = {"mfma_only": T.coissue_rate(1, "mfma")}
+    for kv in (2, 4):
+        co["valu%d" % kv] = {m: T.coissue_rate(kv, m) for m in ("valu", "cross", "interleaved")}
+    c16["coissue_cycles_per_mfma"] = co
     s64, n64 = shells_of(wl, frames[0], "fp64")
     s16, n16 = shells_of(wl, frames[0], "fp16x3")
     m = s64 > 0

@@ -334,6 +334,15 @@ int trj_lds_atomic_rate(int device, double* per_s);
  * second; *clock_mhz (may be NULL): sustained shader clock over the loop.
  * No counterpart in the reference. */
 int trj_mfma_rate(int device, int precision, double* flops_per_s, double* clock_mhz);
+/* Measured co-issue of 16-bit MFMAs (16x16x32 FP16, FP32 accumulation) and
+ * FP32 VALU FMAs on one SIMD, the bound of the 16-bit split contraction: one
+ * 512-thread workgroup per CU, valu_per_mfma (1..4) independent FMAs per MFMA.
+ * mode 0: MFMAs only; 1: FMAs only; 2: wavefronts 0-3 MFMAs and 4-7 FMAs (one
+ * of each per SIMD); 3: every wavefront interleaves its FMAs after each MFMA.
+ * *cycles_per_mfma: the SIMD's shader cycles for the work divided by its MFMA
+ * count (about 16 for MFMAs alone; the sum of the two parts when they do not
+ * overlap).  No counterpart in the reference. */
+int trj_coissue_rate(int device, int valu_per_mfma, int mode, double* cycles_per_mfma);
 
 /* ------------------------------------------------------------------------
  * NetCDF trajectory input (host only): LAMMPS `dump netcdf` files in the

@@ -201,6 +201,27 @@ def test_mfma_rate_probe(prec):
     assert 500.0 < mhz < 3000.0
 
 
+def test_coissue_probe():
+    """trj_coissue_rate: a 16x16x32 FP16 MFMA occupies its SIMD for 16 shader
+    cycles (2.5e15 flop/s = 1024 SIMDs x 16384 flop / 16 cycles at 2.4 GHz);
+    FMAs alone cost cycles in proportion to their count; MFMAs plus FMAs never
+    take less than the MFMAs alone nor more than the two apart."""
+    print("case: co-issue probe", flush=True)
+    mfma = T.coissue_rate(1, "mfma")
+    assert 15.0 < mfma < 20.0
+    for kv in (2, 4):
+        valu = T.coissue_rate(kv, "valu")
+        cross = T.coissue_rate(kv, "cross")
+        inter = T.coissue_rate(kv, "interleaved")
+        print("valu/mfma %d: mfma %.2f valu %.2f cross %.2f interleaved %.2f cycles per MFMA"
+              % (kv, mfma, valu, cross, inter), flush=True)
+        assert 0.5 * kv < valu < 6.0 * kv
+        for both in (cross, inter):
+            assert 0.95 * mfma < both < 1.1 * (mfma + valu)
+    with pytest.raises(T.TrjError):
+        T.coissue_rate(5, "mfma")
+
+
 def test_cluster_centres_sqfndcl():
     """TRJ_SQ_CENTRES (sqfnDcl): per frame, each octant vector's |rho|^2 / Nbcl,
     no axis vectors; frames with different point counts."""

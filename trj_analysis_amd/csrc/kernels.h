@@ -228,6 +228,8 @@ int64_t launch_lds_atomics(int nblocks, int iters, uint32_t* out, hipStream_t s)
 // returns the flops one launch performs; clk[0] += shader-clock cycles and
 // clk[1] += 100 MHz real-time ticks summed over wavefronts.
 double launch_mfma_rate(int precision, int nblocks, int iters, float* out, unsigned long long* clk, hipStream_t s);
+double launch_coissue(int kv, int mode, int ncu, int trips, size_t lds_bytes, float* out, unsigned long long* clk,
+                      hipStream_t s);
 
 // ------------------------------------------------------------------ batches
 // Frames of one launch group, concatenated.  Device-side descriptors drive

@@ -2499,6 +2499,69 @@ __global__ __launch_bounds__(512) void k_mfma_rate(int iters, float* __restrict_
   }
 }
 
+// Co-issue of 16-bit MFMAs and FP32 VALU work on one SIMD.  One 512-thread
+// workgroup per CU (the launch reserves LDS so that no second one joins), two
+// wavefronts per SIMD; per SIMD M = 32 * trips MFMAs (16x16x32 FP16, eight
+// independent accumulations) and KV * M independent FP32 FMAs (16 chains):
+//   mode 0: both wavefronts MFMAs only (M / 2 each);
+//   mode 1: both wavefronts FMAs only;
+//   mode 2: wavefronts 0-3 MFMAs only, 4-7 FMAs only (across wavefronts);
+//   mode 3: both wavefronts interleave KV FMAs after each MFMA (in-wave).
+// clk[0] sums, over workgroups, wavefront 0's shader cycles from the start to
+// the closing barrier (the SIMD's time for its work).
+template <int KV, bool DM, bool DV>
+__device__ __forceinline__ void coissue_loop(int n, f4 (&acc)[8], float (&x)[16], h8 a, h8 b, float y, float z) {
+  for (int t = 0; t < n; ++t) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (DM) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc[j], 0, 0, 0);
+        if (DV) {
+#pragma unroll
+          for (int r = 0; r < KV; ++r) x[(j * KV + r) & 15] = fmaf(x[(j * KV + r) & 15], y, z);
+        }
+      }
+  }
+}
+
+template <int KV, int MODE>
+__global__ __launch_bounds__(512) void k_coissue(int trips, float* __restrict__ out,
+                                                 unsigned long long* __restrict__ clk) {
+  const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+  const int lane = threadIdx.x & (kWave - 1);
+  h8 a, b;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    a[i] = static_cast<_Float16>(0.001f * (lane + i));
+    b[i] = static_cast<_Float16>(0.002f * (lane - i));
+  }
+  f4 acc[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc[j] = f4{0.f, 0.f, 0.f, 0.f};
+  float x[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) x[j] = lane * 0.01f + j;
+  const float y = 0.999f + 1e-7f * lane, z = 0.001f;
+  __syncthreads();
+  const uint64_t t0 = __builtin_amdgcn_s_memtime();
+  if (MODE == 0) coissue_loop<KV, true, false>(trips / 2, acc, x, a, b, y, z);
+  else if (MODE == 1) coissue_loop<KV, false, true>(trips / 2, acc, x, a, b, y, z);
+  else if (MODE == 2) {
+    if (w < 4) coissue_loop<KV, true, false>(trips, acc, x, a, b, y, z);
+    else coissue_loop<KV, false, true>(trips, acc, x, a, b, y, z);
+  } else coissue_loop<KV, true, true>(trips / 2, acc, x, a, b, y, z);
+  __syncthreads();
+  const uint64_t t1 = __builtin_amdgcn_s_memtime();
+  float s = 0.f;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) s += x[j];
+  out[blockIdx.x * 512 + threadIdx.x] = s;
+  if (threadIdx.x == 0) atomicAdd(&clk[0], static_cast<unsigned long long>(t1 - t0));
+}
+
 inline unsigned grid1d(int64_t n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
 
 }  // namespace
@@ -2792,6 +2855,25 @@ double launch_mfma_rate(int precision, int nblocks, int iters, float* out, unsig
   else
     hipLaunchKernelGGL(k_mfma_rate<TRJ_PREC_BF16X3>, dim3(nblocks), dim3(256), 0, s, iters, out, clk);
   return static_cast<double>(nblocks) * 4 * iters * 8 * (16.0 * 16 * 32 * 2);
+}
+
+// Launches k_coissue<KV, mode> on `ncu` workgroups of 512 threads with
+// `lds_bytes` of dynamic LDS; returns the MFMAs per SIMD (0 on bad arguments).
+double launch_coissue(int kv, int mode, int ncu, int trips, size_t lds_bytes, float* out, unsigned long long* clk,
+                      hipStream_t s) {
+  typedef void (*Kern)(int, float*, unsigned long long*);
+  static const Kern table[4][4] = {
+      {k_coissue<1, 0>, k_coissue<1, 1>, k_coissue<1, 2>, k_coissue<1, 3>},
+      {k_coissue<2, 0>, k_coissue<2, 1>, k_coissue<2, 2>, k_coissue<2, 3>},
+      {k_coissue<3, 0>, k_coissue<3, 1>, k_coissue<3, 2>, k_coissue<3, 3>},
+      {k_coissue<4, 0>, k_coissue<4, 1>, k_coissue<4, 2>, k_coissue<4, 3>}};
+  if (kv < 1 || kv > 4 || mode < 0 || mode > 3 || trips < 2) return 0.0;
+  const Kern k = table[kv - 1][mode];
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          static_cast<int>(lds_bytes)) != hipSuccess)
+    return 0.0;
+  hipLaunchKernelGGL(k, dim3(ncu), dim3(512), lds_bytes, s, trips, out, clk);
+  return 32.0 * trips;
 }
 
 void launch_batch_group(const BatchDev& d, int nblocks, size_t pair_lds, hipStream_t s) {

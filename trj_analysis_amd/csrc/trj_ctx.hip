@@ -2384,6 +2384,38 @@ int trj_lds_atomic_rate(int device, double* per_s) {
   return rc;
 }
 
+int trj_coissue_rate(int device, int valu_per_mfma, int mode, double* cycles_per_mfma) {
+  if (!cycles_per_mfma || valu_per_mfma < 1 || valu_per_mfma > 4 || mode < 0 || mode > 3) return TRJ_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return TRJ_EDEVICE;
+  if (hipSetDevice(device) != hipSuccess) return TRJ_EDEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return TRJ_EDEVICE;
+  const int ncu = prop.multiProcessorCount;
+  // More than half of a CU's LDS, so that each CU holds one workgroup.
+  const size_t lds = std::min<size_t>(prop.sharedMemPerBlock, static_cast<size_t>(96) << 10);
+  const int trips = 512;
+  float* out = nullptr;
+  unsigned long long* clk = nullptr;
+  int rc = TRJ_EDEVICE;
+  if (hipMalloc(&out, sizeof(float) * ncu * 512) == hipSuccess &&
+      hipMalloc(&clk, sizeof(unsigned long long)) == hipSuccess) {
+    double m = launch_coissue(valu_per_mfma, mode, ncu, trips, lds, out, clk, nullptr);  // warm-up
+    unsigned long long cyc = 0;
+    if (m > 0.0 && hipMemset(clk, 0, sizeof(unsigned long long)) == hipSuccess) {
+      m = launch_coissue(valu_per_mfma, mode, ncu, trips, lds, out, clk, nullptr);
+      if (m > 0.0 && hipDeviceSynchronize() == hipSuccess &&
+          hipMemcpy(&cyc, clk, sizeof(cyc), hipMemcpyDeviceToHost) == hipSuccess && cyc > 0) {
+        *cycles_per_mfma = static_cast<double>(cyc) / ncu / m;
+        rc = TRJ_OK;
+      }
+    }
+  }
+  if (out) hipFree(out);
+  if (clk) hipFree(clk);
+  return rc;
+}
+
 int trj_mfma_rate(int device, int precision, double* flops_per_s, double* clock_mhz) {
   if (!flops_per_s) return TRJ_EINVAL;
   if (precision != TRJ_PREC_FP64 && precision != TRJ_PREC_FP16X3 && precision != TRJ_PREC_BF16X3)

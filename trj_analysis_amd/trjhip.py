@@ -40,7 +40,7 @@ EXPORTS = [
     "trj_set_slices", "trj_slice_counts", "trj_rdf_xy", "trj_rdf_xy_gmix", "trj_rdf_xy_qq",
     "trj_rdf_xy_gcc", "trj_sq_xy", "trj_sq_xy_columns", "trj_frame_q", "trj_rdf_qq", "trj_rdf_gqq", "trj_sq_qq", "trj_sq_qq_column", "trj_reset", "trj_last_timings", "trj_timing_totals", "trj_timing_reset",
     "trj_device_bytes", "trj_set_pipeline", "trj_state_bytes", "trj_state_save", "trj_state_load",
-    "trj_lds_atomic_rate", "trj_mfma_rate", "trj_nc_open", "trj_nc_close", "trj_nc_last_error", "trj_nc_info", "trj_nc_read",
+    "trj_lds_atomic_rate", "trj_mfma_rate", "trj_coissue_rate", "trj_nc_open", "trj_nc_close", "trj_nc_last_error", "trj_nc_info", "trj_nc_read",
     "trj_nc_time", "trj_nc_frame_ref", "trj_nc_frame_ref_q", "trj_nc_keep_nonperiodic",
     "trj_batch_create", "trj_batch_destroy", "trj_batch_last_error",
     "trj_batch_layout", "trj_batch_run", "trj_batch_autotune", "trj_srv_create", "trj_srv_destroy",
@@ -173,6 +173,7 @@ def _declare(L):
     L.trj_nc_info.argtypes = [vp, C.POINTER(NcFileInfo)]
     L.trj_nc_read.argtypes = [vp, C.c_int64, d, i32, d, d]
     L.trj_mfma_rate.argtypes = [C.c_int, C.c_int, d, d]
+    L.trj_coissue_rate.argtypes = [C.c_int, C.c_int, C.c_int, d]
     L.trj_nc_time.argtypes = [vp, C.c_int64, d]
     L.trj_nc_frame_ref.argtypes = [vp, C.c_int64, C.c_int32, i32, d, i32, i64, d, i32]
     L.trj_nc_frame_ref_q.argtypes = [vp, C.c_int64, C.c_int32, i32, d, i32, d, i64, d, i32]
@@ -317,6 +318,22 @@ def mfma_rate(precision="bf16x3", device: int = 0) -> tuple[float, float]:
     if rc != TRJ_OK:
         raise TrjError(rc, "trj_mfma_rate")
     return f.value, mhz.value
+
+
+COISSUE_MODES = {"mfma": 0, "valu": 1, "cross": 2, "interleaved": 3}
+
+
+def coissue_rate(valu_per_mfma: int, mode="cross", device: int = 0) -> float:
+    """Shader cycles per 16-bit MFMA of one SIMD with valu_per_mfma FP32 FMAs
+    per MFMA (trj_coissue_rate): mode "mfma" (MFMAs only), "valu" (FMAs only),
+    "cross" (one wavefront of each per SIMD) or "interleaved" (both in every
+    wavefront)."""
+    m = COISSUE_MODES[mode] if isinstance(mode, str) else int(mode)
+    c = C.c_double(0.0)
+    rc = lib().trj_coissue_rate(int(device), int(valu_per_mfma), m, C.byref(c))
+    if rc != TRJ_OK:
+        raise TrjError(rc, "trj_coissue_rate")
+    return c.value
 
 
 def _ptr(a: np.ndarray, ct):
