@@ -212,6 +212,43 @@ def bench_serving(rates, nreq: int, slots: int) -> dict:
             "deltar": S.DELTAR, "q_grid": list(S.GRID), "precision": "fp16x3"}
 
 
+# FP32 VALU lane operations per second of the MI355X (157.3 TFLOP/s counts an
+# FMA as two flops; one lane operation per lane per instruction).
+VALU_PEAK = 157.3e12 / 2
+# The FP32 work one candidate pair needs at least: three coordinate
+# differences, r^2 in three multiply-adds, one comparison with the cutoff.
+RDF_OPS_PER_CANDIDATE = 7
+
+
+def rdf_candidates(pos, box, rc) -> float:
+    """Candidate pairs of the pair kernel's cell walk on a periodic 3D frame:
+    cells of side >= rc (floor(L / rc) per axis, at most 160), each cell's own
+    pairs j > i plus every atom against the 13 forward neighbour cells."""
+    nc = [max(1, min(160, int(np.floor(L / (rc * (1.0 + 1e-6)))))) for L in box]
+    idx = [np.minimum((np.mod(pos[:, k], box[k]) * (nc[k] / box[k])).astype(np.int64), nc[k] - 1)
+           for k in range(3)]
+    flat = (idx[2] * nc[1] + idx[1]) * nc[0] + idx[0]
+    cnt = np.bincount(flat, minlength=nc[0] * nc[1] * nc[2]).reshape(nc[2], nc[1], nc[0]).astype(np.float64)
+    c = float((cnt * (cnt - 1.0) / 2.0).sum())
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                if dz > 0 or (dz == 0 and dy > 0) or (dz == 0 and dy == 0 and dx > 0):
+                    c += float((cnt * np.roll(cnt, (-dz, -dy, -dx), axis=(0, 1, 2))).sum())
+    return c
+
+
+def rdf_valu_roofline(frame, rc, rdf_ms) -> dict:
+    """The RDF stage (cell build, pair kernel, merge) priced as candidate
+    lane operations against the FP32 VALU peak."""
+    cand = rdf_candidates(frame.pos.astype(np.float64), frame.box, rc)
+    achieved = cand * RDF_OPS_PER_CANDIDATE / (rdf_ms * 1e-3) if rdf_ms > 0 else 0.0
+    return {"bound": "valu", "candidates_per_frame": cand, "ops_per_candidate": RDF_OPS_PER_CANDIDATE,
+            "achieved": achieved, "peak": VALU_PEAK, "unit": "lane-ops/s", "frac": achieved / VALU_PEAK,
+            "note": "candidate pairs of the half-stencil cell walk x 7 FP32 operations (3 differences, "
+                    "3 multiply-adds, 1 comparison) per second of the RDF stage, against 157.3 TFLOP/s / 2"}
+
+
 class FrameRun:
     """One context timing K frames of one workload back to back (inputs in HBM)."""
 
@@ -335,6 +372,8 @@ def bench_north_star(steps: int, warmup: int) -> dict:
             runs[name]["contraction"] = {"kernel": contraction_kernel(prec), "achieved": ach / 1e12,
                                          "peak": PEAK[prec] / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK[prec]}
             flops = r["flops"]
+    runs["fp16x3_serial"]["rdf_valu"] = rdf_valu_roofline(frames[0], wl.rcrdf,
+                                                          runs["fp16x3_serial"]["kernel_ms"]["rdf"])
     base = runs["fp64_serial"]["frames_per_s"]
     for name in runs:
         runs[name]["vs_fp64_serial"] = runs[name]["frames_per_s"] / base
@@ -527,6 +566,7 @@ def main():
                      "algorithmic": f"8*N*Nq = {flops:.4e} flop per launch (N={r['n_entries']} entries, "
                                     f"Nq={info.nvec_grid})",
                      "measured_mfma": probe},
+        "rdf_valu": rdf_valu_roofline(frames[0], wl.rcrdf, kernel_ms["rdf"]),
         "rdf_roofline": {"bound": "lds_atomic", "achieved": rdf_rate, "peak": lds_rate, "unit": "pairs/s",
                          "frac": rdf_rate / lds_rate, "pairs_per_frame": pairs_frame,
                          "note": "in-range pairs of the RDF stage (cell build + pairs + merge) per second "
