@@ -50,3 +50,23 @@ def test_fortran_netcdf(tmp_path):
     vals = {line.split()[0]: line.split()[1:] for line in out if line.strip()}
     assert vals["nc"] == ["2", "2", "3", "1", "1", "1", "0", "250.00"]
     assert [float(v) for v in vals["ncpos"]] == [1.0, 3.5, -3.0 + 4.0 + 3.0]
+
+
+def test_reference_signature_modules_built():
+    """mod_rdf, mod_sq and mod_common of the reference-signature layer export
+    the reference's procedures (rdf.cuf:51,124,184; sq.cuf:58,128,150,774;
+    common.cuf:368) and ref_driver links against the library."""
+    objdir = os.path.join(ROOT, "trj_analysis_amd", "_build", "fortran", "reference_api")
+    exe = os.path.join(ROOT, "trj_analysis_amd", "fortran", "reference_api", "ref_driver")
+    assert os.path.exists(exe), "run __graft_entry__.build() first (amdflang)"
+    names = set()
+    for mod in ("mod_rdf", "mod_sq", "mod_common"):
+        out = subprocess.run(["nm", os.path.join(objdir, mod + ".o")], capture_output=True, text=True,
+                             check=True).stdout
+        names |= {line.split()[-1] for line in out.splitlines() if " T " in line}
+    for sym in ("_QMmod_rdfPrdf_init", "_QMmod_rdfPrdfcomp", "_QMmod_rdfPrdf_clear", "_QMmod_sqPsq_init",
+                "_QMmod_sqPsqcalc", "_QMmod_sqPsq_transfer_gpu_cpu", "_QMmod_sqPsq_clear",
+                "_QMmod_commonPtransfer_cpu_gpu"):
+        assert sym in names, sym
+    ldd = subprocess.run(["ldd", exe], capture_output=True, text=True, check=True).stdout
+    assert "libtrjhip.so" in ldd and "not found" not in ldd

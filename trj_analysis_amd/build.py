@@ -47,11 +47,13 @@ def build(verbose: bool = False) -> str:
 FORTRAN = ["trj_hip.f90", "trj_hip_modules.f90"]
 FPROGRAMS = ["example_driver.f90", "abi_check.f90"]
 FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
+REF_MODULES = ["mod_precision.f90", "mod_input.f90", "mod_common.f90", "mod_rdf.f90", "mod_sq.f90"]
 
 
 def build_fortran(verbose: bool = False) -> list:
-    """ISO_C_BINDING modules and the two Fortran programs (amdflang), linked
-    against libtrjhip.so with an rpath relative to the executable."""
+    """ISO_C_BINDING modules and the Fortran programs (amdflang): the two over
+    the trj_hip modules and ref_driver over the reference-signature modules,
+    linked against libtrjhip.so with an rpath relative to the executable."""
     fdir = os.path.join(HERE, "fortran")
     objdir = os.path.join(HERE, "_build", "fortran")
     os.makedirs(objdir, exist_ok=True)
@@ -73,6 +75,28 @@ def build_fortran(verbose: bool = False) -> list:
         subprocess.run(cmd, check=True)
         os.replace(exe + ".tmp", exe)
         out.append(exe)
+    # The reference-signature modules (mod_precision, mod_input, mod_common,
+    # mod_rdf, mod_sq) in a module directory of their own, over the trj_hip
+    # interface module, and the driver that runs the reference's frame loop.
+    rdir = os.path.join(fdir, "reference_api")
+    robj = os.path.join(objdir, "reference_api")
+    os.makedirs(robj, exist_ok=True)
+    robjs = [objs[0]]  # trj_hip.o: the C-ABI interfaces
+    for src in REF_MODULES:
+        obj = os.path.join(robj, os.path.splitext(src)[0] + ".o")
+        cmd = [FLANG, "-O2", "-fPIC", "-I", objdir, "-J", robj, "-c", os.path.join(rdir, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        robjs.append(obj)
+    exe = os.path.join(rdir, "ref_driver")
+    cmd = [FLANG, "-O2", "-I", objdir, "-J", robj, os.path.join(rdir, "ref_driver.f90")] + robjs + \
+        ["-L" + HERE, "-ltrjhip", "-Wl,-rpath,$ORIGIN/../..", "-o", exe + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(exe + ".tmp", exe)
+    out.append(exe)
     return out
 
 
