@@ -908,6 +908,10 @@ int contraction_splits(int ntiles, int64_t n, int precision, int nsp) {
   ks = std::max(1, std::min<int>(ks, static_cast<int>(std::max<int64_t>(1, n / 8192))));
   const int64_t span = precision == TRJ_PREC_FP64 ? 32768 : 8192;
   ks = std::max<int>(ks, static_cast<int>((n + span - 1) / span));
+  // Small frames on few tiles: enough splits to fill one round of kWgRound
+  // workgroups while each split keeps at least 1024 atoms (32 chunks), e.g. 9
+  // splits instead of 2 for 1e4 atoms on the 8 tiles of the 64^3 half-space.
+  ks = std::max<int>(ks, static_cast<int>(std::min<int64_t>((kWgRound + ntiles - 1) / ntiles, n / 1024)));
   if ((precision == TRJ_PREC_FP16X3 || precision == TRJ_PREC_BF16X3) && ntiles > 0) {
     const int64_t wgs = static_cast<int64_t>(ntiles) * ks;
     const int64_t rounds = (wgs + kWgRound - 1) / kWgRound;
