@@ -852,7 +852,7 @@ __device__ inline void bin_hits_pf(const PairArgs& a, const double* thr, const P
       k = r2 < a.side2 ? bin_of(r2, thr, a.inv_dr, a.lsmax) : 0;
     }
     if (k >= 1 && k <= a.lsmax) {
-      const int ti = stype[gi], tj = stype[gj];
+      const int ti = en.w & 255, tj = en.w >> 8;  // the pair's species, stored at the push
       const int lo = ti < tj ? ti : tj, hi = ti < tj ? tj : ti;
       const int row = pair_index(lo, hi, a.nsp) - 1 - a.row0;
       if (static_cast<unsigned>(row) < static_cast<unsigned>(a.nrows)) atomicAdd(&hist[row * a.lsmax + (k - 1)], 1u);
@@ -870,7 +870,7 @@ struct PfWave {
 // Step K of a batch: home lanes against neighbour 16 r + K of the batch.
 template <int K, bool D3, bool SELF, bool QQ>
 __device__ inline void pf_step(const PairArgs& a, const double* thr, const PfTables& tb, uint32_t* hist, PfWave& w,
-                               int lane, int jb, int je, int gi, int jrow, float hx, float hy, float hz,
+                               int lane, int jb, int je, int gi, int ti, int jrow, float hx, float hy, float hz,
                                const float4& q, const double* __restrict__ sx, const double* __restrict__ sy,
                                const double* __restrict__ sz, const int32_t* __restrict__ stype,
                                const double* __restrict__ sq, unsigned long long* gqs, double qsc) {
@@ -882,13 +882,16 @@ __device__ inline void pf_step(const PairArgs& a, const double* thr, const PfTab
     r2 = fmaf(dz, dz, r2);
   }
   const int gj = jrow + K;
+  // the neighbour's species (broadcast with every lane active: a DPP read of
+  // a lane outside EXEC would not return its value)
+  const int tj = __builtin_bit_cast(int, row_bcast<K>(q.w));
   bool hit = r2 < a.side2_pf;
   if constexpr (SELF) hit = hit && (gj > gi);
   const unsigned long long m = __ballot(hit);
   if (hit) {
     const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
-    w.ring[(w.tail + below) & (kHitQ - 1)] = make_int4(gi, gj, __float_as_int(r2), 0);
+    w.ring[(w.tail + below) & (kHitQ - 1)] = make_int4(gi, gj, __float_as_int(r2), ti | (tj << 8));
   }
   const uint32_t c = static_cast<uint32_t>(__popcll(m));
   w.tail += c;
@@ -903,12 +906,12 @@ __device__ inline void pf_step(const PairArgs& a, const double* thr, const PfTab
 
 template <bool D3, bool SELF, bool QQ, int... K>
 __device__ inline void pf_batch(const PairArgs& a, const double* thr, const PfTables& tb, uint32_t* hist, PfWave& w,
-                                int lane, int jb, int je, int gi, int jrow, float hx, float hy, float hz,
+                                int lane, int jb, int je, int gi, int ti, int jrow, float hx, float hy, float hz,
                                 const float4& q, const double* __restrict__ sx, const double* __restrict__ sy,
                                 const double* __restrict__ sz, const int32_t* __restrict__ stype,
                                 const double* __restrict__ sq, unsigned long long* gqs, double qsc,
                                 std::integer_sequence<int, K...>) {
-  (pf_step<K, D3, SELF, QQ>(a, thr, tb, hist, w, lane, jb, je, gi, jrow, hx, hy, hz, q, sx, sy, sz, stype, sq, gqs,
+  (pf_step<K, D3, SELF, QQ>(a, thr, tb, hist, w, lane, jb, je, gi, ti, jrow, hx, hy, hz, q, sx, sy, sz, stype, sq, gqs,
                             qsc),
    ...);
 }
@@ -916,7 +919,8 @@ __device__ inline void pf_batch(const PairArgs& a, const double* thr, const PfTa
 // Neighbours [js, je) of one cell; (ox, oy, oz) = corner + image shift.
 template <bool D3, bool SELF, bool QQ>
 __device__ inline void rdf_neighbours_pf(const PairArgs& a, const double* thr, const PfTables& tb, uint32_t* hist,
-                                         PfWave& w, int lane, int js, int je, int gi, float hx, float hy, float hz,
+                                         PfWave& w, int lane, int js, int je, int gi, int ti, float hx, float hy,
+                                         float hz,
                                          double ox, double oy, double oz, const double* __restrict__ sx,
                                          const double* __restrict__ sy, const double* __restrict__ sz,
                                          const int32_t* __restrict__ stype, const double* __restrict__ sq,
@@ -933,8 +937,8 @@ __device__ inline void rdf_neighbours_pf(const PairArgs& a, const double* thr, c
     q.x = vl ? static_cast<float>(sx[jl] - ox) : fnan;
     q.y = vl ? static_cast<float>(sy[jl] - oy) : fnan;
     q.z = (D3 && vl) ? static_cast<float>(sz[jl] - oz) : 0.f;
-    q.w = 0.f;
-    pf_batch<D3, SELF, QQ>(a, thr, tb, hist, w, lane, jb, je, gi, jb + r16, hx, hy, hz, q, sx, sy, sz, stype, sq,
+    q.w = __builtin_bit_cast(float, tl);
+    pf_batch<D3, SELF, QQ>(a, thr, tb, hist, w, lane, jb, je, gi, ti, jb + r16, hx, hy, hz, q, sx, sy, sz, stype, sq,
                            gqs, qsc, std::make_integer_sequence<int, kRow>{});
   }
 }
@@ -977,13 +981,13 @@ __device__ inline void rdf_home_chunk_pf(const PairArgs& a, const double* thr, c
     const double pz = D3 ? oz + static_cast<double>(szi) * a.lmi[2] : 0.0;
     if (o == 0) {
       // own cell: j > i; neighbours below the chunk are all below its atoms
-      rdf_neighbours_pf<D3, true, QQ>(a, thr, tb, hist, w, lane, h0, ce, gi, hx, hy, hz, px, py, pz, sx, sy, sz,
+      rdf_neighbours_pf<D3, true, QQ>(a, thr, tb, hist, w, lane, h0, ce, gi, ti, hx, hy, hz, px, py, pz, sx, sy, sz,
                                       stype, sq, gqs, qsc);
     } else {
       const int nb = (nz * ncy + ny) * ncx + nx;
       const int js = __builtin_amdgcn_readfirstlane(start[nb]);
       const int je = __builtin_amdgcn_readfirstlane(start[nb + 1]);
-      rdf_neighbours_pf<D3, false, QQ>(a, thr, tb, hist, w, lane, js, je, gi, hx, hy, hz, px, py, pz, sx, sy, sz,
+      rdf_neighbours_pf<D3, false, QQ>(a, thr, tb, hist, w, lane, js, je, gi, ti, hx, hy, hz, px, py, pz, sx, sy, sz,
                                        stype, sq, gqs, qsc);
     }
   }
