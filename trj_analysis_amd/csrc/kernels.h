@@ -97,6 +97,17 @@ struct PairArgs {
   double pf_delta;
   float inv_dr;
   const double* thr;  // lsmax+3 edges
+  // Host side of the prefilter, read by launch_rdf_cells only: the edges thr
+  // on the host, the bin width and the rule (1: int(r/deltar) + 1, else
+  // nint(r/deltar)); NULL thr_host: no entry is decided in FP32.
+  const double* thr_host;
+  double deltar;
+  int floor_bins;
+  // Set by launch_rdf_cells: bin k's edges in FP32 registers,
+  // t_k = ((k + pf_off)^2) * pf_dr2, moved inwards by pf_margin (checked on
+  // the host against thr for every k); the bin estimate
+  // int(sqrt(r2) * inv_dr + pf_est).
+  float pf_dr2, pf_off, pf_est, pf_margin;
   // Charge-weighted histogram (NULL q: off): q in the pair kernel's atom
   // order, each in-range pair adds rint(q_i q_j qp[0]) to gq[k-1] (int64,
   // two's complement in unsigned 64-bit atomics).  qp = {scale, avq2} of
@@ -108,7 +119,7 @@ struct PairArgs {
 // LDS of the all-pairs kernel for a histogram tile of `rows` pair-type rows.
 size_t pair_lds_bytes(int rows, int lsmax, bool qq = false);
 // LDS of the cell-list pair kernel: pair_lds_bytes plus its hit rings.
-size_t cells_lds_bytes(int rows, int lsmax, bool qq = false);
+size_t cells_lds_bytes(int rows, int lsmax, bool qq = false, bool pf = true);
 // Wavefronts per workgroup of the cell-list pair kernel: eight share one
 // histogram tile (LDS per wavefront, and so residency, as for its VGPRs).
 constexpr int kCellWaves = 8;

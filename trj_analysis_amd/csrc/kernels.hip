@@ -805,11 +805,12 @@ __device__ inline void rdf_home_chunk(const PairArgs& a, const double* thr, uint
 // broadcast.  |FP32 r2 - FP64 r2| < pf_delta (a bound on the rounding, from
 // the host), so pairs with FP32 r2 below side2_pf >= side2 + pf_delta cover
 // every pair in range.  They go to the wavefront's ring as (home index,
-// neighbour index, FP32 r2); the ring is binned 64 entries at a time, every
-// lane busy.  An entry whose FP32 r2 +- pf_delta lies inside one bin and
-// below side2 has that bin under the exact FP64 rule (edges moved by the
-// bound, in LDS as FP32: lo[k] >= thr[k] + pf_delta, hi[k] <= thr[k] -
-// pf_delta); the others (near an edge) go through the FP64 rule itself:
+// neighbour index, FP32 r2, the two species); the ring is binned 64 entries
+// at a time, every lane busy.  An entry whose FP32 r2 +- pf_delta lies
+// inside one bin and below side2 has that bin under the exact FP64 rule
+// (the bin's edges computed in FP32 registers and moved inwards by a margin
+// the host proves for every bin: at least pf_delta beyond the exact edges);
+// the others (near an edge) go through the FP64 rule itself:
 // d = x_j - x_i, minimum image (on these axes the neighbour cell's image
 // shift), r2 < side2, the bin edges.
 constexpr int kRow = 16;
@@ -821,51 +822,101 @@ __device__ inline float row_bcast(float v) {
 }
 
 struct PfTables {
-  const float* lo;  // lo[k] >= thr[k] + pf_delta
-  const float* hi;  // hi[k] <= thr[k] - pf_delta
-  float cut;        // cut <= side2 - pf_delta
+  float cut;  // cut <= side2 - pf_delta
 };
 
-template <bool D3, bool QQ>
-__device__ inline void bin_hits_pf(const PairArgs& a, const double* thr, const PfTables& tb, uint32_t* hist,
-                                   const int4* ring, uint32_t head, int cnt, int lane, const double* __restrict__ sx,
-                                   const double* __restrict__ sy, const double* __restrict__ sz,
-                                   const int32_t* __restrict__ stype, const double* __restrict__ sq,
-                                   unsigned long long* gqs, double qsc) {
-  if (lane < cnt) {
-    const int4 en = ring[(head + lane) & (kHitQ - 1)];
-    const int gi = en.x, gj = en.y;
-    const float r2f = __int_as_float(en.z);
-    // an estimate (the hardware square root): a wrong bin fails the test below
-    int k = static_cast<int>(__builtin_amdgcn_sqrtf(r2f) * a.inv_dr + 0.5f);
-    k = k > a.lsmax + 1 ? a.lsmax + 1 : k;
-    const bool sure = r2f < tb.cut && r2f >= tb.lo[k] && r2f < tb.hi[k + 1];
-    if (!sure) {
-      // near an edge: the FP64 rule
-      const double dx = min_image(sx[gj] - sx[gi], a.lmi[0], a.half[0]);
-      const double dy = min_image(sy[gj] - sy[gi], a.lmi[1], a.half[1]);
-      double r2 = dx * dx + dy * dy;
-      if constexpr (D3) {
-        const double dz = min_image(sz[gj] - sz[gi], a.lmi[2], a.half[2]);
-        r2 = r2 + dz * dz;
-      }
-      k = r2 < a.side2 ? bin_of(r2, thr, a.inv_dr, a.lsmax) : 0;
-    }
-    if (k >= 1 && k <= a.lsmax) {
-      const int ti = en.w & 255, tj = en.w >> 8;  // the pair's species, stored at the push
-      const int lo = ti < tj ? ti : tj, hi = ti < tj ? tj : ti;
-      const int row = pair_index(lo, hi, a.nsp) - 1 - a.row0;
-      if (static_cast<unsigned>(row) < static_cast<unsigned>(a.nrows)) atomicAdd(&hist[row * a.lsmax + (k - 1)], 1u);
-      if constexpr (QQ)
-        if (a.row0 == 0) atomicAdd(&gqs[k - 1], qq_fixed(sq[gi] * sq[gj], qsc));
-    }
-  }
-}
+constexpr int kSlowQ = 64;  // near-edge entries per wavefront waiting for the FP64 rule
 
 struct PfWave {
   int4* ring;
   uint32_t tail, fill;  // wave-uniform: entries written so far, entries not yet binned
+  int2* slow;           // near-edge entries (home, neighbour), kSlowQ
+  uint32_t nslow;       // wave-uniform: entries in slow
 };
+
+// Adds one binned entry (bin k of the FP64 rule) to the histogram tile and
+// (QQ) its charge product to the block's charge histogram.
+template <bool QQ>
+__device__ inline void add_hit_pf(const PairArgs& a, uint32_t* hist, const int4& en, int k,
+                                  const double* __restrict__ sq, unsigned long long* gqs, double qsc) {
+  if (k >= 1 && k <= a.lsmax) {
+    const int ti = en.w & 255, tj = en.w >> 8;  // the pair's species, stored at the push
+    const int tlo = ti < tj ? ti : tj, thi = ti < tj ? tj : ti;
+    const int row = pair_index(tlo, thi, a.nsp) - 1 - a.row0;
+    if (static_cast<unsigned>(row) < static_cast<unsigned>(a.nrows)) atomicAdd(&hist[row * a.lsmax + (k - 1)], 1u);
+    if constexpr (QQ)
+      if (a.row0 == 0) atomicAdd(&gqs[k - 1], qq_fixed(sq[en.x] * sq[en.y], qsc));
+  }
+}
+
+// The FP64 rule for the cnt entries of the slow queue, one per lane: d =
+// x_j - x_i, minimum image, r2 < side2, the exact edges.
+template <bool D3, bool QQ>
+__device__ inline void bin_slow_pf(const PairArgs& a, const double* thr, uint32_t* hist, const int2* slow, int cnt,
+                                   int lane, const double* __restrict__ sx, const double* __restrict__ sy,
+                                   const double* __restrict__ sz, const int32_t* __restrict__ stype,
+                                   const double* __restrict__ sq, unsigned long long* gqs, double qsc) {
+  if (lane < cnt) {
+    const int2 e = slow[lane];
+    const int gi = e.x, gj = e.y;
+    const int4 en = make_int4(gi, gj, 0, stype[gi] | (stype[gj] << 8));
+    const double dx = min_image(sx[gj] - sx[gi], a.lmi[0], a.half[0]);
+    const double dy = min_image(sy[gj] - sy[gi], a.lmi[1], a.half[1]);
+    double r2 = dx * dx + dy * dy;
+    if constexpr (D3) {
+      const double dz = min_image(sz[gj] - sz[gi], a.lmi[2], a.half[2]);
+      r2 = r2 + dz * dz;
+    }
+    add_hit_pf<QQ>(a, hist, en, r2 < a.side2 ? bin_of(r2, thr, a.inv_dr, a.lsmax) : 0, sq, gqs, qsc);
+  }
+}
+
+// Bins cnt ring entries from head, one per lane.  An entry whose FP32 r2
+// decides its bin is added at once; a near-edge one joins the slow queue,
+// which takes the FP64 rule 64 entries at a time (its loads and FP64 work
+// then run for a full wavefront of such entries, not for one lane of a
+// wavefront of sure ones).
+template <bool D3, bool QQ>
+__device__ inline void bin_hits_pf(const PairArgs& a, const double* thr, const PfTables& tb, uint32_t* hist,
+                                   PfWave& w, uint32_t head, int cnt, int lane, const double* __restrict__ sx,
+                                   const double* __restrict__ sy, const double* __restrict__ sz,
+                                   const int32_t* __restrict__ stype, const double* __restrict__ sq,
+                                   unsigned long long* gqs, double qsc) {
+  const bool act = lane < cnt;
+  int4 en = make_int4(0, 0, 0, 0);
+  int k = 0;
+  bool sure = true;
+  if (act) {
+    en = w.ring[(head + lane) & (kHitQ - 1)];
+    const float r2f = __int_as_float(en.z);
+    // an estimate (the hardware square root): a wrong bin fails the test below
+    k = static_cast<int>(__builtin_amdgcn_sqrtf(r2f) * a.inv_dr + a.pf_est);
+    k = k > a.lsmax + 1 ? a.lsmax + 1 : k;
+    // bin k's edges moved inwards, in FP32 registers: the host checked for
+    // every k that they lie beyond the exact edges +- pf_delta
+    const float x0 = static_cast<float>(k) + a.pf_off, x1 = x0 + 1.0f;
+    const float elo = x0 * x0 * a.pf_dr2 + a.pf_margin;
+    const float ehi = x1 * x1 * a.pf_dr2 - a.pf_margin;
+    sure = r2f < tb.cut && r2f >= elo && r2f < ehi;
+  }
+  const unsigned long long ms = __ballot(act && !sure);
+  if (ms) {
+    const uint32_t ns = static_cast<uint32_t>(__popcll(ms));
+    if (w.nslow + ns > static_cast<uint32_t>(kSlowQ)) {
+      __builtin_amdgcn_wave_barrier();
+      bin_slow_pf<D3, QQ>(a, thr, hist, w.slow, static_cast<int>(w.nslow), lane, sx, sy, sz, stype, sq, gqs, qsc);
+      __builtin_amdgcn_wave_barrier();
+      w.nslow = 0;
+    }
+    if (act && !sure) {
+      const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(ms >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(ms), 0u));
+      w.slow[w.nslow + below] = make_int2(en.x, en.y);
+    }
+    w.nslow += ns;
+  }
+  if (act && sure) add_hit_pf<QQ>(a, hist, en, k, sq, gqs, qsc);
+}
 
 // Step K of a batch: home lanes against neighbour 16 r + K of the batch.
 template <int K, bool D3, bool SELF, bool QQ>
@@ -898,7 +949,7 @@ __device__ inline void pf_step(const PairArgs& a, const double* thr, const PfTab
   w.fill += c;
   if (w.fill >= static_cast<uint32_t>(kWave)) {
     __builtin_amdgcn_wave_barrier();
-    bin_hits_pf<D3, QQ>(a, thr, tb, hist, w.ring, w.tail - w.fill, kWave, lane, sx, sy, sz, stype, sq, gqs, qsc);
+    bin_hits_pf<D3, QQ>(a, thr, tb, hist, w, w.tail - w.fill, kWave, lane, sx, sy, sz, stype, sq, gqs, qsc);
     __builtin_amdgcn_wave_barrier();
     w.fill -= kWave;
   }
@@ -1027,29 +1078,20 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   const double* __restrict__ sx = spos;
   const double* __restrict__ sy = spos + n;
   const double* __restrict__ sz = spos + 2 * n;
-  // Prefilter path: (home, neighbour, FP32 r2) entries, 16 B each, in the
-  // space of the (r2, type) rings and the (QQ) charge rings; home chunks of
-  // 16 atoms; the moved edges after them.
-  PfWave pw{reinterpret_cast<int4*>(smem + qoff) + wave * kHitQ, 0u, 0u};
+  // Prefilter path: (home, neighbour, FP32 r2, species) entries, 16 B each,
+  // in the space of the (r2, type) rings and the (QQ) charge rings; home
+  // chunks of 16 atoms.
+  // The slow queues after the rings and the (QQ) charge histogram, as
+  // cells_lds_bytes lays them out.
+  const size_t soff = qoff + kCellWaves * kHitQ * (QQ ? 20 : 16) +
+                      (QQ ? (static_cast<size_t>(a.lsmax) * 8 + 15) & ~size_t(15) : 0);
+  PfWave pw{reinterpret_cast<int4*>(smem + qoff) + wave * kHitQ, 0u, 0u,
+            reinterpret_cast<int2*>(smem + soff) + wave * kSlowQ, 0u};
   PfTables tb{};
   if constexpr (SHIFT) {
-    // after the rings (16 B per entry, 20 B with QQ) and the (QQ) charge
-    // histogram, as pf_tables_offset lays them out
-    const size_t toff = qoff + kCellWaves * kHitQ * (QQ ? 20 : 16) +
-                        (QQ ? (static_cast<size_t>(a.lsmax) * 8 + 15) & ~size_t(15) : 0);
-    float* lo = reinterpret_cast<float*>(smem + toff);
-    float* hi = lo + (a.lsmax + 3);
-    const double dl = a.pf_delta;
-    for (int b = threadIdx.x; b < a.lsmax + 3; b += blockDim.x) {
-      // the FP32 rounding of thr +- 2 delta stays beyond thr +- delta
-      // (delta >= 2^-21 side2 exceeds that rounding below the cutoff)
-      lo[b] = static_cast<float>(a.thr[b] + 2.0 * dl);
-      hi[b] = static_cast<float>(a.thr[b] - 2.0 * dl);
-    }
-    tb.lo = lo;
-    tb.hi = hi;
-    tb.cut = static_cast<float>(a.side2 - 2.0 * dl);
-    __syncthreads();
+    // the FP32 rounding of side2 - 2 delta stays below side2 - delta (delta
+    // >= 2^-21 side2 exceeds that rounding)
+    tb.cut = static_cast<float>(a.side2 - 2.0 * a.pf_delta);
   }
   const int chunk = SHIFT ? kRow : kWave;
   for (int64_t task = static_cast<int64_t>(blockIdx.x) * kCellWaves + wave; task < ntasks;
@@ -1068,10 +1110,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     }
   }
   __builtin_amdgcn_wave_barrier();
-  if constexpr (SHIFT)
-    bin_hits_pf<D3, QQ>(a, thr, tb, hist, pw.ring, pw.tail - pw.fill, static_cast<int>(pw.fill), lane, sx, sy, sz,
-                        stype, a.q, gqs, qsc);
-  else
+  if constexpr (SHIFT) {
+    bin_hits_pf<D3, QQ>(a, thr, tb, hist, pw, pw.tail - pw.fill, static_cast<int>(pw.fill), lane, sx, sy, sz, stype,
+                        a.q, gqs, qsc);
+    __builtin_amdgcn_wave_barrier();
+    bin_slow_pf<D3, QQ>(a, thr, hist, pw.slow, static_cast<int>(pw.nslow), lane, sx, sy, sz, stype, a.q, gqs, qsc);
+  } else
     bin_hits<QQ>(a, thr, hist, qr2, qtt, head, static_cast<int>(tail - head), lane, qqr, gqs, qsc);
   hist_flush(a, hist, partial);  // begins with a barrier: every bin_hits is done
   if constexpr (QQ) {
@@ -2617,18 +2661,15 @@ size_t pair_lds_bytes(int rows, int lsmax, bool qq) {
 
 // Layout of the cell-list kernel's LDS: the histogram tile and the edges
 // (pair_lds_bytes), the per-wavefront rings (16 B per entry: (r2, types) and
-// (QQ) charge products, or the prefilter's (home, neighbour, r2)), the (QQ)
-// block's charge histogram, then the prefilter's two FP32 edge tables.
-size_t pf_tables_offset(int rows, int lsmax, bool qq) {
+// (QQ) charge products, or the prefilter's (home, neighbour, r2, species)),
+// the (QQ) block's charge histogram, then (pf: the prefilter) the slow
+// queues of near-edge entries.
+size_t cells_lds_bytes(int rows, int lsmax, bool qq, bool pf) {
   const size_t base = (pair_lds_bytes(rows, lsmax) + 15) & ~size_t(15);
-  // 16 B per entry: (r2, types) or the prefilter's (home, neighbour, r2);
-  // 20 B with the charge products of the per-pair path
+  // 16 B per entry; 20 B with the charge products of the per-pair path
   const size_t rings = static_cast<size_t>(kCellWaves) * kHitQ * (qq ? 20 : 16);
-  return base + rings + (qq ? (static_cast<size_t>(lsmax) * 8 + 15) & ~size_t(15) : 0);
-}
-
-size_t cells_lds_bytes(int rows, int lsmax, bool qq) {
-  return pf_tables_offset(rows, lsmax, qq) + static_cast<size_t>(2) * (lsmax + 3) * sizeof(float);
+  const size_t slow = pf ? static_cast<size_t>(kCellWaves) * kSlowQ * 8 : 0;
+  return base + rings + (qq ? (static_cast<size_t>(lsmax) * 8 + 15) & ~size_t(15) : 0) + slow;
 }
 
 void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start, const double* spos,
@@ -2664,7 +2705,7 @@ void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start
   // the FP32 difference (< 2^-24 (rc + 2e)); so e < 2^-23 (2U + rc) +
   // 2^-51 (2L + U), and sum_k (2 rc e_k + e_k^2) bounds the change of r2;
   // 2^-21 side2 covers the FP32 squares and sums of r2 itself (and the
-  // FP32 rounding of the limit and of the moved edges).
+  // FP32 rounding of the limit and of the cut).
   double delta = 0.0;
   float lim_pf = 0.f;
   if (shift) {
@@ -2681,15 +2722,59 @@ void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start
     // the FP32 rounding of side2 + 2 delta stays above side2 + delta
     lim_pf = static_cast<float>(a.side2 + 2.0 * delta);
   }
+  // Bin edges of the prefilter in FP32 registers: t_k = fl(fl(fl(k + off)^2)
+  // * dr2), and an entry of FP32 r2 in [fl(t_k + m), fl(t_{k+1} - m)) is
+  // binned to k without the FP64 rule.  Proven here for every k by
+  // evaluating the same FP32 operations (IEEE, no contraction, as the
+  // kernel does): fl(t_k + m) >= thr[k] + delta for k = 1..lsmax+1 and
+  // fl(t_{k+1} - m) <= thr[k+1] - delta for k = 0..lsmax; with |FP32 r2 -
+  // FP64 r2| < delta, such an entry's FP64 r2 lies in [thr[k], thr[k+1]).
+  // m starts at the largest |t_k - thr[k]| plus 2 delta and doubles until
+  // the check passes; without host edges, or after ten doublings, m = inf
+  // (every entry takes the FP64 rule).
+  float pf_margin = INFINITY;
+  const float pf_off = a.floor_bins ? -1.0f : -0.5f;
+  const float pf_dr2 = static_cast<float>(a.deltar * a.deltar);
+  if (shift && a.thr_host) {
+    const double* th = a.thr_host;
+    auto edge = [&](int k) {
+      const float x = static_cast<float>(k) + pf_off;
+      const float x2 = x * x;
+      return x2 * pf_dr2;
+    };
+    const double tol = delta * (1.0 + std::ldexp(1.0, -30));  // covers FP64 rounding of thr + delta
+    double dev = 0.0;
+    for (int k = 1; k <= a.lsmax + 1; ++k) dev = std::max(dev, std::fabs(static_cast<double>(edge(k)) - th[k]));
+    float m = static_cast<float>((dev + 2.0 * delta) * (1.0 + std::ldexp(1.0, -20)));
+    for (int tries = 0; tries <= 10; ++tries, m *= 2.0f) {
+      bool ok = m > 0.0f;
+      for (int k = 1; ok && k <= a.lsmax + 1; ++k) {
+        const float lo = edge(k) + m;
+        ok = static_cast<double>(lo) >= th[k] + tol;
+      }
+      for (int k = 0; ok && k <= a.lsmax; ++k) {
+        const float hi = edge(k + 1) - m;
+        ok = static_cast<double>(hi) <= th[k + 1] - tol;
+      }
+      if (ok) {
+        pf_margin = m;
+        break;
+      }
+    }
+  }
   // One launch per histogram tile of rows_per_pass pair-type rows.
   const int rows = a.nrows > 0 ? std::min(a.nrows, a.nit) : a.nit;
   for (int row0 = 0; row0 < a.nit; row0 += rows) {
     PairArgs t = a;
     t.side2_pf = lim_pf;
     t.pf_delta = delta;
+    t.pf_dr2 = pf_dr2;
+    t.pf_off = pf_off;
+    t.pf_est = a.floor_bins ? 1.0f : 0.5f;
+    t.pf_margin = pf_margin;
     t.row0 = row0;
     t.nrows = std::min(rows, a.nit - row0);
-    const size_t lds = cells_lds_bytes(t.nrows, t.lsmax, qq);
+    const size_t lds = cells_lds_bytes(t.nrows, t.lsmax, qq, shift);
 #define TRJ_CELLS(S, D, Q)                                                                                        \
   do {                                                                                                            \
     if (lds > 65536)                                                                                              \

@@ -298,6 +298,8 @@ int rdf_slices(trj_ctx* c, PairArgs pa, const FrameGeom& g, const double box[3],
   pa.row0 = 0;
   pa.nrows = R.rows;
   pa.thr = P<double>(c->thr_xy);
+  pa.thr_host = R.thr_xy.size() == R.thr.size() ? R.thr_xy.data() : nullptr;
+  pa.floor_bins = 1;
   pa.cellshift[2] = 0;
   const double rc = std::sqrt(g.side2);
   CellGrid cg{};
@@ -714,6 +716,9 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     pa.side2 = g.side2;
     pa.inv_dr = static_cast<float>(1.0 / R.deltar);
     pa.thr = P<double>(c->thr);
+    pa.thr_host = R.thr.data();
+    pa.deltar = R.deltar;
+    pa.floor_bins = R.floor_bins ? 1 : 0;
     // Species charges: the merge adds the fixed-point sums from the species
     // histogram; the pair kernels then run without their charge path.
     const bool qq_species = rdf_q && qspecies && nsl == 0;
@@ -1120,6 +1125,7 @@ int trj_rdf_init(trj_ctx* c, const trj_rdf_params* p, const double box[3], int32
     std::vector<double> tx = rdf_bin_thresholds_floor(R.deltar, R.lsmax);
     tx.push_back(INFINITY);
     tx[0] = 0.0;
+    R.thr_xy = tx;
     const size_t nb = static_cast<size_t>(nsl) * R.nit * R.lsmax;
     if ((r = ensure(c, c->thr_xy, sizeof(double) * tx.size()))) return r;
     if ((r = ensure(c, c->xyframe, sizeof(int64_t) * nb))) return r;
@@ -1144,6 +1150,7 @@ int trj_rdf_init(trj_ctx* c, const trj_rdf_params* p, const double box[3], int32
   thr.push_back(INFINITY);  // thr[lsmax+2]
   thr[0] = 0.0;
   R.thr = thr;
+  R.floor_bins = (p->flags & TRJ_RDF_FLOOR_BINS) != 0;
   if ((r = ensure(c, c->thr, sizeof(double) * thr.size()))) return r;
   if ((r = ensure(c, c->counts, sizeof(int64_t) * R.nit * R.lsmax))) return r;
   // Species offsets of the last frame, at their largest size: the result

@@ -24,6 +24,8 @@ struct RdfPlan {
   // Exact bin edges in r^2: thr[m] = smallest double r2 with
   // nint(sqrt(r2)/deltar) >= m, m = 1..lsmax+1 (thr[0] unused).
   std::vector<double> thr;
+  bool floor_bins = false;  // thr holds the edges of int(r/deltar) + 1 (TRJ_RDF_FLOOR_BINS)
+  std::vector<double> thr_xy;  // edges of the slice rule int(r/deltar) + 1, when slices are set
 };
 
 // rdf.cuf:55-58 after the cap of netcdf.f90:610-612.
