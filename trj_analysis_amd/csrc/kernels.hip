@@ -821,6 +821,13 @@ __device__ inline float row_bcast(float v) {
                                                                false));
 }
 
+// row_bcast for an integer; bound_ctrl (no old value: the source lane of a
+// row broadcast always exists).
+template <int K>
+__device__ inline int row_bcast_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x150 + K, 0xF, 0xF, true);
+}
+
 struct PfTables {
   float cut;  // cut <= side2 - pf_delta
 };
@@ -919,13 +926,16 @@ __device__ inline void bin_hits_pf(const PairArgs& a, const double* thr, const P
 }
 
 // Step K of a batch: home lanes against neighbour 16 r + K of the batch.
-template <int K, bool D3, bool SELF, bool QQ>
+// With CHECK the step is skipped when no row holds a neighbour at K (only a
+// batch of fewer than 16 neighbours needs the test).
+template <int K, bool D3, bool SELF, bool QQ, bool CHECK>
 __device__ inline void pf_step(const PairArgs& a, const double* thr, const PfTables& tb, uint32_t* hist, PfWave& w,
                                int lane, int jb, int je, int gi, int ti, int jrow, float hx, float hy, float hz,
                                const float4& q, const double* __restrict__ sx, const double* __restrict__ sy,
                                const double* __restrict__ sz, const int32_t* __restrict__ stype,
                                const double* __restrict__ sq, unsigned long long* gqs, double qsc) {
-  if (jb + K >= je) return;  // no row holds a neighbour at K (row 0 fills first)
+  if constexpr (CHECK)
+    if (jb + K >= je) return;  // no row holds a neighbour at K (row 0 fills first)
   const float dx = row_bcast<K>(q.x) - hx, dy = row_bcast<K>(q.y) - hy;
   float r2 = fmaf(dy, dy, dx * dx);
   if constexpr (D3) {
@@ -935,7 +945,7 @@ __device__ inline void pf_step(const PairArgs& a, const double* thr, const PfTab
   const int gj = jrow + K;
   // the neighbour's species (broadcast with every lane active: a DPP read of
   // a lane outside EXEC would not return its value)
-  const int tj = __builtin_bit_cast(int, row_bcast<K>(q.w));
+  const int tj = row_bcast_i<K>(__builtin_bit_cast(int, q.w));
   bool hit = r2 < a.side2_pf;
   if constexpr (SELF) hit = hit && (gj > gi);
   const unsigned long long m = __ballot(hit);
@@ -955,15 +965,15 @@ __device__ inline void pf_step(const PairArgs& a, const double* thr, const PfTab
   }
 }
 
-template <bool D3, bool SELF, bool QQ, int... K>
+template <bool D3, bool SELF, bool QQ, bool CHECK, int... K>
 __device__ inline void pf_batch(const PairArgs& a, const double* thr, const PfTables& tb, uint32_t* hist, PfWave& w,
                                 int lane, int jb, int je, int gi, int ti, int jrow, float hx, float hy, float hz,
                                 const float4& q, const double* __restrict__ sx, const double* __restrict__ sy,
                                 const double* __restrict__ sz, const int32_t* __restrict__ stype,
                                 const double* __restrict__ sq, unsigned long long* gqs, double qsc,
                                 std::integer_sequence<int, K...>) {
-  (pf_step<K, D3, SELF, QQ>(a, thr, tb, hist, w, lane, jb, je, gi, ti, jrow, hx, hy, hz, q, sx, sy, sz, stype, sq, gqs,
-                            qsc),
+  (pf_step<K, D3, SELF, QQ, CHECK>(a, thr, tb, hist, w, lane, jb, je, gi, ti, jrow, hx, hy, hz, q, sx, sy, sz, stype,
+                                   sq, gqs, qsc),
    ...);
 }
 
@@ -989,8 +999,12 @@ __device__ inline void rdf_neighbours_pf(const PairArgs& a, const double* thr, c
     q.y = vl ? static_cast<float>(sy[jl] - oy) : fnan;
     q.z = (D3 && vl) ? static_cast<float>(sz[jl] - oz) : 0.f;
     q.w = __builtin_bit_cast(float, tl);
-    pf_batch<D3, SELF, QQ>(a, thr, tb, hist, w, lane, jb, je, gi, ti, jb + r16, hx, hy, hz, q, sx, sy, sz, stype, sq,
-                           gqs, qsc, std::make_integer_sequence<int, kRow>{});
+    if (je - jb >= kRow)  // every step has a neighbour in row 0
+      pf_batch<D3, SELF, QQ, false>(a, thr, tb, hist, w, lane, jb, je, gi, ti, jb + r16, hx, hy, hz, q, sx, sy, sz,
+                                    stype, sq, gqs, qsc, std::make_integer_sequence<int, kRow>{});
+    else
+      pf_batch<D3, SELF, QQ, true>(a, thr, tb, hist, w, lane, jb, je, gi, ti, jb + r16, hx, hy, hz, q, sx, sy, sz,
+                                   stype, sq, gqs, qsc, std::make_integer_sequence<int, kRow>{});
   }
 }
 
