@@ -1,6 +1,6 @@
 """Plain device copy of a known size, for calibrating HBM byte counters.
 
-    python -m trj_analysis_amd.hbm_copy [MiB] [repeats]
+    python hbm_copy.py [MiB] [repeats]
 
 Copies a float32 buffer of MiB mebibytes into another one `repeats` times
 (torch's copy kernel): each launch reads and writes exactly MiB MiB, so the
