@@ -14,7 +14,9 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libtrj_oracle.so")
+# TRJ_ORACLE_LIB: another build of the oracle (the mutation tests load copies
+# with one planted mistake each).
+_LIB_PATH = os.environ.get("TRJ_ORACLE_LIB") or os.path.join(_HERE, "libtrj_oracle.so")
 _lib = None
 
 
