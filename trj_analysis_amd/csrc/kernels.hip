@@ -1029,32 +1029,63 @@ __device__ inline void rdf_home_chunk_pf(const PairArgs& a, const double* thr, c
   const float hx = static_cast<float>(oki ? sx[gi] - ox : nan);
   const float hy = static_cast<float>(oki ? sy[gi] - oy : nan);
   const float hz = D3 ? static_cast<float>(oki ? sz[gi] - oz : nan) : 0.f;
-  for (int o = 0; o < st.n; ++o) {
-    int nx = cx + st.off[o][0], ny = cy + st.off[o][1], nz = cz + st.off[o][2];
-    // s = +1 below cell 0 (d ~ +L: subtract +L), -1 above the top cell
-    const int sxi = nx < 0 ? 1 : (nx >= ncx ? -1 : 0);
-    const int syi = ny < 0 ? 1 : (ny >= ncy ? -1 : 0);
-    const int szi = D3 ? (nz < 0 ? 1 : (nz >= ncz ? -1 : 0)) : 0;
-    if (g.periodic[0]) nx = (nx + ncx) % ncx; else if (nx < 0 || nx >= ncx) continue;
-    if (g.periodic[1]) ny = (ny + ncy) % ncy; else if (ny < 0 || ny >= ncy) continue;
-    if (D3) {
-      if (g.periodic[2]) nz = (nz + ncz) % ncz; else if (nz < 0 || nz >= ncz) continue;
+  // The forward half stencil as runs of x-adjacent cells, which are adjacent
+  // in the cell order and so share one range of the cell-sorted arrays and
+  // one image shift: the home cell with its +x neighbour (from the chunk's
+  // first atom; j > i also holds for the next cell, whose atoms all come
+  // later), then the rows (dy, dz) = (+1, 0) and, in 3D, (-1..+1, +1), each
+  // over x = cx-1 .. cx+1.  A cell across a periodic x boundary is a run of
+  // its own with the x shift; across a non-periodic one it is skipped.
+  // Shift s = +1 below cell 0 (d ~ +L: subtract +L), -1 above the top cell;
+  // neighbours are taken relative to corner + s: x_j - (corner + s).
+  {
+    // the home run: own cell (j > i) and, without a wrap, the +x cell
+    const int je = __builtin_amdgcn_readfirstlane(cx + 1 < ncx ? start[cellL + 2] : ce);
+    rdf_neighbours_pf<D3, true, QQ>(a, thr, tb, hist, w, lane, h0, je, gi, ti, hx, hy, hz, ox, oy, oz, sx, sy, sz,
+                                    stype, sq, gqs, qsc);
+  }
+  // The other runs, one call site: q = 0 the wrapped +x cell of the home row;
+  // q = 1 + 3 r + k for row r (r = 0: (dy, dz) = (+1, 0); r = 1..3: (r - 2,
+  // +1)) and k = 0 the cell across the -x boundary, 1 the cells inside, 2 the
+  // cell across the +x boundary.
+  const int nq = D3 ? 13 : 4;
+  for (int q = 0; q < nq; ++q) {
+    int c0, c1, sxi, syi = 0, szi = 0;
+    if (q == 0) {
+      if (!(cx + 1 >= ncx && g.periodic[0])) continue;
+      c0 = c1 = cellL - cx;
+      sxi = -1;
+    } else {
+      const int r = (q - 1) / 3, k = (q - 1) % 3;
+      int ny = cy + (r == 0 ? 1 : r - 2), nz = cz + (r == 0 ? 0 : 1);
+      syi = ny < 0 ? 1 : (ny >= ncy ? -1 : 0);
+      szi = D3 ? (nz < 0 ? 1 : (nz >= ncz ? -1 : 0)) : 0;
+      if (g.periodic[1]) ny = (ny + ncy) % ncy; else if (ny < 0 || ny >= ncy) continue;
+      if (D3) {
+        if (g.periodic[2]) nz = (nz + ncz) % ncz; else if (nz < 0 || nz >= ncz) continue;
+      }
+      const int base = (nz * ncy + ny) * ncx;
+      if (k == 0) {
+        if (!(cx - 1 < 0 && g.periodic[0])) continue;
+        c0 = c1 = base + ncx - 1;
+        sxi = 1;
+      } else if (k == 1) {
+        c0 = base + (cx - 1 < 0 ? 0 : cx - 1);
+        c1 = base + (cx + 1 >= ncx ? ncx - 1 : cx + 1);
+        sxi = 0;
+      } else {
+        if (!(cx + 1 >= ncx && g.periodic[0])) continue;
+        c0 = c1 = base;
+        sxi = -1;
+      }
     }
-    // neighbour x_j - s, relative to the corner: x_j - (corner + s)
     const double px = ox + static_cast<double>(sxi) * a.lmi[0];
     const double py = oy + static_cast<double>(syi) * a.lmi[1];
     const double pz = D3 ? oz + static_cast<double>(szi) * a.lmi[2] : 0.0;
-    if (o == 0) {
-      // own cell: j > i; neighbours below the chunk are all below its atoms
-      rdf_neighbours_pf<D3, true, QQ>(a, thr, tb, hist, w, lane, h0, ce, gi, ti, hx, hy, hz, px, py, pz, sx, sy, sz,
-                                      stype, sq, gqs, qsc);
-    } else {
-      const int nb = (nz * ncy + ny) * ncx + nx;
-      const int js = __builtin_amdgcn_readfirstlane(start[nb]);
-      const int je = __builtin_amdgcn_readfirstlane(start[nb + 1]);
-      rdf_neighbours_pf<D3, false, QQ>(a, thr, tb, hist, w, lane, js, je, gi, ti, hx, hy, hz, px, py, pz, sx, sy, sz,
-                                       stype, sq, gqs, qsc);
-    }
+    const int js = __builtin_amdgcn_readfirstlane(start[c0]);
+    const int je = __builtin_amdgcn_readfirstlane(start[c1 + 1]);
+    rdf_neighbours_pf<D3, false, QQ>(a, thr, tb, hist, w, lane, js, je, gi, ti, hx, hy, hz, px, py, pz, sx, sy, sz,
+                                     stype, sq, gqs, qsc);
   }
 }
 
