@@ -157,6 +157,24 @@ def test_cell_shift_path_bit_exact(L, rc, n):
     np.testing.assert_array_equal(g, o)
 
 
+@pytest.mark.parametrize("ndim,L,rc,n", [(3, 230.0, 5.0, 30000), (2, 1800.0, 5.0, 40000)])
+def test_sparse_grid_atomic_cell_path(ndim, L, rc, n):
+    """Grids of more than 81920 cells (45^3, 359^2) overflow the LDS of the
+    per-workgroup cell counts: the cells are sorted by the global-atomic count
+    and scatter instead; the counts match all the same."""
+    rng = np.random.default_rng(ndim * 100 + n % 97)
+    pos = rng.random((n, 3)) * L
+    if ndim == 2:
+        pos[:, 2] = 0.0
+    it = rng.integers(1, 3, n).astype(np.int32)
+    box = [L, L, L if ndim == 3 else 1.0]
+    print("case: sparse grid", ndim, L, n, flush=True)
+    g, _ = _gpu_counts([(pos[:, :ndim] if ndim == 2 else pos, it, box)], ndim, 2, 0.02, rc)
+    o = _oracle_counts([(pos, it, box)], ndim, 2, 0.02, rc, method="cells")
+    np.testing.assert_array_equal(g, o)
+    assert g.sum() > 1000
+
+
 def test_lds_atomic_probe():
     print("case: LDS-atomic rate probe", flush=True)
     r = T.lds_atomic_rate(0)

@@ -75,6 +75,28 @@ void launch_cell_scatter(const double* pos, const int32_t* itype, const int32_t*
                          int64_t n, const int32_t* start, int32_t* cursor, double* spos,
                          int32_t* stype, const double* q, double* sq, hipStream_t s);
 
+// Cell sort without global atomics (k_cell_hist, k_cell_colscan, the
+// exclusive scan, k_cell_place): `blocks` workgroups of 1024 threads take
+// `chunk` atoms each and count them per cell in lds_bytes of LDS.  lds is
+// false where the counters do not fit the device's LDS or a chunk would pass
+// 65535 atoms; the global-atomic count and scatter above serve then.
+constexpr int kCellSortBlocks = 256;
+struct CellSort {
+  bool lds = false;
+  int blocks = 0;
+  int64_t chunk = 0;
+  size_t lds_bytes = 0;
+};
+CellSort cell_sort_plan(int64_t n, int64_t ncell, size_t lds_max);
+// rank: n entries; blkcnt: blocks * ncell; count and start: ncell + 1 (count
+// needs no clearing).  key (may be NULL): the slice keys, as for
+// launch_cell_count_slices, with nslice slices and ncell = nslice * ncx * ncy
+// + 1; else the cells of a frame in ndim dimensions.
+void launch_cell_sort(const CellSort& p, const double* pos, const int32_t* key, int nsp, int nslice, int64_t n,
+                      int ndim, const CellGrid& g, int64_t ncell, int32_t* rank, int32_t* blkcnt, int32_t* count,
+                      int32_t* start, const int32_t* itype, double* spos, int32_t* stype, const double* q,
+                      double* sq, hipStream_t s);
+
 struct PairArgs {
   int ndim, nsp, nit, lsmax;
   // Histogram tile of one pass: pair-type rows row0 .. row0+nrows-1 of the

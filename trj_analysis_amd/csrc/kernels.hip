@@ -99,28 +99,52 @@ __global__ __launch_bounds__(256) void k_species_count(const int32_t* __restrict
   }
 }
 
-// Single-workgroup exclusive scan (count up to a few 1e5 entries).
+// Single-workgroup exclusive scan (count up to a few 1e5 entries): 4096
+// entries a step, four consecutive ones per thread (a serial scan in
+// registers), then the 64 lanes of each wavefront by shuffles and the 16
+// wavefront totals through LDS.
 __global__ __launch_bounds__(1024) void k_exclusive_scan(const int32_t* __restrict__ in,
                                                          int32_t* __restrict__ out, int64_t count) {
-  __shared__ int32_t tmp[1024];
-  __shared__ int32_t carry;
-  const int t = threadIdx.x;
-  if (t == 0) carry = 0;
-  __syncthreads();
-  for (int64_t base = 0; base < count; base += 1024) {
-    const int64_t i = base + t;
-    const int32_t v = (i < count) ? in[i] : 0;
-    tmp[t] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const int32_t x = (t >= off) ? tmp[t - off] : 0;
-      __syncthreads();
-      tmp[t] += x;
-      __syncthreads();
+  __shared__ int32_t wsum[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+  int32_t carry = 0;
+  for (int64_t base = 0; base < count; base += 4096) {
+    const int64_t i = base + 4 * static_cast<int64_t>(t);
+    const bool whole = vec && i + 3 < count;
+    int32_t v[4];
+    if (whole) {
+      const int4 q = *reinterpret_cast<const int4*>(in + i);
+      v[0] = q.x;
+      v[1] = q.y;
+      v[2] = q.z;
+      v[3] = q.w;
+    } else {
+      for (int k = 0; k < 4; ++k) v[k] = i + k < count ? in[i + k] : 0;
     }
-    if (i < count) out[i] = carry + tmp[t] - v;
+    const int32_t s1 = v[0], s2 = s1 + v[1], s3 = s2 + v[2], s4 = s3 + v[3];
+    int32_t x = s4;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int32_t y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
     __syncthreads();
-    if (t == 1023) carry += tmp[1023];
+    int32_t wb = 0, tot = 0;
+    for (int k = 0; k < 16; ++k) {
+      const int32_t sk = wsum[k];
+      wb += k < w ? sk : 0;
+      tot += sk;
+    }
+    const int32_t ex = carry + wb + x - s4;
+    if (whole) {
+      *reinterpret_cast<int4*>(out + i) = make_int4(ex, ex + s1, ex + s2, ex + s3);
+    } else {
+      const int32_t e[4] = {ex, ex + s1, ex + s2, ex + s3};
+      for (int k = 0; k < 4; ++k)
+        if (i + k < count) out[i + k] = e[k];
+    }
+    carry += tot;
     __syncthreads();
   }
 }
@@ -257,6 +281,31 @@ __global__ __launch_bounds__(256) void k_slice_counts(const int32_t* __restrict_
 // Cell of each atom for the slice pairs: 2D cells of slice sl numbered
 // sl * ncell2 + cy * ncx + cx; atoms outside every slice go to cell
 // nslice * ncell2 (never visited).
+__device__ __forceinline__ int slice_cell(const double* __restrict__ pos, const int32_t* __restrict__ key,
+                                          int64_t n, int64_t i, int nsp, const CellGrid& g, int nslice) {
+  const int ncell2 = g.nc[0] * g.nc[1];
+  const int k = key[i];
+  if (k < 0) return nslice * ncell2;
+  int c[2];
+  for (int d = 0; d < 2; ++d) {
+    const int ck = static_cast<int>(floor(pos[d * n + i] * g.inv_h[d]));
+    c[d] = ck < 0 ? 0 : (ck >= g.nc[d] ? g.nc[d] - 1 : ck);
+  }
+  return (k / nsp) * ncell2 + c[1] * g.nc[0] + c[0];
+}
+
+// Cell of each atom of a frame: (cz * ncy + cy) * ncx + cx, coordinates
+// clamped into the grid.
+__device__ __forceinline__ int frame_cell(const double* __restrict__ pos, int64_t n, int64_t i, int ndim,
+                                          const CellGrid& g) {
+  int c[3] = {0, 0, 0};
+  for (int k = 0; k < ndim; ++k) {
+    const int ck = static_cast<int>(floor(pos[k * n + i] * g.inv_h[k]));
+    c[k] = ck < 0 ? 0 : (ck >= g.nc[k] ? g.nc[k] - 1 : ck);
+  }
+  return (c[2] * g.nc[1] + c[1]) * g.nc[0] + c[0];
+}
+
 __global__ __launch_bounds__(256) void k_cell_count_slices(const double* __restrict__ pos,
                                                            const int32_t* __restrict__ key, int64_t n, int nsp,
                                                            CellGrid g, int nslice,
@@ -264,17 +313,7 @@ __global__ __launch_bounds__(256) void k_cell_count_slices(const double* __restr
                                                            int32_t* __restrict__ count) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n) return;
-  const int ncell2 = g.nc[0] * g.nc[1];
-  const int k = key[i];
-  int id = nslice * ncell2;
-  if (k >= 0) {
-    int c[2];
-    for (int d = 0; d < 2; ++d) {
-      int ck = static_cast<int>(floor(pos[d * n + i] * g.inv_h[d]));
-      c[d] = ck < 0 ? 0 : (ck >= g.nc[d] ? g.nc[d] - 1 : ck);
-    }
-    id = (k / nsp) * ncell2 + c[1] * g.nc[0] + c[0];
-  }
+  const int id = slice_cell(pos, key, n, i, nsp, g, nslice);
   cellid[i] = id;
   atomicAdd(&count[id], 1);
 }
@@ -324,14 +363,7 @@ __global__ __launch_bounds__(256) void k_cell_count(const double* __restrict__ p
                                                     int32_t* __restrict__ count) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n) return;
-  int c[3] = {0, 0, 0};
-  for (int k = 0; k < ndim; ++k) {
-    const double x = pos[k * n + i];
-    int ck = static_cast<int>(floor(x * g.inv_h[k]));
-    ck = ck < 0 ? 0 : (ck >= g.nc[k] ? g.nc[k] - 1 : ck);
-    c[k] = ck;
-  }
-  const int id = (c[2] * g.nc[1] + c[1]) * g.nc[0] + c[0];
+  const int id = frame_cell(pos, n, i, ndim, g);
   cellid[i] = id;
   atomicAdd(&count[id], 1);
 }
@@ -356,6 +388,98 @@ __global__ __launch_bounds__(256) void k_cell_scatter(const double* __restrict__
   spos[2 * n + slot] = z;
   stype[slot] = t;
   if (q) sq[slot] = q[i];
+}
+
+// Cell sort without global atomics.  Workgroup b takes atoms
+// [b * chunk, (b + 1) * chunk) and counts them per cell in LDS, two 16-bit
+// counters to a word (chunk <= 65535, so a counter never carries into its
+// neighbour); the value an atom's LDS atomic returns is its rank among the
+// chunk's atoms in its cell.  blkcnt[b * ncell + c] receives the chunk's
+// count in cell c.  With key (slice pairs) the cells are slice_cell's, else
+// frame_cell's.
+__global__ __launch_bounds__(1024) void k_cell_hist(const double* __restrict__ pos,
+                                                    const int32_t* __restrict__ key, int nsp, int nslice,
+                                                    int64_t n, int ndim, CellGrid g, int ncell, int64_t chunk,
+                                                    int32_t* __restrict__ rank, int32_t* __restrict__ blkcnt) {
+  extern __shared__ uint32_t hcell[];
+  const int nw = (ncell + 1) >> 1;
+  for (int k = threadIdx.x; k < nw; k += 1024) hcell[k] = 0u;
+  __syncthreads();
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * chunk;
+  const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 1024) {
+    const int id = key ? slice_cell(pos, key, n, i, nsp, g, nslice) : frame_cell(pos, n, i, ndim, g);
+    const int sh = (id & 1) << 4;
+    const uint32_t old = atomicAdd(&hcell[id >> 1], 1u << sh);
+    rank[i] = static_cast<int32_t>((old >> sh) & 0xffffu);
+  }
+  __syncthreads();
+  int32_t* row = blkcnt + static_cast<int64_t>(blockIdx.x) * ncell;
+  for (int c = threadIdx.x; c < ncell; c += 1024) row[c] = static_cast<int32_t>((hcell[c >> 1] >> ((c & 1) << 4)) & 0xffffu);
+}
+
+// Per cell c (a lane each, 64 cells a workgroup): the exclusive prefix over
+// the workgroups of blkcnt[. * ncell + c], written back in place, and the
+// cell's total to count[c]; count[ncell] = 0 so that the exclusive scan of
+// count[0..ncell] ends at n.  Each of the 16 wavefronts takes up to 16 of
+// the <= 256 workgroup rows.
+__global__ __launch_bounds__(1024) void k_cell_colscan(int32_t* __restrict__ blkcnt, int nblk, int ncell,
+                                                       int32_t* __restrict__ count) {
+  __shared__ int32_t part[16][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + lane;
+  const int rpw = (nblk + 15) >> 4;
+  const int b0 = w * rpw;
+  int32_t v[16];
+  int32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int b = b0 + k;
+    v[k] = (k < rpw && b < nblk && c < ncell) ? blkcnt[static_cast<int64_t>(b) * ncell + c] : 0;
+    s += v[k];
+  }
+  part[w][lane] = s;
+  __syncthreads();
+  int32_t base = 0, tot = 0;
+  for (int k = 0; k < 16; ++k) {
+    const int32_t p = part[k][lane];
+    base += k < w ? p : 0;
+    tot += p;
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int b = b0 + k;
+    if (k < rpw && b < nblk && c < ncell) blkcnt[static_cast<int64_t>(b) * ncell + c] = base;
+    base += v[k];
+  }
+  if (w == 0 && c <= ncell) count[c] = c < ncell ? tot : 0;
+}
+
+// Atom i of workgroup b's chunk goes to slot start[c] + blkcnt[b * ncell + c]
+// + rank[i] of its cell c (recomputed from its position, as k_cell_hist
+// found it): positions, types and charges in cell order.
+__global__ __launch_bounds__(1024) void k_cell_place(const double* __restrict__ pos,
+                                                     const int32_t* __restrict__ key, int nsp, int nslice,
+                                                     int64_t n, int ndim, CellGrid g, int ncell, int64_t chunk,
+                                                     const int32_t* __restrict__ rank,
+                                                     const int32_t* __restrict__ blkcnt,
+                                                     const int32_t* __restrict__ start,
+                                                     const int32_t* __restrict__ itype, double* __restrict__ spos,
+                                                     int32_t* __restrict__ stype, const double* __restrict__ q,
+                                                     double* __restrict__ sq) {
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * chunk;
+  const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
+  const int32_t* base = blkcnt + static_cast<int64_t>(blockIdx.x) * ncell;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 1024) {
+    const int c = key ? slice_cell(pos, key, n, i, nsp, g, nslice) : frame_cell(pos, n, i, ndim, g);
+    const int slot = start[c] + base[c] + rank[i];
+    const double x = pos[i], y = pos[n + i], z = pos[2 * n + i];
+    spos[slot] = x;
+    spos[n + slot] = y;
+    spos[2 * n + slot] = z;
+    stype[slot] = itype[i];
+    if (q) sq[slot] = q[i];
+  }
 }
 
 // rdf.cuf:748-752 for i <= j (callers order the pair).
@@ -2697,6 +2821,39 @@ void launch_cell_scatter(const double* pos, const int32_t* itype, const int32_t*
   if (n <= 0) return;
   hipLaunchKernelGGL(k_cell_scatter, dim3(grid1d(n, 256)), dim3(256), 0, s, pos, itype, cellid, n, start,
                      cursor, spos, stype, q, sq);
+}
+
+CellSort cell_sort_plan(int64_t n, int64_t ncell, size_t lds_max) {
+  CellSort p;
+  const int64_t words = (ncell + 1) / 2;
+  if (n <= 0 || ncell <= 0 || ncell > (int64_t(1) << 30) || static_cast<size_t>(words) * 4 > lds_max ||
+      n > int64_t(kCellSortBlocks) * 65535)
+    return p;
+  // at most 4 n per-workgroup counts, at most 65535 atoms a workgroup
+  int64_t b = std::min<int64_t>(kCellSortBlocks, std::max<int64_t>(1, 4 * n / ncell));
+  b = std::max<int64_t>(b, (n + 65534) / 65535);
+  p.lds = true;
+  p.blocks = static_cast<int>(b);
+  p.chunk = (n + b - 1) / b;
+  p.lds_bytes = static_cast<size_t>(words) * 4;
+  return p;
+}
+
+void launch_cell_sort(const CellSort& p, const double* pos, const int32_t* key, int nsp, int nslice, int64_t n,
+                      int ndim, const CellGrid& g, int64_t ncell, int32_t* rank, int32_t* blkcnt, int32_t* count,
+                      int32_t* start, const int32_t* itype, double* spos, int32_t* stype, const double* q,
+                      double* sq, hipStream_t s) {
+  const int nc = static_cast<int>(ncell);
+  if (p.lds_bytes > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_cell_hist), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(p.lds_bytes));
+  hipLaunchKernelGGL(k_cell_hist, dim3(p.blocks), dim3(1024), p.lds_bytes, s, pos, key, nsp, nslice, n, ndim, g, nc,
+                     p.chunk, rank, blkcnt);
+  hipLaunchKernelGGL(k_cell_colscan, dim3(static_cast<unsigned>((ncell + 1 + 63) / 64)), dim3(1024), 0, s, blkcnt,
+                     p.blocks, nc, count);
+  launch_exclusive_scan(count, start, ncell + 1, s);
+  hipLaunchKernelGGL(k_cell_place, dim3(p.blocks), dim3(1024), 0, s, pos, key, nsp, nslice, n, ndim, g, nc, p.chunk,
+                     rank, blkcnt, start, itype, spos, stype, q, sq);
 }
 
 size_t pair_lds_bytes(int rows, int lsmax, bool qq) {

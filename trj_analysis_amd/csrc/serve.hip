@@ -184,6 +184,7 @@ int64_t trj_request_footprint(const trj_request* r) {
       b += int64_t(2048) * nit * ls * 4;     // per-workgroup histogram slices
       b += int64_t(nit) * ls * 8 + int64_t(ls + 2) * 8;  // totals, bin edges
       b += 12 * (ncell + 1);                 // cell counts, starts, cursors
+      b += std::max<int64_t>(4 * n, ncell) * 4;  // per-workgroup cell counts of the LDS cell sort
     }
     b += n * (4 + 24 + 4);                   // cell ids, cell-sorted positions and types
   }

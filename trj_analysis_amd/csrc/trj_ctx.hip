@@ -48,7 +48,11 @@ struct trj_ctx {
   hipStream_t rdf_stream = nullptr;
   std::string err;
   // device buffers
-  DevBuf raw, itype, pos, spos, stype, cellid, cellcount, cellstart, cursor, partial, counts, thr, blk,
+  // (cellid: per-atom cell ids for the global-atomic cell count, or each
+  // atom's rank in its workgroup's cell count for the LDS cell sort, whose
+  // per-workgroup cell counts are cellblk; cursor: the atomic scatter's
+  // per-cell cursors.)
+  DevBuf raw, itype, pos, spos, stype, cellid, cellcount, cellstart, cursor, cellblk, partial, counts, thr, blk,
       blkoff, sp_off, u, rho, tiles, tile_shell, sqf, sqfp, axis_kv, axis_shell, listpart, flags;
   // Charges (TRJ_RDF_CHARGE / TRJ_SQ_CHARGE): the frame's charges, their
   // cell-sorted copy, the frame's fixed-point g_qq sums, the g_qq and S_qq
@@ -114,6 +118,10 @@ struct trj_ctx {
   // Set while trj_frame_device_noalloc runs: the context may then use only
   // buffers that already have their size (ensure() refuses to allocate).
   bool noalloc = false;
+  // Set by trj_frame_device_noalloc where the LDS cell sort's per-workgroup
+  // counts do not fit its scratch: the frame then sorts its cells with the
+  // global-atomic count and scatter.
+  bool atomic_cells = false;
   // Largest LDS allocation of one workgroup on this device.
   size_t lds_max = 65536;
   // Launch shape of the RDF pair kernels picked on the first frame after
@@ -141,7 +149,7 @@ std::vector<DevBuf*> dev_bufs(trj_ctx* c) {
           &c->slqsum,    &c->slqcnt,    &c->sqqsum,   &c->sqqcnt,     &c->xyframe,  &c->xycounts, &c->xyacc,
           &c->xygqf,     &c->xygqacc,   &c->thr_xy,   &c->rpts,       &c->s2gi,     &c->s2g2i,    &c->fqkv,
           &c->fqbin,     &c->fqr0,      &c->fqrho0,   &c->fqpart,     &c->fqvsum,   &c->fqfkt,    &c->fqfskt,
-          &c->fqtot,     &c->fqtots};
+          &c->fqtot,     &c->fqtots,    &c->cellblk};
 }
 
 }  // namespace
@@ -326,18 +334,26 @@ int rdf_slices(trj_ctx* c, PairArgs pa, const FrameGeom& g, const double box[3],
   if ((r = ensure(c, c->cellid, sizeof(int32_t) * n))) return r;
   if ((r = ensure(c, c->cellcount, sizeof(int32_t) * (ncell + 1)))) return r;
   if ((r = ensure(c, c->cellstart, sizeof(int32_t) * (ncell + 1)))) return r;
-  if ((r = ensure(c, c->cursor, sizeof(int32_t) * ncell))) return r;
   if ((r = ensure(c, c->spos, sizeof(double) * 3 * n))) return r;
   if ((r = ensure(c, c->stype, sizeof(int32_t) * n))) return r;
-  HIPC(c, hipMemsetAsync(c->cellcount.p, 0, sizeof(int32_t) * (ncell + 1), rs));
-  HIPC(c, hipMemsetAsync(c->cursor.p, 0, sizeof(int32_t) * ncell, rs));
-  launch_cell_count_slices(P<double>(c->pos), P<int32_t>(c->slkey), n, nsp, cg, nsl, P<int32_t>(c->cellid),
-                           P<int32_t>(c->cellcount), rs);
-  launch_exclusive_scan(P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), ncell + 1, rs);
   if (q_dev && (r = ensure(c, c->qsort, sizeof(double) * n))) return r;
-  launch_cell_scatter(P<double>(c->pos), itype_dev, P<int32_t>(c->cellid), n, P<int32_t>(c->cellstart),
-                      P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype), q_dev,
-                      q_dev ? P<double>(c->qsort) : nullptr, rs);
+  const CellSort cs = cell_sort_plan(n, ncell, c->lds_max);
+  if (cs.lds) {
+    if ((r = ensure(c, c->cellblk, sizeof(int32_t) * cs.blocks * static_cast<size_t>(ncell)))) return r;
+    launch_cell_sort(cs, P<double>(c->pos), P<int32_t>(c->slkey), nsp, nsl, n, 2, cg, ncell, P<int32_t>(c->cellid),
+                     P<int32_t>(c->cellblk), P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), itype_dev,
+                     P<double>(c->spos), P<int32_t>(c->stype), q_dev, q_dev ? P<double>(c->qsort) : nullptr, rs);
+  } else {
+    if ((r = ensure(c, c->cursor, sizeof(int32_t) * ncell))) return r;
+    HIPC(c, hipMemsetAsync(c->cellcount.p, 0, sizeof(int32_t) * (ncell + 1), rs));
+    HIPC(c, hipMemsetAsync(c->cursor.p, 0, sizeof(int32_t) * ncell, rs));
+    launch_cell_count_slices(P<double>(c->pos), P<int32_t>(c->slkey), n, nsp, cg, nsl, P<int32_t>(c->cellid),
+                             P<int32_t>(c->cellcount), rs);
+    launch_exclusive_scan(P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), ncell + 1, rs);
+    launch_cell_scatter(P<double>(c->pos), itype_dev, P<int32_t>(c->cellid), n, P<int32_t>(c->cellstart),
+                        P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype), q_dev,
+                        q_dev ? P<double>(c->qsort) : nullptr, rs);
+  }
   if (q_dev) {
     pa.q = P<double>(c->qsort);
     pa.qp = P<double>(c->qparm);
@@ -751,19 +767,28 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
         if ((r = ensure(c, c->cellid, sizeof(int32_t) * n))) return r;
         if ((r = ensure(c, c->cellcount, sizeof(int32_t) * (ncell + 1)))) return r;
         if ((r = ensure(c, c->cellstart, sizeof(int32_t) * (ncell + 1)))) return r;
-        if ((r = ensure(c, c->cursor, sizeof(int32_t) * ncell))) return r;
         if ((r = ensure(c, c->spos, sizeof(double) * 3 * n))) return r;
         if ((r = ensure(c, c->stype, sizeof(int32_t) * n))) return r;
-        HIPC(c, hipMemsetAsync(c->cellcount.p, 0, sizeof(int32_t) * (ncell + 1), rs));
-        HIPC(c, hipMemsetAsync(c->cursor.p, 0, sizeof(int32_t) * ncell, rs));
-        launch_cell_count(P<double>(c->pos), n, ndim, cg, P<int32_t>(c->cellid), P<int32_t>(c->cellcount),
-                          rs);
-        launch_exclusive_scan(P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), ncell + 1, rs);
         const bool qq_pairs = rdf_q && !qq_species;
         if (qq_pairs && (r = ensure(c, c->qsort, sizeof(double) * n))) return r;
-        launch_cell_scatter(P<double>(c->pos), itype_dev, P<int32_t>(c->cellid), n, P<int32_t>(c->cellstart),
-                            P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype),
-                            qq_pairs ? q_dev : nullptr, qq_pairs ? P<double>(c->qsort) : nullptr, rs);
+        const CellSort cs = c->atomic_cells ? CellSort{} : cell_sort_plan(n, ncell, c->lds_max);
+        if (cs.lds) {
+          if ((r = ensure(c, c->cellblk, sizeof(int32_t) * cs.blocks * static_cast<size_t>(ncell)))) return r;
+          launch_cell_sort(cs, P<double>(c->pos), nullptr, 0, 0, n, ndim, cg, ncell, P<int32_t>(c->cellid),
+                           P<int32_t>(c->cellblk), P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), itype_dev,
+                           P<double>(c->spos), P<int32_t>(c->stype), qq_pairs ? q_dev : nullptr,
+                           qq_pairs ? P<double>(c->qsort) : nullptr, rs);
+        } else {
+          if ((r = ensure(c, c->cursor, sizeof(int32_t) * ncell))) return r;
+          HIPC(c, hipMemsetAsync(c->cellcount.p, 0, sizeof(int32_t) * (ncell + 1), rs));
+          HIPC(c, hipMemsetAsync(c->cursor.p, 0, sizeof(int32_t) * ncell, rs));
+          launch_cell_count(P<double>(c->pos), n, ndim, cg, P<int32_t>(c->cellid), P<int32_t>(c->cellcount),
+                            rs);
+          launch_exclusive_scan(P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), ncell + 1, rs);
+          launch_cell_scatter(P<double>(c->pos), itype_dev, P<int32_t>(c->cellid), n, P<int32_t>(c->cellstart),
+                              P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype),
+                              qq_pairs ? q_dev : nullptr, qq_pairs ? P<double>(c->qsort) : nullptr, rs);
+        }
         if (qq_pairs) pa.q = P<double>(c->qsort);
         if (!tu.done && tu.enabled && !c->noalloc && !qq_pairs) {
           if ((r = tune_pairs(c, pa, cg, rcl, n, rs))) return r;
@@ -1313,18 +1338,33 @@ int trj_frame_device_noalloc(trj_ctx* c, const void* pos_dev, int32_t dtype, int
   const size_t nblk = static_cast<size_t>((n + 255) / 256);
   const size_t un = static_cast<size_t>(n), nc = static_cast<size_t>(cp.ncell);
   const size_t cells = cp.cells_ok ? 1 : 0;
-  DevBuf* sw[] = {&c->pos, &c->blk, &c->blkoff, &c->cellid, &c->cellcount, &c->cellstart, &c->cursor,
-                  &c->spos, &c->stype, &c->partial};
-  const size_t len[] = {24 * un, 4 * R.nsp * nblk, 4 * R.nsp * nblk, cells * 4 * un, cells * 4 * (nc + 1),
-                        cells * 4 * (nc + 1), cells * 4 * nc, cells * 24 * un, cells * 4 * un,
-                        4 * static_cast<size_t>(R.nit) * R.lsmax * static_cast<size_t>(cp.nblocks)};
+  // The cell sort's own buffers: the LDS sort's per-workgroup cell counts, or
+  // the atomic scatter's per-cell cursors where those do not fit.
+  const CellSort cs = cp.cells_ok ? cell_sort_plan(n, cp.ncell, c->lds_max) : CellSort{};
+  DevBuf* sw[] = {&c->pos, &c->blk, &c->blkoff, &c->cellid, &c->cellcount, &c->cellstart, &c->spos,
+                  &c->stype, &c->partial, &c->cellblk, &c->cursor};
+  size_t len[] = {24 * un, 4 * R.nsp * nblk, 4 * R.nsp * nblk, cells * 4 * un, cells * 4 * (nc + 1),
+                  cells * 4 * (nc + 1), cells * 24 * un, cells * 4 * un,
+                  4 * static_cast<size_t>(R.nit) * R.lsmax * static_cast<size_t>(cp.nblocks),
+                  cs.lds ? 4 * static_cast<size_t>(cs.blocks) * nc : 0, cs.lds ? 0 : cells * 4 * nc};
   constexpr int nb = sizeof(len) / sizeof(len[0]);
   size_t off[nb], total = 0;
-  for (int k = 0; k < nb; ++k) {
-    off[k] = total;
-    total += (len[k] + 255) & ~size_t(255);
-  }
+  auto lay_out = [&]() {
+    total = 0;
+    for (int k = 0; k < nb; ++k) {
+      off[k] = total;
+      total += (len[k] + 255) & ~size_t(255);
+    }
+  };
+  lay_out();
   // 64 KiB stay free at the end of the array as a guard.
+  bool atomic_cells = false;
+  if (cs.lds && total + (size_t(64) << 10) > kScratchBytes) {
+    len[nb - 2] = 0;
+    len[nb - 1] = cells * 4 * nc;
+    atomic_cells = true;
+    lay_out();
+  }
   if (total + (size_t(64) << 10) > kScratchBytes)
     return fail(c, TRJ_ELIMIT, "allocation-free frame needs " + std::to_string(total) +
                                    " bytes of scratch; the library holds " + std::to_string(kScratchBytes) +
@@ -1336,8 +1376,10 @@ int trj_frame_device_noalloc(trj_ctx* c, const void* pos_dev, int32_t dtype, int
     sw[k]->cap = len[k];
   }
   c->noalloc = true;
+  c->atomic_cells = atomic_cells;
   const int r = run_frame(c, pos_dev, dtype, layout, itype_dev, n, box_in, TRJ_WANT_RDF);
   c->noalloc = false;
+  c->atomic_cells = false;
   for (int k = 0; k < nb; ++k) *sw[k] = saved[k];
   return r;
 }
