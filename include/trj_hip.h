@@ -531,6 +531,42 @@ int trj_batch_run(trj_batch* b, int32_t nframes, const trj_batch_frame* fr, cons
 int trj_batch_autotune(trj_batch* b, int32_t nframes, const trj_batch_frame* fr, const void* pos_dev,
                        int32_t dtype, const int32_t* itype_dev, trj_batch_params* chosen);
 
+/* Batched S(Q): the SQcalc of every frame (sq.cuf:58-126, called per frame by
+ * trj_analysis.f90:173-242), each frame on the half-space grid of its own box
+ * (TRJ_QSET_HALFSPACE: nx, ny in [-g/2, g/2), nz in [0, gz/2); q = 2 pi n / L
+ * with the frame's L; shells nint(|q| / dq), dq = min 2 pi / L, 1..nqmax with
+ * nqmax = min(nint(qmax / dq), the grid's largest shell)).  Frames are grouped
+ * (at most frames_per_launch, 4 GiB of partial sums and 2^30 atoms a group);
+ * a group's frames are sorted by species into one set of turn planes and run
+ * as channels of one contraction launch (splits of at most 32768 atoms in
+ * FP64, 8192 otherwise, as the per-frame path), then one shell pass.  With
+ * use_graph the group sequence is captured once and replayed per group.
+ * Every axis is wrapped (as trj_frame does for S(Q)-only frames); the frames'
+ * periodic flags are not used.  This is the per-frame path's S(Q) in one pass
+ * over many frames: a frame's sums equal trj_sq_shells after trj_sq_init on
+ * its box and one trj_frame, up to the order of FP64 additions. */
+typedef struct {
+  int32_t grid[3];    /* half-space grid extents (even, 2..4096) */
+  int32_t precision;  /* TRJ_PREC_* */
+  double qmax;        /* > 0 */
+} trj_batch_sq_params;
+
+/* Host-only: nqmax of each frame and the offsets of its outputs: sqf and nq
+ * hold nqmax+1 values at sqf_off[f], sqfp holds nsp x (nqmax+1) at
+ * sqfp_off[f]; totals through *total_sqf, *total_sqfp.  Any output may be NULL. */
+int trj_batch_sq_layout(const trj_batch_params* p, const trj_batch_sq_params* q, int32_t nframes,
+                        const trj_batch_frame* fr, int32_t* nqmax, int64_t* sqf_off, int64_t* sqfp_off,
+                        int64_t* total_sqf, int64_t* total_sqfp);
+
+/* Run all frames.  bsc: [nframes][10] scattering lengths (NULL: all 1).
+ * sqf: sum over a shell's vectors of |sum_s b_s rho_s|^2; sqfp: of |rho_s|^2;
+ * nq: vectors per shell (the raw sums of trj_sq_shells for each frame, at the
+ * trj_batch_sq_layout offsets).  Any output may be NULL; gpu_ms (may be NULL)
+ * receives the device time of the whole run.  Synchronises. */
+int trj_batch_sq_run(trj_batch* b, const trj_batch_sq_params* q, int32_t nframes, const trj_batch_frame* fr,
+                     const void* pos_dev, int32_t dtype, const int32_t* itype_dev, const double* bsc,
+                     double* sqf, double* sqfp, int64_t* nq, double* gpu_ms);
+
 /* ------------------------------------------------------------------------
  * Request server: continuous batching of independent analysis requests
  * (RDF and/or S(Q) of one frame each).  Each slot is a context with its own

@@ -48,6 +48,15 @@ struct trj_batch {
   BatchDev gdev{};
   int g_nblocks = 0;
   size_t g_lds = 0;
+  // batched S(Q) (trj_batch_sq_run): group descriptors, sort and contraction
+  // buffers, all frames' shell sums, and its own captured sequence
+  Buf sq_hdr, sq_fr, sq_items, sq_plane, sq_blk, sq_blkoff, sq_spoff, sq_u, sq_rho, sq_tiles, sq_sqf, sq_sqfp, sq_nq;
+  void* sq_pin[2] = {nullptr, nullptr};
+  size_t sq_pin_cap[2] = {0, 0};
+  hipEvent_t sq_pin_done[2] = {nullptr, nullptr};
+  hipGraphExec_t sq_gexec = nullptr;
+  BatchSqDev sq_gdev{};
+  int sq_gprec = -1;
 };
 
 namespace {
@@ -73,9 +82,13 @@ int bensure(trj_batch* b, Buf& x, size_t bytes) {
   const size_t want = bytes + bytes / 4;
   if (hipMalloc(&x.p, want) != hipSuccess) return bfail(b, TRJ_ENOMEM, "batch: device allocation failed");
   x.cap = want;
-  if (b->gexec) {  // pointers changed: the captured sequence is stale
+  if (b->gexec) {  // pointers changed: the captured sequences are stale
     (void)hipGraphExecDestroy(b->gexec);
     b->gexec = nullptr;
+  }
+  if (b->sq_gexec) {
+    (void)hipGraphExecDestroy(b->sq_gexec);
+    b->sq_gexec = nullptr;
   }
   return TRJ_OK;
 }
@@ -311,6 +324,329 @@ int run_groups(trj_batch* b, const trj_batch_params& p, int nframes, const trj_b
   return TRJ_OK;
 }
 
+// ------------------------------------------------------------- batched S(Q)
+// Per-frame S(Q) plan on the half-space grid of the frame's own box (the
+// rules of sq_plan for TRJ_QSET_HALFSPACE, plan.cpp): 2 pi / L per axis,
+// dq their minimum, nqmax = min(nint(qmax / dq), the largest shell on the
+// grid).  The largest shell is the grid corner's (|kx| = gx/2, |ky| = gy/2,
+// kz = gz/2 - 1): every operation of the shell rule is monotone in |k|.
+// Splits: at most 32768 atoms (FP64) or 8192 (FP32 accumulation) per
+// workgroup, the per-frame path's bound.
+struct SqFramePlan {
+  int nqmax, ks, nblk;
+  double fk[3], dq;
+  int64_t out_off, outp_off;
+};
+
+constexpr int64_t kSqRhoCap = int64_t(4) << 30;  // rho bytes of one group
+constexpr int64_t kSqAtomCap = int64_t(1) << 30;  // atoms of one group
+
+int plan_sq_frames(const trj_batch_params& p, const trj_batch_sq_params& q, int nframes, const trj_batch_frame* fr,
+                   std::vector<SqFramePlan>& out, int64_t* tot_sqf, int64_t* tot_sqfp, std::string* err) {
+  if (p.ndim != 2 && p.ndim != 3) {
+    *err = "batch sq: ndim must be 2 or 3";
+    return TRJ_EINVAL;
+  }
+  for (int k = 0; k < p.ndim; ++k)
+    if (q.grid[k] < 2 || (q.grid[k] % 2) != 0 || q.grid[k] > 4096) {
+      *err = "batch sq: half-space grid extents must be even, 2..4096";
+      return TRJ_EINVAL;
+    }
+  if (!(q.qmax > 0.0)) {
+    *err = "batch sq: qmax must be positive";
+    return TRJ_EINVAL;
+  }
+  if (q.precision < TRJ_PREC_FP64 || q.precision > TRJ_PREC_BF16X3) {
+    *err = "batch sq: unknown precision";
+    return TRJ_EINVAL;
+  }
+  const int64_t span = q.precision == TRJ_PREC_FP64 ? 32768 : 8192;
+  out.assign(nframes, SqFramePlan{});
+  int64_t o = 0, op = 0;
+  for (int f = 0; f < nframes; ++f) {
+    const trj_batch_frame& F = fr[f];
+    SqFramePlan& P = out[f];
+    if (F.n < 1 || F.n > kSqAtomCap || F.nsp < 1 || F.nsp > kNspMax) {
+      *err = "batch sq: frame " + std::to_string(f) + " has n outside 1..2^30 or nsp outside 1..10";
+      return TRJ_EINVAL;
+    }
+    double box[3] = {1.0, 1.0, 1.0};
+    for (int k = 0; k < p.ndim; ++k) {
+      if (!(F.box[k] > 0.0) || !std::isfinite(F.box[k])) {
+        *err = "batch sq: frame " + std::to_string(f) + " has a non-positive box length";
+        return TRJ_EINVAL;
+      }
+      box[k] = F.box[k];
+    }
+    for (int k = 0; k < 3; ++k) P.fk[k] = 2.0 * M_PI / box[k];
+    P.dq = std::fmin(P.fk[0], P.fk[1]);
+    if (p.ndim == 3) P.dq = std::fmin(P.dq, P.fk[2]);
+    const double qa = (-q.grid[0] / 2) * P.fk[0], qb = (-q.grid[1] / 2) * P.fk[1];
+    const double qc = p.ndim == 3 ? (q.grid[2] / 2 - 1) * P.fk[2] : 0.0;
+    const long maxshell = std::lround(std::sqrt(qa * qa + qb * qb + qc * qc) / P.dq);
+    const double qcap = std::floor(q.qmax / P.dq + 0.5);
+    P.nqmax = qcap < static_cast<double>(maxshell) ? static_cast<int>(qcap) : static_cast<int>(maxshell);
+    if (P.nqmax > 32767) {
+      *err = "batch sq: frame " + std::to_string(f) + " has nqmax above 32767";
+      return TRJ_ELIMIT;
+    }
+    P.ks = static_cast<int>((F.n + span - 1) / span);
+    P.nblk = static_cast<int>((F.n + 255) / 256);
+    P.out_off = o;
+    P.outp_off = op;
+    o += P.nqmax + 1;
+    op += static_cast<int64_t>(F.nsp) * (P.nqmax + 1);
+  }
+  *tot_sqf = o;
+  *tot_sqfp = op;
+  return TRJ_OK;
+}
+
+// The half-space tiles of the contraction kernels for `precision`: every
+// tile of the dense box, all blocks marked (each frame's shells pick their
+// vectors later).
+std::vector<int32_t> halfspace_tiles(int ndim, const int32_t grid[3], int precision, int kmax[3]) {
+  const TileShape tk = tile_shape(precision);
+  const int kmin[3] = {-grid[0] / 2, -grid[1] / 2, 0};
+  kmax[0] = grid[0] / 2 - 1;
+  kmax[1] = grid[1] / 2 - 1;
+  kmax[2] = ndim == 3 ? grid[2] / 2 - 1 : 0;
+  std::vector<int32_t> t;
+  for (int x0 = kmin[0]; x0 <= kmax[0]; x0 += tk.tx)
+    for (int y0 = kmin[1]; y0 <= kmax[1]; y0 += tk.ty)
+      for (int z0 = kmin[2]; z0 <= kmax[2]; z0 += tk.tz) {
+        t.push_back(x0);
+        t.push_back(y0);
+        t.push_back(z0);
+        t.push_back(-1);
+      }
+  return t;
+}
+
+int run_sq_groups(trj_batch* b, const trj_batch_sq_params& q, int nframes, const trj_batch_frame* fr,
+                  const void* pos_dev, int dtype, const int32_t* itype_dev, const double* bsc, double* sqf_host,
+                  double* sqfp_host, int64_t* nq_host, double* gpu_ms) {
+  const trj_batch_params& p = b->p;
+  std::vector<SqFramePlan> plan;
+  int64_t tot_sqf = 0, tot_sqfp = 0;
+  int r = plan_sq_frames(p, q, nframes, fr, plan, &tot_sqf, &tot_sqfp, &b->err);
+  if (r) return r;
+  if (dtype != TRJ_F32 && dtype != TRJ_F64) return bfail(b, TRJ_EINVAL, "batch sq: bad dtype");
+  int kmax[3];
+  const std::vector<int32_t> tiles = halfspace_tiles(p.ndim, q.grid, q.precision, kmax);
+  const int ntiles = static_cast<int>(tiles.size() / 4);
+  const TileShape tk = tile_shape(q.precision);
+  const int64_t plane_bytes = int64_t(16) * ntiles * tk.vec();
+  // Groups: consecutive frames, at most frames_per_launch (0: 64), their rho
+  // planes within kSqRhoCap and their atoms within kSqAtomCap.
+  const int F = p.frames_per_launch > 0 ? p.frames_per_launch : 64;
+  std::vector<int> gstart;
+  int64_t cap_atoms = 1, cap_counts = 1, cap_planes = 1;
+  int cap_items = 1, cap_frames = 1, cap_nch = 1, cap_blocks = 1;
+  {
+    int f = 0;
+    while (f < nframes) {
+      const int64_t planes0 = int64_t(plan[f].ks) * fr[f].nsp;
+      if (planes0 * plane_bytes > kSqRhoCap)
+        return bfail(b, TRJ_ELIMIT, "batch sq: frame " + std::to_string(f) +
+                                        " alone needs more than 4 GiB of partial sums; use the per-frame path");
+      gstart.push_back(f);
+      int64_t atoms = 0, counts = 0, planes = 0;
+      int items = 0, nch = 0, blocks = 0, nf = 0;
+      while (f < nframes && nf < F) {
+        const int64_t pl = int64_t(plan[f].ks) * fr[f].nsp;
+        if (nf > 0 && ((planes + pl) * plane_bytes > kSqRhoCap || atoms + fr[f].n > kSqAtomCap)) break;
+        atoms += fr[f].n;
+        counts += int64_t(fr[f].nsp) * plan[f].nblk;
+        planes += pl;
+        items += plan[f].ks;
+        nch += fr[f].nsp;
+        blocks += plan[f].nblk;
+        ++nf;
+        ++f;
+      }
+      cap_atoms = std::max(cap_atoms, atoms);
+      cap_counts = std::max(cap_counts, counts);
+      cap_planes = std::max(cap_planes, planes);
+      cap_items = std::max(cap_items, items);
+      cap_frames = std::max(cap_frames, nf);
+      cap_nch = std::max(cap_nch, nch);
+      cap_blocks = std::max(cap_blocks, blocks);
+    }
+  }
+  gstart.push_back(nframes);
+  const int ngroups = static_cast<int>(gstart.size()) - 1;
+  // Shell kernel: LDS accumulators for the largest frame's shells when they fit.
+  int max_acc = 0;
+  for (int f = 0; f < nframes; ++f) max_acc = std::max(max_acc, (2 + fr[f].nsp) * (plan[f].nqmax + 1));
+  const int shell_lds = max_acc * 8 <= 48 * 1024 ? max_acc : 0;
+  if ((r = bensure(b, b->sq_hdr, sizeof(BatchSqHeader)))) return r;
+  if ((r = bensure(b, b->sq_fr, sizeof(BatchSqFrame) * cap_frames))) return r;
+  if ((r = bensure(b, b->sq_items, sizeof(int4) * cap_items))) return r;
+  if ((r = bensure(b, b->sq_plane, sizeof(int32_t) * cap_items))) return r;
+  if ((r = bensure(b, b->sq_blk, sizeof(int32_t) * cap_counts))) return r;
+  if ((r = bensure(b, b->sq_blkoff, sizeof(int32_t) * cap_counts))) return r;
+  if ((r = bensure(b, b->sq_spoff, sizeof(int32_t) * (cap_nch + 1)))) return r;
+  if ((r = bensure(b, b->sq_u, sizeof(double) * 12 * cap_atoms))) return r;
+  if ((r = bensure(b, b->sq_rho, static_cast<size_t>(plane_bytes) * cap_planes))) return r;
+  if ((r = bensure(b, b->sq_tiles, sizeof(int32_t) * tiles.size()))) return r;
+  if ((r = bensure(b, b->sq_sqf, sizeof(double) * tot_sqf))) return r;
+  if ((r = bensure(b, b->sq_sqfp, sizeof(double) * tot_sqfp))) return r;
+  if ((r = bensure(b, b->sq_nq, sizeof(int64_t) * tot_sqf))) return r;
+  if ((r = bensure(b, b->flags, sizeof(uint32_t)))) return r;
+  BHIP(b, hipMemcpyAsync(b->sq_tiles.p, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice,
+                         b->stream));
+  BHIP(b, hipMemsetAsync(b->sq_sqf.p, 0, sizeof(double) * tot_sqf, b->stream));
+  BHIP(b, hipMemsetAsync(b->sq_sqfp.p, 0, sizeof(double) * tot_sqfp, b->stream));
+  BHIP(b, hipMemsetAsync(b->sq_nq.p, 0, sizeof(int64_t) * tot_sqf, b->stream));
+  BHIP(b, hipMemsetAsync(b->flags.p, 0, sizeof(uint32_t), b->stream));
+  BatchSqDev d{};
+  d.hdr = static_cast<const BatchSqHeader*>(b->sq_hdr.p);
+  d.fr = static_cast<const BatchSqFrame*>(b->sq_fr.p);
+  d.pos_in = pos_dev;
+  d.itype_in = itype_dev;
+  d.dtype = dtype;
+  d.ndim = p.ndim;
+  d.blk = static_cast<int32_t*>(b->sq_blk.p);
+  d.blkoff = static_cast<int32_t*>(b->sq_blkoff.p);
+  d.sp_off = static_cast<int32_t*>(b->sq_spoff.p);
+  d.u = static_cast<double*>(b->sq_u.p);
+  d.stride = cap_atoms;
+  d.items = static_cast<const int4*>(b->sq_items.p);
+  d.item_plane = static_cast<const int32_t*>(b->sq_plane.p);
+  d.cap_items = cap_items;
+  d.tiles = static_cast<const int32_t*>(b->sq_tiles.p);
+  d.ntiles = ntiles;
+  d.tx = tk.tx;
+  d.ty = tk.ty;
+  d.tz = tk.tz;
+  for (int k = 0; k < 3; ++k) d.kmax[k] = kmax[k];
+  d.rho = static_cast<double*>(b->sq_rho.p);
+  d.sqf = static_cast<double*>(b->sq_sqf.p);
+  d.sqfp = static_cast<double*>(b->sq_sqfp.p);
+  d.nq = static_cast<unsigned long long*>(b->sq_nq.p);
+  d.flags = static_cast<uint32_t*>(b->flags.p);
+  d.nblocks = p.workgroups > 0 ? p.workgroups : 1024;
+  d.shell_parts = 16;
+  d.shell_lds = shell_lds;
+  const bool graph = p.use_graph != 0;
+  const bool same = b->sq_gexec && std::memcmp(&b->sq_gdev, &d, sizeof(d)) == 0 && b->sq_gprec == q.precision;
+  if (graph && !same) {
+    if (b->sq_gexec) {
+      (void)hipGraphExecDestroy(b->sq_gexec);
+      b->sq_gexec = nullptr;
+    }
+    hipGraph_t gr = nullptr;
+    BHIP(b, hipStreamSynchronize(b->stream));
+    BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    launch_batch_sq_group(d, q.precision, b->stream);
+    BHIP(b, hipStreamEndCapture(b->stream, &gr));
+    hipError_t e = hipGraphInstantiate(&b->sq_gexec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) return bfail(b, TRJ_EDEVICE, std::string("batch sq graph: ") + hipGetErrorString(e));
+    b->sq_gdev = d;
+    b->sq_gprec = q.precision;
+  }
+  const size_t pin_bytes = sizeof(BatchSqHeader) + sizeof(BatchSqFrame) * cap_frames +
+                           (sizeof(int4) + sizeof(int32_t)) * cap_items;
+  std::vector<int> order;
+  std::vector<int64_t> aoff(nframes + 1, 0);
+  for (int f = 0; f < nframes; ++f) aoff[f + 1] = aoff[f] + fr[f].n;
+  BHIP(b, hipEventRecord(b->ev0, b->stream));
+  int k = 0;
+  for (int g = 0; g < ngroups; ++g, k ^= 1) {
+    const int g0 = gstart[g], g1 = gstart[g + 1], nf = g1 - g0;
+    BHIP(b, hipEventSynchronize(b->sq_pin_done[k]));
+    if (b->sq_pin_cap[k] < pin_bytes) {
+      if (b->sq_pin[k]) (void)hipHostFree(b->sq_pin[k]);
+      b->sq_pin[k] = nullptr;
+      b->sq_pin_cap[k] = 0;
+      if (hipHostMalloc(&b->sq_pin[k], pin_bytes, hipHostMallocDefault) != hipSuccess)
+        return bfail(b, TRJ_ENOMEM, "batch sq: pinned allocation failed");
+      b->sq_pin_cap[k] = pin_bytes;
+    }
+    char* pin = static_cast<char*>(b->sq_pin[k]);
+    BatchSqHeader* H = reinterpret_cast<BatchSqHeader*>(pin);
+    BatchSqFrame* D = reinterpret_cast<BatchSqFrame*>(pin + sizeof(BatchSqHeader));
+    int4* items = reinterpret_cast<int4*>(pin + sizeof(BatchSqHeader) + sizeof(BatchSqFrame) * cap_frames);
+    int32_t* iplane = reinterpret_cast<int32_t*>(items + cap_items);
+    int blk0 = 0, moff = 0, chan0 = 0, plane0 = 0;
+    for (int f = g0; f < g1; ++f) {
+      BatchSqFrame& x = D[f - g0];
+      std::memset(&x, 0, sizeof(x));
+      x.atom_off = aoff[f];
+      x.out_off = plan[f].out_off;
+      x.outp_off = plan[f].outp_off;
+      x.n = static_cast<int32_t>(fr[f].n);
+      x.nsp = fr[f].nsp;
+      x.blk0 = blk0;
+      x.nblk = plan[f].nblk;
+      x.moff = moff;
+      x.chan0 = chan0;
+      x.plane0 = plane0;
+      x.ks = plan[f].ks;
+      x.nqmax = plan[f].nqmax;
+      for (int a = 0; a < 3; ++a) {
+        x.box[a] = a < p.ndim ? fr[f].box[a] : 1.0;
+        x.fk[a] = plan[f].fk[a];
+      }
+      x.dq = plan[f].dq;
+      for (int s = 0; s < kNspMax; ++s) x.bsc[s] = s < fr[f].nsp ? (bsc ? bsc[int64_t(f) * kNspMax + s] : 1.0) : 0.0;
+      blk0 += x.nblk;
+      moff += x.nsp * x.nblk;
+      chan0 += x.nsp;
+      plane0 += x.ks * x.nsp;
+    }
+    // Work items, the largest frames' first (the grid is dispatched in order).
+    order.resize(nf);
+    for (int j = 0; j < nf; ++j) order[j] = g0 + j;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return fr[a].n > fr[c].n; });
+    int ni = 0;
+    for (int f : order) {
+      const BatchSqFrame& x = D[f - g0];
+      for (int s = 0; s < x.ks; ++s) {
+        items[ni] = make_int4(x.chan0, x.chan0 + x.nsp, s, x.ks);
+        iplane[ni] = x.plane0 + s * x.nsp;
+        ++ni;
+      }
+    }
+    H->nframes = nf;
+    H->nblocks = blk0;
+    H->ncounts = moff;
+    H->nch = chan0;
+    H->nitems = ni;
+    H->pad = 0;
+    BHIP(b, hipMemcpyAsync(b->sq_hdr.p, H, sizeof(BatchSqHeader), hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMemcpyAsync(b->sq_fr.p, D, sizeof(BatchSqFrame) * nf, hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMemcpyAsync(b->sq_items.p, items, sizeof(int4) * ni, hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMemcpyAsync(b->sq_plane.p, iplane, sizeof(int32_t) * ni, hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipEventRecord(b->sq_pin_done[k], b->stream));
+    if (graph)
+      BHIP(b, hipGraphLaunch(b->sq_gexec, b->stream));
+    else
+      launch_batch_sq_group(d, q.precision, b->stream);
+    BHIP(b, hipGetLastError());
+  }
+  BHIP(b, hipEventRecord(b->ev1, b->stream));
+  uint32_t flags = 0;
+  BHIP(b, hipMemcpyAsync(&flags, b->flags.p, sizeof(flags), hipMemcpyDeviceToHost, b->stream));
+  if (sqf_host)
+    BHIP(b, hipMemcpyAsync(sqf_host, b->sq_sqf.p, sizeof(double) * tot_sqf, hipMemcpyDeviceToHost, b->stream));
+  if (sqfp_host)
+    BHIP(b, hipMemcpyAsync(sqfp_host, b->sq_sqfp.p, sizeof(double) * tot_sqfp, hipMemcpyDeviceToHost, b->stream));
+  if (nq_host)
+    BHIP(b, hipMemcpyAsync(nq_host, b->sq_nq.p, sizeof(int64_t) * tot_sqf, hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  if (gpu_ms) {
+    float ms = 0.f;
+    BHIP(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    *gpu_ms = ms;
+  }
+  if (flags & kFlagType) return bfail(b, TRJ_EINVAL, "batch sq: species types outside 1..nsp");
+  if (flags & kFlagRange) return bfail(b, TRJ_ERANGE, "batch sq: non-finite coordinates");
+  return TRJ_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -328,6 +664,7 @@ int trj_batch_create(trj_batch** out, int device, const trj_batch_params* p) {
   bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
   ok = ok && hipEventCreate(&b->ev0) == hipSuccess && hipEventCreate(&b->ev1) == hipSuccess;
   for (auto& e : b->pin_done) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+  for (auto& e : b->sq_pin_done) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     trj_batch_destroy(b);
     return TRJ_EDEVICE;
@@ -341,13 +678,18 @@ void trj_batch_destroy(trj_batch* b) {
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-  Buf* bufs[] = {&b->hdr, &b->fr, &b->pos, &b->spos, &b->stype, &b->cellid, &b->count,
-                 &b->start, &b->cursor, &b->blocksum, &b->thr, &b->hist, &b->flags};
+  if (b->sq_gexec) (void)hipGraphExecDestroy(b->sq_gexec);
+  Buf* bufs[] = {&b->hdr,    &b->fr,     &b->pos,       &b->spos,      &b->stype,    &b->cellid,   &b->count,
+                 &b->start,  &b->cursor, &b->blocksum,  &b->thr,       &b->hist,     &b->flags,    &b->sq_hdr,
+                 &b->sq_fr,  &b->sq_items, &b->sq_plane, &b->sq_blk,   &b->sq_blkoff, &b->sq_spoff, &b->sq_u,
+                 &b->sq_rho, &b->sq_tiles, &b->sq_sqf,  &b->sq_sqfp,   &b->sq_nq};
   for (Buf* x : bufs)
     if (x->p) (void)hipFree(x->p);
   for (int k = 0; k < 2; ++k) {
     if (b->pin[k]) (void)hipHostFree(b->pin[k]);
     if (b->pin_done[k]) (void)hipEventDestroy(b->pin_done[k]);
+    if (b->sq_pin[k]) (void)hipHostFree(b->sq_pin[k]);
+    if (b->sq_pin_done[k]) (void)hipEventDestroy(b->sq_pin_done[k]);
   }
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -415,6 +757,34 @@ int trj_batch_autotune(trj_batch* b, int32_t nframes, const trj_batch_frame* fr,
   b->p = bp;
   if (chosen) *chosen = bp;
   return TRJ_OK;
+}
+
+int trj_batch_sq_layout(const trj_batch_params* p, const trj_batch_sq_params* q, int32_t nframes,
+                        const trj_batch_frame* fr, int32_t* nqmax, int64_t* sqf_off, int64_t* sqfp_off,
+                        int64_t* total_sqf, int64_t* total_sqfp) {
+  if (!p || !q || !fr || nframes < 0) return TRJ_EINVAL;
+  std::vector<SqFramePlan> plan;
+  int64_t a = 0, c = 0;
+  std::string err;
+  int r = plan_sq_frames(*p, *q, nframes, fr, plan, &a, &c, &err);
+  if (r) return r;
+  for (int f = 0; f < nframes; ++f) {
+    if (nqmax) nqmax[f] = plan[f].nqmax;
+    if (sqf_off) sqf_off[f] = plan[f].out_off;
+    if (sqfp_off) sqfp_off[f] = plan[f].outp_off;
+  }
+  if (total_sqf) *total_sqf = a;
+  if (total_sqfp) *total_sqfp = c;
+  return TRJ_OK;
+}
+
+int trj_batch_sq_run(trj_batch* b, const trj_batch_sq_params* q, int32_t nframes, const trj_batch_frame* fr,
+                     const void* pos_dev, int32_t dtype, const int32_t* itype_dev, const double* bsc, double* sqf,
+                     double* sqfp, int64_t* nq, double* gpu_ms) {
+  if (!b) return TRJ_EINVAL;
+  if (!q || nframes < 1 || !fr || !pos_dev || !itype_dev) return bfail(b, TRJ_EINVAL, "batch sq: empty or NULL input");
+  (void)hipSetDevice(b->device);
+  return run_sq_groups(b, *q, nframes, fr, pos_dev, dtype, itype_dev, bsc, sqf, sqfp, nq, gpu_ms);
 }
 
 }  // extern "C"

@@ -46,6 +46,8 @@ void launch_ingest(const IngestArgs& a, hipStream_t s);
 void launch_species_count(const int32_t* itype, const int32_t* key, int64_t n, int nkeys, int32_t* blk, int nblk,
                           hipStream_t s);
 void launch_exclusive_scan(const int32_t* in, int32_t* out, int64_t count, hipStream_t s);
+// The same with the count read from device memory when the kernel starts.
+void launch_exclusive_scan_dev(const int32_t* in, int32_t* out, const int32_t* count, hipStream_t s);
 void launch_species_offsets(const int32_t* blkoff, const int32_t* blk, int nblk, int nkeys, int gsize, int ngroups,
                             int32_t* sp_off, hipStream_t s);
 void launch_species_scatter(const int32_t* itype, const int32_t* key, int64_t n, int nkeys, const int32_t* blkoff,
@@ -231,6 +233,15 @@ struct ContractArgs {
   int nch;                // channels: nsp species, + 1 charge channel
   int64_t n;
   double* rho;            // [split][channel][tile][tile vectors][2]
+  // Work items (batched frames; NULL for one frame): ksplit is then the
+  // number of items (the grid is ntiles x ksplit), item k = items[k] =
+  // {c0, c1, split, splits} takes split `split` of `splits` of each channel
+  // c0 .. c1-1 and writes channel c at rho plane item_plane[k] + c - c0
+  // ([plane][tile][tile vectors][2]).  Items k >= *nitems exit at once (a
+  // fixed grid replayed for groups of fewer items).
+  const int4* items;
+  const int32_t* item_plane;
+  const int32_t* nitems;
 };
 void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s);
 // FP32 MFMA, or FP16 / BF16 hi+lo split with three products (precision =
@@ -308,5 +319,57 @@ struct BatchDev {
 // pair_lds: the largest pair_lds_bytes of the group's frames (the pair kernel
 // adds its hit rings, cells_lds_bytes - pair_lds_bytes).
 void launch_batch_group(const BatchDev& d, int nblocks, size_t pair_lds, hipStream_t s);
+
+// S(Q) of batched frames.  A group's frames are sorted by species into one
+// set of turn planes (frame-major, then species, then input order); frame f's
+// species are channels chan0 .. chan0 + nsp - 1 of one contraction launch
+// over the box-independent half-space tiles, and its shells come from its own
+// box.  Count blocks of 256 atoms never straddle two frames.
+struct BatchSqFrame {
+  int64_t atom_off;    // first atom in the caller's concatenated input
+  int64_t out_off;     // sqf and nq of this frame at out_off (nqmax + 1 values)
+  int64_t outp_off;    // sqfp at outp_off ([nsp][nqmax + 1])
+  int32_t n, nsp;
+  int32_t blk0, nblk;  // count blocks [blk0, blk0 + nblk) of the group
+  int32_t moff;        // its [nsp][nblk] block counts in the group's flat count array
+  int32_t chan0;       // first channel
+  int32_t plane0, ks;  // channel s, split k at rho plane plane0 + k * nsp + s
+  int32_t nqmax, pad;
+  double box[3];
+  double fk[3], dq;    // 2 pi / L per axis (FP64) and their minimum
+  double bsc[kNspMax];
+};
+struct BatchSqHeader {
+  int32_t nframes, nblocks;  // frames and count blocks of the group
+  int32_t ncounts, nch;      // flat block counts, channels
+  int32_t nitems, pad;       // contraction work items
+};
+struct BatchSqDev {
+  const BatchSqHeader* hdr;
+  const BatchSqFrame* fr;
+  const void* pos_in;  // caller's positions (AoS, stride ndim), dtype below
+  const int32_t* itype_in;
+  int dtype, ndim;
+  int32_t* blk;        // flat block counts
+  int32_t* blkoff;     // their exclusive scan
+  int32_t* sp_off;     // channel offsets, nch + 1
+  double* u;           // turn planes, 12 x stride
+  int64_t stride;
+  const int4* items;   // contraction work items and their first rho planes
+  const int32_t* item_plane;
+  int cap_items;       // grid of the contraction: ntiles x cap_items
+  const int32_t* tiles;
+  int ntiles, tx, ty, tz;
+  int kmax[3];
+  double* rho;         // [plane][tile][tile vectors][2]
+  double* sqf;         // all frames' outputs (BatchSqFrame offsets)
+  double* sqfp;
+  unsigned long long* nq;
+  uint32_t* flags;
+  int nblocks;         // fixed grid of the count, scatter and shell kernels
+  int shell_parts;     // vector ranges per frame in the shell kernel
+  int shell_lds;       // doubles of LDS accumulators per workgroup (0: global atomics)
+};
+void launch_batch_sq_group(const BatchSqDev& d, int precision, hipStream_t s);
 
 }  // namespace trj

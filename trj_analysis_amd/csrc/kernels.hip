@@ -104,8 +104,10 @@ __global__ __launch_bounds__(256) void k_species_count(const int32_t* __restrict
 // registers), then the 64 lanes of each wavefront by shuffles and the 16
 // wavefront totals through LDS.
 __global__ __launch_bounds__(1024) void k_exclusive_scan(const int32_t* __restrict__ in,
-                                                         int32_t* __restrict__ out, int64_t count) {
+                                                         int32_t* __restrict__ out, int64_t count,
+                                                         const int32_t* __restrict__ count_dev) {
   __shared__ int32_t wsum[16];
+  if (count_dev) count = *count_dev;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
   int32_t carry = 0;
@@ -1499,6 +1501,35 @@ __device__ inline void contract_chunk_f64(const double2 (*ph)[kRec], int kq, int
   }
 }
 
+// Channels, split and first rho plane of work item `item` (blockIdx.x /
+// ntiles): every channel's split `item` of ksplit for one frame, or the
+// batched frames' item table.
+struct ItemRange {
+  int c0, c1, split, splits;
+  size_t plane0;
+  bool skip;
+};
+__device__ inline ItemRange item_range(const ContractArgs& a, int item) {
+  ItemRange r;
+  if (!a.items) {
+    r.c0 = 0;
+    r.c1 = a.nch;
+    r.split = item;
+    r.splits = a.ksplit;
+    r.plane0 = static_cast<size_t>(item) * a.nch;
+    r.skip = false;
+    return r;
+  }
+  r.skip = item >= *a.nitems;
+  const int4 it = a.items[r.skip ? 0 : item];
+  r.c0 = it.x;
+  r.c1 = it.y;
+  r.split = it.z;
+  r.splits = it.w;
+  r.plane0 = static_cast<size_t>(a.item_plane[r.skip ? 0 : item]);
+  return r;
+}
+
 // rho_s(q) for one tile and one atom split: a complex GEMM over atoms whose
 // operands are generated per 32-atom chunk in LDS (double-buffered: the next
 // chunk is staged while this one is contracted).  Waves 0-2 stage one
@@ -1508,7 +1539,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     ContractArgs a) {
   __shared__ double2 ph[2][KC][kRec];
   const int tile = blockIdx.x % a.ntiles;
-  const int split = blockIdx.x / a.ntiles;
+  const ItemRange wi = item_range(a, blockIdx.x / a.ntiles);
+  if (wi.skip) return;
   const int kx0 = a.tiles[4 * tile], ky0 = a.tiles[4 * tile + 1], kz0 = a.tiles[4 * tile + 2];
   const int t = threadIdx.x, w = t / kWave, lane = t & (kWave - 1);
   const int i16 = lane & 15, kq = lane >> 4;
@@ -1530,11 +1562,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const uint32_t st_base = static_cast<uint32_t>(st_ax) * static_cast<uint32_t>(a.n);
   // The x segment carries the channel weight (1, or the charge of the entry).
   const bool st_w = a.w != nullptr && st_seg == 0;
-  for (int s = 0; s < a.nch; ++s) {
+  for (int s = wi.c0; s < wi.c1; ++s) {
     const int a0 = a.sp_off[s], a1 = a.sp_off[s + 1];
     const int64_t len = a1 - a0;
-    const int b0 = a0 + static_cast<int>(len * split / a.ksplit);
-    const int b1 = a0 + static_cast<int>(len * (split + 1) / a.ksplit);
+    const int b0 = a0 + static_cast<int>(len * wi.split / wi.splits);
+    const int b1 = a0 + static_cast<int>(len * (wi.split + 1) / wi.splits);
     d4 t1[2], t2[2], t3[2];
     double sumR[2][4], sumI[2][4];
     float cmpR[2][4], cmpI[2][4];
@@ -1611,7 +1643,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     // Vector index within the tile: row mrow = w*16 + 4r + kq (= kx*8 + ky),
     // column nb*16 + i16 (kz), i.e. mrow * TKZ + nb * 16 + i16.
     double2* out = reinterpret_cast<double2*>(a.rho) +
-                   ((static_cast<size_t>(split) * a.nch + s) * a.ntiles + tile) * TILE_VEC +
+                   ((wi.plane0 + (s - wi.c0)) * a.ntiles + tile) * TILE_VEC +
                    ((w * 16 + kq) * TKZ + i16);
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) {
@@ -1683,7 +1715,8 @@ struct LpShared32 {
 __global__ __launch_bounds__(256) void k_sq_contract_f32(ContractArgs a) {
   __shared__ LpShared32 sh;
   const int tile = blockIdx.x % a.ntiles;
-  const int split = blockIdx.x / a.ntiles;
+  const ItemRange wi = item_range(a, blockIdx.x / a.ntiles);
+  if (wi.skip) return;
   const int kx0 = a.tiles[4 * tile], ky0 = a.tiles[4 * tile + 1], kz0 = a.tiles[4 * tile + 2];
   const int t = threadIdx.x, w = t / kWave, lane = t & (kWave - 1);
   const int i16 = lane & 15, kq = lane >> 4;
@@ -1693,11 +1726,11 @@ __global__ __launch_bounds__(256) void k_sq_contract_f32(ContractArgs a) {
   const double* __restrict__ lx = a.u + 3 * a.n;
   const double* __restrict__ ly = a.u + 4 * a.n;
   const double* __restrict__ lz = a.u + 5 * a.n;
-  for (int s = 0; s < a.nch; ++s) {
+  for (int s = wi.c0; s < wi.c1; ++s) {
     const int a0 = a.sp_off[s], a1 = a.sp_off[s + 1];
     const int64_t len = a1 - a0;
-    const int b0 = a0 + static_cast<int>(len * split / a.ksplit);
-    const int b1 = a0 + static_cast<int>(len * (split + 1) / a.ksplit);
+    const int b0 = a0 + static_cast<int>(len * wi.split / wi.splits);
+    const int b1 = a0 + static_cast<int>(len * (wi.split + 1) / wi.splits);
     // FP32 accumulation over this workgroup's atoms (at most 8192 per species,
     // set by the host through ksplit); the K-splits are summed in FP64.
     f4 accR[8], accI[8];
@@ -1786,7 +1819,7 @@ __global__ __launch_bounds__(256) void k_sq_contract_f32(ContractArgs a) {
         }
       }
     }
-    const size_t base = ((static_cast<size_t>(split) * a.nch + s) * a.ntiles + tile) * (LX * LN);
+    const size_t base = ((wi.plane0 + (s - wi.c0)) * a.ntiles + tile) * (LX * LN);
 #pragma unroll
     for (int nb = 0; nb < 8; ++nb) {
 #pragma unroll
@@ -1924,7 +1957,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
   __shared__ __attribute__((aligned(16))) float2 rec[3][WROWS][kRS];  // x 0..63, z 64..95, y 96..103
   __shared__ __attribute__((aligned(16))) V As[2][4][LKC / 8][WX];     // re_hi, re_lo, im_hi, im_lo of ex
   const int tile = blockIdx.x % a.ntiles;
-  const int split = blockIdx.x / a.ntiles;
+  const ItemRange wi = item_range(a, blockIdx.x / a.ntiles);
+  if (wi.skip) return;
   const int kx0 = a.tiles[4 * tile], ky0 = a.tiles[4 * tile + 1], kz0 = a.tiles[4 * tile + 2];
   const int t = threadIdx.x, w = t / kWave, lane = t & (kWave - 1);
   const int i16 = lane & 15, kq = lane >> 4;
@@ -1942,13 +1976,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
   const double* wsrc = a.w ? a.w : u_hi;
   // A split: row ar, atoms 4 aq .. 4 aq + 3 (half of one 8-atom entry).
   const int ar = t & 63, aq = t >> 6;
-  for (int s = 0; s < a.nch; ++s) {
+  for (int s = wi.c0; s < wi.c1; ++s) {
     const int a0 = a.sp_off[s], a1 = a.sp_off[s + 1];
     const int64_t len = a1 - a0;
-    const int b0 = a0 + static_cast<int>(len * split / a.ksplit);
-    const int b1 = a0 + static_cast<int>(len * (split + 1) / a.ksplit);
+    const int b0 = a0 + static_cast<int>(len * wi.split / wi.splits);
+    const int b1 = a0 + static_cast<int>(len * (wi.split + 1) / wi.splits);
     if (b0 >= b1) {
-      const size_t base = ((static_cast<size_t>(split) * a.nch + s) * a.ntiles + tile) * (WX * WN);
+      const size_t base = ((wi.plane0 + (s - wi.c0)) * a.ntiles + tile) * (WX * WN);
       for (int v = t; v < WX * WN; v += 512) reinterpret_cast<double2*>(a.rho)[base + v] = make_double2(0.0, 0.0);
       continue;
     }
@@ -2169,7 +2203,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
       }
     }
     // D layout: register r holds row 4*(lane/16) + r, column lane%16.
-    const size_t base = ((static_cast<size_t>(split) * a.nch + s) * a.ntiles + tile) * (WX * WN);
+    const size_t base = ((wi.plane0 + (s - wi.c0)) * a.ntiles + tile) * (WX * WN);
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
@@ -2794,7 +2828,12 @@ void launch_species_count(const int32_t* itype, const int32_t* key, int64_t n, i
 }
 
 void launch_exclusive_scan(const int32_t* in, int32_t* out, int64_t count, hipStream_t s) {
-  hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, s, in, out, count);
+  hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, s, in, out, count,
+                     static_cast<const int32_t*>(nullptr));
+}
+
+void launch_exclusive_scan_dev(const int32_t* in, int32_t* out, const int32_t* count, hipStream_t s) {
+  hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, s, in, out, int64_t(0), count);
 }
 
 void launch_species_offsets(const int32_t* blkoff, const int32_t* blk, int nblk, int nkeys, int gsize, int ngroups,
@@ -3175,6 +3214,223 @@ void launch_batch_group(const BatchDev& d, int nblocks, size_t pair_lds, hipStre
   hipLaunchKernelGGL(bk_scan_apply, dim3(nblocks), dim3(256), 0, s, d);
   hipLaunchKernelGGL(bk_scatter, dim3(nblocks), dim3(256), 0, s, d);
   hipLaunchKernelGGL(bk_pairs, dim3(nblocks), dim3(256), kRingBytes + pair_lds, s, d);
+}
+
+// ------------------------------------------------------- batched S(Q)
+// The frame that holds count block b (frames in block order).
+__device__ inline int bsq_frame_of(const BatchSqFrame* __restrict__ fr, int nframes, int b) {
+  int lo = 0, hi = nframes - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (fr[mid].blk0 <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Species of the count block's atoms: per-wavefront ballot counts in cnt,
+// the lane's own species (-1: past the frame or outside 1..nsp) returned and
+// its ballot mask in *mine.
+__device__ inline int bsq_block_species(const BatchSqDev& d, const BatchSqFrame& F, int lb,
+                                        int (*cnt)[kNspMax], unsigned long long* mine) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int li = lb * 256 + static_cast<int>(threadIdx.x);
+  int s = -1;
+  if (li < F.n) {
+    const int t = d.itype_in[F.atom_off + li];
+    s = (t >= 1 && t <= F.nsp) ? t - 1 : -1;
+  }
+  *mine = 0;
+  for (int sp = 0; sp < F.nsp; ++sp) {
+    const unsigned long long m = __ballot(s == sp);
+    if (lane == 0) cnt[w][sp] = __popcll(m);
+    if (s == sp) *mine = m;
+  }
+  return s;
+}
+
+// Block counts per species: blk[moff + s * nblk + lb].
+__global__ __launch_bounds__(256) void bsq_count(BatchSqDev d) {
+  __shared__ int cnt[4][kNspMax];
+  const int nframes = d.hdr->nframes, nblocks = d.hdr->nblocks;
+  for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const BatchSqFrame& F = d.fr[bsq_frame_of(d.fr, nframes, b)];
+    const int lb = b - F.blk0;
+    unsigned long long mine;
+    (void)bsq_block_species(d, F, lb, cnt, &mine);
+    __syncthreads();
+    for (int sp = threadIdx.x; sp < F.nsp; sp += 256)
+      d.blk[F.moff + sp * F.nblk + lb] = cnt[0][sp] + cnt[1][sp] + cnt[2][sp] + cnt[3][sp];
+    __syncthreads();
+  }
+}
+
+// Channel offsets: channel chan0 + s of frame f starts at the scanned count
+// of its first block; sp_off[nch] is the number of entries.
+__global__ __launch_bounds__(256) void bsq_offsets(BatchSqDev d) {
+  const int nframes = d.hdr->nframes;
+  for (int f = threadIdx.x; f < nframes; f += 256) {
+    const BatchSqFrame& F = d.fr[f];
+    for (int s = 0; s < F.nsp; ++s) d.sp_off[F.chan0 + s] = d.blkoff[F.moff + s * F.nblk];
+  }
+  if (threadIdx.x == 0) {
+    const int last = d.hdr->ncounts - 1;
+    d.sp_off[d.hdr->nch] = d.blkoff[last] + d.blk[last];
+  }
+}
+
+// Each atom's turns into its sorted slot: positions converted to FP64 and
+// wrapped into [0, L) on every axis of its frame (as k_ingest wraps the
+// periodic axes of an S(Q) frame), then hi, lo, cos, sin per axis as
+// k_species_scatter writes them with L the frame's box.
+__global__ __launch_bounds__(256) void bsq_scatter(BatchSqDev d) {
+  __shared__ int cnt[4][kNspMax];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int nframes = d.hdr->nframes, nblocks = d.hdr->nblocks;
+  for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const BatchSqFrame& F = d.fr[bsq_frame_of(d.fr, nframes, b)];
+    const int lb = b - F.blk0;
+    const int li = lb * 256 + static_cast<int>(threadIdx.x);
+    unsigned long long mine;
+    const int s = bsq_block_species(d, F, lb, cnt, &mine);
+    __syncthreads();
+    if (s >= 0) {
+      int off = d.blkoff[F.moff + s * F.nblk + lb];
+      for (int ww = 0; ww < w; ++ww) off += cnt[ww][s];
+      off += __popcll(mine & ((1ull << lane) - 1ull));
+      const int64_t ai = F.atom_off + li;
+      uint32_t bad = 0;
+      for (int k = 0; k < 3; ++k) {
+        double x = 0.0, L = 1.0;
+        if (k < d.ndim) {
+          const int64_t idx = ai * d.ndim + k;
+          x = d.dtype == TRJ_F32 ? static_cast<double>(static_cast<const float*>(d.pos_in)[idx])
+                                 : static_cast<const double*>(d.pos_in)[idx];
+          L = F.box[k];
+          if (!isfinite(x)) bad |= kFlagRange;
+          else x = wrap_periodic(x, L);
+        }
+        const double hi = x / L;
+        const double lo = fma(-hi, L, x) / L;
+        double sn, cs;
+        sincospi(2.0 * frac_turns(1, hi, lo), &sn, &cs);
+        d.u[k * d.stride + off] = hi;
+        d.u[(3 + k) * d.stride + off] = lo;
+        d.u[(6 + k) * d.stride + off] = cs;
+        d.u[(9 + k) * d.stride + off] = sn;
+      }
+      if (bad) atomicOr(d.flags, bad);
+    } else if (li < F.n) {
+      atomicOr(d.flags, kFlagType);
+    }
+    __syncthreads();
+  }
+}
+
+// Shells of every frame: workgroup unit (f, part) takes a range of the tile
+// vectors, sums the frame's splits per channel, and adds |sum_s b_s rho_s|^2,
+// |rho_s|^2 and a count of 1 to the vector's shell nint(|q| / dq) of the
+// frame's box, the FP64 rule of the half-space plan (shells 1 .. nqmax, the
+// origin left out).  Sums gather in LDS per unit, then go to the outputs
+// with FP64 / integer atomics.
+__global__ __launch_bounds__(256) void bsq_shells(BatchSqDev d) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* acc = reinterpret_cast<double*>(smem);
+  const int nframes = d.hdr->nframes;
+  const int tvec = d.tx * d.ty * d.tz;
+  const int64_t total = static_cast<int64_t>(d.ntiles) * tvec;
+  const int64_t chunk = (total + d.shell_parts - 1) / d.shell_parts;
+  const double2* r2 = reinterpret_cast<const double2*>(d.rho);
+  for (int unit = blockIdx.x; unit < nframes * d.shell_parts; unit += gridDim.x) {
+    const BatchSqFrame& F = d.fr[unit / d.shell_parts];
+    const int part = unit % d.shell_parts;
+    const int ns = F.nqmax + 1, nsp = F.nsp;
+    const int nacc = (2 + nsp) * ns;  // sqf, sqfp [nsp], counts
+    const bool lds = nacc <= d.shell_lds;
+    if (lds) {
+      for (int k = threadIdx.x; k < nacc; k += 256) acc[k] = 0.0;
+      __syncthreads();
+    }
+    auto add = [&](int row, int sh, double v) {
+      if (lds) {
+        atomicAdd(&acc[row * ns + sh], v);
+      } else if (row == 0) {
+        atomicAdd(&d.sqf[F.out_off + sh], v);
+      } else if (row <= nsp) {
+        atomicAdd(&d.sqfp[F.outp_off + static_cast<int64_t>(row - 1) * ns + sh], v);
+      } else {
+        atomicAdd(&d.nq[F.out_off + sh], static_cast<unsigned long long>(v));
+      }
+    };
+    const int64_t e1 = (part + 1) * chunk < total ? (part + 1) * chunk : total;
+    for (int64_t e = part * chunk + threadIdx.x; e < e1; e += 256) {
+      const int tile = static_cast<int>(e / tvec), v = static_cast<int>(e % tvec);
+      const int a = d.tiles[4 * tile] + v / (d.ty * d.tz);
+      const int bq = d.tiles[4 * tile + 1] + (v / d.tz) % d.ty;
+      const int c = d.tiles[4 * tile + 2] + v % d.tz;
+      if (a > d.kmax[0] || bq > d.kmax[1] || c > d.kmax[2] || (a == 0 && bq == 0 && c == 0)) continue;
+      const double qa = a * F.fk[0], qb = bq * F.fk[1], qc = c * F.fk[2];
+      const double sh = round(sqrt(qa * qa + qb * qb + qc * qc) / F.dq);
+      if (!(sh >= 1.0 && sh <= static_cast<double>(F.nqmax))) continue;
+      const int ish = static_cast<int>(sh);
+      double tr = 0.0, ti = 0.0;
+      for (int s = 0; s < nsp; ++s) {
+        double rr = 0.0, ri = 0.0;
+        for (int k = 0; k < F.ks; ++k) {
+          const double2 x = r2[static_cast<int64_t>(F.plane0 + k * nsp + s) * total + e];
+          rr += x.x;
+          ri += x.y;
+        }
+        add(1 + s, ish, rr * rr + ri * ri);
+        tr += F.bsc[s] * rr;
+        ti += F.bsc[s] * ri;
+      }
+      add(0, ish, tr * tr + ti * ti);
+      add(1 + nsp, ish, 1.0);
+    }
+    if (lds) {
+      __syncthreads();
+      for (int k = threadIdx.x; k < nacc; k += 256) {
+        const double x = acc[k];
+        if (x == 0.0) continue;
+        const int row = k / ns, sh = k % ns;
+        if (row == 0)
+          atomicAdd(&d.sqf[F.out_off + sh], x);
+        else if (row <= nsp)
+          atomicAdd(&d.sqfp[F.outp_off + static_cast<int64_t>(row - 1) * ns + sh], x);
+        else
+          atomicAdd(&d.nq[F.out_off + sh], static_cast<unsigned long long>(x));
+      }
+      __syncthreads();
+    }
+  }
+}
+
+void launch_batch_sq_group(const BatchSqDev& d, int precision, hipStream_t s) {
+  hipLaunchKernelGGL(bsq_count, dim3(d.nblocks), dim3(256), 0, s, d);
+  launch_exclusive_scan_dev(d.blk, d.blkoff, &d.hdr->ncounts, s);
+  hipLaunchKernelGGL(bsq_offsets, dim3(1), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(bsq_scatter, dim3(d.nblocks), dim3(256), 0, s, d);
+  ContractArgs ca{};
+  ca.u = d.u;
+  ca.w = nullptr;
+  ca.sp_off = d.sp_off;
+  ca.tiles = d.tiles;
+  ca.ntiles = d.ntiles;
+  ca.ksplit = d.cap_items;
+  ca.nsp = kNspMax;
+  ca.ndim = d.ndim;
+  ca.nch = 0;
+  ca.n = d.stride;
+  ca.rho = d.rho;
+  ca.items = d.items;
+  ca.item_plane = d.item_plane;
+  ca.nitems = &d.hdr->nitems;
+  if (precision == TRJ_PREC_FP64)
+    launch_sq_contract_f64(ca, s);
+  else
+    launch_sq_contract_lp(ca, precision, s);
+  hipLaunchKernelGGL(bsq_shells, dim3(d.nblocks), dim3(256), static_cast<size_t>(d.shell_lds) * sizeof(double), s,
+                     d);
 }
 
 }  // namespace trj

@@ -43,7 +43,8 @@ EXPORTS = [
     "trj_lds_atomic_rate", "trj_mfma_rate", "trj_coissue_rate", "trj_nc_open", "trj_nc_close", "trj_nc_last_error", "trj_nc_info", "trj_nc_read",
     "trj_nc_time", "trj_nc_frame_ref", "trj_nc_frame_ref_q", "trj_nc_keep_nonperiodic",
     "trj_batch_create", "trj_batch_destroy", "trj_batch_last_error",
-    "trj_batch_layout", "trj_batch_run", "trj_batch_autotune", "trj_srv_create", "trj_srv_destroy",
+    "trj_batch_layout", "trj_batch_run", "trj_batch_autotune", "trj_batch_sq_layout", "trj_batch_sq_run",
+    "trj_srv_create", "trj_srv_destroy",
     "trj_srv_last_error", "trj_request_footprint", "trj_srv_submit", "trj_srv_poll", "trj_srv_result",
     "trj_srv_stats", "trj_srv_memory",
 ]
@@ -87,6 +88,10 @@ class BatchParams(C.Structure):
     _fields_ = [("ndim", C.c_int32), ("deltar", C.c_double), ("rcrdf", C.c_double),
                 ("frames_per_launch", C.c_int32), ("waves_per_cell", C.c_int32),
                 ("workgroups", C.c_int32), ("use_graph", C.c_int32)]
+
+
+class BatchSqParams(C.Structure):
+    _fields_ = [("grid", C.c_int32 * 3), ("precision", C.c_int32), ("qmax", C.c_double)]
 
 
 class Request(C.Structure):
@@ -186,6 +191,10 @@ def _declare(L):
     L.trj_batch_layout.argtypes = [C.POINTER(BatchParams), C.c_int32, C.POINTER(BatchFrame), i32, i32,
                                    i64, i64]
     L.trj_batch_run.argtypes = [vp, C.c_int32, C.POINTER(BatchFrame), vp, C.c_int32, vp, i64, d]
+    L.trj_batch_sq_layout.argtypes = [C.POINTER(BatchParams), C.POINTER(BatchSqParams), C.c_int32,
+                                      C.POINTER(BatchFrame), i32, i64, i64, i64, i64]
+    L.trj_batch_sq_run.argtypes = [vp, C.POINTER(BatchSqParams), C.c_int32, C.POINTER(BatchFrame), vp, C.c_int32,
+                                   vp, d, d, d, i64, d]
     L.trj_batch_autotune.argtypes = [vp, C.c_int32, C.POINTER(BatchFrame), vp, C.c_int32, vp,
                                      C.POINTER(BatchParams)]
     L.trj_srv_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int32, C.c_int64]
@@ -769,8 +778,31 @@ def batch_layout(p: BatchParams, frames_arr):
     return ls, nit, off, tot.value
 
 
+def batch_sq_params(grid=(64, 64, 64), precision=TRJ_PREC_FP64, qmax=1.0e9) -> BatchSqParams:
+    q = BatchSqParams()
+    for a in range(3):
+        q.grid[a] = int(grid[a])
+    q.precision, q.qmax = int(precision), float(qmax)
+    return q
+
+
+def batch_sq_layout(p: BatchParams, q: BatchSqParams, frames_arr):
+    """nqmax, sqf/nq offsets, sqfp offsets and the two totals of trj_batch_sq_layout."""
+    nf = len(frames_arr)
+    nqmax = np.zeros(nf, dtype=np.int32)
+    off = np.zeros(nf, dtype=np.int64)
+    offp = np.zeros(nf, dtype=np.int64)
+    t1, t2 = C.c_int64(), C.c_int64()
+    r = lib().trj_batch_sq_layout(C.byref(p), C.byref(q), nf, frames_arr, _ptr(nqmax, C.c_int32),
+                                  _ptr(off, C.c_int64), _ptr(offp, C.c_int64), C.byref(t1), C.byref(t2))
+    if r != TRJ_OK:
+        raise TrjError(r, "trj_batch_sq_layout failed")
+    return nqmax, off, offp, t1.value, t2.value
+
+
 class Batch:
-    """Batched frames of uneven size (trj_batch_*): per-frame pair histograms."""
+    """Batched frames of uneven size (trj_batch_*): per-frame pair histograms
+    (run) and per-frame S(Q) shell sums (sq_run)."""
 
     def __init__(self, p: BatchParams, device: int = 0):
         self._L = lib()
@@ -808,6 +840,32 @@ class Batch:
         out = None
         if want_hist:
             out = [hist[off[k]:off[k] + nit[k] * ls[k]].reshape(nit[k], ls[k]) for k in range(len(off))]
+        return out, ms.value
+
+    def sq_run(self, q: BatchSqParams, frames_arr, pos_ptr: int, dtype: int, itype_ptr: int, bsc=None):
+        """trj_batch_sq_run: per frame (sqf, sqfp [nsp][nqmax+1], nq), and the device ms.
+        bsc: per-frame scattering lengths (list of sequences), None for all 1."""
+        nqmax, off, offp, t1, t2 = batch_sq_layout(self.p, q, frames_arr)
+        nf = len(frames_arr)
+        sqf = np.zeros(max(t1, 1))
+        sqfp = np.zeros(max(t2, 1))
+        nq = np.zeros(max(t1, 1), dtype=np.int64)
+        b = None
+        if bsc is not None:
+            b = np.zeros((nf, 10))
+            for k, row in enumerate(bsc):
+                b[k, :len(row)] = row
+        ms = C.c_double()
+        self._check(self._L.trj_batch_sq_run(self._h, C.byref(q), nf, frames_arr, C.c_void_p(pos_ptr), dtype,
+                                             C.c_void_p(itype_ptr), _ptr(b, C.c_double) if b is not None else None,
+                                             _ptr(sqf, C.c_double), _ptr(sqfp, C.c_double), _ptr(nq, C.c_int64),
+                                             C.byref(ms)), "trj_batch_sq_run")
+        out = []
+        for k in range(nf):
+            ns = int(nqmax[k]) + 1
+            nsp = frames_arr[k].nsp
+            out.append((sqf[off[k]:off[k] + ns].copy(), sqfp[offp[k]:offp[k] + nsp * ns].reshape(nsp, ns).copy(),
+                        nq[off[k]:off[k] + ns].copy()))
         return out, ms.value
 
     def autotune(self, frames_arr, pos_ptr: int, dtype: int, itype_ptr: int) -> BatchParams:
