@@ -168,6 +168,7 @@ def bench_batch(nf: int) -> dict:
     t_nograph = time.perf_counter() - t0
     b.close()
     nog.close()
+    sq = {p: bench_batch_sq(frames, pos, it, offs, p) for p in ("fp16x3", "fp64")}
     return {
         "frames": nf, "atoms": natoms, "rcrdf": wl.rcrdf, "deltar": wl.deltar,
         "inputs": "HBM-resident float32",
@@ -180,6 +181,67 @@ def bench_batch(nf: int) -> dict:
         "no_graph_frames_per_s": nf / t_nograph,
         "batch_gpu_ms": gpu_ms,
         "bit_exact_vs_plain": exact,
+        "sq": sq,
+    }
+
+
+def bench_batch_sq(frames, pos, it, offs, precision: str) -> dict:
+    """S(Q) of the uneven frames on the 64^3 half-space grid of each frame's own
+    box: the batched path (trj_batch_sq_run, graph-replayed groups) against
+    the per-frame path (trj_sq_init on the frame's box, untimed, then one
+    trj_frame_device and trj_sq_shells, timed) on the same HBM-resident
+    frames; the largest relative difference of any shell sum between them."""
+    import torch
+    from trj_analysis_amd import trjhip as T
+
+    grid = (64, 64, 64)
+    prec = T.PRECISIONS[precision]
+    nf = len(frames)
+    bsc = [[1.0, 0.6, -0.3, 0.4][:f.nsp] for f in frames]
+    ctx = T.Context(0)
+    plain = []
+    t_plain = 0.0
+    for rep in range(2):  # the first pass warms up
+        t_plain = 0.0
+        plain = []
+        for k, f in enumerate(frames):
+            ctx.sq_init(T.sq_params(3, f.nsp, 1e9, bsc=bsc[k], qset=T.TRJ_QSET_HALFSPACE, grid=grid,
+                                    precision=precision), f.box)
+            ctx.sync()
+            t0 = time.perf_counter()
+            ctx.frame_device(pos.data_ptr() + 12 * int(offs[k]), T.TRJ_F32, T.TRJ_AOS,
+                             it.data_ptr() + 4 * int(offs[k]), len(f.itype), f.box, T.TRJ_WANT_SQ)
+            plain.append(ctx.sq_shells())
+            t_plain += time.perf_counter() - t0
+    ctx.close()
+    fa = T.batch_frames((len(f.itype), f.nsp, f.box, (1, 1, 1)) for f in frames)
+    b = T.Batch(T.batch_params(3, 0.01, 4.0, use_graph=True))
+    q = T.batch_sq_params(grid, prec, 1e9)
+    b.sq_run(q, fa, pos.data_ptr(), T.TRJ_F32, it.data_ptr(), bsc=bsc)  # warm (captures)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    got, gpu_ms = b.sq_run(q, fa, pos.data_ptr(), T.TRJ_F32, it.data_ptr(), bsc=bsc)
+    t_batch = time.perf_counter() - t0
+    b.close()
+    rel = 0.0
+    nq_equal = True
+    for (sqf, sqfp, nq), (wf, wfp, wnq) in zip(got, plain):
+        nq_equal = nq_equal and np.array_equal(nq, wnq)
+        m = wnq > 0
+        rel = max(rel, float(np.max(np.abs(sqf[m] - wf[m]) / np.abs(wf[m]))),
+                  float(np.max(np.abs(sqfp[:, m] - wfp[:, m]) / np.abs(wfp[:, m]))))
+    natoms = int(offs[-1])
+    terms = natoms * (grid[0] * grid[1] * grid[2] // 2)
+    log(f"batched S(Q) {precision}: {nf / t_batch:.1f} frames/s vs per-frame {nf / t_plain:.1f}; "
+        f"max shell rel diff {rel:.2e}")
+    return {
+        "precision": precision, "grid": list(grid), "qset": "halfspace",
+        "frames_per_s": nf / t_batch, "ms_per_frame": t_batch * 1e3 / nf, "batch_gpu_ms": gpu_ms,
+        "q_atom_terms_per_s": terms / t_batch,
+        "plain_frames_per_s": nf / t_plain, "speedup_vs_plain": t_plain / t_batch,
+        "nq_equal": nq_equal, "max_shell_rel_diff_vs_plain": rel,
+        "note": "plain: per frame trj_sq_init on its box (untimed), then trj_frame_device + trj_sq_shells "
+                "(timed); batch: trj_batch_sq_run over all frames (graph-replayed groups of 64)",
     }
 
 

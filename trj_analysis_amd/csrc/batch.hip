@@ -436,7 +436,9 @@ int run_sq_groups(trj_batch* b, const trj_batch_sq_params& q, int nframes, const
   const std::vector<int32_t> tiles = halfspace_tiles(p.ndim, q.grid, q.precision, kmax);
   const int ntiles = static_cast<int>(tiles.size() / 4);
   const TileShape tk = tile_shape(q.precision);
-  const int64_t plane_bytes = int64_t(16) * ntiles * tk.vec();
+  const int64_t tvec_total = int64_t(ntiles) * tk.vec();
+  const int64_t plane_bytes = 16 * tvec_total;
+  constexpr int64_t kUnitLoads = 256 * 48;
   // Groups: consecutive frames, at most frames_per_launch (0: 64), their rho
   // planes within kSqRhoCap and their atoms within kSqAtomCap.
   const int F = p.frames_per_launch > 0 ? p.frames_per_launch : 64;
@@ -527,7 +529,6 @@ int run_sq_groups(trj_batch* b, const trj_batch_sq_params& q, int nframes, const
   d.nq = static_cast<unsigned long long*>(b->sq_nq.p);
   d.flags = static_cast<uint32_t*>(b->flags.p);
   d.nblocks = p.workgroups > 0 ? p.workgroups : 1024;
-  d.shell_parts = 16;
   d.shell_lds = shell_lds;
   const bool graph = p.use_graph != 0;
   const bool same = b->sq_gexec && std::memcmp(&b->sq_gdev, &d, sizeof(d)) == 0 && b->sq_gprec == q.precision;
@@ -570,7 +571,7 @@ int run_sq_groups(trj_batch* b, const trj_batch_sq_params& q, int nframes, const
     BatchSqFrame* D = reinterpret_cast<BatchSqFrame*>(pin + sizeof(BatchSqHeader));
     int4* items = reinterpret_cast<int4*>(pin + sizeof(BatchSqHeader) + sizeof(BatchSqFrame) * cap_frames);
     int32_t* iplane = reinterpret_cast<int32_t*>(items + cap_items);
-    int blk0 = 0, moff = 0, chan0 = 0, plane0 = 0;
+    int blk0 = 0, moff = 0, chan0 = 0, plane0 = 0, units = 0;
     for (int f = g0; f < g1; ++f) {
       BatchSqFrame& x = D[f - g0];
       std::memset(&x, 0, sizeof(x));
@@ -586,6 +587,13 @@ int run_sq_groups(trj_batch* b, const trj_batch_sq_params& q, int nframes, const
       x.plane0 = plane0;
       x.ks = plan[f].ks;
       x.nqmax = plan[f].nqmax;
+      // Shell units: about kUnitLoads split partials each (one vector per
+      // thread at least, at most 32).
+      const int64_t per_vec = int64_t(x.ks) * x.nsp;
+      const int64_t vpu = std::max<int64_t>(256, std::min<int64_t>(32 * 256, (kUnitLoads / per_vec) / 256 * 256));
+      x.vpu = static_cast<int32_t>(vpu);
+      x.unit0 = units;
+      units += static_cast<int>((tvec_total + vpu - 1) / vpu);
       for (int a = 0; a < 3; ++a) {
         x.box[a] = a < p.ndim ? fr[f].box[a] : 1.0;
         x.fk[a] = plan[f].fk[a];
@@ -615,7 +623,7 @@ int run_sq_groups(trj_batch* b, const trj_batch_sq_params& q, int nframes, const
     H->ncounts = moff;
     H->nch = chan0;
     H->nitems = ni;
-    H->pad = 0;
+    H->nunits = units;
     BHIP(b, hipMemcpyAsync(b->sq_hdr.p, H, sizeof(BatchSqHeader), hipMemcpyHostToDevice, b->stream));
     BHIP(b, hipMemcpyAsync(b->sq_fr.p, D, sizeof(BatchSqFrame) * nf, hipMemcpyHostToDevice, b->stream));
     BHIP(b, hipMemcpyAsync(b->sq_items.p, items, sizeof(int4) * ni, hipMemcpyHostToDevice, b->stream));

@@ -334,7 +334,9 @@ struct BatchSqFrame {
   int32_t moff;        // its [nsp][nblk] block counts in the group's flat count array
   int32_t chan0;       // first channel
   int32_t plane0, ks;  // channel s, split k at rho plane plane0 + k * nsp + s
-  int32_t nqmax, pad;
+  int32_t nqmax;
+  int32_t vpu;         // tile vectors per unit of the shell kernel
+  int32_t unit0, pad;  // its first unit
   double box[3];
   double fk[3], dq;    // 2 pi / L per axis (FP64) and their minimum
   double bsc[kNspMax];
@@ -342,7 +344,7 @@ struct BatchSqFrame {
 struct BatchSqHeader {
   int32_t nframes, nblocks;  // frames and count blocks of the group
   int32_t ncounts, nch;      // flat block counts, channels
-  int32_t nitems, pad;       // contraction work items
+  int32_t nitems, nunits;    // contraction work items, shell-kernel units
 };
 struct BatchSqDev {
   const BatchSqHeader* hdr;
@@ -367,7 +369,6 @@ struct BatchSqDev {
   unsigned long long* nq;
   uint32_t* flags;
   int nblocks;         // fixed grid of the count, scatter and shell kernels
-  int shell_parts;     // vector ranges per frame in the shell kernel
   int shell_lds;       // doubles of LDS accumulators per workgroup (0: global atomics)
 };
 void launch_batch_sq_group(const BatchSqDev& d, int precision, hipStream_t s);

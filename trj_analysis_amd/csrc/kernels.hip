@@ -3326,23 +3326,34 @@ __global__ __launch_bounds__(256) void bsq_scatter(BatchSqDev d) {
   }
 }
 
-// Shells of every frame: workgroup unit (f, part) takes a range of the tile
-// vectors, sums the frame's splits per channel, and adds |sum_s b_s rho_s|^2,
-// |rho_s|^2 and a count of 1 to the vector's shell nint(|q| / dq) of the
-// frame's box, the FP64 rule of the half-space plan (shells 1 .. nqmax, the
-// origin left out).  Sums gather in LDS per unit, then go to the outputs
-// with FP64 / integer atomics.
+// The frame whose shell-kernel units hold unit u (frames in unit order).
+__device__ inline int bsq_frame_of_unit(const BatchSqFrame* __restrict__ fr, int nframes, int u) {
+  int lo = 0, hi = nframes - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (fr[mid].unit0 <= u) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Shells of every frame: a unit is vpu consecutive tile vectors of one frame
+// (vpu set by the host so that every unit reads about the same number of
+// split partials); for each vector the frame's splits are summed per
+// channel, and |sum_s b_s rho_s|^2, |rho_s|^2 and a count of 1 go to the
+// vector's shell nint(|q| / dq) of the frame's box, the FP64 rule of the
+// half-space plan (shells 1 .. nqmax, the origin left out).  Sums gather in
+// LDS per unit, then go to the outputs with FP64 / integer atomics.
 __global__ __launch_bounds__(256) void bsq_shells(BatchSqDev d) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* acc = reinterpret_cast<double*>(smem);
-  const int nframes = d.hdr->nframes;
+  const int nframes = d.hdr->nframes, nunits = d.hdr->nunits;
   const int tvec = d.tx * d.ty * d.tz;
   const int64_t total = static_cast<int64_t>(d.ntiles) * tvec;
-  const int64_t chunk = (total + d.shell_parts - 1) / d.shell_parts;
   const double2* r2 = reinterpret_cast<const double2*>(d.rho);
-  for (int unit = blockIdx.x; unit < nframes * d.shell_parts; unit += gridDim.x) {
-    const BatchSqFrame& F = d.fr[unit / d.shell_parts];
-    const int part = unit % d.shell_parts;
+  for (int unit = blockIdx.x; unit < nunits; unit += gridDim.x) {
+    const BatchSqFrame& F = d.fr[bsq_frame_of_unit(d.fr, nframes, unit)];
+    const int part = unit - F.unit0;
+    const int64_t chunk = F.vpu;
     const int ns = F.nqmax + 1, nsp = F.nsp;
     const int nacc = (2 + nsp) * ns;  // sqf, sqfp [nsp], counts
     const bool lds = nacc <= d.shell_lds;
@@ -3375,8 +3386,11 @@ __global__ __launch_bounds__(256) void bsq_shells(BatchSqDev d) {
       double tr = 0.0, ti = 0.0;
       for (int s = 0; s < nsp; ++s) {
         double rr = 0.0, ri = 0.0;
+        const double2* src = r2 + static_cast<int64_t>(F.plane0 + s) * total + e;
+        const int64_t step = static_cast<int64_t>(nsp) * total;
+#pragma unroll 8
         for (int k = 0; k < F.ks; ++k) {
-          const double2 x = r2[static_cast<int64_t>(F.plane0 + k * nsp + s) * total + e];
+          const double2 x = src[k * step];
           rr += x.x;
           ri += x.y;
         }
