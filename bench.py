@@ -206,7 +206,7 @@ def bench_batch_sq(frames, pos, it, offs, precision: str) -> dict:
         plain = []
         for k, f in enumerate(frames):
             ctx.sq_init(T.sq_params(3, f.nsp, 1e9, bsc=bsc[k], qset=T.TRJ_QSET_HALFSPACE, grid=grid,
-                                    precision=precision), f.box)
+                                    precision=precision, flags=T.TRJ_SQ_NO_AUTOTUNE), f.box)
             ctx.sync()
             t0 = time.perf_counter()
             ctx.frame_device(pos.data_ptr() + 12 * int(offs[k]), T.TRJ_F32, T.TRJ_AOS,
@@ -240,8 +240,9 @@ def bench_batch_sq(frames, pos, it, offs, precision: str) -> dict:
         "q_atom_terms_per_s": terms / t_batch,
         "plain_frames_per_s": nf / t_plain, "speedup_vs_plain": t_plain / t_batch,
         "nq_equal": nq_equal, "max_shell_rel_diff_vs_plain": rel,
-        "note": "plain: per frame trj_sq_init on its box (untimed), then trj_frame_device + trj_sq_shells "
-                "(timed); batch: trj_batch_sq_run over all frames (graph-replayed groups of 64)",
+        "note": "plain: per frame trj_sq_init on its box (untimed, default K-split: no first-frame "
+                "autotuning per frame), then trj_frame_device + trj_sq_shells (timed); batch: "
+                "trj_batch_sq_run over all frames (graph-replayed groups of 64)",
     }
 
 

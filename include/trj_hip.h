@@ -79,6 +79,9 @@ extern "C" {
                                    divided by the frame's point count, octant set only
                                    (no axis vectors); sqcl.dat is sqf(i)/(Nconf nq(i)),
                                    i = 1..nqmin (log.f90:637-642) */
+#define TRJ_SQ_NO_AUTOTUNE 8u   /* keep the default K-split of the FP64 contraction
+                                   instead of timing candidate split lengths on the
+                                   first frame (trj_sq_tuning) */
 
 /* RDF flags. */
 #define TRJ_RDF_CHARGE 1u /* also accumulate the charge-weighted histogram g_qq
@@ -132,6 +135,14 @@ int trj_rdf_init(trj_ctx* ctx, const trj_rdf_params* p, const double box[3],
  * with TRJ_RDF_NO_AUTOTUNE).  Any argument may be NULL. */
 int trj_rdf_tuning(trj_ctx* ctx, int32_t* rows_per_pass, int32_t* passes, int32_t* waves_per_cell, double* ms,
                    int32_t* tried);
+
+/* The cell grid of the last RDF frame (no counterpart in the reference, whose
+ * rdf_sh loops over all pairs, rdf.cuf:426-487): cells per axis (0 when the
+ * frame took the all-pairs kernel) and the cell side as a multiple of the
+ * cutoff the autotuner chose (1, 1.25 or 1.5; cells of one cutoff without the
+ * autotuner).  The environment variable TRJ_RDF_CELL_SCALE (1 to 4), read by
+ * trj_rdf_init, fixes that multiple instead.  Any argument may be NULL. */
+int trj_rdf_cell_grid(trj_ctx* ctx, int32_t cells[3], double* side_over_cutoff);
 
 /* /INPUT_SQ/ (input.f90:70) plus the build's options. */
 typedef struct {
@@ -277,6 +288,15 @@ int trj_sq_qq_column(trj_ctx* ctx, int64_t nconf, double* out);
  * to the dense q-set: rho[(v*nsp + s)*2 + {0,1}].  Debug/parity access to the
  * contraction output.  Synchronises. */
 int trj_sq_rho(trj_ctx* ctx, int64_t nvec, const int32_t* kv, double* rho);
+
+/* The S(Q) autotuner's choice (no counterpart in the reference, whose sqfnD
+ * launch shape is fixed, sq.cuf:175-179): K-splits of the last frame's
+ * contraction, the largest atoms per split the FP64 autotuner chose (0 for
+ * the FP32-accumulating precisions, whose split length their accuracy
+ * bound sets), that split's time in ms on the tuning frame and the number of
+ * split lengths timed (0 before the first frame or with TRJ_SQ_NO_AUTOTUNE).
+ * Any argument may be NULL. */
+int trj_sq_tuning(trj_ctx* ctx, int32_t* splits, int32_t* atoms_per_split, double* ms, int32_t* tried);
 
 /* rdf_clear / sq_clear (rdf.cuf:124, sq.cuf:774): zero the accumulators and
  * frame counters, keep the plans. */

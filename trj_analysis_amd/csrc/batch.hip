@@ -437,7 +437,7 @@ int run_sq_groups(trj_batch* b, const trj_batch_sq_params& q, int nframes, const
   const int ntiles = static_cast<int>(tiles.size() / 4);
   const TileShape tk = tile_shape(q.precision);
   const int64_t tvec_total = int64_t(ntiles) * tk.vec();
-  const int64_t plane_bytes = 16 * tvec_total;
+  const int64_t plane_bytes = static_cast<int64_t>(rho_entry_bytes(q.precision)) * tvec_total;
   constexpr int64_t kUnitLoads = 256 * 48;
   // Groups: consecutive frames, at most frames_per_launch (0: 64), their rho
   // planes within kSqRhoCap and their atoms within kSqAtomCap.
@@ -524,6 +524,7 @@ int run_sq_groups(trj_batch* b, const trj_batch_sq_params& q, int nframes, const
   d.tz = tk.tz;
   for (int k = 0; k < 3; ++k) d.kmax[k] = kmax[k];
   d.rho = static_cast<double*>(b->sq_rho.p);
+  d.rho_f32 = q.precision != TRJ_PREC_FP64;
   d.sqf = static_cast<double*>(b->sq_sqf.p);
   d.sqfp = static_cast<double*>(b->sq_sqfp.p);
   d.nq = static_cast<unsigned long long*>(b->sq_nq.p);

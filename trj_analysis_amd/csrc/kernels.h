@@ -232,7 +232,9 @@ struct ContractArgs {
   int ntiles, ksplit, nsp, ndim;
   int nch;                // channels: nsp species, + 1 charge channel
   int64_t n;
-  double* rho;            // [split][channel][tile][tile vectors][2]
+  double* rho;            // [split][channel][tile][tile vectors][2]: FP64
+                          // (k_sq_contract_f64) or FP32 (the FP32 and 16-bit
+                          // kernels, whose split sums are FP32)
   // Work items (batched frames; NULL for one frame): ksplit is then the
   // number of items (the grid is ntiles x ksplit), item k = items[k] =
   // {c0, c1, split, splits} takes split `split` of `splits` of each channel
@@ -244,6 +246,9 @@ struct ContractArgs {
   const int32_t* nitems;
 };
 void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s);
+// Bytes of one (split, channel, vector) partial of rho: FP64 complex from the
+// FP64 contraction, FP32 complex from the others.
+inline size_t rho_entry_bytes(int precision) { return precision == TRJ_PREC_FP64 ? 16 : 8; }
 // FP32 MFMA, or FP16 / BF16 hi+lo split with three products (precision =
 // TRJ_PREC_FP32 / FP16X3 / BF16X3); FP32 accumulation folded into FP64.
 void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s);
@@ -254,8 +259,9 @@ void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s);
 // each vector's values by the atom counts (slices).
 // qsp (may be NULL): charges constant within each species; the charge sums
 // are then sum_s q_s rho_s of the species channels (no charge channel).
-void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, int tile_vec, int ksplit,
-                      int nsp, int nslice, int nch, const double* bsc, int nqmax, const int32_t* norm,
+// rho_f32: the partials are FP32 complex (precision other than FP64).
+void launch_sq_shells(const void* rho, bool rho_f32, const int16_t* tile_shell, int ntiles, int tile_vec,
+                      int ksplit, int nsp, int nslice, int nch, const double* bsc, int nqmax, const int32_t* norm,
                       const double* qsp, double* sqf, double* sqfp, double* sqfq, hipStream_t s);
 // Explicit q-vectors (axis directions): direct FP64 phase sums over atom
 // blocks, combined in a fixed order; part needs sq_list_scratch_bytes().
@@ -363,7 +369,8 @@ struct BatchSqDev {
   const int32_t* tiles;
   int ntiles, tx, ty, tz;
   int kmax[3];
-  double* rho;         // [plane][tile][tile vectors][2]
+  double* rho;         // [plane][tile][tile vectors][2], FP64 or (rho_f32) FP32
+  int rho_f32;
   double* sqf;         // all frames' outputs (BatchSqFrame offsets)
   double* sqfp;
   unsigned long long* nq;

@@ -1732,7 +1732,8 @@ __global__ __launch_bounds__(256) void k_sq_contract_f32(ContractArgs a) {
     const int b0 = a0 + static_cast<int>(len * wi.split / wi.splits);
     const int b1 = a0 + static_cast<int>(len * (wi.split + 1) / wi.splits);
     // FP32 accumulation over this workgroup's atoms (at most 8192 per species,
-    // set by the host through ksplit); the K-splits are summed in FP64.
+    // set by the host through ksplit); each split's sums are stored as they
+    // are (FP32 complex) and the K-splits are summed in FP64.
     f4 accR[8], accI[8];
     for (int nb = 0; nb < 8; ++nb) {
       accR[nb] = f4{0.f, 0.f, 0.f, 0.f};
@@ -1827,8 +1828,7 @@ __global__ __launch_bounds__(256) void k_sq_contract_f32(ContractArgs a) {
         const int kxl = 16 * w + 4 * kq + r;
         const int n = 16 * nb + i16;
         const int v = (kxl * LY + n / LZ) * LZ + (n % LZ);
-        reinterpret_cast<double2*>(a.rho)[base + v] =
-            make_double2(static_cast<double>(accR[nb][r]), static_cast<double>(accI[nb][r]));
+        reinterpret_cast<float2*>(a.rho)[base + v] = make_float2(accR[nb][r], accI[nb][r]);
       }
     }
   }
@@ -1947,7 +1947,7 @@ constexpr int kRS = 34;
 // buffers, two A buffers) cover the half-step offset between them.  The turns
 // of a chunk are loaded one chunk step before they are staged.  FP32
 // accumulation runs over at most 8192 atoms per workgroup and channel (the
-// K-splits are summed in FP64).
+// K-splits, stored as FP32 complex partials, are summed in FP64).
 constexpr int WX = 64, WY = 8, WZ = 32, WN = WY * WZ, WROWS = 104;
 template <int PREC>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k_sq_contract_lp16w(
@@ -1983,7 +1983,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int b1 = a0 + static_cast<int>(len * (wi.split + 1) / wi.splits);
     if (b0 >= b1) {
       const size_t base = ((wi.plane0 + (s - wi.c0)) * a.ntiles + tile) * (WX * WN);
-      for (int v = t; v < WX * WN; v += 512) reinterpret_cast<double2*>(a.rho)[base + v] = make_double2(0.0, 0.0);
+      for (int v = t; v < WX * WN; v += 512) reinterpret_cast<float2*>(a.rho)[base + v] = make_float2(0.f, 0.f);
       continue;
     }
     f4 accR[4][2], accI[4][2], accM[4][2];
@@ -2212,15 +2212,28 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
         for (int r = 0; r < 4; ++r) {
           const int kxl = 16 * mb + 4 * kq + r;
           const int v = (kxl * WY + w) * WZ + 16 * e + i16;
-          reinterpret_cast<double2*>(a.rho)[base + v] =
-              make_double2(static_cast<double>(accR[mb][e][r]) - static_cast<double>(accM[mb][e][r]),
-                           static_cast<double>(accI[mb][e][r]));
+          // Re = r - m rounded once to FP32 (|error| <= 2^-24 |Re| per split,
+          // far below the FP32 accumulation error of the split itself)
+          reinterpret_cast<float2*>(a.rho)[base + v] = make_float2(accR[mb][e][r] - accM[mb][e][r], accI[mb][e][r]);
         }
   }
 }
 
 struct Bsc {
   double b[kNspMax];
+};
+
+// Neumaier's compensated sum: s + c carries the sum of the added values with
+// the rounding error of each addition kept in c (the split partials of a
+// coherent vector can be much larger than their sum).
+struct CSum {
+  double s = 0.0, c = 0.0;
+  __device__ inline void add(double x) {
+    const double t = s + x;
+    c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+    s = t;
+  }
+  __device__ inline double value() const { return s + c; }
 };
 
 // |rho|^2 of one q-vector into the shell sums.  Channels: slice sl, species
@@ -2271,7 +2284,10 @@ __device__ inline void shell_vector(int nsp, int nslice, int nch, const Bsc& bsc
   }
 }
 
-__global__ __launch_bounds__(256) void k_sq_shells(const double* __restrict__ rho,
+// R2 is the type of one split partial: double2 (the FP64 contraction) or
+// float2 (the FP32 and 16-bit ones).
+template <class R2>
+__global__ __launch_bounds__(256) void k_sq_shells(const R2* __restrict__ rho,
                                                    const int16_t* __restrict__ tile_shell, int ntiles,
                                                    int tile_vec, int ksplit, int nsp, int nslice, int nch, Bsc bsc,
                                                    int nqmax, const int32_t* __restrict__ norm, Bsc qsp, int qsp_on,
@@ -2288,19 +2304,20 @@ __global__ __launch_bounds__(256) void k_sq_shells(const double* __restrict__ rh
     __syncthreads();
   }
   const int64_t total = static_cast<int64_t>(ntiles) * tile_vec;
-  const double2* r2 = reinterpret_cast<const double2*>(rho);
+  const R2* r2 = rho;
   for (int64_t e = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; e < total;
        e += static_cast<int64_t>(gridDim.x) * 256) {
     const int sh = tile_shell[e];
     if (sh < 0) continue;
     auto fetch = [&](int c, double& rr, double& ri) {
-      rr = 0.0;
-      ri = 0.0;
+      CSum sr, si;
       for (int sp = 0; sp < ksplit; ++sp) {
-        const double2 v = r2[(static_cast<int64_t>(sp) * nch + c) * total + e];
-        rr += v.x;
-        ri += v.y;
+        const R2 v = r2[(static_cast<int64_t>(sp) * nch + c) * total + e];
+        sr.add(static_cast<double>(v.x));
+        si.add(static_cast<double>(v.y));
       }
+      rr = sr.value();
+      ri = si.value();
     };
     auto add = [&](int kind, int idx, double v) {
       const int off = (kind == 0 ? 0 : (kind == 1 ? o1 : o2)) + idx * ns + sh;
@@ -2314,6 +2331,105 @@ __global__ __launch_bounds__(256) void k_sq_shells(const double* __restrict__ rh
         atomicAdd(&sqfq[idx * ns + sh], v);
     };
     shell_vector(nsp, nslice, nch, bsc, norm, qsp_on ? &qsp : nullptr, fetch, add);
+  }
+  if (use_lds) {
+    __syncthreads();
+    for (int b = threadIdx.x; b < nacc; b += 256) {
+      const double v = acc[b];
+      if (v != 0.0) {
+        if (b < o1)
+          atomicAdd(&sqf[b], v);
+        else if (b < o2)
+          atomicAdd(&sqfp[b - o1], v);
+        else
+          atomicAdd(&sqfq[b - o2], v);
+      }
+    }
+  }
+}
+
+// The same sums with the K-splits of each vector spread over the four
+// wavefronts of a workgroup (one vector per lane, 64 vectors per workgroup
+// step): wavefront w adds splits w, w + 4, ... (compensated); the four sums
+// and their compensations are combined in that order in LDS and wavefront 0
+// bins the vector.  Sixteen times the loads in flight of one thread per
+// vector over all its splits.  LDS: the shell accumulators (use_lds), 4 x 64
+// partial sums and 4 x 64 compensations, then the vector's channel sums
+// [nch][64].
+template <class R2>
+__global__ __launch_bounds__(256) void k_sq_shells4(const R2* __restrict__ rho,
+                                                    const int16_t* __restrict__ tile_shell, int ntiles,
+                                                    int tile_vec, int ksplit, int nsp, int nslice, int nch, Bsc bsc,
+                                                    int nqmax, const int32_t* __restrict__ norm, Bsc qsp, int qsp_on,
+                                                    double* __restrict__ sqf, double* __restrict__ sqfp,
+                                                    double* __restrict__ sqfq, int use_lds) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* acc = reinterpret_cast<double*>(smem);
+  const int ns = nqmax + 1;
+  const int o1 = nslice * ns, o2 = nslice * (1 + nsp) * ns;
+  const int nacc = o2 + ((nch > nslice * nsp || qsp_on) ? nslice * ns : 0);
+  double2* part = reinterpret_cast<double2*>(smem + (use_lds ? ((static_cast<size_t>(nacc) * 8 + 15) & ~size_t(15)) : 0));
+  double2* chan = part + 8 * kWave;
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  if (use_lds) {
+    for (int b = threadIdx.x; b < nacc; b += 256) acc[b] = 0.0;
+    __syncthreads();
+  }
+  const int64_t total = static_cast<int64_t>(ntiles) * tile_vec;
+  for (int64_t base = static_cast<int64_t>(blockIdx.x) * kWave; base < total;
+       base += static_cast<int64_t>(gridDim.x) * kWave) {
+    const int64_t e = base + lane;
+    const int sh = e < total ? tile_shell[e] : -1;
+    for (int c = 0; c < nch; ++c) {
+      CSum sr, si;
+      if (sh >= 0) {
+        const R2* src = rho + static_cast<int64_t>(c) * total + e;
+        const int64_t step = static_cast<int64_t>(nch) * total;
+#pragma unroll 4
+        for (int sp = w; sp < ksplit; sp += 4) {
+          const R2 v = src[sp * step];
+          sr.add(static_cast<double>(v.x));
+          si.add(static_cast<double>(v.y));
+        }
+      }
+      // each wavefront's sum and its compensation
+      part[w * kWave + lane] = make_double2(sr.s, si.s);
+      part[(4 + w) * kWave + lane] = make_double2(sr.c, si.c);
+      __syncthreads();
+      if (w == 0) {
+        CSum tr, ti;
+        for (int k = 0; k < 4; ++k) {
+          tr.add(part[k * kWave + lane].x);
+          ti.add(part[k * kWave + lane].y);
+        }
+        double cr = 0.0, ci = 0.0;
+        for (int k = 0; k < 4; ++k) {
+          cr += part[(4 + k) * kWave + lane].x;
+          ci += part[(4 + k) * kWave + lane].y;
+        }
+        chan[c * kWave + lane] = make_double2(tr.s + (tr.c + cr), ti.s + (ti.c + ci));
+      }
+      __syncthreads();
+    }
+    if (w == 0 && sh >= 0) {
+      auto fetch = [&](int c, double& rr, double& ri) {
+        const double2 t = chan[c * kWave + lane];
+        rr = t.x;
+        ri = t.y;
+      };
+      auto add = [&](int kind, int idx, double v) {
+        const int off = (kind == 0 ? 0 : (kind == 1 ? o1 : o2)) + idx * ns + sh;
+        if (use_lds)
+          atomicAdd(&acc[off], v);
+        else if (kind == 0)
+          atomicAdd(&sqf[idx * ns + sh], v);
+        else if (kind == 1)
+          atomicAdd(&sqfp[static_cast<int64_t>(idx) * ns + sh], v);
+        else
+          atomicAdd(&sqfq[idx * ns + sh], v);
+      };
+      shell_vector(nsp, nslice, nch, bsc, norm, qsp_on ? &qsp : nullptr, fetch, add);
+    }
   }
   if (use_lds) {
     __syncthreads();
@@ -3132,8 +3248,8 @@ void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s) 
     hipLaunchKernelGGL(k_sq_contract_lp16w<TRJ_PREC_BF16X3>, grid, dim3(512), 0, s, a);
 }
 
-void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, int tile_vec, int ksplit,
-                      int nsp, int nslice, int nch, const double* bsc, int nqmax, const int32_t* norm,
+void launch_sq_shells(const void* rho, bool rho_f32, const int16_t* tile_shell, int ntiles, int tile_vec,
+                      int ksplit, int nsp, int nslice, int nch, const double* bsc, int nqmax, const int32_t* norm,
                       const double* qsp, double* sqf, double* sqfp, double* sqfq, hipStream_t s) {
   if (ntiles <= 0) return;
   Bsc b{}, q{};
@@ -3144,11 +3260,32 @@ void launch_sq_shells(const double* rho, const int16_t* tile_shell, int ntiles, 
   const size_t lds = static_cast<size_t>(nacc) * (nqmax + 1) * sizeof(double);
   const int use_lds = lds <= 48 * 1024;
   const int64_t total = static_cast<int64_t>(ntiles) * tile_vec;
+  // Splits over four wavefronts when the channel sums fit in LDS beside the
+  // accumulators (every case but very many slices).
+  const size_t lds4 = (use_lds ? ((lds + 15) & ~size_t(15)) : 0) + sizeof(double) * 2 * kWave * (8 + nch);
+  if (ksplit >= 4 && lds4 <= 64 * 1024) {
+    int64_t g4 = (total + kWave - 1) / kWave;
+    if (g4 > 2048) g4 = 2048;
+    if (rho_f32)
+      hipLaunchKernelGGL(k_sq_shells4<float2>, dim3(static_cast<unsigned>(g4)), dim3(256), lds4, s,
+                         static_cast<const float2*>(rho), tile_shell, ntiles, tile_vec, ksplit, nsp, nslice, nch, b,
+                         nqmax, norm, q, qsp ? 1 : 0, sqf, sqfp, sqfq, use_lds);
+    else
+      hipLaunchKernelGGL(k_sq_shells4<double2>, dim3(static_cast<unsigned>(g4)), dim3(256), lds4, s,
+                         static_cast<const double2*>(rho), tile_shell, ntiles, tile_vec, ksplit, nsp, nslice, nch, b,
+                         nqmax, norm, q, qsp ? 1 : 0, sqf, sqfp, sqfq, use_lds);
+    return;
+  }
   int64_t g = (total + 255) / 256;
   if (g > 1024) g = 1024;
-  hipLaunchKernelGGL(k_sq_shells, dim3(static_cast<unsigned>(g)), dim3(256), use_lds ? lds : 0, s, rho,
-                     tile_shell, ntiles, tile_vec, ksplit, nsp, nslice, nch, b, nqmax, norm, q, qsp ? 1 : 0, sqf,
-                     sqfp, sqfq, use_lds);
+  if (rho_f32)
+    hipLaunchKernelGGL(k_sq_shells<float2>, dim3(static_cast<unsigned>(g)), dim3(256), use_lds ? lds : 0, s,
+                       static_cast<const float2*>(rho), tile_shell, ntiles, tile_vec, ksplit, nsp, nslice, nch, b,
+                       nqmax, norm, q, qsp ? 1 : 0, sqf, sqfp, sqfq, use_lds);
+  else
+    hipLaunchKernelGGL(k_sq_shells<double2>, dim3(static_cast<unsigned>(g)), dim3(256), use_lds ? lds : 0, s,
+                       static_cast<const double2*>(rho), tile_shell, ntiles, tile_vec, ksplit, nsp, nslice, nch, b,
+                       nqmax, norm, q, qsp ? 1 : 0, sqf, sqfp, sqfq, use_lds);
 }
 
 size_t sq_list_scratch_bytes(int nvec, int nch) {
@@ -3350,6 +3487,7 @@ __global__ __launch_bounds__(256) void bsq_shells(BatchSqDev d) {
   const int tvec = d.tx * d.ty * d.tz;
   const int64_t total = static_cast<int64_t>(d.ntiles) * tvec;
   const double2* r2 = reinterpret_cast<const double2*>(d.rho);
+  const float2* r2f = reinterpret_cast<const float2*>(d.rho);
   for (int unit = blockIdx.x; unit < nunits; unit += gridDim.x) {
     const BatchSqFrame& F = d.fr[bsq_frame_of_unit(d.fr, nframes, unit)];
     const int part = unit - F.unit0;
@@ -3385,15 +3523,27 @@ __global__ __launch_bounds__(256) void bsq_shells(BatchSqDev d) {
       const int ish = static_cast<int>(sh);
       double tr = 0.0, ti = 0.0;
       for (int s = 0; s < nsp; ++s) {
-        double rr = 0.0, ri = 0.0;
-        const double2* src = r2 + static_cast<int64_t>(F.plane0 + s) * total + e;
+        CSum sr, si;
+        const int64_t first = static_cast<int64_t>(F.plane0 + s) * total + e;
         const int64_t step = static_cast<int64_t>(nsp) * total;
+        if (d.rho_f32) {
+          const float2* src = r2f + first;
 #pragma unroll 8
-        for (int k = 0; k < F.ks; ++k) {
-          const double2 x = src[k * step];
-          rr += x.x;
-          ri += x.y;
+          for (int k = 0; k < F.ks; ++k) {
+            const float2 x = src[k * step];
+            sr.add(static_cast<double>(x.x));
+            si.add(static_cast<double>(x.y));
+          }
+        } else {
+          const double2* src = r2 + first;
+#pragma unroll 8
+          for (int k = 0; k < F.ks; ++k) {
+            const double2 x = src[k * step];
+            sr.add(x.x);
+            si.add(x.y);
+          }
         }
+        const double rr = sr.value(), ri = si.value();
         add(1 + s, ish, rr * rr + ri * ri);
         tr += F.bsc[s] * rr;
         ti += F.bsc[s] * ri;

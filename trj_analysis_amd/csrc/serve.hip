@@ -75,8 +75,10 @@ void execute(trj_ctx* ctx, Req& q) {
   // autotuning of the pair kernels (it would time candidates per request).
   trj_rdf_params rp = r.rdf;
   rp.flags |= TRJ_RDF_NO_AUTOTUNE;
+  trj_sq_params sp = r.sq;
+  sp.flags |= TRJ_SQ_NO_AUTOTUNE;
   if (r.what & TRJ_WANT_RDF) rc = trj_rdf_init(ctx, &rp, r.box, &ls, &nit);
-  if (rc == TRJ_OK && (r.what & TRJ_WANT_SQ)) rc = trj_sq_init(ctx, &r.sq, r.box, &info);
+  if (rc == TRJ_OK && (r.what & TRJ_WANT_SQ)) rc = trj_sq_init(ctx, &sp, r.box, &info);
   if (rc == TRJ_OK) rc = trj_frame(ctx, r.pos, r.dtype, TRJ_AOS, r.itype, r.n, r.box, r.what);
   if (rc == TRJ_OK && (r.what & TRJ_WANT_RDF)) {
     q.counts.assign(static_cast<size_t>(ls) * nit, 0);
@@ -194,8 +196,13 @@ int64_t trj_request_footprint(const trj_request* r) {
     if (trj::sq_plan(r->sq, r->box, &S, &err) == TRJ_OK) {
       const int64_t tiles = static_cast<int64_t>(S.tiles.size());
       const int64_t nax = static_cast<int64_t>(S.axis_shell.size());
-      const int ks = tiles > 0 ? trj::contraction_splits(static_cast<int>(tiles), n, r->sq.precision, r->sq.nsp) : 0;
-      b += int64_t(ks) * r->sq.nsp * tiles * S.tk.vec() * 16;  // rho partials
+      // the largest split count a fresh context may ask for: the FP64
+      // autotuner's shortest split length unless it is switched off
+      int ks = tiles > 0 ? trj::contraction_splits(static_cast<int>(tiles), n, r->sq.precision, r->sq.nsp) : 0;
+      if (tiles > 0 && r->sq.precision == TRJ_PREC_FP64 && !(r->sq.flags & TRJ_SQ_NO_AUTOTUNE))
+        ks = std::max(ks, trj::contraction_splits(static_cast<int>(tiles), n, r->sq.precision, r->sq.nsp, 16384));
+      const int64_t rho_entry = static_cast<int64_t>(trj::rho_entry_bytes(r->sq.precision));
+      b += int64_t(ks) * r->sq.nsp * tiles * S.tk.vec() * rho_entry;  // rho partials
       b += n * 96;                                             // turn planes
       b += tiles * (16 + 2 * S.tk.vec()) + 32;                 // tiles, shells of tile vectors
       b += int64_t(S.nqmax + 1) * 8 * (1 + r->sq.nsp);         // shell sums

@@ -5,7 +5,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -53,7 +55,7 @@ struct trj_ctx {
   // per-workgroup cell counts are cellblk; cursor: the atomic scatter's
   // per-cell cursors.)
   DevBuf raw, itype, pos, spos, stype, cellid, cellcount, cellstart, cursor, cellblk, partial, counts, thr, blk,
-      blkoff, sp_off, u, rho, tiles, tile_shell, sqf, sqfp, axis_kv, axis_shell, listpart, flags;
+      blkoff, sp_off, u, rho, tiles, tile_shell, sqf, sqfp, axis_kv, axis_shell, listpart, flags, tunebuf;
   // Charges (TRJ_RDF_CHARGE / TRJ_SQ_CHARGE): the frame's charges, their
   // cell-sorted copy, the frame's fixed-point g_qq sums, the g_qq and S_qq
   // accumulators.
@@ -125,14 +127,32 @@ struct trj_ctx {
   // Largest LDS allocation of one workgroup on this device.
   size_t lds_max = 65536;
   // Launch shape of the RDF pair kernels picked on the first frame after
-  // trj_rdf_init by timing the candidates (histogram rows per LDS tile,
-  // wavefronts per home cell); kept until the next trj_rdf_init.
+  // trj_rdf_init by timing the candidates (cell side as a multiple of the
+  // cutoff, histogram rows per LDS tile, wavefronts per home cell); kept
+  // until the next trj_rdf_init.
   struct {
     bool enabled = true, done = false;
     int rows = 0, W = 0;
+    double scale = 1.0;
     double ms = 0.0;
     int tried = 0;
   } rdf_tune;
+  // The cell grid of the last RDF frame (cells per axis, side / cutoff).
+  int rdf_nc[3] = {0, 0, 0};
+  double rdf_cell_scale = 0.0;
+  // TRJ_RDF_CELL_SCALE at trj_rdf_init (>= 1; 0: unset): cells of that side
+  // over the cutoff, and the autotuner tries no other.
+  double rdf_fixed_scale = 0.0;
+  // K-split of the FP64 contraction picked on the first S(Q) frame after
+  // trj_sq_init by timing candidate split lengths (atoms per split); kept
+  // until the next trj_sq_init.  The FP32-accumulating kernels keep the split
+  // their accuracy bound sets.
+  struct {
+    bool enabled = true, done = false;
+    int span = 0;
+    double ms = 0.0;
+    int tried = 0;
+  } sq_tune;
   hipEvent_t tune_ev[2] = {nullptr, nullptr};
 };
 
@@ -149,7 +169,7 @@ std::vector<DevBuf*> dev_bufs(trj_ctx* c) {
           &c->slqsum,    &c->slqcnt,    &c->sqqsum,   &c->sqqcnt,     &c->xyframe,  &c->xycounts, &c->xyacc,
           &c->xygqf,     &c->xygqacc,   &c->thr_xy,   &c->rpts,       &c->s2gi,     &c->s2g2i,    &c->fqkv,
           &c->fqbin,     &c->fqr0,      &c->fqrho0,   &c->fqpart,     &c->fqvsum,   &c->fqfkt,    &c->fqfskt,
-          &c->fqtot,     &c->fqtots,    &c->cellblk};
+          &c->fqtot,     &c->fqtots,    &c->cellblk,  &c->tunebuf};
 }
 
 }  // namespace
@@ -461,12 +481,15 @@ struct RdfCells {
   int cellshift[3] = {0, 0, 0};
   int W = 1;
   int nblocks = 1;
+  double scale = 1.0;  // cell side >= scale * cutoff
 };
 
 RdfCells rdf_cell_plan(int ndim, const double box[3], const int periodic[3], double side2, int64_t n,
-                       int W_override = 0) {
+                       int W_override = 0, double scale = 1.0) {
   RdfCells p;
-  const double rc = std::sqrt(side2);
+  // cells of side >= scale * cutoff (scale >= 1, the autotuner's choice)
+  p.scale = std::max(1.0, scale);
+  const double rc = std::sqrt(side2) * p.scale;
   for (int k = 0; k < 3; ++k) {
     if (k < ndim) {
       int nc = static_cast<int>(std::floor(box[k] / (rc * (1.0 + 1e-6))));
@@ -513,15 +536,19 @@ int rows_fitting(int nit, int lsmax, bool qq, size_t budget) {
 }
 
 // The RDF autotuner: on the first frame after trj_rdf_init, times the pair
-// kernel (all its passes) for each candidate launch shape on this frame's
-// cell list and keeps the fastest in c->rdf_tune.  Candidates: histogram
-// tiles that fit the device's LDS at 1, 2, 3 and 4 workgroups per CU
-// (fewer rows per tile means more passes but more resident workgroups), and
-// the default waves per home cell, one fewer and one more, half and twice.  Every
-// candidate counts the same pairs, so the choice changes speed only; the
-// trial runs leave nothing behind (the frame's own run follows).
-int tune_pairs(trj_ctx* c, const PairArgs& pa0, const CellGrid& cg, const RdfCells& base, int64_t n,
-               hipStream_t rs) {
+// kernel (all its passes) for each candidate launch shape and keeps the
+// fastest in c->rdf_tune.  Candidates on the frame's cell list (cells of side
+// >= the cutoff, already sorted by the caller): histogram tiles that fit the
+// device's LDS at 1, 2, 3 and 4 workgroups per CU (fewer rows per tile means
+// more passes but more resident workgroups), and the default waves per home
+// cell, one fewer and one more, half and twice.  Then, with the best tile,
+// cells of side >= 1.25 and 1.5 cutoffs (sort(p) sorts the frame into plan
+// p's cells): fewer, fuller cells, more candidates per in-range pair; they
+// win where cells of one cutoff hold few atoms.  Every candidate counts the
+// same pairs, so the choice changes speed only.  The trials leave the cells
+// of the last grid tried; the caller sorts again into the chosen one.
+int tune_pairs(trj_ctx* c, const PairArgs& pa0, const RdfCells& base, int64_t n, hipStream_t rs,
+               const std::function<RdfCells(double)>& plan, const std::function<int(const RdfCells&)>& sort) {
   auto& tu = c->rdf_tune;
   const RdfPlan& R = c->rdf;
   std::vector<int> rows;
@@ -529,16 +556,17 @@ int tune_pairs(trj_ctx* c, const PairArgs& pa0, const CellGrid& cg, const RdfCel
     const int r = rows_fitting(R.nit, R.lsmax, false, c->lds_max / per_cu);
     if (r >= 1 && std::find(rows.begin(), rows.end(), r) == rows.end()) rows.push_back(r);
   }
-  std::vector<int> ws = {base.W};
-  for (int w : {base.W / 2, base.W - 1, base.W + 1, 2 * base.W})
-    if (w >= 1 && w <= 16 && std::find(ws.begin(), ws.end(), w) == ws.end()) ws.push_back(w);
+  if (rows.empty()) rows.push_back(R.rows);
+  auto waves = [](int W0) {
+    std::vector<int> ws = {W0};
+    for (int w : {W0 / 2, W0 - 1, W0 + 1, 2 * W0})
+      if (w >= 1 && w <= 16 && std::find(ws.begin(), ws.end(), w) == ws.end()) ws.push_back(w);
+    return ws;
+  };
   tu.rows = R.rows;
   tu.W = base.W;
+  tu.scale = base.scale;
   tu.tried = 0;
-  if (rows.size() * ws.size() <= 1) {
-    tu.done = true;
-    return TRJ_OK;
-  }
   const size_t nbins = static_cast<size_t>(R.nit) * R.lsmax;
   int r;
   if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * 2048))) return r;
@@ -546,31 +574,96 @@ int tune_pairs(trj_ctx* c, const PairArgs& pa0, const CellGrid& cg, const RdfCel
   const double* spos = P<double>(c->spos);
   const int32_t* stype = P<int32_t>(c->stype);
   uint32_t* part = P<uint32_t>(c->partial);
-  // one untimed run (code object and caches warm)
   PairArgs pa = pa0;
-  pa.nrows = rows[0];
-  launch_rdf_cells(pa, cg, start, spos, stype, n, base.W, base.nblocks, part, rs);
   double best = 1e300;
-  for (int rw : rows)
-    for (int W : ws) {
-      const int nb = static_cast<int>(std::min<int64_t>((base.ncell * W + kCellWaves - 1) / kCellWaves, 2048));
-      pa.nrows = rw;
-      HIPC(c, hipEventRecord(c->tune_ev[0], rs));
-      launch_rdf_cells(pa, cg, start, spos, stype, n, W, nb, part, rs);
-      HIPC(c, hipEventRecord(c->tune_ev[1], rs));
-      HIPC(c, hipEventSynchronize(c->tune_ev[1]));
-      float ms = 0.f;
-      HIPC(c, hipEventElapsedTime(&ms, c->tune_ev[0], c->tune_ev[1]));
-      ++tu.tried;
-      if (ms < best) {
-        best = ms;
-        tu.rows = rw;
-        tu.W = W;
-      }
+  auto trial = [&](const RdfCells& p, int rw, int W, bool timed) -> int {
+    const int nb = static_cast<int>(std::min<int64_t>((p.ncell * W + kCellWaves - 1) / kCellWaves, 2048));
+    pa.nrows = rw;
+    for (int k = 0; k < 3; ++k) pa.cellshift[k] = p.cellshift[k];
+    if (!timed) {
+      launch_rdf_cells(pa, p.cg, start, spos, stype, n, W, nb, part, rs);
+      HIPC(c, hipGetLastError());
+      return TRJ_OK;
     }
-  HIPC(c, hipGetLastError());
+    HIPC(c, hipEventRecord(c->tune_ev[0], rs));
+    launch_rdf_cells(pa, p.cg, start, spos, stype, n, W, nb, part, rs);
+    HIPC(c, hipEventRecord(c->tune_ev[1], rs));
+    HIPC(c, hipEventSynchronize(c->tune_ev[1]));
+    float ms = 0.f;
+    HIPC(c, hipEventElapsedTime(&ms, c->tune_ev[0], c->tune_ev[1]));
+    ++tu.tried;
+    if (ms < best) {
+      best = ms;
+      tu.rows = rw;
+      tu.W = W;
+      tu.scale = p.scale;
+    }
+    return TRJ_OK;
+  };
+  // one untimed run (code object and caches warm)
+  if ((r = trial(base, rows[0], base.W, false))) return r;
+  for (int rw : rows)
+    for (int W : waves(base.W))
+      if ((r = trial(base, rw, W, true))) return r;
+  const int rbest = tu.rows;
+  for (double f : {1.25, 1.5}) {
+    if (c->rdf_fixed_scale > 0.0) break;
+    const RdfCells p = plan(f);
+    if (!p.cells_ok || p.ncell == base.ncell) continue;
+    if ((r = sort(p))) return r;
+    for (int W : waves(p.W))
+      if ((r = trial(p, rbest, W, true))) return r;
+  }
   tu.ms = best;
   tu.done = true;
+  return TRJ_OK;
+}
+
+// The S(Q) autotuner (FP64 contraction): on the first S(Q) frame after
+// trj_sq_init, times the contraction of this frame's turns and the shell pass
+// that reads its partial sums (run(ks) launches both for ks splits, the shells
+// into scratch sums) with K-splits of at most 32768 (the default) and 16384
+// atoms, and keeps the fastest span in c->sq_tune.  Short splits keep the
+// turns the running workgroups share in L2 but write and read more partial
+// sums.  Longer splits are not tried: a split's compensated FP64 sums lose
+// accuracy with its length on coherent (Bragg-like) vectors.  The choice
+// changes the FP64 summation order only.
+int tune_contraction(trj_ctx* c, int ntiles, int64_t n, int nch, const std::function<void(int)>& run) {
+  auto& st = c->sq_tune;
+  const SqPlan& S = c->sq;
+  std::vector<int> spans, splits;
+  for (int span : {32768, 16384}) {
+    const int ks = contraction_splits(ntiles, n, S.p.precision, nch, span);
+    if (std::find(splits.begin(), splits.end(), ks) != splits.end()) continue;
+    spans.push_back(span);
+    splits.push_back(ks);
+  }
+  st.span = spans[0];
+  st.tried = 0;
+  st.done = true;
+  if (spans.size() < 2) return TRJ_OK;
+  const int kmax = *std::max_element(splits.begin(), splits.end());
+  int r;
+  if ((r = ensure(c, c->rho, rho_entry_bytes(S.p.precision) * static_cast<size_t>(kmax) * nch * ntiles *
+                                 S.tk.vec())))
+    return r;
+  run(splits[0]);  // untimed: code objects and caches warm
+  double best = 1e300;
+  for (size_t i = 0; i < spans.size(); ++i) {
+    HIPC(c, hipEventRecord(c->tune_ev[0], c->stream));
+    run(splits[i]);
+    HIPC(c, hipEventRecord(c->tune_ev[1], c->stream));
+    HIPC(c, hipEventSynchronize(c->tune_ev[1]));
+    float ms = 0.f;
+    HIPC(c, hipEventElapsedTime(&ms, c->tune_ev[0], c->tune_ev[1]));
+    ++st.tried;
+    if (ms < best) {
+      best = ms;
+      st.span = spans[i];
+    }
+  }
+  HIPC(c, hipGetLastError());
+  st.ms = best;
   return TRJ_OK;
 }
 
@@ -753,49 +846,65 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     if (nsl > 0) {
       if ((r = rdf_slices(c, pa, g, box, n, itype_dev, rdf_q ? q_dev : nullptr, rs))) return r;
     } else {
-      // Cell grid: side >= cutoff on every axis.
+      // Cell grid: side >= cutoff (times the autotuner's scale) on every axis.
       auto& tu = c->rdf_tune;
-      RdfCells rcl = rdf_cell_plan(ndim, box, periodic, g.side2, n, tu.done ? tu.W : 0);
-      const CellGrid& cg = rcl.cg;
+      auto plan = [&](double scale) {
+        return rdf_cell_plan(ndim, box, periodic, g.side2, n, tu.done ? tu.W : 0, scale);
+      };
+      RdfCells rcl = plan(tu.done ? tu.scale : (c->rdf_fixed_scale > 0.0 ? c->rdf_fixed_scale : 1.0));
       const bool cells_ok = rcl.cells_ok;
-      const int64_t ncell = rcl.ncell;
       for (int k = 0; k < 3; ++k) pa.cellshift[k] = rcl.cellshift[k];
       pa.row0 = 0;
       pa.nrows = tu.done ? tu.rows : R.rows;
       int nblocks = rcl.nblocks;
       if (cells_ok) {
-        if ((r = ensure(c, c->cellid, sizeof(int32_t) * n))) return r;
-        if ((r = ensure(c, c->cellcount, sizeof(int32_t) * (ncell + 1)))) return r;
-        if ((r = ensure(c, c->cellstart, sizeof(int32_t) * (ncell + 1)))) return r;
         if ((r = ensure(c, c->spos, sizeof(double) * 3 * n))) return r;
         if ((r = ensure(c, c->stype, sizeof(int32_t) * n))) return r;
         const bool qq_pairs = rdf_q && !qq_species;
         if (qq_pairs && (r = ensure(c, c->qsort, sizeof(double) * n))) return r;
-        const CellSort cs = c->atomic_cells ? CellSort{} : cell_sort_plan(n, ncell, c->lds_max);
-        if (cs.lds) {
-          if ((r = ensure(c, c->cellblk, sizeof(int32_t) * cs.blocks * static_cast<size_t>(ncell)))) return r;
-          launch_cell_sort(cs, P<double>(c->pos), nullptr, 0, 0, n, ndim, cg, ncell, P<int32_t>(c->cellid),
-                           P<int32_t>(c->cellblk), P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), itype_dev,
-                           P<double>(c->spos), P<int32_t>(c->stype), qq_pairs ? q_dev : nullptr,
-                           qq_pairs ? P<double>(c->qsort) : nullptr, rs);
-        } else {
-          if ((r = ensure(c, c->cursor, sizeof(int32_t) * ncell))) return r;
-          HIPC(c, hipMemsetAsync(c->cellcount.p, 0, sizeof(int32_t) * (ncell + 1), rs));
-          HIPC(c, hipMemsetAsync(c->cursor.p, 0, sizeof(int32_t) * ncell, rs));
-          launch_cell_count(P<double>(c->pos), n, ndim, cg, P<int32_t>(c->cellid), P<int32_t>(c->cellcount),
-                            rs);
-          launch_exclusive_scan(P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), ncell + 1, rs);
-          launch_cell_scatter(P<double>(c->pos), itype_dev, P<int32_t>(c->cellid), n, P<int32_t>(c->cellstart),
-                              P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype),
-                              qq_pairs ? q_dev : nullptr, qq_pairs ? P<double>(c->qsort) : nullptr, rs);
-        }
+        // Counting sort of the frame's atoms into the cells of plan p:
+        // cell starts, cell-sorted positions, species (and charges).
+        auto sort_cells = [&](const RdfCells& p) -> int {
+          const int64_t ncell = p.ncell;
+          int rr;
+          if ((rr = ensure(c, c->cellid, sizeof(int32_t) * n))) return rr;
+          if ((rr = ensure(c, c->cellcount, sizeof(int32_t) * (ncell + 1)))) return rr;
+          if ((rr = ensure(c, c->cellstart, sizeof(int32_t) * (ncell + 1)))) return rr;
+          const CellSort cs = c->atomic_cells ? CellSort{} : cell_sort_plan(n, ncell, c->lds_max);
+          if (cs.lds) {
+            if ((rr = ensure(c, c->cellblk, sizeof(int32_t) * cs.blocks * static_cast<size_t>(ncell)))) return rr;
+            launch_cell_sort(cs, P<double>(c->pos), nullptr, 0, 0, n, ndim, p.cg, ncell, P<int32_t>(c->cellid),
+                             P<int32_t>(c->cellblk), P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), itype_dev,
+                             P<double>(c->spos), P<int32_t>(c->stype), qq_pairs ? q_dev : nullptr,
+                             qq_pairs ? P<double>(c->qsort) : nullptr, rs);
+          } else {
+            if ((rr = ensure(c, c->cursor, sizeof(int32_t) * ncell))) return rr;
+            HIPC(c, hipMemsetAsync(c->cellcount.p, 0, sizeof(int32_t) * (ncell + 1), rs));
+            HIPC(c, hipMemsetAsync(c->cursor.p, 0, sizeof(int32_t) * ncell, rs));
+            launch_cell_count(P<double>(c->pos), n, ndim, p.cg, P<int32_t>(c->cellid), P<int32_t>(c->cellcount),
+                              rs);
+            launch_exclusive_scan(P<int32_t>(c->cellcount), P<int32_t>(c->cellstart), ncell + 1, rs);
+            launch_cell_scatter(P<double>(c->pos), itype_dev, P<int32_t>(c->cellid), n, P<int32_t>(c->cellstart),
+                                P<int32_t>(c->cursor), P<double>(c->spos), P<int32_t>(c->stype),
+                                qq_pairs ? q_dev : nullptr, qq_pairs ? P<double>(c->qsort) : nullptr, rs);
+          }
+          HIPC(c, hipGetLastError());
+          return TRJ_OK;
+        };
+        if ((r = sort_cells(rcl))) return r;
         if (qq_pairs) pa.q = P<double>(c->qsort);
         if (!tu.done && tu.enabled && !c->noalloc && !qq_pairs) {
-          if ((r = tune_pairs(c, pa, cg, rcl, n, rs))) return r;
-          rcl = rdf_cell_plan(ndim, box, periodic, g.side2, n, tu.W);
+          if ((r = tune_pairs(c, pa, rcl, n, rs, plan, sort_cells))) return r;
+          rcl = plan(tu.scale);
+          // the trials leave the cells of the last grid tried
+          if ((r = sort_cells(rcl))) return r;
+          for (int k = 0; k < 3; ++k) pa.cellshift[k] = rcl.cellshift[k];
           pa.nrows = tu.rows;
           nblocks = rcl.nblocks;
         }
+        const CellGrid& cg = rcl.cg;
+        for (int k = 0; k < 3; ++k) c->rdf_nc[k] = cg.nc[k];
+        c->rdf_cell_scale = rcl.scale;
         const int W = rcl.W;
         if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * static_cast<size_t>(nblocks)))) return r;
         launch_rdf_cells(pa, cg, P<int32_t>(c->cellstart), P<double>(c->spos), P<int32_t>(c->stype), n, W,
@@ -827,7 +936,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
         sa.pos = cells_ok ? P<double>(c->spos) : P<double>(c->pos);
         sa.n = n;
         sa.start = cells_ok ? P<int32_t>(c->cellstart) : nullptr;
-        sa.g = cg;
+        sa.g = rcl.cg;
         sa.gi = P<unsigned long long>(c->s2gi);
         sa.g2i = P<unsigned long long>(c->s2g2i);
         launch_s2n(sa, rs);
@@ -851,22 +960,43 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     const int nch = nkeys + ngroups;  // (slice,) species channels, then charge channels
     c->sq_nch = nch;
     const int sdim = nsl > 0 ? 2 : ndim;
+    // With slices every vector's values are divided by the slice's (species')
+    // atom count (sqf3xy / sqf2xy, sq.cuf:653-670, 741-765).
+    const int32_t* norm = (nsl > 0 || c->sq_centres) ? P<int32_t>(c->sp_off) : nullptr;
+    const int ngs = nsl > 0 ? nsl : 1;
     if (ntiles > 0) {
-      const int ks = contraction_splits(ntiles, n, S.p.precision, nch);
-      c->ksplit = ks;
-      const size_t rho_bytes = sizeof(double) * 2 * static_cast<size_t>(ks) * nch * ntiles * S.tk.vec();
-      if ((r = ensure(c, c->rho, rho_bytes))) return r;
       ContractArgs ca{};
       ca.u = P<double>(c->u);
       ca.w = sq_ch ? P<double>(c->u) + 12 * ustride : nullptr;
       ca.sp_off = P<int32_t>(c->sp_off);
       ca.tiles = P<int32_t>(c->tiles);
       ca.ntiles = ntiles;
-      ca.ksplit = ks;
       ca.nsp = nsp;
       ca.nch = nch;
       ca.ndim = sdim;
       ca.n = ustride;
+      auto& st = c->sq_tune;
+      if (S.p.precision == TRJ_PREC_FP64 && st.enabled && !st.done && !c->noalloc) {
+        // trials: the contraction and a shell pass into scratch sums
+        const size_t ns = static_cast<size_t>(S.nqmax + 1) * ngs;
+        if ((r = ensure(c, c->tunebuf, sizeof(double) * ns * (2 + nsp)))) return r;
+        double* tsqf = P<double>(c->tunebuf);
+        auto run = [&](int ks) {
+          ca.ksplit = ks;
+          ca.rho = P<double>(c->rho);
+          launch_sq_contract_f64(ca, c->stream);
+          launch_sq_shells(c->rho.p, false, P<int16_t>(c->tile_shell), ntiles, S.tk.vec(), ks, nsp, ngs, nch,
+                           S.p.bsc, S.nqmax, norm, sq_q ? qspecies : nullptr, tsqf, tsqf + ns, tsqf + ns * (1 + nsp),
+                           c->stream);
+        };
+        if ((r = tune_contraction(c, ntiles, n, nch, run))) return r;
+      }
+      const int ks = contraction_splits(ntiles, n, S.p.precision, nch, st.done ? st.span : 0);
+      c->ksplit = ks;
+      const size_t rho_bytes =
+          rho_entry_bytes(S.p.precision) * static_cast<size_t>(ks) * nch * ntiles * S.tk.vec();
+      if ((r = ensure(c, c->rho, rho_bytes))) return r;
+      ca.ksplit = ks;
       ca.rho = P<double>(c->rho);
       if (S.p.precision == TRJ_PREC_FP64)
         launch_sq_contract_f64(ca, c->stream);
@@ -875,12 +1005,9 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
     }
     HIPC(c, hipGetLastError());
     HIPC(c, hipEventRecord(es.ev[3], c->stream));
-    // With slices every vector's values are divided by the slice's (species')
-    // atom count (sqf3xy / sqf2xy, sq.cuf:653-670, 741-765).
-    const int32_t* norm = (nsl > 0 || c->sq_centres) ? P<int32_t>(c->sp_off) : nullptr;
-    const int ngs = nsl > 0 ? nsl : 1;
     if (ntiles > 0)
-      launch_sq_shells(P<double>(c->rho), P<int16_t>(c->tile_shell), ntiles, S.tk.vec(), c->ksplit, nsp, ngs, nch,
+      launch_sq_shells(c->rho.p, S.p.precision != TRJ_PREC_FP64, P<int16_t>(c->tile_shell), ntiles, S.tk.vec(),
+                       c->ksplit, nsp, ngs, nch,
                        S.p.bsc, S.nqmax, norm, sq_q ? qspecies : nullptr, P<double>(c->sqf), P<double>(c->sqfp),
                        P<double>(c->sqfq), c->stream);
     const int nax = static_cast<int>(S.axis_shell.size());
@@ -923,7 +1050,8 @@ hipStream_t context_stream(trj_ctx* c) { return c->stream; }
 
 // K-splits of the contraction (the splits are summed in FP64 by the shells
 // kernel): at least 2048 workgroups when the atoms allow it; for FP64, atom
-// ranges of at most 32768 so that the workgroups running at one time share
+// ranges of at most fp64_span (0: 32768, the autotuner may pick another) so
+// that the workgroups running at one time share
 // their turns (96 B per atom) in L2, with the partial rho buffer (one per
 // split) kept within 4 GiB; for the FP32-accumulating kernels at most 8192
 // atoms per workgroup (their accuracy bound, not capped).  The 16-bit kernel
@@ -932,11 +1060,11 @@ hipStream_t context_stream(trj_ctx* c) { return c->stream; }
 // rounds of kWgRound = 256, the CU count of the MI355X.  The count, and with it
 // the FP32 accumulation length that sets the 16-bit error, depends on the
 // frame only, never on the CU count the device reports.
-int contraction_splits(int ntiles, int64_t n, int precision, int nsp) {
+int contraction_splits(int ntiles, int64_t n, int precision, int nsp, int64_t fp64_span) {
   constexpr int64_t kWgRound = 256;
   int ks = static_cast<int>((2048 + ntiles - 1) / ntiles);
   ks = std::max(1, std::min<int>(ks, static_cast<int>(std::max<int64_t>(1, n / 8192))));
-  const int64_t span = precision == TRJ_PREC_FP64 ? 32768 : 8192;
+  const int64_t span = precision == TRJ_PREC_FP64 ? (fp64_span > 0 ? fp64_span : 32768) : 8192;
   ks = std::max<int>(ks, static_cast<int>((n + span - 1) / span));
   // Small frames on few tiles: enough splits to fill one round of kWgRound
   // workgroups while each split keeps at least 1024 atoms (32 chunks), e.g. 9
@@ -1143,6 +1271,11 @@ int trj_rdf_init(trj_ctx* c, const trj_rdf_params* p, const double box[3], int32
                     "; increase deltar or reduce rcrdf (rdf.cuf:61-64)");
   c->rdf_tune = {};
   c->rdf_tune.enabled = (p->flags & TRJ_RDF_NO_AUTOTUNE) == 0;
+  c->rdf_fixed_scale = 0.0;
+  if (const char* e = std::getenv("TRJ_RDF_CELL_SCALE")) {
+    const double f = std::atof(e);
+    if (f >= 1.0 && f <= 4.0) c->rdf_fixed_scale = f;
+  }
   const int nsl = c->slices.n;
   if (nsl > 0) {
     if (R.ndim != 3) return fail(c, TRJ_EINVAL, "rdf: slices need ndim = 3");
@@ -1249,6 +1382,8 @@ int trj_sq_init(trj_ctx* c, const trj_sq_params* p, const double box[3], trj_sq_
       S.info.nvec_axis = 0;
     }
   }
+  c->sq_tune = {};
+  c->sq_tune.enabled = (p->flags & TRJ_SQ_NO_AUTOTUNE) == 0;
   const int ns = S.nqmax + 1;
   const int ngs = nsl > 0 ? nsl : 1;
   if ((r = ensure(c, c->sqf, sizeof(double) * ns * ngs))) return r;
@@ -1313,6 +1448,26 @@ int trj_rdf_tuning(trj_ctx* c, int32_t* rows_per_pass, int32_t* passes, int32_t*
   return TRJ_OK;
 }
 
+int trj_rdf_cell_grid(trj_ctx* c, int32_t cells[3], double* side_over_cutoff) {
+  if (!c) return TRJ_EINVAL;
+  if (!c->rdf_ready) return fail(c, TRJ_ESTATE, "trj_rdf_cell_grid before trj_rdf_init");
+  if (cells)
+    for (int k = 0; k < 3; ++k) cells[k] = c->rdf_nc[k];
+  if (side_over_cutoff) *side_over_cutoff = c->rdf_cell_scale;
+  return TRJ_OK;
+}
+
+int trj_sq_tuning(trj_ctx* c, int32_t* splits, int32_t* atoms_per_split, double* ms, int32_t* tried) {
+  if (!c) return TRJ_EINVAL;
+  if (!c->sq_ready) return fail(c, TRJ_ESTATE, "trj_sq_tuning before trj_sq_init");
+  const auto& st = c->sq_tune;
+  if (splits) *splits = c->sq_frames > 0 ? c->ksplit : 0;
+  if (atoms_per_split) *atoms_per_split = st.done ? st.span : 0;
+  if (ms) *ms = st.ms;
+  if (tried) *tried = st.tried;
+  return TRJ_OK;
+}
+
 int trj_frame_device_noalloc(trj_ctx* c, const void* pos_dev, int32_t dtype, int32_t layout,
                              const int32_t* itype_dev, int64_t n, const double box_in[3], uint32_t what) {
   if (!c) return TRJ_EINVAL;
@@ -1334,7 +1489,9 @@ int trj_frame_device_noalloc(trj_ctx* c, const void* pos_dev, int32_t dtype, int
   const double box[3] = {box_in[0], box_in[1], ndim == 3 ? box_in[2] : 1.0};
   double rc = R.rc;
   const FrameGeom g = frame_geometry(ndim, box, R.periodic, &rc);
-  const RdfCells cp = rdf_cell_plan(ndim, box, R.periodic, g.side2, n, c->rdf_tune.done ? c->rdf_tune.W : 0);
+  const RdfCells cp = rdf_cell_plan(ndim, box, R.periodic, g.side2, n, c->rdf_tune.done ? c->rdf_tune.W : 0,
+                                    c->rdf_tune.done ? c->rdf_tune.scale
+                                                     : (c->rdf_fixed_scale > 0.0 ? c->rdf_fixed_scale : 1.0));
   const size_t nblk = static_cast<size_t>((n + 255) / 256);
   const size_t un = static_cast<size_t>(n), nc = static_cast<size_t>(cp.ncell);
   const size_t cells = cp.cells_ok ? 1 : 0;
@@ -1757,10 +1914,13 @@ int trj_sq_rho(trj_ctx* c, int64_t nvec, const int32_t* kv, double* rho) {
   const int nsp = S.p.nsp, ntiles = static_cast<int>(S.tiles.size());
   const int nch = c->sq_nch;  // channels of the last frame's phase sums
   // Many vectors: one copy of the frame's phase sums instead of one per value.
-  std::vector<double> all;
+  // The partials are FP64 complex (FP64 contraction) or FP32 complex.
+  const bool f32 = S.p.precision != TRJ_PREC_FP64;
+  const size_t esz = f32 ? sizeof(float) : sizeof(double);
+  std::vector<unsigned char> all;
   if (nvec > 16) {
-    all.resize(static_cast<size_t>(2) * c->ksplit * nch * ntiles * S.tk.vec());
-    HIPC(c, hipMemcpy(all.data(), c->rho.p, sizeof(double) * all.size(), hipMemcpyDeviceToHost));
+    all.resize(esz * 2 * c->ksplit * nch * ntiles * S.tk.vec());
+    HIPC(c, hipMemcpy(all.data(), c->rho.p, all.size(), hipMemcpyDeviceToHost));
   }
   for (int64_t v = 0; v < nvec; ++v) {
     const int x = kv[3 * v], y = kv[3 * v + 1], z = kv[3 * v + 2];
@@ -1777,21 +1937,34 @@ int trj_sq_rho(trj_ctx* c, int64_t nvec, const int32_t* kv, double* rho) {
     if (S.tile_shell[static_cast<size_t>(tile) * TILE_VEC + vi] < 0)
       return fail(c, TRJ_EINVAL, "vector not in the dense q-set");
     for (int s = 0; s < nsp; ++s) {
-      double acc[2] = {0.0, 0.0};
+      // Neumaier's compensated sum over the splits, as the shells kernels add them
+      double acc[2] = {0.0, 0.0}, cmp[2] = {0.0, 0.0};
       for (int sp = 0; sp < c->ksplit; ++sp) {
-        double val[2];
+        unsigned char raw[2 * sizeof(double)];
         const size_t idx = (((static_cast<size_t>(sp) * nch + s) * ntiles + tile) * TILE_VEC + vi) * 2;
         if (!all.empty()) {
-          val[0] = all[idx];
-          val[1] = all[idx + 1];
+          std::memcpy(raw, all.data() + idx * esz, 2 * esz);
         } else {
-          HIPC(c, hipMemcpy(val, static_cast<double*>(c->rho.p) + idx, sizeof(val), hipMemcpyDeviceToHost));
+          HIPC(c, hipMemcpy(raw, static_cast<const unsigned char*>(c->rho.p) + idx * esz, 2 * esz,
+                            hipMemcpyDeviceToHost));
         }
-        acc[0] += val[0];
-        acc[1] += val[1];
+        double val[2];
+        if (f32) {
+          float fv[2];
+          std::memcpy(fv, raw, sizeof(fv));
+          val[0] = fv[0];
+          val[1] = fv[1];
+        } else {
+          std::memcpy(val, raw, sizeof(val));
+        }
+        for (int k = 0; k < 2; ++k) {
+          const double t = acc[k] + val[k];
+          cmp[k] += (std::fabs(acc[k]) >= std::fabs(val[k])) ? (acc[k] - t) + val[k] : (val[k] - t) + acc[k];
+          acc[k] = t;
+        }
       }
-      rho[(v * nsp + s) * 2] = acc[0];
-      rho[(v * nsp + s) * 2 + 1] = acc[1];
+      rho[(v * nsp + s) * 2] = acc[0] + cmp[0];
+      rho[(v * nsp + s) * 2 + 1] = acc[1] + cmp[1];
     }
   }
   return TRJ_OK;

@@ -91,7 +91,7 @@ struct SqPlan {
 int sq_plan(const trj_sq_params& p, const double box[3], SqPlan* out, std::string* err);
 
 // K-splits of the S(Q) contraction for ntiles tiles and n atoms.
-int contraction_splits(int ntiles, int64_t n, int precision, int nsp);
+int contraction_splits(int ntiles, int64_t n, int precision, int nsp, int64_t fp64_span = 0);
 
 // Frees an idle context's device buffers and leaves it uninitialised
 // (the request server's way of returning memory to its HBM budget).

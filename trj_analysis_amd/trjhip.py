@@ -23,7 +23,7 @@ TRJ_AOS, TRJ_SOA = 0, 1
 TRJ_WANT_RDF, TRJ_WANT_SQ, TRJ_WANT_FQT = 1, 2, 4
 TRJ_QSET_REF, TRJ_QSET_HALFSPACE = 0, 1
 TRJ_PREC_FP64, TRJ_PREC_FP32, TRJ_PREC_FP16X3, TRJ_PREC_BF16X3 = 0, 1, 2, 3
-TRJ_SQ_NQ_CONSISTENT, TRJ_SQ_CHARGE, TRJ_SQ_CENTRES = 1, 2, 4
+TRJ_SQ_NQ_CONSISTENT, TRJ_SQ_CHARGE, TRJ_SQ_CENTRES, TRJ_SQ_NO_AUTOTUNE = 1, 2, 4, 8
 TRJ_RDF_CHARGE, TRJ_RDF_FLOOR_BINS, TRJ_RDF_NO_AUTOTUNE = 1, 2, 4
 TRJ_PIPE_SERIAL, TRJ_PIPE_OVERLAP = 0, 1
 TRJ_STATE_REPLACE, TRJ_STATE_ADD = 0, 1
@@ -34,7 +34,7 @@ PRECISIONS = {"fp64": TRJ_PREC_FP64, "fp32": TRJ_PREC_FP32, "fp16x3": TRJ_PREC_F
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTS = [
     "trj_abi_version", "trj_create", "trj_destroy", "trj_last_error", "trj_rdf_init",
-    "trj_sq_init", "trj_sq_plan", "trj_rdf_plan", "trj_frame", "trj_frame_device", "trj_frame_device_q", "trj_frame_device_noalloc", "trj_rdf_tuning", "trj_sync",
+    "trj_sq_init", "trj_sq_plan", "trj_rdf_plan", "trj_frame", "trj_frame_device", "trj_frame_device_q", "trj_frame_device_noalloc", "trj_rdf_tuning", "trj_rdf_cell_grid", "trj_sq_tuning", "trj_sync",
     "trj_rdf_counts", "trj_rdf_gmix", "trj_rdf_first_minimum", "trj_sq_shells", "trj_sq_columns",
     "trj_sq_rho", "trj_fqt_init", "trj_fqt_results", "trj_rdf_sampling", "trj_rdf_sampling_points", "trj_rdf_s2n_sums", "trj_rdf_s2n",
     "trj_set_slices", "trj_slice_counts", "trj_rdf_xy", "trj_rdf_xy_gmix", "trj_rdf_xy_qq",
@@ -151,6 +151,8 @@ def _declare(L):
     L.trj_frame_device_q.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, vp, C.c_int64, d, u32]
     L.trj_frame_device_noalloc.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, C.c_int64, d, u32]
     L.trj_rdf_tuning.argtypes = [vp, i32, i32, i32, d, i32]
+    L.trj_rdf_cell_grid.argtypes = [vp, i32, d]
+    L.trj_sq_tuning.argtypes = [vp, i32, i32, d, i32]
     L.trj_sync.argtypes = [vp]
     L.trj_rdf_counts.argtypes = [vp, i64]
     L.trj_rdf_gmix.argtypes = [vp, C.c_int64, d]
@@ -513,6 +515,22 @@ class Context:
                     "trj_rdf_tuning")
         return {"rows_per_pass": r.value, "passes": p.value, "waves_per_cell": w.value, "ms": ms.value,
                 "tried": t.value}
+
+    def rdf_cell_grid(self) -> dict:
+        """Cells per axis of the last RDF frame and the cell side over the
+        cutoff (trj_rdf_cell_grid)."""
+        nc = (C.c_int32 * 3)()
+        f = C.c_double()
+        self._check(self._L.trj_rdf_cell_grid(self._h, nc, C.byref(f)), "trj_rdf_cell_grid")
+        return {"cells": [nc[0], nc[1], nc[2]], "side_over_cutoff": f.value}
+
+    def sq_tuning(self) -> dict:
+        """The S(Q) autotuner's choice (trj_sq_tuning)."""
+        k, a, t = C.c_int32(), C.c_int32(), C.c_int32()
+        ms = C.c_double()
+        self._check(self._L.trj_sq_tuning(self._h, C.byref(k), C.byref(a), C.byref(ms), C.byref(t)),
+                    "trj_sq_tuning")
+        return {"splits": k.value, "atoms_per_split": a.value, "ms": ms.value, "tried": t.value}
 
     def frame_device_noalloc(self, pos_ptr: int, dtype: int, layout: int, itype_ptr: int, n: int, box):
         """RDF of a device-resident frame without any allocation
