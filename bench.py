@@ -388,8 +388,11 @@ class FrameRun:
         # species channel and in the charge channel.
         n_entries = 2 * self.n if self.charges == "atoms" else self.n
         flops = 8.0 * n_entries * self.info.nvec_grid
+        # the autotuners' choices on the first frame (cell side, LDS tile and
+        # waves per cell of the RDF; K-split and workgroup order of the S(Q))
+        autotune = {"rdf": ctx.rdf_tuning(), "rdf_cells": ctx.rdf_cell_grid(), "sq": ctx.sq_tuning()}
         return {"steps": steps, "elapsed": elapsed, "fps": steps / elapsed, "pairs": pairs,
-                "kernel_ms": kernel_ms, "lat": lat, "flops": flops, "n_entries": n_entries}
+                "kernel_ms": kernel_ms, "lat": lat, "flops": flops, "n_entries": n_entries, "autotune": autotune}
 
     def close(self):
         self.ctx.close()
@@ -429,7 +432,8 @@ def bench_north_star(steps: int, warmup: int) -> dict:
         fr = FrameRun(wl, frames, prec, pipe)
         r = fr.timed(steps, warmup)
         fr.close()
-        runs[name] = {"frames_per_s": r["fps"], "ms_per_frame": 1e3 / r["fps"], "kernel_ms": r["kernel_ms"]}
+        runs[name] = {"frames_per_s": r["fps"], "ms_per_frame": 1e3 / r["fps"], "kernel_ms": r["kernel_ms"],
+                      "autotune": r["autotune"]}
         if pipe == "serial":
             ach = r["flops"] / (r["kernel_ms"]["sq_contract"] * 1e-3)
             runs[name]["contraction"] = {"kernel": contraction_kernel(prec), "achieved": ach / 1e12,
@@ -624,6 +628,7 @@ def main():
         "latency_ms_p50": float(np.percentile(lat, 50)) if lat else None,
         "latency_ms_p99": float(np.percentile(lat, 99)) if lat else None,
         "kernel_ms": kernel_ms,
+        "autotune": r["autotune"],
         "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved / 1e12,
                      "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                      "algorithmic": f"8*N*Nq = {flops:.4e} flop per launch (N={r['n_entries']} entries, "

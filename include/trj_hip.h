@@ -79,9 +79,9 @@ extern "C" {
                                    divided by the frame's point count, octant set only
                                    (no axis vectors); sqcl.dat is sqf(i)/(Nconf nq(i)),
                                    i = 1..nqmin (log.f90:637-642) */
-#define TRJ_SQ_NO_AUTOTUNE 8u   /* keep the default K-split of the FP64 contraction
-                                   instead of timing candidate split lengths on the
-                                   first frame (trj_sq_tuning) */
+#define TRJ_SQ_NO_AUTOTUNE 8u   /* keep the default K-split and workgroup order of
+                                   the FP64 contraction instead of timing candidates
+                                   on the first frame (trj_sq_tuning) */
 
 /* RDF flags. */
 #define TRJ_RDF_CHARGE 1u /* also accumulate the charge-weighted histogram g_qq
@@ -292,11 +292,15 @@ int trj_sq_rho(trj_ctx* ctx, int64_t nvec, const int32_t* kv, double* rho);
 /* The S(Q) autotuner's choice (no counterpart in the reference, whose sqfnD
  * launch shape is fixed, sq.cuf:175-179): K-splits of the last frame's
  * contraction, the largest atoms per split the FP64 autotuner chose (0 for
- * the FP32-accumulating precisions, whose split length their accuracy
- * bound sets), that split's time in ms on the tuning frame and the number of
- * split lengths timed (0 before the first frame or with TRJ_SQ_NO_AUTOTUNE).
- * Any argument may be NULL. */
-int trj_sq_tuning(trj_ctx* ctx, int32_t* splits, int32_t* atoms_per_split, double* ms, int32_t* tried);
+ * the FP32-accumulating precisions, whose split length their accuracy bound
+ * sets, and without the autotuner), whether the contraction's workgroups run
+ * XCD-grouped (the tiles of one split on one XCD; always for the FP32 and
+ * 16-bit kernels, the autotuner's choice for FP64), the chosen shape's time
+ * in ms (contraction and shell pass) on the tuning frame and the number of
+ * shapes timed (0 before the first frame or with TRJ_SQ_NO_AUTOTUNE).  Any
+ * argument may be NULL. */
+int trj_sq_tuning(trj_ctx* ctx, int32_t* splits, int32_t* atoms_per_split, int32_t* xcd_grouped, double* ms,
+                  int32_t* tried);
 
 /* rdf_clear / sq_clear (rdf.cuf:124, sq.cuf:774): zero the accumulators and
  * frame counters, keep the plans. */

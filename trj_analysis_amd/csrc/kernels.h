@@ -244,6 +244,11 @@ struct ContractArgs {
   const int4* items;
   const int32_t* item_plane;
   const int32_t* nitems;
+  // Workgroup order: 0 in (split, tile) order; 1 XCD-grouped, the tiles of
+  // one split on one XCD (its L2 shares the split's turns).  The FP32 and
+  // 16-bit kernels always take the grouped order; the FP64 kernel takes this
+  // field (the autotuner's choice).
+  int xcd_order;
 };
 void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s);
 // Bytes of one (split, channel, vector) partial of rho: FP64 complex from the

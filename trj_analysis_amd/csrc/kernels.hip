@@ -1530,6 +1530,23 @@ __device__ inline ItemRange item_range(const ContractArgs& a, int item) {
   return r;
 }
 
+// Work index (item * ntiles + tile) of this workgroup in the contraction
+// kernels.  Grouped: the dispatcher deals workgroups to the 8 XCDs in turn
+// (workgroup b to XCD b % 8); workgroup b = 8 r + x takes index x * per + r,
+// per = ceil(nwork / 8), so each XCD walks a contiguous range of (split,
+// tile) pairs in order: the tiles of one split run on one XCD at about the
+// same time and read the split's turns through that XCD's L2 instead of each
+// XCD reading them from HBM (measured: 8x fewer HBM reads for the 16-bit
+// kernel on a 1M-atom frame, 6x for the FP64 one on 262k atoms).  Otherwise
+// index b.  The grid is 8 * per workgroups; indices >= nwork return at once.
+// Both mappings are bijections whatever the dispatch order.
+__device__ inline int64_t contract_work_index(const ContractArgs& a, bool grouped) {
+  if (!grouped) return blockIdx.x;
+  const int64_t nwork = static_cast<int64_t>(a.ntiles) * a.ksplit;
+  const int64_t per = (nwork + 7) / 8;
+  return static_cast<int64_t>(blockIdx.x % 8) * per + blockIdx.x / 8;
+}
+
 // rho_s(q) for one tile and one atom split: a complex GEMM over atoms whose
 // operands are generated per 32-atom chunk in LDS (double-buffered: the next
 // chunk is staged while this one is contracted).  Waves 0-2 stage one
@@ -1538,8 +1555,10 @@ __device__ inline ItemRange item_range(const ContractArgs& a, int item) {
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_sq_contract_f64(
     ContractArgs a) {
   __shared__ double2 ph[2][KC][kRec];
-  const int tile = blockIdx.x % a.ntiles;
-  const ItemRange wi = item_range(a, blockIdx.x / a.ntiles);
+  const int64_t wk = contract_work_index(a, a.xcd_order != 0);
+  if (wk >= static_cast<int64_t>(a.ntiles) * a.ksplit) return;
+  const int tile = static_cast<int>(wk % a.ntiles);
+  const ItemRange wi = item_range(a, static_cast<int>(wk / a.ntiles));
   if (wi.skip) return;
   const int kx0 = a.tiles[4 * tile], ky0 = a.tiles[4 * tile + 1], kz0 = a.tiles[4 * tile + 2];
   const int t = threadIdx.x, w = t / kWave, lane = t & (kWave - 1);
@@ -1714,8 +1733,10 @@ struct LpShared32 {
 
 __global__ __launch_bounds__(256) void k_sq_contract_f32(ContractArgs a) {
   __shared__ LpShared32 sh;
-  const int tile = blockIdx.x % a.ntiles;
-  const ItemRange wi = item_range(a, blockIdx.x / a.ntiles);
+  const int64_t wk = contract_work_index(a, true);
+  if (wk >= static_cast<int64_t>(a.ntiles) * a.ksplit) return;
+  const int tile = static_cast<int>(wk % a.ntiles);
+  const ItemRange wi = item_range(a, static_cast<int>(wk / a.ntiles));
   if (wi.skip) return;
   const int kx0 = a.tiles[4 * tile], ky0 = a.tiles[4 * tile + 1], kz0 = a.tiles[4 * tile + 2];
   const int t = threadIdx.x, w = t / kWave, lane = t & (kWave - 1);
@@ -1956,8 +1977,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
   typedef typename LT::V V;
   __shared__ __attribute__((aligned(16))) float2 rec[3][WROWS][kRS];  // x 0..63, z 64..95, y 96..103
   __shared__ __attribute__((aligned(16))) V As[2][4][LKC / 8][WX];     // re_hi, re_lo, im_hi, im_lo of ex
-  const int tile = blockIdx.x % a.ntiles;
-  const ItemRange wi = item_range(a, blockIdx.x / a.ntiles);
+  const int64_t wk = contract_work_index(a, true);
+  if (wk >= static_cast<int64_t>(a.ntiles) * a.ksplit) return;
+  const int tile = static_cast<int>(wk % a.ntiles);
+  const ItemRange wi = item_range(a, static_cast<int>(wk / a.ntiles));
   if (wi.skip) return;
   const int kx0 = a.tiles[4 * tile], ky0 = a.tiles[4 * tile + 1], kz0 = a.tiles[4 * tile + 2];
   const int t = threadIdx.x, w = t / kWave, lane = t & (kWave - 1);
@@ -3232,14 +3255,21 @@ void launch_rdf_merge(const uint32_t* partial, int nslices, int nbins, int64_t* 
                      nbins, counts, w, lsmax > 0 ? lsmax : 1, qw ? gq : nullptr);
 }
 
+// Grid of the contraction kernels: 8 * ceil(ntiles * ksplit / 8) workgroups
+// (contract_work_index).
+static dim3 contract_grid(const ContractArgs& a) {
+  const int64_t nwork = static_cast<int64_t>(a.ntiles) * a.ksplit;
+  return dim3(static_cast<unsigned>(8 * ((nwork + 7) / 8)));
+}
+
 void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s) {
   if (a.ntiles <= 0) return;
-  hipLaunchKernelGGL(k_sq_contract_f64, dim3(a.ntiles * a.ksplit), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_sq_contract_f64, contract_grid(a), dim3(256), 0, s, a);
 }
 
 void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s) {
   if (a.ntiles <= 0) return;
-  const dim3 grid(a.ntiles * a.ksplit), block(256);
+  const dim3 grid = contract_grid(a), block(256);
   if (precision == TRJ_PREC_FP32)
     hipLaunchKernelGGL(k_sq_contract_f32, grid, block, 0, s, a);
   else if (precision == TRJ_PREC_FP16X3)
@@ -3260,10 +3290,11 @@ void launch_sq_shells(const void* rho, bool rho_f32, const int16_t* tile_shell, 
   const size_t lds = static_cast<size_t>(nacc) * (nqmax + 1) * sizeof(double);
   const int use_lds = lds <= 48 * 1024;
   const int64_t total = static_cast<int64_t>(ntiles) * tile_vec;
-  // Splits over four wavefronts when the channel sums fit in LDS beside the
-  // accumulators (every case but very many slices).
+  // Splits over four wavefronts when they are many (one thread per vector is
+  // then bound by its chain of loads; with few, the barriers cost more) and
+  // the channel sums fit in LDS beside the accumulators.
   const size_t lds4 = (use_lds ? ((lds + 15) & ~size_t(15)) : 0) + sizeof(double) * 2 * kWave * (8 + nch);
-  if (ksplit >= 4 && lds4 <= 64 * 1024) {
+  if (ksplit >= 64 && lds4 <= 64 * 1024) {
     int64_t g4 = (total + kWave - 1) / kWave;
     if (g4 > 2048) g4 = 2048;
     if (rho_f32)

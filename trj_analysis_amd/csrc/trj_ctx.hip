@@ -149,7 +149,7 @@ struct trj_ctx {
   // their accuracy bound sets.
   struct {
     bool enabled = true, done = false;
-    int span = 0;
+    int span = 0, order = 0;
     double ms = 0.0;
     int tried = 0;
   } sq_tune;
@@ -474,6 +474,10 @@ int fqt_step(trj_ctx* c, int64_t n, const int32_t* itype_dev) {
 // a periodic axis with fewer than 3 cells sends the frame to the all-pairs
 // kernel.  The image shift replaces the per-pair minimum image on periodic
 // axes with at least 5 cells.  W wavefronts share a home cell.
+// Smallest relative gain for which an autotuner leaves its default for a
+// candidate that costs more memory traffic or work per pair.
+constexpr double kTuneGain = 0.03;
+
 struct RdfCells {
   CellGrid cg{};
   bool cells_ok = true;
@@ -544,9 +548,11 @@ int rows_fitting(int nit, int lsmax, bool qq, size_t budget) {
 // cell, one fewer and one more, half and twice.  Then, with the best tile,
 // cells of side >= 1.25 and 1.5 cutoffs (sort(p) sorts the frame into plan
 // p's cells): fewer, fuller cells, more candidates per in-range pair; they
-// win where cells of one cutoff hold few atoms.  Every candidate counts the
-// same pairs, so the choice changes speed only.  The trials leave the cells
-// of the last grid tried; the caller sorts again into the chosen one.
+// win where cells of one cutoff hold few atoms, and are kept only when at
+// least kTuneGain faster than the best shape on cells of one cutoff.  Every
+// candidate counts the same pairs, so the choice changes speed only.  The
+// trials leave the cells of the last grid tried; the caller sorts again into
+// the chosen one.
 int tune_pairs(trj_ctx* c, const PairArgs& pa0, const RdfCells& base, int64_t n, hipStream_t rs,
                const std::function<RdfCells(double)>& plan, const std::function<int(const RdfCells&)>& sort) {
   auto& tu = c->rdf_tune;
@@ -576,6 +582,10 @@ int tune_pairs(trj_ctx* c, const PairArgs& pa0, const RdfCells& base, int64_t n,
   uint32_t* part = P<uint32_t>(c->partial);
   PairArgs pa = pa0;
   double best = 1e300;
+  // A grid other than the cells of one cutoff is kept only when it is at
+  // least kTuneGain faster (timing noise must not trade a known grid for a
+  // coarser one).
+  double need = 1.0;
   auto trial = [&](const RdfCells& p, int rw, int W, bool timed) -> int {
     const int nb = static_cast<int>(std::min<int64_t>((p.ncell * W + kCellWaves - 1) / kCellWaves, 2048));
     pa.nrows = rw;
@@ -592,8 +602,8 @@ int tune_pairs(trj_ctx* c, const PairArgs& pa0, const RdfCells& base, int64_t n,
     float ms = 0.f;
     HIPC(c, hipEventElapsedTime(&ms, c->tune_ev[0], c->tune_ev[1]));
     ++tu.tried;
-    if (ms < best) {
-      best = ms;
+    if (ms * need < best) {
+      best = ms * need;
       tu.rows = rw;
       tu.W = W;
       tu.scale = p.scale;
@@ -606,6 +616,7 @@ int tune_pairs(trj_ctx* c, const PairArgs& pa0, const RdfCells& base, int64_t n,
     for (int W : waves(base.W))
       if ((r = trial(base, rw, W, true))) return r;
   const int rbest = tu.rows;
+  need = 1.0 + kTuneGain;
   for (double f : {1.25, 1.5}) {
     if (c->rdf_fixed_scale > 0.0) break;
     const RdfCells p = plan(f);
@@ -614,56 +625,65 @@ int tune_pairs(trj_ctx* c, const PairArgs& pa0, const RdfCells& base, int64_t n,
     for (int W : waves(p.W))
       if ((r = trial(p, rbest, W, true))) return r;
   }
-  tu.ms = best;
+  tu.ms = tu.scale == base.scale ? best : best / need;
   tu.done = true;
   return TRJ_OK;
 }
 
 // The S(Q) autotuner (FP64 contraction): on the first S(Q) frame after
 // trj_sq_init, times the contraction of this frame's turns and the shell pass
-// that reads its partial sums (run(ks) launches both for ks splits, the shells
-// into scratch sums) with K-splits of at most 32768 (the default) and 16384
-// atoms, and keeps the fastest span in c->sq_tune.  Short splits keep the
-// turns the running workgroups share in L2 but write and read more partial
-// sums.  Longer splits are not tried: a split's compensated FP64 sums lose
-// accuracy with its length on coherent (Bragg-like) vectors.  The choice
-// changes the FP64 summation order only.
-int tune_contraction(trj_ctx* c, int ntiles, int64_t n, int nch, const std::function<void(int)>& run) {
+// that reads its partial sums (run(ks, order) launches both for ks splits in
+// that workgroup order, the shells into scratch sums) and keeps the fastest
+// (span, order) in c->sq_tune.  Candidates: K-splits of at most 32768 (the
+// default) and 16384 atoms, each in (split, tile) order and XCD-grouped
+// (contract_work_index).  Short splits keep the turns the running workgroups
+// share in L2 but write and read more partial sums, so a 16384 span is kept
+// only when at least kTuneGain faster.  Longer splits are not tried: a
+// split's compensated FP64 sums lose accuracy with its length on coherent
+// (Bragg-like) vectors.  The choice changes the FP64 summation order only.
+int tune_contraction(trj_ctx* c, int ntiles, int64_t n, int nch, const std::function<void(int, int)>& run) {
   auto& st = c->sq_tune;
   const SqPlan& S = c->sq;
-  std::vector<int> spans, splits;
+  struct Cand {
+    int span, ks, order;
+  };
+  std::vector<Cand> cand;
+  std::vector<int> splits;
   for (int span : {32768, 16384}) {
     const int ks = contraction_splits(ntiles, n, S.p.precision, nch, span);
     if (std::find(splits.begin(), splits.end(), ks) != splits.end()) continue;
-    spans.push_back(span);
     splits.push_back(ks);
+    for (int order : {0, 1}) cand.push_back({span, ks, order});
   }
-  st.span = spans[0];
+  st.span = cand[0].span;
+  st.order = 0;
   st.tried = 0;
   st.done = true;
-  if (spans.size() < 2) return TRJ_OK;
   const int kmax = *std::max_element(splits.begin(), splits.end());
   int r;
   if ((r = ensure(c, c->rho, rho_entry_bytes(S.p.precision) * static_cast<size_t>(kmax) * nch * ntiles *
                                  S.tk.vec())))
     return r;
-  run(splits[0]);  // untimed: code objects and caches warm
-  double best = 1e300;
-  for (size_t i = 0; i < spans.size(); ++i) {
+  run(cand[0].ks, 0);  // untimed: code objects and caches warm
+  double best = 1e300, best_ms = 0.0;
+  for (const Cand& k : cand) {
     HIPC(c, hipEventRecord(c->tune_ev[0], c->stream));
-    run(splits[i]);
+    run(k.ks, k.order);
     HIPC(c, hipEventRecord(c->tune_ev[1], c->stream));
     HIPC(c, hipEventSynchronize(c->tune_ev[1]));
     float ms = 0.f;
     HIPC(c, hipEventElapsedTime(&ms, c->tune_ev[0], c->tune_ev[1]));
     ++st.tried;
-    if (ms < best) {
-      best = ms;
-      st.span = spans[i];
+    const double score = k.span == cand[0].span ? ms : ms * (1.0 + kTuneGain);
+    if (score < best) {
+      best = score;
+      best_ms = ms;
+      st.span = k.span;
+      st.order = k.order;
     }
   }
   HIPC(c, hipGetLastError());
-  st.ms = best;
+  st.ms = best_ms;
   return TRJ_OK;
 }
 
@@ -981,8 +1001,9 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
         const size_t ns = static_cast<size_t>(S.nqmax + 1) * ngs;
         if ((r = ensure(c, c->tunebuf, sizeof(double) * ns * (2 + nsp)))) return r;
         double* tsqf = P<double>(c->tunebuf);
-        auto run = [&](int ks) {
+        auto run = [&](int ks, int order) {
           ca.ksplit = ks;
+          ca.xcd_order = order;
           ca.rho = P<double>(c->rho);
           launch_sq_contract_f64(ca, c->stream);
           launch_sq_shells(c->rho.p, false, P<int16_t>(c->tile_shell), ntiles, S.tk.vec(), ks, nsp, ngs, nch,
@@ -997,6 +1018,7 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
           rho_entry_bytes(S.p.precision) * static_cast<size_t>(ks) * nch * ntiles * S.tk.vec();
       if ((r = ensure(c, c->rho, rho_bytes))) return r;
       ca.ksplit = ks;
+      ca.xcd_order = st.done ? st.order : 0;
       ca.rho = P<double>(c->rho);
       if (S.p.precision == TRJ_PREC_FP64)
         launch_sq_contract_f64(ca, c->stream);
@@ -1457,12 +1479,14 @@ int trj_rdf_cell_grid(trj_ctx* c, int32_t cells[3], double* side_over_cutoff) {
   return TRJ_OK;
 }
 
-int trj_sq_tuning(trj_ctx* c, int32_t* splits, int32_t* atoms_per_split, double* ms, int32_t* tried) {
+int trj_sq_tuning(trj_ctx* c, int32_t* splits, int32_t* atoms_per_split, int32_t* xcd_grouped, double* ms,
+                  int32_t* tried) {
   if (!c) return TRJ_EINVAL;
   if (!c->sq_ready) return fail(c, TRJ_ESTATE, "trj_sq_tuning before trj_sq_init");
   const auto& st = c->sq_tune;
   if (splits) *splits = c->sq_frames > 0 ? c->ksplit : 0;
   if (atoms_per_split) *atoms_per_split = st.done ? st.span : 0;
+  if (xcd_grouped) *xcd_grouped = c->sq.p.precision == TRJ_PREC_FP64 ? (st.done ? st.order : 0) : 1;
   if (ms) *ms = st.ms;
   if (tried) *tried = st.tried;
   return TRJ_OK;

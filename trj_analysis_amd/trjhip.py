@@ -152,7 +152,7 @@ def _declare(L):
     L.trj_frame_device_noalloc.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, C.c_int64, d, u32]
     L.trj_rdf_tuning.argtypes = [vp, i32, i32, i32, d, i32]
     L.trj_rdf_cell_grid.argtypes = [vp, i32, d]
-    L.trj_sq_tuning.argtypes = [vp, i32, i32, d, i32]
+    L.trj_sq_tuning.argtypes = [vp, i32, i32, i32, d, i32]
     L.trj_sync.argtypes = [vp]
     L.trj_rdf_counts.argtypes = [vp, i64]
     L.trj_rdf_gmix.argtypes = [vp, C.c_int64, d]
@@ -526,11 +526,12 @@ class Context:
 
     def sq_tuning(self) -> dict:
         """The S(Q) autotuner's choice (trj_sq_tuning)."""
-        k, a, t = C.c_int32(), C.c_int32(), C.c_int32()
+        k, a, x, t = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
         ms = C.c_double()
-        self._check(self._L.trj_sq_tuning(self._h, C.byref(k), C.byref(a), C.byref(ms), C.byref(t)),
+        self._check(self._L.trj_sq_tuning(self._h, C.byref(k), C.byref(a), C.byref(x), C.byref(ms), C.byref(t)),
                     "trj_sq_tuning")
-        return {"splits": k.value, "atoms_per_split": a.value, "ms": ms.value, "tried": t.value}
+        return {"splits": k.value, "atoms_per_split": a.value, "xcd_grouped": x.value, "ms": ms.value,
+                "tried": t.value}
 
     def frame_device_noalloc(self, pos_ptr: int, dtype: int, layout: int, itype_ptr: int, n: int, box):
         """RDF of a device-resident frame without any allocation
