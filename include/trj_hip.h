@@ -139,9 +139,12 @@ int trj_rdf_tuning(trj_ctx* ctx, int32_t* rows_per_pass, int32_t* passes, int32_
 /* The cell grid of the last RDF frame (no counterpart in the reference, whose
  * rdf_sh loops over all pairs, rdf.cuf:426-487): cells per axis (0 when the
  * frame took the all-pairs kernel) and the cell side as a multiple of the
- * cutoff the autotuner chose (1, 1.25 or 1.5; cells of one cutoff without the
- * autotuner).  The environment variable TRJ_RDF_CELL_SCALE (1 to 4), read by
- * trj_rdf_init, fixes that multiple instead.  Any argument may be NULL. */
+ * cutoff the autotuner chose (0.5, 1, 1.25 or 1.5; cells of one cutoff
+ * without the autotuner).  Half-cutoff cells need every periodic axis to hold
+ * at least 7 of them and no sampling points (trj_rdf_sampling).  With the
+ * autotuner on, the environment variable TRJ_RDF_CELL_SCALE (0.5, or 1 to 4),
+ * read by trj_rdf_init, fixes that multiple instead.  Any argument may be
+ * NULL. */
 int trj_rdf_cell_grid(trj_ctx* ctx, int32_t cells[3], double* side_over_cutoff);
 
 /* /INPUT_SQ/ (input.f90:70) plus the build's options. */

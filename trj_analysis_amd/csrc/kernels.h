@@ -67,6 +67,10 @@ struct CellGrid {
   int nc[3];
   double inv_h[3];
   int periodic[3];
+  // Stencil reach of the RDF pair kernel in cells: 1 (0 taken as 1) for cells
+  // of side >= the cutoff; 2 for cells of side >= half the cutoff, walked as
+  // home chunks along x-rows of cells (k_rdf_cells).
+  int reach;
 };
 void launch_cell_count(const double* pos, int64_t n, int ndim, const CellGrid& g,
                        int32_t* cellid, int32_t* count, hipStream_t s);

@@ -1215,6 +1215,91 @@ __device__ inline void rdf_home_chunk_pf(const PairArgs& a, const double* thr, c
   }
 }
 
+// Cells of side >= half the cutoff (reach 2; prefilter path only, every
+// periodic axis with >= 7 cells): one wavefront walks cells [c0, c1) of the
+// x-row `row` (cy, cz) in home chunks of up to 16 consecutive atoms that span
+// at most 3 cells, so that cells of ~14 atoms still fill the 16 home lanes.
+// A chunk whose atoms lie in cells cxa..cxb meets, in each row of the forward
+// half stencil ((dy, dz) = (+1, 0), (+2, 0), (-2..+2, +1), (-2..+2, +2)), the
+// cells cxa - 2 .. cxb + 2: one range of the cell-sorted arrays, split where
+// it wraps across a periodic x boundary (at most 7 cells <= ncx, so no cell
+// twice).  Its own row runs from the chunk's first atom (j > i) to cell
+// cxb + 2.  Every neighbour within the cutoff of a home atom in cell cx lies
+// in cells cx - 2 .. cx + 2 of these rows (2 h >= rc); cells further out only
+// add candidates that the cutoff test rejects.  Each pair i < j is counted
+// once: from i's chunk, in j's row when that row is forward of i's, else
+// (same row) by j > i.
+template <bool D3, bool QQ>
+__device__ inline void rdf_row_pf(const PairArgs& a, const double* thr, const PfTables& tb, uint32_t* hist,
+                                  PfWave& w, int lane, const CellGrid& g, int row, int c0, int c1,
+                                  const int32_t* __restrict__ start, const double* __restrict__ sx,
+                                  const double* __restrict__ sy, const double* __restrict__ sz,
+                                  const int32_t* __restrict__ stype, const double* __restrict__ sq,
+                                  unsigned long long* gqs, double qsc) {
+  const double nan = __builtin_nan("");
+  const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
+  const int cy = row % ncy, cz = row / ncy;
+  const int rb = row * ncx;  // the row's first cell
+  const double oy = static_cast<double>(cy) / g.inv_h[1];
+  const double oz = D3 ? static_cast<double>(cz) / g.inv_h[2] : 0.0;
+  auto st = [&](int c) { return __builtin_amdgcn_readfirstlane(start[c]); };
+  const int hend = st(rb + c1);
+  int cx = c0;
+  int h0 = st(rb + c0);
+  while (h0 < hend) {
+    while (st(rb + cx + 1) <= h0) ++cx;  // the cell of the chunk's first atom
+    const int cmax = (cx + 2 < c1 - 1) ? cx + 2 : c1 - 1;
+    const int lim = st(rb + cmax + 1);
+    const int he = (h0 + kRow < lim) ? h0 + kRow : lim;
+    int cxb = cx;  // the cell of its last atom
+    while (st(rb + cxb + 1) <= he - 1) ++cxb;
+    // the chunk's first cell's corner: the reference point of the FP32 coordinates
+    const double ox = static_cast<double>(cx) / g.inv_h[0];
+    const int gi = h0 + (lane & (kRow - 1));  // the same 16 home atoms in every row
+    const int ti = gi < he ? stype[gi] : 0;
+    const bool oki = static_cast<unsigned>(ti - 1) < static_cast<unsigned>(a.nsp);
+    const float hx = static_cast<float>(oki ? sx[gi] - ox : nan);
+    const float hy = static_cast<float>(oki ? sy[gi] - oy : nan);
+    const float hz = D3 ? static_cast<float>(oki ? sz[gi] - oz : nan) : 0.f;
+    // its own row: j > i up to cell cxb + 2, then the cells across a periodic +x boundary
+    {
+      const int xe = (cxb + 2 < ncx - 1) ? cxb + 2 : ncx - 1;
+      rdf_neighbours_pf<D3, true, QQ>(a, thr, tb, hist, w, lane, h0, st(rb + xe + 1), gi, ti, hx, hy, hz, ox, oy,
+                                      oz, sx, sy, sz, stype, sq, gqs, qsc);
+      if (g.periodic[0] && cxb + 2 >= ncx)
+        rdf_neighbours_pf<D3, false, QQ>(a, thr, tb, hist, w, lane, st(rb), st(rb + cxb + 2 - ncx + 1), gi, ti, hx,
+                                         hy, hz, ox - a.lmi[0], oy, oz, sx, sy, sz, stype, sq, gqs, qsc);
+    }
+    const int nrow = D3 ? 12 : 2;
+    for (int q = 0; q < nrow; ++q) {
+      const int dz = q < 2 ? 0 : 1 + (q - 2) / 5;
+      const int dy = q < 2 ? q + 1 : (q - 2) % 5 - 2;
+      int ny = cy + dy, nz = cz + dz;
+      const int syi = ny < 0 ? 1 : (ny >= ncy ? -1 : 0);
+      const int szi = D3 ? (nz < 0 ? 1 : (nz >= ncz ? -1 : 0)) : 0;
+      if (g.periodic[1]) ny = (ny + ncy) % ncy; else if (ny < 0 || ny >= ncy) continue;
+      if (D3) {
+        if (g.periodic[2]) nz = (nz + ncz) % ncz; else if (nz < 0 || nz >= ncz) continue;
+      }
+      const int base = (nz * ncy + ny) * ncx;
+      const double py = oy + static_cast<double>(syi) * a.lmi[1];
+      const double pz = D3 ? oz + static_cast<double>(szi) * a.lmi[2] : 0.0;
+      const int lo = cx - 2, hi = cxb + 2;
+      // cells inside the row, then (periodic x) those across the -x and +x boundaries
+      const int m0 = lo < 0 ? 0 : lo, m1 = hi > ncx - 1 ? ncx - 1 : hi;
+      rdf_neighbours_pf<D3, false, QQ>(a, thr, tb, hist, w, lane, st(base + m0), st(base + m1 + 1), gi, ti, hx, hy,
+                                       hz, ox, py, pz, sx, sy, sz, stype, sq, gqs, qsc);
+      if (g.periodic[0] && lo < 0)
+        rdf_neighbours_pf<D3, false, QQ>(a, thr, tb, hist, w, lane, st(base + ncx + lo), st(base + ncx), gi, ti, hx,
+                                         hy, hz, ox + a.lmi[0], py, pz, sx, sy, sz, stype, sq, gqs, qsc);
+      if (g.periodic[0] && hi >= ncx)
+        rdf_neighbours_pf<D3, false, QQ>(a, thr, tb, hist, w, lane, st(base), st(base + hi - ncx + 1), gi, ti, hx,
+                                         hy, hz, ox - a.lmi[0], py, pz, sx, sy, sz, stype, sq, gqs, qsc);
+    }
+    h0 = he;
+  }
+}
+
 template <bool SHIFT, bool D3, bool QQ>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_rdf_cells(PairArgs a, CellGrid g, Stencil st,
                                                    const int32_t* __restrict__ start,
@@ -1265,7 +1350,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     tb.cut = static_cast<float>(a.side2 - 2.0 * a.pf_delta);
   }
   const int chunk = SHIFT ? kRow : kWave;
-  for (int64_t task = static_cast<int64_t>(blockIdx.x) * kCellWaves + wave; task < ntasks;
+  if (SHIFT && g.reach == 2) {
+    // tasks (row, part): part p of W walks cells [p ncx / W, (p + 1) ncx / W) of the row
+    for (int64_t task = static_cast<int64_t>(blockIdx.x) * kCellWaves + wave; task < ntasks;
+         task += static_cast<int64_t>(gridDim.x) * kCellWaves) {
+      const int row = static_cast<int>(task / W), part = static_cast<int>(task % W);
+      const int c0 = static_cast<int>(static_cast<int64_t>(part) * g.nc[0] / W);
+      const int c1 = static_cast<int>(static_cast<int64_t>(part + 1) * g.nc[0] / W);
+      if (c0 < c1)
+        rdf_row_pf<D3, QQ>(a, thr, tb, hist, pw, lane, g, row, c0, c1, start, sx, sy, sz, stype, a.q, gqs, qsc);
+    }
+  }
+  for (int64_t task = static_cast<int64_t>(blockIdx.x) * kCellWaves + wave; task < ntasks && !(SHIFT && g.reach == 2);
        task += static_cast<int64_t>(gridDim.x) * kCellWaves) {
     const int cell = static_cast<int>(task / W);
     const int ch0 = static_cast<int>(task % W);
@@ -3072,14 +3168,18 @@ void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start
       }
   st.n = k;
   const int64_t ncell = static_cast<int64_t>(g.nc[0]) * g.nc[1] * g.nc[2];
-  const int64_t ntasks = ncell * waves_per_cell;
   bool shift = true;
   for (int k = 0; k < a.ndim; ++k) shift = shift && a.cellshift[k];
+  // tasks: (home cell, chunk share); with reach 2, (x-row of cells, part)
+  const bool row_tasks = shift && g.reach == 2;
+  const int64_t ntasks = (row_tasks ? ncell / g.nc[0] : ncell) * waves_per_cell;
   const bool qq = a.q != nullptr;
   // FP32 prefilter bound (image-shift axes): |FP32 r2 - FP64 r2| < delta for
   // every pair with r2 <= side2 + delta.  Per axis, with U the largest
-  // |coordinate relative to the home cell's corner| (2 cells, plus L/4 + 1
-  // cell on a non-periodic axis, whose atoms may lie in [-L/4, 5L/4)) and
+  // |coordinate relative to the home cell's corner| (2 cells; with reach 2,
+  // 3 cells, and 5 along x, where a home chunk spans up to 3 cells and meets
+  // 2 more on either side; plus L/4 + 1 cell on a non-periodic axis, whose
+  // atoms may lie in [-L/4, 5L/4)) and
   // L the axis length: corner + shift and x - (corner + shift) in FP64
   // (< 2^-51 (2L + U) together), each coordinate rounded to FP32 (< 2^-24 U),
   // the FP32 difference (< 2^-24 (rc + 2e)); so e < 2^-23 (2U + rc) +
@@ -3094,7 +3194,8 @@ void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start
     for (int k = 0; k < a.ndim; ++k) {
       const double L = g.periodic[k] ? a.lmi[k] : a.lmi[k] / 2.5;
       const double cs = 1.0 / g.inv_h[k];
-      const double U = 2.0 * cs + (g.periodic[k] ? 0.0 : 0.25 * L + cs);
+      const double reach_cells = row_tasks ? (k == 0 ? 5.0 : 3.0) : 2.0;
+      const double U = reach_cells * cs + (g.periodic[k] ? 0.0 : 0.25 * L + cs);
       const double e = std::ldexp(2.0 * U + rc, -23) + std::ldexp(2.0 * L + U, -51);
       err += 2.0 * rc * e + e * e;
     }

@@ -486,13 +486,17 @@ struct RdfCells {
   int W = 1;
   int nblocks = 1;
   double scale = 1.0;  // cell side >= scale * cutoff
+  int64_t units = 1;   // what W wavefronts share: cells, or with reach 2 x-rows of cells
 };
 
 RdfCells rdf_cell_plan(int ndim, const double box[3], const int periodic[3], double side2, int64_t n,
                        int W_override = 0, double scale = 1.0) {
   RdfCells p;
-  // cells of side >= scale * cutoff (scale >= 1, the autotuner's choice)
-  p.scale = std::max(1.0, scale);
+  // cells of side >= scale * cutoff: scale >= 1, or 0.5 (reach 2: cells of
+  // half the cutoff, walked in x-rows; only where every axis takes the image
+  // shift with at least 7 cells on a periodic axis, else the plan is not ok)
+  p.scale = scale == 0.5 ? 0.5 : std::max(1.0, scale);
+  p.cg.reach = p.scale == 0.5 ? 2 : 1;
   const double rc = std::sqrt(side2) * p.scale;
   for (int k = 0; k < 3; ++k) {
     if (k < ndim) {
@@ -514,16 +518,23 @@ RdfCells rdf_cell_plan(int ndim, const double box[3], const int periodic[3], dou
     }
     p.ncell *= p.cg.nc[k];
   }
+  if (p.cg.reach == 2) {
+    for (int k = 0; k < ndim; ++k)
+      if (periodic[k] && p.cg.nc[k] < 7) p.cells_ok = false;
+    if (p.cg.nc[0] < 7) p.cells_ok = false;  // a row's window of 7 cells holds no cell twice
+  }
   if (p.cells_ok) {
     // home chunks: 16 atoms with the image shift on every axis (the
-    // prefiltered kernel), else 64; W wavefronts share a cell's chunks
+    // prefiltered kernel), else 64; W wavefronts share a cell's chunks, or
+    // with reach 2 split an x-row of cells into W parts (about 4 chunks each)
     bool all_shift = true;
     for (int k = 0; k < ndim; ++k) all_shift = all_shift && p.cellshift[k];
-    const double chunk = all_shift ? 16.0 : 64.0;
-    const double mean = static_cast<double>(n) / static_cast<double>(p.ncell);
+    p.units = p.cg.reach == 2 ? p.ncell / p.cg.nc[0] : p.ncell;
+    const double chunk = p.cg.reach == 2 ? 64.0 : (all_shift ? 16.0 : 64.0);
+    const double mean = static_cast<double>(n) / static_cast<double>(p.units);
     p.W = std::max(1, std::min(static_cast<int>(std::lround(mean / chunk)), 16));
     if (W_override > 0) p.W = W_override;
-    p.nblocks = static_cast<int>(std::min<int64_t>((p.ncell * p.W + kCellWaves - 1) / kCellWaves, 2048));
+    p.nblocks = static_cast<int>(std::min<int64_t>((p.units * p.W + kCellWaves - 1) / kCellWaves, 2048));
   } else {
     const int64_t nb = (n + 63) / 64;
     p.nblocks = static_cast<int>(std::min<int64_t>((nb * nb + 3) / 4, 1024));
@@ -576,9 +587,6 @@ int tune_pairs(trj_ctx* c, const PairArgs& pa0, const RdfCells& base, int64_t n,
   const size_t nbins = static_cast<size_t>(R.nit) * R.lsmax;
   int r;
   if ((r = ensure(c, c->partial, sizeof(uint32_t) * nbins * 2048))) return r;
-  const int32_t* start = P<int32_t>(c->cellstart);
-  const double* spos = P<double>(c->spos);
-  const int32_t* stype = P<int32_t>(c->stype);
   uint32_t* part = P<uint32_t>(c->partial);
   PairArgs pa = pa0;
   double best = 1e300;
@@ -587,7 +595,11 @@ int tune_pairs(trj_ctx* c, const PairArgs& pa0, const RdfCells& base, int64_t n,
   // coarser one).
   double need = 1.0;
   auto trial = [&](const RdfCells& p, int rw, int W, bool timed) -> int {
-    const int nb = static_cast<int>(std::min<int64_t>((p.ncell * W + kCellWaves - 1) / kCellWaves, 2048));
+    // the sorted arrays as they are now: sort() may have grown (moved) them
+    const int32_t* start = P<int32_t>(c->cellstart);
+    const double* spos = P<double>(c->spos);
+    const int32_t* stype = P<int32_t>(c->stype);
+    const int nb = static_cast<int>(std::min<int64_t>((p.units * W + kCellWaves - 1) / kCellWaves, 2048));
     pa.nrows = rw;
     for (int k = 0; k < 3; ++k) pa.cellshift[k] = p.cellshift[k];
     if (!timed) {
@@ -617,8 +629,9 @@ int tune_pairs(trj_ctx* c, const PairArgs& pa0, const RdfCells& base, int64_t n,
       if ((r = trial(base, rw, W, true))) return r;
   const int rbest = tu.rows;
   need = 1.0 + kTuneGain;
-  for (double f : {1.25, 1.5}) {
+  for (double f : {0.5, 1.25, 1.5}) {
     if (c->rdf_fixed_scale > 0.0) break;
+    if (f < 1.0 && c->nrand > 0) continue;  // the sampling spheres read cells of one cutoff
     const RdfCells p = plan(f);
     if (!p.cells_ok || p.ncell == base.ncell) continue;
     if ((r = sort(p))) return r;
@@ -872,6 +885,9 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
         return rdf_cell_plan(ndim, box, periodic, g.side2, n, tu.done ? tu.W : 0, scale);
       };
       RdfCells rcl = plan(tu.done ? tu.scale : (c->rdf_fixed_scale > 0.0 ? c->rdf_fixed_scale : 1.0));
+      // half-cutoff cells only where the plan allows them and no sampling
+      // spheres read the cells
+      if (rcl.cg.reach == 2 && (!rcl.cells_ok || c->nrand > 0)) rcl = plan(1.0);
       const bool cells_ok = rcl.cells_ok;
       for (int k = 0; k < 3; ++k) pa.cellshift[k] = rcl.cellshift[k];
       pa.row0 = 0;
@@ -1294,9 +1310,10 @@ int trj_rdf_init(trj_ctx* c, const trj_rdf_params* p, const double box[3], int32
   c->rdf_tune = {};
   c->rdf_tune.enabled = (p->flags & TRJ_RDF_NO_AUTOTUNE) == 0;
   c->rdf_fixed_scale = 0.0;
-  if (const char* e = std::getenv("TRJ_RDF_CELL_SCALE")) {
+  const char* e = c->rdf_tune.enabled ? std::getenv("TRJ_RDF_CELL_SCALE") : nullptr;
+  if (e) {
     const double f = std::atof(e);
-    if (f >= 1.0 && f <= 4.0) c->rdf_fixed_scale = f;
+    if (f == 0.5 || (f >= 1.0 && f <= 4.0)) c->rdf_fixed_scale = f;
   }
   const int nsl = c->slices.n;
   if (nsl > 0) {
@@ -1513,9 +1530,11 @@ int trj_frame_device_noalloc(trj_ctx* c, const void* pos_dev, int32_t dtype, int
   const double box[3] = {box_in[0], box_in[1], ndim == 3 ? box_in[2] : 1.0};
   double rc = R.rc;
   const FrameGeom g = frame_geometry(ndim, box, R.periodic, &rc);
-  const RdfCells cp = rdf_cell_plan(ndim, box, R.periodic, g.side2, n, c->rdf_tune.done ? c->rdf_tune.W : 0,
-                                    c->rdf_tune.done ? c->rdf_tune.scale
-                                                     : (c->rdf_fixed_scale > 0.0 ? c->rdf_fixed_scale : 1.0));
+  RdfCells cp = rdf_cell_plan(ndim, box, R.periodic, g.side2, n, c->rdf_tune.done ? c->rdf_tune.W : 0,
+                              c->rdf_tune.done ? c->rdf_tune.scale
+                                               : (c->rdf_fixed_scale > 0.0 ? c->rdf_fixed_scale : 1.0));
+  if (cp.cg.reach == 2 && (!cp.cells_ok || c->nrand > 0))
+    cp = rdf_cell_plan(ndim, box, R.periodic, g.side2, n, c->rdf_tune.done ? c->rdf_tune.W : 0, 1.0);
   const size_t nblk = static_cast<size_t>((n + 255) / 256);
   const size_t un = static_cast<size_t>(n), nc = static_cast<size_t>(cp.ncell);
   const size_t cells = cp.cells_ok ? 1 : 0;
