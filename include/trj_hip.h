@@ -130,9 +130,11 @@ int trj_rdf_init(trj_ctx* ctx, const trj_rdf_params* p, const double box[3],
 
 /* The RDF autotuner's choice (no counterpart in the reference, whose launch
  * shape is fixed, rdf.cuf:208-214): histogram rows per LDS tile, passes per
- * frame, wavefronts per home cell, the chosen shape's time in ms on the
- * tuning frame and the number of shapes timed (0 before the first frame, or
- * with TRJ_RDF_NO_AUTOTUNE).  Any argument may be NULL. */
+ * frame, wavefronts per home cell (per x-row part on cells of half the
+ * cutoff), the chosen shape's time in ms on the tuning frame (its grid's cell
+ * sort and its pair kernel) and the number of shapes timed (0 before the
+ * first frame, or with TRJ_RDF_NO_AUTOTUNE).  The cell side it chose is
+ * reported by trj_rdf_cell_grid.  Any argument may be NULL. */
 int trj_rdf_tuning(trj_ctx* ctx, int32_t* rows_per_pass, int32_t* passes, int32_t* waves_per_cell, double* ms,
                    int32_t* tried);
 

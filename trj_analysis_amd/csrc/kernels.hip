@@ -1054,7 +1054,10 @@ __device__ inline void bin_hits_pf(const PairArgs& a, const double* thr, const P
 // Step K of a batch: home lanes against neighbour 16 r + K of the batch.
 // With CHECK the step is skipped when no row holds a neighbour at K (only a
 // batch of fewer than 16 neighbours needs the test).
-template <int K, bool D3, bool SELF, bool QQ, bool CHECK>
+// PACKED: the batch's neighbours come from several runs (rdf_packed_pf) and
+// jrow is this lane's own neighbour index, broadcast per step; otherwise the
+// neighbours are contiguous and jrow is the first of this lane's row of 16.
+template <int K, bool D3, bool SELF, bool QQ, bool CHECK, bool PACKED = false>
 __device__ inline void pf_step(const PairArgs& a, const double* thr, const PfTables& tb, uint32_t* hist, PfWave& w,
                                int lane, int jb, int je, int gi, int ti, int jrow, float hx, float hy, float hz,
                                const float4& q, const double* __restrict__ sx, const double* __restrict__ sy,
@@ -1068,7 +1071,7 @@ __device__ inline void pf_step(const PairArgs& a, const double* thr, const PfTab
     const float dz = row_bcast<K>(q.z) - hz;
     r2 = fmaf(dz, dz, r2);
   }
-  const int gj = jrow + K;
+  const int gj = PACKED ? row_bcast_i<K>(jrow) : jrow + K;
   // the neighbour's species (broadcast with every lane active: a DPP read of
   // a lane outside EXEC would not return its value)
   const int tj = row_bcast_i<K>(__builtin_bit_cast(int, q.w));
@@ -1091,15 +1094,15 @@ __device__ inline void pf_step(const PairArgs& a, const double* thr, const PfTab
   }
 }
 
-template <bool D3, bool SELF, bool QQ, bool CHECK, int... K>
+template <bool D3, bool SELF, bool QQ, bool CHECK, bool PACKED, int... K>
 __device__ inline void pf_batch(const PairArgs& a, const double* thr, const PfTables& tb, uint32_t* hist, PfWave& w,
                                 int lane, int jb, int je, int gi, int ti, int jrow, float hx, float hy, float hz,
                                 const float4& q, const double* __restrict__ sx, const double* __restrict__ sy,
                                 const double* __restrict__ sz, const int32_t* __restrict__ stype,
                                 const double* __restrict__ sq, unsigned long long* gqs, double qsc,
                                 std::integer_sequence<int, K...>) {
-  (pf_step<K, D3, SELF, QQ, CHECK>(a, thr, tb, hist, w, lane, jb, je, gi, ti, jrow, hx, hy, hz, q, sx, sy, sz, stype,
-                                   sq, gqs, qsc),
+  (pf_step<K, D3, SELF, QQ, CHECK, PACKED>(a, thr, tb, hist, w, lane, jb, je, gi, ti, jrow, hx, hy, hz, q, sx, sy,
+                                           sz, stype, sq, gqs, qsc),
    ...);
 }
 
@@ -1126,11 +1129,11 @@ __device__ inline void rdf_neighbours_pf(const PairArgs& a, const double* thr, c
     q.z = (D3 && vl) ? static_cast<float>(sz[jl] - oz) : 0.f;
     q.w = __builtin_bit_cast(float, tl);
     if (je - jb >= kRow)  // every step has a neighbour in row 0
-      pf_batch<D3, SELF, QQ, false>(a, thr, tb, hist, w, lane, jb, je, gi, ti, jb + r16, hx, hy, hz, q, sx, sy, sz,
-                                    stype, sq, gqs, qsc, std::make_integer_sequence<int, kRow>{});
+      pf_batch<D3, SELF, QQ, false, false>(a, thr, tb, hist, w, lane, jb, je, gi, ti, jb + r16, hx, hy, hz, q, sx, sy,
+                                           sz, stype, sq, gqs, qsc, std::make_integer_sequence<int, kRow>{});
     else
-      pf_batch<D3, SELF, QQ, true>(a, thr, tb, hist, w, lane, jb, je, gi, ti, jb + r16, hx, hy, hz, q, sx, sy, sz,
-                                   stype, sq, gqs, qsc, std::make_integer_sequence<int, kRow>{});
+      pf_batch<D3, SELF, QQ, true, false>(a, thr, tb, hist, w, lane, jb, je, gi, ti, jb + r16, hx, hy, hz, q, sx, sy,
+                                          sz, stype, sq, gqs, qsc, std::make_integer_sequence<int, kRow>{});
   }
 }
 
@@ -1215,6 +1218,74 @@ __device__ inline void rdf_home_chunk_pf(const PairArgs& a, const double* thr, c
   }
 }
 
+// A wavefront's table of neighbour runs for one home chunk: run k (k < 64,
+// at most 37 here) in lane k of three registers; its first packed index
+// (lane n: the total),
+// its first atom, and its image shift as (sx + 1) + 3 (sy + 1) + 9 (sz + 1),
+// each shift in -1..1 (subtract s L, as the unpacked calls' corner + s L).
+struct RunTable {
+  int cum = 0, js = 0, code = 0;
+  int n = 0, total = 0;  // wave-uniform: runs, packed length
+};
+
+__device__ inline void run_add(RunTable& t, int lane, int js, int je, int sxi, int syi, int szi) {
+  if (je <= js) return;
+  const bool me = lane == t.n;
+  t.cum = me ? t.total : t.cum;
+  t.js = me ? js : t.js;
+  t.code = me ? (sxi + 1) + 3 * (syi + 1) + 9 * (szi + 1) : t.code;
+  t.total += je - js;
+  ++t.n;
+}
+
+// The runs of t against one home chunk as one packed sequence: batches of
+// 64 neighbours straddle runs (lane 16 r + k holds packed index P + 16 r + k,
+// from the run whose first packed index is the largest at or below it), so
+// that short runs do not leave batches mostly empty.  Same steps as
+// rdf_neighbours_pf, with the neighbour index broadcast per step.
+template <bool D3, bool QQ>
+__device__ inline void rdf_packed_pf(const PairArgs& a, const double* thr, const PfTables& tb, uint32_t* hist,
+                                     PfWave& w, int lane, RunTable& t, int gi, int ti, float hx, float hy, float hz,
+                                     double ox, double oy, double oz, const double* __restrict__ sx,
+                                     const double* __restrict__ sy, const double* __restrict__ sz,
+                                     const int32_t* __restrict__ stype, const double* __restrict__ sq,
+                                     unsigned long long* gqs, double qsc) {
+  if (t.n == 0) return;
+  t.cum = lane == t.n ? t.total : t.cum;  // sentinel
+  const float fnan = __builtin_nanf("");
+  int r0 = 0;  // wave-uniform: the run holding packed index P
+  for (int P = 0; P < t.total; P += kWave) {
+    const int v = P + lane;
+    int r = r0;
+    for (int k = r0 + 1; k < t.n; ++k) {
+      const int ck = __builtin_amdgcn_readlane(t.cum, k);
+      if (ck >= P + kWave) break;
+      r += ck <= v ? 1 : 0;
+    }
+    const int cr = __builtin_amdgcn_ds_bpermute(r << 2, t.cum);
+    const int jr = __builtin_amdgcn_ds_bpermute(r << 2, t.js);
+    const int code = __builtin_amdgcn_ds_bpermute(r << 2, t.code);
+    const int jl = jr + (v - cr);
+    const int tl = v < t.total ? stype[jl] : 0;
+    const bool vl = static_cast<unsigned>(tl - 1) < static_cast<unsigned>(a.nsp);
+    const double px = ox + static_cast<double>(code % 3 - 1) * a.lmi[0];
+    const double py = oy + static_cast<double>((code / 3) % 3 - 1) * a.lmi[1];
+    const double pz = D3 ? oz + static_cast<double>(code / 9 - 1) * a.lmi[2] : 0.0;
+    float4 q;
+    q.x = vl ? static_cast<float>(sx[jl] - px) : fnan;
+    q.y = vl ? static_cast<float>(sy[jl] - py) : fnan;
+    q.z = (D3 && vl) ? static_cast<float>(sz[jl] - pz) : 0.f;
+    q.w = __builtin_bit_cast(float, tl);
+    if (t.total - P >= kRow)
+      pf_batch<D3, false, QQ, false, true>(a, thr, tb, hist, w, lane, P, t.total, gi, ti, jl, hx, hy, hz, q, sx, sy,
+                                           sz, stype, sq, gqs, qsc, std::make_integer_sequence<int, kRow>{});
+    else
+      pf_batch<D3, false, QQ, true, true>(a, thr, tb, hist, w, lane, P, t.total, gi, ti, jl, hx, hy, hz, q, sx, sy,
+                                          sz, stype, sq, gqs, qsc, std::make_integer_sequence<int, kRow>{});
+    while (r0 + 1 < t.n && __builtin_amdgcn_readlane(t.cum, r0 + 1) <= P + kWave) ++r0;
+  }
+}
+
 // Cells of side >= half the cutoff (reach 2; prefilter path only, every
 // periodic axis with >= 7 cells): one wavefront walks cells [c0, c1) of the
 // x-row `row` (cy, cz) in home chunks of up to 16 consecutive atoms that span
@@ -1266,10 +1337,10 @@ __device__ inline void rdf_row_pf(const PairArgs& a, const double* thr, const Pf
       const int xe = (cxb + 2 < ncx - 1) ? cxb + 2 : ncx - 1;
       rdf_neighbours_pf<D3, true, QQ>(a, thr, tb, hist, w, lane, h0, st(rb + xe + 1), gi, ti, hx, hy, hz, ox, oy,
                                       oz, sx, sy, sz, stype, sq, gqs, qsc);
-      if (g.periodic[0] && cxb + 2 >= ncx)
-        rdf_neighbours_pf<D3, false, QQ>(a, thr, tb, hist, w, lane, st(rb), st(rb + cxb + 2 - ncx + 1), gi, ti, hx,
-                                         hy, hz, ox - a.lmi[0], oy, oz, sx, sy, sz, stype, sq, gqs, qsc);
     }
+    // the other runs, packed into full batches
+    RunTable rt;
+    if (g.periodic[0] && cxb + 2 >= ncx) run_add(rt, lane, st(rb), st(rb + cxb + 2 - ncx + 1), -1, 0, 0);
     const int nrow = D3 ? 12 : 2;
     for (int q = 0; q < nrow; ++q) {
       const int dz = q < 2 ? 0 : 1 + (q - 2) / 5;
@@ -1282,25 +1353,21 @@ __device__ inline void rdf_row_pf(const PairArgs& a, const double* thr, const Pf
         if (g.periodic[2]) nz = (nz + ncz) % ncz; else if (nz < 0 || nz >= ncz) continue;
       }
       const int base = (nz * ncy + ny) * ncx;
-      const double py = oy + static_cast<double>(syi) * a.lmi[1];
-      const double pz = D3 ? oz + static_cast<double>(szi) * a.lmi[2] : 0.0;
       const int lo = cx - 2, hi = cxb + 2;
       // cells inside the row, then (periodic x) those across the -x and +x boundaries
       const int m0 = lo < 0 ? 0 : lo, m1 = hi > ncx - 1 ? ncx - 1 : hi;
-      rdf_neighbours_pf<D3, false, QQ>(a, thr, tb, hist, w, lane, st(base + m0), st(base + m1 + 1), gi, ti, hx, hy,
-                                       hz, ox, py, pz, sx, sy, sz, stype, sq, gqs, qsc);
-      if (g.periodic[0] && lo < 0)
-        rdf_neighbours_pf<D3, false, QQ>(a, thr, tb, hist, w, lane, st(base + ncx + lo), st(base + ncx), gi, ti, hx,
-                                         hy, hz, ox + a.lmi[0], py, pz, sx, sy, sz, stype, sq, gqs, qsc);
-      if (g.periodic[0] && hi >= ncx)
-        rdf_neighbours_pf<D3, false, QQ>(a, thr, tb, hist, w, lane, st(base), st(base + hi - ncx + 1), gi, ti, hx,
-                                         hy, hz, ox - a.lmi[0], py, pz, sx, sy, sz, stype, sq, gqs, qsc);
+      run_add(rt, lane, st(base + m0), st(base + m1 + 1), 0, syi, szi);
+      if (g.periodic[0] && lo < 0) run_add(rt, lane, st(base + ncx + lo), st(base + ncx), 1, syi, szi);
+      if (g.periodic[0] && hi >= ncx) run_add(rt, lane, st(base), st(base + hi - ncx + 1), -1, syi, szi);
     }
+    rdf_packed_pf<D3, QQ>(a, thr, tb, hist, w, lane, rt, gi, ti, hx, hy, hz, ox, oy, oz, sx, sy, sz, stype, sq, gqs,
+                          qsc);
     h0 = he;
   }
 }
 
-template <bool SHIFT, bool D3, bool QQ>
+// ROWS: cells of half the cutoff walked in x-rows (rdf_row_pf; SHIFT only).
+template <bool SHIFT, bool D3, bool QQ, bool ROWS = false>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_rdf_cells(PairArgs a, CellGrid g, Stencil st,
                                                    const int32_t* __restrict__ start,
                                                    const double* __restrict__ spos,
@@ -1350,7 +1417,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     tb.cut = static_cast<float>(a.side2 - 2.0 * a.pf_delta);
   }
   const int chunk = SHIFT ? kRow : kWave;
-  if (SHIFT && g.reach == 2) {
+  if constexpr (SHIFT && ROWS) {
     // tasks (row, part): part p of W walks cells [p ncx / W, (p + 1) ncx / W) of the row
     for (int64_t task = static_cast<int64_t>(blockIdx.x) * kCellWaves + wave; task < ntasks;
          task += static_cast<int64_t>(gridDim.x) * kCellWaves) {
@@ -1361,7 +1428,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
         rdf_row_pf<D3, QQ>(a, thr, tb, hist, pw, lane, g, row, c0, c1, start, sx, sy, sz, stype, a.q, gqs, qsc);
     }
   }
-  for (int64_t task = static_cast<int64_t>(blockIdx.x) * kCellWaves + wave; task < ntasks && !(SHIFT && g.reach == 2);
+  for (int64_t task = static_cast<int64_t>(blockIdx.x) * kCellWaves + wave; task < ntasks && !(SHIFT && ROWS);
        task += static_cast<int64_t>(gridDim.x) * kCellWaves) {
     const int cell = static_cast<int>(task / W);
     const int ch0 = static_cast<int>(task % W);
@@ -3256,25 +3323,33 @@ void launch_rdf_cells(const PairArgs& a, const CellGrid& g, const int32_t* start
     t.row0 = row0;
     t.nrows = std::min(rows, a.nit - row0);
     const size_t lds = cells_lds_bytes(t.nrows, t.lsmax, qq, shift);
-#define TRJ_CELLS(S, D, Q)                                                                                        \
+#define TRJ_CELLS(S, D, Q, R)                                                                                     \
   do {                                                                                                            \
     if (lds > 65536)                                                                                              \
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rdf_cells<S, D, Q>),                            \
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rdf_cells<S, D, Q, R>),                         \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));              \
-    hipLaunchKernelGGL((k_rdf_cells<S, D, Q>), dim3(nblocks), dim3(64 * kCellWaves), lds, s, t, g, st, start, spos, \
-                       stype, n, waves_per_cell, ntasks, partial);                                            \
+    hipLaunchKernelGGL((k_rdf_cells<S, D, Q, R>), dim3(nblocks), dim3(64 * kCellWaves), lds, s, t, g, st, start,    \
+                       spos, stype, n, waves_per_cell, ntasks, partial);                                      \
   } while (0)
     if (qq) {
       if (a.ndim == 3) {
-        if (shift) TRJ_CELLS(true, true, true); else TRJ_CELLS(false, true, true);
+        if (row_tasks) TRJ_CELLS(true, true, true, true);
+        else if (shift) TRJ_CELLS(true, true, true, false);
+        else TRJ_CELLS(false, true, true, false);
       } else {
-        if (shift) TRJ_CELLS(true, false, true); else TRJ_CELLS(false, false, true);
+        if (row_tasks) TRJ_CELLS(true, false, true, true);
+        else if (shift) TRJ_CELLS(true, false, true, false);
+        else TRJ_CELLS(false, false, true, false);
       }
     } else {
       if (a.ndim == 3) {
-        if (shift) TRJ_CELLS(true, true, false); else TRJ_CELLS(false, true, false);
+        if (row_tasks) TRJ_CELLS(true, true, false, true);
+        else if (shift) TRJ_CELLS(true, true, false, false);
+        else TRJ_CELLS(false, true, false, false);
       } else {
-        if (shift) TRJ_CELLS(true, false, false); else TRJ_CELLS(false, false, false);
+        if (row_tasks) TRJ_CELLS(true, false, false, true);
+        else if (shift) TRJ_CELLS(true, false, false, false);
+        else TRJ_CELLS(false, false, false, false);
       }
     }
 #undef TRJ_CELLS

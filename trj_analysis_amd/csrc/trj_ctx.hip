@@ -590,10 +590,10 @@ int tune_pairs(trj_ctx* c, const PairArgs& pa0, const RdfCells& base, int64_t n,
   uint32_t* part = P<uint32_t>(c->partial);
   PairArgs pa = pa0;
   double best = 1e300;
-  // A grid other than the cells of one cutoff is kept only when it is at
-  // least kTuneGain faster (timing noise must not trade a known grid for a
-  // coarser one).
-  double need = 1.0;
+  // A grid other than the cells of one cutoff is kept only when its sort and
+  // pair kernel together are at least kTuneGain faster (timing noise must not
+  // trade a known grid for another).
+  double need = 1.0, grid_ms = 0.0;
   auto trial = [&](const RdfCells& p, int rw, int W, bool timed) -> int {
     // the sorted arrays as they are now: sort() may have grown (moved) them
     const int32_t* start = P<int32_t>(c->cellstart);
@@ -614,31 +614,49 @@ int tune_pairs(trj_ctx* c, const PairArgs& pa0, const RdfCells& base, int64_t n,
     float ms = 0.f;
     HIPC(c, hipEventElapsedTime(&ms, c->tune_ev[0], c->tune_ev[1]));
     ++tu.tried;
-    if (ms * need < best) {
-      best = ms * need;
+    if ((ms + grid_ms) * need < best) {
+      best = (ms + grid_ms) * need;
       tu.rows = rw;
       tu.W = W;
       tu.scale = p.scale;
     }
     return TRJ_OK;
   };
+  // the cell sort of a grid, timed (a grid's cost is its sort plus its pair kernel)
+  auto timed_sort = [&](const RdfCells& p, double* ms_out) -> int {
+    int rr;
+    HIPC(c, hipEventRecord(c->tune_ev[0], rs));
+    if ((rr = sort(p))) return rr;
+    HIPC(c, hipEventRecord(c->tune_ev[1], rs));
+    HIPC(c, hipEventSynchronize(c->tune_ev[1]));
+    float ms = 0.f;
+    HIPC(c, hipEventElapsedTime(&ms, c->tune_ev[0], c->tune_ev[1]));
+    *ms_out = ms;
+    return TRJ_OK;
+  };
+  double sort_ms = 0.0;
+  if ((r = timed_sort(base, &sort_ms))) return r;
   // one untimed run (code object and caches warm)
   if ((r = trial(base, rows[0], base.W, false))) return r;
   for (int rw : rows)
     for (int W : waves(base.W))
       if ((r = trial(base, rw, W, true))) return r;
   const int rbest = tu.rows;
+  // other grids: compared by sort + pair time
+  best += sort_ms;
   need = 1.0 + kTuneGain;
   for (double f : {0.5, 1.25, 1.5}) {
     if (c->rdf_fixed_scale > 0.0) break;
     if (f < 1.0 && c->nrand > 0) continue;  // the sampling spheres read cells of one cutoff
     const RdfCells p = plan(f);
     if (!p.cells_ok || p.ncell == base.ncell) continue;
-    if ((r = sort(p))) return r;
+    if ((r = timed_sort(p, &sort_ms))) return r;
+    grid_ms = sort_ms;
     for (int W : waves(p.W))
       if ((r = trial(p, rbest, W, true))) return r;
   }
-  tu.ms = tu.scale == base.scale ? best : best / need;
+  grid_ms = 0.0;
+  tu.ms = tu.scale == base.scale ? best : best / need;  // with the grid's sort
   tu.done = true;
   return TRJ_OK;
 }
