@@ -457,23 +457,25 @@ __global__ __launch_bounds__(1024) void k_cell_colscan(int32_t* __restrict__ blk
   if (w == 0 && c <= ncell) count[c] = c < ncell ? tot : 0;
 }
 
-// Atom i of workgroup b's chunk goes to slot start[c] + blkcnt[b * ncell + c]
-// + rank[i] of its cell c (recomputed from its position, as k_cell_hist
-// found it): positions, types and charges in cell order.
-__global__ __launch_bounds__(1024) void k_cell_place(const double* __restrict__ pos,
-                                                     const int32_t* __restrict__ key, int nsp, int nslice,
-                                                     int64_t n, int ndim, CellGrid g, int ncell, int64_t chunk,
-                                                     const int32_t* __restrict__ rank,
-                                                     const int32_t* __restrict__ blkcnt,
-                                                     const int32_t* __restrict__ start,
-                                                     const int32_t* __restrict__ itype, double* __restrict__ spos,
-                                                     int32_t* __restrict__ stype, const double* __restrict__ q,
-                                                     double* __restrict__ sq) {
-  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * chunk;
-  const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
-  const int32_t* base = blkcnt + static_cast<int64_t>(blockIdx.x) * ncell;
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += 1024) {
+// Atom i (of counting workgroup b = i / chunk) goes to slot start[c] +
+// blkcnt[b * ncell + c] + rank[i] of its cell c (recomputed from its
+// position, as k_cell_hist found it): positions, types and charges in cell
+// order.  One thread per atom over the whole grid: the counting workgroups
+// are few when the cells are many (their LDS counters fill a CU), and the
+// placement's scattered writes want every CU.
+__global__ __launch_bounds__(256) void k_cell_place(const double* __restrict__ pos,
+                                                    const int32_t* __restrict__ key, int nsp, int nslice,
+                                                    int64_t n, int ndim, CellGrid g, int ncell, int64_t chunk,
+                                                    const int32_t* __restrict__ rank,
+                                                    const int32_t* __restrict__ blkcnt,
+                                                    const int32_t* __restrict__ start,
+                                                    const int32_t* __restrict__ itype, double* __restrict__ spos,
+                                                    int32_t* __restrict__ stype, const double* __restrict__ q,
+                                                    double* __restrict__ sq) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * 256) {
     const int c = key ? slice_cell(pos, key, n, i, nsp, g, nslice) : frame_cell(pos, n, i, ndim, g);
+    const int32_t* base = blkcnt + (i / chunk) * ncell;
     const int slot = start[c] + base[c] + rank[i];
     const double x = pos[i], y = pos[n + i], z = pos[2 * n + i];
     spos[slot] = x;
@@ -3193,8 +3195,9 @@ void launch_cell_sort(const CellSort& p, const double* pos, const int32_t* key, 
   hipLaunchKernelGGL(k_cell_colscan, dim3(static_cast<unsigned>((ncell + 1 + 63) / 64)), dim3(1024), 0, s, blkcnt,
                      p.blocks, nc, count);
   launch_exclusive_scan(count, start, ncell + 1, s);
-  hipLaunchKernelGGL(k_cell_place, dim3(p.blocks), dim3(1024), 0, s, pos, key, nsp, nslice, n, ndim, g, nc, p.chunk,
-                     rank, blkcnt, start, itype, spos, stype, q, sq);
+  hipLaunchKernelGGL(k_cell_place, dim3(static_cast<unsigned>(std::min<int64_t>((n + 255) / 256, 8192))), dim3(256),
+                     0, s, pos, key, nsp, nslice, n, ndim, g, nc, p.chunk, rank, blkcnt, start, itype, spos, stype, q,
+                     sq);
 }
 
 size_t pair_lds_bytes(int rows, int lsmax, bool qq) {
