@@ -249,9 +249,10 @@ struct ContractArgs {
   const int32_t* item_plane;
   const int32_t* nitems;
   // Workgroup order: 0 in (split, tile) order; 1 XCD-grouped, the tiles of
-  // one split on one XCD (its L2 shares the split's turns).  The FP32 and
-  // 16-bit kernels always take the grouped order; the FP64 kernel takes this
-  // field (the autotuner's choice).
+  // one split on one XCD (its L2 shares the split's turns).  For one frame
+  // (items NULL) the FP32 and 16-bit kernels always take the grouped order
+  // and the FP64 kernel takes this field (the autotuner's choice); batches
+  // keep (split, tile) order.
   int xcd_order;
 };
 void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s);

@@ -1696,7 +1696,10 @@ __device__ inline ItemRange item_range(const ContractArgs& a, int item) {
 }
 
 // Work index (item * ntiles + tile) of this workgroup in the contraction
-// kernels.  Grouped: the dispatcher deals workgroups to the 8 XCDs in turn
+// kernels.  Grouped (one frame, whose splits all cost the same; a batch's
+// grid is sized for its largest group and its items differ, so that
+// contiguous ranges per XCD would load the XCDs unevenly): the dispatcher
+// deals workgroups to the 8 XCDs in turn
 // (workgroup b to XCD b % 8); workgroup b = 8 r + x takes index x * per + r,
 // per = ceil(nwork / 8), so each XCD walks a contiguous range of (split,
 // tile) pairs in order: the tiles of one split run on one XCD at about the
@@ -1898,7 +1901,7 @@ struct LpShared32 {
 
 __global__ __launch_bounds__(256) void k_sq_contract_f32(ContractArgs a) {
   __shared__ LpShared32 sh;
-  const int64_t wk = contract_work_index(a, true);
+  const int64_t wk = contract_work_index(a, a.items == nullptr);
   if (wk >= static_cast<int64_t>(a.ntiles) * a.ksplit) return;
   const int tile = static_cast<int>(wk % a.ntiles);
   const ItemRange wi = item_range(a, static_cast<int>(wk / a.ntiles));
@@ -2142,7 +2145,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
   typedef typename LT::V V;
   __shared__ __attribute__((aligned(16))) float2 rec[3][WROWS][kRS];  // x 0..63, z 64..95, y 96..103
   __shared__ __attribute__((aligned(16))) V As[2][4][LKC / 8][WX];     // re_hi, re_lo, im_hi, im_lo of ex
-  const int64_t wk = contract_work_index(a, true);
+  const int64_t wk = contract_work_index(a, a.items == nullptr);
   if (wk >= static_cast<int64_t>(a.ntiles) * a.ksplit) return;
   const int tile = static_cast<int>(wk % a.ntiles);
   const ItemRange wi = item_range(a, static_cast<int>(wk / a.ntiles));
