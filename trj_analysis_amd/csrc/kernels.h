@@ -254,8 +254,19 @@ struct ContractArgs {
   // and the FP64 kernel takes this field (the autotuner's choice); batches
   // keep (split, tile) order.
   int xcd_order;
+  // FP64 kernel, one frame: the phase anchors e^{2 pi i n0 u} of its 8-value
+  // segments (n0 = anc_base[axis] + 8 m) as a table, row anc_row[axis] + m,
+  // entry j at anc[row * n + j] (cos, sin); NULL: the kernel reduces and
+  // evaluates them itself.  The values are the same either way.
+  const double2* anc;
+  int anc_row[3], anc_base[3];
 };
 void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s);
+// The anchor table of the FP64 contraction (ContractArgs::anc): rows[axis]
+// rows of anchors n0 = base[axis] + 8 m for every entry of the turn planes u
+// (plane stride n), axes in order x, y, z.
+void launch_sq_anchors(const double* u, int64_t n, const int rows[3], const int base[3], double2* anc,
+                       hipStream_t s);
 // Bytes of one (split, channel, vector) partial of rho: FP64 complex from the
 // FP64 contraction, FP32 complex from the others.
 inline size_t rho_entry_bytes(int precision) { return precision == TRJ_PREC_FP64 ? 16 : 8; }

@@ -1614,11 +1614,10 @@ constexpr int TKX = 8, TKY = 8, TKZ = 32, TILE_VEC = TKX * TKY * TKZ;
 // conflict-free.
 constexpr int kRec = 49;
 
-// m e^{2 pi i n u} for n = n0 .. n0+7 into dst: the anchor n0 (times the
-// channel weight m), then angle-addition steps by e^{2 pi i u} = cs + i ss.
-__device__ inline void phase8(double uh, double ul, int n0, double cs, double ss, double2* dst, double m) {
-  double s, c;
-  sincospi(2.0 * frac_turns(n0, uh, ul), &s, &c);
+// m e^{2 pi i n u} for n = n0 .. n0+7 into dst from the anchor e^{2 pi i n0 u}
+// = c + i s (times the channel weight m), then angle-addition steps by
+// e^{2 pi i u} = cs + i ss.
+__device__ inline void phase8_from(double c, double s, double cs, double ss, double2* dst, double m) {
   double re = c * m, im = s * m;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
@@ -1628,6 +1627,29 @@ __device__ inline void phase8(double uh, double ul, int n0, double cs, double ss
     re = nr;
     im = ni;
   }
+}
+// The same with the anchor from the turns u = uh + ul: exact reduction of
+// n0 u, then sincospi.
+__device__ inline void phase8(double uh, double ul, int n0, double cs, double ss, double2* dst, double m) {
+  double s, c;
+  sincospi(2.0 * frac_turns(n0, uh, ul), &s, &c);
+  phase8_from(c, s, cs, ss, dst, m);
+}
+
+// Anchor table of the FP64 contraction: row r of axis a (rows[a] rows from
+// row0[a]) holds e^{2 pi i n0 u_j} with n0 = base[a] + 8 (r - row0[a]) for
+// every entry j, computed as phase8 would (bit-identical values).
+__global__ __launch_bounds__(256) void k_sq_anchors(const double* __restrict__ u, int64_t n, int rx, int ry,
+                                                    int bx, int by, int bz, double2* __restrict__ anc) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (j >= n) return;
+  const int row = blockIdx.y;
+  const int ax = row < rx ? 0 : (row < rx + ry ? 1 : 2);
+  const int m = row - (ax == 0 ? 0 : (ax == 1 ? rx : rx + ry));
+  const int n0 = (ax == 0 ? bx : (ax == 1 ? by : bz)) + 8 * m;
+  double s, c;
+  sincospi(2.0 * frac_turns(n0, u[ax * n + j], u[(3 + ax) * n + j]), &s, &c);
+  anc[row * n + j] = make_double2(c, s);
 }
 
 // s + c += x with Knuth's TwoSum: s keeps the rounded sum, c the error, kept
@@ -1720,6 +1742,7 @@ __device__ inline int64_t contract_work_index(const ContractArgs& a, bool groupe
 // chunk is staged while this one is contracted).  Waves 0-2 stage one
 // 8-value segment per lane (lane = atom; wave 0: x | y, waves 1-2: z); the
 // next chunk's turns are prefetched into registers a chunk ahead.
+template <bool TABLE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_sq_contract_f64(
     ContractArgs a) {
   __shared__ double2 ph[2][KC][kRec];
@@ -1747,6 +1770,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const double* u_c = a.u + 6 * a.n;
   const double* u_s = a.u + 9 * a.n;
   const uint32_t st_base = static_cast<uint32_t>(st_ax) * static_cast<uint32_t>(a.n);
+  // With the anchor table, the segment's anchors (cos, sin) replace the
+  // turns hi, lo: one 16-byte load instead of the FP64 reduction and
+  // sincospi per segment and chunk.
+  const double2* anc = (TABLE && stager)
+                           ? a.anc + static_cast<int64_t>(a.anc_row[st_ax] + (st_n0 - a.anc_base[st_ax]) / 8) * a.n
+                           : a.anc;
   // The x segment carries the channel weight (1, or the charge of the entry).
   const bool st_w = a.w != nullptr && st_seg == 0;
   for (int s = wi.c0; s < wi.c1; ++s) {
@@ -1769,8 +1798,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     double uh = 0.0, ul = 0.0, uc = 1.0, us = 0.0, uw = 1.0;
     if (stager && b0 + st_at < b1) {
       const uint32_t o = st_base + static_cast<uint32_t>(b0 + st_at);
-      uh = u_hi[o];
-      ul = u_lo[o];
+      if (TABLE) {
+        const double2 an = anc[b0 + st_at];
+        uh = an.x;
+        ul = an.y;
+      } else {
+        uh = u_hi[o];
+        ul = u_lo[o];
+      }
       uc = u_c[o];
       us = u_s[o];
       if (st_w) uw = a.w[b0 + st_at];
@@ -1781,7 +1816,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       if (!stager) return;
       double2* dst = ph[buf][st_at] + st_slot;
       if (c0 + st_at < b1) {
-        phase8(uh, ul, st_n0, uc, us, dst, uw);
+        if (TABLE)
+          phase8_from(uh, ul, uc, us, dst, uw);
+        else
+          phase8(uh, ul, st_n0, uc, us, dst, uw);
       } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k) dst[k] = make_double2(0.0, 0.0);
@@ -1789,8 +1827,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       const int nx = c0 + KC + st_at;
       if (nx < b1) {
         const uint32_t o = st_base + static_cast<uint32_t>(nx);
-        uh = u_hi[o];
-        ul = u_lo[o];
+        if (TABLE) {
+          const double2 an = anc[nx];
+          uh = an.x;
+          ul = an.y;
+        } else {
+          uh = u_hi[o];
+          ul = u_lo[o];
+        }
         uc = u_c[o];
         us = u_s[o];
         if (st_w) uw = a.w[nx];
@@ -3446,7 +3490,18 @@ static dim3 contract_grid(const ContractArgs& a) {
 
 void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s) {
   if (a.ntiles <= 0) return;
-  hipLaunchKernelGGL(k_sq_contract_f64, contract_grid(a), dim3(256), 0, s, a);
+  if (a.anc)
+    hipLaunchKernelGGL(k_sq_contract_f64<true>, contract_grid(a), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_sq_contract_f64<false>, contract_grid(a), dim3(256), 0, s, a);
+}
+
+void launch_sq_anchors(const double* u, int64_t n, const int rows[3], const int base[3], double2* anc,
+                       hipStream_t s) {
+  const int nrow = rows[0] + rows[1] + rows[2];
+  if (n <= 0 || nrow <= 0) return;
+  hipLaunchKernelGGL(k_sq_anchors, dim3(static_cast<unsigned>((n + 255) / 256), nrow), dim3(256), 0, s, u, n,
+                     rows[0], rows[1], base[0], base[1], base[2], anc);
 }
 
 void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s) {

@@ -347,6 +347,26 @@ int sq_plan(const trj_sq_params& p_in, const double box[3], SqPlan* out, std::st
           }
         }
   }
+  if (p.precision == TRJ_PREC_FP64 && !P.tiles.empty()) {
+    // Segment origins of the FP64 tiles (8 kx | 8 ky | 4 x 8 kz per tile).
+    const int seg[3] = {TKX / 8, TKY / 8, TKZ / 8};
+    int lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = INT32_MAX;
+      hi[k] = INT32_MIN;
+    }
+    for (const SqTile& t : P.tiles) {
+      const int o[3] = {t.kx0, t.ky0, t.kz0};
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], o[k]);
+        hi[k] = std::max(hi[k], o[k] + 8 * (seg[k] - 1));
+      }
+    }
+    for (int k = 0; k < 3; ++k) {
+      P.anc_base[k] = lo[k];
+      P.anc_rows[k] = (hi[k] - lo[k]) / 8 + 1;
+    }
+  }
   if (P.nqmax > 32767) {
     *err = "sq: nqmax above 32767 is not supported";
     return TRJ_ELIMIT;

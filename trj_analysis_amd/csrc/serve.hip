@@ -204,6 +204,7 @@ int64_t trj_request_footprint(const trj_request* r) {
       const int64_t rho_entry = static_cast<int64_t>(trj::rho_entry_bytes(r->sq.precision));
       b += int64_t(ks) * r->sq.nsp * tiles * S.tk.vec() * rho_entry;  // rho partials
       b += n * 96;                                             // turn planes
+      b += trj::sq_anchor_bytes(S, n);                         // FP64 anchor table
       b += tiles * (16 + 2 * S.tk.vec()) + 32;                 // tiles, shells of tile vectors
       b += int64_t(S.nqmax + 1) * 8 * (1 + r->sq.nsp);         // shell sums
       b += nax * 16 + static_cast<int64_t>(trj::sq_list_scratch_bytes(static_cast<int>(nax), r->sq.nsp));

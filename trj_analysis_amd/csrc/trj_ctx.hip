@@ -55,7 +55,7 @@ struct trj_ctx {
   // per-workgroup cell counts are cellblk; cursor: the atomic scatter's
   // per-cell cursors.)
   DevBuf raw, itype, pos, spos, stype, cellid, cellcount, cellstart, cursor, cellblk, partial, counts, thr, blk,
-      blkoff, sp_off, u, rho, tiles, tile_shell, sqf, sqfp, axis_kv, axis_shell, listpart, flags, tunebuf;
+      blkoff, sp_off, u, rho, tiles, tile_shell, sqf, sqfp, axis_kv, axis_shell, listpart, flags, tunebuf, anc;
   // Charges (TRJ_RDF_CHARGE / TRJ_SQ_CHARGE): the frame's charges, their
   // cell-sorted copy, the frame's fixed-point g_qq sums, the g_qq and S_qq
   // accumulators.
@@ -154,6 +154,10 @@ struct trj_ctx {
     int tried = 0;
   } sq_tune;
   hipEvent_t tune_ev[2] = {nullptr, nullptr};
+  // FP64 S(Q) frames precompute the contraction's phase anchors as a table;
+  // TRJ_SQ_ANCHOR_TABLE=0 at trj_sq_init has the kernel evaluate them itself
+  // (the same values: a check of the table).
+  bool sq_anchor_table = true;
 };
 
 namespace {
@@ -169,7 +173,7 @@ std::vector<DevBuf*> dev_bufs(trj_ctx* c) {
           &c->slqsum,    &c->slqcnt,    &c->sqqsum,   &c->sqqcnt,     &c->xyframe,  &c->xycounts, &c->xyacc,
           &c->xygqf,     &c->xygqacc,   &c->thr_xy,   &c->rpts,       &c->s2gi,     &c->s2g2i,    &c->fqkv,
           &c->fqbin,     &c->fqr0,      &c->fqrho0,   &c->fqpart,     &c->fqvsum,   &c->fqfkt,    &c->fqfskt,
-          &c->fqtot,     &c->fqtots,    &c->cellblk,  &c->tunebuf};
+          &c->fqtot,     &c->fqtots,    &c->cellblk,  &c->tunebuf,    &c->anc};
 }
 
 }  // namespace
@@ -1038,6 +1042,20 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
       ca.nch = nch;
       ca.ndim = sdim;
       ca.n = ustride;
+      // FP64: the phase anchors of the tiles' segments as a table, computed
+      // once per frame (for lattice_262k 82 rows x 262144 entries, 344 MB,
+      // against 4.7e9 anchor evaluations inside the contraction).  Not for
+      // the allocation-free entry point.
+      const int64_t anc_bytes = sq_anchor_bytes(S, ustride);
+      if (c->sq_anchor_table && !c->noalloc && anc_bytes > 0) {
+        if ((r = ensure(c, c->anc, static_cast<size_t>(anc_bytes)))) return r;
+        launch_sq_anchors(P<double>(c->u), ustride, S.anc_rows, S.anc_base, P<double2>(c->anc), c->stream);
+        ca.anc = P<double2>(c->anc);
+        ca.anc_row[0] = 0;
+        ca.anc_row[1] = S.anc_rows[0];
+        ca.anc_row[2] = S.anc_rows[0] + S.anc_rows[1];
+        for (int k = 0; k < 3; ++k) ca.anc_base[k] = S.anc_base[k];
+      }
       auto& st = c->sq_tune;
       if (S.p.precision == TRJ_PREC_FP64 && st.enabled && !st.done && !c->noalloc) {
         // trials: the contraction and a shell pass into scratch sums
@@ -1450,6 +1468,10 @@ int trj_sq_init(trj_ctx* c, const trj_sq_params* p, const double box[3], trj_sq_
   }
   c->sq_tune = {};
   c->sq_tune.enabled = (p->flags & TRJ_SQ_NO_AUTOTUNE) == 0;
+  {
+    const char* e = std::getenv("TRJ_SQ_ANCHOR_TABLE");
+    c->sq_anchor_table = !(e && std::strcmp(e, "0") == 0);
+  }
   const int ns = S.nqmax + 1;
   const int ngs = nsl > 0 ? nsl : 1;
   if ((r = ensure(c, c->sqf, sizeof(double) * ns * ngs))) return r;

@@ -85,7 +85,18 @@ struct SqPlan {
   std::vector<int32_t> axis_kv;      // axis vectors (x,y,z) triples
   std::vector<int32_t> axis_shell;   // their shells
   int kmin[3], kmax[3];              // inclusive index range of the dense box
+  // FP64 tiles: their 8-value phase segments start at anc_base[a] + 8 m,
+  // m < anc_rows[a] (x: kx0, y: ky0, z: kz0 + 0, 8, 16, 24); zero otherwise.
+  int anc_rows[3] = {0, 0, 0}, anc_base[3] = {0, 0, 0};
 };
+
+// Bytes of the FP64 contraction's anchor table for n turn entries; 0 when
+// the frame goes without one (other precisions, or a table above 8 GiB: the
+// kernel then evaluates the anchors itself).
+inline int64_t sq_anchor_bytes(const SqPlan& S, int64_t n) {
+  const int64_t b = int64_t(16) * (S.anc_rows[0] + S.anc_rows[1] + S.anc_rows[2]) * n;
+  return b <= (int64_t(8) << 30) ? b : 0;
+}
 
 // sq.cuf:156-200 (REF) or the half-space grid; fills everything above.
 int sq_plan(const trj_sq_params& p, const double box[3], SqPlan* out, std::string* err);
