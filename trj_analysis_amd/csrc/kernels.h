@@ -260,6 +260,11 @@ struct ContractArgs {
   // evaluates them itself.  The values are the same either way.
   const double2* anc;
   int anc_row[3], anc_base[3];
+  // 16-bit kernels, one frame: the same anchors in FP32 (e^{2 pi i f} of the
+  // FP64-reduced turns f, as the kernel evaluates them), rows as above, then
+  // row anc_nrows + axis: the unit step e^{2 pi i u} of each axis in FP32.
+  const float2* anc32;
+  int anc_nrows;
 };
 void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s);
 // The anchor table of the FP64 contraction (ContractArgs::anc): rows[axis]
@@ -267,6 +272,9 @@ void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s);
 // (plane stride n), axes in order x, y, z.
 void launch_sq_anchors(const double* u, int64_t n, const int rows[3], const int base[3], double2* anc,
                        hipStream_t s);
+// The FP32 anchor table of the 16-bit contraction (ContractArgs::anc32).
+void launch_sq_anchors32(const double* u, int64_t n, const int rows[3], const int base[3], float2* anc,
+                         hipStream_t s);
 // Bytes of one (split, channel, vector) partial of rho: FP64 complex from the
 // FP64 contraction, FP32 complex from the others.
 inline size_t rho_entry_bytes(int precision) { return precision == TRJ_PREC_FP64 ? 16 : 8; }

@@ -2154,6 +2154,25 @@ __device__ inline float2 cis_turns_f32(double f) {
   return make_float2(((qi + 1) & 2) ? -re : re, (qi & 2) ? -im : im);
 }
 
+// FP32 anchor table of the 16-bit contraction: rows as k_sq_anchors
+// (cis_turns_f32 of the exactly reduced turns, the values the kernel would
+// compute), then three rows of the FP32 unit steps e^{2 pi i u} per axis.
+__global__ __launch_bounds__(256) void k_sq_anchors32(const double* __restrict__ u, int64_t n, int rx, int ry,
+                                                      int rz, int bx, int by, int bz, float2* __restrict__ anc) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (j >= n) return;
+  const int row = blockIdx.y, nrow = rx + ry + rz;
+  if (row >= nrow) {
+    const int ax = row - nrow;
+    anc[row * n + j] = make_float2(static_cast<float>(u[(6 + ax) * n + j]), static_cast<float>(u[(9 + ax) * n + j]));
+    return;
+  }
+  const int ax = row < rx ? 0 : (row < rx + ry ? 1 : 2);
+  const int m = row - (ax == 0 ? 0 : (ax == 1 ? rx : rx + ry));
+  const int n0 = (ax == 0 ? bx : (ax == 1 ? by : bz)) + 8 * m;
+  anc[row * n + j] = cis_turns_f32(frac_turns(n0, u[ax * n + j], u[(3 + ax) * n + j]));
+}
+
 // Phase records of the 16-bit kernel: e^{2 pi i n u} in FP32 complex, value-
 // major (row = one kx, kz or ky value, column = atom), row stride kRS float2
 // (32 atoms + 2: lane-per-atom writes and the operand reads are free of bank
@@ -2182,7 +2201,7 @@ constexpr int kRS = 34;
 // accumulation runs over at most 8192 atoms per workgroup and channel (the
 // K-splits, stored as FP32 complex partials, are summed in FP64).
 constexpr int WX = 64, WY = 8, WZ = 32, WN = WY * WZ, WROWS = 104;
-template <int PREC>
+template <int PREC, bool TABLE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k_sq_contract_lp16w(
     ContractArgs a) {
   typedef LpTypes<PREC> LT;
@@ -2209,6 +2228,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
   const double* u_c = a.u + (6 + axis) * n32;
   const double* u_s = a.u + (9 + axis) * n32;
   const double* wsrc = a.w ? a.w : u_hi;
+  // With the anchor table: the segment's FP32 anchors and the axis' FP32
+  // unit step replace the turns hi, lo, cos, sin (no FP64 reduction here).
+  const float2* t_anc =
+      TABLE ? a.anc32 + static_cast<int64_t>(a.anc_row[axis] + (sn0 - a.anc_base[axis]) / 8) * a.n : nullptr;
+  const float2* t_step = TABLE ? a.anc32 + static_cast<int64_t>(a.anc_nrows + axis) * a.n : nullptr;
   // A split: row ar, atoms 4 aq .. 4 aq + 3 (half of one 8-atom entry).
   const int ar = t & 63, aq = t >> 6;
   for (int s = wi.c0; s < wi.c1; ++s) {
@@ -2233,15 +2257,21 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
     // Turns of one chunk, loaded one chunk step before they are staged.
     struct Turns {
       double h, l, c, s, w;
+      float2 an, st;
       float fw, f1;
     } F;
     auto fetch = [&](Turns& f, int c0) {
       const int ai = c0 + sa;
       const uint32_t o = static_cast<uint32_t>(ai < b1 ? ai : b1 - 1);
-      f.h = u_hi[o];
-      f.l = u_lo[o];
-      f.c = u_c[o];
-      f.s = u_s[o];
+      if (TABLE) {
+        f.an = t_anc[o];
+        f.st = t_step[o];
+      } else {
+        f.h = u_hi[o];
+        f.l = u_lo[o];
+        f.c = u_c[o];
+        f.s = u_s[o];
+      }
       // The weight is loaded and used unconditionally (from the turns when
       // there are none; times 0 then) so that no branch splits the loop body:
       // the factor is w (ex, weights), 1 (ex without weights, ey, ez), 0 (past b1).
@@ -2254,10 +2284,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
     // (the rotation state carries over between pieces).
     float pre, pim, psc, pss;
     auto stage_begin = [&](const Turns& f) {
-      const float2 an = cis_turns_f32(frac_turns(sn0, f.h, f.l));
+      const float2 an = TABLE ? f.an : cis_turns_f32(frac_turns(sn0, f.h, f.l));
       const float m = fmaf(static_cast<float>(f.w), f.fw, f.f1);
-      psc = static_cast<float>(f.c);
-      pss = static_cast<float>(f.s);
+      psc = TABLE ? f.st.x : static_cast<float>(f.c);
+      pss = TABLE ? f.st.y : static_cast<float>(f.s);
       pre = an.x * m;
       pim = an.y * m;
     };
@@ -3496,6 +3526,14 @@ void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_sq_contract_f64<false>, contract_grid(a), dim3(256), 0, s, a);
 }
 
+void launch_sq_anchors32(const double* u, int64_t n, const int rows[3], const int base[3], float2* anc,
+                         hipStream_t s) {
+  const int nrow = rows[0] + rows[1] + rows[2];
+  if (n <= 0 || nrow <= 0) return;
+  hipLaunchKernelGGL(k_sq_anchors32, dim3(static_cast<unsigned>((n + 255) / 256), nrow + 3), dim3(256), 0, s, u, n,
+                     rows[0], rows[1], rows[2], base[0], base[1], base[2], anc);
+}
+
 void launch_sq_anchors(const double* u, int64_t n, const int rows[3], const int base[3], double2* anc,
                        hipStream_t s) {
   const int nrow = rows[0] + rows[1] + rows[2];
@@ -3510,9 +3548,15 @@ void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s) 
   if (precision == TRJ_PREC_FP32)
     hipLaunchKernelGGL(k_sq_contract_f32, grid, block, 0, s, a);
   else if (precision == TRJ_PREC_FP16X3)
-    hipLaunchKernelGGL(k_sq_contract_lp16w<TRJ_PREC_FP16X3>, grid, dim3(512), 0, s, a);
+    if (a.anc32)
+      hipLaunchKernelGGL((k_sq_contract_lp16w<TRJ_PREC_FP16X3, true>), grid, dim3(512), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_sq_contract_lp16w<TRJ_PREC_FP16X3, false>), grid, dim3(512), 0, s, a);
   else
-    hipLaunchKernelGGL(k_sq_contract_lp16w<TRJ_PREC_BF16X3>, grid, dim3(512), 0, s, a);
+    if (a.anc32)
+      hipLaunchKernelGGL((k_sq_contract_lp16w<TRJ_PREC_BF16X3, true>), grid, dim3(512), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_sq_contract_lp16w<TRJ_PREC_BF16X3, false>), grid, dim3(512), 0, s, a);
 }
 
 void launch_sq_shells(const void* rho, bool rho_f32, const int16_t* tile_shell, int ntiles, int tile_vec,

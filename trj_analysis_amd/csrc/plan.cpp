@@ -347,8 +347,9 @@ int sq_plan(const trj_sq_params& p_in, const double box[3], SqPlan* out, std::st
           }
         }
   }
-  if (p.precision == TRJ_PREC_FP64 && !P.tiles.empty()) {
-    // Segment origins of the FP64 tiles (8 kx | 8 ky | 4 x 8 kz per tile).
+  if (p.precision != TRJ_PREC_FP32 && !P.tiles.empty()) {
+    // Segment origins of the tiles (FP64: 8 kx | 8 ky | 4 x 8 kz per tile;
+    // 16-bit: 8 x 8 kx | 8 ky | 4 x 8 kz).
     const int seg[3] = {TKX / 8, TKY / 8, TKZ / 8};
     int lo[3], hi[3];
     for (int k = 0; k < 3; ++k) {

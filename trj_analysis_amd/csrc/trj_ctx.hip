@@ -1042,15 +1042,21 @@ int run_frame(trj_ctx* c, const void* pos_dev, int dtype, int layout, const int3
       ca.nch = nch;
       ca.ndim = sdim;
       ca.n = ustride;
-      // FP64: the phase anchors of the tiles' segments as a table, computed
-      // once per frame (for lattice_262k 82 rows x 262144 entries, 344 MB,
-      // against 4.7e9 anchor evaluations inside the contraction).  Not for
-      // the allocation-free entry point.
+      // FP64 and 16-bit: the phase anchors of the tiles' segments as a
+      // table, computed once per frame (for lattice_262k in FP64 82 rows x
+      // 262144 entries, 344 MB, against 4.7e9 anchor evaluations inside the
+      // contraction).  Not for the allocation-free entry point.
       const int64_t anc_bytes = sq_anchor_bytes(S, ustride);
       if (c->sq_anchor_table && !c->noalloc && anc_bytes > 0) {
         if ((r = ensure(c, c->anc, static_cast<size_t>(anc_bytes)))) return r;
-        launch_sq_anchors(P<double>(c->u), ustride, S.anc_rows, S.anc_base, P<double2>(c->anc), c->stream);
-        ca.anc = P<double2>(c->anc);
+        if (S.p.precision == TRJ_PREC_FP64) {
+          launch_sq_anchors(P<double>(c->u), ustride, S.anc_rows, S.anc_base, P<double2>(c->anc), c->stream);
+          ca.anc = P<double2>(c->anc);
+        } else {
+          launch_sq_anchors32(P<double>(c->u), ustride, S.anc_rows, S.anc_base, P<float2>(c->anc), c->stream);
+          ca.anc32 = P<float2>(c->anc);
+          ca.anc_nrows = S.anc_rows[0] + S.anc_rows[1] + S.anc_rows[2];
+        }
         ca.anc_row[0] = 0;
         ca.anc_row[1] = S.anc_rows[0];
         ca.anc_row[2] = S.anc_rows[0] + S.anc_rows[1];

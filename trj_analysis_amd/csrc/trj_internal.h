@@ -85,16 +85,20 @@ struct SqPlan {
   std::vector<int32_t> axis_kv;      // axis vectors (x,y,z) triples
   std::vector<int32_t> axis_shell;   // their shells
   int kmin[3], kmax[3];              // inclusive index range of the dense box
-  // FP64 tiles: their 8-value phase segments start at anc_base[a] + 8 m,
-  // m < anc_rows[a] (x: kx0, y: ky0, z: kz0 + 0, 8, 16, 24); zero otherwise.
+  // FP64 and 16-bit tiles: their 8-value phase segments start at
+  // anc_base[a] + 8 m, m < anc_rows[a] (FP64: x kx0, y ky0, z kz0 + 0, 8, 16,
+  // 24; 16-bit: x kx0 + 0 .. 56, y ky0, z kz0 + 0 .. 24); zero for FP32.
   int anc_rows[3] = {0, 0, 0}, anc_base[3] = {0, 0, 0};
 };
 
-// Bytes of the FP64 contraction's anchor table for n turn entries; 0 when
-// the frame goes without one (other precisions, or a table above 8 GiB: the
-// kernel then evaluates the anchors itself).
+// Bytes of the contraction's anchor table for n turn entries: FP64 anchors
+// (16 B), or FP32 anchors and three FP32 unit-step rows (8 B each) for the
+// 16-bit kernels; 0 when the frame goes without one (FP32, or a table above
+// 8 GiB: the kernel then evaluates the anchors itself).
 inline int64_t sq_anchor_bytes(const SqPlan& S, int64_t n) {
-  const int64_t b = int64_t(16) * (S.anc_rows[0] + S.anc_rows[1] + S.anc_rows[2]) * n;
+  const int64_t rows = S.anc_rows[0] + S.anc_rows[1] + S.anc_rows[2];
+  if (rows == 0) return 0;
+  const int64_t b = S.p.precision == TRJ_PREC_FP64 ? int64_t(16) * rows * n : int64_t(8) * (rows + 3) * n;
   return b <= (int64_t(8) << 30) ? b : 0;
 }
 
