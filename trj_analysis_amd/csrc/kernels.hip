@@ -1636,20 +1636,26 @@ __device__ inline void phase8(double uh, double ul, int n0, double cs, double ss
   phase8_from(c, s, cs, ss, dst, m);
 }
 
-// Anchor table of the FP64 contraction: row r of axis a (rows[a] rows from
-// row0[a]) holds e^{2 pi i n0 u_j} with n0 = base[a] + 8 (r - row0[a]) for
-// every entry j, computed as phase8 would (bit-identical values).
-__global__ __launch_bounds__(256) void k_sq_anchors(const double* __restrict__ u, int64_t n, int rx, int ry,
-                                                    int bx, int by, int bz, double2* __restrict__ anc) {
+// Anchor table of the FP64 contraction: row row0 + m of an axis holds
+// e^{2 pi i n0 u_j} with n0 = base + 8 m (m < rows) for every entry j,
+// computed as phase8 would (bit-identical values).  One thread per entry
+// and axis (blockIdx.y): the turns are read once, each row written
+// coalesced.
+struct AnchorAxes {
+  int rows[3], base[3], row0[3];
+};
+__global__ __launch_bounds__(256) void k_sq_anchors(const double* __restrict__ u, int64_t n, AnchorAxes ax,
+                                                    double2* __restrict__ anc) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
   if (j >= n) return;
-  const int row = blockIdx.y;
-  const int ax = row < rx ? 0 : (row < rx + ry ? 1 : 2);
-  const int m = row - (ax == 0 ? 0 : (ax == 1 ? rx : rx + ry));
-  const int n0 = (ax == 0 ? bx : (ax == 1 ? by : bz)) + 8 * m;
-  double s, c;
-  sincospi(2.0 * frac_turns(n0, u[ax * n + j], u[(3 + ax) * n + j]), &s, &c);
-  anc[row * n + j] = make_double2(c, s);
+  const int a = blockIdx.y;
+  const double uh = u[a * n + j], ul = u[(3 + a) * n + j];
+  double2* dst = anc + static_cast<int64_t>(ax.row0[a]) * n + j;
+  for (int m = 0; m < ax.rows[a]; ++m) {
+    double s, c;
+    sincospi(2.0 * frac_turns(ax.base[a] + 8 * m, uh, ul), &s, &c);
+    dst[m * n] = make_double2(c, s);
+  }
 }
 
 // s + c += x with Knuth's TwoSum: s keeps the rounded sum, c the error, kept
@@ -2156,21 +2162,17 @@ __device__ inline float2 cis_turns_f32(double f) {
 
 // FP32 anchor table of the 16-bit contraction: rows as k_sq_anchors
 // (cis_turns_f32 of the exactly reduced turns, the values the kernel would
-// compute), then three rows of the FP32 unit steps e^{2 pi i u} per axis.
-__global__ __launch_bounds__(256) void k_sq_anchors32(const double* __restrict__ u, int64_t n, int rx, int ry,
-                                                      int rz, int bx, int by, int bz, float2* __restrict__ anc) {
+// compute), then row nrow + axis: the FP32 unit step e^{2 pi i u} of the axis.
+__global__ __launch_bounds__(256) void k_sq_anchors32(const double* __restrict__ u, int64_t n, AnchorAxes ax,
+                                                      int nrow, float2* __restrict__ anc) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
   if (j >= n) return;
-  const int row = blockIdx.y, nrow = rx + ry + rz;
-  if (row >= nrow) {
-    const int ax = row - nrow;
-    anc[row * n + j] = make_float2(static_cast<float>(u[(6 + ax) * n + j]), static_cast<float>(u[(9 + ax) * n + j]));
-    return;
-  }
-  const int ax = row < rx ? 0 : (row < rx + ry ? 1 : 2);
-  const int m = row - (ax == 0 ? 0 : (ax == 1 ? rx : rx + ry));
-  const int n0 = (ax == 0 ? bx : (ax == 1 ? by : bz)) + 8 * m;
-  anc[row * n + j] = cis_turns_f32(frac_turns(n0, u[ax * n + j], u[(3 + ax) * n + j]));
+  const int a = blockIdx.y;
+  const double uh = u[a * n + j], ul = u[(3 + a) * n + j];
+  anc[static_cast<int64_t>(nrow + a) * n + j] =
+      make_float2(static_cast<float>(u[(6 + a) * n + j]), static_cast<float>(u[(9 + a) * n + j]));
+  float2* dst = anc + static_cast<int64_t>(ax.row0[a]) * n + j;
+  for (int m = 0; m < ax.rows[a]; ++m) dst[m * n] = cis_turns_f32(frac_turns(ax.base[a] + 8 * m, uh, ul));
 }
 
 // Phase records of the 16-bit kernel: e^{2 pi i n u} in FP32 complex, value-
@@ -3526,20 +3528,30 @@ void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_sq_contract_f64<false>, contract_grid(a), dim3(256), 0, s, a);
 }
 
+static AnchorAxes anchor_axes(const int rows[3], const int base[3]) {
+  AnchorAxes ax;
+  for (int k = 0; k < 3; ++k) {
+    ax.rows[k] = rows[k];
+    ax.base[k] = base[k];
+    ax.row0[k] = k == 0 ? 0 : ax.row0[k - 1] + rows[k - 1];
+  }
+  return ax;
+}
+
 void launch_sq_anchors32(const double* u, int64_t n, const int rows[3], const int base[3], float2* anc,
                          hipStream_t s) {
   const int nrow = rows[0] + rows[1] + rows[2];
   if (n <= 0 || nrow <= 0) return;
-  hipLaunchKernelGGL(k_sq_anchors32, dim3(static_cast<unsigned>((n + 255) / 256), nrow + 3), dim3(256), 0, s, u, n,
-                     rows[0], rows[1], rows[2], base[0], base[1], base[2], anc);
+  hipLaunchKernelGGL(k_sq_anchors32, dim3(static_cast<unsigned>((n + 255) / 256), 3), dim3(256), 0, s, u, n,
+                     anchor_axes(rows, base), nrow, anc);
 }
 
 void launch_sq_anchors(const double* u, int64_t n, const int rows[3], const int base[3], double2* anc,
                        hipStream_t s) {
   const int nrow = rows[0] + rows[1] + rows[2];
   if (n <= 0 || nrow <= 0) return;
-  hipLaunchKernelGGL(k_sq_anchors, dim3(static_cast<unsigned>((n + 255) / 256), nrow), dim3(256), 0, s, u, n,
-                     rows[0], rows[1], base[0], base[1], base[2], anc);
+  hipLaunchKernelGGL(k_sq_anchors, dim3(static_cast<unsigned>((n + 255) / 256), 3), dim3(256), 0, s, u, n,
+                     anchor_axes(rows, base), anc);
 }
 
 void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s) {
