@@ -99,6 +99,25 @@ def test_batch_sq_reduced_precision(prec, tol):
     assert worst < tol
 
 
+@pytest.mark.parametrize("prec", [T.TRJ_PREC_FP64, T.TRJ_PREC_FP16X3])
+def test_batch_sq_anchor_table_on_off(monkeypatch, prec):
+    """The batch's contraction with its table of phase anchors (the default)
+    and without (TRJ_SQ_ANCHOR_TABLE=0): the table holds the values the
+    kernel would evaluate, so the partials are equal and the shells agree to
+    the order of their FP64 atomics; groups of 3 frames over a shared
+    buffer (the table covers each group's entries only)."""
+    frames = _frames(15, 7, nmin=3e3, nmax=5e4)
+    res = {}
+    for table in ("1", "0"):
+        monkeypatch.setenv("TRJ_SQ_ANCHOR_TABLE", table)
+        print("case: batch sq anchor table", table, "precision", prec, flush=True)
+        res[table] = _batch(frames, 3, (32, 32, 32), prec, fpl=3)[0]
+    for k, ((sqf, sqfp, nq), (wf, wfp, wnq)) in enumerate(zip(res["1"], res["0"])):
+        np.testing.assert_array_equal(nq, wnq)
+        np.testing.assert_allclose(sqf, wf, rtol=1e-13, atol=0, err_msg=f"frame {k} sqf")
+        np.testing.assert_allclose(sqfp, wfp, rtol=1e-13, atol=0, err_msg=f"frame {k} sqfp")
+
+
 def test_batch_sq_against_oracle():
     """A small frame inside a batch: shells summed from the oracle's long-double
     rho of every half-space vector (the origin left out), shells by the plan's

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -50,7 +51,8 @@ struct trj_batch {
   size_t g_lds = 0;
   // batched S(Q) (trj_batch_sq_run): group descriptors, sort and contraction
   // buffers, all frames' shell sums, and its own captured sequence
-  Buf sq_hdr, sq_fr, sq_items, sq_plane, sq_blk, sq_blkoff, sq_spoff, sq_u, sq_rho, sq_tiles, sq_sqf, sq_sqfp, sq_nq;
+  Buf sq_hdr, sq_fr, sq_items, sq_plane, sq_blk, sq_blkoff, sq_spoff, sq_u, sq_rho, sq_tiles, sq_sqf, sq_sqfp, sq_nq,
+      sq_anc;
   void* sq_pin[2] = {nullptr, nullptr};
   size_t sq_pin_cap[2] = {0, 0};
   hipEvent_t sq_pin_done[2] = {nullptr, nullptr};
@@ -496,6 +498,15 @@ int run_sq_groups(trj_batch* b, const trj_batch_sq_params& q, int nframes, const
   if ((r = bensure(b, b->sq_sqfp, sizeof(double) * tot_sqfp))) return r;
   if ((r = bensure(b, b->sq_nq, sizeof(int64_t) * tot_sqf))) return r;
   if ((r = bensure(b, b->flags, sizeof(uint32_t)))) return r;
+  // Anchor table of the contraction (as for one frame: the same values the
+  // kernel would evaluate), for the largest group's entries; off with
+  // TRJ_SQ_ANCHOR_TABLE=0 or above 8 GiB.
+  int anc_rows[3], anc_base[3];
+  anchor_layout(tiles.data(), 4, tiles.size() / 4, tk, anc_rows, anc_base);
+  const char* anc_env = std::getenv("TRJ_SQ_ANCHOR_TABLE");
+  const int64_t anc_bytes =
+      (anc_env && std::strcmp(anc_env, "0") == 0) ? 0 : anchor_table_bytes(q.precision, anc_rows, cap_atoms);
+  if (anc_bytes > 0 && (r = bensure(b, b->sq_anc, static_cast<size_t>(anc_bytes)))) return r;
   BHIP(b, hipMemcpyAsync(b->sq_tiles.p, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice,
                          b->stream));
   BHIP(b, hipMemsetAsync(b->sq_sqf.p, 0, sizeof(double) * tot_sqf, b->stream));
@@ -525,6 +536,11 @@ int run_sq_groups(trj_batch* b, const trj_batch_sq_params& q, int nframes, const
   for (int k = 0; k < 3; ++k) d.kmax[k] = kmax[k];
   d.rho = static_cast<double*>(b->sq_rho.p);
   d.rho_f32 = q.precision != TRJ_PREC_FP64;
+  d.anc = anc_bytes > 0 ? b->sq_anc.p : nullptr;
+  for (int k = 0; k < 3; ++k) {
+    d.anc_rows[k] = anc_rows[k];
+    d.anc_base[k] = anc_base[k];
+  }
   d.sqf = static_cast<double*>(b->sq_sqf.p);
   d.sqfp = static_cast<double*>(b->sq_sqfp.p);
   d.nq = static_cast<unsigned long long*>(b->sq_nq.p);
@@ -691,7 +707,7 @@ void trj_batch_destroy(trj_batch* b) {
   Buf* bufs[] = {&b->hdr,    &b->fr,     &b->pos,       &b->spos,      &b->stype,    &b->cellid,   &b->count,
                  &b->start,  &b->cursor, &b->blocksum,  &b->thr,       &b->hist,     &b->flags,    &b->sq_hdr,
                  &b->sq_fr,  &b->sq_items, &b->sq_plane, &b->sq_blk,   &b->sq_blkoff, &b->sq_spoff, &b->sq_u,
-                 &b->sq_rho, &b->sq_tiles, &b->sq_sqf,  &b->sq_sqfp,   &b->sq_nq};
+                 &b->sq_rho, &b->sq_tiles, &b->sq_sqf,  &b->sq_sqfp,   &b->sq_nq,     &b->sq_anc};
   for (Buf* x : bufs)
     if (x->p) (void)hipFree(x->p);
   for (int k = 0; k < 2; ++k) {

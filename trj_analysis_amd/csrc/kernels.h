@@ -270,11 +270,14 @@ void launch_sq_contract_f64(const ContractArgs& a, hipStream_t s);
 // The anchor table of the FP64 contraction (ContractArgs::anc): rows[axis]
 // rows of anchors n0 = base[axis] + 8 m for every entry of the turn planes u
 // (plane stride n), axes in order x, y, z.
+// live_off, live_nch (both NULL, or both device pointers): only entries below
+// live_off[*live_nch] are evaluated (a batch group's entries in a buffer of
+// stride n).
 void launch_sq_anchors(const double* u, int64_t n, const int rows[3], const int base[3], double2* anc,
-                       hipStream_t s);
+                       hipStream_t s, const int32_t* live_off = nullptr, const int32_t* live_nch = nullptr);
 // The FP32 anchor table of the 16-bit contraction (ContractArgs::anc32).
 void launch_sq_anchors32(const double* u, int64_t n, const int rows[3], const int base[3], float2* anc,
-                         hipStream_t s);
+                         hipStream_t s, const int32_t* live_off = nullptr, const int32_t* live_nch = nullptr);
 // Bytes of one (split, channel, vector) partial of rho: FP64 complex from the
 // FP64 contraction, FP32 complex from the others.
 inline size_t rho_entry_bytes(int precision) { return precision == TRJ_PREC_FP64 ? 16 : 8; }
@@ -406,6 +409,8 @@ struct BatchSqDev {
   uint32_t* flags;
   int nblocks;         // fixed grid of the count, scatter and shell kernels
   int shell_lds;       // doubles of LDS accumulators per workgroup (0: global atomics)
+  void* anc;           // anchor table (stride entries per row) or NULL
+  int anc_rows[3], anc_base[3];
 };
 void launch_batch_sq_group(const BatchSqDev& d, int precision, hipStream_t s);
 

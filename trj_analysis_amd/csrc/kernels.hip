@@ -1645,9 +1645,10 @@ struct AnchorAxes {
   int rows[3], base[3], row0[3];
 };
 __global__ __launch_bounds__(256) void k_sq_anchors(const double* __restrict__ u, int64_t n, AnchorAxes ax,
-                                                    double2* __restrict__ anc) {
+                                                    double2* __restrict__ anc, const int32_t* live_off,
+                                                    const int32_t* live_nch) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (j >= n) return;
+  if (j >= (live_off ? static_cast<int64_t>(live_off[*live_nch]) : n)) return;
   const int a = blockIdx.y;
   const double uh = u[a * n + j], ul = u[(3 + a) * n + j];
   double2* dst = anc + static_cast<int64_t>(ax.row0[a]) * n + j;
@@ -2164,9 +2165,10 @@ __device__ inline float2 cis_turns_f32(double f) {
 // (cis_turns_f32 of the exactly reduced turns, the values the kernel would
 // compute), then row nrow + axis: the FP32 unit step e^{2 pi i u} of the axis.
 __global__ __launch_bounds__(256) void k_sq_anchors32(const double* __restrict__ u, int64_t n, AnchorAxes ax,
-                                                      int nrow, float2* __restrict__ anc) {
+                                                      int nrow, float2* __restrict__ anc, const int32_t* live_off,
+                                                      const int32_t* live_nch) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (j >= n) return;
+  if (j >= (live_off ? static_cast<int64_t>(live_off[*live_nch]) : n)) return;
   const int a = blockIdx.y;
   const double uh = u[a * n + j], ul = u[(3 + a) * n + j];
   anc[static_cast<int64_t>(nrow + a) * n + j] =
@@ -3539,19 +3541,19 @@ static AnchorAxes anchor_axes(const int rows[3], const int base[3]) {
 }
 
 void launch_sq_anchors32(const double* u, int64_t n, const int rows[3], const int base[3], float2* anc,
-                         hipStream_t s) {
+                         hipStream_t s, const int32_t* live_off, const int32_t* live_nch) {
   const int nrow = rows[0] + rows[1] + rows[2];
   if (n <= 0 || nrow <= 0) return;
   hipLaunchKernelGGL(k_sq_anchors32, dim3(static_cast<unsigned>((n + 255) / 256), 3), dim3(256), 0, s, u, n,
-                     anchor_axes(rows, base), nrow, anc);
+                     anchor_axes(rows, base), nrow, anc, live_off, live_nch);
 }
 
 void launch_sq_anchors(const double* u, int64_t n, const int rows[3], const int base[3], double2* anc,
-                       hipStream_t s) {
+                       hipStream_t s, const int32_t* live_off, const int32_t* live_nch) {
   const int nrow = rows[0] + rows[1] + rows[2];
   if (n <= 0 || nrow <= 0) return;
   hipLaunchKernelGGL(k_sq_anchors, dim3(static_cast<unsigned>((n + 255) / 256), 3), dim3(256), 0, s, u, n,
-                     anchor_axes(rows, base), anc);
+                     anchor_axes(rows, base), anc, live_off, live_nch);
 }
 
 void launch_sq_contract_lp(const ContractArgs& a, int precision, hipStream_t s) {
@@ -3913,6 +3915,23 @@ void launch_batch_sq_group(const BatchSqDev& d, int precision, hipStream_t s) {
   ca.items = d.items;
   ca.item_plane = d.item_plane;
   ca.nitems = &d.hdr->nitems;
+  if (d.anc) {
+    // The group's entries are sp_off[0 .. nch): anchors for those only.
+    if (precision == TRJ_PREC_FP64) {
+      launch_sq_anchors(d.u, d.stride, d.anc_rows, d.anc_base, static_cast<double2*>(d.anc), s, d.sp_off,
+                        &d.hdr->nch);
+      ca.anc = static_cast<const double2*>(d.anc);
+    } else {
+      launch_sq_anchors32(d.u, d.stride, d.anc_rows, d.anc_base, static_cast<float2*>(d.anc), s, d.sp_off,
+                          &d.hdr->nch);
+      ca.anc32 = static_cast<const float2*>(d.anc);
+      ca.anc_nrows = d.anc_rows[0] + d.anc_rows[1] + d.anc_rows[2];
+    }
+    ca.anc_row[0] = 0;
+    ca.anc_row[1] = d.anc_rows[0];
+    ca.anc_row[2] = d.anc_rows[0] + d.anc_rows[1];
+    for (int k = 0; k < 3; ++k) ca.anc_base[k] = d.anc_base[k];
+  }
   if (precision == TRJ_PREC_FP64)
     launch_sq_contract_f64(ca, s);
   else

@@ -5,6 +5,8 @@
 // Compiled with -ffp-contract=off: the FP32 arithmetic below must round once
 // per operation, as the reference's FP32 sq_init does (myprec = sp,
 // src/core/precision.f90:22).
+#include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -139,6 +141,32 @@ inline int ref_octant_shell(const SqPlan& P, int ndim, int kx, int ky, int kz) {
 }
 
 }  // namespace
+
+void anchor_layout(const int32_t* org, int stride, size_t ntiles, const TileShape& tk, int rows[3], int base[3]) {
+  // Segment origins of a tile: x kx0 + 8 s (s < tx / 8), y ky0 + 8 s
+  // (s < ty / 8), z kz0 + 8 s (s < tz / 8).
+  const int seg[3] = {tk.tx / 8, tk.ty / 8, tk.tz / 8};
+  for (int k = 0; k < 3; ++k) {
+    rows[k] = 0;
+    base[k] = 0;
+  }
+  if (ntiles == 0) return;
+  int lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = INT32_MAX;
+    hi[k] = INT32_MIN;
+  }
+  for (size_t t = 0; t < ntiles; ++t)
+    for (int k = 0; k < 3; ++k) {
+      const int o = org[t * stride + k];
+      lo[k] = std::min(lo[k], o);
+      hi[k] = std::max(hi[k], o + 8 * (seg[k] - 1));
+    }
+  for (int k = 0; k < 3; ++k) {
+    base[k] = lo[k];
+    rows[k] = (hi[k] - lo[k]) / 8 + 1;
+  }
+}
 
 int sq_plan(const trj_sq_params& p_in, const double box[3], SqPlan* out, std::string* err) {
   SqPlan& P = *out;
@@ -348,25 +376,14 @@ int sq_plan(const trj_sq_params& p_in, const double box[3], SqPlan* out, std::st
         }
   }
   if (p.precision != TRJ_PREC_FP32 && !P.tiles.empty()) {
-    // Segment origins of the tiles (FP64: 8 kx | 8 ky | 4 x 8 kz per tile;
-    // 16-bit: 8 x 8 kx | 8 ky | 4 x 8 kz).
-    const int seg[3] = {TKX / 8, TKY / 8, TKZ / 8};
-    int lo[3], hi[3];
-    for (int k = 0; k < 3; ++k) {
-      lo[k] = INT32_MAX;
-      hi[k] = INT32_MIN;
-    }
+    std::vector<int32_t> org;
+    org.reserve(3 * P.tiles.size());
     for (const SqTile& t : P.tiles) {
-      const int o[3] = {t.kx0, t.ky0, t.kz0};
-      for (int k = 0; k < 3; ++k) {
-        lo[k] = std::min(lo[k], o[k]);
-        hi[k] = std::max(hi[k], o[k] + 8 * (seg[k] - 1));
-      }
+      org.push_back(t.kx0);
+      org.push_back(t.ky0);
+      org.push_back(t.kz0);
     }
-    for (int k = 0; k < 3; ++k) {
-      P.anc_base[k] = lo[k];
-      P.anc_rows[k] = (hi[k] - lo[k]) / 8 + 1;
-    }
+    anchor_layout(org.data(), 3, P.tiles.size(), P.tk, P.anc_rows, P.anc_base);
   }
   if (P.nqmax > 32767) {
     *err = "sq: nqmax above 32767 is not supported";

@@ -91,16 +91,24 @@ struct SqPlan {
   int anc_rows[3] = {0, 0, 0}, anc_base[3] = {0, 0, 0};
 };
 
+// Anchor rows of the FP64 and 16-bit contractions' tiles (tile origins kx0,
+// ky0, kz0 at org[t * stride + 0..2]): the 8-value phase segments of the
+// tiles start at base[a] + 8 m, m < rows[a].
+void anchor_layout(const int32_t* org, int stride, size_t ntiles, const TileShape& tk, int rows[3], int base[3]);
+// Bytes of an anchor table with the given rows for n turn entries (FP64:
+// 16 B per anchor; 16-bit: 8 B, plus three rows of FP32 unit steps); 0 above
+// 8 GiB (the kernel then evaluates the anchors itself).
+inline int64_t anchor_table_bytes(int precision, const int rows[3], int64_t n) {
+  const int64_t nr = rows[0] + rows[1] + rows[2];
+  if (nr == 0 || precision == TRJ_PREC_FP32) return 0;
+  const int64_t b = precision == TRJ_PREC_FP64 ? int64_t(16) * nr * n : int64_t(8) * (nr + 3) * n;
+  return b <= (int64_t(8) << 30) ? b : 0;
+}
 // Bytes of the contraction's anchor table for n turn entries: FP64 anchors
 // (16 B), or FP32 anchors and three FP32 unit-step rows (8 B each) for the
 // 16-bit kernels; 0 when the frame goes without one (FP32, or a table above
 // 8 GiB: the kernel then evaluates the anchors itself).
-inline int64_t sq_anchor_bytes(const SqPlan& S, int64_t n) {
-  const int64_t rows = S.anc_rows[0] + S.anc_rows[1] + S.anc_rows[2];
-  if (rows == 0) return 0;
-  const int64_t b = S.p.precision == TRJ_PREC_FP64 ? int64_t(16) * rows * n : int64_t(8) * (rows + 3) * n;
-  return b <= (int64_t(8) << 30) ? b : 0;
-}
+inline int64_t sq_anchor_bytes(const SqPlan& S, int64_t n) { return anchor_table_bytes(S.p.precision, S.anc_rows, n); }
 
 // sq.cuf:156-200 (REF) or the half-space grid; fills everything above.
 int sq_plan(const trj_sq_params& p, const double box[3], SqPlan* out, std::string* err);
