@@ -598,6 +598,10 @@ def main():
         ceil = rate * peak / NOMINAL_MFMA[args.precision]
         probe = {"mfma_rate": rate / 1e12, "clock_mhz": mhz, "ceiling": ceil / 1e12,
                  "frac_of_ceiling": achieved / ceil}
+        if args.precision == "fp64":
+            probe["note"] = ("achieved counts 8 flop per complex term; the kernel executes 6 of them on the "
+                             "MFMA pipe (three real products per complex product) and skips empty blocks, so "
+                             "frac_of_ceiling can pass 1; the MFMA busy fraction from counters is in profiles/")
     # RDF: in-range pairs per second of the RDF stage against the device's
     # measured LDS-atomic rate (one histogram add per in-range pair).
     lds_rate = T.lds_atomic_rate(0)
